@@ -1,0 +1,1610 @@
+/*
+ * snac_oracle.c -- CPU FP64 restatement of SNaC's time-step hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker for the HIP path; never shipped, never
+ * called by the product library).  Paths cited are relative to the SNaC tree.
+ *
+ * Parity pins (each checked by tests/):
+ *   grid (initgrid.f90)           : closed form (uniform spacing, tanh map),
+ *                                   reference LDC golden end-to-end
+ *   block placement (initmpi.f90) : index fixtures for the reference's own
+ *                                   multi-block examples, LDC golden
+ *   rk_mom / mom_*                : closed-form checks (linear field has zero
+ *                                   diffusion, periodic integral of the
+ *                                   vv-advection is zero), LDC golden
+ *   bounduvw/boundp/inflow        : ghost-value fixtures on a tiny multi-block
+ *                                   case, LDC golden
+ *   fillps/correc/updt_pressure   : discrete div-free field gives zero rhs,
+ *                                   projection gives div ~ 0, LDC golden
+ *   r2r kinds                     : scipy.fft dct/dst (types II,III,IV) and
+ *                                   numpy rfft fixtures; direct vs fast path
+ *   eigenvalues                   : numpy eigvalsh of the 1D operator
+ *   operator assembly + solvers   : scipy sparse direct solve; residual check
+ *   chkdiv / chkdt                : closed-form fields
+ * The HYPRE iterates and FFTW rounding of the reference cannot be reproduced
+ * bit-wise (third-party, not vendored); the oracle solves the same assembled
+ * systems to a tight tolerance instead.
+ */
+#include "snac_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define MAXB 64
+#define PI 3.14159265358979323846264338327950288
+
+typedef struct {
+  int n[3], lo_g[3], nb[2][3], is_bound[2][3];
+  double lmin[3], lmax[3], gr[3];
+  int gt[3];
+  char cbcvel[3][3][2];  /* [comp][dir][side] */
+  double bcvel[3][3][2];
+  char cbcpre[3][2];
+  double bcpre[3][2];
+  int inflow_type[3][2], is_inflow[3][2];
+  double *dc[3], *df[3], *rc[3], *rf[3]; /* n+2 entries */
+  double *f[6];                           /* u v w p pp po, ghosted */
+  double *dto[3];                         /* dudtrko... interior */
+  double *vin[3][3][2];                   /* inflow planes [idir][comp][side] */
+  double *rhsp[3][2];                     /* BC rhs planes [dir][side] */
+  long sx, sy, ntot;                      /* strides of ghosted arrays */
+  long off;                               /* offset in the solver vector */
+} ob;
+
+struct oc_case {
+  int nb, fft_axis;
+  double visc, bforce[3], vol_all;
+  int periods[3], is_periodic[3];
+  ob b[MAXB];
+  /* solver */
+  int method, maxiter;
+  double tol;
+  int nmodes, kind_fwd, kind_bwd;
+  double *lambda, normfft;
+  int nd, ax[3];   /* reduced problem dimensionality and its axes */
+  long nunk;
+  double *diag0, *wgt, *coef; /* per unknown: base diagonal, lambda weight, nd*2 coefs */
+  long *nbr;                   /* per unknown nd*2 neighbour indices (-1 none) */
+  int has_dirichlet;
+  /* band factorizations, lazily built per mode */
+  long *perm, *iperm; long bw;
+  double **bandL;
+  int band_ok;
+  /* tensor solver data */
+  int tensor_ok, tt[2], nt, ts; double th[2];
+  int t_kind_fwd[2], t_kind_bwd[2]; double t_norm[2]; double *t_lambda[2];
+  double *work;
+};
+
+static int nthreads_set = 0;
+int oc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) { omp_set_num_threads(n); nthreads_set = n; }
+  return omp_get_max_threads();
+#else
+  (void)n; return 1;
+#endif
+}
+
+/* ------------------------------------------------------------------------ */
+/* grid: initgrid.f90:9-83 and the stretching functions 171-287             */
+/* ------------------------------------------------------------------------ */
+static double gp_geometric_one_end(double alpha, double r0) {
+  const double power = 1.0 / 3.0;
+  if (r0 == 1.0) return r0;
+  return r0 * pow((1.0 - pow(r0, alpha)) / (1.0 - r0) / alpha, power);
+}
+static double gridpoint(int gt, double a, double r0) {
+  switch (gt) {
+  default:
+  case 0: /* two ends, initgrid.f90:171-184 */
+    return (a != 0.0) ? 0.5 * (1.0 + tanh((r0 - 0.5) * a) / tanh(a / 2.0)) : r0;
+  case 1: /* one end (lower) */
+    return (a != 0.0) ? 1.0 * (1.0 + tanh((r0 - 1.0) * a) / tanh(a / 1.0)) : r0;
+  case 2: /* middle */
+    if (a == 0.0) return r0;
+    if (r0 <= 0.5) return 0.5 * (1.0 - 1.0 + tanh(2.0 * a * (r0 - 0.0)) / tanh(a));
+    return 0.5 * (1.0 + 1.0 + tanh(2.0 * a * (r0 - 1.0)) / tanh(a));
+  case 3: /* one end (upper) */
+    return (a != 0.0) ? 1.0 - 1.0 * (1.0 + tanh((1.0 - r0 - 1.0) * a) / tanh(a / 1.0)) : r0;
+  case 4: return gp_geometric_one_end(a, r0);
+  case 5: return 1.0 - gp_geometric_one_end(a, 1.0 - r0);
+  case 6:
+    if (r0 <= 0.5) return gp_geometric_one_end(a / 2.0, 2.0 * r0) / 2.0;
+    return 1.0 - gp_geometric_one_end(a / 2.0, 2.0 * (1.0 - r0)) / 2.0;
+  case 7:
+    if (r0 <= 0.5) return (1.0 - gp_geometric_one_end(a / 2.0, (0.5 - r0) * 2.0)) / 2.0;
+    return (1.0 + gp_geometric_one_end(a / 2.0, (r0 - 0.5) * 2.0)) / 2.0;
+  }
+}
+
+int oc_initgrid(int n, int gt, double gr, double lmin, double lmax,
+                double *drc, double *drf, double *rc, double *rf) {
+  if (n < 1) return -1;
+  for (int q = 1; q <= n; q++) {
+    double r0 = (double)q / (double)n;
+    rf[q] = lmin + gridpoint(gt, gr, r0) * (lmax - lmin);
+  }
+  rf[0] = lmin;
+  for (int q = 1; q <= n; q++) drf[q] = rf[q] - rf[q - 1];
+  drf[0] = drf[1];
+  drf[n + 1] = drf[n];
+  for (int q = 0; q <= n; q++) drc[q] = 0.5 * (drf[q] + drf[q + 1]);
+  drc[n + 1] = drc[n];
+  rc[0] = lmin - drc[0] / 2.0;
+  rf[0] = lmin;
+  for (int q = 1; q <= n + 1; q++) {
+    rc[q] = rc[q - 1] + drc[q - 1];
+    rf[q] = rf[q - 1] + drf[q];
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* field helpers                                                             */
+/* ------------------------------------------------------------------------ */
+#define IDX(B, i, j, k) ((long)(i) + (B)->sx * ((long)(j) + (B)->sy * (long)(k)))
+#define IIDX(B, i, j, k) ((long)((i)-1) + (long)(B)->n[0] * ((long)((j)-1) + (long)(B)->n[1] * (long)((k)-1)))
+
+static void *xcalloc(size_t n, size_t s) {
+  void *p = calloc(n ? n : 1, s);
+  if (!p) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
+  return p;
+}
+
+/* plane of a ghosted field: dir d, plane index q (0..n+1), element (t1,t2)
+ * over the full ghosted tangential ranges (t1 along the lower remaining axis). */
+static inline long plane_idx(const ob *B, int d, int q, int t1, int t2) {
+  switch (d) {
+  case 0: return IDX(B, q, t1, t2);
+  case 1: return IDX(B, t1, q, t2);
+  default: return IDX(B, t1, t2, q);
+  }
+}
+static inline void tang(int d, int *a1, int *a2) {
+  if (d == 0) { *a1 = 1; *a2 = 2; }
+  else if (d == 1) { *a1 = 0; *a2 = 2; }
+  else { *a1 = 0; *a2 = 1; }
+}
+
+/* ------------------------------------------------------------------------ */
+/* boundary conditions: bound.f90                                            */
+/* ------------------------------------------------------------------------ */
+/* set_bc bound.f90:108-248 on one ghost plane of one block                  */
+static void set_bc(ob *B, double *p, char ctype, int ibound, int d, int centered,
+                   double rvalue, double dr) {
+  double factor = rvalue, sgn = 0.0;
+  if (ctype == 'D' && centered) { factor = 2.0 * factor; sgn = -1.0; }
+  if (ctype == 'N') {
+    if (ibound == 0) factor = -dr * factor; else factor = dr * factor;
+    sgn = 1.0;
+  }
+  if (ctype != 'D' && ctype != 'N') return;
+  int a1, a2; tang(d, &a1, &a2);
+  int m1 = B->n[a1] + 2, m2 = B->n[a2] + 2, n = B->n[d];
+  for (int t2 = 0; t2 < m2; t2++)
+    for (int t1 = 0; t1 < m1; t1++) {
+      if (centered) {
+        if (ibound == 0) p[plane_idx(B, d, 0, t1, t2)] = factor + sgn * p[plane_idx(B, d, 1, t1, t2)];
+        else p[plane_idx(B, d, n + 1, t1, t2)] = factor + sgn * p[plane_idx(B, d, n, t1, t2)];
+      } else if (ctype == 'D') {
+        if (ibound == 0) p[plane_idx(B, d, 0, t1, t2)] = factor;
+        else {
+          p[plane_idx(B, d, n, t1, t2)] = factor;
+          p[plane_idx(B, d, n + 1, t1, t2)] = p[plane_idx(B, d, n - 1, t1, t2)];
+        }
+      } else { /* staggered N */
+        if (ibound == 0) p[plane_idx(B, d, 0, t1, t2)] = factor + p[plane_idx(B, d, 1, t1, t2)];
+        else {
+          p[plane_idx(B, d, n, t1, t2)] = factor + p[plane_idx(B, d, n - 1, t1, t2)];
+          p[plane_idx(B, d, n + 1, t1, t2)] = p[plane_idx(B, d, n, t1, t2)];
+        }
+      }
+    }
+}
+
+/* updthalo bound.f90:587-650: with every block on one address space, the two
+ * MPI_SENDRECVs of direction d become plane copies from the neighbour blocks'
+ * first/last interior planes (full ghosted tangential extents, which match
+ * between friend blocks, initmpi.f90:165-175).  Reads touch only interior
+ * planes along d and writes only ghost planes along d, so the sweep over
+ * blocks is order independent.                                               */
+static void updthalo(oc_case *c, int fid, int d) {
+  for (int ib = 0; ib < c->nb; ib++) {
+    ob *B = &c->b[ib];
+    int a1, a2; tang(d, &a1, &a2);
+    int m1 = B->n[a1] + 2, m2 = B->n[a2] + 2;
+    for (int side = 0; side < 2; side++) {
+      int jb = B->nb[side][d];
+      if (jb < 0) continue;
+      ob *N = &c->b[jb];
+      int qdst = side == 0 ? 0 : B->n[d] + 1;
+      int qsrc = side == 0 ? N->n[d] : 1;
+      double *dst = B->f[fid];
+      const double *src = N->f[fid];
+      for (int t2 = 0; t2 < m2; t2++)
+        for (int t1 = 0; t1 < m1; t1++)
+          dst[plane_idx(B, d, qdst, t1, t2)] = src[plane_idx(N, d, qsrc, t1, t2)];
+    }
+  }
+}
+
+/* bounduvw bound.f90:8-66                                                   */
+void oc_bounduvw(oc_case *c, int is_correc) {
+  for (int f = 0; f < 3; f++)
+    for (int d = 0; d < 3; d++) updthalo(c, f, d);
+  for (int ib = 0; ib < c->nb; ib++) {
+    ob *B = &c->b[ib];
+    for (int d = 0; d < 3; d++)
+      for (int side = 0; side < 2; side++) {
+        if (!B->is_bound[side][d]) continue;
+        int qm = side == 0 ? 0 : B->n[d]; /* dr index: lo-1 or hi */
+        for (int comp = 0; comp < 3; comp++) {
+          int centered = comp != d;
+          if (!centered && is_correc) continue;
+          double dr = centered ? B->dc[d][qm] : B->df[d][qm];
+          set_bc(B, B->f[comp], B->cbcvel[comp][d][side], side, d, centered,
+                 B->bcvel[comp][d][side], dr);
+        }
+      }
+  }
+}
+
+/* boundp bound.f90:68-106                                                   */
+void oc_boundp(oc_case *c, int which) {
+  for (int d = 0; d < 3; d++) updthalo(c, which, d);
+  for (int ib = 0; ib < c->nb; ib++) {
+    ob *B = &c->b[ib];
+    for (int d = 0; d < 3; d++)
+      for (int side = 0; side < 2; side++) {
+        if (!B->is_bound[side][d]) continue;
+        int qm = side == 0 ? 0 : B->n[d];
+        set_bc(B, B->f[which], B->cbcpre[d][side], side, d, 1, B->bcpre[d][side], B->dc[d][qm]);
+      }
+  }
+}
+
+/* inflow bound.f90:438-477                                                  */
+void oc_inflow(oc_case *c, int is_correc) {
+  for (int ib = 0; ib < c->nb; ib++) {
+    ob *B = &c->b[ib];
+    for (int d = 0; d < 3; d++)
+      for (int side = 0; side < 2; side++) {
+        if (!B->is_inflow[d][side]) continue;
+        int n = B->n[d];
+        int qn = side == 0 ? 0 : n;
+        int qt = side == 0 ? 0 : n + 1;
+        int dq = side == 0 ? 1 : -1;
+        int a1, a2; tang(d, &a1, &a2);
+        int m1 = B->n[a1] + 2, m2 = B->n[a2] + 2;
+        for (int comp = 0; comp < 3; comp++) {
+          double *F = B->f[comp];
+          const double *vin = B->vin[d][comp][side];
+          for (int t2 = 0; t2 < m2; t2++)
+            for (int t1 = 0; t1 < m1; t1++) {
+              double val = vin[t1 + (long)m1 * t2];
+              if (comp == d) {
+                if (!is_correc) F[plane_idx(B, d, qn, t1, t2)] = val;
+              } else {
+                F[plane_idx(B, d, qt, t1, t2)] = -F[plane_idx(B, d, qt + dq, t1, t2)] + 2.0 * val;
+              }
+            }
+        }
+      }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* momentum: rk.f90:7-128 with mom.f90:112-386                              */
+/* ------------------------------------------------------------------------ */
+static const double RKC[3][2] = {{32.0 / 60.0, 0.0}, {25.0 / 60.0, -17.0 / 60.0}, {45.0 / 60.0, -25.0 / 60.0}};
+
+static void rk_mom_block(oc_case *c, ob *B, int irk, double dt) {
+  const double f1 = RKC[irk - 1][0] * dt, f2 = RKC[irk - 1][1] * dt, f12 = f1 + f2;
+  const double visc = c->visc;
+  const int nx = B->n[0], ny = B->n[1], nz = B->n[2];
+  const double *dxc = B->dc[0], *dxf = B->df[0], *dyc = B->dc[1], *dyf = B->df[1],
+               *dzc = B->dc[2], *dzf = B->df[2];
+  double *u = B->f[0], *v = B->f[1], *w = B->f[2];
+  const double *p = B->f[3];
+  long nin = (long)nx * ny * nz;
+  double *du = xcalloc(nin, sizeof(double)), *dv = xcalloc(nin, sizeof(double)),
+         *dw = xcalloc(nin, sizeof(double));
+#define U(i, j, k) u[IDX(B, i, j, k)]
+#define V(i, j, k) v[IDX(B, i, j, k)]
+#define W(i, j, k) w[IDX(B, i, j, k)]
+#define P(i, j, k) p[IDX(B, i, j, k)]
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        long q = IIDX(B, i, j, k);
+        double a = 0.0, b = 0.0, cc = 0.0;
+        /* momx_d mom.f90:221-256 */
+        {
+          double dudxp = (U(i + 1, j, k) - U(i, j, k)) / dxf[i + 1];
+          double dudxm = (U(i, j, k) - U(i - 1, j, k)) / dxf[i];
+          double dudyp = (U(i, j + 1, k) - U(i, j, k)) / dyc[j];
+          double dudym = (U(i, j, k) - U(i, j - 1, k)) / dyc[j - 1];
+          double dudzp = (U(i, j, k + 1) - U(i, j, k)) / dzc[k];
+          double dudzm = (U(i, j, k) - U(i, j, k - 1)) / dzc[k - 1];
+          a = a + visc * (dudxp - dudxm) / dxc[i] + visc * (dudyp - dudym) / dyf[j] +
+              visc * (dudzp - dudzm) / dzf[k];
+        }
+        /* momy_d mom.f90:257-292 */
+        {
+          double dvdxp = (V(i + 1, j, k) - V(i, j, k)) / dxc[i];
+          double dvdxm = (V(i, j, k) - V(i - 1, j, k)) / dxc[i - 1];
+          double dvdyp = (V(i, j + 1, k) - V(i, j, k)) / dyf[j + 1];
+          double dvdym = (V(i, j, k) - V(i, j - 1, k)) / dyf[j];
+          double dvdzp = (V(i, j, k + 1) - V(i, j, k)) / dzc[k];
+          double dvdzm = (V(i, j, k) - V(i, j, k - 1)) / dzc[k - 1];
+          b = b + visc * (dvdxp - dvdxm) / dxf[i] + visc * (dvdyp - dvdym) / dyc[j] +
+              visc * (dvdzp - dvdzm) / dzf[k];
+        }
+        /* momz_d mom.f90:293-327 */
+        {
+          double dwdxp = (W(i + 1, j, k) - W(i, j, k)) / dxc[i];
+          double dwdxm = (W(i, j, k) - W(i - 1, j, k)) / dxc[i - 1];
+          double dwdyp = (W(i, j + 1, k) - W(i, j, k)) / dyc[j];
+          double dwdym = (W(i, j, k) - W(i, j - 1, k)) / dyc[j - 1];
+          double dwdzp = (W(i, j, k + 1) - W(i, j, k)) / dzf[k + 1];
+          double dwdzm = (W(i, j, k) - W(i, j, k - 1)) / dzf[k];
+          cc = cc + visc * (dwdxp - dwdxm) / dxf[i] + visc * (dwdyp - dwdym) / dyf[j] +
+               visc * (dwdzp - dwdzm) / dzc[k];
+        }
+        /* momx_a_vv mom.f90:112-147 */
+        {
+          double uuip = 0.25 * (U(i, j, k) * dyf[j] * dzf[k] + U(i + 1, j, k) * dyf[j] * dzf[k]) * U(i + 1, j, k);
+          double uuim = 0.25 * (U(i, j, k) * dyf[j] * dzf[k] + U(i - 1, j, k) * dyf[j] * dzf[k]) * U(i - 1, j, k);
+          double uvjp = 0.25 * (V(i, j, k) * dxf[i] * dzf[k] + V(i + 1, j, k) * dxf[i + 1] * dzf[k]) * U(i, j + 1, k);
+          double uvjm = 0.25 * (V(i, j - 1, k) * dxf[i] * dzf[k] + V(i + 1, j - 1, k) * dxf[i + 1] * dzf[k]) * U(i, j - 1, k);
+          double uwkp = 0.25 * (W(i, j, k) * dxf[i] * dyf[j] + W(i + 1, j, k) * dxf[i + 1] * dyf[j]) * U(i, j, k + 1);
+          double uwkm = 0.25 * (W(i, j, k - 1) * dxf[i] * dyf[j] + W(i + 1, j, k - 1) * dxf[i + 1] * dyf[j]) * U(i, j, k - 1);
+          a = a + ((-uuip + uuim) + (-uvjp + uvjm) + (-uwkp + uwkm)) / (dxc[i] * dyf[j] * dzf[k]);
+        }
+        /* momy_a_vv mom.f90:148-183 */
+        {
+          double uvip = 0.25 * (U(i, j, k) * dyf[j] * dzf[k] + U(i, j + 1, k) * dyf[j + 1] * dzf[k]) * V(i + 1, j, k);
+          double uvim = 0.25 * (U(i - 1, j, k) * dyf[j] * dzf[k] + U(i - 1, j + 1, k) * dyf[j + 1] * dzf[k]) * V(i - 1, j, k);
+          double vvjp = 0.25 * (V(i, j, k) * dxf[i] * dzf[k] + V(i, j + 1, k) * dxf[i] * dzf[k]) * V(i, j + 1, k);
+          double vvjm = 0.25 * (V(i, j, k) * dxf[i] * dzf[k] + V(i, j - 1, k) * dxf[i] * dzf[k]) * V(i, j - 1, k);
+          double wvkp = 0.25 * (W(i, j, k) * dyf[j] * dxf[i] + W(i, j + 1, k) * dyf[j + 1] * dxf[i]) * V(i, j, k + 1);
+          double wvkm = 0.25 * (W(i, j, k - 1) * dyf[j] * dxf[i] + W(i, j + 1, k - 1) * dyf[j + 1] * dxf[i]) * V(i, j, k - 1);
+          b = b + ((-uvip + uvim) + (-vvjp + vvjm) + (-wvkp + wvkm)) / (dxf[i] * dyc[j] * dzf[k]);
+        }
+        /* momz_a_vv mom.f90:185-219 */
+        {
+          double uwip = 0.25 * (U(i, j, k) * dzf[k] * dyf[j] + U(i, j, k + 1) * dzf[k + 1] * dyf[j]) * W(i + 1, j, k);
+          double uwim = 0.25 * (U(i - 1, j, k) * dzf[k] * dyf[j] + U(i - 1, j, k + 1) * dzf[k + 1] * dyf[j]) * W(i - 1, j, k);
+          double vwjp = 0.25 * (V(i, j, k) * dzf[k] * dxf[i] + V(i, j, k + 1) * dzf[k + 1] * dxf[i]) * W(i, j + 1, k);
+          double vwjm = 0.25 * (V(i, j - 1, k) * dzf[k] * dxf[i] + V(i, j - 1, k + 1) * dzf[k + 1] * dxf[i]) * W(i, j - 1, k);
+          double wwkp = 0.25 * (W(i, j, k) * dxf[i] * dyf[j] + W(i, j, k + 1) * dxf[i] * dyf[j]) * W(i, j, k + 1);
+          double wwkm = 0.25 * (W(i, j, k) * dxf[i] * dyf[j] + W(i, j, k - 1) * dxf[i] * dyf[j]) * W(i, j, k - 1);
+          cc = cc + ((-uwip + uwim) + (-vwjp + vwjm) + (-wwkp + wwkm)) / (dxf[i] * dyf[j] * dzc[k]);
+        }
+        du[q] = a; dv[q] = b; dw[q] = cc;
+      }
+  /* explicit update rk.f90:68-85 */
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        long q = IIDX(B, i, j, k), g = IDX(B, i, j, k);
+        u[g] = u[g] + f1 * du[q] + f2 * B->dto[0][q];
+        v[g] = v[g] + f1 * dv[q] + f2 * B->dto[1][q];
+        w[g] = w[g] + f1 * dw[q] + f2 * B->dto[2][q];
+        B->dto[0][q] = du[q]; B->dto[1][q] = dv[q]; B->dto[2][q] = dw[q];
+      }
+  /* pressure gradient + body force mom.f90:328-386, update rk.f90:86-111 */
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        long g = IDX(B, i, j, k);
+        double px = -(P(i + 1, j, k) - P(i, j, k)) / dxc[i] + c->bforce[0];
+        double py = -(P(i, j + 1, k) - P(i, j, k)) / dyc[j] + c->bforce[1];
+        double pz = -(P(i, j, k + 1) - P(i, j, k)) / dzc[k] + c->bforce[2];
+        u[g] = u[g] + f12 * px;
+        v[g] = v[g] + f12 * py;
+        w[g] = w[g] + f12 * pz;
+      }
+#undef U
+#undef V
+#undef W
+#undef P
+  free(du); free(dv); free(dw);
+}
+
+void oc_rk_mom(oc_case *c, int irk, double dt) {
+  for (int ib = 0; ib < c->nb; ib++) rk_mom_block(c, &c->b[ib], irk, dt);
+}
+
+/* ------------------------------------------------------------------------ */
+/* fillps.f90:7-54 + updt_rhs bound.f90:543-585                              */
+/* ------------------------------------------------------------------------ */
+void oc_fillps(oc_case *c, double dtrk) {
+  for (int ib = 0; ib < c->nb; ib++) {
+    ob *B = &c->b[ib];
+    const double *dxf = B->df[0], *dyf = B->df[1], *dzf = B->df[2];
+    const double *u = B->f[0], *v = B->f[1], *w = B->f[2];
+    double *pp = B->f[4];
+    int nx = B->n[0], ny = B->n[1], nz = B->n[2];
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k <= nz; k++)
+      for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+          double dw = w[IDX(B, i, j, k)] - w[IDX(B, i, j, k - 1)];
+          double dv = v[IDX(B, i, j, k)] - v[IDX(B, i, j - 1, k)];
+          double du = u[IDX(B, i, j, k)] - u[IDX(B, i - 1, j, k)];
+          double s;
+          switch (c->fft_axis) {
+          case 3: s = dw / dzf[k] * dxf[i] * dyf[j] + dv * dxf[i] + du * dyf[j]; break;
+          case 2: s = dw * dxf[i] + dv / dyf[j] * dxf[i] * dzf[k] + du * dzf[k]; break;
+          case 1: s = dw * dyf[j] + dv * dzf[k] + du / dxf[i] * dyf[j] * dzf[k]; break;
+          default: s = dw * dxf[i] * dyf[j] + dv * dxf[i] * dzf[k] + du * dyf[j] * dzf[k]; break;
+          }
+          pp[IDX(B, i, j, k)] = s / dtrk;
+        }
+    /* updt_rhs: x, y, z boundary planes in that order */
+    for (int d = 0; d < 3; d++)
+      for (int side = 0; side < 2; side++) {
+        if (!B->is_bound[side][d] || !B->rhsp[d][side]) continue;
+        int a1, a2; tang(d, &a1, &a2);
+        int q = side == 0 ? 1 : B->n[d];
+        for (int t2 = 1; t2 <= B->n[a2]; t2++)
+          for (int t1 = 1; t1 <= B->n[a1]; t1++)
+            pp[plane_idx(B, d, q, t1, t2)] += B->rhsp[d][side][(t1 - 1) + (long)B->n[a1] * (t2 - 1)];
+      }
+  }
+}
+
+/* correc.f90:7-48                                                           */
+void oc_correc(oc_case *c, double dtrk) {
+  for (int ib = 0; ib < c->nb; ib++) {
+    ob *B = &c->b[ib];
+    const double *p = B->f[4];
+    double *u = B->f[0], *v = B->f[1], *w = B->f[2];
+    int nx = B->n[0], ny = B->n[1], nz = B->n[2];
+    for (int k = 0; k <= nz + 1; k++)
+      for (int j = 0; j <= ny + 1; j++)
+        for (int i = 0; i <= nx; i++)
+          u[IDX(B, i, j, k)] = u[IDX(B, i, j, k)] - dtrk * (p[IDX(B, i + 1, j, k)] - p[IDX(B, i, j, k)]) / B->dc[0][i];
+    for (int k = 0; k <= nz + 1; k++)
+      for (int j = 0; j <= ny; j++)
+        for (int i = 0; i <= nx + 1; i++)
+          v[IDX(B, i, j, k)] = v[IDX(B, i, j, k)] - dtrk * (p[IDX(B, i, j + 1, k)] - p[IDX(B, i, j, k)]) / B->dc[1][j];
+    for (int k = 0; k <= nz; k++)
+      for (int j = 0; j <= ny + 1; j++)
+        for (int i = 0; i <= nx + 1; i++)
+          w[IDX(B, i, j, k)] = w[IDX(B, i, j, k)] - dtrk * (p[IDX(B, i, j, k + 1)] - p[IDX(B, i, j, k)]) / B->dc[2][k];
+  }
+}
+
+/* updt_pressure.f90:23-29 (no _IMPDIFF term)                                */
+void oc_updt_pressure(oc_case *c) {
+  for (int ib = 0; ib < c->nb; ib++) {
+    ob *B = &c->b[ib];
+    for (int k = 1; k <= B->n[2]; k++)
+      for (int j = 1; j <= B->n[1]; j++)
+        for (int i = 1; i <= B->n[0]; i++) B->f[3][IDX(B, i, j, k)] += B->f[4][IDX(B, i, j, k)];
+  }
+}
+
+/* chkdiv.f90:8-44                                                           */
+void oc_chkdiv(const oc_case *c, double *divtot, double *divmax) {
+  double tot = 0.0, mx = 0.0;
+  for (int ib = 0; ib < c->nb; ib++) {
+    const ob *B = &c->b[ib];
+    const double *u = B->f[0], *v = B->f[1], *w = B->f[2];
+    for (int k = 1; k <= B->n[2]; k++)
+      for (int j = 1; j <= B->n[1]; j++)
+        for (int i = 1; i <= B->n[0]; i++) {
+          double div = (w[IDX(B, i, j, k)] - w[IDX(B, i, j, k - 1)]) / B->df[2][k] +
+                       (v[IDX(B, i, j, k)] - v[IDX(B, i, j - 1, k)]) / B->df[1][j] +
+                       (u[IDX(B, i, j, k)] - u[IDX(B, i - 1, j, k)]) / B->df[0][i];
+          if (fabs(div) > mx) mx = fabs(div);
+          tot = tot + div * B->df[0][i] * B->df[1][j] * B->df[2][k] / c->vol_all;
+        }
+  }
+  *divtot = tot;
+  *divmax = mx;
+}
+
+/* chkdt.f90:8-56 (explicit diffusion branch)                                */
+double oc_chkdt(const oc_case *c) {
+  double dti = 0.0, dlmin = 1e300;
+  for (int ib = 0; ib < c->nb; ib++) {
+    const ob *B = &c->b[ib];
+    const double *u = B->f[0], *v = B->f[1], *w = B->f[2];
+    const double *dxc = B->dc[0], *dxf = B->df[0], *dyc = B->dc[1], *dyf = B->df[1],
+                 *dzc = B->dc[2], *dzf = B->df[2];
+    for (int k = 1; k <= B->n[2]; k++)
+      for (int j = 1; j <= B->n[1]; j++)
+        for (int i = 1; i <= B->n[0]; i++) {
+#define U(i, j, k) u[IDX(B, i, j, k)]
+#define V(i, j, k) v[IDX(B, i, j, k)]
+#define W(i, j, k) w[IDX(B, i, j, k)]
+          double ux = fabs(U(i, j, k));
+          double vx = 0.25 * fabs(V(i, j, k) + V(i, j - 1, k) + V(i + 1, j, k) + V(i + 1, j - 1, k));
+          double wx = 0.25 * fabs(W(i, j, k) + W(i, j, k - 1) + W(i + 1, j, k) + W(i + 1, j, k - 1));
+          double dtix = ux / dxc[i] + vx / dyf[j] + wx / dzf[k];
+          double uy = 0.25 * fabs(U(i, j, k) + U(i, j + 1, k) + U(i - 1, j + 1, k) + U(i - 1, j, k));
+          double vy = fabs(V(i, j, k));
+          double wy = 0.25 * fabs(W(i, j, k) + W(i, j + 1, k) + W(i, j + 1, k - 1) + W(i, j, k - 1));
+          double dtiy = uy / dxf[i] + vy / dyc[j] + wy / dzf[k];
+          double uz = 0.25 * fabs(U(i, j, k) + U(i - 1, j, k) + U(i - 1, j, k + 1) + U(i, j, k + 1));
+          double vz = 0.25 * fabs(V(i, j, k) + V(i, j - 1, k) + V(i, j - 1, k + 1) + V(i, j, k + 1));
+          double wz = fabs(W(i, j, k));
+          double dtiz = uz / dxf[i] + vz / dyf[j] + wz / dzc[k];
+#undef U
+#undef V
+#undef W
+          if (dtix > dti) dti = dtix;
+          if (dtiy > dti) dti = dtiy;
+          if (dtiz > dti) dti = dtiz;
+        }
+    for (int d = 0; d < 3; d++)
+      for (int q = 0; q <= B->n[d] + 1; q++)
+        if (B->df[d][q] < dlmin) dlmin = B->df[d][q];
+  }
+  if (dti == 0.0) dti = 1.0;
+  double dtmax = sqrt(3.0) / dti;
+  double dd = 1.65 / 12.0 / c->visc * dlmin * dlmin;
+  return dtmax < dd ? dtmax : dd;
+}
+
+/* ------------------------------------------------------------------------ */
+/* r2r transforms: FFTW kinds as documented (fft.f90:96-149 chooses them)    */
+/* ------------------------------------------------------------------------ */
+int oc_r2r_direct(int kind, int n, const double *x, double *y) {
+  if (n < 1) return -1;
+  for (int k = 0; k < n; k++) {
+    double s = 0.0;
+    switch (kind) {
+    case OC_R2HC: {
+      /* halfcomplex: r0..r_{n/2}, i_{(n+1)/2-1}..i_1 ; forward sign -1 */
+      if (k <= n / 2) {
+        for (int j = 0; j < n; j++) s += x[j] * cos(2.0 * PI * (double)j * k / n);
+      } else {
+        int kk = n - k;
+        for (int j = 0; j < n; j++) s += -x[j] * sin(2.0 * PI * (double)j * kk / n);
+      }
+      break;
+    }
+    case OC_HC2R: {
+      /* y_k = r0 + 2 sum_{j=1}^{(n-1)/2}(r_j cos - i_j sin) + r_{n/2}(-1)^k (n even) */
+      s = x[0];
+      for (int j = 1; j < (n + 1) / 2; j++)
+        s += 2.0 * (x[j] * cos(2.0 * PI * (double)j * k / n) - x[n - j] * sin(2.0 * PI * (double)j * k / n));
+      if (n % 2 == 0) s += x[n / 2] * ((k % 2) ? -1.0 : 1.0);
+      break;
+    }
+    case OC_REDFT10:
+      for (int j = 0; j < n; j++) s += 2.0 * x[j] * cos(PI * (j + 0.5) * k / n);
+      break;
+    case OC_REDFT01:
+      s = x[0];
+      for (int j = 1; j < n; j++) s += 2.0 * x[j] * cos(PI * j * (k + 0.5) / n);
+      break;
+    case OC_RODFT10:
+      for (int j = 0; j < n; j++) s += 2.0 * x[j] * sin(PI * (j + 0.5) * (k + 1) / n);
+      break;
+    case OC_RODFT01:
+      s = ((k % 2) ? -1.0 : 1.0) * x[n - 1];
+      for (int j = 0; j < n - 1; j++) s += 2.0 * x[j] * sin(PI * (j + 1) * (k + 0.5) / n);
+      break;
+    case OC_REDFT11:
+      for (int j = 0; j < n; j++) s += 2.0 * x[j] * cos(PI * (j + 0.5) * (k + 0.5) / n);
+      break;
+    case OC_RODFT11:
+      for (int j = 0; j < n; j++) s += 2.0 * x[j] * sin(PI * (j + 0.5) * (k + 0.5) / n);
+      break;
+    case OC_REDFT00:
+      if (n < 2) return -1;
+      s = x[0] + ((k % 2) ? -1.0 : 1.0) * x[n - 1];
+      for (int j = 1; j < n - 1; j++) s += 2.0 * x[j] * cos(PI * j * k / (n - 1));
+      break;
+    case OC_RODFT00:
+      for (int j = 0; j < n; j++) s += 2.0 * x[j] * sin(PI * (j + 1) * (k + 1) / (n + 1));
+      break;
+    default: return -2;
+    }
+    y[k] = s;
+  }
+  return 0;
+}
+
+/* iterative radix-2 complex FFT, sign = -1 forward, +1 backward (unscaled)  */
+static void cfft_pow2(int m, double *re, double *im, int sign) {
+  for (int i = 1, j = 0; i < m; i++) {
+    int bit = m >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) { double t = re[i]; re[i] = re[j]; re[j] = t; t = im[i]; im[i] = im[j]; im[j] = t; }
+  }
+  for (int len = 2; len <= m; len <<= 1) {
+    double ang = sign * 2.0 * PI / len;
+    for (int i = 0; i < m; i += len)
+      for (int j = 0; j < len / 2; j++) {
+        double wr = cos(ang * j), wi = sin(ang * j);
+        int a = i + j, b = i + j + len / 2;
+        double xr = re[b] * wr - im[b] * wi, xi = re[b] * wi + im[b] * wr;
+        re[b] = re[a] - xr; im[b] = im[a] - xi;
+        re[a] += xr; im[a] += xi;
+      }
+  }
+}
+
+static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+/* r2r through complex FFTs of the symmetric extension (n a power of two)   */
+int oc_r2r(int kind, int n, const double *x, double *y) {
+  if (!is_pow2(n) || n < 2 || kind == OC_REDFT00 || kind == OC_RODFT00) return oc_r2r_direct(kind, n, x, y);
+  int m = 2 * n;
+  double *re = xcalloc(m, sizeof(double)), *im = xcalloc(m, sizeof(double));
+  switch (kind) {
+  case OC_R2HC:
+    for (int j = 0; j < n; j++) re[j] = x[j];
+    cfft_pow2(n, re, im, -1);
+    for (int k = 0; k <= n / 2; k++) y[k] = re[k];
+    for (int k = 1; k < (n + 1) / 2; k++) y[n - k] = im[k];
+    break;
+  case OC_HC2R:
+    re[0] = x[0];
+    for (int k = 1; k < (n + 1) / 2; k++) {
+      re[k] = x[k]; im[k] = x[n - k];
+      re[n - k] = x[k]; im[n - k] = -x[n - k];
+    }
+    if (n % 2 == 0) re[n / 2] = x[n / 2];
+    cfft_pow2(n, re, im, +1);
+    for (int k = 0; k < n; k++) y[k] = re[k];
+    break;
+  case OC_REDFT10:
+  case OC_RODFT10: {
+    double sg = 1.0;
+    for (int j = 0; j < n; j++) {
+      double xv = (kind == OC_RODFT10) ? sg * x[j] : x[j];
+      re[j] = xv; re[m - 1 - j] = xv;
+      sg = -sg;
+    }
+    cfft_pow2(m, re, im, -1);
+    for (int k = 0; k < n; k++) {
+      double a = -PI * k / m; /* e^{-i pi k/(2n)} */
+      double val = re[k] * cos(a) - im[k] * sin(a);
+      if (kind == OC_REDFT10) y[k] = val; else y[n - 1 - k] = val;
+    }
+    break;
+  }
+  case OC_REDFT01:
+  case OC_RODFT01: {
+    for (int j = 0; j < n; j++) {
+      double xv = (kind == OC_RODFT01) ? x[n - 1 - j] : x[j];
+      double wgt = j == 0 ? 1.0 : 2.0;
+      double a = PI * j / m;
+      re[j] = wgt * xv * cos(a); im[j] = wgt * xv * sin(a);
+    }
+    cfft_pow2(m, re, im, +1);
+    for (int k = 0; k < n; k++) y[k] = (kind == OC_RODFT01 && (k % 2)) ? -re[k] : re[k];
+    break;
+  }
+  case OC_REDFT11:
+  case OC_RODFT11: {
+    for (int j = 0; j < n; j++) {
+      double a = -PI * j / m;
+      re[j] = x[j] * cos(a); im[j] = x[j] * sin(a);
+    }
+    cfft_pow2(m, re, im, -1);
+    for (int k = 0; k < n; k++) {
+      double a = -PI * (k + 0.5) / m;
+      double zr = re[k] * cos(a) - im[k] * sin(a), zi = re[k] * sin(a) + im[k] * cos(a);
+      y[k] = (kind == OC_REDFT11) ? 2.0 * zr : -2.0 * zi;
+    }
+    break;
+  }
+  default: free(re); free(im); return -2;
+  }
+  free(re); free(im);
+  return 0;
+}
+
+/* find_fft fft.f90:96-149                                                   */
+int oc_find_fft(const char *bc, int is_centered, int *kf, int *kb, double *norm) {
+  char a = bc[0], b = bc[1];
+  if (is_centered) {
+    if (a == 'F' && b == 'F') { *kf = OC_R2HC; *kb = OC_HC2R; norm[0] = 1; norm[1] = 0; }
+    else if (a == 'N' && b == 'N') { *kf = OC_REDFT10; *kb = OC_REDFT01; norm[0] = 2; norm[1] = 0; }
+    else if (a == 'D' && b == 'D') { *kf = OC_RODFT10; *kb = OC_RODFT01; norm[0] = 2; norm[1] = 0; }
+    else if (a == 'N' && b == 'D') { *kf = OC_REDFT11; *kb = OC_REDFT11; norm[0] = 2; norm[1] = 0; }
+    else if (a == 'D' && b == 'N') { *kf = OC_RODFT11; *kb = OC_RODFT11; norm[0] = 2; norm[1] = 0; }
+    else return -1;
+  } else {
+    if (a == 'F' && b == 'F') { *kf = OC_R2HC; *kb = OC_HC2R; norm[0] = 1; norm[1] = 0; }
+    else if (a == 'N' && b == 'N') { *kf = OC_REDFT00; *kb = OC_REDFT00; norm[0] = 2; norm[1] = -1; }
+    else if (a == 'D' && b == 'D') { *kf = OC_RODFT00; *kb = OC_RODFT00; norm[0] = 2; norm[1] = 1; }
+    else if (a == 'N' && b == 'D') { *kf = OC_REDFT10; *kb = OC_REDFT01; norm[0] = 2; norm[1] = 0; }
+    else if (a == 'D' && b == 'N') { *kf = OC_RODFT01; *kb = OC_RODFT10; norm[0] = 2; norm[1] = 0; }
+    else return -1;
+  }
+  return 0;
+}
+
+/* eigenvalues fft.f90:151-189; the reference has no 'DN' case (its lambda
+ * would stay undefined); DN has the spectrum of ND by reflection.            */
+int oc_eigenvalues(int n, const char *bc, int is_centered, double *lam) {
+  char a = bc[0], b = bc[1];
+  if (a == 'F' && b == 'F') {
+    for (int l = 1; l <= n; l++) { double s = sin((double)(l - 1) * PI / (double)n); lam[l - 1] = -4.0 * s * s; }
+  } else if (a == 'N' && b == 'N') {
+    for (int l = 1; l <= n; l++) {
+      double s = is_centered ? sin((double)(l - 1) * PI / (2.0 * n)) : sin((double)(l - 1) * PI / (2.0 * (n - 1 + 1)));
+      lam[l - 1] = -4.0 * s * s;
+    }
+  } else if (a == 'D' && b == 'D') {
+    if (is_centered) {
+      for (int l = 1; l <= n; l++) { double s = sin((double)l * PI / (2.0 * n)); lam[l - 1] = -4.0 * s * s; }
+    } else {
+      for (int l = 1; l <= n - 1; l++) { double s = sin((double)l * PI / (2.0 * (n + 1 - 1))); lam[l - 1] = -4.0 * s * s; }
+      lam[n - 1] = 0.0;
+    }
+  } else if ((a == 'N' && b == 'D') || (a == 'D' && b == 'N')) {
+    for (int l = 1; l <= n; l++) { double s = sin((double)(2 * l - 1) * PI / (4.0 * n)); lam[l - 1] = -4.0 * s * s; }
+  } else return -1;
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* case set-up                                                               */
+/* ------------------------------------------------------------------------ */
+static void seterr(char *buf, int len, const char *msg, int blk) {
+  if (buf && len > 0) snprintf(buf, (size_t)len, "%s (block %d)", msg, blk);
+}
+
+/* poiseuille_square initflow.f90:~355-367                                   */
+static double poiseuille(double r, double lmin, double lmax) {
+  double rr = (r - lmin) / (lmax - lmin);
+  return 6.0 * (rr * (1.0 - rr));
+}
+
+static void assemble_operator(oc_case *c);
+
+oc_case *oc_create(int nblocks, const int *ng, const double *lmin, const double *lmax,
+                   const int *gt, const double *gr, const char *cbcvel, const double *bcvel,
+                   const char *cbcpre, const double *bcpre, const int *inflow_type,
+                   int fft_axis, double visci, const double *bforce, char *errbuf, int errlen) {
+  if (nblocks < 1 || nblocks > MAXB) { seterr(errbuf, errlen, "invalid number of blocks", nblocks); return NULL; }
+  oc_case *c = xcalloc(1, sizeof(oc_case));
+  c->nb = nblocks;
+  c->fft_axis = fft_axis;
+  c->visc = 1.0 / visci;
+  for (int d = 0; d < 3; d++) c->bforce[d] = bforce[d];
+  c->method = OC_SOLVE_AUTO; c->tol = 1e-13; c->maxiter = 20000;
+  for (int ib = 0; ib < nblocks; ib++) {
+    ob *B = &c->b[ib];
+    for (int d = 0; d < 3; d++) {
+      B->n[d] = ng[ib * 3 + d];
+      B->lmin[d] = lmin[ib * 3 + d]; B->lmax[d] = lmax[ib * 3 + d];
+      B->gt[d] = gt[ib * 3 + d]; B->gr[d] = gr[ib * 3 + d];
+      if (B->n[d] < 1) { seterr(errbuf, errlen, "invalid grid size", ib + 1); oc_destroy(c); return NULL; }
+      for (int s = 0; s < 2; s++) {
+        B->cbcpre[d][s] = cbcpre[(ib * 3 + d) * 2 + s];
+        B->bcpre[d][s] = bcpre[(ib * 3 + d) * 2 + s];
+        B->inflow_type[d][s] = inflow_type[(ib * 3 + d) * 2 + s];
+        for (int cp = 0; cp < 3; cp++) {
+          B->cbcvel[cp][d][s] = cbcvel[((ib * 3 + cp) * 3 + d) * 2 + s];
+          B->bcvel[cp][d][s] = bcvel[((ib * 3 + cp) * 3 + d) * 2 + s];
+        }
+      }
+    }
+  }
+  /* is_periodic param.f90:220-225: FF pressure pairs on every block          */
+  for (int d = 0; d < 3; d++) {
+    c->is_periodic[d] = 1;
+    for (int ib = 0; ib < nblocks; ib++)
+      if (!(c->b[ib].cbcpre[d][0] == 'F' && c->b[ib].cbcpre[d][1] == 'F')) c->is_periodic[d] = 0;
+  }
+  /* block placement initmpi.f90:61-98 (BFS over 'F' links, block 1 at lo=1) */
+  {
+    int done[MAXB] = {0}, unl[MAXB] = {0};
+    int ib = 0;
+    c->b[0].lo_g[0] = c->b[0].lo_g[1] = c->b[0].lo_g[2] = 1;
+    unl[0] = 1;
+    for (;;) {
+      if (ib < 0) { seterr(errbuf, errlen, "invalid connectivity", 0); oc_destroy(c); return NULL; }
+      ob *B = &c->b[ib];
+      for (int d = 0; d < 3; d++)
+        for (int inb = 0; inb < 2; inb++) {
+          if (B->cbcpre[d][inb] != 'F') continue;
+          int fr = (int)lround(B->bcpre[d][inb]) - 1;
+          if (fr < 0 || fr >= nblocks) { seterr(errbuf, errlen, "friend block does not exist", ib + 1); oc_destroy(c); return NULL; }
+          if (unl[fr]) continue;
+          if (inb == 0 && c->b[fr].lmin[d] > B->lmin[d]) continue;
+          if (inb == 1 && c->b[fr].lmin[d] < B->lmin[d]) continue;
+          int ioff = -(1 - inb) * c->b[fr].n[d] + inb * B->n[d];
+          for (int e = 0; e < 3; e++) c->b[fr].lo_g[e] = B->lo_g[e];
+          c->b[fr].lo_g[d] = B->lo_g[d] + ioff;
+          unl[fr] = 1;
+        }
+      done[ib] = 1;
+      int all = 1;
+      for (int q = 0; q < nblocks; q++) all &= done[q];
+      if (all) break;
+      ib = -1;
+      for (int q = 0; q < nblocks; q++) if (!done[q] && unl[q]) { ib = q; break; }
+    }
+  }
+  /* periods initmpi.f90:104-113 */
+  for (int d = 0; d < 3; d++) {
+    int mn = 1 << 30, mx = -(1 << 30);
+    for (int ib = 0; ib < nblocks; ib++) {
+      if (c->b[ib].lo_g[d] < mn) mn = c->b[ib].lo_g[d];
+      int hi = c->b[ib].lo_g[d] + c->b[ib].n[d] - 1;
+      if (hi > mx) mx = hi;
+    }
+    c->periods[d] = c->is_periodic[d] ? mx - mn + 1 : 0;
+  }
+  /* neighbours initmpi.f90:150-239 (one piece per block) */
+  for (int ib = 0; ib < nblocks; ib++) {
+    ob *B = &c->b[ib];
+    for (int d = 0; d < 3; d++)
+      for (int s = 0; s < 2; s++) {
+        B->nb[s][d] = -1;
+        if (B->cbcpre[d][s] != 'F') continue;
+        int fr = (int)lround(B->bcpre[d][s]) - 1;
+        ob *F = &c->b[fr];
+        int ok = F->cbcpre[d][1 - s] == 'F' && (int)lround(F->bcpre[d][1 - s]) - 1 == ib;
+        for (int e = 0; e < 3; e++) {
+          if (e == d) continue;
+          ok = ok && F->lo_g[e] == B->lo_g[e] && F->n[e] == B->n[e] && F->lmin[e] == B->lmin[e] &&
+               F->lmax[e] == B->lmax[e] && F->gr[e] == B->gr[e] && F->gt[e] == B->gt[e];
+        }
+        int blo = B->lo_g[d], bhi = B->lo_g[d] + B->n[d] - 1;
+        int flo = F->lo_g[d], fhi = F->lo_g[d] + F->n[d] - 1;
+        if (s == 0) ok = ok && (blo == fhi + 1 || c->periods[d] == fhi - blo + 1);
+        else ok = ok && (bhi == flo - 1 || c->periods[d] == bhi - flo + 1);
+        if (!ok) { seterr(errbuf, errlen, "expected connectivity between blocks is not possible", ib + 1); oc_destroy(c); return NULL; }
+        B->nb[s][d] = fr;
+      }
+    for (int d = 0; d < 3; d++)
+      for (int s = 0; s < 2; s++) B->is_bound[s][d] = B->nb[s][d] < 0;
+  }
+  /* vol_all param.f90:239-240 */
+  c->vol_all = 0.0;
+  for (int ib = 0; ib < nblocks; ib++) {
+    ob *B = &c->b[ib];
+    c->vol_all += (B->lmax[0] - B->lmin[0]) * (B->lmax[1] - B->lmin[1]) * (B->lmax[2] - B->lmin[2]);
+  }
+  /* grid initgrid.f90:9-83 then bound_grid 84-127 */
+  for (int ib = 0; ib < nblocks; ib++) {
+    ob *B = &c->b[ib];
+    for (int d = 0; d < 3; d++) {
+      int n = B->n[d];
+      B->dc[d] = xcalloc(n + 2, sizeof(double)); B->df[d] = xcalloc(n + 2, sizeof(double));
+      B->rc[d] = xcalloc(n + 2, sizeof(double)); B->rf[d] = xcalloc(n + 2, sizeof(double));
+      oc_initgrid(n, B->gt[d], B->gr[d], B->lmin[d], B->lmax[d], B->dc[d], B->df[d], B->rc[d], B->rf[d]);
+    }
+  }
+  {
+    /* bound_grid: the ghost face spacing is the neighbour's first/last
+     * interior spacing, all exchanged from the pre-exchange values          */
+    double lov[MAXB][3], hiv[MAXB][3];
+    for (int ib = 0; ib < nblocks; ib++)
+      for (int d = 0; d < 3; d++) { lov[ib][d] = c->b[ib].df[d][1]; hiv[ib][d] = c->b[ib].df[d][c->b[ib].n[d]]; }
+    for (int ib = 0; ib < nblocks; ib++) {
+      ob *B = &c->b[ib];
+      for (int d = 0; d < 3; d++) {
+        int n = B->n[d];
+        if (B->nb[0][d] >= 0) B->df[d][0] = hiv[B->nb[0][d]][d];
+        if (B->nb[1][d] >= 0) B->df[d][n + 1] = lov[B->nb[1][d]][d];
+        B->dc[d][0] = (B->df[d][1] + B->df[d][0]) / 2.0;
+        B->dc[d][n] = (B->df[d][n + 1] + B->df[d][n]) / 2.0;
+        B->dc[d][n + 1] = B->dc[d][n];
+      }
+    }
+  }
+  /* fields */
+  for (int ib = 0; ib < nblocks; ib++) {
+    ob *B = &c->b[ib];
+    B->sx = B->n[0] + 2; B->sy = B->n[1] + 2;
+    B->ntot = B->sx * B->sy * (B->n[2] + 2);
+    for (int f = 0; f < 6; f++) B->f[f] = xcalloc(B->ntot, sizeof(double));
+    long nin = (long)B->n[0] * B->n[1] * B->n[2];
+    for (int f = 0; f < 3; f++) B->dto[f] = xcalloc(nin, sizeof(double));
+  }
+  /* inflow planes main.f90:600-636 with init_inflow initflow.f90:369-429 */
+  for (int ib = 0; ib < nblocks; ib++) {
+    ob *B = &c->b[ib];
+    for (int d = 0; d < 3; d++)
+      for (int s = 0; s < 2; s++) {
+        B->is_inflow[d][s] = B->is_bound[s][d] && B->inflow_type[d][s] > 0 && B->cbcvel[d][d][s] == 'D';
+        if (!B->is_inflow[d][s]) continue;
+        int a1, a2; tang(d, &a1, &a2);
+        int m1 = B->n[a1] + 2, m2 = B->n[a2] + 2;
+        for (int cp = 0; cp < 3; cp++) B->vin[d][cp][s] = xcalloc((long)m1 * m2, sizeof(double));
+        double uref = B->bcvel[d][d][s];
+        int q1 = c->periods[a1] != 0 ? 0 : 1, q2 = c->periods[a2] != 0 ? 0 : 1;
+        for (int t2 = 0; t2 < m2; t2++)
+          for (int t1 = 0; t1 < m1; t1++) {
+            double val = 0.0;
+            if (B->inflow_type[d][s] == 1) {
+              double f1 = q1 ? poiseuille(B->rc[a1][t1], B->lmin[a1], B->lmax[a1]) : 1.0;
+              double f2 = q2 ? poiseuille(B->rc[a2][t2], B->lmin[a2], B->lmax[a2]) : 1.0;
+              val = uref * f1 * f2;
+            }
+            B->vin[d][d][s][t1 + (long)m1 * t2] = val;
+          }
+      }
+  }
+  /* BC rhs planes for the pressure: init_bc_rhs solver.f90:44-157           */
+  for (int ib = 0; ib < nblocks; ib++) {
+    ob *B = &c->b[ib];
+    double dl[2][3];
+    for (int d = 0; d < 3; d++) {
+      /* dl = dxc_g(lo_g-1), dxc_g(hi_g) of the undistributed grid (main.f90:717-719) */
+      /* dxc_g(lo_g-1) = drf_g(lo_g), dxc_g(hi_g) = drf_g(hi_g): the interior
+       * face spacings, which bound_grid leaves untouched                   */
+      dl[0][d] = B->df[d][1];
+      dl[1][d] = B->df[d][B->n[d]];
+    }
+    for (int d = 0; d < 3; d++) {
+      if (c->periods[d] != 0) continue;
+      int a1, a2; tang(d, &a1, &a2);
+      for (int s = 0; s < 2; s++) {
+        if (!B->is_bound[s][d]) continue;
+        double factor = 0.0;
+        char ct = B->cbcpre[d][s];
+        if (ct == 'N') { factor = dl[s][d] * B->bcpre[d][s]; if (s == 1) factor = -factor; }
+        else if (ct == 'D') factor = -2.0 * B->bcpre[d][s];
+        int q = s == 0 ? 1 : B->n[d];
+        long m1 = B->n[a1], m2 = B->n[a2];
+        B->rhsp[d][s] = xcalloc(m1 * m2, sizeof(double));
+        for (int t2 = 1; t2 <= m2; t2++)
+          for (int t1 = 1; t1 <= m1; t1++) {
+            int ijk[3]; ijk[d] = q; ijk[a1] = t1; ijk[a2] = t2;
+            double weight;
+            switch (c->fft_axis) {
+            case 1: weight = B->df[1][ijk[1]] * B->df[2][ijk[2]]; break;
+            case 2: weight = B->df[0][ijk[0]] * B->df[2][ijk[2]]; break;
+            case 3: weight = B->df[0][ijk[0]] * B->df[1][ijk[1]]; break;
+            default: weight = B->df[0][ijk[0]] * B->df[1][ijk[1]] * B->df[2][ijk[2]]; break;
+            }
+            double rhs = factor * weight / (B->dc[d][q - (1 - s)] * B->df[d][q]);
+            B->rhsp[d][s][(t1 - 1) + m1 * (t2 - 1)] = rhs;
+          }
+      }
+    }
+  }
+  assemble_operator(c);
+  return c;
+}
+
+void oc_destroy(oc_case *c) {
+  if (!c) return;
+  for (int ib = 0; ib < c->nb; ib++) {
+    ob *B = &c->b[ib];
+    for (int d = 0; d < 3; d++) { free(B->dc[d]); free(B->df[d]); free(B->rc[d]); free(B->rf[d]); }
+    for (int f = 0; f < 6; f++) free(B->f[f]);
+    for (int f = 0; f < 3; f++) free(B->dto[f]);
+    for (int d = 0; d < 3; d++)
+      for (int s = 0; s < 2; s++) {
+        free(B->rhsp[d][s]);
+        for (int cp = 0; cp < 3; cp++) free(B->vin[d][cp][s]);
+      }
+  }
+  free(c->lambda); free(c->diag0); free(c->wgt); free(c->coef); free(c->nbr);
+  free(c->perm); free(c->iperm);
+  if (c->bandL) { for (int m = 0; m < c->nmodes; m++) free(c->bandL[m]); free(c->bandL); }
+  free(c->t_lambda[0]); free(c->t_lambda[1]); free(c->work);
+  free(c);
+}
+
+int oc_nblocks(const oc_case *c) { return c->nb; }
+double oc_vol_all(const oc_case *c) { return c->vol_all; }
+void oc_block_info(const oc_case *c, int b, int *n3, int *lo3, int *nb6, int *per3) {
+  const ob *B = &c->b[b];
+  for (int d = 0; d < 3; d++) {
+    n3[d] = B->n[d]; lo3[d] = B->lo_g[d]; per3[d] = c->periods[d];
+    nb6[2 * d] = B->nb[0][d]; nb6[2 * d + 1] = B->nb[1][d];
+  }
+}
+int oc_get_metric(const oc_case *c, int b, int which, double *out) {
+  if (b < 0 || b >= c->nb || which < 0 || which > 11) return -1;
+  const ob *B = &c->b[b];
+  int d = (which % 6) / 2, cf = which % 2; /* 0 = c, 1 = f */
+  const double *src = which < 6 ? (cf == 0 ? B->dc[d] : B->df[d]) : (cf == 0 ? B->rc[d] : B->rf[d]);
+  memcpy(out, src, sizeof(double) * (B->n[d] + 2));
+  return 0;
+}
+int oc_field_size(const oc_case *c, int b, int fid) {
+  const ob *B = &c->b[b];
+  if (fid >= 0 && fid < 6) return (int)B->ntot;
+  return B->n[0] * B->n[1] * B->n[2];
+}
+int oc_get_field(const oc_case *c, int b, int fid, double *out) {
+  if (b < 0 || b >= c->nb || fid < 0 || fid > 8) return -1;
+  const ob *B = &c->b[b];
+  if (fid < 6) memcpy(out, B->f[fid], sizeof(double) * B->ntot);
+  else memcpy(out, B->dto[fid - 6], sizeof(double) * B->n[0] * B->n[1] * B->n[2]);
+  return 0;
+}
+int oc_set_field(oc_case *c, int b, int fid, const double *in) {
+  if (b < 0 || b >= c->nb || fid < 0 || fid > 8) return -1;
+  ob *B = &c->b[b];
+  if (fid < 6) memcpy(B->f[fid], in, sizeof(double) * B->ntot);
+  else memcpy(B->dto[fid - 6], in, sizeof(double) * B->n[0] * B->n[1] * B->n[2]);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Poisson operator: init_matrix_2d solver.f90:687-842 + init_n_2d_matrices  */
+/* 981-1020 (lambda on the diagonal), init_matrix_3d solver.f90:158-372      */
+/* ------------------------------------------------------------------------ */
+static void assemble_operator(oc_case *c) {
+  int fa = c->fft_axis;
+  if (fa >= 1 && fa <= 3) {
+    int d = fa - 1;
+    c->nd = 2;
+    int k = 0;
+    for (int e = 0; e < 3; e++) if (e != d) c->ax[k++] = e;
+    ob *B0 = &c->b[0];
+    c->nmodes = B0->n[d];
+    c->lambda = xcalloc(c->nmodes, sizeof(double));
+    char bc2[2] = {B0->cbcpre[d][0], B0->cbcpre[d][1]};
+    double norm[2];
+    oc_find_fft(bc2, 1, &c->kind_fwd, &c->kind_bwd, norm);
+    c->normfft = 1.0 / (norm[0] * (c->nmodes + norm[1]));
+    oc_eigenvalues(c->nmodes, bc2, 1, c->lambda);
+    /* dl = dxc_g(lo_g-1) of the FFT direction (main.f90:717-728)          */
+    double dl = B0->df[d][1];
+    for (int m = 0; m < c->nmodes; m++) c->lambda[m] /= dl * dl;
+  } else {
+    c->nd = 3; c->ax[0] = 0; c->ax[1] = 1; c->ax[2] = 2;
+    c->nmodes = 1;
+    c->lambda = xcalloc(1, sizeof(double));
+    c->normfft = 1.0;
+  }
+  int nd = c->nd;
+  long nunk = 0;
+  for (int ib = 0; ib < c->nb; ib++) {
+    ob *B = &c->b[ib];
+    B->off = nunk;
+    long cnt = 1;
+    for (int e = 0; e < nd; e++) cnt *= B->n[c->ax[e]];
+    nunk += cnt;
+  }
+  c->nunk = nunk;
+  c->diag0 = xcalloc(nunk, sizeof(double));
+  c->wgt = xcalloc(nunk, sizeof(double));
+  c->coef = xcalloc(nunk * 2 * nd, sizeof(double));
+  c->nbr = xcalloc(nunk * 2 * nd, sizeof(long));
+  c->has_dirichlet = 0;
+  for (int ib = 0; ib < c->nb; ib++) {
+    ob *B = &c->b[ib];
+    int nn[3] = {1, 1, 1};
+    for (int e = 0; e < nd; e++) nn[e] = B->n[c->ax[e]];
+    for (int i3 = 1; i3 <= nn[2]; i3++)
+      for (int i2 = 1; i2 <= nn[1]; i2++)
+        for (int i1 = 1; i1 <= nn[0]; i1++) {
+          int idx[3] = {i1, i2, i3};
+          long q = B->off + (i1 - 1) + (long)nn[0] * ((i2 - 1) + (long)nn[1] * (i3 - 1));
+          /* weight and coefficients */
+          double weight = 1.0;
+          for (int e = 0; e < nd; e++) weight *= B->df[c->ax[e]][idx[e]];
+          double cc = 0.0;
+          double cm[3], cp[3];
+          for (int e = 0; e < nd; e++) {
+            double other = 1.0;
+            for (int o = 0; o < nd; o++) if (o != e) other *= B->df[c->ax[o]][idx[o]];
+            cm[e] = 1.0 / (B->dc[c->ax[e]][idx[e] - 1]) * other;
+            cp[e] = 1.0 / (B->dc[c->ax[e]][idx[e]]) * other;
+          }
+          double ssum = 0.0;
+          for (int e = 0; e < nd; e++) ssum += cm[e] + cp[e];
+          /* the reference sums (c1m+c1p+c2m+c2p) / (cxm+cxp+cym+cyp+czm+czp) */
+          cc = -ssum;
+          for (int e = 0; e < nd; e++) {
+            int a = c->ax[e];
+            if (c->periods[a] != 0) continue;
+            if (B->is_bound[0][a] && idx[e] == 1) {
+              double sgn = B->cbcpre[a][0] == 'N' ? 1.0 : (B->cbcpre[a][0] == 'D' ? -1.0 : 0.0);
+              if (B->cbcpre[a][0] == 'D') c->has_dirichlet = 1;
+              cc = cc + sgn * cm[e]; cm[e] = 0.0;
+            }
+            if (B->is_bound[1][a] && idx[e] == nn[e]) {
+              double sgn = B->cbcpre[a][1] == 'N' ? 1.0 : (B->cbcpre[a][1] == 'D' ? -1.0 : 0.0);
+              if (B->cbcpre[a][1] == 'D') c->has_dirichlet = 1;
+              cc = cc + sgn * cp[e]; cp[e] = 0.0;
+            }
+          }
+          c->diag0[q] = cc;
+          c->wgt[q] = weight;
+          for (int e = 0; e < nd; e++) {
+            int a = c->ax[e];
+            for (int s = 0; s < 2; s++) {
+              long slot = q * 2 * nd + 2 * e + s;
+              double cf = s == 0 ? cm[e] : cp[e];
+              c->coef[slot] = cf;
+              c->nbr[slot] = -1;
+              if (cf == 0.0) continue;
+              int j2[3] = {idx[0], idx[1], idx[2]};
+              const ob *T = B;
+              if (s == 0 && idx[e] == 1) { T = &c->b[B->nb[0][a]]; j2[e] = T->n[a]; }
+              else if (s == 1 && idx[e] == nn[e]) { T = &c->b[B->nb[1][a]]; j2[e] = 1; }
+              else j2[e] += s == 0 ? -1 : 1;
+              int tn[3] = {1, 1, 1};
+              for (int o = 0; o < nd; o++) tn[o] = T->n[c->ax[o]];
+              c->nbr[slot] = T->off + (j2[0] - 1) + (long)tn[0] * ((j2[1] - 1) + (long)tn[1] * (j2[2] - 1));
+            }
+          }
+        }
+  }
+  c->work = xcalloc(nunk * 6, sizeof(double));
+}
+
+int oc_num_unknowns(const oc_case *c) { return (int)c->nunk; }
+int oc_get_lambda(const oc_case *c, double *lambda, double *normfft) {
+  if (lambda) memcpy(lambda, c->lambda, sizeof(double) * c->nmodes);
+  if (normfft) *normfft = c->normfft;
+  return c->nmodes;
+}
+
+static void apply_op(const oc_case *c, int mode, const double *x, double *y) {
+  int nd = c->nd;
+  double lam = c->lambda[mode];
+#pragma omp parallel for schedule(static)
+  for (long q = 0; q < c->nunk; q++) {
+    double s = (c->diag0[q] + lam * c->wgt[q]) * x[q];
+    for (int t = 0; t < 2 * nd; t++) {
+      long nb = c->nbr[q * 2 * nd + t];
+      if (nb >= 0) s += c->coef[q * 2 * nd + t] * x[nb];
+    }
+    y[q] = s;
+  }
+}
+
+int oc_apply_operator(const oc_case *c, int mode, const double *x, double *y) {
+  if (mode < 0 || mode >= c->nmodes) return -1;
+  apply_op(c, mode, x, y);
+  return 0;
+}
+
+static int mode_singular(const oc_case *c, int mode) {
+  return !c->has_dirichlet && c->lambda[mode] == 0.0;
+}
+
+/* Jacobi-preconditioned CG on -A (A is negative semi-definite)             */
+static int solve_cg(oc_case *c, int mode, const double *b_in, double *x) {
+  long n = c->nunk;
+  double *r = c->work, *z = r + n, *p = z + n, *q = p + n, *b = q + n;
+  double lam = c->lambda[mode];
+  memcpy(b, b_in, sizeof(double) * n);
+  if (mode_singular(c, mode)) {
+    double m = 0; for (long i = 0; i < n; i++) m += b[i]; m /= n;
+    for (long i = 0; i < n; i++) b[i] -= m;
+  }
+  double bn = 0; for (long i = 0; i < n; i++) bn += b[i] * b[i];
+  bn = sqrt(bn);
+  if (bn == 0.0) { memset(x, 0, sizeof(double) * n); return 0; }
+  apply_op(c, mode, x, q);
+  for (long i = 0; i < n; i++) r[i] = -(b[i] - q[i]); /* residual of -A x = -b */
+  double rz = 0;
+  for (long i = 0; i < n; i++) { z[i] = r[i] / (-(c->diag0[i] + lam * c->wgt[i])); p[i] = z[i]; rz += r[i] * z[i]; }
+  int it;
+  for (it = 0; it < c->maxiter; it++) {
+    double rn = 0; for (long i = 0; i < n; i++) rn += r[i] * r[i];
+    if (sqrt(rn) <= c->tol * bn) break;
+    apply_op(c, mode, p, q);
+    double pq = 0; for (long i = 0; i < n; i++) { q[i] = -q[i]; pq += p[i] * q[i]; }
+    double al = rz / pq;
+    for (long i = 0; i < n; i++) { x[i] += al * p[i]; r[i] -= al * q[i]; }
+    double rz2 = 0;
+    for (long i = 0; i < n; i++) { z[i] = r[i] / (-(c->diag0[i] + lam * c->wgt[i])); rz2 += r[i] * z[i]; }
+    double be = rz2 / rz; rz = rz2;
+    for (long i = 0; i < n; i++) p[i] = z[i] + be * p[i];
+  }
+  return it;
+}
+
+/* banded Cholesky of -A in an ordering of the union by global coordinates  */
+static int band_setup(oc_case *c) {
+  if (c->perm) return c->band_ok;
+  long n = c->nunk; int nd = c->nd;
+  c->perm = xcalloc(n, sizeof(long)); c->iperm = xcalloc(n, sizeof(long));
+  /* global coords of each unknown; order: highest reduced axis that is not
+   * periodic is the major key (wrap couplings stay inside the band)         */
+  long *key = xcalloc(n, sizeof(long));
+  int order[3] = {0, 1, 2};
+  if (nd == 2) { order[0] = 0; order[1] = 1; if (c->periods[c->ax[1]] != 0 && c->periods[c->ax[0]] == 0) { order[0] = 1; order[1] = 0; } }
+  else {
+    /* major axis must be non-periodic if possible */
+    int maj = 2;
+    for (int e = 2; e >= 0; e--) if (c->periods[c->ax[e]] == 0) { maj = e; break; }
+    int k = 0; for (int e = 0; e < 3; e++) if (e != maj) order[k++] = e;
+    order[2] = maj;
+  }
+  int gmin[3] = {1 << 30, 1 << 30, 1 << 30}, gmax[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+  for (int ib = 0; ib < c->nb; ib++)
+    for (int e = 0; e < nd; e++) {
+      int a = c->ax[e];
+      if (c->b[ib].lo_g[a] < gmin[e]) gmin[e] = c->b[ib].lo_g[a];
+      int hi = c->b[ib].lo_g[a] + c->b[ib].n[a] - 1;
+      if (hi > gmax[e]) gmax[e] = hi;
+    }
+  long ext[3] = {1, 1, 1};
+  for (int e = 0; e < nd; e++) ext[e] = gmax[e] - gmin[e] + 1;
+  for (int ib = 0; ib < c->nb; ib++) {
+    ob *B = &c->b[ib];
+    int nn[3] = {1, 1, 1};
+    for (int e = 0; e < nd; e++) nn[e] = B->n[c->ax[e]];
+    for (int i3 = 0; i3 < nn[2]; i3++)
+      for (int i2 = 0; i2 < nn[1]; i2++)
+        for (int i1 = 0; i1 < nn[0]; i1++) {
+          long q = B->off + i1 + (long)nn[0] * (i2 + (long)nn[1] * i3);
+          int loc[3] = {i1, i2, i3};
+          long g[3] = {0, 0, 0};
+          for (int e = 0; e < nd; e++) g[e] = B->lo_g[c->ax[e]] + loc[e] - gmin[e];
+          long kk = 0;
+          for (int o = nd - 1; o >= 0; o--) kk = kk * ext[order[o]] + g[order[o]];
+          key[q] = kk;
+        }
+  }
+  /* sort unknowns by key (counting via index into a dense key space)       */
+  long kmax = 1; for (int e = 0; e < nd; e++) kmax *= ext[e];
+  long *slot = xcalloc(kmax, sizeof(long));
+  for (long k = 0; k < kmax; k++) slot[k] = -1;
+  for (long q = 0; q < n; q++) slot[key[q]] = q;
+  long pos = 0;
+  for (long k = 0; k < kmax; k++) if (slot[k] >= 0) { c->perm[pos] = slot[k]; c->iperm[slot[k]] = pos; pos++; }
+  free(slot); free(key);
+  long bw = 0;
+  for (long q = 0; q < n; q++)
+    for (int t = 0; t < 2 * nd; t++) {
+      long nb = c->nbr[q * 2 * nd + t];
+      if (nb < 0) continue;
+      long dd = labs(c->iperm[q] - c->iperm[nb]);
+      if (dd > bw) bw = dd;
+    }
+  c->bw = bw;
+  c->band_ok = (double)n * (double)(bw + 1) * (double)c->nmodes < 4.0e8;
+  if (c->band_ok) c->bandL = xcalloc(c->nmodes, sizeof(double *));
+  return c->band_ok;
+}
+
+static int band_factor(oc_case *c, int mode) {
+  long n = c->nunk, bw = c->bw; int nd = c->nd;
+  double *L = xcalloc(n * (bw + 1), sizeof(double)); /* L[i*(bw+1) + (i-j)] */
+  double lam = c->lambda[mode];
+  int sing = mode_singular(c, mode);
+  for (long q = 0; q < n; q++) {
+    long i = c->iperm[q];
+    L[i * (bw + 1)] += -(c->diag0[q] + lam * c->wgt[q]);
+    for (int t = 0; t < 2 * nd; t++) {
+      long nb = c->nbr[q * 2 * nd + t];
+      if (nb < 0) continue;
+      long j = c->iperm[nb];
+      if (j < i) L[i * (bw + 1) + (i - j)] += -c->coef[q * 2 * nd + t];
+    }
+  }
+  if (sing) { /* pin the first unknown of the ordering */
+    for (long t = 1; t <= bw && t < n; t++) L[t * (bw + 1) + t] = 0.0;
+    L[0] = 1.0;
+  }
+  for (long i = 0; i < n; i++) {
+    long j0 = i - bw < 0 ? 0 : i - bw;
+    for (long j = j0; j <= i; j++) {
+      double s = L[i * (bw + 1) + (i - j)];
+      long k0 = j0 > j - bw ? j0 : j - bw;
+      if (k0 < 0) k0 = 0;
+      for (long k = k0; k < j; k++) s -= L[i * (bw + 1) + (i - k)] * L[j * (bw + 1) + (j - k)];
+      if (i == j) {
+        if (s <= 0.0) { free(L); return -1; }
+        L[i * (bw + 1)] = sqrt(s);
+      } else L[i * (bw + 1) + (i - j)] = s / L[j * (bw + 1)];
+    }
+  }
+  c->bandL[mode] = L;
+  return 0;
+}
+
+static int solve_band(oc_case *c, int mode, const double *b, double *x) {
+  if (!c->bandL[mode] && band_factor(c, mode)) return -1;
+  long n = c->nunk, bw = c->bw;
+  const double *L = c->bandL[mode];
+  double *y = c->work;
+  for (long i = 0; i < n; i++) y[i] = -b[c->perm[i]];
+  if (mode_singular(c, mode)) y[0] = 0.0;
+  for (long i = 0; i < n; i++) {
+    double s = y[i];
+    long k0 = i - bw < 0 ? 0 : i - bw;
+    for (long k = k0; k < i; k++) s -= L[i * (bw + 1) + (i - k)] * y[k];
+    y[i] = s / L[i * (bw + 1)];
+  }
+  for (long i = n - 1; i >= 0; i--) {
+    double s = y[i];
+    long k1 = i + bw >= n ? n - 1 : i + bw;
+    for (long k = i + 1; k <= k1; k++) s -= L[k * (bw + 1) + (k - i)] * y[k];
+    y[i] = s / L[i * (bw + 1)];
+  }
+  for (long i = 0; i < n; i++) x[c->perm[i]] = y[i];
+  return 0;
+}
+
+/* tensor solver: r2r along one uniform in-plane axis (two for 3D) and a
+ * tridiagonal solve along the remaining axis; single-block cases only in the
+ * oracle.                                                                   */
+static int tensor_setup(oc_case *c) {
+  if (c->tensor_ok) return c->tensor_ok > 0;
+  c->tensor_ok = -1;
+  if (c->nb != 1) return 0;
+  ob *B = &c->b[0];
+  int nd = c->nd, nt = nd - 1;
+  int cand[3], nc = 0;
+  for (int e = 0; e < nd; e++) {
+    int a = c->ax[e];
+    int n = B->n[a];
+    double h = B->df[a][1], dev = 0;
+    for (int q = 0; q <= n + 1; q++) { dev = fmax(dev, fabs(B->df[a][q] - h)); dev = fmax(dev, fabs(B->dc[a][q] - h)); }
+    char bc2[2] = {B->cbcpre[a][0], B->cbcpre[a][1]};
+    int kf, kb; double nrm[2];
+    if (dev <= 1e-12 * h && oc_find_fft(bc2, 1, &kf, &kb, nrm) == 0) cand[nc++] = e;
+  }
+  if (nc < nt) return 0;
+  /* the line axis: a non-candidate if any, else the last candidate; it must
+   * not be periodic                                                         */
+  int ls = -1;
+  for (int e = nd - 1; e >= 0; e--) {
+    int isc = 0; for (int q = 0; q < nc; q++) if (cand[q] == e) isc = 1;
+    if (!isc) { ls = e; break; }
+  }
+  if (ls < 0) {
+    for (int q = nc - 1; q >= 0; q--) if (c->periods[c->ax[cand[q]]] == 0) { ls = cand[q]; break; }
+  }
+  if (ls < 0 || c->periods[c->ax[ls]] != 0) return 0;
+  int k = 0;
+  for (int e = 0; e < nd; e++) if (e != ls) c->tt[k++] = e;
+  for (int t = 0; t < nt; t++) {
+    int e = c->tt[t]; int isc = 0;
+    for (int q = 0; q < nc; q++) if (cand[q] == e) isc = 1;
+    if (!isc) return 0;
+  }
+  c->nt = nt; c->ts = ls;
+  for (int t = 0; t < nt; t++) {
+    int a = c->ax[c->tt[t]], n = B->n[a];
+    char bc2[2] = {B->cbcpre[a][0], B->cbcpre[a][1]};
+    double nrm[2];
+    oc_find_fft(bc2, 1, &c->t_kind_fwd[t], &c->t_kind_bwd[t], nrm);
+    c->t_norm[t] = 1.0 / (nrm[0] * (n + nrm[1]));
+    c->t_lambda[t] = xcalloc(n, sizeof(double));
+    oc_eigenvalues(n, bc2, 1, c->t_lambda[t]);
+    c->th[t] = B->df[a][1];
+    for (int q = 0; q < n; q++) c->t_lambda[t][q] /= c->th[t] * c->th[t];
+  }
+  c->tensor_ok = 1;
+  return 1;
+}
+
+static int solve_tensor(oc_case *c, int mode, const double *b, double *x) {
+  ob *B = &c->b[0];
+  int nd = c->nd;
+  int nn[3] = {1, 1, 1};
+  for (int e = 0; e < nd; e++) nn[e] = B->n[c->ax[e]];
+  long st[3] = {1, nn[0], (long)nn[0] * nn[1]};
+  long n = c->nunk;
+  double *y = xcalloc(n, sizeof(double));
+  memcpy(y, b, sizeof(double) * n);
+  int maxn = 0; for (int e = 0; e < 3; e++) if (nn[e] > maxn) maxn = nn[e];
+  double *lin = xcalloc(maxn, sizeof(double)), *lout = xcalloc(maxn, sizeof(double));
+  /* forward transforms along the uniform axes */
+  for (int t = 0; t < c->nt; t++) {
+    int e = c->tt[t], o1 = -1, o2 = -1;
+    for (int q = 0; q < 3; q++) if (q != e) { if (o1 < 0) o1 = q; else o2 = q; }
+    for (int j2 = 0; j2 < nn[o2]; j2++)
+      for (int j1 = 0; j1 < nn[o1]; j1++) {
+        long base = j1 * st[o1] + j2 * st[o2];
+        for (int q = 0; q < nn[e]; q++) lin[q] = y[base + q * st[e]];
+        oc_r2r(c->t_kind_fwd[t], nn[e], lin, lout);
+        for (int q = 0; q < nn[e]; q++) y[base + q * st[e]] = lout[q];
+      }
+  }
+  /* tridiagonal solves along the line axis; the line-axis part of the
+   * operator is rebuilt from the metrics exactly as the assembly does, the
+   * transformed axes enter through their eigenvalues times the cell weight */
+  int e = c->ts, a = c->ax[e], m = nn[e], o1 = -1, o2 = -1;
+  for (int q = 0; q < 3; q++) if (q != e) { if (o1 < 0) o1 = q; else o2 = q; }
+  double sg0 = B->cbcpre[a][0] == 'N' ? 1.0 : (B->cbcpre[a][0] == 'D' ? -1.0 : 0.0);
+  double sg1 = B->cbcpre[a][1] == 'N' ? 1.0 : (B->cbcpre[a][1] == 'D' ? -1.0 : 0.0);
+  double *cp = xcalloc(m, sizeof(double)), *dp = xcalloc(m, sizeof(double));
+  for (int j2 = 0; j2 < nn[o2]; j2++)
+    for (int j1 = 0; j1 < nn[o1]; j1++) {
+      long base = j1 * st[o1] + j2 * st[o2];
+      int jj[3] = {0, 0, 0}; jj[o1] = j1; jj[o2] = j2;
+      double lsum = c->lambda[mode];
+      for (int t = 0; t < c->nt; t++) lsum += c->t_lambda[t][jj[c->tt[t]]];
+      int sing = !c->has_dirichlet && lsum == 0.0;
+      int mm = sing ? m - 1 : m; /* singular line: pin the last unknown */
+      for (int q = 0; q < mm; q++) {
+        long cell = base + q * st[e];
+        int idx[3]; for (int o = 0; o < 3; o++) idx[o] = (o == e) ? q : jj[o];
+        double other = 1.0;
+        for (int o = 0; o < nd; o++) if (o != e) other *= B->df[c->ax[o]][idx[o] + 1];
+        double rm = 1.0 / B->dc[a][q] * other, rp = 1.0 / B->dc[a][q + 1] * other;
+        double lo = rm, up = rp, dg = -(rm + rp);
+        if (c->periods[a] == 0) {
+          if (B->is_bound[0][a] && q == 0) { dg += sg0 * rm; lo = 0.0; }
+          if (B->is_bound[1][a] && q == m - 1) { dg += sg1 * rp; up = 0.0; }
+        }
+        dg += lsum * c->wgt[cell];
+        double rhs = y[cell];
+        if (q == 0) { cp[0] = up / dg; dp[0] = rhs / dg; }
+        else { double den = dg - lo * cp[q - 1]; cp[q] = up / den; dp[q] = (rhs - lo * dp[q - 1]) / den; }
+      }
+      if (mm < m) y[base + (m - 1) * st[e]] = 0.0;
+      if (mm == 0) continue;
+      double xv = dp[mm - 1];
+      y[base + (mm - 1) * st[e]] = xv;
+      for (int q = mm - 2; q >= 0; q--) { xv = dp[q] - cp[q] * xv; y[base + q * st[e]] = xv; }
+    }
+  free(cp); free(dp);
+  /* inverse transforms */
+  for (int t = c->nt - 1; t >= 0; t--) {
+    int et = c->tt[t], p1 = -1, p2 = -1;
+    for (int q = 0; q < 3; q++) if (q != et) { if (p1 < 0) p1 = q; else p2 = q; }
+    for (int j2 = 0; j2 < nn[p2]; j2++)
+      for (int j1 = 0; j1 < nn[p1]; j1++) {
+        long base = j1 * st[p1] + j2 * st[p2];
+        for (int q = 0; q < nn[et]; q++) lin[q] = y[base + q * st[et]];
+        oc_r2r(c->t_kind_bwd[t], nn[et], lin, lout);
+        for (int q = 0; q < nn[et]; q++) y[base + q * st[et]] = lout[q] * c->t_norm[t];
+      }
+  }
+  memcpy(x, y, sizeof(double) * n);
+  free(y); free(lin); free(lout);
+  return 0;
+}
+
+int oc_set_solver(oc_case *c, int method, double tol, int maxiter) {
+  c->method = method;
+  if (tol > 0) c->tol = tol;
+  if (maxiter > 0) c->maxiter = maxiter;
+  return 0;
+}
+
+static int pick_method(oc_case *c) {
+  int m = c->method;
+  if (m == OC_SOLVE_AUTO) {
+    if (tensor_setup(c)) return OC_SOLVE_TENSOR;
+    if (band_setup(c)) return OC_SOLVE_BAND;
+    return OC_SOLVE_CG;
+  }
+  if (m == OC_SOLVE_TENSOR && !tensor_setup(c)) return -1;
+  if (m == OC_SOLVE_BAND && !band_setup(c)) return -1;
+  return m;
+}
+
+int oc_solve_operator(oc_case *c, int mode, const double *b, double *x) {
+  if (mode < 0 || mode >= c->nmodes) return -1;
+  int m = pick_method(c);
+  switch (m) {
+  case OC_SOLVE_TENSOR: return solve_tensor(c, mode, b, x);
+  case OC_SOLVE_BAND: return solve_band(c, mode, b, x);
+  case OC_SOLVE_CG: return solve_cg(c, mode, b, x) >= c->maxiter ? -2 : 0;
+  default: return -1;
+  }
+}
+
+/* gather/scatter one mode plane (2D) or the whole interior (3D)             */
+static void gather(const oc_case *c, int fid, int mode, double *vec) {
+  int d = c->fft_axis - 1;
+  for (int ib = 0; ib < c->nb; ib++) {
+    const ob *B = &c->b[ib];
+    const double *F = B->f[fid];
+    if (c->nd == 3) {
+      long q = B->off;
+      for (int k = 1; k <= B->n[2]; k++)
+        for (int j = 1; j <= B->n[1]; j++)
+          for (int i = 1; i <= B->n[0]; i++) vec[q++] = F[IDX(B, i, j, k)];
+    } else {
+      int a0 = c->ax[0], a1 = c->ax[1];
+      long q = B->off;
+      for (int i2 = 1; i2 <= B->n[a1]; i2++)
+        for (int i1 = 1; i1 <= B->n[a0]; i1++) {
+          int ijk[3]; ijk[d] = mode + 1; ijk[a0] = i1; ijk[a1] = i2;
+          vec[q++] = F[IDX(B, ijk[0], ijk[1], ijk[2])];
+        }
+    }
+  }
+}
+static void scatter(oc_case *c, int fid, int mode, const double *vec) {
+  int d = c->fft_axis - 1;
+  for (int ib = 0; ib < c->nb; ib++) {
+    ob *B = &c->b[ib];
+    double *F = B->f[fid];
+    if (c->nd == 3) {
+      long q = B->off;
+      for (int k = 1; k <= B->n[2]; k++)
+        for (int j = 1; j <= B->n[1]; j++)
+          for (int i = 1; i <= B->n[0]; i++) F[IDX(B, i, j, k)] = vec[q++];
+    } else {
+      int a0 = c->ax[0], a1 = c->ax[1];
+      long q = B->off;
+      for (int i2 = 1; i2 <= B->n[a1]; i2++)
+        for (int i1 = 1; i1 <= B->n[a0]; i1++) {
+          int ijk[3]; ijk[d] = mode + 1; ijk[a0] = i1; ijk[a1] = i2;
+          F[IDX(B, ijk[0], ijk[1], ijk[2])] = vec[q++];
+        }
+    }
+  }
+}
+
+/* in-place r2r along the FFT axis on the interior of pp, every block        */
+static void fft_axis_transform(oc_case *c, int kind, double scale) {
+  int d = c->fft_axis - 1;
+  for (int ib = 0; ib < c->nb; ib++) {
+    ob *B = &c->b[ib];
+    int a1, a2; tang(d, &a1, &a2);
+    int n = B->n[d];
+    double *F = B->f[4];
+#pragma omp parallel
+    {
+      double *lin = xcalloc(n, sizeof(double)), *lout = xcalloc(n, sizeof(double));
+#pragma omp for collapse(2) schedule(static)
+      for (int t2 = 1; t2 <= B->n[a2]; t2++)
+        for (int t1 = 1; t1 <= B->n[a1]; t1++) {
+          for (int q = 0; q < n; q++) lin[q] = F[plane_idx(B, d, q + 1, t1, t2)];
+          oc_r2r(kind, n, lin, lout);
+          for (int q = 0; q < n; q++) F[plane_idx(B, d, q + 1, t1, t2)] = lout[q] * scale;
+        }
+      free(lin); free(lout);
+    }
+  }
+}
+
+/* main.f90:1004-1022: transform, per-mode solve (po = warm start and is
+ * overwritten with the solution, solver.f90:909-920), inverse, scale        */
+int oc_solve_pressure(oc_case *c) {
+  long n = c->nunk;
+  double *b = xcalloc(n, sizeof(double)), *x = xcalloc(n, sizeof(double));
+  int rc = 0;
+  if (c->nd == 2) fft_axis_transform(c, c->kind_fwd, 1.0);
+  for (int m = 0; m < c->nmodes && rc == 0; m++) {
+    gather(c, 4, m, b);
+    gather(c, 5, m, x);
+    rc = oc_solve_operator(c, m, b, x);
+    scatter(c, 4, m, x);
+    scatter(c, 5, m, x);
+  }
+  if (c->nd == 2) fft_axis_transform(c, c->kind_bwd, c->normfft);
+  free(b); free(x);
+  return rc;
+}
+
+void oc_init_bounds(oc_case *c) {
+  oc_bounduvw(c, 0);
+  oc_boundp(c, 3);
+  oc_inflow(c, 0);
+}
+
+/* main.f90:893-1033, one time step (nscal = 0)                              */
+int oc_step(oc_case *c, double dt) {
+  for (int irk = 1; irk <= 3; irk++) {
+    double dtrk = (RKC[irk - 1][0] + RKC[irk - 1][1]) * dt;
+    oc_rk_mom(c, irk, dt);
+    oc_bounduvw(c, 0);
+    oc_inflow(c, 0);
+    oc_fillps(c, dtrk);
+    int rc = oc_solve_pressure(c);
+    if (rc) return rc;
+    oc_boundp(c, 4);
+    oc_correc(c, dtrk);
+    oc_bounduvw(c, 1);
+    oc_inflow(c, 1);
+    oc_updt_pressure(c);
+    oc_boundp(c, 3);
+  }
+  return 0;
+}
