@@ -1,0 +1,120 @@
+/*
+ * snac_amd.h -- C-ABI of the MI355X (gfx950) time-step hot path of SNaC.
+ *
+ * A Fortran host binds these with iso_c_binding (the bind(C) interface module
+ * is given in INTEGRATION.md); Python binds them with ctypes (snac_amd/).
+ * Plain pointers and sizes only.  Every function returns 0 on success and a
+ * negative code on failure; snac_last_error() then holds the message (the
+ * Fortran shim turns a non-zero status into `error stop`, which is how the
+ * reference reports failures).
+ *
+ * Threading: one host thread per context; calls on one context are not
+ * re-entrant.  All fields live in device memory owned by the context; host
+ * arrays are touched only by snac_upload_field / snac_download_field.
+ *
+ * Array layout at the boundary (identical to the reference's arrays,
+ * main.f90:226-236): column-major, i fastest, one ghost layer per side, i.e.
+ * a block with interior n = (nx,ny,nz) exchanges (nx+2)*(ny+2)*(nz+2) doubles
+ * for u,v,w,p,pp,po and nx*ny*nz doubles for dudtrko/dvdtrko/dwdtrko.
+ *
+ * Reference interfaces replaced (paths relative to the SNaC source tree):
+ * each declaration names the routine(s) it stands in for.
+ */
+#ifndef SNAC_AMD_H
+#define SNAC_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct snac_ctx snac_ctx;
+
+/* field ids for upload/download */
+enum {
+  SNAC_U = 0, SNAC_V = 1, SNAC_W = 2, SNAC_P = 3, SNAC_PP = 4, SNAC_PO = 5,
+  SNAC_DUDTRKO = 6, SNAC_DVDTRKO = 7, SNAC_DWDTRKO = 8
+};
+
+/* Poisson solver choice for snac_ctx_set_solver */
+enum { SNAC_SOLVER_AUTO = 0, SNAC_SOLVER_TENSOR = 1 };
+
+/* Message of the last failure on this host thread ("" if none). */
+const char *snac_last_error(void);
+
+/* Reads dns.nml (&dns, &hypre) and blocks.nml (&blocks) exactly as
+ * read_input does (param.f90:68-277), places the blocks and finds their
+ * neighbours (initmpi.f90:9-254, one piece per block), builds the grids
+ * (initgrid.f90:9-127), the inflow planes (main.f90:600-636, init_inflow
+ * initflow.f90:369-429), the pressure BC rhs planes (init_bc_rhs
+ * solver.f90:44-157) and the Poisson set-up (init_fft_reduction
+ * solver.f90:1124-1138, init_n_2d_matrices solver.f90:981-1020 or
+ * init_matrix_3d solver.f90:158-372, create/setup_n_solvers 1058-1077).
+ * fft_axis 0..3 stands in for the compile-time FFT_AXIS (flags-default.mk).
+ * device: HIP device ordinal, or -1 for a host-only context (set-up and
+ * queries only; every compute entry point then fails).  Fields start at 0. */
+int snac_ctx_create_from_nml(snac_ctx **ctx, const char *dns_nml, const char *blocks_nml,
+                             int fft_axis, int device);
+/* finalize_n_matrices/fftend (main.f90:1144-1150) */
+int snac_ctx_destroy(snac_ctx *ctx);
+
+/* geometry and parameter queries */
+int snac_ctx_num_blocks(const snac_ctx *ctx);
+/* n3 = interior sizes, lo_g3 = global placement, nb6 = neighbour block per
+ * (dir,side) as [x0,x1,y0,y1,z0,z1] (-1 = physical boundary), periods3     */
+int snac_ctx_block_info(const snac_ctx *ctx, int b, int *n3, int *lo_g3, int *nb6, int *periods3);
+/* which: 0..5 = dxc,dxf,dyc,dyf,dzc,dzf ; 6..11 = xc,xf,yc,yf,zc,zf (n+2) */
+int snac_ctx_get_metric(const snac_ctx *ctx, int b, int which, double *out);
+/* dparams: [visc, cfl, dtmax, dt_f, bforce(3), vol_all, time_max, tw_max,
+ *           hypre_tol] (11); iparams: [nstep, icheck, nscal, hypre_maxiter,
+ *           restart, stop_type(3)] (8)                                       */
+int snac_ctx_get_params(const snac_ctx *ctx, double *dparams, int *iparams);
+/* Poisson solver selection (the reference's &hypre choice has no effect on
+ * the discrete solution at tight tolerance)                                 */
+int snac_ctx_set_solver(snac_ctx *ctx, int solver);
+
+/* host <-> device copies of one field of one block (layout above)           */
+int snac_upload_field(snac_ctx *ctx, int b, int fid, const double *host);
+int snac_download_field(snac_ctx *ctx, int b, int fid, double *host);
+
+/* ---- hot-path routines, all blocks at once, queued on the context's
+ *      stream; results are visible after snac_sync or a download ----       */
+/* rk_mom rk.f90:7-128 with momx/y/z_d, momx/y/z_a_vv, momx/y/z_p
+ * (mom.f90:112-386); irk in 1..3 selects rkcoeff (param.f90:11-13)          */
+int snac_rk_mom(snac_ctx *ctx, int irk, double dt);
+/* bounduvw bound.f90:8-66 (updthalo 587-650 becomes device plane copies)    */
+int snac_bounduvw(snac_ctx *ctx, int is_correc);
+/* inflow bound.f90:438-477                                                  */
+int snac_inflow(snac_ctx *ctx, int is_correc);
+/* boundp bound.f90:68-106; fid = SNAC_P or SNAC_PP                          */
+int snac_boundp(snac_ctx *ctx, int fid);
+/* fillps fillps.f90:7-54 followed by updt_rhs bound.f90:543-585             */
+int snac_fillps(snac_ctx *ctx, double dtrk);
+/* fft(arrplan_p(1)) + solve_n_helmholtz_2d (or solve_helmholtz) +
+ * fft(arrplan_p(2)) + normfft scaling, main.f90:1004-1022                   */
+int snac_solve_pressure(snac_ctx *ctx);
+/* correc correc.f90:7-48                                                    */
+int snac_correc(snac_ctx *ctx, double dtrk);
+/* updt_pressure updt_pressure.f90:7-47 (explicit-diffusion form)            */
+int snac_updt_pressure(snac_ctx *ctx);
+/* chkdiv chkdiv.f90:8-44 (synchronises)                                     */
+int snac_chkdiv(snac_ctx *ctx, double *divtot, double *divmax);
+/* chkdt chkdt.f90:8-56 (synchronises)                                       */
+int snac_chkdt(snac_ctx *ctx, double *dtmax);
+/* bounduvw + boundp + inflow of the initial fields, main.f90:562-639        */
+int snac_init_bounds(snac_ctx *ctx);
+/* one RK3 time step in the order of main.f90:906-1033                       */
+int snac_step(snac_ctx *ctx, double dt);
+/* nsteps time steps with dt fixed (no host synchronisation in between)      */
+int snac_steps(snac_ctx *ctx, double dt, int nsteps);
+/* waits for the context's stream                                            */
+int snac_sync(snac_ctx *ctx);
+
+/* ---- building blocks, exposed for their own parity tests ----
+ * Batched r2r transform of howmany contiguous lines of length n, kinds
+ * numbered as FFTW's (fftw.f90:31-54), on the device; host in/out.          */
+int snac_r2r_lines(snac_ctx *ctx, int kind, int n, int howmany, const double *in, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

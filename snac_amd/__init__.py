@@ -1,0 +1,195 @@
+"""snac_amd -- MI355X (gfx950) time-step hot path of SNaC behind a C-ABI.
+
+The compute lives in ``libsnac_amd.so`` (HIP kernels + C++ host set-up), built
+in-tree by ``snac_amd/csrc/Makefile``.  This module is the Python mirror of the
+reference's operator interface for the path: a :class:`Context` is created from
+the same ``dns.nml``/``blocks.nml`` inputs and exposes the reference routines
+by name (``rk_mom``, ``bounduvw``, ``boundp``, ``inflow``, ``fillps``,
+``solve_pressure``, ``correc``, ``updt_pressure``, ``chkdiv``, ``chkdt``) plus a
+whole RK3 ``step``.  There is no CPU fallback: if the library is missing the
+import raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libsnac_amd.so")
+
+FIELDS = {"u": 0, "v": 1, "w": 2, "p": 3, "pp": 4, "po": 5, "dudtrko": 6, "dvdtrko": 7, "dwdtrko": 8}
+RKCOEFF = ((32.0 / 60.0, 0.0), (25.0 / 60.0, -17.0 / 60.0), (45.0 / 60.0, -25.0 / 60.0))
+R2R_KINDS = {"R2HC": 0, "HC2R": 1, "REDFT00": 3, "REDFT01": 4, "REDFT10": 5, "REDFT11": 6,
+             "RODFT00": 7, "RODFT01": 8, "RODFT10": 9, "RODFT11": 10}
+
+
+class SnacError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIBPATH):
+        raise ImportError(f"{LIBPATH} is missing: build it with `make -C snac_amd/csrc` "
+                          "(or __graft_entry__.build()); there is no CPU fallback")
+    L = C.CDLL(LIBPATH)
+    dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+    vp = C.c_void_p
+    L.snac_last_error.restype = C.c_char_p
+    L.snac_ctx_create_from_nml.argtypes = [C.POINTER(vp), C.c_char_p, C.c_char_p, C.c_int, C.c_int]
+    L.snac_ctx_destroy.argtypes = [vp]
+    L.snac_ctx_num_blocks.argtypes = [vp]
+    L.snac_ctx_block_info.argtypes = [vp, C.c_int, ip, ip, ip, ip]
+    L.snac_ctx_get_metric.argtypes = [vp, C.c_int, C.c_int, dp]
+    L.snac_ctx_get_params.argtypes = [vp, dp, ip]
+    L.snac_ctx_set_solver.argtypes = [vp, C.c_int]
+    L.snac_upload_field.argtypes = [vp, C.c_int, C.c_int, dp]
+    L.snac_download_field.argtypes = [vp, C.c_int, C.c_int, dp]
+    L.snac_rk_mom.argtypes = [vp, C.c_int, C.c_double]
+    L.snac_bounduvw.argtypes = [vp, C.c_int]
+    L.snac_inflow.argtypes = [vp, C.c_int]
+    L.snac_boundp.argtypes = [vp, C.c_int]
+    L.snac_fillps.argtypes = [vp, C.c_double]
+    L.snac_solve_pressure.argtypes = [vp]
+    L.snac_correc.argtypes = [vp, C.c_double]
+    L.snac_updt_pressure.argtypes = [vp]
+    L.snac_chkdiv.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.snac_chkdt.argtypes = [vp, C.POINTER(C.c_double)]
+    L.snac_init_bounds.argtypes = [vp]
+    L.snac_step.argtypes = [vp, C.c_double]
+    L.snac_steps.argtypes = [vp, C.c_double, C.c_int]
+    L.snac_sync.argtypes = [vp]
+    L.snac_r2r_lines.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
+    return L
+
+
+lib = _load()
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise SnacError(f"{what}: {lib.snac_last_error().decode()}")
+
+
+class Context:
+    """One case on one GPU; mirrors the reference's hot-path routines."""
+
+    def __init__(self, dns_nml, blocks_nml, fft_axis, device=0):
+        h = C.c_void_p()
+        _check(lib.snac_ctx_create_from_nml(C.byref(h), str(dns_nml).encode(), str(blocks_nml).encode(),
+                                            int(fft_axis), int(device)), "snac_ctx_create_from_nml")
+        self.h = h
+        self.fft_axis = fft_axis
+        self.nblocks = lib.snac_ctx_num_blocks(h)
+        dpar = np.zeros(11)
+        ipar = np.zeros(8, np.int32)
+        _check(lib.snac_ctx_get_params(h, dpar, ipar), "snac_ctx_get_params")
+        self.visc, self.cfl, self.dtmax, self.dt_f = dpar[:4]
+        self.bforce = dpar[4:7].copy()
+        self.vol_all = dpar[7]
+        self.nstep, self.icheck = int(ipar[0]), int(ipar[1])
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.snac_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- geometry ----
+    def block_info(self, b):
+        n = np.zeros(3, np.int32); lo = np.zeros(3, np.int32); nb = np.zeros(6, np.int32); per = np.zeros(3, np.int32)
+        _check(lib.snac_ctx_block_info(self.h, b, n, lo, nb, per), "snac_ctx_block_info")
+        return dict(n=n, lo_g=lo, nb=nb.reshape(3, 2), periods=per)
+
+    def metric(self, b, which):
+        n = self.block_info(b)["n"][(which % 6) // 2]
+        out = np.zeros(n + 2)
+        _check(lib.snac_ctx_get_metric(self.h, b, which, out), "snac_ctx_get_metric")
+        return out
+
+    def shape(self, b, fid):
+        n = self.block_info(b)["n"]
+        return tuple(int(x) + 2 for x in n) if FIELDS[fid] < 6 else tuple(int(x) for x in n)
+
+    # ---- fields (Fortran-ordered numpy arrays, ghosts included) ----
+    def get(self, b, fid):
+        shp = self.shape(b, fid)
+        out = np.zeros(int(np.prod(shp)))
+        _check(lib.snac_download_field(self.h, b, FIELDS[fid], out), "snac_download_field")
+        return out.reshape(shp, order="F")
+
+    def set(self, b, fid, arr):
+        shp = self.shape(b, fid)
+        a = np.asarray(arr, dtype=np.float64)
+        if a.shape != shp:
+            raise ValueError(f"field {fid} of block {b} must have shape {shp}, got {a.shape}")
+        _check(lib.snac_upload_field(self.h, b, FIELDS[fid], np.ascontiguousarray(a.reshape(-1, order="F"))),
+               "snac_upload_field")
+
+    # ---- reference routines ----
+    def rk_mom(self, irk, dt):
+        _check(lib.snac_rk_mom(self.h, irk, dt), "rk_mom")
+
+    def bounduvw(self, is_correc):
+        _check(lib.snac_bounduvw(self.h, int(bool(is_correc))), "bounduvw")
+
+    def inflow(self, is_correc):
+        _check(lib.snac_inflow(self.h, int(bool(is_correc))), "inflow")
+
+    def boundp(self, which="p"):
+        _check(lib.snac_boundp(self.h, FIELDS[which]), "boundp")
+
+    def fillps(self, dtrk):
+        _check(lib.snac_fillps(self.h, dtrk), "fillps")
+
+    def solve_pressure(self):
+        _check(lib.snac_solve_pressure(self.h), "solve_pressure")
+
+    def correc(self, dtrk):
+        _check(lib.snac_correc(self.h, dtrk), "correc")
+
+    def updt_pressure(self):
+        _check(lib.snac_updt_pressure(self.h), "updt_pressure")
+
+    def chkdiv(self):
+        a, b = C.c_double(), C.c_double()
+        _check(lib.snac_chkdiv(self.h, C.byref(a), C.byref(b)), "chkdiv")
+        return a.value, b.value
+
+    def chkdt(self):
+        a = C.c_double()
+        _check(lib.snac_chkdt(self.h, C.byref(a)), "chkdt")
+        return a.value
+
+    def init_bounds(self):
+        _check(lib.snac_init_bounds(self.h), "init_bounds")
+
+    def step(self, dt):
+        _check(lib.snac_step(self.h, dt), "step")
+
+    def steps(self, dt, n):
+        _check(lib.snac_steps(self.h, dt, int(n)), "steps")
+
+    def sync(self):
+        _check(lib.snac_sync(self.h), "sync")
+
+    def r2r_lines(self, kind, x):
+        x = np.ascontiguousarray(np.atleast_2d(np.asarray(x, dtype=np.float64)))
+        out = np.zeros_like(x)
+        _check(lib.snac_r2r_lines(self.h, R2R_KINDS[kind], x.shape[1], x.shape[0], x.reshape(-1), out.reshape(-1)),
+               "r2r_lines")
+        return out
+
+    def new_dt(self):
+        """main.f90:660-661: dt = dt_f if > 0 else min(cfl*dt_cfl, dtmax)."""
+        if self.dt_f > 0:
+            return self.dt_f
+        return min(self.cfl * self.chkdt(), self.dtmax)

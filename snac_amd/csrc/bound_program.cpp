@@ -1,0 +1,205 @@
+// Ghost-fill programs.  bounduvw/boundp/inflow (bound.f90:8-106, 438-477)
+// are expanded into a sequence of plane operations in the reference's order
+// (halos per direction x, y, z -- the per-field order of the reference is
+// immaterial because fields do not interact -- then set_bc per block in the
+// order x0, x1, y0, y1, z0, z1, then the inflow planes).  Consecutive
+// operations are packed into one phase as long as no operation of the phase
+// writes a plane another one reads or writes; each phase is one kernel
+// launch.  Two planes of one array intersect when their normals differ
+// (they share an edge line) or when they are the same plane.  This keeps the
+// edge and corner ghost values exactly as the reference's sequential fill
+// leaves them, which matters because the advection stencil reads edge
+// ghosts (e.g. v(i+1,j-1,k) in momx_a_vv, mom.f90:112-147).
+#include "bound_program.hpp"
+
+#include <algorithm>
+
+namespace snac {
+namespace {
+
+struct Rec {
+  FaceOp op;
+  const void *warr, *rarr;  // array identities (null when no read)
+};
+
+bool planes_meet(const void *a1, int d1, int q1, const void *a2, int d2, int q2) {
+  if (!a1 || !a2 || a1 != a2) return false;
+  return d1 != d2 || q1 == q2;
+}
+
+bool conflict(const Rec &x, const Rec &y) {
+  if (planes_meet(x.warr, x.op.d, x.op.qdst, y.warr, y.op.d, y.op.qdst)) return true;
+  if (planes_meet(x.warr, x.op.d, x.op.qdst, y.rarr, y.op.d, y.op.qsrc)) return true;
+  if (planes_meet(y.warr, y.op.d, y.op.qdst, x.rarr, x.op.d, x.op.qsrc)) return true;
+  return false;
+}
+
+void tang(int d, int &a1, int &a2) {
+  a1 = d == 0 ? 1 : 0;
+  a2 = d == 2 ? 1 : 2;
+}
+
+Rec mk(const BlockArrays &B, double *dst, int d, int qdst, const double *src, const BlockArrays *SB, int qsrc,
+       const double *arr, double c0, double c1, double c2) {
+  Rec r;
+  FaceOp &o = r.op;
+  int a1, a2;
+  tang(d, a1, a2);
+  o.dst = dst;
+  o.src = src;
+  o.arr = arr;
+  o.dsx = B.g.sx;
+  o.dsy = B.g.sy;
+  const BlockArrays &S = SB ? *SB : B;
+  o.ssx = S.g.sx;
+  o.ssy = S.g.sy;
+  o.d = d;
+  o.qdst = qdst;
+  o.qsrc = qsrc;
+  int n3[3] = {B.g.nx, B.g.ny, B.g.nz};
+  o.m1 = n3[a1] + 2;
+  o.m2 = n3[a2] + 2;
+  o.c0 = c0;
+  o.c1 = c1;
+  o.c2 = c2;
+  r.warr = dst;
+  r.rarr = src;
+  return r;
+}
+
+// set_bc bound.f90:108-248 as plane operations
+void set_bc_ops(std::vector<Rec> &out, const BlockArrays &B, double *p, char ctype, int side, int d, bool centered,
+                double rvalue, double dr) {
+  if (ctype != 'D' && ctype != 'N') return;
+  int n3[3] = {B.g.nx, B.g.ny, B.g.nz};
+  int n = n3[d];
+  double factor = rvalue, sgn = 0.0;
+  if (ctype == 'D' && centered) { factor = 2.0 * factor; sgn = -1.0; }
+  if (ctype == 'N') { factor = (side == 0 ? -dr : dr) * factor; sgn = 1.0; }
+  if (centered) {
+    if (side == 0) out.push_back(mk(B, p, d, 0, p, nullptr, 1, nullptr, factor, sgn, 0.0));
+    else out.push_back(mk(B, p, d, n + 1, p, nullptr, n, nullptr, factor, sgn, 0.0));
+  } else if (ctype == 'D') {
+    if (side == 0) out.push_back(mk(B, p, d, 0, nullptr, nullptr, 0, nullptr, factor, 0.0, 0.0));
+    else {
+      out.push_back(mk(B, p, d, n, nullptr, nullptr, 0, nullptr, factor, 0.0, 0.0));
+      out.push_back(mk(B, p, d, n + 1, p, nullptr, n - 1, nullptr, 0.0, 1.0, 0.0));
+    }
+  } else {
+    if (side == 0) out.push_back(mk(B, p, d, 0, p, nullptr, 1, nullptr, factor, 1.0, 0.0));
+    else {
+      out.push_back(mk(B, p, d, n, p, nullptr, n - 1, nullptr, factor, 1.0, 0.0));
+      out.push_back(mk(B, p, d, n + 1, p, nullptr, n - 1, nullptr, factor, 1.0, 0.0));
+    }
+  }
+}
+
+void halo_ops(std::vector<Rec> &out, const std::vector<BlockArrays> &bl, const Case &cs, int d,
+              double *(*field)(const BlockArrays &, int), int fid) {
+  for (size_t b = 0; b < bl.size(); b++) {
+    const Block &B = cs.blocks[b];
+    int n3[3] = {B.n[0], B.n[1], B.n[2]};
+    for (int side = 0; side < 2; side++) {
+      int nb = B.nb[d][side];
+      if (nb < 0) continue;
+      int qdst = side == 0 ? 0 : n3[d] + 1;
+      int qsrc = side == 0 ? cs.blocks[nb].n[d] : 1;
+      out.push_back(mk(bl[b], field(bl[b], fid), d, qdst, field(bl[nb], fid), &bl[nb], qsrc, nullptr, 0.0, 1.0, 0.0));
+    }
+  }
+}
+
+}  // namespace
+
+namespace {
+std::vector<std::vector<FaceOp>> pack(const std::vector<Rec> &seq) {
+  std::vector<std::vector<FaceOp>> phases;
+  std::vector<Rec> cur;
+  for (const Rec &r : seq) {
+    bool clash = false;
+    for (const Rec &c : cur)
+      if (conflict(r, c)) { clash = true; break; }
+    if (clash) {
+      std::vector<FaceOp> ph;
+      for (auto &c : cur) ph.push_back(c.op);
+      phases.push_back(ph);
+      cur.clear();
+    }
+    cur.push_back(r);
+  }
+  if (!cur.empty()) {
+    std::vector<FaceOp> ph;
+    for (auto &c : cur) ph.push_back(c.op);
+    phases.push_back(ph);
+  }
+  return phases;
+}
+}  // namespace
+
+std::vector<std::vector<FaceOp>> build_velocity_program(const Case &cs, const std::vector<BlockArrays> &bl, int par,
+                                                        bool is_correc, bool with_halo_bc, bool with_inflow) {
+  std::vector<Rec> seq;
+  std::vector<BlockArrays> sel = bl;
+  for (auto &B : sel) B.par_sel = par;
+  auto field = [](const BlockArrays &B, int comp) -> double * { return B.vel[B.par_sel][comp]; };
+  if (with_halo_bc) {
+    for (int d = 0; d < 3; d++)
+      for (int comp = 0; comp < 3; comp++) halo_ops(seq, sel, cs, d, field, comp);
+    for (size_t b = 0; b < sel.size(); b++) {
+      const Block &B = cs.blocks[b];
+      for (int d = 0; d < 3; d++)
+        for (int side = 0; side < 2; side++) {
+          if (!B.is_bound[d][side]) continue;
+          int qm = side == 0 ? 0 : B.n[d];
+          for (int comp = 0; comp < 3; comp++) {
+            bool centered = comp != d;
+            if (!centered && is_correc) continue;
+            double dr = centered ? B.dc[d][qm] : B.df[d][qm];
+            set_bc_ops(seq, sel[b], field(sel[b], comp), B.cbcvel[comp][d][side], side, d, centered,
+                       B.bcvel[comp][d][side], dr);
+          }
+        }
+    }
+  }
+  if (with_inflow) {
+    for (size_t b = 0; b < sel.size(); b++) {
+      const Block &B = cs.blocks[b];
+      for (int d = 0; d < 3; d++)
+        for (int side = 0; side < 2; side++) {
+          if (!B.is_inflow[d][side]) continue;
+          int n = B.n[d];
+          int qn = side == 0 ? 0 : n, qt = side == 0 ? 0 : n + 1, dq = side == 0 ? 1 : -1;
+          for (int comp = 0; comp < 3; comp++) {
+            double *F = field(sel[b], comp);
+            const double *arr = sel[b].vin[d][comp][side];
+            if (comp == d) {
+              if (!is_correc) seq.push_back(mk(sel[b], F, d, qn, nullptr, nullptr, 0, arr, 0.0, 0.0, 1.0));
+            } else {
+              seq.push_back(mk(sel[b], F, d, qt, F, nullptr, qt + dq, arr, 0.0, -1.0, 2.0));
+            }
+          }
+        }
+    }
+  }
+  return pack(seq);
+}
+
+std::vector<std::vector<FaceOp>> build_pressure_program(const Case &cs, const std::vector<BlockArrays> &bl, int which) {
+  std::vector<Rec> seq;
+  std::vector<BlockArrays> sel = bl;
+  for (auto &B : sel) B.par_sel = which;  // 0 = p, 1 = pp
+  auto field = [](const BlockArrays &B, int) -> double * { return B.par_sel == 0 ? B.p : B.pp; };
+  for (int d = 0; d < 3; d++) halo_ops(seq, sel, cs, d, field, 0);
+  for (size_t b = 0; b < sel.size(); b++) {
+    const Block &B = cs.blocks[b];
+    for (int d = 0; d < 3; d++)
+      for (int side = 0; side < 2; side++) {
+        if (!B.is_bound[d][side]) continue;
+        int qm = side == 0 ? 0 : B.n[d];
+        set_bc_ops(seq, sel[b], field(sel[b], 0), B.cbcpre[d][side], side, d, true, B.bcpre[d][side], B.dc[d][qm]);
+      }
+  }
+  return pack(seq);
+}
+
+}  // namespace snac

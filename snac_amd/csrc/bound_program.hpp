@@ -1,0 +1,26 @@
+// Ghost-fill programs: per-phase lists of FaceOps (see bound_program.cpp).
+#pragma once
+#include <vector>
+
+#include "case.hpp"
+#include "common.hpp"
+
+namespace snac {
+
+// Device arrays of one block that the fill programs address.
+struct BlockArrays {
+  Geo g;
+  double *vel[2][3];          // velocity ping-pong buffers [parity][u,v,w]
+  double *p, *pp;
+  const double *vin[3][3][2]; // device inflow planes [dir][comp][side] (null if none)
+  int par_sel = 0;            // selector used while building
+};
+
+// bounduvw (halo + set_bc, when with_halo_bc) and/or inflow on the velocity
+// buffers of parity `par`; returns the phases in execution order.
+std::vector<std::vector<FaceOp>> build_velocity_program(const Case &cs, const std::vector<BlockArrays> &bl, int par,
+                                                        bool is_correc, bool with_halo_bc, bool with_inflow);
+// boundp on p (which = 0) or pp (which = 1)
+std::vector<std::vector<FaceOp>> build_pressure_program(const Case &cs, const std::vector<BlockArrays> &bl, int which);
+
+}  // namespace snac

@@ -1,0 +1,103 @@
+// Shared declarations of the gfx950 kernels and their host launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+
+namespace snac {
+
+#define SNAC_HIP(x)                                                                              \
+  do {                                                                                           \
+    hipError_t e_ = (x);                                                                         \
+    if (e_ != hipSuccess)                                                                        \
+      throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + " (" + #x + \
+                               ") at " + __FILE__ + ":" + std::to_string(__LINE__));            \
+  } while (0)
+
+// Extents of one block: interior sizes and strides of the ghosted arrays
+// (column-major, i fastest, index 0 = the lo-1 ghost).
+struct Geo {
+  int nx, ny, nz;
+  long sx, sy;  // sx = nx+2, sy = ny+2
+};
+
+// 1D grid metrics of one block on the device (n+2 entries each).
+struct Metrics {
+  const double *dxc, *dxf, *dyc, *dyf, *dzc, *dzf;
+};
+
+// One ghost-plane operation: dst plane (normal d, index qdst) =
+//   c0 + c1 * src plane (index qsrc, same tangential element) + c2 * arr[t]
+// over the full ghosted tangential extent m1 x m2.  Covers halo copies
+// (updthalo), every set_bc form and the inflow planes.
+struct FaceOp {
+  double *dst;
+  const double *src;
+  const double *arr;
+  long dsx, dsy, ssx, ssy;  // strides of dst / src arrays
+  int d, qdst, qsrc, m1, m2;
+  double c0, c1, c2;
+};
+
+// ---- stencil kernels (kernels_stencil.hip) ----
+// rk_mom: reads u,v,w,p (ghosted) and dudtrko; writes the updated interior
+// velocities to un,vn,wn and dudtrk to dto.
+void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v,
+                   const double *w, const double *p, double *un, double *vn, double *wn, double *dtu,
+                   double *dtv, double *dtw, double f1, double f2, double f12, double visc, double bfx,
+                   double bfy, double bfz);
+// fillps with updt_rhs fused; rhs planes may be null (ordered x0,x1,y0,y1,z0,z1)
+void launch_fillps(hipStream_t s, const Geo &g, const Metrics &m, int fft_axis, const double *u,
+                   const double *v, const double *w, double *pp, double dtrk, const double *const rhsp[6]);
+// correc out-of-place over the whole ghosted array (copy outside the
+// reference's loop ranges); optionally fuses updt_pressure (p += pp).
+void launch_correc(hipStream_t s, const Geo &g, const Metrics &m, const double *pp, const double *u,
+                   const double *v, const double *w, double *un, double *vn, double *wn, double dtrk,
+                   double *p_update);
+void launch_updt_pressure(hipStream_t s, const Geo &g, const double *pp, double *p);
+// reductions: partial results into part (size >= nparts(g)); final on device
+int reduce_nparts(const Geo &g);
+void launch_chkdiv_part(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v,
+                        const double *w, double vol, double *part_sum, double *part_max);
+void launch_chkdt_part(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v,
+                       const double *w, double *part_max);
+// reduces n partial sums and maxima into out[0] (sum) and out[1] (max)
+void launch_reduce_final(hipStream_t s, const double *part_sum, const double *part_max, int n, double *out);
+
+// ---- ghost planes (kernels_bound.hip) ----
+void launch_face_ops(hipStream_t s, const FaceOp *dev_ops, int nops, int max_plane);
+
+// ---- transforms (kernels_r2r.hip) ----
+// Batched r2r along lines: line (b1,b2) element q at in[q*ies + b1*is1 + b2*is2],
+// written to out[q*oes + b1*os1 + b2*os2] times scale.  tw: per-size table
+// from r2r_table(); mat: dense n x n matrix (row = output) or null.
+struct R2RPlan {
+  int n = 0, kind = 0;
+  bool fft = false;        // LDS FFT path (n a power of two >= 4, supported kind)
+  const double *tab = nullptr;  // device twiddles (fft) or dense matrix
+};
+R2RPlan r2r_plan(int kind, int n);  // cached per (kind, n); throws if unsupported
+void launch_r2r(hipStream_t s, const R2RPlan &pl, const double *in, long ies, long is1, long is2,
+                double *out, long oes, long os1, long os2, int nb1, int nb2, double scale);
+// host reference of FFTW's r2r definitions, used to build dense matrices
+double r2r_coef(int kind, int n, int k, int j);
+
+// ---- tridiagonal line solves (kernels_tridiag.hip) ----
+// Lines along axis s of a (G0,G1,G2) array (i fastest).  For the line at
+// transformed indices (t0,t1) of the other two axes:
+//   lam = lamA[idxA] + lamB[idxB] ; diag[g] = d0[g] + lam * wt[g]
+//   lower[g] = am[g], upper[g] = ap[g]
+// idxA/idxB are the indices along the two non-line axes (lamA/lamB sized to
+// them).  Lines with lam == 0 and `pin_when_zero` get their last unknown
+// fixed to zero (singular Neumann-Neumann line).
+struct TridiagLine {
+  int G[3];
+  int s;  // line axis
+  const double *am, *ap, *d0, *wt;  // size G[s]
+  const double *lamA, *lamB;        // along the other axes (ascending order)
+  int pin_when_zero;
+};
+void launch_tridiag(hipStream_t st, const TridiagLine &t, double *x, double *scratch);
+
+}  // namespace snac

@@ -1,0 +1,699 @@
+// Device context and C-ABI (include/snac_amd.h).  Owns every block's fields
+// in HBM, the ghost-fill programs, the Poisson plan, and drives one RK3 step
+// in the order of main.f90:906-1033 on one HIP stream.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/snac_amd.h"
+#include "bound_program.hpp"
+#include "case.hpp"
+#include "common.hpp"
+
+namespace snac {
+namespace {
+
+thread_local std::string g_err;
+
+const double RKC[3][2] = {{32.0 / 60.0, 0.0}, {25.0 / 60.0, -17.0 / 60.0}, {45.0 / 60.0, -25.0 / 60.0}};
+
+struct DevBlock {
+  Geo g;
+  Metrics m;
+  double *met[6] = {};          // dxc dxf dyc dyf dzc dzf (device, owned)
+  double *vel[2][3] = {};       // ping-pong velocity buffers
+  double *p = nullptr, *pp = nullptr, *po = nullptr;
+  double *dto[3] = {};
+  double *rhsp[6] = {};
+  double *vin[3][3][2] = {};
+  long ntot = 0, nin = 0;
+  int part_off = 0, nparts = 0;
+};
+
+struct Program {
+  std::vector<FaceOp *> dev;  // one device array per phase
+  std::vector<int> nops, maxplane;
+};
+
+// Tensor-product Poisson plan: r2r along every uniform axis (the FFT axis
+// first), tridiagonal solves along the remaining axis.
+struct TensorPlan {
+  bool ok = false;
+  std::string why;
+  int G[3] = {1, 1, 1}, gmin[3] = {0, 0, 0};
+  std::vector<int> xaxes;          // transformed axes, FFT axis first
+  std::vector<R2RPlan> fwd, bwd;
+  std::vector<double> norm;
+  int s = -1;
+  double *lam[3] = {};             // device eigenvalue arrays per axis (line axis: zeros)
+  double *am = nullptr, *ap = nullptr, *d0 = nullptr, *wt = nullptr;
+  int pin = 0;
+  double *ph = nullptr, *scratch = nullptr;
+};
+
+}  // namespace
+}  // namespace snac
+
+struct snac_ctx {
+  snac::Case cs;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<snac::DevBlock> blk;
+  int cur = 0;  // parity of the current velocity buffers
+  snac::Program prog_uvw[2][2], prog_bound_only[2][2], prog_inflow[2][2], prog_p, prog_pp;
+  snac::TensorPlan tp;
+  int solver = SNAC_SOLVER_AUTO;
+  double *part_sum = nullptr, *part_max = nullptr, *red = nullptr;
+  int nparts = 0;
+  double dlmin = 0;
+  std::vector<void *> owned;
+};
+
+namespace snac {
+namespace {
+
+// device allocations are skipped in a host-only context (device < 0)
+template <class T>
+T *dalloc(snac_ctx *c, size_t n) {
+  if (c->device < 0) return nullptr;
+  void *p = nullptr;
+  SNAC_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+  SNAC_HIP(hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)));
+  c->owned.push_back(p);
+  return (T *)p;
+}
+template <class T>
+T *dupload(snac_ctx *c, const std::vector<T> &h) {
+  T *p = dalloc<T>(c, h.size());
+  if (p && !h.empty()) SNAC_HIP(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+std::vector<BlockArrays> block_arrays(snac_ctx *c) {
+  std::vector<BlockArrays> out(c->blk.size());
+  for (size_t b = 0; b < c->blk.size(); b++) {
+    DevBlock &D = c->blk[b];
+    BlockArrays &A = out[b];
+    A.g = D.g;
+    for (int par = 0; par < 2; par++)
+      for (int q = 0; q < 3; q++) A.vel[par][q] = D.vel[par][q];
+    A.p = D.p;
+    A.pp = D.pp;
+    for (int d = 0; d < 3; d++)
+      for (int q = 0; q < 3; q++)
+        for (int s = 0; s < 2; s++) A.vin[d][q][s] = D.vin[d][q][s];
+  }
+  return out;
+}
+
+Program upload_program(snac_ctx *c, const std::vector<std::vector<FaceOp>> &phases) {
+  Program pr;
+  if (c->device < 0) return pr;
+  for (auto &ph : phases) {
+    FaceOp *d = dalloc<FaceOp>(c, ph.size());
+    SNAC_HIP(hipMemcpy(d, ph.data(), ph.size() * sizeof(FaceOp), hipMemcpyHostToDevice));
+    int mp = 0;
+    for (auto &o : ph) mp = std::max(mp, o.m1 * o.m2);
+    pr.dev.push_back(d);
+    pr.nops.push_back((int)ph.size());
+    pr.maxplane.push_back(mp);
+  }
+  return pr;
+}
+
+void run_program(snac_ctx *c, const Program &pr) {
+  for (size_t q = 0; q < pr.dev.size(); q++) launch_face_ops(c->stream, pr.dev[q], pr.nops[q], pr.maxplane[q]);
+}
+
+// ---- tensor Poisson planner ----
+void plan_tensor(snac_ctx *c) {
+  TensorPlan &t = c->tp;
+  const Case &cs = c->cs;
+  int fa = cs.fft_axis - 1;
+  // global extents of the union
+  int gmax[3];
+  for (int a = 0; a < 3; a++) {
+    t.gmin[a] = 1 << 30;
+    gmax[a] = -(1 << 30);
+    for (auto &B : cs.blocks) {
+      t.gmin[a] = std::min(t.gmin[a], B.lo_g[a]);
+      gmax[a] = std::max(gmax[a], B.lo_g[a] + B.n[a] - 1);
+    }
+    t.G[a] = gmax[a] - t.gmin[a] + 1;
+  }
+  double vol = 0;
+  for (auto &B : cs.blocks) vol += (double)B.n[0] * B.n[1] * B.n[2];
+  if (vol != (double)t.G[0] * t.G[1] * t.G[2]) { t.why = "the blocks do not tile a box"; return; }
+  // per-axis global 1D grids: df[g], dcm[g] (between g-1 and g), dcp[g]
+  std::vector<double> gdf[3], gdcm[3], gdcp[3];
+  char bclo[3], bchi[3];
+  for (int a = 0; a < 3; a++) {
+    gdf[a].assign(t.G[a], 0.0); gdcm[a].assign(t.G[a], 0.0); gdcp[a].assign(t.G[a], 0.0);
+    std::vector<int> seen(t.G[a], 0);
+    bclo[a] = bchi[a] = 0;
+    for (auto &B : cs.blocks) {
+      for (int i = 1; i <= B.n[a]; i++) {
+        int g = B.lo_g[a] - t.gmin[a] + i - 1;
+        double f = B.df[a][i], cm = B.dc[a][i - 1], cp = B.dc[a][i];
+        if (seen[g] && (std::fabs(gdf[a][g] - f) > 1e-14 * f || std::fabs(gdcm[a][g] - cm) > 1e-14 * cm ||
+                        std::fabs(gdcp[a][g] - cp) > 1e-14 * cp)) {
+          t.why = "inconsistent grids among blocks";
+          return;
+        }
+        gdf[a][g] = f; gdcm[a][g] = cm; gdcp[a][g] = cp; seen[g] = 1;
+      }
+      if (B.lo_g[a] == t.gmin[a]) {
+        char bc = B.is_bound[a][0] ? B.cbcpre[a][0] : 'F';
+        if (bclo[a] && bclo[a] != bc) { t.why = "mixed boundary conditions on one face"; return; }
+        bclo[a] = bc;
+      }
+      if (B.lo_g[a] + B.n[a] - 1 == gmax[a]) {
+        char bc = B.is_bound[a][1] ? B.cbcpre[a][1] : 'F';
+        if (bchi[a] && bchi[a] != bc) { t.why = "mixed boundary conditions on one face"; return; }
+        bchi[a] = bc;
+      }
+    }
+    if ((bclo[a] == 'F') != (bchi[a] == 'F')) { t.why = "half-periodic axis"; return; }
+  }
+  // candidate transform axes: uniform spacing and a transformable BC pair
+  std::vector<int> red;
+  for (int a = 0; a < 3; a++) if (a != fa) red.push_back(a);
+  auto uniform = [&](int a) {
+    double h = gdf[a][0];
+    for (int g = 0; g < t.G[a]; g++)
+      if (std::fabs(gdf[a][g] - h) > 1e-12 * h || std::fabs(gdcm[a][g] - h) > 1e-12 * h ||
+          std::fabs(gdcp[a][g] - h) > 1e-12 * h)
+        return false;
+    return true;
+  };
+  auto transformable = [&](int a) {
+    int kf, kb;
+    double nr[2];
+    return find_fft(bclo[a], bchi[a], kf, kb, nr);
+  };
+  std::vector<int> cand, non;
+  for (int a : red) ((uniform(a) && transformable(a)) ? cand : non).push_back(a);
+  if (non.size() > 1) { t.why = "more than one non-uniform axis"; return; }
+  int s;
+  if (non.size() == 1) s = non[0];
+  else {
+    s = -1;
+    for (int q = (int)cand.size() - 1; q >= 0; q--)
+      if (bclo[cand[q]] != 'F') { s = cand[q]; break; }
+    if (s < 0) { t.why = "no non-periodic axis for the line solves"; return; }
+  }
+  if (bclo[s] == 'F') { t.why = "periodic line axis"; return; }
+  t.s = s;
+  if (fa >= 0) t.xaxes.push_back(fa);
+  for (int a : red) if (a != s) t.xaxes.push_back(a);
+  // the first transform runs block by block from pp into the work array, so
+  // its axis must be spanned by every block (always true for the FFT axis)
+  {
+    int first = -1;
+    for (size_t q = 0; q < t.xaxes.size() && first < 0; q++) {
+      bool full = true;
+      for (auto &B : cs.blocks) full = full && B.n[t.xaxes[q]] == t.G[t.xaxes[q]];
+      if (full) first = (int)q;
+    }
+    if (first < 0) { t.why = "no transform axis spans every block"; return; }
+    std::swap(t.xaxes[0], t.xaxes[first]);
+  }
+  // transforms, eigenvalues
+  std::vector<double> lamh[3];
+  for (int a = 0; a < 3; a++) lamh[a].assign(t.G[a], 0.0);
+  for (int a : t.xaxes) {
+    int kf, kb;
+    double nr[2];
+    char c0 = (a == fa) ? cs.blocks[0].cbcpre[a][0] : bclo[a];
+    char c1 = (a == fa) ? cs.blocks[0].cbcpre[a][1] : bchi[a];
+    if (!find_fft(c0, c1, kf, kb, nr)) { t.why = "unsupported BC pair along a transformed axis"; return; }
+    if (c->device >= 0) {
+      t.fwd.push_back(r2r_plan(kf, t.G[a]));
+      t.bwd.push_back(r2r_plan(kb, t.G[a]));
+    }
+    t.norm.push_back(1.0 / (nr[0] * (t.G[a] + nr[1])));
+    eigenvalues(t.G[a], c0, c1, lamh[a]);
+    // spacing used by the reference: dxc_g(lo_g-1) of the FFT axis
+    // (main.f90:717-728); the uniform spacing for the other axes
+    double h = (a == fa) ? cs.blocks[0].df[a][1] : gdf[a][0];
+    for (auto &l : lamh[a]) l /= h * h;
+  }
+  // the two non-line axes in ascending order feed lamA / lamB
+  for (int a = 0; a < 3; a++) t.lam[a] = dupload(c, lamh[a]);
+  // line coefficients: h_other = product of the uniform spacings of the
+  // transformed in-plane axes (the FFT axis is not part of the 2D operator)
+  double hother = 1.0;
+  for (int a : red) if (a != s) hother *= gdf[a][0];
+  int m = t.G[s];
+  std::vector<double> am(m), ap(m), d0(m), wt(m);
+  double sg0 = bclo[s] == 'N' ? 1.0 : -1.0, sg1 = bchi[s] == 'N' ? 1.0 : -1.0;
+  for (int g = 0; g < m; g++) {
+    double rm = 1.0 / gdcm[s][g] * hother, rp = 1.0 / gdcp[s][g] * hother;
+    double dg = -(rm + rp);
+    am[g] = rm; ap[g] = rp;
+    if (g == 0) { dg += sg0 * rm; am[g] = 0.0; }
+    if (g == m - 1) { dg += sg1 * rp; ap[g] = 0.0; }
+    d0[g] = dg;
+    wt[g] = hother * gdf[s][g];
+  }
+  t.am = dupload(c, am); t.ap = dupload(c, ap); t.d0 = dupload(c, d0); t.wt = dupload(c, wt);
+  t.pin = (bclo[s] == 'N' && bchi[s] == 'N') ? 1 : 0;
+  size_t tot = (size_t)t.G[0] * t.G[1] * t.G[2];
+  t.ph = dalloc<double>(c, tot);
+  t.scratch = dalloc<double>(c, tot);
+  t.ok = true;
+}
+
+void check_solver(snac_ctx *c) {
+  if (!c->tp.ok)
+    throw std::runtime_error("no GPU Poisson solver for this case: the tensor-product solver needs " +
+                             std::string("a box-shaped union with uniform transform axes (") + c->tp.why + ")");
+}
+
+// strides of axis a in a ghosted block array, and of the interior origin
+long gstride(const DevBlock &D, int a) { return a == 0 ? 1 : (a == 1 ? D.g.sx : D.g.sx * D.g.sy); }
+int bext(const DevBlock &D, int a) { return a == 0 ? D.g.nx : (a == 1 ? D.g.ny : D.g.nz); }
+
+void solve_pressure(snac_ctx *c) {
+  check_solver(c);
+  TensorPlan &t = c->tp;
+  const long pst[3] = {1, (long)t.G[0], (long)t.G[0] * t.G[1]};
+  auto others = [](int a, int &b1, int &b2) { b1 = a == 0 ? 1 : 0; b2 = a == 2 ? 1 : 2; };
+  // first transform: block pp interiors -> global work array
+  {
+    int a = t.xaxes[0], b1, b2;
+    others(a, b1, b2);
+    for (size_t b = 0; b < c->blk.size(); b++) {
+      DevBlock &D = c->blk[b];
+      const Block &B = c->cs.blocks[b];
+      const double *in = D.pp + 1 + D.g.sx + D.g.sx * D.g.sy;
+      long off = 0;
+      for (int q = 0; q < 3; q++) off += (long)(B.lo_g[q] - t.gmin[q]) * pst[q];
+      launch_r2r(c->stream, t.fwd[0], in, gstride(D, a), gstride(D, b1), gstride(D, b2), t.ph + off, pst[a], pst[b1],
+                 pst[b2], bext(D, b1), bext(D, b2), 1.0);
+    }
+  }
+  for (size_t q = 1; q < t.xaxes.size(); q++) {
+    int a = t.xaxes[q], b1, b2;
+    others(a, b1, b2);
+    launch_r2r(c->stream, t.fwd[q], t.ph, pst[a], pst[b1], pst[b2], t.ph, pst[a], pst[b1], pst[b2], t.G[b1], t.G[b2],
+               1.0);
+  }
+  {
+    TridiagLine tl;
+    for (int q = 0; q < 3; q++) tl.G[q] = t.G[q];
+    tl.s = t.s;
+    int o1 = t.s == 0 ? 1 : 0, o2 = t.s == 2 ? 1 : 2;
+    tl.am = t.am; tl.ap = t.ap; tl.d0 = t.d0; tl.wt = t.wt;
+    tl.lamA = t.lam[o1];
+    tl.lamB = t.lam[o2];
+    tl.pin_when_zero = t.pin;
+    launch_tridiag(c->stream, tl, t.ph, t.scratch);
+  }
+  for (size_t q = t.xaxes.size() - 1; q >= 1; q--) {
+    int a = t.xaxes[q], b1, b2;
+    others(a, b1, b2);
+    launch_r2r(c->stream, t.bwd[q], t.ph, pst[a], pst[b1], pst[b2], t.ph, pst[a], pst[b1], pst[b2], t.G[b1], t.G[b2],
+               t.norm[q]);
+  }
+  {
+    int a = t.xaxes[0], b1, b2;
+    others(a, b1, b2);
+    for (size_t b = 0; b < c->blk.size(); b++) {
+      DevBlock &D = c->blk[b];
+      const Block &B = c->cs.blocks[b];
+      double *out = D.pp + 1 + D.g.sx + D.g.sx * D.g.sy;
+      long off = 0;
+      for (int q = 0; q < 3; q++) off += (long)(B.lo_g[q] - t.gmin[q]) * pst[q];
+      launch_r2r(c->stream, t.bwd[0], t.ph + off, pst[a], pst[b1], pst[b2], out, gstride(D, a), gstride(D, b1),
+                 gstride(D, b2), bext(D, b1), bext(D, b2), t.norm[0]);
+    }
+  }
+}
+
+void rk_mom(snac_ctx *c, int irk, double dt) {
+  const double f1 = RKC[irk - 1][0] * dt, f2 = RKC[irk - 1][1] * dt, f12 = f1 + f2;
+  int nx = 1 - c->cur;
+  for (auto &D : c->blk)
+    launch_rk_mom(c->stream, D.g, D.m, D.vel[c->cur][0], D.vel[c->cur][1], D.vel[c->cur][2], D.p, D.vel[nx][0],
+                  D.vel[nx][1], D.vel[nx][2], D.dto[0], D.dto[1], D.dto[2], f1, f2, f12, c->cs.visc, c->cs.bforce[0],
+                  c->cs.bforce[1], c->cs.bforce[2]);
+  c->cur = nx;
+}
+
+void fillps(snac_ctx *c, double dtrk) {
+  for (auto &D : c->blk)
+    launch_fillps(c->stream, D.g, D.m, c->cs.fft_axis, D.vel[c->cur][0], D.vel[c->cur][1], D.vel[c->cur][2], D.pp,
+                  dtrk, D.rhsp);
+}
+
+void correc(snac_ctx *c, double dtrk, bool fuse_pressure) {
+  int nx = 1 - c->cur;
+  for (auto &D : c->blk)
+    launch_correc(c->stream, D.g, D.m, D.pp, D.vel[c->cur][0], D.vel[c->cur][1], D.vel[c->cur][2], D.vel[nx][0],
+                  D.vel[nx][1], D.vel[nx][2], dtrk, fuse_pressure ? D.p : nullptr);
+  c->cur = nx;
+}
+
+void substage(snac_ctx *c, int irk, double dt) {
+  double dtrk = (RKC[irk - 1][0] + RKC[irk - 1][1]) * dt;
+  rk_mom(c, irk, dt);
+  run_program(c, c->prog_uvw[c->cur][0]);   // bounduvw + inflow
+  fillps(c, dtrk);
+  solve_pressure(c);
+  run_program(c, c->prog_pp);               // boundp(pp)
+  correc(c, dtrk, true);                    // correc + updt_pressure
+  run_program(c, c->prog_uvw[c->cur][1]);   // bounduvw(correc) + inflow(correc)
+  run_program(c, c->prog_p);                // boundp(p)
+}
+
+double *field_ptr(snac_ctx *c, int b, int fid, long &count) {
+  DevBlock &D = c->blk.at(b);
+  if (fid >= 0 && fid < 6) count = D.ntot;
+  else count = D.nin;
+  switch (fid) {
+  case SNAC_U: return D.vel[c->cur][0];
+  case SNAC_V: return D.vel[c->cur][1];
+  case SNAC_W: return D.vel[c->cur][2];
+  case SNAC_P: return D.p;
+  case SNAC_PP: return D.pp;
+  case SNAC_PO: return D.po;
+  case SNAC_DUDTRKO: return D.dto[0];
+  case SNAC_DVDTRKO: return D.dto[1];
+  case SNAC_DWDTRKO: return D.dto[2];
+  default: throw std::runtime_error("invalid field id " + std::to_string(fid));
+  }
+}
+
+void reduce(snac_ctx *c, bool div, double out[2]) {
+  for (auto &D : c->blk) {
+    if (div)
+      launch_chkdiv_part(c->stream, D.g, D.m, D.vel[c->cur][0], D.vel[c->cur][1], D.vel[c->cur][2], c->cs.vol_all,
+                         c->part_sum + D.part_off, c->part_max + D.part_off);
+    else
+      launch_chkdt_part(c->stream, D.g, D.m, D.vel[c->cur][0], D.vel[c->cur][1], D.vel[c->cur][2],
+                        c->part_max + D.part_off);
+  }
+  launch_reduce_final(c->stream, div ? c->part_sum : nullptr, c->part_max, c->nparts, c->red);
+  SNAC_HIP(hipMemcpyAsync(out, c->red, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  SNAC_HIP(hipStreamSynchronize(c->stream));
+}
+
+void create(snac_ctx *c) {
+  if (c->device >= 0) {
+    SNAC_HIP(hipSetDevice(c->device));
+    SNAC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  }
+  const Case &cs = c->cs;
+  c->blk.resize(cs.blocks.size());
+  c->dlmin = 1e300;
+  for (size_t b = 0; b < cs.blocks.size(); b++) {
+    const Block &B = cs.blocks[b];
+    DevBlock &D = c->blk[b];
+    D.g.nx = B.n[0]; D.g.ny = B.n[1]; D.g.nz = B.n[2];
+    D.g.sx = B.n[0] + 2; D.g.sy = B.n[1] + 2;
+    D.ntot = D.g.sx * D.g.sy * (long)(B.n[2] + 2);
+    D.nin = (long)B.n[0] * B.n[1] * B.n[2];
+    for (int a = 0; a < 3; a++) {
+      D.met[2 * a] = dupload(c, B.dc[a]);
+      D.met[2 * a + 1] = dupload(c, B.df[a]);
+      for (double v : B.df[a]) c->dlmin = std::min(c->dlmin, v);
+    }
+    D.m = Metrics{D.met[0], D.met[1], D.met[2], D.met[3], D.met[4], D.met[5]};
+    for (int par = 0; par < 2; par++)
+      for (int q = 0; q < 3; q++) D.vel[par][q] = dalloc<double>(c, D.ntot);
+    D.p = dalloc<double>(c, D.ntot);
+    D.pp = dalloc<double>(c, D.ntot);
+    D.po = dalloc<double>(c, D.ntot);
+    for (int q = 0; q < 3; q++) D.dto[q] = dalloc<double>(c, D.nin);
+    for (int a = 0; a < 3; a++)
+      for (int s = 0; s < 2; s++) {
+        D.rhsp[2 * a + s] = B.rhsp[a][s].empty() ? nullptr : dupload(c, B.rhsp[a][s]);
+        for (int q = 0; q < 3; q++) D.vin[a][q][s] = B.vin[a][q][s].empty() ? nullptr : dupload(c, B.vin[a][q][s]);
+      }
+    D.part_off = c->nparts;
+    D.nparts = reduce_nparts(D.g);
+    c->nparts += D.nparts;
+  }
+  c->part_sum = dalloc<double>(c, c->nparts);
+  c->part_max = dalloc<double>(c, c->nparts);
+  c->red = dalloc<double>(c, 2);
+  auto arrs = block_arrays(c);
+  for (int par = 0; par < 2; par++)
+    for (int cr = 0; cr < 2; cr++) {
+      c->prog_uvw[par][cr] = upload_program(c, build_velocity_program(cs, arrs, par, cr, true, true));
+      c->prog_bound_only[par][cr] = upload_program(c, build_velocity_program(cs, arrs, par, cr, true, false));
+      c->prog_inflow[par][cr] = upload_program(c, build_velocity_program(cs, arrs, par, cr, false, true));
+    }
+  c->prog_p = upload_program(c, build_pressure_program(cs, arrs, 0));
+  c->prog_pp = upload_program(c, build_pressure_program(cs, arrs, 1));
+  plan_tensor(c);
+  if (c->device >= 0) SNAC_HIP(hipDeviceSynchronize());
+}
+
+void destroy(snac_ctx *c) {
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void *p : c->owned) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+template <class F>
+int guard(F &&f) {
+  try {
+    g_err.clear();
+    f();
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  } catch (...) {
+    g_err = "unknown error";
+    return -1;
+  }
+}
+
+snac_ctx *checked(snac_ctx *c) {
+  if (!c) throw std::runtime_error("null context");
+  return c;
+}
+
+// compute entry points need a device context
+snac_ctx *on_device(snac_ctx *c) {
+  checked(c);
+  if (c->device < 0) throw std::runtime_error("host-only context (device < 0): set-up and queries only");
+  return c;
+}
+
+}  // namespace
+}  // namespace snac
+
+using namespace snac;
+
+extern "C" {
+
+const char *snac_last_error(void) { return g_err.c_str(); }
+
+int snac_ctx_create_from_nml(snac_ctx **out, const char *dns, const char *blocks, int fft_axis, int device) {
+  return guard([&] {
+    if (!out || !dns || !blocks) throw std::runtime_error("null argument");
+    *out = nullptr;
+    std::unique_ptr<snac_ctx> c(new snac_ctx());
+    c->cs = build_case(dns, blocks, fft_axis);
+    c->device = device;
+    try {
+      create(c.get());
+    } catch (...) {
+      destroy(c.get());
+      throw;
+    }
+    *out = c.release();
+  });
+}
+
+int snac_ctx_destroy(snac_ctx *c) {
+  return guard([&] {
+    if (!c) return;
+    destroy(c);
+    delete c;
+  });
+}
+
+int snac_ctx_num_blocks(const snac_ctx *c) { return c ? (int)c->cs.blocks.size() : -1; }
+
+int snac_ctx_block_info(const snac_ctx *c, int b, int *n3, int *lo3, int *nb6, int *per3) {
+  return guard([&] {
+    const Block &B = checked(const_cast<snac_ctx *>(c))->cs.blocks.at(b);
+    for (int a = 0; a < 3; a++) {
+      if (n3) n3[a] = B.n[a];
+      if (lo3) lo3[a] = B.lo_g[a];
+      if (nb6) { nb6[2 * a] = B.nb[a][0]; nb6[2 * a + 1] = B.nb[a][1]; }
+      if (per3) per3[a] = c->cs.periods[a];
+    }
+  });
+}
+
+int snac_ctx_get_metric(const snac_ctx *c, int b, int which, double *out) {
+  return guard([&] {
+    const Block &B = checked(const_cast<snac_ctx *>(c))->cs.blocks.at(b);
+    if (which < 0 || which > 11) throw std::runtime_error("invalid metric id");
+    int a = (which % 6) / 2;
+    const std::vector<double> &v = which < 6 ? ((which % 2) ? B.df[a] : B.dc[a]) : ((which % 2) ? B.rf[a] : B.rc[a]);
+    std::memcpy(out, v.data(), v.size() * sizeof(double));
+  });
+}
+
+int snac_ctx_get_params(const snac_ctx *c, double *dp, int *ip) {
+  return guard([&] {
+    const Case &s = checked(const_cast<snac_ctx *>(c))->cs;
+    if (dp) {
+      double v[11] = {s.visc, s.cfl, s.dtmax, s.dt_f, s.bforce[0], s.bforce[1], s.bforce[2], s.vol_all, s.time_max,
+                      s.tw_max, s.hypre_tol};
+      std::memcpy(dp, v, sizeof(v));
+    }
+    if (ip) {
+      int v[8] = {s.nstep, s.icheck, s.nscal, s.hypre_maxiter, (int)s.restart, (int)s.stop_type[0],
+                  (int)s.stop_type[1], (int)s.stop_type[2]};
+      std::memcpy(ip, v, sizeof(v));
+    }
+  });
+}
+
+int snac_ctx_set_solver(snac_ctx *c, int solver) {
+  return guard([&] {
+    checked(c);
+    if (solver != SNAC_SOLVER_AUTO && solver != SNAC_SOLVER_TENSOR) throw std::runtime_error("unknown solver id");
+    c->solver = solver;
+    check_solver(c);
+  });
+}
+
+int snac_upload_field(snac_ctx *c, int b, int fid, const double *h) {
+  return guard([&] {
+    long n;
+    double *d = field_ptr(on_device(c), b, fid, n);
+    SNAC_HIP(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    SNAC_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int snac_download_field(snac_ctx *c, int b, int fid, double *h) {
+  return guard([&] {
+    long n;
+    double *d = field_ptr(on_device(c), b, fid, n);
+    SNAC_HIP(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SNAC_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int snac_rk_mom(snac_ctx *c, int irk, double dt) {
+  return guard([&] {
+    on_device(c);
+    if (irk < 1 || irk > 3) throw std::runtime_error("irk must be 1..3");
+    // standalone call: carry the current ghost layer over, as the in-place
+    // reference update leaves it untouched
+    int nx = 1 - c->cur;
+    for (auto &D : c->blk)
+      for (int q = 0; q < 3; q++)
+        SNAC_HIP(hipMemcpyAsync(D.vel[nx][q], D.vel[c->cur][q], D.ntot * sizeof(double), hipMemcpyDeviceToDevice,
+                                c->stream));
+    rk_mom(c, irk, dt);
+  });
+}
+
+int snac_bounduvw(snac_ctx *c, int is_correc) {
+  return guard([&] { run_program(on_device(c), c->prog_bound_only[c->cur][is_correc ? 1 : 0]); });
+}
+
+int snac_inflow(snac_ctx *c, int is_correc) {
+  return guard([&] { run_program(on_device(c), c->prog_inflow[c->cur][is_correc ? 1 : 0]); });
+}
+
+int snac_boundp(snac_ctx *c, int fid) {
+  return guard([&] {
+    on_device(c);
+    if (fid == SNAC_P) run_program(c, c->prog_p);
+    else if (fid == SNAC_PP) run_program(c, c->prog_pp);
+    else throw std::runtime_error("boundp: field must be SNAC_P or SNAC_PP");
+  });
+}
+
+int snac_fillps(snac_ctx *c, double dtrk) { return guard([&] { fillps(on_device(c), dtrk); }); }
+
+int snac_solve_pressure(snac_ctx *c) { return guard([&] { solve_pressure(on_device(c)); }); }
+
+int snac_correc(snac_ctx *c, double dtrk) { return guard([&] { correc(on_device(c), dtrk, false); }); }
+
+int snac_updt_pressure(snac_ctx *c) {
+  return guard([&] {
+    for (auto &D : on_device(c)->blk) launch_updt_pressure(c->stream, D.g, D.pp, D.p);
+  });
+}
+
+int snac_chkdiv(snac_ctx *c, double *divtot, double *divmax) {
+  return guard([&] {
+    double r[2];
+    reduce(on_device(c), true, r);
+    if (divtot) *divtot = r[0];
+    if (divmax) *divmax = r[1];
+  });
+}
+
+int snac_chkdt(snac_ctx *c, double *dtmax) {
+  return guard([&] {
+    double r[2];
+    reduce(on_device(c), false, r);
+    double dti = r[1] == 0.0 ? 1.0 : r[1];
+    double v = std::sqrt(3.0) / dti;
+    v = std::min(v, 1.65 / 12.0 / c->cs.visc * c->dlmin * c->dlmin);
+    if (dtmax) *dtmax = v;
+  });
+}
+
+int snac_init_bounds(snac_ctx *c) {
+  return guard([&] {
+    run_program(on_device(c), c->prog_bound_only[c->cur][0]);
+    run_program(c, c->prog_p);
+    run_program(c, c->prog_inflow[c->cur][0]);
+  });
+}
+
+int snac_step(snac_ctx *c, double dt) {
+  return guard([&] {
+    on_device(c);
+    check_solver(c);
+    for (int irk = 1; irk <= 3; irk++) substage(c, irk, dt);
+  });
+}
+
+int snac_steps(snac_ctx *c, double dt, int nsteps) {
+  return guard([&] {
+    on_device(c);
+    check_solver(c);
+    for (int s = 0; s < nsteps; s++)
+      for (int irk = 1; irk <= 3; irk++) substage(c, irk, dt);
+  });
+}
+
+int snac_sync(snac_ctx *c) { return guard([&] { SNAC_HIP(hipStreamSynchronize(on_device(c)->stream)); }); }
+
+int snac_r2r_lines(snac_ctx *c, int kind, int n, int howmany, const double *in, double *out) {
+  return guard([&] {
+    on_device(c);
+    R2RPlan pl = r2r_plan(kind, n);
+    size_t tot = (size_t)n * howmany;
+    double *din = nullptr, *dout = nullptr;
+    SNAC_HIP(hipMalloc(&din, tot * sizeof(double)));
+    SNAC_HIP(hipMalloc(&dout, tot * sizeof(double)));
+    SNAC_HIP(hipMemcpyAsync(din, in, tot * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    launch_r2r(c->stream, pl, din, 1, n, 0, dout, 1, n, 0, howmany, 1, 1.0);
+    SNAC_HIP(hipMemcpyAsync(out, dout, tot * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SNAC_HIP(hipStreamSynchronize(c->stream));
+    SNAC_HIP(hipFree(din));
+    SNAC_HIP(hipFree(dout));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,301 @@
+// Volume kernels of one RK3 substage: the fused momentum RHS + low-storage
+// RK3 update (rk.f90:7-128, mom.f90:112-386), fillps (fillps.f90:7-54) with
+// updt_rhs (bound.f90:543-585), correc (correc.f90:7-48) with updt_pressure
+// (updt_pressure.f90:23-29), and the chkdiv / chkdt reductions
+// (chkdiv.f90:8-44, chkdt.f90:8-56).  FP64 throughout; one thread per cell,
+// 64-wide rows along i so that every wavefront reads whole 512-byte rows.
+#include "common.hpp"
+
+namespace snac {
+namespace {
+
+constexpr int TX = 64, TY = 4;
+
+__device__ __forceinline__ long gidx(long i, long j, long k, long sx, long sy) { return i + sx * (j + sy * k); }
+
+__global__ __launch_bounds__(TX *TY) void k_rk_mom(Geo g, Metrics m, const double *__restrict__ u,
+                                                    const double *__restrict__ v, const double *__restrict__ w,
+                                                    const double *__restrict__ p, double *__restrict__ un,
+                                                    double *__restrict__ vn, double *__restrict__ wn,
+                                                    double *__restrict__ dtu, double *__restrict__ dtv,
+                                                    double *__restrict__ dtw, double f1, double f2, double f12,
+                                                    double visc, double bfx, double bfy, double bfz) {
+  const int i = blockIdx.x * TX + threadIdx.x + 1;
+  const int j = blockIdx.y * TY + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  if (i > g.nx || j > g.ny) return;
+  const long sx = g.sx, sy = g.sy;
+  const long c = gidx(i, j, k, sx, sy);
+  const long xp = c + 1, xm = c - 1, yp = c + sx, ym = c - sx, zp = c + sx * sy, zm = c - sx * sy;
+  const double dxci = m.dxc[i], dxcim = m.dxc[i - 1], dxfi = m.dxf[i], dxfip = m.dxf[i + 1];
+  const double dycj = m.dyc[j], dycjm = m.dyc[j - 1], dyfj = m.dyf[j], dyfjp = m.dyf[j + 1];
+  const double dzck = m.dzc[k], dzckm = m.dzc[k - 1], dzfk = m.dzf[k], dzfkp = m.dzf[k + 1];
+  const double uc = u[c], vc = v[c], wc = w[c];
+  double a, b, cc;
+  // diffusion, momx_d / momy_d / momz_d (mom.f90:221-327)
+  {
+    double dudxp = (u[xp] - uc) / dxfip, dudxm = (uc - u[xm]) / dxfi;
+    double dudyp = (u[yp] - uc) / dycj, dudym = (uc - u[ym]) / dycjm;
+    double dudzp = (u[zp] - uc) / dzck, dudzm = (uc - u[zm]) / dzckm;
+    a = visc * (dudxp - dudxm) / dxci + visc * (dudyp - dudym) / dyfj + visc * (dudzp - dudzm) / dzfk;
+    double dvdxp = (v[xp] - vc) / dxci, dvdxm = (vc - v[xm]) / dxcim;
+    double dvdyp = (v[yp] - vc) / dyfjp, dvdym = (vc - v[ym]) / dyfj;
+    double dvdzp = (v[zp] - vc) / dzck, dvdzm = (vc - v[zm]) / dzckm;
+    b = visc * (dvdxp - dvdxm) / dxfi + visc * (dvdyp - dvdym) / dycj + visc * (dvdzp - dvdzm) / dzfk;
+    double dwdxp = (w[xp] - wc) / dxci, dwdxm = (wc - w[xm]) / dxcim;
+    double dwdyp = (w[yp] - wc) / dycj, dwdym = (wc - w[ym]) / dycjm;
+    double dwdzp = (w[zp] - wc) / dzfkp, dwdzm = (wc - w[zm]) / dzfk;
+    cc = visc * (dwdxp - dwdxm) / dxfi + visc * (dwdyp - dwdym) / dyfj + visc * (dwdzp - dwdzm) / dzck;
+  }
+  // volume-weighted advection, momx/y/z_a_vv (mom.f90:112-219)
+  {
+    double uuip = 0.25 * (uc * dyfj * dzfk + u[xp] * dyfj * dzfk) * u[xp];
+    double uuim = 0.25 * (uc * dyfj * dzfk + u[xm] * dyfj * dzfk) * u[xm];
+    double uvjp = 0.25 * (vc * dxfi * dzfk + v[xp] * dxfip * dzfk) * u[yp];
+    double uvjm = 0.25 * (v[ym] * dxfi * dzfk + v[ym + 1] * dxfip * dzfk) * u[ym];
+    double uwkp = 0.25 * (wc * dxfi * dyfj + w[xp] * dxfip * dyfj) * u[zp];
+    double uwkm = 0.25 * (w[zm] * dxfi * dyfj + w[zm + 1] * dxfip * dyfj) * u[zm];
+    a = a + ((-uuip + uuim) + (-uvjp + uvjm) + (-uwkp + uwkm)) / (dxci * dyfj * dzfk);
+    double uvip = 0.25 * (uc * dyfj * dzfk + u[yp] * dyfjp * dzfk) * v[xp];
+    double uvim = 0.25 * (u[xm] * dyfj * dzfk + u[xm + sx] * dyfjp * dzfk) * v[xm];
+    double vvjp = 0.25 * (vc * dxfi * dzfk + v[yp] * dxfi * dzfk) * v[yp];
+    double vvjm = 0.25 * (vc * dxfi * dzfk + v[ym] * dxfi * dzfk) * v[ym];
+    double wvkp = 0.25 * (wc * dyfj * dxfi + w[yp] * dyfjp * dxfi) * v[zp];
+    double wvkm = 0.25 * (w[zm] * dyfj * dxfi + w[zm + sx] * dyfjp * dxfi) * v[zm];
+    b = b + ((-uvip + uvim) + (-vvjp + vvjm) + (-wvkp + wvkm)) / (dxfi * dycj * dzfk);
+    double uwip = 0.25 * (uc * dzfk * dyfj + u[zp] * dzfkp * dyfj) * w[xp];
+    double uwim = 0.25 * (u[xm] * dzfk * dyfj + u[xm + sx * sy] * dzfkp * dyfj) * w[xm];
+    double vwjp = 0.25 * (vc * dzfk * dxfi + v[zp] * dzfkp * dxfi) * w[yp];
+    double vwjm = 0.25 * (v[ym] * dzfk * dxfi + v[ym + sx * sy] * dzfkp * dxfi) * w[ym];
+    double wwkp = 0.25 * (wc * dxfi * dyfj + w[zp] * dxfi * dyfj) * w[zp];
+    double wwkm = 0.25 * (wc * dxfi * dyfj + w[zm] * dxfi * dyfj) * w[zm];
+    cc = cc + ((-uwip + uwim) + (-vwjp + vwjm) + (-wwkp + wwkm)) / (dxfi * dyfj * dzck);
+  }
+  // RK3 update (rk.f90:68-85), then pressure gradient + body force (86-111)
+  const long q = (long)(i - 1) + (long)g.nx * ((long)(j - 1) + (long)g.ny * (long)(k - 1));
+  double un_ = uc + f1 * a + f2 * dtu[q];
+  double vn_ = vc + f1 * b + f2 * dtv[q];
+  double wn_ = wc + f1 * cc + f2 * dtw[q];
+  dtu[q] = a;
+  dtv[q] = b;
+  dtw[q] = cc;
+  const double pc = p[c];
+  un_ = un_ + f12 * (-(p[xp] - pc) / dxci + bfx);
+  vn_ = vn_ + f12 * (-(p[yp] - pc) / dycj + bfy);
+  wn_ = wn_ + f12 * (-(p[zp] - pc) / dzck + bfz);
+  un[c] = un_;
+  vn[c] = vn_;
+  wn[c] = wn_;
+}
+
+__global__ __launch_bounds__(TX *TY) void k_fillps(Geo g, Metrics m, int fa, const double *__restrict__ u,
+                                                    const double *__restrict__ v, const double *__restrict__ w,
+                                                    double *__restrict__ pp, double dtrk, const double *rx0,
+                                                    const double *rx1, const double *ry0, const double *ry1,
+                                                    const double *rz0, const double *rz1) {
+  const int i = blockIdx.x * TX + threadIdx.x + 1;
+  const int j = blockIdx.y * TY + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  if (i > g.nx || j > g.ny) return;
+  const long sx = g.sx, sy = g.sy;
+  const long c = gidx(i, j, k, sx, sy);
+  const double dw = w[c] - w[c - sx * sy], dv = v[c] - v[c - sx], du = u[c] - u[c - 1];
+  const double dxf = m.dxf[i], dyf = m.dyf[j], dzf = m.dzf[k];
+  double s;
+  switch (fa) {
+  case 3: s = dw / dzf * dxf * dyf + dv * dxf + du * dyf; break;
+  case 2: s = dw * dxf + dv / dyf * dxf * dzf + du * dzf; break;
+  case 1: s = dw * dyf + dv * dzf + du / dxf * dyf * dzf; break;
+  default: s = dw * dxf * dyf + dv * dxf * dzf + du * dyf * dzf; break;
+  }
+  double r = s / dtrk;
+  // updt_rhs, in the reference's order x0,x1,y0,y1,z0,z1
+  if (rx0 && i == 1) r += rx0[(j - 1) + (long)g.ny * (k - 1)];
+  if (rx1 && i == g.nx) r += rx1[(j - 1) + (long)g.ny * (k - 1)];
+  if (ry0 && j == 1) r += ry0[(i - 1) + (long)g.nx * (k - 1)];
+  if (ry1 && j == g.ny) r += ry1[(i - 1) + (long)g.nx * (k - 1)];
+  if (rz0 && k == 1) r += rz0[(i - 1) + (long)g.nx * (j - 1)];
+  if (rz1 && k == g.nz) r += rz1[(i - 1) + (long)g.nx * (j - 1)];
+  pp[c] = r;
+}
+
+// correc over the whole ghosted box; cells outside the reference's loop
+// ranges (i <= nx for u, j <= ny for v, k <= nz for w) are copied
+__global__ __launch_bounds__(TX *TY) void k_correc(Geo g, Metrics m, const double *__restrict__ pp,
+                                                    const double *__restrict__ u, const double *__restrict__ v,
+                                                    const double *__restrict__ w, double *__restrict__ un,
+                                                    double *__restrict__ vn, double *__restrict__ wn, double dtrk,
+                                                    double *__restrict__ p) {
+  const int i = blockIdx.x * TX + threadIdx.x;
+  const int j = blockIdx.y * TY + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i > g.nx + 1 || j > g.ny + 1) return;
+  const long sx = g.sx, sy = g.sy;
+  const long c = gidx(i, j, k, sx, sy);
+  const double pc = pp[c];
+  double uu = u[c], vv = v[c], ww = w[c];
+  if (i <= g.nx) uu = uu - dtrk * (pp[c + 1] - pc) / m.dxc[i];
+  if (j <= g.ny) vv = vv - dtrk * (pp[c + sx] - pc) / m.dyc[j];
+  if (k <= g.nz) ww = ww - dtrk * (pp[c + sx * sy] - pc) / m.dzc[k];
+  un[c] = uu;
+  vn[c] = vv;
+  wn[c] = ww;
+  if (p && i >= 1 && i <= g.nx && j >= 1 && j <= g.ny && k >= 1 && k <= g.nz) p[c] = p[c] + pc;
+}
+
+__global__ __launch_bounds__(TX *TY) void k_updt_pressure(Geo g, const double *__restrict__ pp,
+                                                           double *__restrict__ p) {
+  const int i = blockIdx.x * TX + threadIdx.x + 1;
+  const int j = blockIdx.y * TY + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  if (i > g.nx || j > g.ny) return;
+  const long c = gidx(i, j, k, g.sx, g.sy);
+  p[c] = p[c] + pp[c];
+}
+
+// ---- reductions ----
+__device__ __forceinline__ double wave_sum(double x) {
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+__device__ __forceinline__ double wave_max(double x) {
+  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+  return x;
+}
+
+__global__ __launch_bounds__(TX *TY) void k_chkdiv(Geo g, Metrics m, const double *__restrict__ u,
+                                                    const double *__restrict__ v, const double *__restrict__ w,
+                                                    double vol, double *psum, double *pmax) {
+  __shared__ double ss[TX * TY / 64], sm[TX * TY / 64];
+  const int i = blockIdx.x * TX + threadIdx.x + 1;
+  const int j = blockIdx.y * TY + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  double s = 0.0, mx = 0.0;
+  if (i <= g.nx && j <= g.ny) {
+    const long sx = g.sx, sy = g.sy;
+    const long c = gidx(i, j, k, sx, sy);
+    double div = (w[c] - w[c - sx * sy]) / m.dzf[k] + (v[c] - v[c - sx]) / m.dyf[j] + (u[c] - u[c - 1]) / m.dxf[i];
+    mx = fabs(div);
+    s = div * m.dxf[i] * m.dyf[j] * m.dzf[k] / vol;
+  }
+  s = wave_sum(s);
+  mx = wave_max(mx);
+  const int t = threadIdx.x + TX * threadIdx.y, wv = t / 64;
+  if ((t & 63) == 0) { ss[wv] = s; sm[wv] = mx; }
+  __syncthreads();
+  if (t == 0) {
+    double a = 0.0, b = 0.0;
+    for (int q = 0; q < TX * TY / 64; q++) { a += ss[q]; b = fmax(b, sm[q]); }
+    const long part = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+    psum[part] = a;
+    pmax[part] = b;
+  }
+}
+
+__global__ __launch_bounds__(TX *TY) void k_chkdt(Geo g, Metrics m, const double *__restrict__ u,
+                                                   const double *__restrict__ v, const double *__restrict__ w,
+                                                   double *pmax) {
+  __shared__ double sm[TX * TY / 64];
+  const int i = blockIdx.x * TX + threadIdx.x + 1;
+  const int j = blockIdx.y * TY + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  double dti = 0.0;
+  if (i <= g.nx && j <= g.ny) {
+    const long sx = g.sx, sy = g.sy, sz = sx * sy;
+    const long c = gidx(i, j, k, sx, sy);
+    double ux = fabs(u[c]);
+    double vx = 0.25 * fabs(v[c] + v[c - sx] + v[c + 1] + v[c + 1 - sx]);
+    double wx = 0.25 * fabs(w[c] + w[c - sz] + w[c + 1] + w[c + 1 - sz]);
+    double dtix = ux / m.dxc[i] + vx / m.dyf[j] + wx / m.dzf[k];
+    double uy = 0.25 * fabs(u[c] + u[c + sx] + u[c - 1 + sx] + u[c - 1]);
+    double vy = fabs(v[c]);
+    double wy = 0.25 * fabs(w[c] + w[c + sx] + w[c + sx - sz] + w[c - sz]);
+    double dtiy = uy / m.dxf[i] + vy / m.dyc[j] + wy / m.dzf[k];
+    double uz = 0.25 * fabs(u[c] + u[c - 1] + u[c - 1 + sz] + u[c + sz]);
+    double vz = 0.25 * fabs(v[c] + v[c - sx] + v[c - sx + sz] + v[c + sz]);
+    double wz = fabs(w[c]);
+    double dtiz = uz / m.dxf[i] + vz / m.dyf[j] + wz / m.dzc[k];
+    dti = fmax(dtix, fmax(dtiy, dtiz));
+  }
+  dti = wave_max(dti);
+  const int t = threadIdx.x + TX * threadIdx.y, wv = t / 64;
+  if ((t & 63) == 0) sm[wv] = dti;
+  __syncthreads();
+  if (t == 0) {
+    double b = 0.0;
+    for (int q = 0; q < TX * TY / 64; q++) b = fmax(b, sm[q]);
+    pmax[blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z)] = b;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_final(const double *psum, const double *pmax, int n, double *out) {
+  __shared__ double ss[4], sm[4];
+  double s = 0.0, mx = 0.0;
+  for (int q = threadIdx.x; q < n; q += 256) {
+    if (psum) s += psum[q];
+    if (pmax) mx = fmax(mx, pmax[q]);
+  }
+  s = wave_sum(s);
+  mx = wave_max(mx);
+  if ((threadIdx.x & 63) == 0) { ss[threadIdx.x / 64] = s; sm[threadIdx.x / 64] = mx; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[0] = ss[0] + ss[1] + ss[2] + ss[3];
+    out[1] = fmax(fmax(sm[0], sm[1]), fmax(sm[2], sm[3]));
+  }
+}
+
+dim3 interior_grid(const Geo &g) { return dim3((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, g.nz); }
+dim3 ghosted_grid(const Geo &g) { return dim3((g.nx + 2 + TX - 1) / TX, (g.ny + 2 + TY - 1) / TY, g.nz + 2); }
+
+}  // namespace
+
+void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v, const double *w,
+                   const double *p, double *un, double *vn, double *wn, double *dtu, double *dtv, double *dtw,
+                   double f1, double f2, double f12, double visc, double bfx, double bfy, double bfz) {
+  hipLaunchKernelGGL(k_rk_mom, interior_grid(g), dim3(TX, TY), 0, s, g, m, u, v, w, p, un, vn, wn, dtu, dtv, dtw,
+                     f1, f2, f12, visc, bfx, bfy, bfz);
+  SNAC_HIP(hipGetLastError());
+}
+
+void launch_fillps(hipStream_t s, const Geo &g, const Metrics &m, int fa, const double *u, const double *v,
+                   const double *w, double *pp, double dtrk, const double *const r[6]) {
+  hipLaunchKernelGGL(k_fillps, interior_grid(g), dim3(TX, TY), 0, s, g, m, fa, u, v, w, pp, dtrk, r[0], r[1], r[2],
+                     r[3], r[4], r[5]);
+  SNAC_HIP(hipGetLastError());
+}
+
+void launch_correc(hipStream_t s, const Geo &g, const Metrics &m, const double *pp, const double *u, const double *v,
+                   const double *w, double *un, double *vn, double *wn, double dtrk, double *p) {
+  hipLaunchKernelGGL(k_correc, ghosted_grid(g), dim3(TX, TY), 0, s, g, m, pp, u, v, w, un, vn, wn, dtrk, p);
+  SNAC_HIP(hipGetLastError());
+}
+
+void launch_updt_pressure(hipStream_t s, const Geo &g, const double *pp, double *p) {
+  hipLaunchKernelGGL(k_updt_pressure, interior_grid(g), dim3(TX, TY), 0, s, g, pp, p);
+  SNAC_HIP(hipGetLastError());
+}
+
+int reduce_nparts(const Geo &g) {
+  dim3 gr = interior_grid(g);
+  return (int)(gr.x * gr.y * gr.z);
+}
+
+void launch_chkdiv_part(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v,
+                        const double *w, double vol, double *psum, double *pmax) {
+  hipLaunchKernelGGL(k_chkdiv, interior_grid(g), dim3(TX, TY), 0, s, g, m, u, v, w, vol, psum, pmax);
+  SNAC_HIP(hipGetLastError());
+}
+
+void launch_chkdt_part(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v,
+                       const double *w, double *pmax) {
+  hipLaunchKernelGGL(k_chkdt, interior_grid(g), dim3(TX, TY), 0, s, g, m, u, v, w, pmax);
+  SNAC_HIP(hipGetLastError());
+}
+
+void launch_reduce_final(hipStream_t s, const double *psum, const double *pmax, int n, double *out) {
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, psum, pmax, n, out);
+  SNAC_HIP(hipGetLastError());
+}
+
+}  // namespace snac

@@ -1,0 +1,107 @@
+"""Case definitions for the tests and the benchmark, written as the
+reference's own input files (dns.nml / blocks.nml, param.f90:88-106), so
+that the product reads exactly what the reference would.
+
+channel(): the periodic channel of the benchmark configuration (x periodic,
+y wall-normal tanh-stretched, z periodic and the FFT axis), any size.
+cavity(): the 3D lid-driven cavity (all walls, lid u=1 on z+, no FFT).
+"""
+import os
+
+import numpy as np
+
+
+def _fmt(v):
+    if isinstance(v, bool):
+        return "T" if v else "F"
+    if isinstance(v, str):
+        return "'" + v + "'"
+    if isinstance(v, (int, np.integer)):
+        return str(int(v))
+    return repr(float(v))
+
+
+def write_dns(path, visci, nstep=10, dt_f=-1.0, cfl=0.5, dtmax=1e5, icheck=10, bforce=(0, 0, 0)):
+    txt = ("&dns\n"
+           f"cfl = {_fmt(cfl)}, dtmax = {_fmt(dtmax)}, dt_f = {_fmt(dt_f)}\n"
+           f"visci = {_fmt(visci)}\n"
+           f"nstep = {nstep}, time_max = 100., tw_max = 0.1\n"
+           "stop_type(1:3) = T, F, F\n"
+           "restart = F, is_overwrite_save = T, nsaves_max = 0\n"
+           f"icheck = {icheck}, iout0d = 0, iout1d = 0, iout2d = 0, iout3d = 0, isave = 0\n"
+           f"bforce(1:3) = {', '.join(_fmt(b) for b in bforce)}\n"
+           "/\n&hypre\nhypre_solver_i = 2, hypre_tol = 1.e-13, hypre_maxiter = 1000\n/\n")
+    open(path, "w").write(txt)
+
+
+def write_blocks(path, blocks):
+    """blocks: list of dicts with ng, lmin, lmax, gt, gr, cbcvel[comp][dir][side],
+    bcvel, cbcpre[dir][side], bcpre, inflow_type (optional)."""
+    out = [f"&blocks\nnblocks = {len(blocks)}\n"]
+    for b, B in enumerate(blocks, start=1):
+        out.append(f"block_dims(1:3,{b}) = 1, 1, 1\n")
+        for key in ("ng", "lmin", "lmax", "gt", "gr"):
+            out.append(f"block_{key}(1:3,{b}) = {', '.join(_fmt(v) for v in B[key])}\n")
+        for c in range(3):
+            out.append(f"block_cbcvel(0:1,1:3,{c + 1},{b}) = " +
+                       ", ".join(_fmt(B["cbcvel"][c][d][s]) for d in range(3) for s in range(2)) + "\n")
+            out.append(f"block_bcvel(0:1,1:3,{c + 1},{b}) = " +
+                       ", ".join(_fmt(B["bcvel"][c][d][s]) for d in range(3) for s in range(2)) + "\n")
+        out.append(f"block_cbcpre(0:1,1:3,{b}) = " +
+                   ", ".join(_fmt(B["cbcpre"][d][s]) for d in range(3) for s in range(2)) + "\n")
+        out.append(f"block_bcpre(0:1,1:3,{b}) = " +
+                   ", ".join(_fmt(B["bcpre"][d][s]) for d in range(3) for s in range(2)) + "\n")
+        it = B.get("inflow_type", [[0, 0]] * 3)
+        out.append(f"block_inflow_type(0:1,1:3,{b}) = " +
+                   ", ".join(_fmt(it[d][s]) for d in range(3) for s in range(2)) + "\n")
+        out.append(f"block_inivel({b}) = 'zer'\n\n")
+    out.append("/\n")
+    open(path, "w").write("".join(out))
+
+
+def _vel(pattern, values):
+    """pattern: [dir][side] BC chars for all three components, with the
+    wall-normal component sharing them."""
+    return [[[pattern[d][s] for s in range(2)] for d in range(3)] for _ in range(3)], values
+
+
+def channel_block(ng, lengths=(6.0, 1.0, 3.0), gr_y=1.8, friend=1):
+    F = float(friend)
+    cbcv = [[["F", "F"], ["D", "D"], ["F", "F"]] for _ in range(3)]
+    bcv = [[[F, F], [0.0, 0.0], [F, F]] for _ in range(3)]
+    return dict(ng=list(ng), lmin=[0.0, 0.0, 0.0], lmax=list(lengths), gt=[0, 0, 0], gr=[0.0, gr_y, 0.0],
+                cbcvel=cbcv, bcvel=bcv, cbcpre=[["F", "F"], ["N", "N"], ["F", "F"]],
+                bcpre=[[F, F], [0.0, 0.0], [F, F]])
+
+
+def cavity_block(ng, lid=1.0, gr=(0.0, 0.0, 0.0)):
+    cbcv = [[["D", "D"], ["D", "D"], ["D", "D"]] for _ in range(3)]
+    bcv = [[[0.0, 0.0], [0.0, 0.0], [0.0, 0.0]] for _ in range(3)]
+    bcv[0][2][1] = lid  # u = lid on z+
+    return dict(ng=list(ng), lmin=[0.0, 0.0, 0.0], lmax=[1.0, 1.0, 1.0], gt=[0, 0, 0], gr=list(gr),
+                cbcvel=cbcv, bcvel=bcv, cbcpre=[["N", "N"], ["N", "N"], ["N", "N"]],
+                bcpre=[[0.0, 0.0], [0.0, 0.0], [0.0, 0.0]])
+
+
+def write_case(dirpath, blocks, visci, **dns):
+    os.makedirs(dirpath, exist_ok=True)
+    d = os.path.join(dirpath, "dns.nml")
+    b = os.path.join(dirpath, "blocks.nml")
+    write_dns(d, visci, **dns)
+    write_blocks(b, blocks)
+    return d, b
+
+
+def channel_ic(shape_u, yc, seed=20251202, amp=0.1):
+    """Laminar Poiseuille u = 6 y (1-y) at the cell centres plus seeded uniform
+    +-amp noise on u, v, w (interior), p = 0; ghosts are filled by the boundary
+    routines afterwards."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = (s - 2 for s in shape_u)
+    u = np.zeros(shape_u); v = np.zeros(shape_u); w = np.zeros(shape_u)
+    prof = 6.0 * yc[1:-1] * (1.0 - yc[1:-1])
+    u[1:-1, 1:-1, 1:-1] = prof[None, :, None]
+    u[1:-1, 1:-1, 1:-1] += amp * rng.uniform(-1, 1, (nx, ny, nz))
+    v[1:-1, 1:-1, 1:-1] = amp * rng.uniform(-1, 1, (nx, ny, nz))
+    w[1:-1, 1:-1, 1:-1] = amp * rng.uniform(-1, 1, (nx, ny, nz))
+    return u, v, w

@@ -1,0 +1,229 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU oracle.
+
+Per-routine checks start both sides from identical seeded fields (ghosts
+included) and compare every value the routine writes.  Whole-step checks run
+10 RK3 steps and compare every field with the relative L2 gate 1e-10 that
+the benchmark contract states for FP64; p is compared modulo its mean
+(singular all-Neumann/periodic Poisson problems fix p only up to a constant,
+and the reference never removes it, updt_pressure.f90:23-29).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import cases
+import nmlparse
+import oracle_py
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+FIELDS3 = ("u", "v", "w")
+
+
+def make_pair(tmp_path, blocks, fft_axis, visci=5600.0, solver="band", **dns):
+    import snac_amd
+    d, b = cases.write_case(str(tmp_path), blocks, visci, **dns)
+    print(f"case {tmp_path.name}: fft_axis={fft_axis} blocks={[B['ng'] for B in blocks]}", flush=True)
+    ctx = snac_amd.Context(d, b, fft_axis)
+    oc = oracle_py.OracleCase(nmlparse.load_case(d, b, fft_axis))
+    oc.set_solver(solver)
+    return ctx, oc
+
+
+def seed_fields(ctx, oc, seed=7, names=("u", "v", "w", "p", "pp", "dudtrko", "dvdtrko", "dwdtrko")):
+    rng = np.random.default_rng(seed)
+    for b in range(ctx.nblocks):
+        for f in names:
+            a = rng.uniform(-1.0, 1.0, ctx.shape(b, f))
+            ctx.set(b, f, a)
+            oc.set(b, f, a)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def assert_fields(ctx, oc, names, tol, interior=False, modmean=()):
+    for b in range(ctx.nblocks):
+        for f in names:
+            g, o = ctx.get(b, f), oc.get(b, f)
+            if interior and g.ndim == 3 and f not in ("dudtrko", "dvdtrko", "dwdtrko"):
+                g, o = g[1:-1, 1:-1, 1:-1], o[1:-1, 1:-1, 1:-1]
+            if f in modmean:
+                g, o = g - g.mean(), o - o.mean()
+            r = rel(g, o)
+            assert r <= tol, f"block {b} field {f}: rel L2 {r:.3e} > {tol}"
+
+
+def channel(tmp_path, n=(32, 24, 16)):
+    return make_pair(tmp_path, [cases.channel_block(n)], 3, bforce=(12 / 5600, 0, 0))
+
+
+# ---------------------------------------------------------------- transforms
+@pytest.mark.parametrize("kind", ["R2HC", "HC2R", "REDFT10", "REDFT01", "RODFT10", "RODFT01", "REDFT11", "RODFT11"])
+@pytest.mark.parametrize("n", [2, 4, 6, 16, 64, 256])
+def test_r2r_lines(tmp_path, kind, n):
+    import snac_amd
+    d, b = cases.write_case(str(tmp_path), [cases.channel_block((4, 4, 4))], 5600.0)
+    ctx = snac_amd.Context(d, b, 3)
+    x = np.random.default_rng(n).uniform(-1, 1, (37, n))
+    y = ctx.r2r_lines(kind, x)
+    ref = np.array([oracle_py.r2r(kind, row, direct=True) for row in x])
+    assert np.abs(y - ref).max() <= 1e-13 * n * np.abs(ref).max()
+    ctx.close()
+
+
+# ---------------------------------------------------------------- routines
+def test_rk_mom(tmp_path):
+    ctx, oc = channel(tmp_path)
+    for irk in (1, 2, 3):
+        seed_fields(ctx, oc, seed=irk)
+        ctx.rk_mom(irk, 1.3e-3)
+        oc.rk_mom(irk, 1.3e-3)
+        assert_fields(ctx, oc, FIELDS3 + ("dudtrko", "dvdtrko", "dwdtrko"), 1e-13)
+
+
+@pytest.mark.parametrize("is_correc", [False, True])
+def test_bounduvw_channel(tmp_path, is_correc):
+    ctx, oc = channel(tmp_path)
+    seed_fields(ctx, oc)
+    ctx.bounduvw(is_correc)
+    oc.bounduvw(is_correc)
+    assert_fields(ctx, oc, FIELDS3, 0.0)
+
+
+@pytest.mark.parametrize("is_correc", [False, True])
+def test_bounduvw_multiblock(tmp_path, is_correc):
+    import snac_amd
+    ctx = snac_amd.Context(os.path.join(GOLD, "ldc", "dns.nml"), os.path.join(GOLD, "ldc", "blocks.nml"), 1)
+    oc = oracle_py.OracleCase(nmlparse.load_case(os.path.join(GOLD, "ldc", "dns.nml"),
+                                                 os.path.join(GOLD, "ldc", "blocks.nml"), 1))
+    seed_fields(ctx, oc)
+    ctx.bounduvw(is_correc)
+    oc.bounduvw(is_correc)
+    assert_fields(ctx, oc, FIELDS3, 0.0)
+    ctx.boundp("p")
+    oc.boundp("p")
+    assert_fields(ctx, oc, ("p",), 0.0)
+
+
+def test_boundp(tmp_path):
+    ctx, oc = channel(tmp_path)
+    seed_fields(ctx, oc)
+    for f in ("p", "pp"):
+        ctx.boundp(f)
+        oc.boundp(f)
+    assert_fields(ctx, oc, ("p", "pp"), 0.0)
+
+
+def test_fillps_correc_updt_pressure(tmp_path):
+    ctx, oc = channel(tmp_path)
+    seed_fields(ctx, oc)
+    ctx.fillps(2e-3)
+    oc.fillps(2e-3)
+    assert_fields(ctx, oc, ("pp",), 1e-14, interior=True)
+    seed_fields(ctx, oc, seed=11)
+    ctx.correc(2e-3)
+    oc.correc(2e-3)
+    assert_fields(ctx, oc, FIELDS3, 1e-14)
+    ctx.updt_pressure()
+    oc.updt_pressure()
+    assert_fields(ctx, oc, ("p",), 1e-15)
+
+
+def test_chkdiv_chkdt(tmp_path):
+    ctx, oc = channel(tmp_path)
+    seed_fields(ctx, oc)
+    gt, gm = ctx.chkdiv()
+    ot, om = oc.chkdiv()
+    assert gm == pytest.approx(om, rel=1e-14)
+    assert gt == pytest.approx(ot, rel=1e-10, abs=1e-15)
+    assert ctx.chkdt() == pytest.approx(oc.chkdt(), rel=1e-14)
+
+
+@pytest.mark.parametrize("geom", ["channel", "cavity", "ldc_fft", "ldc_3d"])
+def test_solve_pressure(tmp_path, geom):
+    import snac_amd
+    if geom == "channel":
+        ctx, oc = channel(tmp_path)
+    elif geom == "cavity":
+        ctx, oc = make_pair(tmp_path, [cases.cavity_block((12, 10, 16))], 0, visci=1000.0)
+    else:
+        fa = 1 if geom == "ldc_fft" else 0
+        dn, bn = os.path.join(GOLD, "ldc", "dns.nml"), os.path.join(GOLD, "ldc", "blocks.nml")
+        ctx = snac_amd.Context(dn, bn, fa)
+        oc = oracle_py.OracleCase(nmlparse.load_case(dn, bn, fa))
+        oc.set_solver("band")
+    # a compatible right-hand side: fillps of a random velocity whose ghosts
+    # obey the boundary conditions
+    seed_fields(ctx, oc)
+    ctx.bounduvw(False); oc.bounduvw(False)
+    ctx.fillps(1e-3); oc.fillps(1e-3)
+    ctx.solve_pressure()
+    oc.solve_pressure()
+    assert_fields(ctx, oc, ("pp",), 1e-11, interior=True, modmean=("pp",))
+
+
+# ---------------------------------------------------------------- whole steps
+def run_both(ctx, oc, nsteps):
+    ctx.init_bounds()
+    oc.init_bounds()
+    for _ in range(nsteps):
+        dt = ctx.new_dt()
+        dto = min(ctx.cfl * oc.chkdt(), ctx.dtmax) if ctx.dt_f <= 0 else ctx.dt_f
+        assert dt == pytest.approx(dto, rel=1e-12)
+        ctx.step(dt)
+        oc.step(dt)
+
+
+def test_channel_10_steps(tmp_path):
+    ctx, oc = channel(tmp_path)
+    yc = ctx.metric(0, 8)
+    u, v, w = cases.channel_ic(ctx.shape(0, "u"), yc)
+    for f, a in zip(FIELDS3, (u, v, w)):
+        ctx.set(0, f, a)
+        oc.set(0, f, a)
+    run_both(ctx, oc, 10)
+    assert_fields(ctx, oc, FIELDS3, 1e-10, interior=True)
+    assert_fields(ctx, oc, ("p",), 1e-10, interior=True, modmean=("p",))
+    _, divmax = ctx.chkdiv()
+    assert divmax < 1e-10
+
+
+def test_cavity_10_steps(tmp_path):
+    ctx, oc = make_pair(tmp_path, [cases.cavity_block((16, 16, 16))], 0, visci=1000.0)
+    run_both(ctx, oc, 10)
+    assert_fields(ctx, oc, FIELDS3, 1e-10, interior=True)
+    assert_fields(ctx, oc, ("p",), 1e-10, interior=True, modmean=("p",))
+
+
+def test_ldc_multiblock_10_steps():
+    import snac_amd
+    dn, bn = os.path.join(GOLD, "ldc", "dns.nml"), os.path.join(GOLD, "ldc", "blocks.nml")
+    ctx = snac_amd.Context(dn, bn, 1)
+    oc = oracle_py.OracleCase(nmlparse.load_case(dn, bn, 1))
+    run_both(ctx, oc, 10)
+    assert_fields(ctx, oc, FIELDS3, 1e-10, interior=True)
+
+
+@pytest.mark.parametrize("fft_axis", [1, 0])
+def test_ldc_golden_gpu(fft_axis):
+    """The reference's known-answer test (tests/lid_driven_cavity) through
+    the HIP path: 1500 steps, v-profile rtol 1e-7 / atol 5e-8."""
+    import snac_amd
+    from test_oracle_ldc_golden import ldc_profile
+    dn, bn = os.path.join(GOLD, "ldc", "dns.nml"), os.path.join(GOLD, "ldc", "blocks.nml")
+    ctx = snac_amd.Context(dn, bn, fft_axis)
+    ctx.init_bounds()
+    dt = ctx.new_dt()
+    for istep in range(1, ctx.nstep + 1):
+        ctx.step(dt)
+        if istep % ctx.icheck == 0:
+            dt = ctx.new_dt()
+            divtot, divmax = ctx.chkdiv()
+            assert divmax < 1e-10
+    prof = ldc_profile(lambda b: ctx.get(b, "v"))
+    ref = np.loadtxt(os.path.join(GOLD, "ldc", "data_ldc_re1000.txt"))
+    np.testing.assert_allclose(prof, ref[:, 1], rtol=1e-7, atol=5e-8)
