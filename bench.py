@@ -1,0 +1,155 @@
+#!/usr/bin/env python3
+"""Benchmark of the SNaC time-step hot path on one MI355X.
+
+Workload: the FFT-accelerated periodic channel, plain FP64 -- 256x128x256
+cells (x periodic, y wall-normal tanh-stretched with factor 1.8, z periodic
+and the FFT axis), Re_b = 5600 with the laminar body force bforce(1) = 12/5600,
+laminar Poiseuille profile plus seeded uniform +-10% noise.  A step is one
+full RK3 time step (3 substages: fused momentum RHS + update, ghost fills,
+fillps, r2r transforms along z and x, tridiagonal solves along y, inverse
+transforms, correc + pressure update).  Inputs are resident in HBM before
+the timed region; dt is fixed from the CFL check at the start.
+
+Prints one JSON line: Mcell-steps/s, the per-phase split (Poisson ms/step),
+the roofline of the dominant kernel (fused rk_mom, 104 algorithmic bytes
+per cell) from HIP events on its own stream, and the CPU port's rate on the
+host cores (a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "Mcell-steps/s + Poisson ms/step, learned/mixed vs plain FP64, equal div tol"
+HBM_PEAK_GBS = 8000.0
+RK_MOM_BYTES_PER_CELL = 104  # R u,v,w,p,3x dudtrko + W u,v,w,3x dudtrko, FP64
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "rk_mom_channel_256x128x256_traffic.json")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, nargs=3, default=[256, 128, 256])
+    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
+    ap.add_argument("--calib", action="store_true", help="also run the plain copy kernel (counter calibration)")
+    return ap.parse_args()
+
+
+def channel_setup(tmp, size):
+    import cases
+    d, b = cases.write_case(tmp, [cases.channel_block(size)], 5600.0, cfl=0.5, bforce=(12.0 / 5600.0, 0.0, 0.0))
+    return d, b
+
+
+def cpu_baseline(d, b, size, steps, threads):
+    import cases
+    import nmlparse
+    import oracle_py
+    n = oracle_py.set_threads(threads)
+    oc = oracle_py.OracleCase(nmlparse.load_case(d, b, 3))
+    u, v, w = cases.channel_ic(oc.shape(0, "u"), oc.metric(0, 8))
+    for f, a in (("u", u), ("v", v), ("w", w)):
+        oc.set(0, f, a)
+    oc.init_bounds()
+    dt = 0.5 * oc.chkdt()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        oc.step(dt)
+    el = time.perf_counter() - t0
+    cells = size[0] * size[1] * size[2]
+    return {"value": cells * steps / el / 1e6, "unit": "Mcell-steps/s", "cores": int(n), "kind": "port",
+            "sample": f"{steps} full RK3 steps of the {size[0]}x{size[1]}x{size[2]} channel on the C oracle "
+                      f"(OpenMP, tensor Poisson solver, radix-2 transforms), {el:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is a design, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+
+    import cases
+    import snac_amd
+
+    size = args.size
+    cells = size[0] * size[1] * size[2]
+    with tempfile.TemporaryDirectory() as tmp:
+        d, b = channel_setup(tmp, size)
+        print(f"[bench] channel {size} on device 0", file=sys.stderr, flush=True)
+        ctx = snac_amd.Context(d, b, 3, device=0)
+        u, v, w = cases.channel_ic(ctx.shape(0, "u"), ctx.metric(0, 8))
+        for f, a in (("u", u), ("v", v), ("w", w)):
+            ctx.set(0, f, a)
+        ctx.init_bounds()
+        dt = min(0.5 * ctx.chkdt(), ctx.dtmax)
+        copy_gbs = ctx.stream_copy(1 << 27, reps=10 if args.calib else 3)
+        ctx.steps(dt, args.warmup)
+        ctx.sync()
+        print(f"[bench] warmup done, dt={dt:.3e}", file=sys.stderr, flush=True)
+        ctx.set_timing(True)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.steps(dt, args.steps)
+        ctx.sync()
+        el = time.perf_counter() - t0
+        ph = ctx.get_timing()
+        ctx.set_timing(False)
+        divtot, divmax = ctx.chkdiv()
+        print(f"[bench] timed {args.steps} steps in {el:.3f} s, divmax={divmax:.2e}", file=sys.stderr, flush=True)
+        ctx.close()
+
+        ms_step = el * 1e3 / args.steps
+        value = cells * args.steps / el / 1e6
+        phases = {k: round(v[0] / args.steps, 4) for k, v in ph.items()}
+        poisson = sum(ph[k][0] for k in ("r2r_fwd", "tridiag", "r2r_bwd")) / args.steps
+        rk_ms, rk_n = ph["rk_mom"]
+        rk_avg_s = rk_ms / rk_n * 1e-3
+        achieved = RK_MOM_BYTES_PER_CELL * cells / rk_avg_s / 1e9
+        traffic = None
+        if os.path.exists(TRAFFIC_FILE):
+            tr = json.load(open(TRAFFIC_FILE))
+            if tr.get("grid") == list(size):
+                traffic = tr.get("bytes_per_launch")
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "kernel": "k_rk_mom (fused rk_mom, 104 B/cell)", "avg_launch_ms": round(rk_avg_s * 1e3, 4),
+                    "stream_copy_GBs": round(copy_gbs, 1)}
+
+        cpu = None
+        if args.cpu_steps > 0:
+            thr = args.cpu_threads or min(16, os.cpu_count() or 1)
+            print(f"[bench] CPU baseline: {args.cpu_steps} oracle steps on {thr} threads", file=sys.stderr,
+                  flush=True)
+            cpu = cpu_baseline(d, b, size, args.cpu_steps, thr)
+
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mcell-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (laminar Poiseuille + seeded +-10% uniform noise, seed 20251202)",
+        "config": {"workload": "plain FP64 FFT-accelerated periodic channel",
+                   "grid": list(size), "stretching": "y tanh two-end, factor 1.8", "fft_axis": "z",
+                   "Re_b": 5600, "forcing": "bforce(1)=12/5600",
+                   "poisson": "r2r along z and x, tridiagonal along y (exact)", "parallelism": "1 GPU"},
+        "poisson_ms_per_step": round(poisson, 4),
+        "phase_ms_per_step": phases,
+        "divtot": divtot, "divmax": divmax,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -109,10 +109,33 @@ int snac_steps(snac_ctx *ctx, double dt, int nsteps);
 /* waits for the context's stream                                            */
 int snac_sync(snac_ctx *ctx);
 
+/* ---- per-phase timers (the reference's _TIMING hook, main.f90:1136-1143,
+ *      split by phase).  While enabled, every phase of snac_step is
+ *      bracketed by HIP events on the context's stream. ---- */
+enum {
+  SNAC_PH_RK_MOM = 0,   /* fused rk_mom                                     */
+  SNAC_PH_BOUND = 1,    /* bounduvw / boundp / inflow ghost fills           */
+  SNAC_PH_FILLPS = 2,   /* fillps + updt_rhs                                */
+  SNAC_PH_R2R_FWD = 3,  /* forward transforms (FFT axis first)              */
+  SNAC_PH_TRIDIAG = 4,  /* tridiagonal line solves                          */
+  SNAC_PH_R2R_BWD = 5,  /* inverse transforms and normfft scaling           */
+  SNAC_PH_CORREC = 6,   /* correc + updt_pressure                           */
+  SNAC_NPHASE = 7
+};
+int snac_set_timing(snac_ctx *ctx, int enable);
+/* waits for the stream, returns accumulated milliseconds and the number of
+ * timed intervals per phase since the last call, then resets them          */
+int snac_get_timing(snac_ctx *ctx, double *ms, int *count);
+
 /* ---- building blocks, exposed for their own parity tests ----
  * Batched r2r transform of howmany contiguous lines of length n, kinds
  * numbered as FFTW's (fftw.f90:31-54), on the device; host in/out.          */
 int snac_r2r_lines(snac_ctx *ctx, int kind, int n, int howmany, const double *in, double *out);
+/* STREAM-like copy of n doubles (read n*8 + write n*8 bytes per repetition)
+ * on the context's stream, timed with HIP events; returns GB/s.  The HBM
+ * reference of the roofline and of the byte-counter calibration; it has no
+ * counterpart in the reference.                                             */
+int snac_stream_copy(snac_ctx *ctx, long n, int reps, double *gbps);
 
 #ifdef __cplusplus
 }

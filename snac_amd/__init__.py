@@ -19,6 +19,7 @@ LIBPATH = os.path.join(_HERE, "libsnac_amd.so")
 
 FIELDS = {"u": 0, "v": 1, "w": 2, "p": 3, "pp": 4, "po": 5, "dudtrko": 6, "dvdtrko": 7, "dwdtrko": 8}
 RKCOEFF = ((32.0 / 60.0, 0.0), (25.0 / 60.0, -17.0 / 60.0), (45.0 / 60.0, -25.0 / 60.0))
+PHASES = ("rk_mom", "bound", "fillps", "r2r_fwd", "tridiag", "r2r_bwd", "correc")
 R2R_KINDS = {"R2HC": 0, "HC2R": 1, "REDFT00": 3, "REDFT01": 4, "REDFT10": 5, "REDFT11": 6,
              "RODFT00": 7, "RODFT01": 8, "RODFT10": 9, "RODFT11": 10}
 
@@ -60,6 +61,9 @@ def _load():
     L.snac_steps.argtypes = [vp, C.c_double, C.c_int]
     L.snac_sync.argtypes = [vp]
     L.snac_r2r_lines.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
+    L.snac_stream_copy.argtypes = [vp, C.c_long, C.c_int, C.POINTER(C.c_double)]
+    L.snac_set_timing.argtypes = [vp, C.c_int]
+    L.snac_get_timing.argtypes = [vp, dp, ip]
     return L
 
 
@@ -187,6 +191,22 @@ class Context:
         _check(lib.snac_r2r_lines(self.h, R2R_KINDS[kind], x.shape[1], x.shape[0], x.reshape(-1), out.reshape(-1)),
                "r2r_lines")
         return out
+
+    def set_timing(self, enable):
+        _check(lib.snac_set_timing(self.h, int(bool(enable))), "set_timing")
+
+    def get_timing(self):
+        """{phase: (total ms, number of timed intervals)} since the last call."""
+        ms = np.zeros(len(PHASES))
+        cnt = np.zeros(len(PHASES), np.int32)
+        _check(lib.snac_get_timing(self.h, ms, cnt), "get_timing")
+        return {p: (float(ms[q]), int(cnt[q])) for q, p in enumerate(PHASES)}
+
+    def stream_copy(self, n, reps=10):
+        """GB/s of a plain device copy of n doubles (HBM reference)."""
+        g = C.c_double()
+        _check(lib.snac_stream_copy(self.h, int(n), int(reps), C.byref(g)), "stream_copy")
+        return g.value
 
     def new_dt(self):
         """main.f90:660-661: dt = dt_f if > 0 else min(cfl*dt_cfl, dtmax)."""

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 
@@ -64,6 +65,9 @@ void launch_chkdt_part(hipStream_t s, const Geo &g, const Metrics &m, const doub
                        const double *w, double *part_max);
 // reduces n partial sums and maxima into out[0] (sum) and out[1] (max)
 void launch_reduce_final(hipStream_t s, const double *part_sum, const double *part_max, int n, double *out);
+
+// plain device copy of n doubles (n even), the HBM reference kernel
+void launch_stream_copy(hipStream_t s, const double *a, double *b, long n);
 
 // ---- ghost planes (kernels_bound.hip) ----
 void launch_face_ops(hipStream_t s, const FaceOp *dev_ops, int nops, int max_plane);

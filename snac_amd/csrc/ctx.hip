@@ -70,6 +70,13 @@ struct snac_ctx {
   int nparts = 0;
   double dlmin = 0;
   std::vector<void *> owned;
+  // phase timers
+  bool timing = false;
+  struct TRec { int ph; hipEvent_t a, b; };
+  std::vector<TRec> pending;
+  std::vector<hipEvent_t> pool;
+  double tacc[SNAC_NPHASE] = {};
+  int tcnt[SNAC_NPHASE] = {};
 };
 
 namespace snac {
@@ -126,6 +133,48 @@ Program upload_program(snac_ctx *c, const std::vector<std::vector<FaceOp>> &phas
 
 void run_program(snac_ctx *c, const Program &pr) {
   for (size_t q = 0; q < pr.dev.size(); q++) launch_face_ops(c->stream, pr.dev[q], pr.nops[q], pr.maxplane[q]);
+}
+
+hipEvent_t ev_get(snac_ctx *c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  SNAC_HIP(hipEventCreate(&e));
+  return e;
+}
+
+// brackets one phase with events when timing is on
+struct Phase {
+  snac_ctx *c;
+  int ph;
+  hipEvent_t a{}, b{};
+  Phase(snac_ctx *cc, int p) : c(cc), ph(p) {
+    if (!c->timing) return;
+    a = ev_get(c);
+    SNAC_HIP(hipEventRecord(a, c->stream));
+  }
+  ~Phase() {
+    if (!c->timing) return;
+    b = ev_get(c);
+    if (hipEventRecord(b, c->stream) == hipSuccess) c->pending.push_back({ph, a, b});
+  }
+};
+
+void collect_timing(snac_ctx *c) {
+  if (c->pending.empty()) return;
+  SNAC_HIP(hipStreamSynchronize(c->stream));
+  for (auto &r : c->pending) {
+    float ms = 0.f;
+    SNAC_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    c->tacc[r.ph] += ms;
+    c->tcnt[r.ph] += 1;
+    c->pool.push_back(r.a);
+    c->pool.push_back(r.b);
+  }
+  c->pending.clear();
 }
 
 // ---- tensor Poisson planner ----
@@ -283,6 +332,7 @@ void solve_pressure(snac_ctx *c) {
   const long pst[3] = {1, (long)t.G[0], (long)t.G[0] * t.G[1]};
   auto others = [](int a, int &b1, int &b2) { b1 = a == 0 ? 1 : 0; b2 = a == 2 ? 1 : 2; };
   // first transform: block pp interiors -> global work array
+  Phase *tf = new Phase(c, SNAC_PH_R2R_FWD);
   {
     int a = t.xaxes[0], b1, b2;
     others(a, b1, b2);
@@ -302,7 +352,9 @@ void solve_pressure(snac_ctx *c) {
     launch_r2r(c->stream, t.fwd[q], t.ph, pst[a], pst[b1], pst[b2], t.ph, pst[a], pst[b1], pst[b2], t.G[b1], t.G[b2],
                1.0);
   }
+  delete tf;
   {
+    Phase tt(c, SNAC_PH_TRIDIAG);
     TridiagLine tl;
     for (int q = 0; q < 3; q++) tl.G[q] = t.G[q];
     tl.s = t.s;
@@ -313,6 +365,7 @@ void solve_pressure(snac_ctx *c) {
     tl.pin_when_zero = t.pin;
     launch_tridiag(c->stream, tl, t.ph, t.scratch);
   }
+  Phase tb(c, SNAC_PH_R2R_BWD);
   for (size_t q = t.xaxes.size() - 1; q >= 1; q--) {
     int a = t.xaxes[q], b1, b2;
     others(a, b1, b2);
@@ -360,14 +413,18 @@ void correc(snac_ctx *c, double dtrk, bool fuse_pressure) {
 
 void substage(snac_ctx *c, int irk, double dt) {
   double dtrk = (RKC[irk - 1][0] + RKC[irk - 1][1]) * dt;
-  rk_mom(c, irk, dt);
-  run_program(c, c->prog_uvw[c->cur][0]);   // bounduvw + inflow
-  fillps(c, dtrk);
+  { Phase t(c, SNAC_PH_RK_MOM); rk_mom(c, irk, dt); }
+  { Phase t(c, SNAC_PH_BOUND); run_program(c, c->prog_uvw[c->cur][0]); }  // bounduvw + inflow
+  { Phase t(c, SNAC_PH_FILLPS); fillps(c, dtrk); }
   solve_pressure(c);
-  run_program(c, c->prog_pp);               // boundp(pp)
-  correc(c, dtrk, true);                    // correc + updt_pressure
-  run_program(c, c->prog_uvw[c->cur][1]);   // bounduvw(correc) + inflow(correc)
-  run_program(c, c->prog_p);                // boundp(p)
+  { Phase t(c, SNAC_PH_BOUND); run_program(c, c->prog_pp); }               // boundp(pp)
+  { Phase t(c, SNAC_PH_CORREC); correc(c, dtrk, true); }                   // correc + updt_pressure
+  {
+    Phase t(c, SNAC_PH_BOUND);
+    run_program(c, c->prog_uvw[c->cur][1]);  // bounduvw(correc) + inflow(correc)
+    run_program(c, c->prog_p);               // boundp(p)
+  }
+  if (c->pending.size() > 4096) collect_timing(c);  // bounds the event pool in long timed runs
 }
 
 double *field_ptr(snac_ctx *c, int b, int fid, long &count) {
@@ -456,6 +513,8 @@ void create(snac_ctx *c) {
 
 void destroy(snac_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto &r : c->pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+  for (auto &e : c->pool) (void)hipEventDestroy(e);
   for (void *p : c->owned) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
@@ -677,6 +736,29 @@ int snac_steps(snac_ctx *c, double dt, int nsteps) {
   });
 }
 
+int snac_set_timing(snac_ctx *c, int enable) {
+  return guard([&] {
+    on_device(c);
+    collect_timing(c);
+    c->timing = enable != 0;
+    std::fill(c->tacc, c->tacc + SNAC_NPHASE, 0.0);
+    std::fill(c->tcnt, c->tcnt + SNAC_NPHASE, 0);
+  });
+}
+
+int snac_get_timing(snac_ctx *c, double *ms, int *count) {
+  return guard([&] {
+    on_device(c);
+    collect_timing(c);
+    for (int q = 0; q < SNAC_NPHASE; q++) {
+      if (ms) ms[q] = c->tacc[q];
+      if (count) count[q] = c->tcnt[q];
+    }
+    std::fill(c->tacc, c->tacc + SNAC_NPHASE, 0.0);
+    std::fill(c->tcnt, c->tcnt + SNAC_NPHASE, 0);
+  });
+}
+
 int snac_sync(snac_ctx *c) { return guard([&] { SNAC_HIP(hipStreamSynchronize(on_device(c)->stream)); }); }
 
 int snac_r2r_lines(snac_ctx *c, int kind, int n, int howmany, const double *in, double *out) {
@@ -693,6 +775,33 @@ int snac_r2r_lines(snac_ctx *c, int kind, int n, int howmany, const double *in, 
     SNAC_HIP(hipStreamSynchronize(c->stream));
     SNAC_HIP(hipFree(din));
     SNAC_HIP(hipFree(dout));
+  });
+}
+
+int snac_stream_copy(snac_ctx *c, long n, int reps, double *gbps) {
+  return guard([&] {
+    on_device(c);
+    if (n < 2 || reps < 1) throw std::runtime_error("stream_copy: n >= 2 and reps >= 1 required");
+    n &= ~1L;
+    double *a = nullptr, *b = nullptr;
+    SNAC_HIP(hipMalloc(&a, n * sizeof(double)));
+    SNAC_HIP(hipMalloc(&b, n * sizeof(double)));
+    SNAC_HIP(hipMemsetAsync(a, 0, n * sizeof(double), c->stream));
+    launch_stream_copy(c->stream, a, b, n);  // warm-up
+    hipEvent_t e0, e1;
+    SNAC_HIP(hipEventCreate(&e0));
+    SNAC_HIP(hipEventCreate(&e1));
+    SNAC_HIP(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; r++) launch_stream_copy(c->stream, a, b, n);
+    SNAC_HIP(hipEventRecord(e1, c->stream));
+    SNAC_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    SNAC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (gbps) *gbps = 16.0 * (double)n * reps / (ms * 1e-3) / 1e9;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    SNAC_HIP(hipFree(a));
+    SNAC_HIP(hipFree(b));
   });
 }
 

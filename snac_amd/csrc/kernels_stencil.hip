@@ -299,3 +299,23 @@ void launch_reduce_final(hipStream_t s, const double *psum, const double *pmax, 
 }
 
 }  // namespace snac
+
+namespace snac {
+namespace {
+// plain copy, 16 bytes per lane per iteration: the HBM reference kernel for
+// the roofline and for scaling the FETCH_SIZE / WRITE_SIZE counters
+__global__ __launch_bounds__(256) void k_stream_copy(const double2 *__restrict__ a, double2 *__restrict__ b, long n2) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n2; i += (long)gridDim.x * 256L) b[i] = a[i];
+}
+}  // namespace
+
+void launch_stream_copy(hipStream_t s, const double *a, double *b, long n) {
+  long n2 = n / 2;
+  int dev = 0, ncu = 0;
+  SNAC_HIP(hipGetDevice(&dev));
+  SNAC_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  long blocks = std::min<long>((n2 + 255) / 256, (long)ncu * 16);
+  hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)blocks), dim3(256), 0, s, (const double2 *)a, (double2 *)b, n2);
+  SNAC_HIP(hipGetLastError());
+}
+}  // namespace snac
