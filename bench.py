@@ -92,6 +92,7 @@ def main():
         for f, a in (("u", u), ("v", v), ("w", w)):
             ctx.set(0, f, a)
         ctx.init_bounds()
+        tune_choices, tune_ms = ctx.autotune()
         dt = min(0.5 * ctx.chkdt(), ctx.dtmax)
         copy_gbs = ctx.stream_copy(1 << 27, reps=10 if args.calib else 3)
         ctx.steps(dt, args.warmup)
@@ -141,7 +142,8 @@ def main():
         "config": {"workload": "plain FP64 FFT-accelerated periodic channel",
                    "grid": list(size), "stretching": "y tanh two-end, factor 1.8", "fft_axis": "z",
                    "Re_b": 5600, "forcing": "bforce(1)=12/5600",
-                   "poisson": "r2r along z and x, tridiagonal along y (exact)", "parallelism": "1 GPU"},
+                   "poisson": "r2r along z and x, tridiagonal along y (exact)", "parallelism": "1 GPU",
+                   "autotune": {"choices": tune_choices, "best_ms": tune_ms}},
         "poisson_ms_per_step": round(poisson, 4),
         "phase_ms_per_step": phases,
         "divtot": divtot, "divmax": divmax,

@@ -127,6 +127,12 @@ int snac_set_timing(snac_ctx *ctx, int enable);
  * timed intervals per phase since the last call, then resets them          */
 int snac_get_timing(snac_ctx *ctx, double *ms, int *count);
 
+/* Choices of the autotuner that runs at context creation (tile shape of each
+ * transform call site: forward ones, then inverse ones, then the
+ * tridiagonal variant; -1 = dense path) and the best times in ms.  No
+ * counterpart in the reference (its FFTW plans use FFTW_ESTIMATE).          */
+int snac_get_autotune(snac_ctx *ctx, int *choices, double *ms, int n);
+
 /* ---- building blocks, exposed for their own parity tests ----
  * Batched r2r transform of howmany contiguous lines of length n, kinds
  * numbered as FFTW's (fftw.f90:31-54), on the device; host in/out.          */

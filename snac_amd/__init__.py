@@ -62,6 +62,7 @@ def _load():
     L.snac_sync.argtypes = [vp]
     L.snac_r2r_lines.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
     L.snac_stream_copy.argtypes = [vp, C.c_long, C.c_int, C.POINTER(C.c_double)]
+    L.snac_get_autotune.argtypes = [vp, ip, dp, C.c_int]
     L.snac_set_timing.argtypes = [vp, C.c_int]
     L.snac_get_timing.argtypes = [vp, dp, ip]
     return L
@@ -207,6 +208,15 @@ class Context:
         g = C.c_double()
         _check(lib.snac_stream_copy(self.h, int(n), int(reps), C.byref(g)), "stream_copy")
         return g.value
+
+    def autotune(self):
+        """(choices, best ms) of the autotuner: transform tile per call site
+        (forward, then inverse), then the tridiagonal variant."""
+        ch = np.zeros(16, np.int32)
+        ms = np.zeros(16)
+        _check(lib.snac_get_autotune(self.h, ch, ms, 16), "get_autotune")
+        k = int(np.argmax(ch == -2)) if (ch == -2).any() else 16
+        return ch[:k].tolist(), [round(float(x), 4) for x in ms[:k] if x > 0]
 
     def new_dt(self):
         """main.f90:660-661: dt = dt_f if > 0 else min(cfl*dt_cfl, dtmax)."""

@@ -20,6 +20,7 @@ namespace {
 struct Rec {
   FaceOp op;
   const void *warr, *rarr;  // array identities (null when no read)
+  int stage = 0;            // 0..2 halo x/y/z, 3 set_bc, 4 inflow
 };
 
 bool planes_meet(const void *a1, int d1, int q1, const void *a2, int d2, int q2) {
@@ -96,6 +97,7 @@ void set_bc_ops(std::vector<Rec> &out, const BlockArrays &B, double *p, char cty
 
 void halo_ops(std::vector<Rec> &out, const std::vector<BlockArrays> &bl, const Case &cs, int d,
               double *(*field)(const BlockArrays &, int), int fid) {
+  size_t first = out.size();
   for (size_t b = 0; b < bl.size(); b++) {
     const Block &B = cs.blocks[b];
     int n3[3] = {B.n[0], B.n[1], B.n[2]};
@@ -107,6 +109,7 @@ void halo_ops(std::vector<Rec> &out, const std::vector<BlockArrays> &bl, const C
       out.push_back(mk(bl[b], field(bl[b], fid), d, qdst, field(bl[nb], fid), &bl[nb], qsrc, nullptr, 0.0, 1.0, 0.0));
     }
   }
+  for (size_t q = first; q < out.size(); q++) out[q].stage = d;
 }
 
 }  // namespace
@@ -136,8 +139,13 @@ std::vector<std::vector<FaceOp>> pack(const std::vector<Rec> &seq) {
 }
 }  // namespace
 
-std::vector<std::vector<FaceOp>> build_velocity_program(const Case &cs, const std::vector<BlockArrays> &bl, int par,
-                                                        bool is_correc, bool with_halo_bc, bool with_inflow) {
+namespace {
+void tag_from(std::vector<Rec> &seq, size_t first, int stage) {
+  for (size_t q = first; q < seq.size(); q++) seq[q].stage = stage;
+}
+
+std::vector<Rec> velocity_ops(const Case &cs, const std::vector<BlockArrays> &bl, int par, bool is_correc,
+                              bool with_halo_bc, bool with_inflow) {
   std::vector<Rec> seq;
   std::vector<BlockArrays> sel = bl;
   for (auto &B : sel) B.par_sel = par;
@@ -145,6 +153,7 @@ std::vector<std::vector<FaceOp>> build_velocity_program(const Case &cs, const st
   if (with_halo_bc) {
     for (int d = 0; d < 3; d++)
       for (int comp = 0; comp < 3; comp++) halo_ops(seq, sel, cs, d, field, comp);
+    size_t first_bc = seq.size();
     for (size_t b = 0; b < sel.size(); b++) {
       const Block &B = cs.blocks[b];
       for (int d = 0; d < 3; d++)
@@ -160,8 +169,10 @@ std::vector<std::vector<FaceOp>> build_velocity_program(const Case &cs, const st
           }
         }
     }
+    tag_from(seq, first_bc, 3);
   }
   if (with_inflow) {
+    size_t first_in = seq.size();
     for (size_t b = 0; b < sel.size(); b++) {
       const Block &B = cs.blocks[b];
       for (int d = 0; d < 3; d++)
@@ -180,16 +191,18 @@ std::vector<std::vector<FaceOp>> build_velocity_program(const Case &cs, const st
           }
         }
     }
+    tag_from(seq, first_in, 4);
   }
-  return pack(seq);
+  return seq;
 }
 
-std::vector<std::vector<FaceOp>> build_pressure_program(const Case &cs, const std::vector<BlockArrays> &bl, int which) {
+std::vector<Rec> pressure_ops(const Case &cs, const std::vector<BlockArrays> &bl, int which) {
   std::vector<Rec> seq;
   std::vector<BlockArrays> sel = bl;
   for (auto &B : sel) B.par_sel = which;  // 0 = p, 1 = pp
   auto field = [](const BlockArrays &B, int) -> double * { return B.par_sel == 0 ? B.p : B.pp; };
   for (int d = 0; d < 3; d++) halo_ops(seq, sel, cs, d, field, 0);
+  size_t first_bc = seq.size();
   for (size_t b = 0; b < sel.size(); b++) {
     const Block &B = cs.blocks[b];
     for (int d = 0; d < 3; d++)
@@ -199,7 +212,33 @@ std::vector<std::vector<FaceOp>> build_pressure_program(const Case &cs, const st
         set_bc_ops(seq, sel[b], field(sel[b], 0), B.cbcpre[d][side], side, d, true, B.bcpre[d][side], B.dc[d][qm]);
       }
   }
+  tag_from(seq, first_bc, 3);
+  return seq;
+}
+
+// Fields do not interact, so the ops of several fills may be interleaved as
+// long as each array keeps its own order: a stable sort by stage does that.
+std::vector<std::vector<FaceOp>> merge_and_pack(std::vector<Rec> seq) {
+  std::stable_sort(seq.begin(), seq.end(), [](const Rec &a, const Rec &b) { return a.stage < b.stage; });
   return pack(seq);
+}
+}  // namespace
+
+std::vector<std::vector<FaceOp>> build_velocity_program(const Case &cs, const std::vector<BlockArrays> &bl, int par,
+                                                        bool is_correc, bool with_halo_bc, bool with_inflow) {
+  return merge_and_pack(velocity_ops(cs, bl, par, is_correc, with_halo_bc, with_inflow));
+}
+
+std::vector<std::vector<FaceOp>> build_pressure_program(const Case &cs, const std::vector<BlockArrays> &bl, int which) {
+  return merge_and_pack(pressure_ops(cs, bl, which));
+}
+
+std::vector<std::vector<FaceOp>> build_correc_fill_program(const Case &cs, const std::vector<BlockArrays> &bl,
+                                                           int par) {
+  std::vector<Rec> seq = velocity_ops(cs, bl, par, true, true, true);
+  std::vector<Rec> ps = pressure_ops(cs, bl, 0);
+  seq.insert(seq.end(), ps.begin(), ps.end());
+  return merge_and_pack(seq);
 }
 
 }  // namespace snac

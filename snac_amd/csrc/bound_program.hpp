@@ -22,5 +22,9 @@ std::vector<std::vector<FaceOp>> build_velocity_program(const Case &cs, const st
                                                         bool is_correc, bool with_halo_bc, bool with_inflow);
 // boundp on p (which = 0) or pp (which = 1)
 std::vector<std::vector<FaceOp>> build_pressure_program(const Case &cs, const std::vector<BlockArrays> &bl, int which);
+// bounduvw(is_correc) + inflow(is_correc) + boundp(p), the three fills that end
+// a substage (main.f90:1026-1032), merged into shared phases
+std::vector<std::vector<FaceOp>> build_correc_fill_program(const Case &cs, const std::vector<BlockArrays> &bl,
+                                                           int par);
 
 }  // namespace snac

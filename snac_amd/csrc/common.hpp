@@ -23,9 +23,13 @@ struct Geo {
   long sx, sy;  // sx = nx+2, sy = ny+2
 };
 
-// 1D grid metrics of one block on the device (n+2 entries each).
+// 1D grid metrics of one block on the device (n+2 entries each), and their
+// reciprocals: the kernels multiply by 1/d instead of dividing by d, which
+// removes the FP64 division sequences from the stencils (a rounding-level
+// difference from the reference's divisions).
 struct Metrics {
   const double *dxc, *dxf, *dyc, *dyf, *dzc, *dzf;
+  const double *idxc, *idxf, *idyc, *idyf, *idzc, *idzf;
 };
 
 // One ghost-plane operation: dst plane (normal d, index qdst) =
@@ -79,6 +83,7 @@ void launch_face_ops(hipStream_t s, const FaceOp *dev_ops, int nops, int max_pla
 struct R2RPlan {
   int n = 0, kind = 0;
   bool fft = false;        // LDS FFT path (n a power of two >= 4, supported kind)
+  int lshift = 0;          // FFT tile: lines per workgroup = max >> lshift (0..2, autotuned)
   const double *tab = nullptr;  // device twiddles (fft) or dense matrix
 };
 R2RPlan r2r_plan(int kind, int n);  // cached per (kind, n); throws if unsupported
@@ -97,11 +102,14 @@ double r2r_coef(int kind, int n, int k, int j);
 // fixed to zero (singular Neumann-Neumann line).
 struct TridiagLine {
   int G[3];
-  int s;  // line axis
+  int s;        // line axis
+  int variant;  // 0: c' in LDS (lines up to 128), 1: c' in cscratch (autotuned)
   const double *am, *ap, *d0, *wt;  // size G[s]
   const double *lamA, *lamB;        // along the other axes (ascending order)
   int pin_when_zero;
 };
-void launch_tridiag(hipStream_t st, const TridiagLine &t, double *x, double *scratch);
+// dscratch (d') and cscratch (c', used only for lines longer than the LDS
+// budget) have the size and layout of x
+void launch_tridiag(hipStream_t st, const TridiagLine &t, double *x, double *dscratch, double *cscratch);
 
 }  // namespace snac

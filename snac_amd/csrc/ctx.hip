@@ -23,7 +23,7 @@ const double RKC[3][2] = {{32.0 / 60.0, 0.0}, {25.0 / 60.0, -17.0 / 60.0}, {45.0
 struct DevBlock {
   Geo g;
   Metrics m;
-  double *met[6] = {};          // dxc dxf dyc dyf dzc dzf (device, owned)
+  double *met[12] = {};         // dxc dxf dyc dyf dzc dzf and their reciprocals (device)
   double *vel[2][3] = {};       // ping-pong velocity buffers
   double *p = nullptr, *pp = nullptr, *po = nullptr;
   double *dto[3] = {};
@@ -51,7 +51,9 @@ struct TensorPlan {
   double *lam[3] = {};             // device eigenvalue arrays per axis (line axis: zeros)
   double *am = nullptr, *ap = nullptr, *d0 = nullptr, *wt = nullptr;
   int pin = 0;
-  double *ph = nullptr, *scratch = nullptr;
+  int tvariant = 1;                // tridiagonal kernel variant (autotuned)
+  double *ph = nullptr, *scratch = nullptr, *scratch2 = nullptr;
+  std::vector<double> tune_ms;     // best time per tuned stage (diagnostics)
 };
 
 }  // namespace
@@ -63,7 +65,7 @@ struct snac_ctx {
   hipStream_t stream = nullptr;
   std::vector<snac::DevBlock> blk;
   int cur = 0;  // parity of the current velocity buffers
-  snac::Program prog_uvw[2][2], prog_bound_only[2][2], prog_inflow[2][2], prog_p, prog_pp;
+  snac::Program prog_uvw[2][2], prog_bound_only[2][2], prog_inflow[2][2], prog_p, prog_pp, prog_end[2];
   snac::TensorPlan tp;
   int solver = SNAC_SOLVER_AUTO;
   double *part_sum = nullptr, *part_max = nullptr, *red = nullptr;
@@ -92,6 +94,15 @@ T *dalloc(snac_ctx *c, size_t n) {
   c->owned.push_back(p);
   return (T *)p;
 }
+// Ghosted field with the leading dimension padded to a multiple of 16
+// doubles and the base shifted so that interior element (1, j, k) of every
+// row starts on a 128-byte boundary (hipMalloc returns 256-byte aligned
+// memory): rows of 16 or 64 neighbouring lines are whole cache lines.
+double *dalloc_field(snac_ctx *c, long ntot) {
+  double *p = dalloc<double>(c, (size_t)ntot + 16);
+  return p ? p + 15 : nullptr;
+}
+
 template <class T>
 T *dupload(snac_ctx *c, const std::vector<T> &h) {
   T *p = dalloc<T>(c, h.size());
@@ -313,6 +324,7 @@ void plan_tensor(snac_ctx *c) {
   size_t tot = (size_t)t.G[0] * t.G[1] * t.G[2];
   t.ph = dalloc<double>(c, tot);
   t.scratch = dalloc<double>(c, tot);
+  t.scratch2 = dalloc<double>(c, tot);
   t.ok = true;
 }
 
@@ -326,65 +338,144 @@ void check_solver(snac_ctx *c) {
 long gstride(const DevBlock &D, int a) { return a == 0 ? 1 : (a == 1 ? D.g.sx : D.g.sx * D.g.sy); }
 int bext(const DevBlock &D, int a) { return a == 0 ? D.g.nx : (a == 1 ? D.g.ny : D.g.nz); }
 
+void others(int a, int &b1, int &b2) {
+  b1 = a == 0 ? 1 : 0;
+  b2 = a == 2 ? 1 : 2;
+}
+
+// first transform: block pp interiors -> global work array
+void stage_first(snac_ctx *c) {
+  TensorPlan &t = c->tp;
+  const long pst[3] = {1, (long)t.G[0], (long)t.G[0] * t.G[1]};
+  int a = t.xaxes[0], b1, b2;
+  others(a, b1, b2);
+  for (size_t b = 0; b < c->blk.size(); b++) {
+    DevBlock &D = c->blk[b];
+    const Block &B = c->cs.blocks[b];
+    const double *in = D.pp + 1 + D.g.sx + D.g.sx * D.g.sy;
+    long off = 0;
+    for (int q = 0; q < 3; q++) off += (long)(B.lo_g[q] - t.gmin[q]) * pst[q];
+    launch_r2r(c->stream, t.fwd[0], in, gstride(D, a), gstride(D, b1), gstride(D, b2), t.ph + off, pst[a], pst[b1],
+               pst[b2], bext(D, b1), bext(D, b2), 1.0);
+  }
+}
+
+// in-place transform number q on the work array (forward or inverse)
+void stage_inplace(snac_ctx *c, size_t q, bool fwd) {
+  TensorPlan &t = c->tp;
+  const long pst[3] = {1, (long)t.G[0], (long)t.G[0] * t.G[1]};
+  int a = t.xaxes[q], b1, b2;
+  others(a, b1, b2);
+  launch_r2r(c->stream, fwd ? t.fwd[q] : t.bwd[q], t.ph, pst[a], pst[b1], pst[b2], t.ph, pst[a], pst[b1], pst[b2],
+             t.G[b1], t.G[b2], fwd ? 1.0 : t.norm[q]);
+}
+
+void stage_tridiag(snac_ctx *c) {
+  TensorPlan &t = c->tp;
+  TridiagLine tl;
+  for (int q = 0; q < 3; q++) tl.G[q] = t.G[q];
+  tl.s = t.s;
+  tl.variant = t.tvariant;
+  int o1 = t.s == 0 ? 1 : 0, o2 = t.s == 2 ? 1 : 2;
+  tl.am = t.am; tl.ap = t.ap; tl.d0 = t.d0; tl.wt = t.wt;
+  tl.lamA = t.lam[o1];
+  tl.lamB = t.lam[o2];
+  tl.pin_when_zero = t.pin;
+  launch_tridiag(c->stream, tl, t.ph, t.scratch, t.scratch2);
+}
+
+// last inverse transform: global work array -> block pp interiors, scaled
+void stage_last(snac_ctx *c) {
+  TensorPlan &t = c->tp;
+  const long pst[3] = {1, (long)t.G[0], (long)t.G[0] * t.G[1]};
+  int a = t.xaxes[0], b1, b2;
+  others(a, b1, b2);
+  for (size_t b = 0; b < c->blk.size(); b++) {
+    DevBlock &D = c->blk[b];
+    const Block &B = c->cs.blocks[b];
+    double *out = D.pp + 1 + D.g.sx + D.g.sx * D.g.sy;
+    long off = 0;
+    for (int q = 0; q < 3; q++) off += (long)(B.lo_g[q] - t.gmin[q]) * pst[q];
+    launch_r2r(c->stream, t.bwd[0], t.ph + off, pst[a], pst[b1], pst[b2], out, gstride(D, a), gstride(D, b1),
+               gstride(D, b2), bext(D, b1), bext(D, b2), t.norm[0]);
+  }
+}
+
 void solve_pressure(snac_ctx *c) {
   check_solver(c);
   TensorPlan &t = c->tp;
-  const long pst[3] = {1, (long)t.G[0], (long)t.G[0] * t.G[1]};
-  auto others = [](int a, int &b1, int &b2) { b1 = a == 0 ? 1 : 0; b2 = a == 2 ? 1 : 2; };
-  // first transform: block pp interiors -> global work array
-  Phase *tf = new Phase(c, SNAC_PH_R2R_FWD);
   {
-    int a = t.xaxes[0], b1, b2;
-    others(a, b1, b2);
-    for (size_t b = 0; b < c->blk.size(); b++) {
-      DevBlock &D = c->blk[b];
-      const Block &B = c->cs.blocks[b];
-      const double *in = D.pp + 1 + D.g.sx + D.g.sx * D.g.sy;
-      long off = 0;
-      for (int q = 0; q < 3; q++) off += (long)(B.lo_g[q] - t.gmin[q]) * pst[q];
-      launch_r2r(c->stream, t.fwd[0], in, gstride(D, a), gstride(D, b1), gstride(D, b2), t.ph + off, pst[a], pst[b1],
-                 pst[b2], bext(D, b1), bext(D, b2), 1.0);
-    }
+    Phase tf(c, SNAC_PH_R2R_FWD);
+    stage_first(c);
+    for (size_t q = 1; q < t.xaxes.size(); q++) stage_inplace(c, q, true);
   }
-  for (size_t q = 1; q < t.xaxes.size(); q++) {
-    int a = t.xaxes[q], b1, b2;
-    others(a, b1, b2);
-    launch_r2r(c->stream, t.fwd[q], t.ph, pst[a], pst[b1], pst[b2], t.ph, pst[a], pst[b1], pst[b2], t.G[b1], t.G[b2],
-               1.0);
-  }
-  delete tf;
   {
     Phase tt(c, SNAC_PH_TRIDIAG);
-    TridiagLine tl;
-    for (int q = 0; q < 3; q++) tl.G[q] = t.G[q];
-    tl.s = t.s;
-    int o1 = t.s == 0 ? 1 : 0, o2 = t.s == 2 ? 1 : 2;
-    tl.am = t.am; tl.ap = t.ap; tl.d0 = t.d0; tl.wt = t.wt;
-    tl.lamA = t.lam[o1];
-    tl.lamB = t.lam[o2];
-    tl.pin_when_zero = t.pin;
-    launch_tridiag(c->stream, tl, t.ph, t.scratch);
-  }
-  Phase tb(c, SNAC_PH_R2R_BWD);
-  for (size_t q = t.xaxes.size() - 1; q >= 1; q--) {
-    int a = t.xaxes[q], b1, b2;
-    others(a, b1, b2);
-    launch_r2r(c->stream, t.bwd[q], t.ph, pst[a], pst[b1], pst[b2], t.ph, pst[a], pst[b1], pst[b2], t.G[b1], t.G[b2],
-               t.norm[q]);
+    stage_tridiag(c);
   }
   {
-    int a = t.xaxes[0], b1, b2;
-    others(a, b1, b2);
-    for (size_t b = 0; b < c->blk.size(); b++) {
-      DevBlock &D = c->blk[b];
-      const Block &B = c->cs.blocks[b];
-      double *out = D.pp + 1 + D.g.sx + D.g.sx * D.g.sy;
-      long off = 0;
-      for (int q = 0; q < 3; q++) off += (long)(B.lo_g[q] - t.gmin[q]) * pst[q];
-      launch_r2r(c->stream, t.bwd[0], t.ph + off, pst[a], pst[b1], pst[b2], out, gstride(D, a), gstride(D, b1),
-                 gstride(D, b2), bext(D, b1), bext(D, b2), t.norm[0]);
-    }
+    Phase tb(c, SNAC_PH_R2R_BWD);
+    for (size_t q = t.xaxes.size() - 1; q >= 1; q--) stage_inplace(c, q, false);
+    stage_last(c);
   }
+}
+
+// Autotuner: times every candidate tile shape of each transform call site
+// and both tridiagonal variants on this case's own arrays, keeps the
+// fastest.  Runs once at context creation.
+template <class F>
+double time_stage(snac_ctx *c, F &&f) {
+  hipEvent_t e0, e1;
+  SNAC_HIP(hipEventCreate(&e0));
+  SNAC_HIP(hipEventCreate(&e1));
+  f();  // warm-up
+  SNAC_HIP(hipEventRecord(e0, c->stream));
+  const int reps = 5;
+  for (int r = 0; r < reps; r++) f();
+  SNAC_HIP(hipEventRecord(e1, c->stream));
+  SNAC_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  SNAC_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return ms / reps;
+}
+
+void autotune(snac_ctx *c) {
+  TensorPlan &t = c->tp;
+  if (!t.ok || c->device < 0) return;
+  t.tune_ms.clear();
+  auto tune = [&](R2RPlan &pl, auto &&run) {
+    if (!pl.fft) return;
+    int best = 0;
+    double bt = 1e300;
+    for (int ls = 0; ls <= 2; ls++) {
+      pl.lshift = ls;
+      double ms = time_stage(c, run);
+      if (ms < bt) { bt = ms; best = ls; }
+    }
+    pl.lshift = best;
+    t.tune_ms.push_back(bt);
+  };
+  tune(t.fwd[0], [&] { stage_first(c); });
+  for (size_t q = 1; q < t.xaxes.size(); q++) {
+    tune(t.fwd[q], [&] { stage_inplace(c, q, true); });
+    tune(t.bwd[q], [&] { stage_inplace(c, q, false); });
+  }
+  tune(t.bwd[0], [&] { stage_last(c); });
+  int best = 1;
+  double bt = 1e300;
+  for (int v = (t.G[t.s] <= 128 ? 0 : 1); v <= 1; v++) {
+    t.tvariant = v;
+    double ms = time_stage(c, [&] { stage_tridiag(c); });
+    if (ms < bt) { bt = ms; best = v; }
+  }
+  t.tvariant = best;
+  t.tune_ms.push_back(bt);
+  // the tuning runs left transformed zeros behind; restore a clean state
+  SNAC_HIP(hipMemsetAsync(t.ph, 0, (size_t)t.G[0] * t.G[1] * t.G[2] * sizeof(double), c->stream));
+  for (auto &D : c->blk) SNAC_HIP(hipMemsetAsync(D.pp, 0, D.ntot * sizeof(double), c->stream));
+  SNAC_HIP(hipStreamSynchronize(c->stream));
 }
 
 void rk_mom(snac_ctx *c, int irk, double dt) {
@@ -419,11 +510,7 @@ void substage(snac_ctx *c, int irk, double dt) {
   solve_pressure(c);
   { Phase t(c, SNAC_PH_BOUND); run_program(c, c->prog_pp); }               // boundp(pp)
   { Phase t(c, SNAC_PH_CORREC); correc(c, dtrk, true); }                   // correc + updt_pressure
-  {
-    Phase t(c, SNAC_PH_BOUND);
-    run_program(c, c->prog_uvw[c->cur][1]);  // bounduvw(correc) + inflow(correc)
-    run_program(c, c->prog_p);               // boundp(p)
-  }
+  { Phase t(c, SNAC_PH_BOUND); run_program(c, c->prog_end[c->cur]); }  // bounduvw/inflow(correc) + boundp(p)
   if (c->pending.size() > 4096) collect_timing(c);  // bounds the event pool in long timed runs
 }
 
@@ -471,20 +558,26 @@ void create(snac_ctx *c) {
     const Block &B = cs.blocks[b];
     DevBlock &D = c->blk[b];
     D.g.nx = B.n[0]; D.g.ny = B.n[1]; D.g.nz = B.n[2];
-    D.g.sx = B.n[0] + 2; D.g.sy = B.n[1] + 2;
+    D.g.sx = ((B.n[0] + 2 + 15) / 16) * 16;  // padded leading dimension
+    D.g.sy = B.n[1] + 2;
     D.ntot = D.g.sx * D.g.sy * (long)(B.n[2] + 2);
     D.nin = (long)B.n[0] * B.n[1] * B.n[2];
     for (int a = 0; a < 3; a++) {
       D.met[2 * a] = dupload(c, B.dc[a]);
       D.met[2 * a + 1] = dupload(c, B.df[a]);
+      std::vector<double> ic(B.dc[a].size()), iff(B.df[a].size());
+      for (size_t q = 0; q < ic.size(); q++) { ic[q] = 1.0 / B.dc[a][q]; iff[q] = 1.0 / B.df[a][q]; }
+      D.met[6 + 2 * a] = dupload(c, ic);
+      D.met[6 + 2 * a + 1] = dupload(c, iff);
       for (double v : B.df[a]) c->dlmin = std::min(c->dlmin, v);
     }
-    D.m = Metrics{D.met[0], D.met[1], D.met[2], D.met[3], D.met[4], D.met[5]};
+    D.m = Metrics{D.met[0], D.met[1], D.met[2], D.met[3], D.met[4], D.met[5],
+                  D.met[6], D.met[7], D.met[8], D.met[9], D.met[10], D.met[11]};
     for (int par = 0; par < 2; par++)
-      for (int q = 0; q < 3; q++) D.vel[par][q] = dalloc<double>(c, D.ntot);
-    D.p = dalloc<double>(c, D.ntot);
-    D.pp = dalloc<double>(c, D.ntot);
-    D.po = dalloc<double>(c, D.ntot);
+      for (int q = 0; q < 3; q++) D.vel[par][q] = dalloc_field(c, D.ntot);
+    D.p = dalloc_field(c, D.ntot);
+    D.pp = dalloc_field(c, D.ntot);
+    D.po = dalloc_field(c, D.ntot);
     for (int q = 0; q < 3; q++) D.dto[q] = dalloc<double>(c, D.nin);
     for (int a = 0; a < 3; a++)
       for (int s = 0; s < 2; s++) {
@@ -505,9 +598,11 @@ void create(snac_ctx *c) {
       c->prog_bound_only[par][cr] = upload_program(c, build_velocity_program(cs, arrs, par, cr, true, false));
       c->prog_inflow[par][cr] = upload_program(c, build_velocity_program(cs, arrs, par, cr, false, true));
     }
+  for (int par = 0; par < 2; par++) c->prog_end[par] = upload_program(c, build_correc_fill_program(cs, arrs, par));
   c->prog_p = upload_program(c, build_pressure_program(cs, arrs, 0));
   c->prog_pp = upload_program(c, build_pressure_program(cs, arrs, 1));
   plan_tensor(c);
+  autotune(c);
   if (c->device >= 0) SNAC_HIP(hipDeviceSynchronize());
 }
 
@@ -629,22 +724,30 @@ int snac_ctx_set_solver(snac_ctx *c, int solver) {
   });
 }
 
+// ghosted fields cross the boundary in the reference layout ((nx+2) doubles
+// per row) and live on the device with the padded row pitch
+void copy_field(snac_ctx *c, int b, int fid, double *dst_host, const double *src_host) {
+  long n;
+  double *d = field_ptr(c, b, fid, n);
+  DevBlock &D = c->blk.at(b);
+  if (fid >= 0 && fid < 6) {
+    size_t w = (size_t)(D.g.nx + 2) * sizeof(double), pitch = (size_t)D.g.sx * sizeof(double);
+    size_t rows = (size_t)(D.g.ny + 2) * (D.g.nz + 2);
+    if (src_host) SNAC_HIP(hipMemcpy2DAsync(d, pitch, src_host, w, w, rows, hipMemcpyHostToDevice, c->stream));
+    else SNAC_HIP(hipMemcpy2DAsync(dst_host, w, d, pitch, w, rows, hipMemcpyDeviceToHost, c->stream));
+  } else {
+    if (src_host) SNAC_HIP(hipMemcpyAsync(d, src_host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    else SNAC_HIP(hipMemcpyAsync(dst_host, d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  SNAC_HIP(hipStreamSynchronize(c->stream));
+}
+
 int snac_upload_field(snac_ctx *c, int b, int fid, const double *h) {
-  return guard([&] {
-    long n;
-    double *d = field_ptr(on_device(c), b, fid, n);
-    SNAC_HIP(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    SNAC_HIP(hipStreamSynchronize(c->stream));
-  });
+  return guard([&] { copy_field(on_device(c), b, fid, nullptr, h); });
 }
 
 int snac_download_field(snac_ctx *c, int b, int fid, double *h) {
-  return guard([&] {
-    long n;
-    double *d = field_ptr(on_device(c), b, fid, n);
-    SNAC_HIP(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    SNAC_HIP(hipStreamSynchronize(c->stream));
-  });
+  return guard([&] { copy_field(on_device(c), b, fid, h, nullptr); });
 }
 
 int snac_rk_mom(snac_ctx *c, int irk, double dt) {
@@ -756,6 +859,21 @@ int snac_get_timing(snac_ctx *c, double *ms, int *count) {
     }
     std::fill(c->tacc, c->tacc + SNAC_NPHASE, 0.0);
     std::fill(c->tcnt, c->tcnt + SNAC_NPHASE, 0);
+  });
+}
+
+int snac_get_autotune(snac_ctx *c, int *choices, double *ms, int n) {
+  return guard([&] {
+    on_device(c);
+    const TensorPlan &t = c->tp;
+    std::vector<int> ch;
+    for (auto &p : t.fwd) ch.push_back(p.fft ? p.lshift : -1);
+    for (auto &p : t.bwd) ch.push_back(p.fft ? p.lshift : -1);
+    ch.push_back(t.tvariant);
+    for (int q = 0; q < n; q++) {
+      if (choices) choices[q] = q < (int)ch.size() ? ch[q] : -2;
+      if (ms) ms[q] = q < (int)t.tune_ms.size() ? t.tune_ms[q] : 0.0;
+    }
   });
 }
 

@@ -1,16 +1,17 @@
 // Batched real-to-real transforms along one axis, the kinds FFTW offers and
 // SNaC selects by boundary condition (fft.f90:96-149): R2HC/HC2R (periodic),
 // REDFT10/REDFT01 (Neumann-Neumann), RODFT10/RODFT01 (Dirichlet-Dirichlet);
-// REDFT11/RODFT11 (mixed) and any length that is not a power of two go
-// through a dense matrix path.  Unnormalised, as FFTW's; callers scale.
+// REDFT11/RODFT11 (mixed) and lengths that are not powers of two go through
+// a dense matrix path.  Unnormalised, as FFTW's; callers scale.
 //
-// FFT path: a workgroup owns L whole lines in LDS as complex values, the
-// n-point sequence of the kind's pre-mapping (Makhoul's reordering for the
-// cosine/sine kinds, halfcomplex unpacking for HC2R) is written straight
-// from global memory into bit-reversed slots, radix-2 DIT stages run in LDS
-// and the kind's post-mapping is applied while storing.  Loads and stores
-// walk the batch index with unit stride when the line stride is not 1, so
-// every row is read as one coalesced segment.
+// FFT path (n = 4..1024, powers of two): a workgroup stages L whole lines in
+// LDS, forms the kind's n-point complex sequence (halfcomplex unpacking for
+// HC2R, Makhoul's even/odd reordering for the cosine and sine kinds), and
+// runs a two-pass Cooley-Tukey split n = N1*N2: every thread does one N1- or
+// N2-point DFT entirely in registers, with one LDS transpose in between
+// instead of log2(n) radix-2 stages through LDS.  Loads and stores walk the
+// batch index with unit stride when the line stride is not 1, so a row of L
+// neighbouring lines is one coalesced segment.
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -39,108 +40,187 @@ struct Args {
 
 __device__ __forceinline__ int brev(int x, int logn) { return (int)(__brev((unsigned)x) >> (32 - logn)); }
 
+// lines per workgroup for an N1 x N2 FFT: one task per thread in each pass
+// and at most LDS_COMPLEX complex values of LDS
+constexpr int fft_lines(int n1, int n2) {
+  return (NT / (n1 > n2 ? n1 : n2)) < (LDS_COMPLEX / (n1 * n2)) ? (NT / (n1 > n2 ? n1 : n2))
+                                                                : (LDS_COMPLEX / (n1 * n2));
+}
+
 __device__ __forceinline__ void elem_of(const Args &a, int e, int &line, int &q) {
   if (a.coal) { line = e % a.L; q = e / a.L; }
   else { q = e % a.n; line = e / a.n; }
 }
 
+// radix-2 DIT on N values held in registers (fully unrolled; twiddle
+// indices are compile-time, so the loads are uniform across the wavefront).
+// tw[m] = e^{-2 pi i m/n}; stride = n/N maps W_N^m onto that table.
+template <int N>
+__device__ __forceinline__ void fft_reg(double2 (&v)[N], const double2 *__restrict__ tw, int stride, bool inv) {
+  constexpr int LOG = (N >= 2) + (N >= 4) + (N >= 8) + (N >= 16) + (N >= 32);
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    int j = 0;
+#pragma unroll
+    for (int b = 0; b < LOG; b++) j |= ((i >> b) & 1) << (LOG - 1 - b);
+    if (j > i) { double2 t = v[i]; v[i] = v[j]; v[j] = t; }
+  }
+#pragma unroll
+  for (int len = 2; len <= N; len <<= 1) {
+#pragma unroll
+    for (int i = 0; i < N; i += len) {
+#pragma unroll
+      for (int j = 0; j < len / 2; j++) {
+        double2 w = tw[j * (N / len) * stride];
+        if (inv) w.y = -w.y;
+        double2 x0 = v[i + j], x1 = v[i + j + len / 2];
+        double2 c = make_double2(x1.x * w.x - x1.y * w.y, x1.x * w.y + x1.y * w.x);
+        v[i + j] = make_double2(x0.x + c.x, x0.y + c.y);
+        v[i + j + len / 2] = make_double2(x0.x - c.x, x0.y - c.y);
+      }
+    }
+  }
+}
+
+// complex input slot s of the n-point sequence for KIND, from the raw line r
+// (raw index n maps to zero where the kind needs x_n).  tk[k] = (cos, sin)(pi k / 2n).
 template <int KIND>
+__device__ __forceinline__ double2 pre_slot(const double *r, int s, int n, const double2 *tk) {
+  const int h = n >> 1;
+  if (KIND == K_R2HC) return make_double2(r[s], 0.0);
+  if (KIND == K_HC2R) {
+    if (s == 0 || s == h) return make_double2(r[s], 0.0);
+    if (s < h) return make_double2(r[s], r[n - s]);
+    return make_double2(r[n - s], -r[s]);
+  }
+  if (KIND == K_REDFT10 || KIND == K_RODFT10) {
+    int q = (s < h) ? 2 * s : 2 * (n - 1 - s) + 1;
+    double x = r[q];
+    if (KIND == K_RODFT10 && (q & 1)) x = -x;
+    return make_double2(x, 0.0);
+  }
+  // REDFT01 / RODFT01: e^{i pi s/(2n)} (a - i b)
+  double a, b;
+  if (KIND == K_REDFT01) { a = r[s]; b = (s == 0) ? 0.0 : r[n - s]; }
+  else { a = r[n - 1 - s]; b = (s == 0) ? 0.0 : r[s - 1]; }
+  double2 t = tk[s];
+  return make_double2(a * t.x + b * t.y, a * t.y - b * t.x);
+}
+
+template <int KIND>
+__device__ __forceinline__ double post_val(const double2 *z, int q, int n, const double2 *tk) {
+  if (KIND == K_R2HC) return (q <= (n >> 1)) ? z[q].x : z[n - q].y;
+  if (KIND == K_HC2R) return z[q].x;
+  if (KIND == K_REDFT10 || KIND == K_RODFT10) {
+    int k = (KIND == K_RODFT10) ? n - 1 - q : q;
+    double2 t = tk[k], v = z[k];
+    return 2.0 * (v.x * t.x + v.y * t.y);
+  }
+  int slot = (q & 1) ? n - 1 - (q >> 1) : (q >> 1);
+  double y = z[slot].x;
+  if (KIND == K_RODFT01 && (q & 1)) y = -y;
+  return y;
+}
+
+// n = N1*N2 points per line; j = N2*j1 + j2, k = k1 + N1*k2:
+//   Y[j2][k1] = sum_j1 z[N2 j1 + j2] W_N1^{j1 k1}, times W_n^{j2 k1}
+//   Z[k1 + N1 k2] = sum_j2 Y[j2][k1] W_N2^{j2 k2}
+// One task per thread in each pass (the host sizes L so that L*N1 and L*N2
+// do not exceed the workgroup).  Y is stored row k1 with its N2 entries
+// XOR-swizzled by k1, so both the pass-1 writes (consecutive j2) and the
+// pass-2 reads (consecutive k1) spread over the LDS banks.  All IT loads of
+// a thread are issued before its first LDS write.
+template <int N1, int N2, int KIND, int LSHIFT>
 __global__ __launch_bounds__(NT) void k_r2r_fft(Args a) {
   extern __shared__ double2 S[];
-  const int n = a.n, logn = a.logn, L = a.L;
+  constexpr int n = N1 * N2;
+  constexpr int L = (fft_lines(N1, N2) >> LSHIFT) > 0 ? (fft_lines(N1, N2) >> LSHIFT) : 1;
+  constexpr int IT = (L * n + NT - 1) / NT;  // elements per thread in the load / store phases
   const int line0 = blockIdx.x * L;
-  const double2 *tw = reinterpret_cast<const double2 *>(a.tab);          // e^{-2 pi i j/n}, j < n/2
-  const double2 *tk = reinterpret_cast<const double2 *>(a.tab + n);      // e^{-i pi k/(2n)}, k < n
-  // ---- pre-mapping: global -> bit-reversed complex slots ----
-  for (int e = threadIdx.x; e < L * n; e += NT) {
-    int l, q;
-    elem_of(a, e, l, q);
-    int line = line0 + l;
-    if (line >= a.nlines) continue;
-    int b1 = line % a.nb1, b2 = line / a.nb1;
-    double x = a.in[q * a.ies + b1 * a.is1 + b2 * a.is2];
-    double2 *s = S + l * n;
-    if (KIND == K_R2HC) {
-      s[brev(q, logn)] = make_double2(x, 0.0);
-    } else if (KIND == K_HC2R) {
-      const int h = n >> 1;
-      if (q == 0 || q == h) s[brev(q, logn)] = make_double2(x, 0.0);
-      else if (q < h) { s[brev(q, logn)].x = x; s[brev(n - q, logn)].x = x; }
-      else { s[brev(n - q, logn)].y = x; s[brev(q, logn)].y = -x; }
-    } else if (KIND == K_REDFT10 || KIND == K_RODFT10) {
-      if (KIND == K_RODFT10 && (q & 1)) x = -x;
-      int slot = (q & 1) ? n - 1 - (q >> 1) : (q >> 1);
-      s[brev(slot, logn)] = make_double2(x, 0.0);
-    } else {  // REDFT01 / RODFT01: slot k = e^{i pi k/(2n)} (x_k - i x_{n-k}); first term
-      int k = (KIND == K_RODFT01) ? n - 1 - q : q;
-      double2 t = tk[k];  // (cos, -sin)
-      s[brev(k, logn)] = make_double2(t.x * x, -t.y * x);
+  const double2 *tw = reinterpret_cast<const double2 *>(a.tab);      // e^{-2 pi i m/n}, m < n
+  const double2 *tk = reinterpret_cast<const double2 *>(a.tab) + n;  // (cos, sin)(pi k/2n)
+  constexpr bool inv = (KIND == K_HC2R || KIND == K_REDFT01 || KIND == K_RODFT01);
+  double *R = reinterpret_cast<double *>(S);
+  // 1. raw lines into LDS, line l at doubles [2 n l, 2 n l + n)
+  {
+    double xr[IT];
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+      const int e = threadIdx.x + it * NT;
+      const int l = a.coal ? e % L : e / n, q = a.coal ? e / L : e % n;
+      const int line = line0 + l;
+      xr[it] = 0.0;
+      if (e < L * n && line < a.nlines) {
+        const int b1 = line % a.nb1, b2 = line / a.nb1;
+        xr[it] = a.in[q * a.ies + b1 * a.is1 + b2 * a.is2];
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+      const int e = threadIdx.x + it * NT;
+      const int l = a.coal ? e % L : e / n, q = a.coal ? e / L : e % n;
+      if (e < L * n) R[2 * n * l + q] = xr[it];
     }
   }
   __syncthreads();
-  if (KIND == K_REDFT01 || KIND == K_RODFT01) {  // second term into slot n-k
-    for (int e = threadIdx.x; e < L * n; e += NT) {
-      int l, q;
-      elem_of(a, e, l, q);
-      int line = line0 + l;
-      if (line >= a.nlines) continue;
-      int k = (KIND == K_RODFT01) ? n - 1 - q : q;
-      if (k == 0) continue;
-      int b1 = line % a.nb1, b2 = line / a.nb1;
-      double x = a.in[q * a.ies + b1 * a.is1 + b2 * a.is2];
-      int m = n - k;
-      double2 t = tk[m];  // e^{i pi m/(2n)} * (-i x) = (sin x, -cos x)
-      double2 *s = S + l * n;
-      double2 v = s[brev(m, logn)];
-      v.x += -t.y * x;
-      v.y += -t.x * x;
-      s[brev(m, logn)] = v;
+  // 2. first pass: N1-point DFTs over j1 for every (line, j2)
+  {
+    const int t = threadIdx.x;
+    const bool act = t < L * N2;
+    const int j2 = t % N2, l = t / N2;
+    double2 v[N1];
+    if (act) {
+      const double *r = R + 2 * n * l;
+#pragma unroll
+      for (int j1 = 0; j1 < N1; j1++) v[j1] = pre_slot<KIND>(r, N2 * j1 + j2, n, tk);
+      fft_reg<N1>(v, tw, N2, inv);
+#pragma unroll
+      for (int k1 = 1; k1 < N1; k1++) {
+        double2 w = tw[(j2 * k1) & (n - 1)];
+        if (inv) w.y = -w.y;
+        double2 y = v[k1];
+        v[k1] = make_double2(y.x * w.x - y.y * w.y, y.x * w.y + y.y * w.x);
+      }
+    }
+    __syncthreads();
+    if (act) {
+      double2 *T = S + n * l;
+#pragma unroll
+      for (int k1 = 0; k1 < N1; k1++) T[k1 * N2 + (j2 ^ (k1 & (N2 - 1)))] = v[k1];
     }
     __syncthreads();
   }
-  // ---- radix-2 DIT stages ----
-  const bool inv = (KIND == K_HC2R || KIND == K_REDFT01 || KIND == K_RODFT01);
-  const int hn = n >> 1;
-  for (int ls = 1; ls <= logn; ls++) {
-    const int half = 1 << (ls - 1);
-    const int tshift = logn - ls;
-    for (int b = threadIdx.x; b < L * hn; b += NT) {
-      int l = b / hn, r = b - l * hn;
-      int grp = r >> (ls - 1), jj = r & (half - 1);
-      int i0 = (grp << ls) + jj, i1 = i0 + half;
-      double2 w = tw[jj << tshift];
-      if (inv) w.y = -w.y;
-      double2 *s = S + l * n;
-      double2 x0 = s[i0], x1 = s[i1];
-      double2 c = make_double2(x1.x * w.x - x1.y * w.y, x1.x * w.y + x1.y * w.x);
-      s[i0] = make_double2(x0.x + c.x, x0.y + c.y);
-      s[i1] = make_double2(x0.x - c.x, x0.y - c.y);
+  // 3. second pass: N2-point DFTs over j2 for every (line, k1)
+  {
+    const int t = threadIdx.x;
+    const bool act = t < L * N1;
+    const int k1 = t % N1, l = t / N1;
+    double2 v[N2];
+    if (act) {
+      const double2 *T = S + n * l;
+#pragma unroll
+      for (int j2 = 0; j2 < N2; j2++) v[j2] = T[k1 * N2 + (j2 ^ (k1 & (N2 - 1)))];
+      fft_reg<N2>(v, tw, N1, inv);
+    }
+    __syncthreads();
+    if (act) {
+      double2 *Z = S + n * l;
+#pragma unroll
+      for (int k2 = 0; k2 < N2; k2++) Z[k1 + N1 * k2] = v[k2];
     }
     __syncthreads();
   }
-  // ---- post-mapping and store ----
-  for (int e = threadIdx.x; e < L * n; e += NT) {
-    int l, q;
-    elem_of(a, e, l, q);
-    int line = line0 + l;
-    if (line >= a.nlines) continue;
-    const double2 *s = S + l * n;
-    double y;
-    if (KIND == K_R2HC) {
-      y = (q <= hn) ? s[q].x : s[n - q].y;
-    } else if (KIND == K_HC2R) {
-      y = s[q].x;
-    } else if (KIND == K_REDFT10 || KIND == K_RODFT10) {
-      int k = (KIND == K_RODFT10) ? n - 1 - q : q;
-      double2 t = tk[k], z = s[k];
-      y = 2.0 * (z.x * t.x - z.y * t.y);
-    } else {
-      int slot = (q & 1) ? n - 1 - (q >> 1) : (q >> 1);
-      y = s[slot].x;
-      if (KIND == K_RODFT01 && (q & 1)) y = -y;
+  // 4. post-mapping and store
+#pragma unroll
+  for (int it = 0; it < IT; it++) {
+    const int e = threadIdx.x + it * NT;
+    const int l = a.coal ? e % L : e / n, q = a.coal ? e / L : e % n;
+    const int line = line0 + l;
+    if (e < L * n && line < a.nlines) {
+      const double y = post_val<KIND>(S + n * l, q, n, tk);
+      const int b1 = line % a.nb1, b2 = line / a.nb1;
+      a.out[q * a.oes + b1 * a.os1 + b2 * a.os2] = y * a.scale;
     }
-    int b1 = line % a.nb1, b2 = line / a.nb1;
-    a.out[q * a.oes + b1 * a.os1 + b2 * a.os2] = y * a.scale;
   }
 }
 
@@ -168,6 +248,52 @@ __global__ __launch_bounds__(NT) void k_r2r_dense(Args a) {
     for (int j = 0; j < n; j++) y += row[j] * x[j];
     int b1 = line % a.nb1, b2 = line / a.nb1;
     a.out[q * a.oes + b1 * a.os1 + b2 * a.os2] = y * a.scale;
+  }
+}
+
+// n = N1 * N2 with both factors as even as possible
+void fft_split(int n, int &n1, int &n2) {
+  int lg = 0;
+  while ((1 << lg) < n) lg++;
+  n2 = 1 << (lg / 2);
+  n1 = n / n2;
+}
+
+template <int N1, int N2, int LS>
+void launch_sized_l(int kind, dim3 grid, size_t lds, hipStream_t s, const Args &a) {
+  switch (kind) {
+  case K_R2HC: hipLaunchKernelGGL((k_r2r_fft<N1, N2, K_R2HC, LS>), grid, dim3(NT), lds, s, a); break;
+  case K_HC2R: hipLaunchKernelGGL((k_r2r_fft<N1, N2, K_HC2R, LS>), grid, dim3(NT), lds, s, a); break;
+  case K_REDFT10: hipLaunchKernelGGL((k_r2r_fft<N1, N2, K_REDFT10, LS>), grid, dim3(NT), lds, s, a); break;
+  case K_REDFT01: hipLaunchKernelGGL((k_r2r_fft<N1, N2, K_REDFT01, LS>), grid, dim3(NT), lds, s, a); break;
+  case K_RODFT10: hipLaunchKernelGGL((k_r2r_fft<N1, N2, K_RODFT10, LS>), grid, dim3(NT), lds, s, a); break;
+  case K_RODFT01: hipLaunchKernelGGL((k_r2r_fft<N1, N2, K_RODFT01, LS>), grid, dim3(NT), lds, s, a); break;
+  default: throw std::runtime_error("r2r: kind has no FFT path");
+  }
+}
+
+template <int N1, int N2>
+void launch_sized(int kind, int lshift, dim3 grid, size_t lds, hipStream_t s, const Args &a) {
+  switch (lshift) {
+  case 0: launch_sized_l<N1, N2, 0>(kind, grid, lds, s, a); break;
+  case 1: launch_sized_l<N1, N2, 1>(kind, grid, lds, s, a); break;
+  case 2: launch_sized_l<N1, N2, 2>(kind, grid, lds, s, a); break;
+  default: throw std::runtime_error("r2r: bad tile choice");
+  }
+}
+
+void launch_fft_kind(int kind, int n, int ls, dim3 grid, size_t lds, hipStream_t s, const Args &a) {
+  switch (n) {
+  case 4: launch_sized<2, 2>(kind, ls, grid, lds, s, a); break;
+  case 8: launch_sized<4, 2>(kind, ls, grid, lds, s, a); break;
+  case 16: launch_sized<4, 4>(kind, ls, grid, lds, s, a); break;
+  case 32: launch_sized<8, 4>(kind, ls, grid, lds, s, a); break;
+  case 64: launch_sized<8, 8>(kind, ls, grid, lds, s, a); break;
+  case 128: launch_sized<16, 8>(kind, ls, grid, lds, s, a); break;
+  case 256: launch_sized<16, 16>(kind, ls, grid, lds, s, a); break;
+  case 512: launch_sized<32, 16>(kind, ls, grid, lds, s, a); break;
+  case 1024: launch_sized<32, 32>(kind, ls, grid, lds, s, a); break;
+  default: throw std::runtime_error("r2r: no FFT path for length " + std::to_string(n));
   }
 }
 
@@ -211,18 +337,18 @@ R2RPlan r2r_plan(int kind, int n) {
   R2RPlan pl;
   pl.n = n;
   pl.kind = kind;
-  pl.fft = fft_kind(kind) && n >= 4 && (n & (n - 1)) == 0 && n <= LDS_COMPLEX;
+  pl.fft = fft_kind(kind) && n >= 4 && (n & (n - 1)) == 0 && n <= 1024;
   std::vector<double> h;
   const double pi = std::acos(-1.0);
   if (pl.fft) {
-    h.resize(3 * (size_t)n);
-    for (int j = 0; j < n / 2; j++) {
-      h[2 * j] = std::cos(-2.0 * pi * j / n);
-      h[2 * j + 1] = std::sin(-2.0 * pi * j / n);
+    h.resize(4 * (size_t)n);
+    for (int m = 0; m < n; m++) {
+      h[2 * m] = std::cos(-2.0 * pi * m / n);
+      h[2 * m + 1] = std::sin(-2.0 * pi * m / n);
     }
     for (int k = 0; k < n; k++) {
-      h[n + 2 * k] = std::cos(-pi * k / (2.0 * n));
-      h[n + 2 * k + 1] = std::sin(-pi * k / (2.0 * n));
+      h[2 * n + 2 * k] = std::cos(pi * k / (2.0 * n));
+      h[2 * n + 2 * k + 1] = std::sin(pi * k / (2.0 * n));
     }
   } else {
     if (n > 2048) throw std::runtime_error("r2r: length " + std::to_string(n) + " needs the FFT path (power of two)");
@@ -249,27 +375,23 @@ void launch_r2r(hipStream_t s, const R2RPlan &pl, const double *in, long ies, lo
   a.nb1 = nb1;
   a.nlines = nb1 * nb2;
   a.coal = (ies != 1) ? 1 : 0;
-  int per = pl.fft ? LDS_COMPLEX : 2 * LDS_COMPLEX;  // lines per workgroup from the LDS budget
-  int L = per / pl.n;
-  if (L < 1) L = 1;
-  if (L > a.nlines) L = a.nlines;
-  a.L = L;
-  a.scale = scale;
-  a.tab = pl.tab;
   if (a.nlines <= 0) return;
-  dim3 grid((a.nlines + L - 1) / L);
   if (pl.fft) {
+    int n1 = 0, n2 = 0;
+    fft_split(pl.n, n1, n2);
+    const int L = std::max(1, fft_lines(n1, n2) >> pl.lshift);  // the kernel's compile-time line count
+    a.L = L;
+    a.scale = scale;
+    a.tab = pl.tab;
+    dim3 grid((unsigned)((a.nlines + L - 1) / L));
     size_t lds = (size_t)L * pl.n * sizeof(double2);
-    switch (pl.kind) {
-    case K_R2HC: hipLaunchKernelGGL(k_r2r_fft<K_R2HC>, grid, dim3(NT), lds, s, a); break;
-    case K_HC2R: hipLaunchKernelGGL(k_r2r_fft<K_HC2R>, grid, dim3(NT), lds, s, a); break;
-    case K_REDFT10: hipLaunchKernelGGL(k_r2r_fft<K_REDFT10>, grid, dim3(NT), lds, s, a); break;
-    case K_REDFT01: hipLaunchKernelGGL(k_r2r_fft<K_REDFT01>, grid, dim3(NT), lds, s, a); break;
-    case K_RODFT10: hipLaunchKernelGGL(k_r2r_fft<K_RODFT10>, grid, dim3(NT), lds, s, a); break;
-    case K_RODFT01: hipLaunchKernelGGL(k_r2r_fft<K_RODFT01>, grid, dim3(NT), lds, s, a); break;
-    default: throw std::runtime_error("r2r: bad fft kind");
-    }
+    launch_fft_kind(pl.kind, pl.n, pl.lshift, grid, lds, s, a);
   } else {
+    int L = std::max(1, std::min(2 * LDS_COMPLEX / pl.n, a.nlines));
+    a.L = L;
+    a.scale = scale;
+    a.tab = pl.tab;
+    dim3 grid((a.nlines + L - 1) / L);
     size_t lds = (size_t)L * pl.n * sizeof(double);
     hipLaunchKernelGGL(k_r2r_dense, grid, dim3(NT), lds, s, a);
   }

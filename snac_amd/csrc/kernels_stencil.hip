@@ -13,6 +13,21 @@ constexpr int TX = 64, TY = 4;
 
 __device__ __forceinline__ long gidx(long i, long j, long k, long sx, long sy) { return i + sx * (j + sy * k); }
 
+// Tile coordinates of this workgroup in a (gx, gy, *) tile grid launched as a
+// 1D grid.  Workgroups are dispatched round-robin over the 8 XCDs; remapping
+// gives each XCD one contiguous run of tiles (a k-slab), so the i/j/k
+// neighbours a stencil re-reads sit in the same XCD's L2.  A bijection for
+// any grid size.
+__device__ __forceinline__ void tile_of(int gx, int gy, int &bx, int &by, int &bz) {
+  const int nblk = gridDim.x, bid = blockIdx.x;
+  const int xcd = bid & 7, q = bid >> 3, per = nblk >> 3, rem = nblk & 7;
+  const int lid = (xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + q;
+  bx = lid % gx;
+  const int t = lid / gx;
+  by = t % gy;
+  bz = t / gy;
+}
+
 __global__ __launch_bounds__(TX *TY) void k_rk_mom(Geo g, Metrics m, const double *__restrict__ u,
                                                     const double *__restrict__ v, const double *__restrict__ w,
                                                     const double *__restrict__ p, double *__restrict__ un,
@@ -20,32 +35,36 @@ __global__ __launch_bounds__(TX *TY) void k_rk_mom(Geo g, Metrics m, const doubl
                                                     double *__restrict__ dtu, double *__restrict__ dtv,
                                                     double *__restrict__ dtw, double f1, double f2, double f12,
                                                     double visc, double bfx, double bfy, double bfz) {
-  const int i = blockIdx.x * TX + threadIdx.x + 1;
-  const int j = blockIdx.y * TY + threadIdx.y + 1;
-  const int k = blockIdx.z + 1;
+  int bx, by, bz;
+  tile_of((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, bx, by, bz);
+  const int i = bx * TX + threadIdx.x + 1;
+  const int j = by * TY + threadIdx.y + 1;
+  const int k = bz + 1;
   if (i > g.nx || j > g.ny) return;
   const long sx = g.sx, sy = g.sy;
   const long c = gidx(i, j, k, sx, sy);
   const long xp = c + 1, xm = c - 1, yp = c + sx, ym = c - sx, zp = c + sx * sy, zm = c - sx * sy;
-  const double dxci = m.dxc[i], dxcim = m.dxc[i - 1], dxfi = m.dxf[i], dxfip = m.dxf[i + 1];
-  const double dycj = m.dyc[j], dycjm = m.dyc[j - 1], dyfj = m.dyf[j], dyfjp = m.dyf[j + 1];
-  const double dzck = m.dzc[k], dzckm = m.dzc[k - 1], dzfk = m.dzf[k], dzfkp = m.dzf[k + 1];
+  const double dxfi = m.dxf[i], dxfip = m.dxf[i + 1], dyfj = m.dyf[j], dyfjp = m.dyf[j + 1];
+  const double dzfk = m.dzf[k], dzfkp = m.dzf[k + 1];
+  const double ixci = m.idxc[i], ixcim = m.idxc[i - 1], ixfi = m.idxf[i], ixfip = m.idxf[i + 1];
+  const double iycj = m.idyc[j], iycjm = m.idyc[j - 1], iyfj = m.idyf[j], iyfjp = m.idyf[j + 1];
+  const double izck = m.idzc[k], izckm = m.idzc[k - 1], izfk = m.idzf[k], izfkp = m.idzf[k + 1];
   const double uc = u[c], vc = v[c], wc = w[c];
   double a, b, cc;
   // diffusion, momx_d / momy_d / momz_d (mom.f90:221-327)
   {
-    double dudxp = (u[xp] - uc) / dxfip, dudxm = (uc - u[xm]) / dxfi;
-    double dudyp = (u[yp] - uc) / dycj, dudym = (uc - u[ym]) / dycjm;
-    double dudzp = (u[zp] - uc) / dzck, dudzm = (uc - u[zm]) / dzckm;
-    a = visc * (dudxp - dudxm) / dxci + visc * (dudyp - dudym) / dyfj + visc * (dudzp - dudzm) / dzfk;
-    double dvdxp = (v[xp] - vc) / dxci, dvdxm = (vc - v[xm]) / dxcim;
-    double dvdyp = (v[yp] - vc) / dyfjp, dvdym = (vc - v[ym]) / dyfj;
-    double dvdzp = (v[zp] - vc) / dzck, dvdzm = (vc - v[zm]) / dzckm;
-    b = visc * (dvdxp - dvdxm) / dxfi + visc * (dvdyp - dvdym) / dycj + visc * (dvdzp - dvdzm) / dzfk;
-    double dwdxp = (w[xp] - wc) / dxci, dwdxm = (wc - w[xm]) / dxcim;
-    double dwdyp = (w[yp] - wc) / dycj, dwdym = (wc - w[ym]) / dycjm;
-    double dwdzp = (w[zp] - wc) / dzfkp, dwdzm = (wc - w[zm]) / dzfk;
-    cc = visc * (dwdxp - dwdxm) / dxfi + visc * (dwdyp - dwdym) / dyfj + visc * (dwdzp - dwdzm) / dzck;
+    double dudxp = (u[xp] - uc) * ixfip, dudxm = (uc - u[xm]) * ixfi;
+    double dudyp = (u[yp] - uc) * iycj, dudym = (uc - u[ym]) * iycjm;
+    double dudzp = (u[zp] - uc) * izck, dudzm = (uc - u[zm]) * izckm;
+    a = visc * (dudxp - dudxm) * ixci + visc * (dudyp - dudym) * iyfj + visc * (dudzp - dudzm) * izfk;
+    double dvdxp = (v[xp] - vc) * ixci, dvdxm = (vc - v[xm]) * ixcim;
+    double dvdyp = (v[yp] - vc) * iyfjp, dvdym = (vc - v[ym]) * iyfj;
+    double dvdzp = (v[zp] - vc) * izck, dvdzm = (vc - v[zm]) * izckm;
+    b = visc * (dvdxp - dvdxm) * ixfi + visc * (dvdyp - dvdym) * iycj + visc * (dvdzp - dvdzm) * izfk;
+    double dwdxp = (w[xp] - wc) * ixci, dwdxm = (wc - w[xm]) * ixcim;
+    double dwdyp = (w[yp] - wc) * iycj, dwdym = (wc - w[ym]) * iycjm;
+    double dwdzp = (w[zp] - wc) * izfkp, dwdzm = (wc - w[zm]) * izfk;
+    cc = visc * (dwdxp - dwdxm) * ixfi + visc * (dwdyp - dwdym) * iyfj + visc * (dwdzp - dwdzm) * izck;
   }
   // volume-weighted advection, momx/y/z_a_vv (mom.f90:112-219)
   {
@@ -55,21 +74,21 @@ __global__ __launch_bounds__(TX *TY) void k_rk_mom(Geo g, Metrics m, const doubl
     double uvjm = 0.25 * (v[ym] * dxfi * dzfk + v[ym + 1] * dxfip * dzfk) * u[ym];
     double uwkp = 0.25 * (wc * dxfi * dyfj + w[xp] * dxfip * dyfj) * u[zp];
     double uwkm = 0.25 * (w[zm] * dxfi * dyfj + w[zm + 1] * dxfip * dyfj) * u[zm];
-    a = a + ((-uuip + uuim) + (-uvjp + uvjm) + (-uwkp + uwkm)) / (dxci * dyfj * dzfk);
+    a = a + ((-uuip + uuim) + (-uvjp + uvjm) + (-uwkp + uwkm)) * (ixci * iyfj * izfk);
     double uvip = 0.25 * (uc * dyfj * dzfk + u[yp] * dyfjp * dzfk) * v[xp];
     double uvim = 0.25 * (u[xm] * dyfj * dzfk + u[xm + sx] * dyfjp * dzfk) * v[xm];
     double vvjp = 0.25 * (vc * dxfi * dzfk + v[yp] * dxfi * dzfk) * v[yp];
     double vvjm = 0.25 * (vc * dxfi * dzfk + v[ym] * dxfi * dzfk) * v[ym];
     double wvkp = 0.25 * (wc * dyfj * dxfi + w[yp] * dyfjp * dxfi) * v[zp];
     double wvkm = 0.25 * (w[zm] * dyfj * dxfi + w[zm + sx] * dyfjp * dxfi) * v[zm];
-    b = b + ((-uvip + uvim) + (-vvjp + vvjm) + (-wvkp + wvkm)) / (dxfi * dycj * dzfk);
+    b = b + ((-uvip + uvim) + (-vvjp + vvjm) + (-wvkp + wvkm)) * (ixfi * iycj * izfk);
     double uwip = 0.25 * (uc * dzfk * dyfj + u[zp] * dzfkp * dyfj) * w[xp];
     double uwim = 0.25 * (u[xm] * dzfk * dyfj + u[xm + sx * sy] * dzfkp * dyfj) * w[xm];
     double vwjp = 0.25 * (vc * dzfk * dxfi + v[zp] * dzfkp * dxfi) * w[yp];
     double vwjm = 0.25 * (v[ym] * dzfk * dxfi + v[ym + sx * sy] * dzfkp * dxfi) * w[ym];
     double wwkp = 0.25 * (wc * dxfi * dyfj + w[zp] * dxfi * dyfj) * w[zp];
     double wwkm = 0.25 * (wc * dxfi * dyfj + w[zm] * dxfi * dyfj) * w[zm];
-    cc = cc + ((-uwip + uwim) + (-vwjp + vwjm) + (-wwkp + wwkm)) / (dxfi * dyfj * dzck);
+    cc = cc + ((-uwip + uwim) + (-vwjp + vwjm) + (-wwkp + wwkm)) * (ixfi * iyfj * izck);
   }
   // RK3 update (rk.f90:68-85), then pressure gradient + body force (86-111)
   const long q = (long)(i - 1) + (long)g.nx * ((long)(j - 1) + (long)g.ny * (long)(k - 1));
@@ -80,9 +99,9 @@ __global__ __launch_bounds__(TX *TY) void k_rk_mom(Geo g, Metrics m, const doubl
   dtv[q] = b;
   dtw[q] = cc;
   const double pc = p[c];
-  un_ = un_ + f12 * (-(p[xp] - pc) / dxci + bfx);
-  vn_ = vn_ + f12 * (-(p[yp] - pc) / dycj + bfy);
-  wn_ = wn_ + f12 * (-(p[zp] - pc) / dzck + bfz);
+  un_ = un_ + f12 * (-(p[xp] - pc) * ixci + bfx);
+  vn_ = vn_ + f12 * (-(p[yp] - pc) * iycj + bfy);
+  wn_ = wn_ + f12 * (-(p[zp] - pc) * izck + bfz);
   un[c] = un_;
   vn[c] = vn_;
   wn[c] = wn_;
@@ -90,12 +109,14 @@ __global__ __launch_bounds__(TX *TY) void k_rk_mom(Geo g, Metrics m, const doubl
 
 __global__ __launch_bounds__(TX *TY) void k_fillps(Geo g, Metrics m, int fa, const double *__restrict__ u,
                                                     const double *__restrict__ v, const double *__restrict__ w,
-                                                    double *__restrict__ pp, double dtrk, const double *rx0,
+                                                    double *__restrict__ pp, double idtrk, const double *rx0,
                                                     const double *rx1, const double *ry0, const double *ry1,
                                                     const double *rz0, const double *rz1) {
-  const int i = blockIdx.x * TX + threadIdx.x + 1;
-  const int j = blockIdx.y * TY + threadIdx.y + 1;
-  const int k = blockIdx.z + 1;
+  int bx, by, bz;
+  tile_of((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, bx, by, bz);
+  const int i = bx * TX + threadIdx.x + 1;
+  const int j = by * TY + threadIdx.y + 1;
+  const int k = bz + 1;
   if (i > g.nx || j > g.ny) return;
   const long sx = g.sx, sy = g.sy;
   const long c = gidx(i, j, k, sx, sy);
@@ -103,12 +124,12 @@ __global__ __launch_bounds__(TX *TY) void k_fillps(Geo g, Metrics m, int fa, con
   const double dxf = m.dxf[i], dyf = m.dyf[j], dzf = m.dzf[k];
   double s;
   switch (fa) {
-  case 3: s = dw / dzf * dxf * dyf + dv * dxf + du * dyf; break;
-  case 2: s = dw * dxf + dv / dyf * dxf * dzf + du * dzf; break;
-  case 1: s = dw * dyf + dv * dzf + du / dxf * dyf * dzf; break;
+  case 3: s = dw * m.idzf[k] * dxf * dyf + dv * dxf + du * dyf; break;
+  case 2: s = dw * dxf + dv * m.idyf[j] * dxf * dzf + du * dzf; break;
+  case 1: s = dw * dyf + dv * dzf + du * m.idxf[i] * dyf * dzf; break;
   default: s = dw * dxf * dyf + dv * dxf * dzf + du * dyf * dzf; break;
   }
-  double r = s / dtrk;
+  double r = s * idtrk;
   // updt_rhs, in the reference's order x0,x1,y0,y1,z0,z1
   if (rx0 && i == 1) r += rx0[(j - 1) + (long)g.ny * (k - 1)];
   if (rx1 && i == g.nx) r += rx1[(j - 1) + (long)g.ny * (k - 1)];
@@ -126,17 +147,19 @@ __global__ __launch_bounds__(TX *TY) void k_correc(Geo g, Metrics m, const doubl
                                                     const double *__restrict__ w, double *__restrict__ un,
                                                     double *__restrict__ vn, double *__restrict__ wn, double dtrk,
                                                     double *__restrict__ p) {
-  const int i = blockIdx.x * TX + threadIdx.x;
-  const int j = blockIdx.y * TY + threadIdx.y;
-  const int k = blockIdx.z;
+  int bx, by, bz;
+  tile_of((g.nx + 2 + TX - 1) / TX, (g.ny + 2 + TY - 1) / TY, bx, by, bz);
+  const int i = bx * TX + threadIdx.x;
+  const int j = by * TY + threadIdx.y;
+  const int k = bz;
   if (i > g.nx + 1 || j > g.ny + 1) return;
   const long sx = g.sx, sy = g.sy;
   const long c = gidx(i, j, k, sx, sy);
   const double pc = pp[c];
   double uu = u[c], vv = v[c], ww = w[c];
-  if (i <= g.nx) uu = uu - dtrk * (pp[c + 1] - pc) / m.dxc[i];
-  if (j <= g.ny) vv = vv - dtrk * (pp[c + sx] - pc) / m.dyc[j];
-  if (k <= g.nz) ww = ww - dtrk * (pp[c + sx * sy] - pc) / m.dzc[k];
+  if (i <= g.nx) uu = uu - dtrk * (pp[c + 1] - pc) * m.idxc[i];
+  if (j <= g.ny) vv = vv - dtrk * (pp[c + sx] - pc) * m.idyc[j];
+  if (k <= g.nz) ww = ww - dtrk * (pp[c + sx * sy] - pc) * m.idzc[k];
   un[c] = uu;
   vn[c] = vv;
   wn[c] = ww;
@@ -246,28 +269,30 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double *psum, const 
 }
 
 dim3 interior_grid(const Geo &g) { return dim3((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, g.nz); }
-dim3 ghosted_grid(const Geo &g) { return dim3((g.nx + 2 + TX - 1) / TX, (g.ny + 2 + TY - 1) / TY, g.nz + 2); }
+dim3 flat(dim3 g) { return dim3(g.x * g.y * g.z); }
+dim3 interior_tiles(const Geo &g) { return flat(interior_grid(g)); }
+dim3 ghosted_tiles(const Geo &g) { return dim3(((g.nx + 2 + TX - 1) / TX) * ((g.ny + 2 + TY - 1) / TY) * (g.nz + 2)); }
 
 }  // namespace
 
 void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v, const double *w,
                    const double *p, double *un, double *vn, double *wn, double *dtu, double *dtv, double *dtw,
                    double f1, double f2, double f12, double visc, double bfx, double bfy, double bfz) {
-  hipLaunchKernelGGL(k_rk_mom, interior_grid(g), dim3(TX, TY), 0, s, g, m, u, v, w, p, un, vn, wn, dtu, dtv, dtw,
+  hipLaunchKernelGGL(k_rk_mom, interior_tiles(g), dim3(TX, TY), 0, s, g, m, u, v, w, p, un, vn, wn, dtu, dtv, dtw,
                      f1, f2, f12, visc, bfx, bfy, bfz);
   SNAC_HIP(hipGetLastError());
 }
 
 void launch_fillps(hipStream_t s, const Geo &g, const Metrics &m, int fa, const double *u, const double *v,
                    const double *w, double *pp, double dtrk, const double *const r[6]) {
-  hipLaunchKernelGGL(k_fillps, interior_grid(g), dim3(TX, TY), 0, s, g, m, fa, u, v, w, pp, dtrk, r[0], r[1], r[2],
+  hipLaunchKernelGGL(k_fillps, interior_tiles(g), dim3(TX, TY), 0, s, g, m, fa, u, v, w, pp, 1.0 / dtrk, r[0], r[1], r[2],
                      r[3], r[4], r[5]);
   SNAC_HIP(hipGetLastError());
 }
 
 void launch_correc(hipStream_t s, const Geo &g, const Metrics &m, const double *pp, const double *u, const double *v,
                    const double *w, double *un, double *vn, double *wn, double dtrk, double *p) {
-  hipLaunchKernelGGL(k_correc, ghosted_grid(g), dim3(TX, TY), 0, s, g, m, pp, u, v, w, un, vn, wn, dtrk, p);
+  hipLaunchKernelGGL(k_correc, ghosted_tiles(g), dim3(TX, TY), 0, s, g, m, pp, u, v, w, un, vn, wn, dtrk, p);
   SNAC_HIP(hipGetLastError());
 }
 
@@ -302,20 +327,19 @@ void launch_reduce_final(hipStream_t s, const double *psum, const double *pmax, 
 
 namespace snac {
 namespace {
-// plain copy, 16 bytes per lane per iteration: the HBM reference kernel for
-// the roofline and for scaling the FETCH_SIZE / WRITE_SIZE counters
+// plain copy, one 16-byte element per lane over an exact grid (the fastest
+// copy form measured on MI355X): the HBM reference kernel for the roofline
+// and for scaling the FETCH_SIZE / WRITE_SIZE counters
 __global__ __launch_bounds__(256) void k_stream_copy(const double2 *__restrict__ a, double2 *__restrict__ b, long n2) {
-  for (long i = blockIdx.x * 256L + threadIdx.x; i < n2; i += (long)gridDim.x * 256L) b[i] = a[i];
+  const long i = blockIdx.x * 256L + threadIdx.x;
+  if (i < n2) b[i] = a[i];
 }
 }  // namespace
 
 void launch_stream_copy(hipStream_t s, const double *a, double *b, long n) {
   long n2 = n / 2;
-  int dev = 0, ncu = 0;
-  SNAC_HIP(hipGetDevice(&dev));
-  SNAC_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  long blocks = std::min<long>((n2 + 255) / 256, (long)ncu * 16);
-  hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)blocks), dim3(256), 0, s, (const double2 *)a, (double2 *)b, n2);
+  hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, (const double2 *)a,
+                     (double2 *)b, n2);
   SNAC_HIP(hipGetLastError());
 }
 }  // namespace snac

@@ -16,8 +16,15 @@ namespace {
 constexpr int WG = 64;
 constexpr int LDS_M = 128;  // line length served from LDS (64 lines x 128 x 8 B)
 
+// Thomas algorithm, one thread per line.  The forward sweep reads the rhs
+// from x and writes d' to `dsc`, a separate array, so the loads of the next
+// U rows can be issued ahead of the dependent divisions; c' lives in LDS
+// (or in `csc` for lines longer than LDS_M).  The backward sweep reads d'
+// and c' and writes the solution back into x.
 template <bool USE_LDS>
-__global__ __launch_bounds__(WG) void k_tridiag(TridiagLine t, double *__restrict__ x, double *__restrict__ scratch) {
+__global__ __launch_bounds__(WG) void k_tridiag(TridiagLine t, double *__restrict__ x, double *__restrict__ dsc,
+                                                 double *__restrict__ csc) {
+  constexpr int U = 8;
   __shared__ double cps[USE_LDS ? WG * LDS_M : 1];
   const int s = t.s;
   const int o1 = s == 0 ? 1 : 0, o2 = s == 2 ? 1 : 2;
@@ -28,41 +35,67 @@ __global__ __launch_bounds__(WG) void k_tridiag(TridiagLine t, double *__restric
   const int t1 = (int)(line % n1), t2 = (int)(line / n1);
   const double lam = t.lamA[t1] + t.lamB[t2];
   const int mm = (t.pin_when_zero && lam == 0.0) ? m - 1 : m;
-  double *xl = x + t1 * st[o1] + t2 * st[o2];
+  const long base = t1 * st[o1] + t2 * st[o2];
   const long ss = st[s];
-  double *cp = USE_LDS ? cps + threadIdx.x : scratch + t1 * st[o1] + t2 * st[o2];
-  const long cstride = USE_LDS ? WG : ss;
-  double dprev = 0.0, cprev = 0.0;
-  for (int q = 0; q < mm; q++) {
-    const double a = t.am[q], c = t.ap[q];
-    const double b = t.d0[q] + lam * t.wt[q];
-    const double den = (q == 0) ? b : b - a * cprev;
-    const double cq = c / den;
-    const double dq = (xl[q * ss] - (q == 0 ? 0.0 : a * dprev)) / den;
-    cp[q * cstride] = cq;
-    xl[q * ss] = dq;
-    cprev = cq;
-    dprev = dq;
-  }
-  double xv = dprev;
-  for (int q = mm - 2; q >= 0; q--) {
-    xv = xl[q * ss] - cp[q * cstride] * xv;
-    xl[q * ss] = xv;
+  double *xl = x + base;
+  double *dl = dsc + base;
+  double *cp = USE_LDS ? cps + threadIdx.x : csc + base;
+  const long cs = USE_LDS ? WG : ss;
+  double cprev = 0.0, dprev = 0.0;
+  for (int q0 = 0; q0 < mm; q0 += U) {
+    double xv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) xv[u] = (q0 + u < mm) ? xl[(q0 + u) * ss] : 0.0;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int q = q0 + u;
+      if (q < mm) {
+        const double a = t.am[q], c = t.ap[q];
+        const double b = t.d0[q] + lam * t.wt[q];
+        const double den = (q == 0) ? b : b - a * cprev;
+        const double cq = c / den;
+        const double dq = (xv[u] - (q == 0 ? 0.0 : a * dprev)) / den;
+        cp[q * cs] = cq;
+        dl[q * ss] = dq;
+        cprev = cq;
+        dprev = dq;
+      }
+    }
   }
   if (mm < m) xl[(m - 1) * ss] = 0.0;
+  if (mm == 0) return;
+  double xv = dprev;
+  xl[(mm - 1) * ss] = xv;
+  for (int q0 = mm - 2; q0 >= 0; q0 -= U) {
+    double dv[U], cv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int q = q0 - u;
+      dv[u] = (q >= 0) ? dl[q * ss] : 0.0;
+      cv[u] = (q >= 0) ? cp[q * cs] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int q = q0 - u;
+      if (q >= 0) {
+        xv = dv[u] - cv[u] * xv;
+        xl[q * ss] = xv;
+      }
+    }
+  }
 }
 
 }  // namespace
 
-void launch_tridiag(hipStream_t st, const TridiagLine &t, double *x, double *scratch) {
+void launch_tridiag(hipStream_t st, const TridiagLine &t, double *x, double *dscratch, double *cscratch) {
   const int s = t.s;
   const int o1 = s == 0 ? 1 : 0, o2 = s == 2 ? 1 : 2;
   long nlines = (long)t.G[o1] * t.G[o2];
   dim3 grid((unsigned)((nlines + WG - 1) / WG));
-  if (t.G[s] <= LDS_M)
-    hipLaunchKernelGGL(k_tridiag<true>, grid, dim3(WG), 0, st, t, x, scratch);
+  if (t.G[s] <= LDS_M && t.variant == 0)
+    hipLaunchKernelGGL(k_tridiag<true>, grid, dim3(WG), 0, st, t, x, dscratch, cscratch);
   else
-    hipLaunchKernelGGL(k_tridiag<false>, grid, dim3(WG), 0, st, t, x, scratch);
+    hipLaunchKernelGGL(k_tridiag<false>, grid, dim3(WG), 0, st, t, x, dscratch, cscratch);
   SNAC_HIP(hipGetLastError());
 }
 
