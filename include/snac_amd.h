@@ -106,6 +106,10 @@ int snac_init_bounds(snac_ctx *ctx);
 int snac_step(snac_ctx *ctx, double dt);
 /* nsteps time steps with dt fixed (no host synchronisation in between)      */
 int snac_steps(snac_ctx *ctx, double dt, int nsteps);
+/* snac_step/snac_steps replay one hipGraph-captured step (default on; the
+ * graph is re-captured when dt changes).  enable = 0 launches eagerly.
+ * While phase timing is on, steps always run eagerly.                       */
+int snac_set_graph(snac_ctx *ctx, int enable);
 /* waits for the context's stream                                            */
 int snac_sync(snac_ctx *ctx);
 

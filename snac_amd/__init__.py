@@ -60,6 +60,7 @@ def _load():
     L.snac_step.argtypes = [vp, C.c_double]
     L.snac_steps.argtypes = [vp, C.c_double, C.c_int]
     L.snac_sync.argtypes = [vp]
+    L.snac_set_graph.argtypes = [vp, C.c_int]
     L.snac_r2r_lines.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
     L.snac_stream_copy.argtypes = [vp, C.c_long, C.c_int, C.POINTER(C.c_double)]
     L.snac_get_autotune.argtypes = [vp, ip, dp, C.c_int]
@@ -182,6 +183,9 @@ class Context:
 
     def steps(self, dt, n):
         _check(lib.snac_steps(self.h, dt, int(n)), "steps")
+
+    def set_graph(self, enable):
+        _check(lib.snac_set_graph(self.h, int(bool(enable))), "set_graph")
 
     def sync(self):
         _check(lib.snac_sync(self.h), "sync")

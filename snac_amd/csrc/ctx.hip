@@ -79,6 +79,11 @@ struct snac_ctx {
   std::vector<hipEvent_t> pool;
   double tacc[SNAC_NPHASE] = {};
   int tcnt[SNAC_NPHASE] = {};
+  // one captured RK3 step, replayed by snac_step / snac_steps
+  bool use_graph = true;
+  hipGraphExec_t gexec = nullptr;
+  double graph_dt = 0.0;
+  int graph_cur = -1;
 };
 
 namespace snac {
@@ -514,6 +519,45 @@ void substage(snac_ctx *c, int irk, double dt) {
   if (c->pending.size() > 4096) collect_timing(c);  // bounds the event pool in long timed runs
 }
 
+void drop_graph(snac_ctx *c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  c->gexec = nullptr;
+}
+
+// nsteps RK3 steps.  Eagerly when timing (events bracket every phase) or
+// when graphs are off; otherwise one step is captured into a hipGraph (the
+// kernels of a step address the same buffers every step: the velocity
+// parity flips six times per step) and replayed, re-captured when dt or the
+// starting parity changes.
+void run_steps(snac_ctx *c, double dt, int nsteps) {
+  check_solver(c);
+  if (!c->use_graph || c->timing) {
+    for (int s = 0; s < nsteps; s++)
+      for (int irk = 1; irk <= 3; irk++) substage(c, irk, dt);
+    return;
+  }
+  if (!c->gexec || c->graph_dt != dt || c->graph_cur != c->cur) {
+    drop_graph(c);
+    hipGraph_t g;
+    SNAC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int cur0 = c->cur;
+    try {
+      for (int irk = 1; irk <= 3; irk++) substage(c, irk, dt);
+    } catch (...) {
+      (void)hipStreamEndCapture(c->stream, &g);
+      c->cur = cur0;
+      throw;
+    }
+    SNAC_HIP(hipStreamEndCapture(c->stream, &g));
+    if (c->cur != cur0) throw std::runtime_error("internal: a step must return to the same buffer parity");
+    SNAC_HIP(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+    SNAC_HIP(hipGraphDestroy(g));
+    c->graph_dt = dt;
+    c->graph_cur = c->cur;
+  }
+  for (int s = 0; s < nsteps; s++) SNAC_HIP(hipGraphLaunch(c->gexec, c->stream));
+}
+
 double *field_ptr(snac_ctx *c, int b, int fid, long &count) {
   DevBlock &D = c->blk.at(b);
   if (fid >= 0 && fid < 6) count = D.ntot;
@@ -608,6 +652,7 @@ void create(snac_ctx *c) {
 
 void destroy(snac_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
   for (auto &r : c->pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto &e : c->pool) (void)hipEventDestroy(e);
   for (void *p : c->owned) (void)hipFree(p);
@@ -822,20 +867,16 @@ int snac_init_bounds(snac_ctx *c) {
   });
 }
 
-int snac_step(snac_ctx *c, double dt) {
-  return guard([&] {
-    on_device(c);
-    check_solver(c);
-    for (int irk = 1; irk <= 3; irk++) substage(c, irk, dt);
-  });
-}
+int snac_step(snac_ctx *c, double dt) { return guard([&] { run_steps(on_device(c), dt, 1); }); }
 
-int snac_steps(snac_ctx *c, double dt, int nsteps) {
+int snac_steps(snac_ctx *c, double dt, int nsteps) { return guard([&] { run_steps(on_device(c), dt, nsteps); }); }
+
+int snac_set_graph(snac_ctx *c, int enable) {
   return guard([&] {
     on_device(c);
-    check_solver(c);
-    for (int s = 0; s < nsteps; s++)
-      for (int irk = 1; irk <= 3; irk++) substage(c, irk, dt);
+    SNAC_HIP(hipStreamSynchronize(c->stream));
+    c->use_graph = enable != 0;
+    if (!c->use_graph) drop_graph(c);
   });
 }
 
