@@ -35,8 +35,10 @@ enum {
   SNAC_DUDTRKO = 6, SNAC_DVDTRKO = 7, SNAC_DWDTRKO = 8
 };
 
-/* Poisson solver choice for snac_ctx_set_solver */
-enum { SNAC_SOLVER_AUTO = 0, SNAC_SOLVER_TENSOR = 1 };
+/* Poisson solver choice for snac_ctx_set_solver: AUTO takes the direct
+ * tensor-product solver when the union of blocks is a box with uniform
+ * transform axes, else the batched MG-preconditioned CG (any union) */
+enum { SNAC_SOLVER_AUTO = 0, SNAC_SOLVER_TENSOR = 1, SNAC_SOLVER_MG = 2 };
 
 /* Message of the last failure on this host thread ("" if none). */
 const char *snac_last_error(void);
@@ -69,8 +71,15 @@ int snac_ctx_get_metric(const snac_ctx *ctx, int b, int which, double *out);
  *           restart, stop_type(3)] (8)                                       */
 int snac_ctx_get_params(const snac_ctx *ctx, double *dparams, int *iparams);
 /* Poisson solver selection (the reference's &hypre choice has no effect on
- * the discrete solution at tight tolerance)                                 */
+ * the discrete solution at tight tolerance; the MG-PCG path stops each
+ * mode at ||r|| <= hypre_tol ||b|| or after hypre_maxiter iterations, the
+ * &hypre parameters of param.f90:101, solver.f90:389-451)                   */
 int snac_ctx_set_solver(snac_ctx *ctx, int solver);
+/* CG iterations of the last MG-PCG pressure solve and its largest relative
+ * residual over the modes (HYPRE's StructPCGGetNumIterations /
+ * GetFinalRelativeResidualNorm, solver.f90:553-569); synchronises.  Both
+ * are 0 when the tensor-product solver ran.                                 */
+int snac_solver_stats(snac_ctx *ctx, int *iters, double *max_relres);
 
 /* host <-> device copies of one field of one block (layout above)           */
 int snac_upload_field(snac_ctx *ctx, int b, int fid, const double *host);
@@ -121,7 +130,7 @@ enum {
   SNAC_PH_BOUND = 1,    /* bounduvw / boundp / inflow ghost fills           */
   SNAC_PH_FILLPS = 2,   /* fillps + updt_rhs                                */
   SNAC_PH_R2R_FWD = 3,  /* forward transforms (FFT axis first)              */
-  SNAC_PH_TRIDIAG = 4,  /* tridiagonal line solves                          */
+  SNAC_PH_TRIDIAG = 4,  /* per-mode solves: tridiagonal lines or MG-PCG      */
   SNAC_PH_R2R_BWD = 5,  /* inverse transforms and normfft scaling           */
   SNAC_PH_CORREC = 6,   /* correc + updt_pressure                           */
   SNAC_NPHASE = 7

@@ -44,6 +44,7 @@ def _load():
     L.snac_ctx_get_metric.argtypes = [vp, C.c_int, C.c_int, dp]
     L.snac_ctx_get_params.argtypes = [vp, dp, ip]
     L.snac_ctx_set_solver.argtypes = [vp, C.c_int]
+    L.snac_solver_stats.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.snac_upload_field.argtypes = [vp, C.c_int, C.c_int, dp]
     L.snac_download_field.argtypes = [vp, C.c_int, C.c_int, dp]
     L.snac_rk_mom.argtypes = [vp, C.c_int, C.c_double]
@@ -206,6 +207,21 @@ class Context:
         cnt = np.zeros(len(PHASES), np.int32)
         _check(lib.snac_get_timing(self.h, ms, cnt), "get_timing")
         return {p: (float(ms[q]), int(cnt[q])) for q, p in enumerate(PHASES)}
+
+    SOLVERS = {"auto": 0, "tensor": 1, "mg": 2}
+
+    def set_solver(self, name):
+        """Poisson solver: 'auto' (tensor-product when the union is a box with
+        uniform transform axes, else MG-PCG), 'tensor' or 'mg'."""
+        _check(lib.snac_ctx_set_solver(self.h, self.SOLVERS[name]), "set_solver")
+
+    def solver_stats(self):
+        """(CG iterations, largest relative residual over the modes) of the
+        last MG-PCG pressure solve; (0, 0.0) after a tensor-product solve."""
+        it = C.c_int()
+        rel = C.c_double()
+        _check(lib.snac_solver_stats(self.h, C.byref(it), C.byref(rel)), "solver_stats")
+        return it.value, rel.value
 
     def stream_copy(self, n, reps=10):
         """GB/s of a plain device copy of n doubles (HBM reference)."""

@@ -12,6 +12,7 @@
 #include "bound_program.hpp"
 #include "case.hpp"
 #include "common.hpp"
+#include "mg.hpp"
 
 namespace snac {
 namespace {
@@ -67,6 +68,8 @@ struct snac_ctx {
   int cur = 0;  // parity of the current velocity buffers
   snac::Program prog_uvw[2][2], prog_bound_only[2][2], prog_inflow[2][2], prog_p, prog_pp, prog_end[2];
   snac::TensorPlan tp;
+  snac::MGPlan mg;
+  bool mg_planned = false;
   int solver = SNAC_SOLVER_AUTO;
   double *part_sum = nullptr, *part_max = nullptr, *red = nullptr;
   int nparts = 0;
@@ -263,17 +266,23 @@ void plan_tensor(snac_ctx *c) {
   for (int a : red) ((uniform(a) && transformable(a)) ? cand : non).push_back(a);
   if (non.size() > 1) { t.why = "more than one non-uniform axis"; return; }
   int s;
+  bool all_tr = false;  // every axis periodic: transformed too, diagonal "lines"
   if (non.size() == 1) s = non[0];
   else {
     s = -1;
     for (int q = (int)cand.size() - 1; q >= 0; q--)
       if (bclo[cand[q]] != 'F') { s = cand[q]; break; }
-    if (s < 0) { t.why = "no non-periodic axis for the line solves"; return; }
+    if (s < 0) {
+      if (cand.empty()) { t.why = "no axis for the line solves"; return; }
+      s = cand.back();
+      all_tr = true;
+    }
   }
-  if (bclo[s] == 'F') { t.why = "periodic line axis"; return; }
+  if (bclo[s] == 'F' && !all_tr) { t.why = "periodic line axis"; return; }
   t.s = s;
   if (fa >= 0) t.xaxes.push_back(fa);
   for (int a : red) if (a != s) t.xaxes.push_back(a);
+  if (all_tr) t.xaxes.push_back(s);
   // the first transform runs block by block from pp into the work array, so
   // its axis must be spanned by every block (always true for the FFT axis)
   {
@@ -316,16 +325,23 @@ void plan_tensor(snac_ctx *c) {
   std::vector<double> am(m), ap(m), d0(m), wt(m);
   double sg0 = bclo[s] == 'N' ? 1.0 : -1.0, sg1 = bchi[s] == 'N' ? 1.0 : -1.0;
   for (int g = 0; g < m; g++) {
+    wt[g] = hother * gdf[s][g];
+    if (all_tr) {  // the line axis is transformed as well: a diagonal system
+      am[g] = ap[g] = 0.0;
+      d0[g] = lamh[s][g] * wt[g];
+      continue;
+    }
     double rm = 1.0 / gdcm[s][g] * hother, rp = 1.0 / gdcp[s][g] * hother;
     double dg = -(rm + rp);
     am[g] = rm; ap[g] = rp;
     if (g == 0) { dg += sg0 * rm; am[g] = 0.0; }
     if (g == m - 1) { dg += sg1 * rp; ap[g] = 0.0; }
     d0[g] = dg;
-    wt[g] = hother * gdf[s][g];
   }
   t.am = dupload(c, am); t.ap = dupload(c, ap); t.d0 = dupload(c, d0); t.wt = dupload(c, wt);
-  t.pin = (bclo[s] == 'N' && bchi[s] == 'N') ? 1 : 0;
+  // singular N-N line: pin its last unknown; the all-periodic zero mode has
+  // a zero pivot instead, which the line solver maps to x = 0
+  t.pin = (!all_tr && bclo[s] == 'N' && bchi[s] == 'N') ? 1 : 0;
   size_t tot = (size_t)t.G[0] * t.G[1] * t.G[2];
   t.ph = dalloc<double>(c, tot);
   t.scratch = dalloc<double>(c, tot);
@@ -333,10 +349,36 @@ void plan_tensor(snac_ctx *c) {
   t.ok = true;
 }
 
+// batched MG-PCG plan with the FFT-axis transforms and the bounding-box
+// work array of the FFT path; planned once, on first need
+void plan_mg(snac_ctx *c) {
+  if (c->mg_planned) return;
+  c->mg_planned = true;
+  MGPlan &P = c->mg;
+  mg_plan(P, c->cs, c->device);
+  if (!P.ok || c->device < 0 || P.fa < 0) return;
+  P.fwd = r2r_plan(P.kind_fwd, P.G[P.fa]);
+  P.bwd = r2r_plan(P.kind_bwd, P.G[P.fa]);
+  P.ph = dalloc<double>(c, (size_t)P.G[0] * P.G[1] * P.G[2]);
+}
+
+// true when the pressure solve runs the MG-PCG path
+bool use_mg(snac_ctx *c) {
+  if (c->solver == SNAC_SOLVER_MG) return true;
+  if (c->solver == SNAC_SOLVER_TENSOR) return false;
+  return !c->tp.ok;
+}
+
 void check_solver(snac_ctx *c) {
-  if (!c->tp.ok)
-    throw std::runtime_error("no GPU Poisson solver for this case: the tensor-product solver needs " +
-                             std::string("a box-shaped union with uniform transform axes (") + c->tp.why + ")");
+  if (c->solver == SNAC_SOLVER_TENSOR && !c->tp.ok)
+    throw std::runtime_error("the tensor-product solver needs a box-shaped union with uniform transform axes (" +
+                             c->tp.why + ")");
+  if (use_mg(c)) {
+    plan_mg(c);
+    if (!c->mg.ok)
+      throw std::runtime_error("no GPU Poisson solver for this case: tensor-product solver: " + c->tp.why +
+                               "; multigrid solver: " + c->mg.why);
+  }
 }
 
 // strides of axis a in a ghosted block array, and of the interior origin
@@ -406,8 +448,78 @@ void stage_last(snac_ctx *c) {
   }
 }
 
+// MG path: block pp interiors -> (transform along the FFT axis) -> plan
+// layout scaled by -1 (the solver works with -A) -> MG-PCG -> back
+void mg_stage_in(snac_ctx *c) {
+  MGPlan &P = c->mg;
+  const long pst[3] = {1, (long)P.G[0], (long)P.G[0] * P.G[1]};
+  const long na = P.G[P.ax_a], ns = P.G[P.ax_s];
+  if (P.fa >= 0) {
+    int a = P.fa, b1, b2;
+    others(a, b1, b2);
+    for (size_t b = 0; b < c->blk.size(); b++) {
+      DevBlock &D = c->blk[b];
+      const Block &B = c->cs.blocks[b];
+      const double *in = D.pp + 1 + D.g.sx + D.g.sx * D.g.sy;
+      long off = 0;
+      for (int q = 0; q < 3; q++) off += (long)(B.lo_g[q] - P.gmin[q]) * pst[q];
+      launch_r2r(c->stream, P.fwd, in, gstride(D, a), gstride(D, b1), gstride(D, b2), P.ph + off, pst[a], pst[b1],
+                 pst[b2], bext(D, b1), bext(D, b2), 1.0);
+    }
+    mg_copy3(c->stream, P.ph, pst[P.ax_m], pst[P.ax_a], pst[P.ax_s], P.r, na * ns, ns, 1, P.G[P.ax_m], (int)na,
+             (int)ns, -1.0, pst[P.ax_a] == 1 ? 1 : 0);
+  } else {
+    for (size_t b = 0; b < c->blk.size(); b++) {
+      DevBlock &D = c->blk[b];
+      const Block &B = c->cs.blocks[b];
+      const double *in = D.pp + 1 + D.g.sx + D.g.sx * D.g.sy;
+      long off = ((long)(B.lo_g[P.ax_m] - P.gmin[P.ax_m]) * na + (B.lo_g[P.ax_a] - P.gmin[P.ax_a])) * ns +
+                 (B.lo_g[P.ax_s] - P.gmin[P.ax_s]);
+      mg_copy3(c->stream, in, gstride(D, P.ax_m), gstride(D, P.ax_a), gstride(D, P.ax_s), P.r + off, na * ns, ns, 1,
+               B.n[P.ax_m], B.n[P.ax_a], B.n[P.ax_s], -1.0, P.ax_a == 0 ? 1 : 0);
+    }
+  }
+}
+
+void mg_stage_out(snac_ctx *c) {
+  MGPlan &P = c->mg;
+  const long pst[3] = {1, (long)P.G[0], (long)P.G[0] * P.G[1]};
+  const long na = P.G[P.ax_a], ns = P.G[P.ax_s];
+  if (P.fa >= 0) {
+    mg_copy3(c->stream, P.x, na * ns, ns, 1, P.ph, pst[P.ax_m], pst[P.ax_a], pst[P.ax_s], P.G[P.ax_m], (int)na,
+             (int)ns, 1.0, pst[P.ax_a] == 1 ? 2 : 0);
+    int a = P.fa, b1, b2;
+    others(a, b1, b2);
+    for (size_t b = 0; b < c->blk.size(); b++) {
+      DevBlock &D = c->blk[b];
+      const Block &B = c->cs.blocks[b];
+      double *out = D.pp + 1 + D.g.sx + D.g.sx * D.g.sy;
+      long off = 0;
+      for (int q = 0; q < 3; q++) off += (long)(B.lo_g[q] - P.gmin[q]) * pst[q];
+      launch_r2r(c->stream, P.bwd, P.ph + off, pst[a], pst[b1], pst[b2], out, gstride(D, a), gstride(D, b1),
+                 gstride(D, b2), bext(D, b1), bext(D, b2), P.norm);
+    }
+  } else {
+    for (size_t b = 0; b < c->blk.size(); b++) {
+      DevBlock &D = c->blk[b];
+      const Block &B = c->cs.blocks[b];
+      double *out = D.pp + 1 + D.g.sx + D.g.sx * D.g.sy;
+      long off = ((long)(B.lo_g[P.ax_m] - P.gmin[P.ax_m]) * na + (B.lo_g[P.ax_a] - P.gmin[P.ax_a])) * ns +
+                 (B.lo_g[P.ax_s] - P.gmin[P.ax_s]);
+      mg_copy3(c->stream, P.x + off, na * ns, ns, 1, out, gstride(D, P.ax_m), gstride(D, P.ax_a), gstride(D, P.ax_s),
+               B.n[P.ax_m], B.n[P.ax_a], B.n[P.ax_s], 1.0, P.ax_a == 0 ? 2 : 0);
+    }
+  }
+}
+
 void solve_pressure(snac_ctx *c) {
   check_solver(c);
+  if (use_mg(c)) {
+    { Phase tf(c, SNAC_PH_R2R_FWD); mg_stage_in(c); }
+    { Phase tt(c, SNAC_PH_TRIDIAG); mg_solve(c->mg, c->stream); }
+    { Phase tb(c, SNAC_PH_R2R_BWD); mg_stage_out(c); }
+    return;
+  }
   TensorPlan &t = c->tp;
   {
     Phase tf(c, SNAC_PH_R2R_FWD);
@@ -531,7 +643,7 @@ void drop_graph(snac_ctx *c) {
 // starting parity changes.
 void run_steps(snac_ctx *c, double dt, int nsteps) {
   check_solver(c);
-  if (!c->use_graph || c->timing) {
+  if (!c->use_graph || c->timing || use_mg(c)) {
     for (int s = 0; s < nsteps; s++)
       for (int irk = 1; irk <= 3; irk++) substage(c, irk, dt);
     return;
@@ -647,11 +759,13 @@ void create(snac_ctx *c) {
   c->prog_pp = upload_program(c, build_pressure_program(cs, arrs, 1));
   plan_tensor(c);
   autotune(c);
+  if (!c->tp.ok) plan_mg(c);
   if (c->device >= 0) SNAC_HIP(hipDeviceSynchronize());
 }
 
 void destroy(snac_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  mg_free(c->mg);
   drop_graph(c);
   for (auto &r : c->pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto &e : c->pool) (void)hipEventDestroy(e);
@@ -760,12 +874,31 @@ int snac_ctx_get_params(const snac_ctx *c, double *dp, int *ip) {
   });
 }
 
+int snac_solver_stats(snac_ctx *c, int *iters, double *max_relres) {
+  return guard([&] {
+    on_device(c);
+    int it = 0;
+    double rel = 0.0;
+    if (use_mg(c) && c->mg.ok) mg_stats(c->mg, c->stream, it, rel);
+    if (iters) *iters = it;
+    if (max_relres) *max_relres = rel;
+  });
+}
+
 int snac_ctx_set_solver(snac_ctx *c, int solver) {
   return guard([&] {
     checked(c);
-    if (solver != SNAC_SOLVER_AUTO && solver != SNAC_SOLVER_TENSOR) throw std::runtime_error("unknown solver id");
+    if (solver != SNAC_SOLVER_AUTO && solver != SNAC_SOLVER_TENSOR && solver != SNAC_SOLVER_MG)
+      throw std::runtime_error("unknown solver id");
+    const int old = c->solver;
     c->solver = solver;
-    check_solver(c);
+    try {
+      check_solver(c);
+    } catch (...) {
+      c->solver = old;
+      throw;
+    }
+    drop_graph(c);
   });
 }
 

@@ -1,0 +1,503 @@
+// Kernels of the batched MG-preconditioned CG (mg.hpp): zebra line
+// relaxation with one wavefront per line (partitioned Thomas over the lanes'
+// row chunks, parallel cyclic reduction of the 2-per-lane interface system
+// with cross-lane shuffles), fused residual + restriction, prolongation,
+// operator application, per-group dot products with wavefront reductions and
+// the CG vector updates.
+#include "mg.hpp"
+
+namespace snac {
+namespace {
+
+__device__ __forceinline__ long vidx(const MGLev &L, int m, int a, int s) { return ((long)m * L.na + a) * L.ns + s; }
+__device__ __forceinline__ long cidx(const MGLev &L, int m, int a, int s) { return m * L.qm + (long)a * L.ns + s; }
+
+// sum of the couplings to the neighbouring lines (a and m directions) times x
+__device__ __forceinline__ double nb_sum(const MGLev &L, const double *__restrict__ x, int m, int a, long e, long q) {
+  double r = 0.0;
+  const long sa = L.ns;
+  if (a < L.na - 1) r += L.ca[q] * x[e + sa];
+  else if (L.per_a) r += L.ca[q] * x[e - (long)(L.na - 1) * sa];
+  if (a > 0) r += L.ca[q - sa] * x[e - sa];
+  else if (L.per_a) r += L.ca[q + (long)(L.na - 1) * sa] * x[e + (long)(L.na - 1) * sa];
+  if (L.coupled) {
+    const long sm = (long)L.na * L.ns;
+    if (m < L.nm - 1) r += L.cm[q] * x[e + sm];
+    else if (L.per_m) r += L.cm[q] * x[e - (long)(L.nm - 1) * sm];
+    if (m > 0) r += L.cm[q - L.qm] * x[e - sm];
+    else if (L.per_m) r += L.cm[q + (long)(L.nm - 1) * L.qm] * x[e + (long)(L.nm - 1) * sm];
+  }
+  return r;
+}
+
+__device__ __forceinline__ double mx_at(const MGLev &L, double lm, const double *__restrict__ x, int m, int a, int s) {
+  const long e = vidx(L, m, a, s), q = cidx(L, m, a, s);
+  double y = (L.dg[q] - lm * L.w[q]) * x[e];
+  if (s > 0) y -= L.cs[q - 1] * x[e - 1];
+  if (s < L.ns - 1) y -= L.cs[q] * x[e + 1];
+  return y - nb_sum(L, x, m, a, e, q);
+}
+
+__device__ __forceinline__ int group_of(const MGLev &L, int m) { return L.coupled ? 0 : m; }
+
+__device__ __forceinline__ double shfl(double v, int src) { return __shfl(v, src, 64); }
+
+// One reduced row  al*u[i-d] + be*u[i] + ga*u[i+d] = de  of the interface
+// system; rows outside the line are the identity.
+struct Row {
+  double al, be, ga, de;
+};
+
+__device__ __forceinline__ Row fetch(const Row &r, int src, bool valid) {
+  Row o;
+  o.al = shfl(r.al, src);
+  o.be = shfl(r.be, src);
+  o.ga = shfl(r.ga, src);
+  o.de = shfl(r.de, src);
+  if (!valid) { o.al = 0.0; o.be = 1.0; o.ga = 0.0; o.de = 0.0; }
+  return o;
+}
+
+// PCR elimination of row r against its neighbours lo (index i-d) and hi (i+d)
+__device__ __forceinline__ Row pcr(const Row &r, const Row &lo, const Row &hi) {
+  const double k1 = r.al / lo.be, k2 = r.ga / hi.be;
+  Row o;
+  o.al = -k1 * lo.al;
+  o.be = r.be - k1 * lo.ga - k2 * hi.al;
+  o.ga = -k2 * hi.ga;
+  o.de = r.de - k1 * lo.de - k2 * hi.de;
+  return o;
+}
+
+// Line relaxation: one wavefront per line (m, a), 4 lines per workgroup.
+// Lane c owns rows [p0, p0+cnt) of the line.  Each lane eliminates its
+// chunk's interior (Thomas downwards, then upwards) leaving two interface
+// rows per lane that form a tridiagonal system in (x_first, x_last) of all
+// chunks; PCR over the lanes solves it in log2(2*nch) steps; the interior
+// rows follow from the two interface values.
+template <int K>
+__global__ __launch_bounds__(256) void k_line(MGLev L, const double *__restrict__ lam, double *__restrict__ x,
+                                              const double *__restrict__ b, int col_a, int col_m, int nla, int nlm,
+                                              int zero_nb, int pin_s, const int *__restrict__ singular,
+                                              const int *__restrict__ done) {
+  const int lane = threadIdx.x & 63;
+  const long line = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (line >= (long)nla * nlm) return;
+  const int a = 2 * (int)(line % nla) + col_a;
+  const int lm_ = (int)(line / nla);
+  const int m = L.coupled ? 2 * lm_ + col_m : lm_;
+  const int g = group_of(L, m);
+  if (done && done[g]) return;
+  const double lmv = lam ? lam[m] : 0.0;
+  const bool pin = pin_s >= 0 && singular && singular[g];
+  const long e0 = vidx(L, m, a, 0), q0 = cidx(L, m, a, 0);
+  const int ns = L.ns;
+  const int cnt = lane < L.nch ? L.base + (lane < L.extra ? 1 : 0) : 0;
+  const int p0 = lane * L.base + min(lane, L.extra);
+
+  double lo[K], dg[K], up[K], rh[K];
+#pragma unroll
+  for (int j = 0; j < K; j++) {
+    lo[j] = 0.0; dg[j] = 1.0; up[j] = 0.0; rh[j] = 0.0;
+    if (j < cnt) {
+      const int s = p0 + j;
+      const long e = e0 + s, q = q0 + s;
+      double d = L.dg[q] - lmv * L.w[q];
+      double l = s > 0 ? -L.cs[q - 1] : 0.0;
+      double u = s < ns - 1 ? -L.cs[q] : 0.0;
+      double r = b[e];
+      if (!zero_nb) r += nb_sum(L, x, m, a, e, q);
+      if (pin && s == pin_s) { d = 1.0; l = 0.0; u = 0.0; r = 0.0; }
+      lo[j] = l; dg[j] = d; up[j] = u; rh[j] = r;
+    }
+  }
+  if (ns == 1) {  // a one-row line: no interface system
+    if (lane == 0) x[e0] = rh[0] / dg[0];
+    return;
+  }
+  // downward elimination: row j (1 <= j < cnt) becomes A x_p0 + x_j + C x_{j+1} = D
+  double A[K], C[K], D[K];
+  double aq = 0.0, cq = 0.0, dq = 0.0;
+#pragma unroll
+  for (int j = 0; j < K; j++) { A[j] = 0.0; C[j] = 0.0; D[j] = 0.0; }
+#pragma unroll
+  for (int j = 1; j < K; j++) {
+    if (j < cnt) {
+      if (j == 1) {
+        const double ir = 1.0 / dg[1];
+        A[1] = lo[1] * ir; C[1] = up[1] * ir; D[1] = rh[1] * ir;
+      } else {
+        const double ir = 1.0 / (dg[j] - lo[j] * C[j - 1]);
+        A[j] = -lo[j] * A[j - 1] * ir; C[j] = up[j] * ir; D[j] = (rh[j] - lo[j] * D[j - 1]) * ir;
+      }
+      if (j == cnt - 1) { aq = A[j]; cq = C[j]; dq = D[j]; }
+    }
+  }
+  // upward elimination: x_j = D - A x_p0 - C x_last for 1 <= j <= cnt-2
+  double dn = 0.0, an = 0.0, cn = 0.0;
+#pragma unroll
+  for (int j = K - 2; j >= 1; j--) {
+    if (j <= cnt - 2) {
+      if (j == cnt - 2) { dn = D[j]; an = A[j]; cn = C[j]; }
+      else { dn = D[j] - C[j] * dn; an = A[j] - C[j] * an; cn = -C[j] * cn; }
+      D[j] = dn; A[j] = an; C[j] = cn;
+    }
+  }
+  Row r0, r1;
+  if (cnt >= 2) {
+    r0.al = lo[0];
+    if (cnt == 2) { r0.be = dg[0]; r0.ga = up[0]; r0.de = rh[0]; }
+    else { r0.be = dg[0] - up[0] * an; r0.ga = -up[0] * cn; r0.de = rh[0] - up[0] * dn; }
+    r1.al = aq; r1.be = 1.0; r1.ga = cq; r1.de = dq;
+  } else {
+    r0 = Row{0.0, 1.0, 0.0, 0.0};
+    r1 = r0;
+  }
+  // interface system: u[2c] = x_first(c), u[2c+1] = x_last(c); PCR, d = 1
+  {
+    const Row r1m = fetch(r1, lane > 0 ? lane - 1 : 0, lane > 0);
+    const Row r0p = fetch(r0, lane < 63 ? lane + 1 : 63, lane < 63);
+    const Row n0 = pcr(r0, r1m, r1);
+    const Row n1 = pcr(r1, r0, r0p);
+    r0 = n0;
+    r1 = n1;
+  }
+  const int nred = 2 * L.nch;
+  for (int h = 1; 2 * h < nred; h <<= 1) {  // d = 2h: lanes -+h, same slot
+    const bool vm = lane >= h, vp = lane + h < 64;
+    const int sm = vm ? lane - h : lane, sp = vp ? lane + h : lane;
+    const Row a0 = fetch(r0, sm, vm), b0 = fetch(r0, sp, vp);
+    const Row a1 = fetch(r1, sm, vm), b1 = fetch(r1, sp, vp);
+    r0 = pcr(r0, a0, b0);
+    r1 = pcr(r1, a1, b1);
+  }
+  if (cnt == 0) return;
+  const double xp = r0.de / r0.be, xq = r1.de / r1.be;
+  x[e0 + p0] = xp;
+  x[e0 + p0 + cnt - 1] = xq;
+#pragma unroll
+  for (int j = 1; j < K - 1; j++)
+    if (j <= cnt - 2) x[e0 + p0 + j] = D[j] - A[j] * xp - C[j] * xq;
+}
+
+// coarse b = sum over the active fine members of (b - M x)
+__global__ __launch_bounds__(256) void k_restrict(MGLev F, MGLev Cl, const double *__restrict__ lamf,
+                                                  const double *__restrict__ x, const double *__restrict__ b,
+                                                  double *__restrict__ bc, const int *__restrict__ done) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  const long n = (long)Cl.nm * Cl.na * Cl.ns;
+  if (idx >= n) return;
+  const int s = (int)(idx % Cl.ns);
+  const long t = idx / Cl.ns;
+  const int I = (int)(t % Cl.na), M = (int)(t / Cl.na);
+  if (done && done[group_of(Cl, M)]) return;
+  const int ra = F.na != Cl.na, rm = F.nm != Cl.nm;
+  double sum = 0.0;
+  for (int dm = 0; dm <= rm; dm++) {
+    const int m = rm ? 2 * M + dm : M;
+    if (m >= F.nm) break;
+    const double lm = lamf ? lamf[m] : 0.0;
+    for (int da = 0; da <= ra; da++) {
+      const int a = ra ? 2 * I + da : I;
+      if (a >= F.na) break;
+      if (!F.act[cidx(F, m, a, s)]) continue;
+      sum += b[vidx(F, m, a, s)] - mx_at(F, lm, x, m, a, s);
+    }
+  }
+  bc[idx] = sum;
+}
+
+__global__ __launch_bounds__(256) void k_prolong(MGLev F, MGLev Cl, const double *__restrict__ xc,
+                                                 double *__restrict__ xf, const int *__restrict__ done) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  const long n = (long)F.nm * F.na * F.ns;
+  if (idx >= n) return;
+  const int s = (int)(idx % F.ns);
+  const long t = idx / F.ns;
+  const int a = (int)(t % F.na), m = (int)(t / F.na);
+  if (done && done[group_of(F, m)]) return;
+  if (!F.act[cidx(F, m, a, s)]) return;
+  const int I = F.na != Cl.na ? a >> 1 : a, M = F.nm != Cl.nm ? m >> 1 : m;
+  xf[idx] += xc[vidx(Cl, M, I, s)];
+}
+
+__global__ __launch_bounds__(256) void k_apply(MGLev L, const double *__restrict__ lam, const double *__restrict__ x,
+                                               double *__restrict__ out, const int *__restrict__ done) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  const long n = (long)L.nm * L.na * L.ns;
+  if (idx >= n) return;
+  const int s = (int)(idx % L.ns);
+  const long t = idx / L.ns;
+  const int a = (int)(t % L.na), m = (int)(t / L.na);
+  if (done && done[group_of(L, m)]) return;
+  out[idx] = mx_at(L, lam ? lam[m] : 0.0, x, m, a, s);
+}
+
+__global__ __launch_bounds__(256) void k_zero(long n, double *__restrict__ x) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx < n) x[idx] = 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_izero(long n, int *__restrict__ x) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx < n) x[idx] = 0;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// blockIdx.y = group, blockIdx.x = chunk of the group's elements
+__global__ __launch_bounds__(256) void k_dot(long gsize, const double *__restrict__ x, const double *__restrict__ y,
+                                             const double *__restrict__ x2, const double *__restrict__ y2,
+                                             double *__restrict__ part, int nchunk, const int *__restrict__ done,
+                                             const int *__restrict__ only) {
+  __shared__ double sh[2][4];
+  const int g = blockIdx.y;
+  const bool skip = (done && done[g]) || (only && !only[g]);
+  double s1 = 0.0, s2 = 0.0;
+  if (!skip) {
+    const long base = (long)g * gsize;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < gsize; i += (long)nchunk * 256) {
+      if (x) s1 += x[base + i] * y[base + i];
+      if (x2) s2 += y2 ? x2[base + i] * y2[base + i] : x2[base + i];
+    }
+  }
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  if ((threadIdx.x & 63) == 0) { sh[0][threadIdx.x >> 6] = s1; sh[1][threadIdx.x >> 6] = s2; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[((long)g * nchunk + blockIdx.x) * 2] = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
+    part[((long)g * nchunk + blockIdx.x) * 2 + 1] = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+  }
+}
+
+// one wavefront per group sums its chunks (skipped groups keep their values)
+__global__ __launch_bounds__(64) void k_dot_final(const double *__restrict__ part, int nchunk, double *__restrict__ out,
+                                                  const int *__restrict__ done, const int *__restrict__ only) {
+  const int g = blockIdx.x;
+  if ((done && done[g]) || (only && !only[g])) return;
+  double a = 0.0, b = 0.0;
+  for (int c = threadIdx.x; c < nchunk; c += 64) {
+    a += part[((long)g * nchunk + c) * 2];
+    b += part[((long)g * nchunk + c) * 2 + 1];
+  }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if (threadIdx.x == 0) { out[g * 2] = a; out[g * 2 + 1] = b; }
+}
+
+__global__ __launch_bounds__(256) void k_cg_update(int ng, long gsize, const double *__restrict__ rz,
+                                                   const double *__restrict__ pq, double *__restrict__ x,
+                                                   const double *__restrict__ p, double *__restrict__ r,
+                                                   const double *__restrict__ q, const int *__restrict__ done) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)ng * gsize) return;
+  const int g = (int)(idx / gsize);
+  if (done[g]) return;
+  const double den = pq[g * 2];
+  const double al = den != 0.0 ? rz[g * 2] / den : 0.0;
+  x[idx] += al * p[idx];
+  r[idx] -= al * q[idx];
+}
+
+__global__ __launch_bounds__(256) void k_cg_dir(int ng, long gsize, const double *__restrict__ rzn,
+                                                const double *__restrict__ rzo, const double *__restrict__ z,
+                                                double *__restrict__ p, const int *__restrict__ done) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (long)ng * gsize) return;
+  const int g = (int)(idx / gsize);
+  if (done[g]) return;
+  double be = 0.0;
+  if (rzo) {
+    const double den = rzo[g * 2];
+    be = den != 0.0 ? rzn[g * 2] / den : 0.0;
+  }
+  p[idx] = z[idx] + be * p[idx];
+}
+
+__global__ __launch_bounds__(256) void k_sub_mean(MGLev L, double *__restrict__ x, const double *__restrict__ sums,
+                                                  const int *__restrict__ singular, const double *__restrict__ nact,
+                                                  const int *__restrict__ done) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  const long n = (long)L.nm * L.na * L.ns;
+  if (idx >= n) return;
+  const int s = (int)(idx % L.ns);
+  const long t = idx / L.ns;
+  const int a = (int)(t % L.na), m = (int)(t / L.na);
+  const int g = group_of(L, m);
+  if (!singular[g] || (done && done[g])) return;
+  if (!L.act[cidx(L, m, a, s)]) return;
+  x[idx] -= sums[g * 2 + 1] / nact[g];
+}
+
+__global__ __launch_bounds__(256) void k_flags(int ng, const double *__restrict__ rr, const double *__restrict__ bb,
+                                               double tol2, int *__restrict__ done, int *__restrict__ left) {
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  int mine = 0;
+  for (int g = threadIdx.x; g < ng; g += 256) {
+    if (!done[g] && rr[g * 2] <= tol2 * bb[g * 2]) done[g] = 1;
+    mine += done[g] ? 0 : 1;
+  }
+  if (mine) atomicAdd(&cnt, mine);
+  __syncthreads();
+  if (threadIdx.x == 0) left[0] = cnt;
+}
+
+// strided 3D copy; TILE 1: lanes along a on the read side (src stride(a) 1),
+// along s on the write side (dst stride(s) 1); TILE 2 the reverse
+constexpr int TT = 32;
+template <int TILE>
+__global__ __launch_bounds__(256) void k_copy3(const double *__restrict__ src, long sm, long sa, long ss,
+                                               double *__restrict__ dst, long dm, long da, long ds, int ea, int es,
+                                               double scale) {
+  const long m = blockIdx.z;
+  if (TILE == 0) {
+    const int s = blockIdx.x * 256 + threadIdx.x, a = blockIdx.y;
+    if (s >= es) return;
+    dst[m * dm + a * da + s * ds] = scale * src[m * sm + a * sa + s * ss];
+    return;
+  }
+  __shared__ double t[TT][TT + 1];
+  const int a0 = blockIdx.y * TT, s0 = blockIdx.x * TT;
+  const int l = threadIdx.x % TT, r = threadIdx.x / TT;  // 8 rows per pass
+  if (TILE == 1) {
+#pragma unroll
+    for (int k = 0; k < TT; k += 8) {
+      const int a = a0 + l, s = s0 + r + k;
+      if (a < ea && s < es) t[r + k][l] = src[m * sm + a * sa + s * ss];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < TT; k += 8) {
+      const int s = s0 + l, a = a0 + r + k;
+      if (a < ea && s < es) dst[m * dm + a * da + s * ds] = scale * t[l][r + k];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < TT; k += 8) {
+      const int s = s0 + l, a = a0 + r + k;
+      if (a < ea && s < es) t[r + k][l] = src[m * sm + a * sa + s * ss];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < TT; k += 8) {
+      const int a = a0 + l, s = s0 + r + k;
+      if (a < ea && s < es) dst[m * dm + a * da + s * ds] = scale * t[l][r + k];
+    }
+  }
+}
+
+unsigned blocks_for(long n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+void mg_line(hipStream_t st, const MGLev &L, const double *lam, double *x, const double *b, int col_a, int col_m,
+             int zero_nb, int pin_s, const int *singular, const int *done) {
+  const int nla = (L.na - col_a + 1) / 2;
+  const int nlm = L.coupled ? (L.nm - col_m + 1) / 2 : L.nm;
+  if (nla <= 0 || nlm <= 0) return;
+  const long lines = (long)nla * nlm;
+  dim3 grid((unsigned)((lines + 3) / 4));
+  const int rows = L.base + (L.extra ? 1 : 0);
+  if (rows <= 3)
+    hipLaunchKernelGGL(k_line<3>, grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb, pin_s,
+                       singular, done);
+  else if (rows <= 5)
+    hipLaunchKernelGGL(k_line<5>, grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb, pin_s,
+                       singular, done);
+  else if (rows <= 9)
+    hipLaunchKernelGGL(k_line<9>, grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb, pin_s,
+                       singular, done);
+  else if (rows <= 17)
+    hipLaunchKernelGGL(k_line<17>, grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb, pin_s,
+                       singular, done);
+  else
+    throw std::runtime_error("multigrid line relaxation: lines longer than 1088 cells are not supported");
+  SNAC_HIP(hipGetLastError());
+}
+
+void mg_restrict_residual(hipStream_t st, const MGLev &F, const MGLev &C, const double *lam_f, const double *,
+                          const double *x, const double *b, double *bc, const int *done) {
+  const long n = (long)C.nm * C.na * C.ns;
+  hipLaunchKernelGGL(k_restrict, dim3(blocks_for(n)), dim3(256), 0, st, F, C, lam_f, x, b, bc, done);
+  SNAC_HIP(hipGetLastError());
+}
+
+void mg_prolong(hipStream_t st, const MGLev &F, const MGLev &C, const double *xc, double *xf, const int *done) {
+  const long n = (long)F.nm * F.na * F.ns;
+  hipLaunchKernelGGL(k_prolong, dim3(blocks_for(n)), dim3(256), 0, st, F, C, xc, xf, done);
+  SNAC_HIP(hipGetLastError());
+}
+
+void mg_apply(hipStream_t st, const MGLev &L, const double *lam, const double *x, double *out, const int *done) {
+  const long n = (long)L.nm * L.na * L.ns;
+  hipLaunchKernelGGL(k_apply, dim3(blocks_for(n)), dim3(256), 0, st, L, lam, x, out, done);
+  SNAC_HIP(hipGetLastError());
+}
+
+void mg_zero(hipStream_t st, long n, double *x) {
+  hipLaunchKernelGGL(k_zero, dim3(blocks_for(n)), dim3(256), 0, st, n, x);
+  SNAC_HIP(hipGetLastError());
+}
+
+void mg_izero(hipStream_t st, long n, int *x) {
+  hipLaunchKernelGGL(k_izero, dim3(blocks_for(n)), dim3(256), 0, st, n, x);
+  SNAC_HIP(hipGetLastError());
+}
+
+void mg_dot(hipStream_t st, int ng, long gsize, const double *x, const double *y, const double *x2, const double *y2,
+            double *part, int nchunk, double *out, const int *done, const int *only) {
+  hipLaunchKernelGGL(k_dot, dim3(nchunk, ng), dim3(256), 0, st, gsize, x, y, x2, y2, part, nchunk, done, only);
+  SNAC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_dot_final, dim3(ng), dim3(64), 0, st, part, nchunk, out, done, only);
+  SNAC_HIP(hipGetLastError());
+}
+
+void mg_cg_update(hipStream_t st, int ng, long gsize, const double *rz, const double *pq, double *x, const double *p,
+                  double *r, const double *q, const int *done) {
+  hipLaunchKernelGGL(k_cg_update, dim3(blocks_for((long)ng * gsize)), dim3(256), 0, st, ng, gsize, rz, pq, x, p, r, q,
+                     done);
+  SNAC_HIP(hipGetLastError());
+}
+
+void mg_cg_dir(hipStream_t st, int ng, long gsize, const double *rzn, const double *rzo, const double *z, double *p,
+               const int *done) {
+  hipLaunchKernelGGL(k_cg_dir, dim3(blocks_for((long)ng * gsize)), dim3(256), 0, st, ng, gsize, rzn, rzo, z, p, done);
+  SNAC_HIP(hipGetLastError());
+}
+
+void mg_sub_mean(hipStream_t st, const MGLev &L, double *x, const double *sums, const int *singular,
+                 const double *nact, const int *done) {
+  const long n = (long)L.nm * L.na * L.ns;
+  hipLaunchKernelGGL(k_sub_mean, dim3(blocks_for(n)), dim3(256), 0, st, L, x, sums, singular, nact, done);
+  SNAC_HIP(hipGetLastError());
+}
+
+void mg_flags(hipStream_t st, int ng, const double *rr, const double *bb, double tol2, int *done, int *left) {
+  hipLaunchKernelGGL(k_flags, dim3(1), dim3(256), 0, st, ng, rr, bb, tol2, done, left);
+  SNAC_HIP(hipGetLastError());
+}
+
+void mg_copy3(hipStream_t st, const double *src, long sm, long sa, long ss, double *dst, long dm, long da, long ds,
+              int em, int ea, int es, double scale, int tile) {
+  if (em <= 0 || ea <= 0 || es <= 0) return;
+  if (tile == 0) {
+    hipLaunchKernelGGL(k_copy3<0>, dim3((es + 255) / 256, ea, em), dim3(256), 0, st, src, sm, sa, ss, dst, dm, da, ds,
+                       ea, es, scale);
+  } else {
+    dim3 grid((es + TT - 1) / TT, (ea + TT - 1) / TT, em);
+    if (tile == 1)
+      hipLaunchKernelGGL(k_copy3<1>, grid, dim3(256), 0, st, src, sm, sa, ss, dst, dm, da, ds, ea, es, scale);
+    else
+      hipLaunchKernelGGL(k_copy3<2>, grid, dim3(256), 0, st, src, sm, sa, ss, dst, dm, da, ds, ea, es, scale);
+  }
+  SNAC_HIP(hipGetLastError());
+}
+
+}  // namespace snac

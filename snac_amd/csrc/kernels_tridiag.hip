@@ -53,8 +53,10 @@ __global__ __launch_bounds__(WG) void k_tridiag(TridiagLine t, double *__restric
         const double a = t.am[q], c = t.ap[q];
         const double b = t.d0[q] + lam * t.wt[q];
         const double den = (q == 0) ? b : b - a * cprev;
-        const double cq = c / den;
-        const double dq = (xv[u] - (q == 0 ? 0.0 : a * dprev)) / den;
+        // a zero pivot only occurs for the singular zero mode of an
+        // all-periodic problem (diagonal system): its unknown is set to 0
+        const double cq = den != 0.0 ? c / den : 0.0;
+        const double dq = den != 0.0 ? (xv[u] - (q == 0 ? 0.0 : a * dprev)) / den : 0.0;
         cp[q * cs] = cq;
         dl[q * ss] = dq;
         cprev = cq;

@@ -1,0 +1,123 @@
+// Batched multigrid-preconditioned conjugate gradients for the pressure
+// Poisson problems on any union of blocks: the per-mode 2D Helmholtz
+// problems of the FFT path (one CG per Fourier mode, all modes at once) or
+// the single 3D problem without FFT.  This takes the place of HYPRE's
+// per-mode StructPFMG solves (solver.f90:843-922) and of the 3D solve
+// (solver.f90:524-573).
+//
+// Unknowns live on the bounding box of the union; cells outside it are
+// inactive identity rows kept at zero.  Axes: s = the line axis (relaxed by
+// line solves, never coarsened, contiguous in memory), a = the coarsened
+// in-plane axis, m = the mode index (FFT path) or the third spatial axis
+// (3D path, coupled and coarsened like a).  Vector element (m, a, s) sits at
+// (m*na + a)*ns + s.
+//
+// Sign convention: the kernels work with M = -A, A being the reference's
+// assembled operator (init_matrix_2d + lambda*dxf*dyf on the diagonal,
+// solver.f90:687-842,981-1020; init_matrix_3d 158-372), so M is symmetric
+// positive (semi-)definite:
+//   (M x)(m,a,s) = (dg - lam[m]*w) x - cs(s-1) x(s-1) - cs(s) x(s+1)
+//                  - ca(a-1) x(a-1) - ca(a) x(a+1) [- cm(m-1) x(m-1) - cm(m) x(m+1)]
+// with c?(i) the coupling of the link i -> i+1 stored at cell i.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "case.hpp"
+#include "common.hpp"
+
+namespace snac {
+
+// one level, passed by value to the kernels
+struct MGLev {
+  int nm, na, ns;
+  int coupled;          // couplings along m (3D path)
+  int per_a, per_m;     // periodic links along a / m
+  long qm;              // coefficient stride of m: na*ns when coupled, else 0 (mode-independent)
+  const double *dg, *w, *cs, *ca, *cm;
+  const unsigned char *act;
+  int nch, base, extra; // a line of ns rows over nch lanes: base (+1 for the first `extra`) rows each
+};
+
+// ---- kernels (kernels_mg.hip) ----
+// zebra line relaxation of the lines (m, a) with a % 2 == col_a (and, when
+// coupled, m % 2 == col_m); zero_nb: neighbouring lines read as zero;
+// pin_s >= 0: in singular groups row pin_s is replaced by x = 0
+void mg_line(hipStream_t st, const MGLev &L, const double *lam, double *x, const double *b, int col_a, int col_m,
+             int zero_nb, int pin_s, const int *singular, const int *done);
+// coarse b = restriction of the fine residual b - M x (sum over the active
+// fine cells of each coarse cell)
+void mg_restrict_residual(hipStream_t st, const MGLev &F, const MGLev &C, const double *lam_f, const double *lam_c,
+                          const double *x, const double *b, double *bc, const int *done);
+// fine x += coarse x on the active fine cells
+void mg_prolong(hipStream_t st, const MGLev &F, const MGLev &C, const double *xc, double *xf, const int *done);
+void mg_apply(hipStream_t st, const MGLev &L, const double *lam, const double *x, double *out, const int *done);
+void mg_zero(hipStream_t st, long n, double *x);
+void mg_izero(hipStream_t st, long n, int *x);
+// per-group dot products out[g*2] = x.y, out[g*2+1] = x2.y2 (or sum of x2
+// when y2 is null); groups are contiguous runs of gsize elements; groups
+// with done[g] or !only[g] (when given) are skipped
+void mg_dot(hipStream_t st, int ng, long gsize, const double *x, const double *y, const double *x2, const double *y2,
+            double *part, int nchunk, double *out, const int *done, const int *only);
+// x += alpha p ; r -= alpha q  with alpha = rz/pq per group
+void mg_cg_update(hipStream_t st, int ng, long gsize, const double *rz, const double *pq, double *x, const double *p,
+                  double *r, const double *q, const int *done);
+// p = z + beta p with beta = rzn/rzo (beta = 0 when rzo is null)
+void mg_cg_dir(hipStream_t st, int ng, long gsize, const double *rzn, const double *rzo, const double *z, double *p,
+               const int *done);
+// x -= (sum / nact) on the active cells of the singular groups
+void mg_sub_mean(hipStream_t st, const MGLev &L, double *x, const double *sums, const int *singular,
+                 const double *nact, const int *done);
+// done[g] |= rr[g] <= tol2 * bb[g]; *left = number of groups not done
+void mg_flags(hipStream_t st, int ng, const double *rr, const double *bb, double tol2, int *done, int *left);
+// strided 3D copy (em x ea x es elements) with scaling; `tile` selects an
+// LDS transpose for src stride(a) == 1 -> dst stride(s) == 1 (1) or the
+// reverse (2); 0 = direct with lanes along s
+void mg_copy3(hipStream_t st, const double *src, long sm, long sa, long ss, double *dst, long dm, long da, long ds,
+              int em, int ea, int es, double scale, int tile);
+
+// ---- host planner / solver (mg_solver.cpp) ----
+struct MGPlan {
+  bool ok = false;
+  std::string why;
+  int fa = -1;                 // FFT axis (0..2) or -1 (3D path)
+  int ax_m = -1, ax_a = -1, ax_s = -1;
+  int gmin[3] = {0, 0, 0}, G[3] = {1, 1, 1};
+  int ng = 0;                  // CG groups: modes (FFT path) or 1
+  long gsize = 0;
+  std::vector<MGLev> lev;
+  std::vector<double *> lam;   // per level (null on the 3D path)
+  std::vector<double *> xl, bl;  // per level (level 0: z and r of the CG)
+  double *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  double *part = nullptr, *bb = nullptr, *rr = nullptr, *pq = nullptr, *rz[2] = {nullptr, nullptr}, *sums = nullptr;
+  double *nact = nullptr;
+  int *done = nullptr, *singular = nullptr, *left = nullptr, *left_host = nullptr;
+  int nchunk = 1;
+  int pin_s = -1;              // coarsest-level pinned row of singular groups
+  bool any_singular = false;
+  double tol = 1e-4;
+  int maxiter = 50;
+  int colors = 2;              // zebra colours per sweep (2, or 4 when coupled)
+  int nlev = 0;
+  int last_iters = 0;
+  // FFT path: transform kinds and normalisation along the FFT axis, the
+  // bounding-box work array and the transform plans (set up by the caller)
+  int kind_fwd = 0, kind_bwd = 0;
+  double norm = 1.0;
+  R2RPlan fwd, bwd;
+  double *ph = nullptr;
+  std::vector<void *> owned;
+};
+
+// Builds the hierarchy for the case (returns with ok == false and a reason
+// when the case cannot use it); device < 0 plans without allocating.
+void mg_plan(MGPlan &P, const Case &cs, int device);
+void mg_free(MGPlan &P);
+// Solves M x = -rhs where rhs (in the plan's layout) has been copied into
+// P.r scaled by -1; on return P.x holds the solution of A x = rhs.
+void mg_solve(MGPlan &P, hipStream_t st);
+// iterations of the last solve and the largest relative residual
+// ||r_g|| / ||b_g|| over the groups (synchronises the stream)
+void mg_stats(MGPlan &P, hipStream_t st, int &iters, double &max_relres);
+
+}  // namespace snac

@@ -1,0 +1,466 @@
+// Host side of the batched MG-preconditioned CG (mg.hpp): operator assembly
+// on the bounding box of the union (the formulas of init_matrix_2d
+// solver.f90:687-842 with lambda*dxf*dyf on the diagonal, init_n_2d_matrices
+// 981-1020, and of init_matrix_3d 158-372), Galerkin coarse levels by
+// semi-coarsening along a (and m on the 3D path), the V-cycle and the CG
+// iteration of all groups at once.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "mg.hpp"
+
+namespace snac {
+namespace {
+
+template <class T>
+T *dev_upload(MGPlan &P, const std::vector<T> &h) {
+  void *p = nullptr;
+  SNAC_HIP(hipMalloc(&p, std::max<size_t>(h.size(), 1) * sizeof(T)));
+  if (!h.empty()) SNAC_HIP(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  P.owned.push_back(p);
+  return (T *)p;
+}
+
+template <class T>
+T *dev_zeros(MGPlan &P, size_t n) {
+  void *p = nullptr;
+  SNAC_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+  SNAC_HIP(hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)));
+  P.owned.push_back(p);
+  return (T *)p;
+}
+
+// host copy of one level's coefficients, dims (nmq, na, ns), nmq = nm when
+// coupled, else 1
+struct HostLev {
+  int nm, na, ns, nmq;
+  bool per_a, per_m;
+  std::vector<double> dg, w, cs, ca, cm;
+  std::vector<unsigned char> act;
+  long at(int m, int a, int s) const { return ((long)m * na + a) * ns + s; }
+  void resize() {
+    size_t n = (size_t)nmq * na * ns;
+    dg.assign(n, 1.0); w.assign(n, 0.0); cs.assign(n, 0.0); ca.assign(n, 0.0); cm.assign(n, 0.0);
+    act.assign(n, 0);
+  }
+};
+
+bool uniform_axis(const Case &cs, int a) {
+  double h = cs.blocks[0].df[a][1];
+  for (auto &B : cs.blocks)
+    for (int i = 0; i <= B.n[a] + 1; i++) {
+      if (std::fabs(B.df[a][i] - h) > 1e-12 * h) return false;
+      if (i <= B.n[a] && std::fabs(B.dc[a][i] - h) > 1e-12 * h) return false;
+    }
+  return true;
+}
+
+HostLev coarsen(const HostLev &F, bool coupled) {
+  HostLev C;
+  const bool ra = F.na > 1, rm = coupled && F.nm > 1;
+  C.na = ra ? (F.na + 1) / 2 : F.na;
+  C.nm = rm ? (F.nm + 1) / 2 : F.nm;
+  C.ns = F.ns;
+  C.nmq = coupled ? C.nm : 1;
+  C.per_a = F.per_a && C.na > 1;
+  C.per_m = F.per_m && C.nm > 1;
+  C.resize();
+  for (int M = 0; M < C.nmq; M++)
+    for (int I = 0; I < C.na; I++)
+      for (int s = 0; s < C.ns; s++) {
+        const long qc = C.at(M, I, s);
+        double dg = 0.0, w = 0.0, csum = 0.0, ca = 0.0, cm = 0.0;
+        bool act = false;
+        const int m0 = rm ? 2 * M : M, m1 = rm ? std::min(2 * M + 1, F.nmq - 1) : M;
+        const int a0 = ra ? 2 * I : I, a1 = ra ? std::min(2 * I + 1, F.na - 1) : I;
+        for (int m = m0; m <= m1; m++)
+          for (int a = a0; a <= a1; a++) {
+            const long q = F.at(m, a, s);
+            if (F.act[q]) {
+              act = true;
+              dg += F.dg[q];
+              w += F.w[q];
+            }
+            csum += F.cs[q];
+          }
+        // links inside the coarse cell count twice on the diagonal (P^T M P)
+        for (int m = m0; m <= m1; m++) {
+          if (ra && a1 > a0) dg -= 2.0 * F.ca[F.at(m, a0, s)];
+          if (ra && C.na == 1 && F.per_a && F.na >= 2) dg -= 2.0 * F.ca[F.at(m, F.na - 1, s)];
+        }
+        for (int a = a0; a <= a1; a++) {
+          if (rm && m1 > m0) dg -= 2.0 * F.cm[F.at(m0, a, s)];
+          if (rm && C.nm == 1 && F.per_m && F.nm >= 2) dg -= 2.0 * F.cm[F.at(F.nm - 1, a, s)];
+        }
+        // links to the next coarse cell along a / m
+        if (C.na > 1 || !ra)
+          for (int m = m0; m <= m1; m++) ca += F.ca[F.at(m, a1, s)];
+        if (coupled && (C.nm > 1 || !rm))
+          for (int a = a0; a <= a1; a++) cm += F.cm[F.at(m1, a, s)];
+        if (!act) continue;  // inactive: identity row
+        C.act[qc] = 1;
+        C.dg[qc] = dg;
+        C.w[qc] = w;
+        C.cs[qc] = csum;
+        C.ca[qc] = ca;
+        C.cm[qc] = cm;
+      }
+  return C;
+}
+
+}  // namespace
+
+void mg_free(MGPlan &P) {
+  for (void *p : P.owned) (void)hipFree(p);
+  P.owned.clear();
+  if (P.left_host) (void)hipHostFree(P.left_host);
+  P.left_host = nullptr;
+}
+
+void mg_plan(MGPlan &P, const Case &cs, int device) {
+  P.ok = false;
+  const int fa = cs.fft_axis - 1;
+  P.fa = fa;
+  const bool coupled = fa < 0;
+  int gmax[3];
+  for (int a = 0; a < 3; a++) {
+    P.gmin[a] = 1 << 30;
+    gmax[a] = -(1 << 30);
+    for (auto &B : cs.blocks) {
+      P.gmin[a] = std::min(P.gmin[a], B.lo_g[a]);
+      gmax[a] = std::max(gmax[a], B.lo_g[a] + B.n[a] - 1);
+    }
+    P.G[a] = gmax[a] - P.gmin[a] + 1;
+  }
+  if (fa >= 0)
+    for (auto &B : cs.blocks)
+      if (B.n[fa] != P.G[fa]) { P.why = "a block does not span the whole FFT axis"; return; }
+  // axes: s = line axis (non-periodic; stretched preferred, then longest)
+  std::vector<int> cand;
+  for (int a = 0; a < 3; a++)
+    if (a != fa) cand.push_back(a);
+  int s = -1;
+  for (int pass = 0; pass < 2 && s < 0; pass++)
+    for (int a : cand) {
+      if (cs.periods[a] != 0) continue;
+      if (pass == 0 && uniform_axis(cs, a)) continue;
+      if (s < 0 || P.G[a] > P.G[s]) s = a;
+    }
+  if (s < 0) { P.why = "every axis of the Poisson problem is periodic"; return; }
+  std::vector<int> rest;
+  for (int a : cand)
+    if (a != s) rest.push_back(a);
+  P.ax_s = s;
+  if (coupled) {
+    // a: the larger of the remaining axes; m: the other one
+    if (P.G[rest[0]] >= P.G[rest[1]]) { P.ax_a = rest[0]; P.ax_m = rest[1]; }
+    else { P.ax_a = rest[1]; P.ax_m = rest[0]; }
+  } else {
+    P.ax_a = rest[0];
+    P.ax_m = fa;
+  }
+  const int ns = P.G[P.ax_s], na = P.G[P.ax_a], nm = P.G[P.ax_m];
+  if (ns > 64 * 17) { P.why = "line axis longer than 1088 cells"; return; }
+
+  // fine-level operator
+  HostLev L0;
+  L0.nm = nm; L0.na = na; L0.ns = ns; L0.nmq = coupled ? nm : 1;
+  L0.per_a = cs.periods[P.ax_a] != 0;
+  L0.per_m = coupled && cs.periods[P.ax_m] != 0;
+  L0.resize();
+  std::vector<double> cmm((size_t)L0.nmq * na * ns * 3, 0.0);  // minus-side couplings (check only)
+  const int nd = coupled ? 3 : 2;
+  int ax[3];
+  if (coupled) { ax[0] = P.ax_s; ax[1] = P.ax_a; ax[2] = P.ax_m; }
+  else { ax[0] = P.ax_s; ax[1] = P.ax_a; ax[2] = -1; }
+  bool has_dirichlet = false;
+  for (auto &B : cs.blocks) {
+    const int n0 = B.n[ax[0]], n1 = B.n[ax[1]], n2 = coupled ? B.n[ax[2]] : 1;
+    for (int i2 = 1; i2 <= n2; i2++)
+      for (int i1 = 1; i1 <= n1; i1++)
+        for (int i0 = 1; i0 <= n0; i0++) {
+          const int idx[3] = {i0, i1, i2};
+          double weight = 1.0;
+          for (int e = 0; e < nd; e++) weight *= B.df[ax[e]][idx[e]];
+          double cmv[3], cpv[3], ssum = 0.0;
+          for (int e = 0; e < nd; e++) {
+            double other = 1.0;
+            for (int o = 0; o < nd; o++)
+              if (o != e) other *= B.df[ax[o]][idx[o]];
+            cmv[e] = other / B.dc[ax[e]][idx[e] - 1];
+            cpv[e] = other / B.dc[ax[e]][idx[e]];
+            ssum += cmv[e] + cpv[e];
+          }
+          double diag0 = -ssum;
+          for (int e = 0; e < nd; e++) {
+            const int a = ax[e];
+            if (cs.periods[a] != 0) continue;
+            const int nn = B.n[a];
+            if (B.is_bound[a][0] && idx[e] == 1) {
+              const char bc = B.cbcpre[a][0];
+              const double sg = bc == 'N' ? 1.0 : (bc == 'D' ? -1.0 : 0.0);
+              if (bc == 'D') has_dirichlet = true;
+              diag0 += sg * cmv[e];
+              cmv[e] = 0.0;
+            }
+            if (B.is_bound[a][1] && idx[e] == nn) {
+              const char bc = B.cbcpre[a][1];
+              const double sg = bc == 'N' ? 1.0 : (bc == 'D' ? -1.0 : 0.0);
+              if (bc == 'D') has_dirichlet = true;
+              diag0 += sg * cpv[e];
+              cpv[e] = 0.0;
+            }
+          }
+          const int gs = B.lo_g[ax[0]] - P.gmin[ax[0]] + i0 - 1;
+          const int ga = B.lo_g[ax[1]] - P.gmin[ax[1]] + i1 - 1;
+          const int gm = coupled ? B.lo_g[ax[2]] - P.gmin[ax[2]] + i2 - 1 : 0;
+          const long q = L0.at(gm, ga, gs);
+          if (L0.act[q]) { P.why = "overlapping blocks"; return; }
+          L0.act[q] = 1;
+          L0.dg[q] = -diag0;
+          L0.w[q] = coupled ? 0.0 : weight;
+          L0.cs[q] = cpv[0];
+          L0.ca[q] = cpv[1];
+          if (coupled) L0.cm[q] = cpv[2];
+          for (int e = 0; e < nd; e++) cmm[q * 3 + e] = cmv[e];
+        }
+  }
+  // every link must be symmetric and join two active cells that are
+  // neighbours in the global index space (block faces conform)
+  {
+    const double *cp[3] = {L0.cs.data(), L0.ca.data(), L0.cm.data()};
+    const int ext[3] = {ns, na, L0.nmq};
+    const bool per[3] = {false, L0.per_a, L0.per_m};
+    for (int m = 0; m < L0.nmq; m++)
+      for (int a = 0; a < na; a++)
+        for (int s2 = 0; s2 < ns; s2++) {
+          const long q = L0.at(m, a, s2);
+          if (!L0.act[q]) continue;
+          const int id[3] = {s2, a, m};
+          for (int e = 0; e < nd; e++) {
+            // the + link of this cell against the - link of its neighbour
+            int j[3] = {id[0], id[1], id[2]};
+            j[e] += 1;
+            bool wrap = false;
+            if (j[e] == ext[e]) {
+              if (!per[e]) {
+                if (cp[e][q] != 0.0) { P.why = "internal: coupling across the box boundary"; return; }
+                continue;
+              }
+              j[e] = 0;
+              wrap = true;
+            }
+            (void)wrap;
+            const long qn = L0.at(j[2], j[1], j[0]);
+            const double c1 = cp[e][q], c2 = L0.act[qn] ? cmm[qn * 3 + e] : 0.0;
+            if (std::fabs(c1 - c2) > 1e-12 * std::max(std::fabs(c1), std::fabs(c2))) {
+              P.why = "non-conforming block interfaces (asymmetric Poisson operator)";
+              return;
+            }
+          }
+          // a - link must lead to an active cell
+          for (int e = 0; e < nd; e++) {
+            if (cmm[q * 3 + e] == 0.0) continue;
+            int j[3] = {id[0], id[1], id[2]};
+            j[e] -= 1;
+            if (j[e] < 0) {
+              if (!per[e]) { P.why = "internal: coupling across the box boundary"; return; }
+              j[e] = ext[e] - 1;
+            }
+            if (!L0.act[L0.at(j[2], j[1], j[0])]) {
+              P.why = "non-conforming block interfaces (link to a cell outside the union)";
+              return;
+            }
+          }
+        }
+  }
+  // one-cell periodic axes couple a cell to itself: fold into the diagonal
+  if (L0.per_a && na == 1) {
+    for (size_t q = 0; q < L0.dg.size(); q++)
+      if (L0.act[q]) { L0.dg[q] -= 2.0 * L0.ca[q]; L0.ca[q] = 0.0; }
+    L0.per_a = false;
+  }
+  if (L0.per_m && nm == 1) {
+    for (size_t q = 0; q < L0.dg.size(); q++)
+      if (L0.act[q]) { L0.dg[q] -= 2.0 * L0.cm[q]; L0.cm[q] = 0.0; }
+    L0.per_m = false;
+  }
+  // zero row sums at lambda = 0: all-Neumann/periodic, singular
+  bool zero_rowsum = !has_dirichlet;
+  double nact_plane = 0.0;
+  for (size_t q = 0; q < L0.dg.size(); q++) if (L0.act[q]) nact_plane += 1.0;
+
+  // hierarchy
+  std::vector<HostLev> H;
+  H.push_back(std::move(L0));
+  while (H.back().na > 1 || (coupled && H.back().nm > 1)) H.push_back(coarsen(H.back(), coupled));
+  const HostLev &Lc = H.back();
+  P.pin_s = -1;
+  for (int s2 = ns - 1; s2 >= 0 && P.pin_s < 0; s2--)
+    if (Lc.act[Lc.at(0, 0, s2)]) P.pin_s = s2;
+
+  // eigenvalues of the FFT axis (fft.f90:151-189, divided by dl^2 with
+  // dl = dxc_g(lo_g-1), main.f90:717-728)
+  std::vector<double> lamh;
+  if (!coupled) {
+    const Block &B0 = cs.blocks[0];
+    int kf, kb;
+    double nr[2];
+    if (!find_fft(B0.cbcpre[fa][0], B0.cbcpre[fa][1], kf, kb, nr)) { P.why = "unsupported BC pair along the FFT axis"; return; }
+    eigenvalues(nm, B0.cbcpre[fa][0], B0.cbcpre[fa][1], lamh);
+    const double dl = B0.df[fa][1];
+    for (auto &l : lamh) l /= dl * dl;
+    P.norm = 1.0 / (nr[0] * (nm + nr[1]));
+    P.kind_fwd = kf;
+    P.kind_bwd = kb;
+  }
+  P.ng = coupled ? 1 : nm;
+  P.gsize = coupled ? (long)nm * na * ns : (long)na * ns;
+  std::vector<int> sing(P.ng, 0);
+  std::vector<double> nact(P.ng, nact_plane);  // active cells per group
+  P.any_singular = false;
+  for (int g = 0; g < P.ng; g++) {
+    sing[g] = zero_rowsum && (coupled || lamh[g] == 0.0);
+    P.any_singular = P.any_singular || sing[g];
+  }
+  if (P.any_singular && P.pin_s < 0) { P.why = "internal: empty coarsest line"; return; }
+  P.tol = cs.hypre_tol;
+  P.maxiter = cs.hypre_maxiter;
+  P.colors = coupled ? 4 : 2;
+  P.nlev = (int)H.size();
+  if (device < 0) { P.ok = true; return; }
+
+  // device arrays
+  P.lev.clear();
+  P.lam.clear();
+  P.xl.clear();
+  P.bl.clear();
+  const int nch = ns >= 2 ? std::min(64, ns / 2) : 1;
+  double *lam_dev = coupled ? nullptr : dev_upload(P, lamh);
+  for (size_t l = 0; l < H.size(); l++) {
+    const HostLev &h = H[l];
+    MGLev L;
+    L.nm = h.nm; L.na = h.na; L.ns = h.ns;
+    L.coupled = coupled;
+    L.per_a = h.per_a; L.per_m = h.per_m;
+    L.qm = coupled ? (long)h.na * h.ns : 0;
+    L.dg = dev_upload(P, h.dg);
+    L.w = dev_upload(P, h.w);
+    L.cs = dev_upload(P, h.cs);
+    L.ca = dev_upload(P, h.ca);
+    L.cm = coupled ? dev_upload(P, h.cm) : nullptr;
+    L.act = dev_upload(P, h.act);
+    L.nch = nch;
+    L.base = ns >= 2 ? ns / nch : 1;
+    L.extra = ns >= 2 ? ns % nch : 0;
+    P.lev.push_back(L);
+    P.lam.push_back(lam_dev);
+    const size_t n = (size_t)h.nm * h.na * h.ns;
+    if (l == 0) {
+      P.x = dev_zeros<double>(P, n);
+      P.r = dev_zeros<double>(P, n);
+      P.z = dev_zeros<double>(P, n);
+      P.p = dev_zeros<double>(P, n);
+      P.q = dev_zeros<double>(P, n);
+      P.xl.push_back(P.z);
+      P.bl.push_back(P.r);
+    } else {
+      P.xl.push_back(dev_zeros<double>(P, n));
+      P.bl.push_back(dev_zeros<double>(P, n));
+    }
+  }
+  P.nchunk = (int)std::max<long>(1, std::min<long>((P.gsize + 2047) / 2048, std::max(1, 8192 / P.ng)));
+  P.part = dev_zeros<double>(P, (size_t)2 * P.ng * P.nchunk);
+  P.bb = dev_zeros<double>(P, 2 * P.ng);
+  P.rr = dev_zeros<double>(P, 2 * P.ng);
+  P.pq = dev_zeros<double>(P, 2 * P.ng);
+  P.rz[0] = dev_zeros<double>(P, 2 * P.ng);
+  P.rz[1] = dev_zeros<double>(P, 2 * P.ng);
+  P.sums = dev_zeros<double>(P, 2 * P.ng);
+  P.nact = dev_upload(P, nact);
+  P.done = dev_zeros<int>(P, P.ng);
+  P.singular = dev_upload(P, sing);
+  P.left = dev_zeros<int>(P, 1);
+  SNAC_HIP(hipHostMalloc((void **)&P.left_host, sizeof(int), hipHostMallocDefault));
+  P.ok = true;
+}
+
+namespace {
+
+void vcycle(MGPlan &P, hipStream_t st, size_t l) {
+  const MGLev &L = P.lev[l];
+  double *x = P.xl[l];
+  const double *b = P.bl[l];
+  if (l + 1 == P.lev.size()) {  // one line per group: direct solve
+    mg_line(st, L, P.lam[l], x, b, 0, 0, 1, P.pin_s, P.singular, P.done);
+    return;
+  }
+  int col[4][2], nc = 0;
+  const bool two_a = L.na > 1, two_m = L.coupled && L.nm > 1;
+  for (int cm = 0; cm <= (two_m ? 1 : 0); cm++)
+    for (int ca = 0; ca <= (two_a ? 1 : 0); ca++) { col[nc][0] = ca; col[nc][1] = cm; nc++; }
+  const bool zero_first = nc > 2;
+  if (zero_first) mg_zero(st, (long)L.nm * L.na * L.ns, x);
+  for (int k = 0; k < nc; k++)
+    mg_line(st, L, P.lam[l], x, b, col[k][0], col[k][1], (!zero_first && k == 0) ? 1 : 0, -1, nullptr, P.done);
+  mg_restrict_residual(st, L, P.lev[l + 1], P.lam[l], P.lam[l + 1], x, b, P.bl[l + 1], P.done);
+  vcycle(P, st, l + 1);
+  mg_prolong(st, L, P.lev[l + 1], P.xl[l + 1], x, P.done);
+  for (int k = nc - 1; k >= 0; k--)
+    mg_line(st, L, P.lam[l], x, b, col[k][0], col[k][1], 0, -1, nullptr, P.done);
+}
+
+void project(MGPlan &P, hipStream_t st, double *v) {
+  if (!P.any_singular) return;
+  mg_dot(st, P.ng, P.gsize, nullptr, nullptr, v, nullptr, P.part, P.nchunk, P.sums, P.done, P.singular);
+  mg_sub_mean(st, P.lev[0], v, P.sums, P.singular, P.nact, P.done);
+}
+
+}  // namespace
+
+void mg_solve(MGPlan &P, hipStream_t st) {
+  const long n = (long)P.ng * P.gsize;
+  const double tol2 = P.tol * P.tol;
+  mg_zero(st, n, P.x);
+  mg_izero(st, P.ng, P.done);
+  project(P, st, P.r);
+  mg_dot(st, P.ng, P.gsize, P.r, P.r, nullptr, nullptr, P.part, P.nchunk, P.bb, P.done, nullptr);
+  mg_flags(st, P.ng, P.bb, P.bb, tol2, P.done, P.left);
+  vcycle(P, st, 0);
+  project(P, st, P.z);
+  int cur = 0;
+  mg_dot(st, P.ng, P.gsize, P.r, P.z, nullptr, nullptr, P.part, P.nchunk, P.rz[cur], P.done, nullptr);
+  mg_cg_dir(st, P.ng, P.gsize, nullptr, nullptr, P.z, P.p, P.done);
+  int it = 0;
+  for (it = 1; it <= P.maxiter; it++) {
+    mg_apply(st, P.lev[0], P.lam[0], P.p, P.q, P.done);
+    mg_dot(st, P.ng, P.gsize, P.p, P.q, nullptr, nullptr, P.part, P.nchunk, P.pq, P.done, nullptr);
+    mg_cg_update(st, P.ng, P.gsize, P.rz[cur], P.pq, P.x, P.p, P.r, P.q, P.done);
+    mg_dot(st, P.ng, P.gsize, P.r, P.r, nullptr, nullptr, P.part, P.nchunk, P.rr, P.done, nullptr);
+    mg_flags(st, P.ng, P.rr, P.bb, tol2, P.done, P.left);
+    SNAC_HIP(hipMemcpyAsync(P.left_host, P.left, sizeof(int), hipMemcpyDeviceToHost, st));
+    SNAC_HIP(hipStreamSynchronize(st));
+    if (*P.left_host == 0) break;
+    vcycle(P, st, 0);
+    project(P, st, P.z);
+    mg_dot(st, P.ng, P.gsize, P.r, P.z, nullptr, nullptr, P.part, P.nchunk, P.rz[1 - cur], P.done, nullptr);
+    mg_cg_dir(st, P.ng, P.gsize, P.rz[1 - cur], P.rz[cur], P.z, P.p, P.done);
+    cur = 1 - cur;
+  }
+  P.last_iters = std::min(it, P.maxiter);
+}
+
+void mg_stats(MGPlan &P, hipStream_t st, int &iters, double &max_relres) {
+  std::vector<double> rr(2 * P.ng), bb(2 * P.ng);
+  SNAC_HIP(hipMemcpyAsync(rr.data(), P.rr, rr.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  SNAC_HIP(hipMemcpyAsync(bb.data(), P.bb, bb.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  SNAC_HIP(hipStreamSynchronize(st));
+  iters = P.last_iters;
+  max_relres = 0.0;
+  for (int g = 0; g < P.ng; g++)
+    if (bb[2 * g] > 0.0) max_relres = std::max(max_relres, std::sqrt(rr[2 * g] / bb[2 * g]));
+}
+
+}  // namespace snac

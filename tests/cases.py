@@ -83,6 +83,72 @@ def cavity_block(ng, lid=1.0, gr=(0.0, 0.0, 0.0)):
                 bcpre=[[0.0, 0.0], [0.0, 0.0], [0.0, 0.0]])
 
 
+def _block(ng, lmin, lmax, vel, pre, gr=(0.0, 0.0, 0.0), vals=None, inflow=None):
+    """vel / pre: [dir][side] BC characters (shared by the three velocity
+    components); vals: {(comp, dir, side): value} for velocity BC values and
+    friend numbers are taken from `friends` entries ('F', b)."""
+    cbcv = [[[vel[d][s] if isinstance(vel[d][s], str) else vel[d][s][0] for s in range(2)] for d in range(3)]
+            for _ in range(3)]
+    fr = [[0.0 if isinstance(vel[d][s], str) else float(vel[d][s][1]) for s in range(2)] for d in range(3)]
+    bcv = [[[fr[d][s] for s in range(2)] for d in range(3)] for _ in range(3)]
+    for (c, d, s), v in (vals or {}).items():
+        bcv[c][d][s] = v
+    cbcp = [[pre[d][s] if isinstance(pre[d][s], str) else pre[d][s][0] for s in range(2)] for d in range(3)]
+    bcp = [[fr[d][s] for s in range(2)] for d in range(3)]
+    B = dict(ng=list(ng), lmin=list(lmin), lmax=list(lmax), gt=[0, 0, 0], gr=list(gr),
+             cbcvel=cbcv, bcvel=bcv, cbcpre=cbcp, bcpre=bcp)
+    if inflow:
+        it = [[0, 0], [0, 0], [0, 0]]
+        for (d, s), t in inflow.items():
+            it[d][s] = t
+        B["inflow_type"] = it
+    return B
+
+
+def triperiodic_block(ng, lengths=(6.283185307179586,) * 3):
+    """A periodic box along all three axes (examples/triperiodic)."""
+    F = ("F", 1)
+    return _block(ng, (0, 0, 0), lengths, vel=[[F, F]] * 3, pre=[[F, F]] * 3)
+
+
+def lchannel_blocks(n=8, nz=16):
+    """The reference's multi-block L-shaped channel
+    (examples/__MULTI_BLOCK_GEOMETRY/L_channel/blocks.nml) at n x n cells per
+    unit block: v = 1 enters at y = 0 under block 1, turns into block 3 and
+    leaves at x = 2 (velocity N, pressure D); walls at z = 0, 1 (the FFT
+    axis, a cosine transform)."""
+    F = lambda b: ("F", b)
+    b1 = _block((n, n, nz), (0, 0, 0), (0.5, 0.5, 1.0),
+                vel=[["D", "D"], ["D", F(2)], ["D", "D"]], pre=[["N", "N"], ["N", F(2)], ["N", "N"]],
+                vals={(1, 1, 0): 1.0})
+    b2 = _block((n, n, nz), (0, 0.5, 0), (0.5, 1.0, 1.0),
+                vel=[["D", F(3)], [F(1), "D"], ["D", "D"]], pre=[["N", F(3)], [F(1), "N"], ["N", "N"]])
+    b3 = _block((3 * n, n, nz), (0.5, 0.5, 0), (2.0, 1.0, 1.0),
+                vel=[[F(2), "N"], ["D", "D"], ["D", "D"]], pre=[[F(2), "D"], ["N", "N"], ["N", "N"]])
+    return [b1, b2, b3]
+
+
+def bfs_blocks(n_in=(192, 64), n_out=(320, 64, 96), nz=128, lengths=(3.0, 5.0, 1.0, 1.5, 2.0), gr=(1.2, 1.2)):
+    """Backward-facing step of three blocks: inlet (n_in[0] x n_in[1]) with a
+    Poiseuille inflow at x- (inflow_type 1), lower block (n_out[0] x n_out[1])
+    downstream of it sharing its y-grid, upper block (n_out[0] x n_out[2]) on
+    top of the lower one; outflow at x+ (velocity N, pressure D), walls in y
+    (tanh stretching gr per wall-normal block) and at the step face, z
+    periodic (the FFT axis).  lengths = (L_inlet, L_out, h_low, h_up, L_z)."""
+    F = lambda b: ("F", b)
+    lin, lout, h1, h2, lz = lengths
+    inlet = _block((n_in[0], n_in[1], nz), (0, 0, 0), (lin, h1, lz),
+                   vel=[["D", F(2)], ["D", "D"], [F(1), F(1)]], pre=[["N", F(2)], ["N", "N"], [F(1), F(1)]],
+                   gr=(0.0, gr[0], 0.0), vals={(0, 0, 0): 1.0}, inflow={(0, 0): 1})
+    lower = _block((n_out[0], n_out[1], nz), (lin, 0, 0), (lin + lout, h1, lz),
+                   vel=[[F(1), "N"], ["D", F(3)], [F(2), F(2)]], pre=[[F(1), "D"], ["N", F(3)], [F(2), F(2)]],
+                   gr=(0.0, gr[0], 0.0))
+    upper = _block((n_out[0], n_out[2], nz), (lin, h1, 0), (lin + lout, h1 + h2, lz),
+                   vel=[["D", "N"], [F(2), "D"], [F(3), F(3)]], pre=[["N", "D"], [F(2), "N"], [F(3), F(3)]],
+                   gr=(0.0, gr[1], 0.0))
+    return [inlet, lower, upper]
+
+
 def write_case(dirpath, blocks, visci, **dns):
     os.makedirs(dirpath, exist_ok=True)
     d = os.path.join(dirpath, "dns.nml")

@@ -143,11 +143,14 @@ def test_chkdiv_chkdt(tmp_path):
     assert ctx.chkdt() == pytest.approx(oc.chkdt(), rel=1e-14)
 
 
-@pytest.mark.parametrize("geom", ["channel", "cavity", "ldc_fft", "ldc_3d"])
+@pytest.mark.parametrize("geom", ["channel", "cavity", "ldc_fft", "ldc_3d", "triperiodic", "triperiodic_3d"])
 def test_solve_pressure(tmp_path, geom):
     import snac_amd
     if geom == "channel":
         ctx, oc = channel(tmp_path)
+    elif geom.startswith("triperiodic"):
+        ctx, oc = make_pair(tmp_path, [cases.triperiodic_block((16, 12, 8))], 0 if geom.endswith("3d") else 3,
+                            visci=100.0)
     elif geom == "cavity":
         ctx, oc = make_pair(tmp_path, [cases.cavity_block((12, 10, 16))], 0, visci=1000.0)
     else:
@@ -194,6 +197,14 @@ def test_channel_10_steps(tmp_path):
 
 def test_cavity_10_steps(tmp_path):
     ctx, oc = make_pair(tmp_path, [cases.cavity_block((16, 16, 16))], 0, visci=1000.0)
+    run_both(ctx, oc, 10)
+    assert_fields(ctx, oc, FIELDS3, 1e-10, interior=True)
+    assert_fields(ctx, oc, ("p",), 1e-10, interior=True, modmean=("p",))
+
+
+def test_triperiodic_10_steps(tmp_path):
+    ctx, oc = make_pair(tmp_path, [cases.triperiodic_block((16, 12, 8))], 3, visci=100.0)
+    seed_fields(ctx, oc, names=("u", "v", "w"))
     run_both(ctx, oc, 10)
     assert_fields(ctx, oc, FIELDS3, 1e-10, interior=True)
     assert_fields(ctx, oc, ("p",), 1e-10, interior=True, modmean=("p",))
