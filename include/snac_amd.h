@@ -15,7 +15,8 @@
  * Array layout at the boundary (identical to the reference's arrays,
  * main.f90:226-236): column-major, i fastest, one ghost layer per side, i.e.
  * a block with interior n = (nx,ny,nz) exchanges (nx+2)*(ny+2)*(nz+2) doubles
- * for u,v,w,p,pp,po and nx*ny*nz doubles for dudtrko/dvdtrko/dwdtrko.
+ * for u,v,w,p,pp,po and the scalars, nx*ny*nz doubles for dudtrko/dvdtrko/
+ * dwdtrko and the scalars' dsdtrko.
  *
  * Reference interfaces replaced (paths relative to the SNaC source tree):
  * each declaration names the routine(s) it stands in for.
@@ -29,10 +30,11 @@ extern "C" {
 
 typedef struct snac_ctx snac_ctx;
 
-/* field ids for upload/download */
+/* field ids for upload/download; scalar i (0-based, scalars(i+1)%val of
+ * scal.f90:22-49) is SNAC_SCAL0 + 2*i, its dsdtrko SNAC_SCAL0 + 2*i + 1 */
 enum {
   SNAC_U = 0, SNAC_V = 1, SNAC_W = 2, SNAC_P = 3, SNAC_PP = 4, SNAC_PO = 5,
-  SNAC_DUDTRKO = 6, SNAC_DVDTRKO = 7, SNAC_DWDTRKO = 8
+  SNAC_DUDTRKO = 6, SNAC_DVDTRKO = 7, SNAC_DWDTRKO = 8, SNAC_SCAL0 = 16
 };
 
 /* Poisson solver choice for snac_ctx_set_solver: AUTO takes the direct
@@ -88,13 +90,24 @@ int snac_download_field(snac_ctx *ctx, int b, int fid, double *host);
 /* ---- hot-path routines, all blocks at once, queued on the context's
  *      stream; results are visible after snac_sync or a download ----       */
 /* rk_mom rk.f90:7-128 with momx/y/z_d, momx/y/z_a_vv, momx/y/z_p
- * (mom.f90:112-386); irk in 1..3 selects rkcoeff (param.f90:11-13)          */
+ * (mom.f90:112-386); irk in 1..3 selects rkcoeff (param.f90:11-13).  With
+ * scalars (nscal > 0) it adds the Boussinesq buoyancy of scalar 1
+ * (rk.f90:104-108, the reference's _BOUSSINESQ_BUOYANCY build; beta = 0 or
+ * gacc = 0 switch it off)                                                   */
 int snac_rk_mom(snac_ctx *ctx, int irk, double dt);
+/* rk_scal rk.f90:130-204 (scal_d / scal_a scal.f90:89-155) followed by
+ * bulk_mean_s / bulk_forcing_s scal.f90:157-199 for every scalar, as
+ * main.f90:908-913; ghosts are filled by snac_boundp(SNAC_SCAL0 + 2*i)      */
+int snac_rk_scal(snac_ctx *ctx, int irk, double dt);
+/* the bulk forcing f of every scalar from the last snac_rk_scal (0 for the
+ * scalars that are not forced); nscal doubles; synchronises                 */
+int snac_scalar_forcing(snac_ctx *ctx, double *f);
 /* bounduvw bound.f90:8-66 (updthalo 587-650 becomes device plane copies)    */
 int snac_bounduvw(snac_ctx *ctx, int is_correc);
 /* inflow bound.f90:438-477                                                  */
 int snac_inflow(snac_ctx *ctx, int is_correc);
-/* boundp bound.f90:68-106; fid = SNAC_P or SNAC_PP                          */
+/* boundp bound.f90:68-106; fid = SNAC_P, SNAC_PP or a scalar
+ * (SNAC_SCAL0 + 2*i, with that scalar's cbcscal/bcscal)                     */
 int snac_boundp(snac_ctx *ctx, int fid);
 /* fillps fillps.f90:7-54 followed by updt_rhs bound.f90:543-585             */
 int snac_fillps(snac_ctx *ctx, double dtrk);
@@ -133,7 +146,8 @@ enum {
   SNAC_PH_TRIDIAG = 4,  /* per-mode solves: tridiagonal lines or MG-PCG      */
   SNAC_PH_R2R_BWD = 5,  /* inverse transforms and normfft scaling           */
   SNAC_PH_CORREC = 6,   /* correc + updt_pressure                           */
-  SNAC_NPHASE = 7
+  SNAC_PH_RK_SCAL = 7,  /* rk_scal + bulk forcing of the scalars            */
+  SNAC_NPHASE = 8
 };
 int snac_set_timing(snac_ctx *ctx, int enable);
 /* waits for the stream, returns accumulated milliseconds and the number of

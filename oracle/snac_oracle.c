@@ -36,6 +36,7 @@
 #endif
 
 #define MAXB 64
+#define MAXS 8
 #define PI 3.14159265358979323846264338327950288
 
 typedef struct {
@@ -52,6 +53,9 @@ typedef struct {
   double *dto[3];                         /* dudtrko... interior */
   double *vin[3][3][2];                   /* inflow planes [idir][comp][side] */
   double *rhsp[3][2];                     /* BC rhs planes [dir][side] */
+  double *sv[MAXS], *sdo[MAXS];           /* scalars (ghosted) and dsdtrko (interior) */
+  char cbcsca[MAXS][3][2];                /* scalar BCs [iscal][dir][side] */
+  double bcsca[MAXS][3][2];
   long sx, sy, ntot;                      /* strides of ghosted arrays */
   long off;                               /* offset in the solver vector */
 } ob;
@@ -59,6 +63,10 @@ typedef struct {
 struct oc_case {
   int nb, fft_axis;
   double visc, bforce[3], vol_all;
+  /* scalars (param.f90:96-100, scal.f90:22-49) */
+  int nscal;
+  double salpha[MAXS], ssource[MAXS], scalf[MAXS], sf[MAXS], beta, gacc[3], alpha_max;
+  int is_sforced[MAXS];
   int periods[3], is_periodic[3];
   ob b[MAXB];
   /* solver */
@@ -214,6 +222,13 @@ static void set_bc(ob *B, double *p, char ctype, int ibound, int d, int centered
  * between friend blocks, initmpi.f90:165-175).  Reads touch only interior
  * planes along d and writes only ghost planes along d, so the sweep over
  * blocks is order independent.                                               */
+/* field id -> array: 0..5 u,v,w,p,pp,po ; 9+2i scalar i                    */
+static double *farr(ob *B, int fid) {
+  if (fid >= 0 && fid < 6) return B->f[fid];
+  if (fid >= 9 && (fid - 9) % 2 == 0 && (fid - 9) / 2 < MAXS) return B->sv[(fid - 9) / 2];
+  return NULL;
+}
+
 static void updthalo(oc_case *c, int fid, int d) {
   for (int ib = 0; ib < c->nb; ib++) {
     ob *B = &c->b[ib];
@@ -225,8 +240,8 @@ static void updthalo(oc_case *c, int fid, int d) {
       ob *N = &c->b[jb];
       int qdst = side == 0 ? 0 : B->n[d] + 1;
       int qsrc = side == 0 ? N->n[d] : 1;
-      double *dst = B->f[fid];
-      const double *src = N->f[fid];
+      double *dst = farr(B, fid);
+      const double *src = farr(N, fid);
       for (int t2 = 0; t2 < m2; t2++)
         for (int t1 = 0; t1 < m1; t1++)
           dst[plane_idx(B, d, qdst, t1, t2)] = src[plane_idx(N, d, qsrc, t1, t2)];
@@ -255,8 +270,11 @@ void oc_bounduvw(oc_case *c, int is_correc) {
   }
 }
 
-/* boundp bound.f90:68-106                                                   */
+/* boundp bound.f90:68-106 ; for a scalar (which = 9 + 2*iscal) with the
+ * scalar's own BCs, as main.f90:565-568,934 call it                         */
 void oc_boundp(oc_case *c, int which) {
+  int is = which >= 9 ? (which - 9) / 2 : -1;
+  if (is >= c->nscal) return;
   for (int d = 0; d < 3; d++) updthalo(c, which, d);
   for (int ib = 0; ib < c->nb; ib++) {
     ob *B = &c->b[ib];
@@ -264,7 +282,10 @@ void oc_boundp(oc_case *c, int which) {
       for (int side = 0; side < 2; side++) {
         if (!B->is_bound[side][d]) continue;
         int qm = side == 0 ? 0 : B->n[d];
-        set_bc(B, B->f[which], B->cbcpre[d][side], side, d, 1, B->bcpre[d][side], B->dc[d][qm]);
+        if (is < 0)
+          set_bc(B, B->f[which], B->cbcpre[d][side], side, d, 1, B->bcpre[d][side], B->dc[d][qm]);
+        else
+          set_bc(B, B->sv[is], B->cbcsca[is][d][side], side, d, 1, B->bcsca[is][d][side], B->dc[d][qm]);
       }
   }
 }
@@ -413,6 +434,12 @@ static void rk_mom_block(oc_case *c, ob *B, int irk, double dt) {
         u[g] = u[g] + f12 * px;
         v[g] = v[g] + f12 * py;
         w[g] = w[g] + f12 * pz;
+        if (c->nscal > 0) { /* Boussinesq buoyancy rk.f90:104-108 */
+          const double *S = B->sv[0];
+          u[g] = u[g] - f12 * c->gacc[0] * c->beta * 0.5 * (S[IDX(B, i + 1, j, k)] + S[g]);
+          v[g] = v[g] - f12 * c->gacc[1] * c->beta * 0.5 * (S[IDX(B, i, j + 1, k)] + S[g]);
+          w[g] = w[g] - f12 * c->gacc[2] * c->beta * 0.5 * (S[IDX(B, i, j, k + 1)] + S[g]);
+        }
       }
 #undef U
 #undef V
@@ -555,7 +582,9 @@ double oc_chkdt(const oc_case *c) {
   }
   if (dti == 0.0) dti = 1.0;
   double dtmax = sqrt(3.0) / dti;
-  double dd = 1.65 / 12.0 / c->visc * dlmin * dlmin;
+  /* main.f90:660 passes max(visc, alpha_max) */
+  double ve = c->nscal > 0 && c->alpha_max > c->visc ? c->alpha_max : c->visc;
+  double dd = 1.65 / 12.0 / ve * dlmin * dlmin;
   return dtmax < dd ? dtmax : dd;
 }
 
@@ -990,6 +1019,7 @@ void oc_destroy(oc_case *c) {
     for (int d = 0; d < 3; d++) { free(B->dc[d]); free(B->df[d]); free(B->rc[d]); free(B->rf[d]); }
     for (int f = 0; f < 6; f++) free(B->f[f]);
     for (int f = 0; f < 3; f++) free(B->dto[f]);
+    for (int is = 0; is < c->nscal; is++) { free(B->sv[is]); free(B->sdo[is]); }
     for (int d = 0; d < 3; d++)
       for (int s = 0; s < 2; s++) {
         free(B->rhsp[d][s]);
@@ -1020,24 +1050,149 @@ int oc_get_metric(const oc_case *c, int b, int which, double *out) {
   memcpy(out, src, sizeof(double) * (B->n[d] + 2));
   return 0;
 }
+/* field id -> (array, ghosted?) including scalars 9+2i / dsdtrko 10+2i     */
+static double *field_of(const oc_case *c, int b, int fid, int *ghosted) {
+  if (b < 0 || b >= c->nb || fid < 0) return NULL;
+  ob *B = (ob *)&c->b[b];
+  if (fid < 6) { *ghosted = 1; return B->f[fid]; }
+  if (fid < 9) { *ghosted = 0; return B->dto[fid - 6]; }
+  int is = (fid - 9) / 2;
+  if (is >= c->nscal) return NULL;
+  *ghosted = (fid - 9) % 2 == 0;
+  return *ghosted ? B->sv[is] : B->sdo[is];
+}
 int oc_field_size(const oc_case *c, int b, int fid) {
+  int gh = 0;
+  if (!field_of(c, b, fid, &gh)) return -1;
   const ob *B = &c->b[b];
-  if (fid >= 0 && fid < 6) return (int)B->ntot;
-  return B->n[0] * B->n[1] * B->n[2];
+  return gh ? (int)B->ntot : B->n[0] * B->n[1] * B->n[2];
 }
 int oc_get_field(const oc_case *c, int b, int fid, double *out) {
-  if (b < 0 || b >= c->nb || fid < 0 || fid > 8) return -1;
-  const ob *B = &c->b[b];
-  if (fid < 6) memcpy(out, B->f[fid], sizeof(double) * B->ntot);
-  else memcpy(out, B->dto[fid - 6], sizeof(double) * B->n[0] * B->n[1] * B->n[2]);
+  int n = oc_field_size(c, b, fid), gh;
+  if (n < 0) return -1;
+  memcpy(out, field_of(c, b, fid, &gh), sizeof(double) * n);
   return 0;
 }
 int oc_set_field(oc_case *c, int b, int fid, const double *in) {
-  if (b < 0 || b >= c->nb || fid < 0 || fid > 8) return -1;
-  ob *B = &c->b[b];
-  if (fid < 6) memcpy(B->f[fid], in, sizeof(double) * B->ntot);
-  else memcpy(B->dto[fid - 6], in, sizeof(double) * B->n[0] * B->n[1] * B->n[2]);
+  int n = oc_field_size(c, b, fid), gh;
+  if (n < 0) return -1;
+  memcpy(field_of(c, b, fid, &gh), in, sizeof(double) * n);
   return 0;
+}
+
+/* scalar inputs: &scalar of dns.nml (param.f90:96-100,148-163) and
+ * block_cbcscal/block_bcscal of blocks.nml (flat [b][iscal][dir][side])    */
+int oc_set_scalars(oc_case *c, int nscal, const double *alphai, double beta, const double *gacc,
+                   const double *ssource, const int *is_sforced, const double *scalf,
+                   const char *cbcscal, const double *bcscal) {
+  if (nscal < 0 || nscal > MAXS) return -1;
+  for (int ib = 0; ib < c->nb; ib++)
+    for (int is = 0; is < c->nscal; is++) { free(c->b[ib].sv[is]); free(c->b[ib].sdo[is]); }
+  c->nscal = nscal;
+  c->beta = beta;
+  for (int q = 0; q < 3; q++) c->gacc[q] = gacc[q];
+  c->alpha_max = 0.0;
+  for (int is = 0; is < nscal; is++) {
+    c->salpha[is] = 1.0 / alphai[is];
+    if (is == 0 || c->salpha[is] > c->alpha_max) c->alpha_max = c->salpha[is];
+    c->ssource[is] = ssource[is];
+    c->is_sforced[is] = is_sforced[is];
+    c->scalf[is] = scalf[is];
+    c->sf[is] = 0.0;
+  }
+  for (int ib = 0; ib < c->nb; ib++) {
+    ob *B = &c->b[ib];
+    long nin = (long)B->n[0] * B->n[1] * B->n[2];
+    for (int is = 0; is < nscal; is++) {
+      B->sv[is] = xcalloc(B->ntot, sizeof(double));
+      B->sdo[is] = xcalloc(nin, sizeof(double));
+      for (int d = 0; d < 3; d++)
+        for (int sd = 0; sd < 2; sd++) {
+          long q = (((long)ib * nscal + is) * 3 + d) * 2 + sd;
+          B->cbcsca[is][d][sd] = cbcscal[q];
+          B->bcsca[is][d][sd] = bcscal[q];
+        }
+    }
+  }
+  return 0;
+}
+
+/* rk_scal rk.f90:130-204 with scal_d / scal_a scal.f90:89-155, then
+ * bulk_mean_s / bulk_forcing_s scal.f90:157-199 (main.f90:908-913)          */
+void oc_rk_scal(oc_case *c, int irk, double dt, double *fout) {
+  const double f1 = RKC[irk - 1][0] * dt, f2 = RKC[irk - 1][1] * dt, f12 = f1 + f2;
+  for (int is = 0; is < c->nscal; is++) {
+    const double alpha = c->salpha[is];
+    for (int ib = 0; ib < c->nb; ib++) {
+      ob *B = &c->b[ib];
+      const int nx = B->n[0], ny = B->n[1], nz = B->n[2];
+      const double *u = B->f[0], *v = B->f[1], *w = B->f[2];
+      double *s = B->sv[is], *dso = B->sdo[is];
+      const double *dxc = B->dc[0], *dxf = B->df[0], *dyc = B->dc[1], *dyf = B->df[1],
+                   *dzc = B->dc[2], *dzf = B->df[2];
+      double *ds = xcalloc((long)nx * ny * nz, sizeof(double));
+#define S(i, j, k) s[IDX(B, i, j, k)]
+#define U(i, j, k) u[IDX(B, i, j, k)]
+#define V(i, j, k) v[IDX(B, i, j, k)]
+#define W(i, j, k) w[IDX(B, i, j, k)]
+#pragma omp parallel for collapse(2) schedule(static)
+      for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+          for (int i = 1; i <= nx; i++) {
+            double d = 0.0;
+            double dsdxp = (S(i + 1, j, k) - S(i, j, k)) / dxc[i];
+            double dsdxm = (S(i, j, k) - S(i - 1, j, k)) / dxc[i - 1];
+            double dsdyp = (S(i, j + 1, k) - S(i, j, k)) / dyc[j];
+            double dsdym = (S(i, j, k) - S(i, j - 1, k)) / dyc[j - 1];
+            double dsdzp = (S(i, j, k + 1) - S(i, j, k)) / dzc[k];
+            double dsdzm = (S(i, j, k) - S(i, j, k - 1)) / dzc[k - 1];
+            d = d + alpha * (dsdxp - dsdxm) / dxf[i] + alpha * (dsdyp - dsdym) / dyf[j] +
+                alpha * (dsdzp - dsdzm) / dzf[k];
+            double usim = 0.5 * (S(i - 1, j, k) + S(i, j, k)) * U(i - 1, j, k);
+            double usip = 0.5 * (S(i + 1, j, k) + S(i, j, k)) * U(i, j, k);
+            double vsjm = 0.5 * (S(i, j - 1, k) + S(i, j, k)) * V(i, j - 1, k);
+            double vsjp = 0.5 * (S(i, j + 1, k) + S(i, j, k)) * V(i, j, k);
+            double wskm = 0.5 * (S(i, j, k - 1) + S(i, j, k)) * W(i, j, k - 1);
+            double wskp = 0.5 * (S(i, j, k + 1) + S(i, j, k)) * W(i, j, k);
+            d = d + (-usip + usim) / dxf[i] + (-vsjp + vsjm) / dyf[j] + (-wskp + wskm) / dzf[k];
+            ds[IIDX(B, i, j, k)] = d;
+          }
+#pragma omp parallel for collapse(2) schedule(static)
+      for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+          for (int i = 1; i <= nx; i++) {
+            long q = IIDX(B, i, j, k);
+            S(i, j, k) = S(i, j, k) + f1 * ds[q] + f2 * dso[q] + f12 * c->ssource[is];
+            dso[q] = ds[q];
+          }
+#undef S
+#undef U
+#undef V
+#undef W
+      free(ds);
+    }
+    double f = 0.0;
+    if (c->is_sforced[is]) {
+      double mean = 0.0;
+      for (int ib = 0; ib < c->nb; ib++) {
+        ob *B = &c->b[ib];
+        for (int k = 1; k <= B->n[2]; k++)
+          for (int j = 1; j <= B->n[1]; j++)
+            for (int i = 1; i <= B->n[0]; i++)
+              mean += B->sv[is][IDX(B, i, j, k)] * B->df[0][i] * B->df[1][j] * B->df[2][k];
+      }
+      mean = mean / c->vol_all;
+      f = c->scalf[is] - mean;
+      for (int ib = 0; ib < c->nb; ib++) {
+        ob *B = &c->b[ib];
+        for (int k = 1; k <= B->n[2]; k++)
+          for (int j = 1; j <= B->n[1]; j++)
+            for (int i = 1; i <= B->n[0]; i++) B->sv[is][IDX(B, i, j, k)] += f;
+      }
+    }
+    c->sf[is] = f;
+    if (fout) fout[is] = f;
+  }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1586,13 +1741,18 @@ int oc_solve_pressure(oc_case *c) {
 void oc_init_bounds(oc_case *c) {
   oc_bounduvw(c, 0);
   oc_boundp(c, 3);
+  for (int is = 0; is < c->nscal; is++) oc_boundp(c, 9 + 2 * is);
   oc_inflow(c, 0);
 }
 
-/* main.f90:893-1033, one time step (nscal = 0)                              */
+/* main.f90:893-1033, one time step                                          */
 int oc_step(oc_case *c, double dt) {
   for (int irk = 1; irk <= 3; irk++) {
     double dtrk = (RKC[irk - 1][0] + RKC[irk - 1][1]) * dt;
+    if (c->nscal > 0) {
+      oc_rk_scal(c, irk, dt, NULL);
+      for (int is = 0; is < c->nscal; is++) oc_boundp(c, 9 + 2 * is);
+    }
     oc_rk_mom(c, irk, dt);
     oc_bounduvw(c, 0);
     oc_inflow(c, 0);

@@ -57,17 +57,29 @@ double oc_vol_all(const oc_case *c);
 int  oc_get_metric(const oc_case *c, int b, int which, double *out);
 
 /* field access; fid: 0 u,1 v,2 w,3 p,4 pp,5 po (ghosted (n+2)^3),
- *                    6 dudtrko,7 dvdtrko,8 dwdtrko (interior n^3) */
+ *                    6 dudtrko,7 dvdtrko,8 dwdtrko (interior n^3),
+ *                    9+2i scalar i (ghosted), 10+2i its dsdtrko (interior) */
 int  oc_field_size(const oc_case *c, int b, int fid);
 int  oc_get_field(const oc_case *c, int b, int fid, double *out);
 int  oc_set_field(oc_case *c, int b, int fid, const double *in);
 
+/* scalar transport inputs (param.f90:96-100,148-163): per-scalar alphai,
+ * ssource, is_sforced, scalf; beta and gacc(3) of the Boussinesq term
+ * (applied whenever nscal > 0, the reference's _BOUSSINESQ_BUOYANCY build);
+ * cbcscal/bcscal flat [b][iscal][dir][side].                                */
+int oc_set_scalars(oc_case *c, int nscal, const double *alphai, double beta, const double *gacc,
+                   const double *ssource, const int *is_sforced, const double *scalf,
+                   const char *cbcscal, const double *bcscal);
+
 /* ---- hot-path routines (all blocks at once) ---- */
+/* rk_scal rk.f90:130-204 + bulk_forcing_s scal.f90:182-199 for every scalar;
+ * f (nscal entries, may be NULL) receives the bulk forcing of this substage */
+void oc_rk_scal(oc_case *c, int irk, double dt, double *f);
 /* rk.f90:7-128 + mom.f90:112-386, rkcoeff param.f90:11-13; irk in 1..3      */
 void oc_rk_mom(oc_case *c, int irk, double dt);
 /* bound.f90:8-66 (+ updthalo 587-650 over the block neighbour table)        */
 void oc_bounduvw(oc_case *c, int is_correc);
-/* bound.f90:68-106 ; which: 3 = p, 4 = pp                                    */
+/* bound.f90:68-106 ; which: 3 = p, 4 = pp, 9+2i = scalar i (its own BCs)    */
 void oc_boundp(oc_case *c, int which);
 /* bound.f90:438-477                                                          */
 void oc_inflow(oc_case *c, int is_correc);

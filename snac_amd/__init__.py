@@ -18,8 +18,17 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "libsnac_amd.so")
 
 FIELDS = {"u": 0, "v": 1, "w": 2, "p": 3, "pp": 4, "po": 5, "dudtrko": 6, "dvdtrko": 7, "dwdtrko": 8}
+SCAL0 = 16
+for _i in range(1, 9):  # scalars s1.. (scalars(i)%val) and their dsdtrko1..
+    FIELDS[f"s{_i}"] = SCAL0 + 2 * (_i - 1)
+    FIELDS[f"dsdtrko{_i}"] = SCAL0 + 2 * (_i - 1) + 1
+
+
+def _ghosted(fid):
+    f = FIELDS[fid]
+    return f < 6 or (f >= SCAL0 and (f - SCAL0) % 2 == 0)
 RKCOEFF = ((32.0 / 60.0, 0.0), (25.0 / 60.0, -17.0 / 60.0), (45.0 / 60.0, -25.0 / 60.0))
-PHASES = ("rk_mom", "bound", "fillps", "r2r_fwd", "tridiag", "r2r_bwd", "correc")
+PHASES = ("rk_mom", "bound", "fillps", "r2r_fwd", "tridiag", "r2r_bwd", "correc", "rk_scal")
 R2R_KINDS = {"R2HC": 0, "HC2R": 1, "REDFT00": 3, "REDFT01": 4, "REDFT10": 5, "REDFT11": 6,
              "RODFT00": 7, "RODFT01": 8, "RODFT10": 9, "RODFT11": 10}
 
@@ -48,6 +57,8 @@ def _load():
     L.snac_upload_field.argtypes = [vp, C.c_int, C.c_int, dp]
     L.snac_download_field.argtypes = [vp, C.c_int, C.c_int, dp]
     L.snac_rk_mom.argtypes = [vp, C.c_int, C.c_double]
+    L.snac_rk_scal.argtypes = [vp, C.c_int, C.c_double]
+    L.snac_scalar_forcing.argtypes = [vp, dp]
     L.snac_bounduvw.argtypes = [vp, C.c_int]
     L.snac_inflow.argtypes = [vp, C.c_int]
     L.snac_boundp.argtypes = [vp, C.c_int]
@@ -95,6 +106,7 @@ class Context:
         self.bforce = dpar[4:7].copy()
         self.vol_all = dpar[7]
         self.nstep, self.icheck = int(ipar[0]), int(ipar[1])
+        self.nscal = int(ipar[2])
 
     def close(self):
         if getattr(self, "h", None):
@@ -124,7 +136,7 @@ class Context:
 
     def shape(self, b, fid):
         n = self.block_info(b)["n"]
-        return tuple(int(x) + 2 for x in n) if FIELDS[fid] < 6 else tuple(int(x) for x in n)
+        return tuple(int(x) + 2 for x in n) if _ghosted(fid) else tuple(int(x) for x in n)
 
     # ---- fields (Fortran-ordered numpy arrays, ghosts included) ----
     def get(self, b, fid):
@@ -150,6 +162,14 @@ class Context:
 
     def inflow(self, is_correc):
         _check(lib.snac_inflow(self.h, int(bool(is_correc))), "inflow")
+
+    def rk_scal(self, irk, dt):
+        _check(lib.snac_rk_scal(self.h, irk, dt), "rk_scal")
+
+    def scalar_forcing(self):
+        f = np.zeros(max(self.nscal, 1))
+        _check(lib.snac_scalar_forcing(self.h, f), "scalar_forcing")
+        return f[:self.nscal]
 
     def boundp(self, which="p"):
         _check(lib.snac_boundp(self.h, FIELDS[which]), "boundp")

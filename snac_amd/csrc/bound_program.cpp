@@ -216,6 +216,28 @@ std::vector<Rec> pressure_ops(const Case &cs, const std::vector<BlockArrays> &bl
   return seq;
 }
 
+// boundp with the scalar's BCs (B.cbcsca / B.bcsca), same rules as pressure
+std::vector<Rec> scalar_ops(const Case &cs, const std::vector<BlockArrays> &bl, int par, int iscal) {
+  std::vector<Rec> seq;
+  std::vector<BlockArrays> sel = bl;
+  for (auto &B : sel) B.par_sel = par;
+  auto field = [](const BlockArrays &B, int is) -> double * { return B.sv[B.par_sel][is]; };
+  for (int d = 0; d < 3; d++) halo_ops(seq, sel, cs, d, field, iscal);
+  size_t first_bc = seq.size();
+  for (size_t b = 0; b < sel.size(); b++) {
+    const Block &B = cs.blocks[b];
+    for (int d = 0; d < 3; d++)
+      for (int side = 0; side < 2; side++) {
+        if (!B.is_bound[d][side]) continue;
+        int qm = side == 0 ? 0 : B.n[d];
+        const size_t k = (size_t)iscal * 6 + d * 2 + side;
+        set_bc_ops(seq, sel[b], field(sel[b], iscal), B.cbcsca[k], side, d, true, B.bcsca[k], B.dc[d][qm]);
+      }
+  }
+  tag_from(seq, first_bc, 3);
+  return seq;
+}
+
 // Fields do not interact, so the ops of several fills may be interleaved as
 // long as each array keeps its own order: a stable sort by stage does that.
 std::vector<std::vector<FaceOp>> merge_and_pack(std::vector<Rec> seq) {
@@ -231,6 +253,17 @@ std::vector<std::vector<FaceOp>> build_velocity_program(const Case &cs, const st
 
 std::vector<std::vector<FaceOp>> build_pressure_program(const Case &cs, const std::vector<BlockArrays> &bl, int which) {
   return merge_and_pack(pressure_ops(cs, bl, which));
+}
+
+std::vector<std::vector<FaceOp>> build_scalar_program(const Case &cs, const std::vector<BlockArrays> &bl, int par,
+                                                      int iscal) {
+  std::vector<Rec> seq;
+  for (int is = 0; is < cs.nscal; is++) {
+    if (iscal >= 0 && is != iscal) continue;
+    std::vector<Rec> one = scalar_ops(cs, bl, par, is);
+    seq.insert(seq.end(), one.begin(), one.end());
+  }
+  return merge_and_pack(seq);
 }
 
 std::vector<std::vector<FaceOp>> build_correc_fill_program(const Case &cs, const std::vector<BlockArrays> &bl,

@@ -4,6 +4,7 @@
 // init_inflow initflow.f90:369-429) and pressure BC rhs planes (init_bc_rhs
 // solver.f90:44-157).
 #include <cmath>
+#include <limits>
 #include <stdexcept>
 
 #include "case.hpp"
@@ -33,6 +34,12 @@ bool logical(const NmlGroup &g, const std::string &n, bool def, const std::vecto
   if (!v) return def;
   if (v->kind == NmlValue::Bool) return v->b;
   throw std::runtime_error("namelist: " + n + " is not a logical");
+}
+std::string str(const NmlGroup &g, const std::string &n, const std::string &def, const std::vector<int> &idx) {
+  auto *v = find(g, n, idx);
+  if (!v) return def;
+  if (v->kind == NmlValue::Str) return v->str;
+  throw std::runtime_error("namelist: " + n + " is not a string");
 }
 char chr(const NmlGroup &g, const std::string &n, char def, const std::vector<int> &idx) {
   auto *v = find(g, n, idx);
@@ -152,7 +159,22 @@ Case build_case(const std::string &dns_path, const std::string &blocks_path, int
     c.gacc[q] = num(d, "gacc", 0.0, {q + 1});
   }
   c.nscal = (int)num(d, "nscal", 0);
-  if (c.nscal > 0) throw std::runtime_error("scalar transport (nscal > 0) is not available in this build");
+  if (c.nscal < 0) c.nscal = 0;
+  if (c.nscal > 8) throw std::runtime_error("at most 8 scalars are supported");
+  if (c.nscal > 0) {
+    if (!dnsf.count("scalar")) throw std::runtime_error("Error reading scalar namelist");
+    const NmlGroup &sg = dnsf["scalar"];
+    for (int i = 0; i < c.nscal; i++) {
+      const double ai = num(sg, "alphai", std::numeric_limits<double>::max(), {i + 1});
+      c.alpha.push_back(1.0 / ai);
+      c.ssource.push_back(num(sg, "ssource", 0.0, {i + 1}));
+      c.is_sforced.push_back(logical(sg, "is_sforced", false, {i + 1}) ? 1 : 0);
+      c.scalf.push_back(num(sg, "scalf", 0.0, {i + 1}));
+      c.iniscal.push_back(str(sg, "iniscal", "zer", {i + 1}));
+    }
+    c.beta = num(sg, "beta", 0.0);
+    c.alpha_max = *std::max_element(c.alpha.begin(), c.alpha.end());
+  }
   if (dnsf.count("hypre")) {
     const NmlGroup &h = dnsf["hypre"];
     c.hypre_solver_i = (int)num(h, "hypre_solver_i", 2);
@@ -172,7 +194,13 @@ Case build_case(const std::string &dns_path, const std::string &blocks_path, int
       B.gr[q] = num(bl, "block_gr", 0.0, {q + 1, b + 1});
       if (B.n[q] < 1) throw std::runtime_error("invalid block_ng");
       if (B.gr[q] < 0) throw std::runtime_error("grid growth parameter must be positive.");
+      B.cbcsca.resize((size_t)c.nscal * 6, 'N');
+      B.bcsca.resize((size_t)c.nscal * 6, 0.0);
       for (int s = 0; s < 2; s++) {
+        for (int i = 0; i < c.nscal; i++) {
+          B.cbcsca[(size_t)i * 6 + q * 2 + s] = chr(bl, "block_cbcscal", 'N', {s, q + 1, i + 1, b + 1});
+          B.bcsca[(size_t)i * 6 + q * 2 + s] = num(bl, "block_bcscal", 0.0, {s, q + 1, i + 1, b + 1});
+        }
         B.cbcpre[q][s] = chr(bl, "block_cbcpre", 'N', {s, q + 1, b + 1});
         B.bcpre[q][s] = num(bl, "block_bcpre", 0.0, {s, q + 1, b + 1});
         B.inflow_type[q][s] = (int)num(bl, "block_inflow_type", 0, {s, q + 1, b + 1});

@@ -42,6 +42,9 @@ struct Block {
   std::vector<double> vin[3][3][2];
   // pressure BC rhs planes [dir][side] over the interior tangential plane
   std::vector<double> rhsp[3][2];
+  // scalar BCs, flat [iscal][dir][side] (block_cbcscal / block_bcscal)
+  std::vector<char> cbcsca;
+  std::vector<double> bcsca;
 };
 
 struct Case {
@@ -54,6 +57,11 @@ struct Case {
   // &hypre
   int hypre_solver_i = 2, hypre_maxiter = 50;
   double hypre_tol = 1e-4;
+  // &scalar (param.f90:96-100,148-163); alpha = 1/alphai
+  std::vector<double> alpha, ssource, scalf;
+  std::vector<int> is_sforced;
+  std::vector<std::string> iniscal;
+  double beta = 0.0, alpha_max = 0.0;
   // derived
   int fft_axis = 0;  // 0 none, 1..3
   std::vector<Block> blocks;

@@ -48,10 +48,20 @@ struct FaceOp {
 // ---- stencil kernels (kernels_stencil.hip) ----
 // rk_mom: reads u,v,w,p (ghosted) and dudtrko; writes the updated interior
 // velocities to un,vn,wn and dudtrk to dto.
+// s1 (scalar 1, ghosted) non-null adds the Boussinesq term with gb = gacc*beta
 void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v,
                    const double *w, const double *p, double *un, double *vn, double *wn, double *dtu,
                    double *dtv, double *dtw, double f1, double f2, double f12, double visc, double bfx,
-                   double bfy, double bfz);
+                   double bfy, double bfz, const double *s1, const double gb[3]);
+// rk_scal of one scalar: reads sc (ghosted), writes the interior of sn and dso
+void launch_rk_scal(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v, const double *w,
+                    const double *sc, double *sn, double *dso, double alpha, double f1, double f2, double f12,
+                    double src);
+// partial volume integrals of f (reduce_nparts(g) entries), and the bulk
+// forcing f += scalf - red[0]*ivol (fout, if given, receives that value)
+void launch_volsum_part(hipStream_t s, const Geo &g, const Metrics &m, const double *f, double *psum);
+void launch_bulk_force(hipStream_t s, const Geo &g, double *f, const double *red, double scalf, double ivol,
+                       double *fout);
 // fillps with updt_rhs fused; rhs planes may be null (ordered x0,x1,y0,y1,z0,z1)
 void launch_fillps(hipStream_t s, const Geo &g, const Metrics &m, int fft_axis, const double *u,
                    const double *v, const double *w, double *pp, double dtrk, const double *const rhsp[6]);

@@ -30,6 +30,8 @@ struct DevBlock {
   double *dto[3] = {};
   double *rhsp[6] = {};
   double *vin[3][3][2] = {};
+  std::vector<double *> sv[2];  // scalar ping-pong buffers [parity][iscal]
+  std::vector<double *> sdo;    // dsdtrko per scalar (interior)
   long ntot = 0, nin = 0;
   int part_off = 0, nparts = 0;
 };
@@ -65,8 +67,12 @@ struct snac_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::vector<snac::DevBlock> blk;
-  int cur = 0;  // parity of the current velocity buffers
+  int cur = 0;   // parity of the current velocity buffers
+  int scur = 0;  // parity of the current scalar buffers (flips once per substage)
   snac::Program prog_uvw[2][2], prog_bound_only[2][2], prog_inflow[2][2], prog_p, prog_pp, prog_end[2];
+  snac::Program prog_scal[2];                 // boundp of every scalar, per parity
+  std::vector<snac::Program> prog_scal_one[2];  // boundp of one scalar
+  double *sred = nullptr, *sforce = nullptr;  // scalar bulk mean / forcing
   snac::TensorPlan tp;
   snac::MGPlan mg;
   bool mg_planned = false;
@@ -82,11 +88,12 @@ struct snac_ctx {
   std::vector<hipEvent_t> pool;
   double tacc[SNAC_NPHASE] = {};
   int tcnt[SNAC_NPHASE] = {};
-  // one captured RK3 step, replayed by snac_step / snac_steps
+  // one captured RK3 step per starting scalar parity, replayed by
+  // snac_step / snac_steps
   bool use_graph = true;
-  hipGraphExec_t gexec = nullptr;
-  double graph_dt = 0.0;
-  int graph_cur = -1;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  double graph_dt[2] = {0.0, 0.0};
+  int graph_cur[2] = {-1, -1};
 };
 
 namespace snac {
@@ -128,6 +135,7 @@ std::vector<BlockArrays> block_arrays(snac_ctx *c) {
       for (int q = 0; q < 3; q++) A.vel[par][q] = D.vel[par][q];
     A.p = D.p;
     A.pp = D.pp;
+    for (int par = 0; par < 2; par++) A.sv[par] = D.sv[par];
     for (int d = 0; d < 3; d++)
       for (int q = 0; q < 3; q++)
         for (int s = 0; s < 2; s++) A.vin[d][q][s] = D.vin[d][q][s];
@@ -595,14 +603,42 @@ void autotune(snac_ctx *c) {
   SNAC_HIP(hipStreamSynchronize(c->stream));
 }
 
+// Boussinesq buoyancy of scalar 1 whenever scalars are transported (the
+// reference's _BOUSSINESQ_BUOYANCY build); skipped when gacc*beta vanishes,
+// where the term is exactly zero
 void rk_mom(snac_ctx *c, int irk, double dt) {
   const double f1 = RKC[irk - 1][0] * dt, f2 = RKC[irk - 1][1] * dt, f12 = f1 + f2;
+  const Case &cs = c->cs;
+  const double gb[3] = {cs.gacc[0] * cs.beta, cs.gacc[1] * cs.beta, cs.gacc[2] * cs.beta};
+  const bool buoy = cs.nscal > 0 && (gb[0] != 0.0 || gb[1] != 0.0 || gb[2] != 0.0);
   int nx = 1 - c->cur;
   for (auto &D : c->blk)
     launch_rk_mom(c->stream, D.g, D.m, D.vel[c->cur][0], D.vel[c->cur][1], D.vel[c->cur][2], D.p, D.vel[nx][0],
-                  D.vel[nx][1], D.vel[nx][2], D.dto[0], D.dto[1], D.dto[2], f1, f2, f12, c->cs.visc, c->cs.bforce[0],
-                  c->cs.bforce[1], c->cs.bforce[2]);
+                  D.vel[nx][1], D.vel[nx][2], D.dto[0], D.dto[1], D.dto[2], f1, f2, f12, cs.visc, cs.bforce[0],
+                  cs.bforce[1], cs.bforce[2], buoy ? D.sv[c->scur][0] : nullptr, gb);
   c->cur = nx;
+}
+
+// rk_scal + bulk_forcing_s of every scalar (main.f90:908-913): the update
+// goes to the other buffer of each scalar's pair; the forced ones then get
+// scalf minus their volume mean over all blocks, computed on the device
+void rk_scal(snac_ctx *c, int irk, double dt) {
+  const Case &cs = c->cs;
+  if (cs.nscal == 0) return;
+  const double f1 = RKC[irk - 1][0] * dt, f2 = RKC[irk - 1][1] * dt, f12 = f1 + f2;
+  const int nx = 1 - c->scur;
+  for (int is = 0; is < cs.nscal; is++) {
+    for (auto &D : c->blk)
+      launch_rk_scal(c->stream, D.g, D.m, D.vel[c->cur][0], D.vel[c->cur][1], D.vel[c->cur][2], D.sv[c->scur][is],
+                     D.sv[nx][is], D.sdo[is], cs.alpha[is], f1, f2, f12, cs.ssource[is]);
+    if (!cs.is_sforced[is]) continue;
+    for (auto &D : c->blk) launch_volsum_part(c->stream, D.g, D.m, D.sv[nx][is], c->part_sum + D.part_off);
+    launch_reduce_final(c->stream, c->part_sum, nullptr, c->nparts, c->sred);
+    for (size_t b = 0; b < c->blk.size(); b++)
+      launch_bulk_force(c->stream, c->blk[b].g, c->blk[b].sv[nx][is], c->sred, cs.scalf[is], 1.0 / cs.vol_all,
+                        b == 0 ? c->sforce + is : nullptr);
+  }
+  c->scur = nx;
 }
 
 void fillps(snac_ctx *c, double dtrk) {
@@ -621,6 +657,10 @@ void correc(snac_ctx *c, double dtrk, bool fuse_pressure) {
 
 void substage(snac_ctx *c, int irk, double dt) {
   double dtrk = (RKC[irk - 1][0] + RKC[irk - 1][1]) * dt;
+  if (c->cs.nscal > 0) {
+    { Phase t(c, SNAC_PH_RK_SCAL); rk_scal(c, irk, dt); }
+    { Phase t(c, SNAC_PH_BOUND); run_program(c, c->prog_scal[c->scur]); }  // boundp of the scalars
+  }
   { Phase t(c, SNAC_PH_RK_MOM); rk_mom(c, irk, dt); }
   { Phase t(c, SNAC_PH_BOUND); run_program(c, c->prog_uvw[c->cur][0]); }  // bounduvw + inflow
   { Phase t(c, SNAC_PH_FILLPS); fillps(c, dtrk); }
@@ -632,8 +672,10 @@ void substage(snac_ctx *c, int irk, double dt) {
 }
 
 void drop_graph(snac_ctx *c) {
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-  c->gexec = nullptr;
+  for (int q = 0; q < 2; q++) {
+    if (c->gexec[q]) (void)hipGraphExecDestroy(c->gexec[q]);
+    c->gexec[q] = nullptr;
+  }
 }
 
 // nsteps RK3 steps.  Eagerly when timing (events bracket every phase) or
@@ -648,26 +690,35 @@ void run_steps(snac_ctx *c, double dt, int nsteps) {
       for (int irk = 1; irk <= 3; irk++) substage(c, irk, dt);
     return;
   }
-  if (!c->gexec || c->graph_dt != dt || c->graph_cur != c->cur) {
-    drop_graph(c);
-    hipGraph_t g;
-    SNAC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int cur0 = c->cur;
-    try {
-      for (int irk = 1; irk <= 3; irk++) substage(c, irk, dt);
-    } catch (...) {
-      (void)hipStreamEndCapture(c->stream, &g);
-      c->cur = cur0;
-      throw;
+  // a step flips the velocity parity six times (back to the start) and the
+  // scalar parity three times: one graph per starting scalar parity
+  for (int s = 0; s < nsteps; s++) {
+    const int key = c->scur;
+    if (!c->gexec[key] || c->graph_dt[key] != dt || c->graph_cur[key] != c->cur) {
+      if (c->gexec[key]) (void)hipGraphExecDestroy(c->gexec[key]);
+      c->gexec[key] = nullptr;
+      hipGraph_t g;
+      SNAC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      const int cur0 = c->cur;
+      try {
+        for (int irk = 1; irk <= 3; irk++) substage(c, irk, dt);
+      } catch (...) {
+        (void)hipStreamEndCapture(c->stream, &g);
+        c->cur = cur0;
+        c->scur = key;
+        throw;
+      }
+      SNAC_HIP(hipStreamEndCapture(c->stream, &g));
+      if (c->cur != cur0) throw std::runtime_error("internal: a step must return to the same buffer parity");
+      c->scur = key;  // captured, not run yet
+      SNAC_HIP(hipGraphInstantiate(&c->gexec[key], g, nullptr, nullptr, 0));
+      SNAC_HIP(hipGraphDestroy(g));
+      c->graph_dt[key] = dt;
+      c->graph_cur[key] = c->cur;
     }
-    SNAC_HIP(hipStreamEndCapture(c->stream, &g));
-    if (c->cur != cur0) throw std::runtime_error("internal: a step must return to the same buffer parity");
-    SNAC_HIP(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
-    SNAC_HIP(hipGraphDestroy(g));
-    c->graph_dt = dt;
-    c->graph_cur = c->cur;
+    SNAC_HIP(hipGraphLaunch(c->gexec[key], c->stream));
+    if (c->cs.nscal > 0) c->scur = 1 - key;
   }
-  for (int s = 0; s < nsteps; s++) SNAC_HIP(hipGraphLaunch(c->gexec, c->stream));
 }
 
 double *field_ptr(snac_ctx *c, int b, int fid, long &count) {
@@ -684,8 +735,15 @@ double *field_ptr(snac_ctx *c, int b, int fid, long &count) {
   case SNAC_DUDTRKO: return D.dto[0];
   case SNAC_DVDTRKO: return D.dto[1];
   case SNAC_DWDTRKO: return D.dto[2];
-  default: throw std::runtime_error("invalid field id " + std::to_string(fid));
+  default: break;
   }
+  if (fid >= SNAC_SCAL0 && fid < SNAC_SCAL0 + 2 * c->cs.nscal) {
+    const int is = (fid - SNAC_SCAL0) / 2;
+    if ((fid - SNAC_SCAL0) % 2 == 0) { count = D.ntot; return D.sv[c->scur][is]; }
+    count = D.nin;
+    return D.sdo[is];
+  }
+  throw std::runtime_error("invalid field id " + std::to_string(fid));
 }
 
 void reduce(snac_ctx *c, bool div, double out[2]) {
@@ -735,6 +793,10 @@ void create(snac_ctx *c) {
     D.pp = dalloc_field(c, D.ntot);
     D.po = dalloc_field(c, D.ntot);
     for (int q = 0; q < 3; q++) D.dto[q] = dalloc<double>(c, D.nin);
+    for (int is = 0; is < cs.nscal; is++) {
+      for (int par = 0; par < 2; par++) D.sv[par].push_back(dalloc_field(c, D.ntot));
+      D.sdo.push_back(dalloc<double>(c, D.nin));
+    }
     for (int a = 0; a < 3; a++)
       for (int s = 0; s < 2; s++) {
         D.rhsp[2 * a + s] = B.rhsp[a][s].empty() ? nullptr : dupload(c, B.rhsp[a][s]);
@@ -757,6 +819,15 @@ void create(snac_ctx *c) {
   for (int par = 0; par < 2; par++) c->prog_end[par] = upload_program(c, build_correc_fill_program(cs, arrs, par));
   c->prog_p = upload_program(c, build_pressure_program(cs, arrs, 0));
   c->prog_pp = upload_program(c, build_pressure_program(cs, arrs, 1));
+  if (cs.nscal > 0) {
+    for (int par = 0; par < 2; par++) {
+      c->prog_scal[par] = upload_program(c, build_scalar_program(cs, arrs, par, -1));
+      for (int is = 0; is < cs.nscal; is++)
+        c->prog_scal_one[par].push_back(upload_program(c, build_scalar_program(cs, arrs, par, is)));
+    }
+    c->sred = dalloc<double>(c, 2);
+    c->sforce = dalloc<double>(c, cs.nscal);
+  }
   plan_tensor(c);
   autotune(c);
   if (!c->tp.ok) plan_mg(c);
@@ -908,7 +979,8 @@ void copy_field(snac_ctx *c, int b, int fid, double *dst_host, const double *src
   long n;
   double *d = field_ptr(c, b, fid, n);
   DevBlock &D = c->blk.at(b);
-  if (fid >= 0 && fid < 6) {
+  const bool ghosted = (fid >= 0 && fid < 6) || (fid >= SNAC_SCAL0 && (fid - SNAC_SCAL0) % 2 == 0);
+  if (ghosted) {
     size_t w = (size_t)(D.g.nx + 2) * sizeof(double), pitch = (size_t)D.g.sx * sizeof(double);
     size_t rows = (size_t)(D.g.ny + 2) * (D.g.nz + 2);
     if (src_host) SNAC_HIP(hipMemcpy2DAsync(d, pitch, src_host, w, w, rows, hipMemcpyHostToDevice, c->stream));
@@ -956,7 +1028,9 @@ int snac_boundp(snac_ctx *c, int fid) {
     on_device(c);
     if (fid == SNAC_P) run_program(c, c->prog_p);
     else if (fid == SNAC_PP) run_program(c, c->prog_pp);
-    else throw std::runtime_error("boundp: field must be SNAC_P or SNAC_PP");
+    else if (fid >= SNAC_SCAL0 && fid < SNAC_SCAL0 + 2 * c->cs.nscal && (fid - SNAC_SCAL0) % 2 == 0)
+      run_program(c, c->prog_scal_one[c->scur][(fid - SNAC_SCAL0) / 2]);
+    else throw std::runtime_error("boundp: field must be SNAC_P, SNAC_PP or a scalar");
   });
 }
 
@@ -987,7 +1061,9 @@ int snac_chkdt(snac_ctx *c, double *dtmax) {
     reduce(on_device(c), false, r);
     double dti = r[1] == 0.0 ? 1.0 : r[1];
     double v = std::sqrt(3.0) / dti;
-    v = std::min(v, 1.65 / 12.0 / c->cs.visc * c->dlmin * c->dlmin);
+    // main.f90:660 passes max(visc, alpha_max) as the diffusivity
+    const double ve = c->cs.nscal > 0 ? std::max(c->cs.visc, c->cs.alpha_max) : c->cs.visc;
+    v = std::min(v, 1.65 / 12.0 / ve * c->dlmin * c->dlmin);
     if (dtmax) *dtmax = v;
   });
 }
@@ -996,7 +1072,25 @@ int snac_init_bounds(snac_ctx *c) {
   return guard([&] {
     run_program(on_device(c), c->prog_bound_only[c->cur][0]);
     run_program(c, c->prog_p);
+    if (c->cs.nscal > 0) run_program(c, c->prog_scal[c->scur]);
     run_program(c, c->prog_inflow[c->cur][0]);
+  });
+}
+
+int snac_rk_scal(snac_ctx *c, int irk, double dt) {
+  return guard([&] {
+    on_device(c);
+    if (irk < 1 || irk > 3) throw std::runtime_error("irk must be 1..3");
+    rk_scal(c, irk, dt);
+  });
+}
+
+int snac_scalar_forcing(snac_ctx *c, double *f) {
+  return guard([&] {
+    on_device(c);
+    if (c->cs.nscal == 0) return;
+    SNAC_HIP(hipMemcpyAsync(f, c->sforce, c->cs.nscal * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SNAC_HIP(hipStreamSynchronize(c->stream));
   });
 }
 

@@ -28,13 +28,18 @@ __device__ __forceinline__ void tile_of(int gx, int gy, int &bx, int &by, int &b
   bz = t / gy;
 }
 
+// BUOY: Boussinesq buoyancy of scalar 1, -f12*gacc*beta*(face average of s)
+// (rk.f90:104-108), gb = gacc*beta
+template <bool BUOY>
 __global__ __launch_bounds__(TX *TY) void k_rk_mom(Geo g, Metrics m, const double *__restrict__ u,
                                                     const double *__restrict__ v, const double *__restrict__ w,
                                                     const double *__restrict__ p, double *__restrict__ un,
                                                     double *__restrict__ vn, double *__restrict__ wn,
                                                     double *__restrict__ dtu, double *__restrict__ dtv,
                                                     double *__restrict__ dtw, double f1, double f2, double f12,
-                                                    double visc, double bfx, double bfy, double bfz) {
+                                                    double visc, double bfx, double bfy, double bfz,
+                                                    const double *__restrict__ s1, double gbx, double gby,
+                                                    double gbz) {
   int bx, by, bz;
   tile_of((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, bx, by, bz);
   const int i = bx * TX + threadIdx.x + 1;
@@ -102,9 +107,46 @@ __global__ __launch_bounds__(TX *TY) void k_rk_mom(Geo g, Metrics m, const doubl
   un_ = un_ + f12 * (-(p[xp] - pc) * ixci + bfx);
   vn_ = vn_ + f12 * (-(p[yp] - pc) * iycj + bfy);
   wn_ = wn_ + f12 * (-(p[zp] - pc) * izck + bfz);
+  if (BUOY) {
+    const double sc = s1[c];
+    un_ = un_ - f12 * gbx * 0.5 * (s1[xp] + sc);
+    vn_ = vn_ - f12 * gby * 0.5 * (s1[yp] + sc);
+    wn_ = wn_ - f12 * gbz * 0.5 * (s1[zp] + sc);
+  }
   un[c] = un_;
   vn[c] = vn_;
   wn[c] = wn_;
+}
+
+// rk_scal (rk.f90:130-204) with scal_d / scal_a (scal.f90:89-155) fused:
+// reads s (ghosted) and u, v, w; writes the updated interior into sn (the
+// other buffer of the pair) and dsdtrk into dso.
+__global__ __launch_bounds__(TX *TY) void k_rk_scal(Geo g, Metrics m, const double *__restrict__ u,
+                                                     const double *__restrict__ v, const double *__restrict__ w,
+                                                     const double *__restrict__ s, double *__restrict__ sn,
+                                                     double *__restrict__ dso, double alpha, double f1, double f2,
+                                                     double f12, double src) {
+  int bx, by, bz;
+  tile_of((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, bx, by, bz);
+  const int i = bx * TX + threadIdx.x + 1;
+  const int j = by * TY + threadIdx.y + 1;
+  const int k = bz + 1;
+  if (i > g.nx || j > g.ny) return;
+  const long sx = g.sx, sy = g.sy;
+  const long c = gidx(i, j, k, sx, sy);
+  const long xp = c + 1, xm = c - 1, yp = c + sx, ym = c - sx, zp = c + sx * sy, zm = c - sx * sy;
+  const double sc = s[c], sxp = s[xp], sxm = s[xm], syp = s[yp], sym = s[ym], szp = s[zp], szm = s[zm];
+  const double ixfi = m.idxf[i], iyfj = m.idyf[j], izfk = m.idzf[k];
+  double d = alpha * ((sxp - sc) * m.idxc[i] - (sc - sxm) * m.idxc[i - 1]) * ixfi +
+             alpha * ((syp - sc) * m.idyc[j] - (sc - sym) * m.idyc[j - 1]) * iyfj +
+             alpha * ((szp - sc) * m.idzc[k] - (sc - szm) * m.idzc[k - 1]) * izfk;
+  const double usim = 0.5 * (sxm + sc) * u[xm], usip = 0.5 * (sxp + sc) * u[c];
+  const double vsjm = 0.5 * (sym + sc) * v[ym], vsjp = 0.5 * (syp + sc) * v[c];
+  const double wskm = 0.5 * (szm + sc) * w[zm], wskp = 0.5 * (szp + sc) * w[c];
+  d = d + (-usip + usim) * ixfi + (-vsjp + vsjm) * iyfj + (-wskp + wskm) * izfk;
+  const long q = (long)(i - 1) + (long)g.nx * ((long)(j - 1) + (long)g.ny * (long)(k - 1));
+  sn[c] = sc + f1 * d + f2 * dso[q] + f12 * src;
+  dso[q] = d;
 }
 
 __global__ __launch_bounds__(TX *TY) void k_fillps(Geo g, Metrics m, int fa, const double *__restrict__ u,
@@ -251,6 +293,37 @@ __global__ __launch_bounds__(TX *TY) void k_chkdt(Geo g, Metrics m, const double
   }
 }
 
+// partial sums of s*dxf*dyf*dzf (bulk_mean_s scal.f90:157-180)
+__global__ __launch_bounds__(TX *TY) void k_volsum(Geo g, Metrics m, const double *__restrict__ s, double *psum) {
+  __shared__ double ss[TX * TY / 64];
+  const int i = blockIdx.x * TX + threadIdx.x + 1;
+  const int j = blockIdx.y * TY + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  double a = 0.0;
+  if (i <= g.nx && j <= g.ny) a = s[gidx(i, j, k, g.sx, g.sy)] * m.dxf[i] * m.dyf[j] * m.dzf[k];
+  a = wave_sum(a);
+  const int t = threadIdx.x + TX * threadIdx.y;
+  if ((t & 63) == 0) ss[t / 64] = a;
+  __syncthreads();
+  if (t == 0) {
+    double b = 0.0;
+    for (int q = 0; q < TX * TY / 64; q++) b += ss[q];
+    psum[blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z)] = b;
+  }
+}
+
+// bulk_forcing_s scal.f90:182-199 with f = scalf - (volume sum)/vol_all read
+// from the device; the first thread also records f
+__global__ __launch_bounds__(TX *TY) void k_bulk_force(Geo g, double *__restrict__ s, const double *__restrict__ red,
+                                                        double scalf, double ivol, double *fout) {
+  const int i = blockIdx.x * TX + threadIdx.x + 1;
+  const int j = blockIdx.y * TY + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  const double f = scalf - red[0] * ivol;
+  if (fout && i == 1 && j == 1 && k == 1) fout[0] = f;
+  if (i <= g.nx && j <= g.ny) s[gidx(i, j, k, g.sx, g.sy)] += f;
+}
+
 __global__ __launch_bounds__(256) void k_reduce_final(const double *psum, const double *pmax, int n, double *out) {
   __shared__ double ss[4], sm[4];
   double s = 0.0, mx = 0.0;
@@ -277,9 +350,33 @@ dim3 ghosted_tiles(const Geo &g) { return dim3(((g.nx + 2 + TX - 1) / TX) * ((g.
 
 void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v, const double *w,
                    const double *p, double *un, double *vn, double *wn, double *dtu, double *dtv, double *dtw,
-                   double f1, double f2, double f12, double visc, double bfx, double bfy, double bfz) {
-  hipLaunchKernelGGL(k_rk_mom, interior_tiles(g), dim3(TX, TY), 0, s, g, m, u, v, w, p, un, vn, wn, dtu, dtv, dtw,
-                     f1, f2, f12, visc, bfx, bfy, bfz);
+                   double f1, double f2, double f12, double visc, double bfx, double bfy, double bfz,
+                   const double *s1, const double gb[3]) {
+  if (s1)
+    hipLaunchKernelGGL(k_rk_mom<true>, interior_tiles(g), dim3(TX, TY), 0, s, g, m, u, v, w, p, un, vn, wn, dtu, dtv,
+                       dtw, f1, f2, f12, visc, bfx, bfy, bfz, s1, gb[0], gb[1], gb[2]);
+  else
+    hipLaunchKernelGGL(k_rk_mom<false>, interior_tiles(g), dim3(TX, TY), 0, s, g, m, u, v, w, p, un, vn, wn, dtu, dtv,
+                       dtw, f1, f2, f12, visc, bfx, bfy, bfz, nullptr, 0.0, 0.0, 0.0);
+  SNAC_HIP(hipGetLastError());
+}
+
+void launch_rk_scal(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v, const double *w,
+                    const double *sc, double *sn, double *dso, double alpha, double f1, double f2, double f12,
+                    double src) {
+  hipLaunchKernelGGL(k_rk_scal, interior_tiles(g), dim3(TX, TY), 0, s, g, m, u, v, w, sc, sn, dso, alpha, f1, f2, f12,
+                     src);
+  SNAC_HIP(hipGetLastError());
+}
+
+void launch_volsum_part(hipStream_t s, const Geo &g, const Metrics &m, const double *f, double *psum) {
+  hipLaunchKernelGGL(k_volsum, interior_grid(g), dim3(TX, TY), 0, s, g, m, f, psum);
+  SNAC_HIP(hipGetLastError());
+}
+
+void launch_bulk_force(hipStream_t s, const Geo &g, double *f, const double *red, double scalf, double ivol,
+                       double *fout) {
+  hipLaunchKernelGGL(k_bulk_force, interior_grid(g), dim3(TX, TY), 0, s, g, f, red, scalf, ivol, fout);
   SNAC_HIP(hipGetLastError());
 }
 

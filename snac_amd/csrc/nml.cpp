@@ -3,6 +3,7 @@
 // (filled with the first index fastest, as Fortran does), quoted strings,
 // logicals (T/F/.true./.false.), numbers with e/d exponents, `n*value`
 // repeats and `!` comments.
+#include <algorithm>
 #include <cctype>
 #include <fstream>
 #include <sstream>
@@ -77,12 +78,16 @@ NmlValue to_value(const Tok &tk) {
   return v;
 }
 
-std::vector<std::vector<int>> expand_index(const std::vector<std::string> &parts) {
+// an open bound (`alphai(:)`) starts at 1 and extends over the nvals values
+std::vector<std::vector<int>> expand_index(const std::vector<std::string> &parts, int nvals) {
   std::vector<std::pair<int, int>> rng;
   for (const auto &p : parts) {
     size_t c = p.find(':');
-    if (c == std::string::npos) { int v = std::stoi(p); rng.push_back({v, v}); }
-    else rng.push_back({std::stoi(p.substr(0, c)), std::stoi(p.substr(c + 1))});
+    if (c == std::string::npos) { int v = std::stoi(p); rng.push_back({v, v}); continue; }
+    const std::string a = p.substr(0, c), b = p.substr(c + 1);
+    const int lo = a.empty() ? 1 : std::stoi(a);
+    const int hi = b.empty() ? lo + std::max(nvals, 1) - 1 : std::stoi(b);
+    rng.push_back({lo, hi});
   }
   std::vector<std::vector<int>> out;
   std::vector<int> cur(rng.size());
@@ -158,7 +163,7 @@ Nml read_namelist_file(const std::string &path) {
       if (vals.size() == 1) entry[{}] = vals[0];
       else for (size_t q = 0; q < vals.size(); q++) entry[{(int)q + 1}] = vals[q];
     } else {
-      auto combos = expand_index(idx);
+      auto combos = expand_index(idx, (int)vals.size());
       for (size_t q = 0; q < vals.size() && q < combos.size(); q++) entry[combos[q]] = vals[q];
     }
   }

@@ -21,7 +21,19 @@ def _fmt(v):
     return repr(float(v))
 
 
-def write_dns(path, visci, nstep=10, dt_f=-1.0, cfl=0.5, dtmax=1e5, icheck=10, bforce=(0, 0, 0)):
+def write_dns(path, visci, nstep=10, dt_f=-1.0, cfl=0.5, dtmax=1e5, icheck=10, bforce=(0, 0, 0), nscal=0,
+              gacc=(0, 0, 0), scalar=None):
+    """scalar: the &scalar group as {name: value or list} (alphai, beta,
+    iniscal, ssource, is_sforced, scalf), written when nscal > 0."""
+    sc = ""
+    if nscal > 0:
+        sc = "&scalar\n"
+        for k, v in (scalar or {}).items():
+            if isinstance(v, (list, tuple)):
+                sc += f"{k}(1:{len(v)}) = {', '.join(_fmt(x) for x in v)}\n"
+            else:
+                sc += f"{k} = {_fmt(v)}\n"
+        sc += "/\n"
     txt = ("&dns\n"
            f"cfl = {_fmt(cfl)}, dtmax = {_fmt(dtmax)}, dt_f = {_fmt(dt_f)}\n"
            f"visci = {_fmt(visci)}\n"
@@ -30,7 +42,9 @@ def write_dns(path, visci, nstep=10, dt_f=-1.0, cfl=0.5, dtmax=1e5, icheck=10, b
            "restart = F, is_overwrite_save = T, nsaves_max = 0\n"
            f"icheck = {icheck}, iout0d = 0, iout1d = 0, iout2d = 0, iout3d = 0, isave = 0\n"
            f"bforce(1:3) = {', '.join(_fmt(b) for b in bforce)}\n"
-           "/\n&hypre\nhypre_solver_i = 2, hypre_tol = 1.e-13, hypre_maxiter = 1000\n/\n")
+           f"gacc(1:3) = {', '.join(_fmt(g) for g in gacc)}\n"
+           f"nscal = {nscal}\n"
+           "/\n&hypre\nhypre_solver_i = 2, hypre_tol = 1.e-13, hypre_maxiter = 1000\n/\n" + sc)
     open(path, "w").write(txt)
 
 
@@ -51,6 +65,11 @@ def write_blocks(path, blocks):
                    ", ".join(_fmt(B["cbcpre"][d][s]) for d in range(3) for s in range(2)) + "\n")
         out.append(f"block_bcpre(0:1,1:3,{b}) = " +
                    ", ".join(_fmt(B["bcpre"][d][s]) for d in range(3) for s in range(2)) + "\n")
+        for i, (cbs, bs) in enumerate(zip(B.get("cbcscal", []), B.get("bcscal", [])), start=1):
+            out.append(f"block_cbcscal(0:1,1:3,{i},{b}) = " +
+                       ", ".join(_fmt(cbs[d][s]) for d in range(3) for s in range(2)) + "\n")
+            out.append(f"block_bcscal(0:1,1:3,{i},{b}) = " +
+                       ", ".join(_fmt(bs[d][s]) for d in range(3) for s in range(2)) + "\n")
         it = B.get("inflow_type", [[0, 0]] * 3)
         out.append(f"block_inflow_type(0:1,1:3,{b}) = " +
                    ", ".join(_fmt(it[d][s]) for d in range(3) for s in range(2)) + "\n")

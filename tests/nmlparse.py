@@ -56,14 +56,17 @@ def _value(tok):
         return s
 
 
-def _expand_index(spec):
-    """'0:1,1:3,2,1' -> list of index tuples in Fortran order."""
+def _expand_index(spec, nvals=1):
+    """'0:1,1:3,2,1' -> list of index tuples in Fortran order; an open bound
+    (':' as in alphai(:)) starts at 1 and extends over the values given."""
     ranges = []
     for part in spec.split(","):
         part = part.strip()
         if ":" in part:
             a, b = part.split(":")
-            ranges.append(range(int(a), int(b) + 1))
+            a = int(a) if a.strip() else 1
+            b = int(b) if b.strip() else a + nvals - 1
+            ranges.append(range(a, b + 1))
         else:
             ranges.append(range(int(part), int(part) + 1))
     combos = []  # first index fastest
@@ -134,8 +137,8 @@ def parse(path):
                 for q, v in enumerate(vals):
                     entry[(q + 1,)] = v
         else:
-            combos = _expand_index(index)
-            for q, v in enumerate(vals):
+            combos = _expand_index(index, len(vals))
+            for q, v in enumerate(vals[:len(combos)]):
                 entry[combos[q]] = v
     return groups
 
@@ -183,7 +186,25 @@ def load_case(dns_path, blocks_path, fft_axis):
                     cbcvel[b, c, q, s] = get(blk, "block_cbcvel", "D", (s, q + 1, c + 1, b + 1))
                     bcvel[b, c, q, s] = get(blk, "block_bcvel", 0.0, (s, q + 1, c + 1, b + 1))
     bforce = np.array([get(d, "bforce", 0.0, (q + 1,)) for q in range(3)], dtype=float)
+    gacc = np.array([get(d, "gacc", 0.0, (q + 1,)) for q in range(3)], dtype=float)
     hyp = dns.get("hypre", {})
+    # scalars: &scalar (param.f90:96-100,148-163), block_cbcscal/bcscal
+    nscal = int(get(d, "nscal", 0))
+    sc = dns.get("scalar", {})
+    if nscal > 0 and "scalar" not in dns:
+        raise ValueError("Error reading scalar namelist")
+    alphai = np.array([float(get(sc, "alphai", 1.7976931348623157e308, (i + 1,))) for i in range(nscal)])
+    ssource = np.array([float(get(sc, "ssource", 0.0, (i + 1,))) for i in range(nscal)])
+    is_sforced = np.array([int(bool(get(sc, "is_sforced", False, (i + 1,)))) for i in range(nscal)], np.int32)
+    scalf = np.array([float(get(sc, "scalf", 0.0, (i + 1,))) for i in range(nscal)])
+    cbcscal = np.full((nb, max(nscal, 1), 3, 2), "N")
+    bcscal = np.zeros((nb, max(nscal, 1), 3, 2))
+    for b in range(nb):
+        for i in range(nscal):
+            for q in range(3):
+                for s in range(2):
+                    cbcscal[b, i, q, s] = get(blk, "block_cbcscal", "N", (s, q + 1, i + 1, b + 1))
+                    bcscal[b, i, q, s] = get(blk, "block_bcscal", 0.0, (s, q + 1, i + 1, b + 1))
     return dict(
         nblocks=nb, ng=ng, lmin=lmin, lmax=lmax, gt=gt, gr=gr, cbcvel=cbcvel, bcvel=bcvel,
         cbcpre=cbcpre, bcpre=bcpre, inflow_type=inflow, fft_axis=fft_axis,
@@ -191,4 +212,6 @@ def load_case(dns_path, blocks_path, fft_axis):
         cfl=float(get(d, "cfl")), dtmax=float(get(d, "dtmax")), dt_f=float(get(d, "dt_f", -1.0)),
         nstep=int(get(d, "nstep")), icheck=int(get(d, "icheck")),
         hypre_tol=float(get(hyp, "hypre_tol", 1e-4)), hypre_maxiter=int(get(hyp, "hypre_maxiter", 50)),
+        nscal=nscal, gacc=gacc, beta=float(get(sc, "beta", 0.0)), alphai=alphai, ssource=ssource,
+        is_sforced=is_sforced, scalf=scalf, cbcscal=cbcscal[:, :nscal], bcscal=bcscal[:, :nscal],
     )

@@ -12,6 +12,14 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "oracle", "_build", "libsnac_oracle.so")
 
 FIELDS = {"u": 0, "v": 1, "w": 2, "p": 3, "pp": 4, "po": 5, "dudtrko": 6, "dvdtrko": 7, "dwdtrko": 8}
+for _i in range(1, 9):  # scalars s1.., their dsdtrko1..
+    FIELDS[f"s{_i}"] = 9 + 2 * (_i - 1)
+    FIELDS[f"dsdtrko{_i}"] = 10 + 2 * (_i - 1)
+
+
+def ghosted(fid):
+    f = FIELDS[fid]
+    return f < 6 or (f >= 9 and (f - 9) % 2 == 0)
 SOLVERS = {"band": 0, "cg": 1, "tensor": 2, "auto": 3}
 R2R = {"R2HC": 0, "HC2R": 1, "REDFT00": 3, "REDFT01": 4, "REDFT10": 5, "REDFT11": 6,
        "RODFT00": 7, "RODFT01": 8, "RODFT10": 9, "RODFT11": 10}
@@ -54,6 +62,8 @@ def lib():
         L.oc_chkdt.restype = C.c_double
         L.oc_chkdt.argtypes = [C.c_void_p]
         L.oc_step.argtypes = [C.c_void_p, C.c_double]
+        L.oc_set_scalars.argtypes = [C.c_void_p, C.c_int, dp, C.c_double, dp, dp, ip, dp, C.c_char_p, dp]
+        L.oc_rk_scal.argtypes = [C.c_void_p, C.c_int, C.c_double, dp]
         L.oc_init_bounds.argtypes = [C.c_void_p]
         L.oc_r2r_direct.argtypes = [C.c_int, C.c_int, dp, dp]
         L.oc_r2r.argtypes = [C.c_int, C.c_int, dp, dp]
@@ -94,6 +104,14 @@ class OracleCase:
         if not self.h:
             raise ValueError("oracle rejected the case: " + err.value.decode())
         self.nblocks = nb
+        self.nscal = int(case.get("nscal", 0))
+        if self.nscal > 0:
+            L.oc_set_scalars(self.h, self.nscal, np.ascontiguousarray(case["alphai"], dtype=float),
+                             float(case["beta"]), np.ascontiguousarray(case["gacc"], dtype=float),
+                             np.ascontiguousarray(case["ssource"], dtype=float),
+                             np.ascontiguousarray(case["is_sforced"], dtype=np.int32),
+                             np.ascontiguousarray(case["scalf"], dtype=float), _chars(case["cbcscal"]),
+                             np.ascontiguousarray(case["bcscal"], dtype=float).reshape(-1))
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -116,7 +134,7 @@ class OracleCase:
 
     def shape(self, b, fid):
         n = self.block_info(b)["n"]
-        if FIELDS[fid] < 6:
+        if ghosted(fid):
             return (n[0] + 2, n[1] + 2, n[2] + 2)
         return tuple(n)
 
@@ -134,6 +152,12 @@ class OracleCase:
 
     def rk_mom(self, irk, dt):
         lib().oc_rk_mom(self.h, irk, dt)
+
+    def rk_scal(self, irk, dt):
+        """rk_scal + bulk forcing of every scalar; returns the forcings f."""
+        f = np.zeros(max(self.nscal, 1))
+        lib().oc_rk_scal(self.h, irk, dt, f)
+        return f[:self.nscal]
 
     def bounduvw(self, is_correc):
         lib().oc_bounduvw(self.h, int(is_correc))
