@@ -156,9 +156,14 @@ int snac_get_timing(snac_ctx *ctx, double *ms, int *count);
 
 /* Choices of the autotuner that runs at context creation (tile shape of each
  * transform call site: forward ones, then inverse ones, then the
- * tridiagonal variant; -1 = dense path) and the best times in ms.  No
+ * tridiagonal variant -- these only with the tensor-product solver; -1 =
+ * dense path -- and last the rk_mom kernel variant: 0 one cell per thread,
+ * 1/2/3 k-march over 8/16/32 planes) and the best times in ms.  No
  * counterpart in the reference (its FFTW plans use FFTW_ESTIMATE).          */
 int snac_get_autotune(snac_ctx *ctx, int *choices, double *ms, int n);
+/* overrides the autotuned rk_mom variant (0..3, as above); every variant
+ * computes the same result.  No counterpart in the reference.              */
+int snac_set_rk_variant(snac_ctx *ctx, int variant);
 
 /* ---- building blocks, exposed for their own parity tests ----
  * Batched r2r transform of howmany contiguous lines of length n, kinds

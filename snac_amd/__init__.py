@@ -76,6 +76,7 @@ def _load():
     L.snac_r2r_lines.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
     L.snac_stream_copy.argtypes = [vp, C.c_long, C.c_int, C.POINTER(C.c_double)]
     L.snac_get_autotune.argtypes = [vp, ip, dp, C.c_int]
+    L.snac_set_rk_variant.argtypes = [vp, C.c_int]
     L.snac_set_timing.argtypes = [vp, C.c_int]
     L.snac_get_timing.argtypes = [vp, dp, ip]
     return L
@@ -257,6 +258,10 @@ class Context:
         _check(lib.snac_get_autotune(self.h, ch, ms, 16), "get_autotune")
         k = int(np.argmax(ch == -2)) if (ch == -2).any() else 16
         return ch[:k].tolist(), [round(float(x), 4) for x in ms[:k] if x > 0]
+
+    def set_rk_variant(self, v):
+        """Overrides the autotuned rk_mom kernel variant (0..3)."""
+        _check(lib.snac_set_rk_variant(self.h, int(v)), "set_rk_variant")
 
     def new_dt(self):
         """main.f90:660-661: dt = dt_f if > 0 else min(cfl*dt_cfl, dtmax)."""

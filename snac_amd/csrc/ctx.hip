@@ -67,6 +67,8 @@ struct snac_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::vector<snac::DevBlock> blk;
+  int rk_variant = 0;     // rk_mom kernel variant (autotuned)
+  double rk_tune_ms = 0.0;
   int cur = 0;   // parity of the current velocity buffers
   int scur = 0;  // parity of the current scalar buffers (flips once per substage)
   snac::Program prog_uvw[2][2], prog_bound_only[2][2], prog_inflow[2][2], prog_p, prog_pp, prog_end[2];
@@ -615,8 +617,34 @@ void rk_mom(snac_ctx *c, int irk, double dt) {
   for (auto &D : c->blk)
     launch_rk_mom(c->stream, D.g, D.m, D.vel[c->cur][0], D.vel[c->cur][1], D.vel[c->cur][2], D.p, D.vel[nx][0],
                   D.vel[nx][1], D.vel[nx][2], D.dto[0], D.dto[1], D.dto[2], f1, f2, f12, cs.visc, cs.bforce[0],
-                  cs.bforce[1], cs.bforce[2], buoy ? D.sv[c->scur][0] : nullptr, gb);
+                  cs.bforce[1], cs.bforce[2], buoy ? D.sv[c->scur][0] : nullptr, gb, c->rk_variant);
   c->cur = nx;
+}
+
+// rk_mom kernel variant: one cell per thread or k-march over 8/16/32-plane
+// chunks, timed on this case's own arrays at context creation (the arrays
+// are still zero then; the buffers it writes are cleared afterwards)
+void autotune_rk(snac_ctx *c) {
+  if (c->device < 0) return;
+  double best = 1e300;
+  int bv = 0;
+  for (int v = 0; v <= 3; v++) {
+    c->rk_variant = v;
+    const int cur0 = c->cur;
+    double ms = time_stage(c, [&] {
+      rk_mom(c, 1, 1e-3);
+      c->cur = cur0;
+    });
+    if (ms < best) { best = ms; bv = v; }
+  }
+  c->rk_variant = bv;
+  c->rk_tune_ms = best;
+  for (auto &D : c->blk) {
+    for (int par = 0; par < 2; par++)
+      for (int q = 0; q < 3; q++) SNAC_HIP(hipMemsetAsync(D.vel[par][q] - 15, 0, (D.ntot + 16) * sizeof(double), c->stream));
+    for (int q = 0; q < 3; q++) SNAC_HIP(hipMemsetAsync(D.dto[q], 0, D.nin * sizeof(double), c->stream));
+  }
+  SNAC_HIP(hipStreamSynchronize(c->stream));
 }
 
 // rk_scal + bulk_forcing_s of every scalar (main.f90:908-913): the update
@@ -830,6 +858,7 @@ void create(snac_ctx *c) {
   }
   plan_tensor(c);
   autotune(c);
+  autotune_rk(c);
   if (!c->tp.ok) plan_mg(c);
   if (c->device >= 0) SNAC_HIP(hipDeviceSynchronize());
 }
@@ -1137,11 +1166,23 @@ int snac_get_autotune(snac_ctx *c, int *choices, double *ms, int n) {
     std::vector<int> ch;
     for (auto &p : t.fwd) ch.push_back(p.fft ? p.lshift : -1);
     for (auto &p : t.bwd) ch.push_back(p.fft ? p.lshift : -1);
-    ch.push_back(t.tvariant);
+    std::vector<double> tm = t.tune_ms;
+    if (t.ok) ch.push_back(t.tvariant);
+    ch.push_back(c->rk_variant);
+    tm.push_back(c->rk_tune_ms);
     for (int q = 0; q < n; q++) {
       if (choices) choices[q] = q < (int)ch.size() ? ch[q] : -2;
-      if (ms) ms[q] = q < (int)t.tune_ms.size() ? t.tune_ms[q] : 0.0;
+      if (ms) ms[q] = q < (int)tm.size() ? tm[q] : 0.0;
     }
+  });
+}
+
+int snac_set_rk_variant(snac_ctx *c, int variant) {
+  return guard([&] {
+    on_device(c);
+    if (variant < 0 || variant > 3) throw std::runtime_error("rk_mom variant must be 0..3");
+    c->rk_variant = variant;
+    drop_graph(c);
   });
 }
 

@@ -118,6 +118,111 @@ __global__ __launch_bounds__(TX *TY) void k_rk_mom(Geo g, Metrics m, const doubl
   wn[c] = wn_;
 }
 
+// The same substage with each thread marching along k over a chunk of KB
+// planes: its own column's values at k-1, k, k+1 of u, v, w and at k, k+1
+// of p stay in registers from one plane to the next, so every plane of the
+// fields is fetched once per column; the in-plane neighbours and the few
+// mixed (i-1,k+1)/(j-1,k+1)/(i+1,k-1)/(j+1,k-1) terms come through L1/L2
+// from loads that neighbouring lanes and wavefronts issue for their own
+// columns.  Metrics along i, j are hoisted out of the march; those along k
+// are uniform per plane.
+template <bool BUOY, int KB>
+__global__ __launch_bounds__(TX *TY) void k_rk_mom_kz(Geo g, Metrics m, const double *__restrict__ u,
+                                                       const double *__restrict__ v, const double *__restrict__ w,
+                                                       const double *__restrict__ p, double *__restrict__ un,
+                                                       double *__restrict__ vn, double *__restrict__ wn,
+                                                       double *__restrict__ dtu, double *__restrict__ dtv,
+                                                       double *__restrict__ dtw, double f1, double f2, double f12,
+                                                       double visc, double bfx, double bfy, double bfz,
+                                                       const double *__restrict__ s1, double gbx, double gby,
+                                                       double gbz) {
+  int bx, by, bz;
+  tile_of((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, bx, by, bz);
+  const int i = bx * TX + threadIdx.x + 1;
+  const int j = by * TY + threadIdx.y + 1;
+  const int k0 = bz * KB + 1;
+  const int k1 = min(k0 + KB - 1, g.nz);
+  if (i > g.nx || j > g.ny) return;
+  const long sx = g.sx, sy = g.sy, sz = sx * sy;
+  const double dxfi = m.dxf[i], dxfip = m.dxf[i + 1], dyfj = m.dyf[j], dyfjp = m.dyf[j + 1];
+  const double ixci = m.idxc[i], ixcim = m.idxc[i - 1], ixfi = m.idxf[i], ixfip = m.idxf[i + 1];
+  const double iycj = m.idyc[j], iycjm = m.idyc[j - 1], iyfj = m.idyf[j], iyfjp = m.idyf[j + 1];
+  long c = gidx(i, j, k0, sx, sy);
+  double um = u[c - sz], vm = v[c - sz], wm = w[c - sz];
+  double uc = u[c], vc = v[c], wc = w[c], pc = p[c];
+  long q = (long)(i - 1) + (long)g.nx * ((long)(j - 1) + (long)g.ny * (long)(k0 - 1));
+  const long qs = (long)g.nx * g.ny;
+  for (int k = k0; k <= k1; k++, c += sz, q += qs) {
+    const long xp = c + 1, xm = c - 1, yp = c + sx, ym = c - sx, zp = c + sz, zm = c - sz;
+    const double up = u[zp], vp = v[zp], wp = w[zp], pz = p[zp];
+    const double dzfk = m.dzf[k], dzfkp = m.dzf[k + 1];
+    const double izck = m.idzc[k], izckm = m.idzc[k - 1], izfk = m.idzf[k], izfkp = m.idzf[k + 1];
+    const double uxp = u[xp], uxm = u[xm], uyp = u[yp], uym = u[ym];
+    const double vxp = v[xp], vxm = v[xm], vyp = v[yp], vym = v[ym];
+    const double wxp = w[xp], wxm = w[xm], wyp = w[yp], wym = w[ym];
+    double a, b, cc;
+    {
+      double dudxp = (uxp - uc) * ixfip, dudxm = (uc - uxm) * ixfi;
+      double dudyp = (uyp - uc) * iycj, dudym = (uc - uym) * iycjm;
+      double dudzp = (up - uc) * izck, dudzm = (uc - um) * izckm;
+      a = visc * (dudxp - dudxm) * ixci + visc * (dudyp - dudym) * iyfj + visc * (dudzp - dudzm) * izfk;
+      double dvdxp = (vxp - vc) * ixci, dvdxm = (vc - vxm) * ixcim;
+      double dvdyp = (vyp - vc) * iyfjp, dvdym = (vc - vym) * iyfj;
+      double dvdzp = (vp - vc) * izck, dvdzm = (vc - vm) * izckm;
+      b = visc * (dvdxp - dvdxm) * ixfi + visc * (dvdyp - dvdym) * iycj + visc * (dvdzp - dvdzm) * izfk;
+      double dwdxp = (wxp - wc) * ixci, dwdxm = (wc - wxm) * ixcim;
+      double dwdyp = (wyp - wc) * iycj, dwdym = (wc - wym) * iycjm;
+      double dwdzp = (wp - wc) * izfkp, dwdzm = (wc - wm) * izfk;
+      cc = visc * (dwdxp - dwdxm) * ixfi + visc * (dwdyp - dwdym) * iyfj + visc * (dwdzp - dwdzm) * izck;
+    }
+    {
+      double uuip = 0.25 * (uc * dyfj * dzfk + uxp * dyfj * dzfk) * uxp;
+      double uuim = 0.25 * (uc * dyfj * dzfk + uxm * dyfj * dzfk) * uxm;
+      double uvjp = 0.25 * (vc * dxfi * dzfk + vxp * dxfip * dzfk) * uyp;
+      double uvjm = 0.25 * (vym * dxfi * dzfk + v[ym + 1] * dxfip * dzfk) * uym;
+      double uwkp = 0.25 * (wc * dxfi * dyfj + wxp * dxfip * dyfj) * up;
+      double uwkm = 0.25 * (wm * dxfi * dyfj + w[zm + 1] * dxfip * dyfj) * um;
+      a = a + ((-uuip + uuim) + (-uvjp + uvjm) + (-uwkp + uwkm)) * (ixci * iyfj * izfk);
+      double uvip = 0.25 * (uc * dyfj * dzfk + uyp * dyfjp * dzfk) * vxp;
+      double uvim = 0.25 * (uxm * dyfj * dzfk + u[xm + sx] * dyfjp * dzfk) * vxm;
+      double vvjp = 0.25 * (vc * dxfi * dzfk + vyp * dxfi * dzfk) * vyp;
+      double vvjm = 0.25 * (vc * dxfi * dzfk + vym * dxfi * dzfk) * vym;
+      double wvkp = 0.25 * (wc * dyfj * dxfi + wyp * dyfjp * dxfi) * vp;
+      double wvkm = 0.25 * (wm * dyfj * dxfi + w[zm + sx] * dyfjp * dxfi) * vm;
+      b = b + ((-uvip + uvim) + (-vvjp + vvjm) + (-wvkp + wvkm)) * (ixfi * iycj * izfk);
+      double uwip = 0.25 * (uc * dzfk * dyfj + up * dzfkp * dyfj) * wxp;
+      double uwim = 0.25 * (uxm * dzfk * dyfj + u[xm + sz] * dzfkp * dyfj) * wxm;
+      double vwjp = 0.25 * (vc * dzfk * dxfi + vp * dzfkp * dxfi) * wyp;
+      double vwjm = 0.25 * (vym * dzfk * dxfi + v[ym + sz] * dzfkp * dxfi) * wym;
+      double wwkp = 0.25 * (wc * dxfi * dyfj + wp * dxfi * dyfj) * wp;
+      double wwkm = 0.25 * (wc * dxfi * dyfj + wm * dxfi * dyfj) * wm;
+      cc = cc + ((-uwip + uwim) + (-vwjp + vwjm) + (-wwkp + wwkm)) * (ixfi * iyfj * izck);
+    }
+    double un_ = uc + f1 * a + f2 * dtu[q];
+    double vn_ = vc + f1 * b + f2 * dtv[q];
+    double wn_ = wc + f1 * cc + f2 * dtw[q];
+    dtu[q] = a;
+    dtv[q] = b;
+    dtw[q] = cc;
+    un_ = un_ + f12 * (-(p[xp] - pc) * ixci + bfx);
+    vn_ = vn_ + f12 * (-(p[yp] - pc) * iycj + bfy);
+    wn_ = wn_ + f12 * (-(pz - pc) * izck + bfz);
+    if (BUOY) {
+      const double sc = s1[c];
+      un_ = un_ - f12 * gbx * 0.5 * (s1[xp] + sc);
+      vn_ = vn_ - f12 * gby * 0.5 * (s1[yp] + sc);
+      wn_ = wn_ - f12 * gbz * 0.5 * (s1[zp] + sc);
+    }
+    un[c] = un_;
+    vn[c] = vn_;
+    wn[c] = wn_;
+    um = uc; uc = up;
+    vm = vc; vc = vp;
+    wm = wc; wc = wp;
+    pc = pz;
+  }
+}
+
 // rk_scal (rk.f90:130-204) with scal_d / scal_a (scal.f90:89-155) fused:
 // reads s (ghosted) and u, v, w; writes the updated interior into sn (the
 // other buffer of the pair) and dsdtrk into dso.
@@ -351,7 +456,22 @@ dim3 ghosted_tiles(const Geo &g) { return dim3(((g.nx + 2 + TX - 1) / TX) * ((g.
 void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v, const double *w,
                    const double *p, double *un, double *vn, double *wn, double *dtu, double *dtv, double *dtw,
                    double f1, double f2, double f12, double visc, double bfx, double bfy, double bfz,
-                   const double *s1, const double gb[3]) {
+                   const double *s1, const double gb[3], int variant) {
+  if (variant > 0) {
+    const int kb = variant == 1 ? 8 : (variant == 2 ? 16 : 32);
+    dim3 grid((unsigned)(((g.nx + TX - 1) / TX) * ((g.ny + TY - 1) / TY) * ((g.nz + kb - 1) / kb)));
+#define SNAC_KZ(B, KB)                                                                                              \
+  hipLaunchKernelGGL((k_rk_mom_kz<B, KB>), grid, dim3(TX, TY), 0, s, g, m, u, v, w, p, un, vn, wn, dtu, dtv, dtw, f1, \
+                     f2, f12, visc, bfx, bfy, bfz, s1, gb[0], gb[1], gb[2])
+    if (s1) {
+      if (kb == 8) SNAC_KZ(true, 8); else if (kb == 16) SNAC_KZ(true, 16); else SNAC_KZ(true, 32);
+    } else {
+      if (kb == 8) SNAC_KZ(false, 8); else if (kb == 16) SNAC_KZ(false, 16); else SNAC_KZ(false, 32);
+    }
+#undef SNAC_KZ
+    SNAC_HIP(hipGetLastError());
+    return;
+  }
   if (s1)
     hipLaunchKernelGGL(k_rk_mom<true>, interior_tiles(g), dim3(TX, TY), 0, s, g, m, u, v, w, p, un, vn, wn, dtu, dtv,
                        dtw, f1, f2, f12, visc, bfx, bfy, bfz, s1, gb[0], gb[1], gb[2]);

@@ -1,9 +1,22 @@
 // Host side of the batched MG-preconditioned CG (mg.hpp): operator assembly
 // on the bounding box of the union (the formulas of init_matrix_2d
 // solver.f90:687-842 with lambda*dxf*dyf on the diagonal, init_n_2d_matrices
-// 981-1020, and of init_matrix_3d 158-372), Galerkin coarse levels by
-// semi-coarsening along a (and m on the 3D path), the V-cycle and the CG
-// iteration of all groups at once.
+// 981-1020, and of init_matrix_3d 158-372), coarse levels by semi-coarsening
+// along a (and m on the 3D path), the V-cycle and the CG iteration of all
+// groups at once.
+//
+// Coarse operators: cells are merged in pairs along the coarsened axis and
+// the operator is re-discretised on the merged cells rather than formed as
+// P^T A P with piecewise-constant P.  The Galerkin product keeps the fine
+// coupling across a merged face although the coarse cell centres are twice
+// as far apart, so every level would double the coarsened-direction
+// coupling relative to the others and the V-cycle would degrade with depth.
+// Re-discretised: couplings across the line axis add up (the merged face is
+// the sum of the faces), couplings along a coarsened axis and the Dirichlet
+// boundary terms in that direction are halved (twice the distance), the
+// lambda weight adds up (the merged cell area), and the diagonal is the sum
+// of the couplings plus the boundary terms, so Neumann/periodic problems
+// stay singular with the constant null vector.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -38,12 +51,27 @@ struct HostLev {
   int nm, na, ns, nmq;
   bool per_a, per_m;
   std::vector<double> dg, w, cs, ca, cm;
+  std::vector<double> bd[3];  // boundary terms of the diagonal from the s / a / m faces
   std::vector<unsigned char> act;
   long at(int m, int a, int s) const { return ((long)m * na + a) * ns + s; }
   void resize() {
     size_t n = (size_t)nmq * na * ns;
     dg.assign(n, 1.0); w.assign(n, 0.0); cs.assign(n, 0.0); ca.assign(n, 0.0); cm.assign(n, 0.0);
+    for (auto &v : bd) v.assign(n, 0.0);
     act.assign(n, 0);
+  }
+  // diagonal = couplings to all neighbours + boundary terms (active cells)
+  void set_diagonal() {
+    for (int m = 0; m < nmq; m++)
+      for (int a = 0; a < na; a++)
+        for (int s = 0; s < ns; s++) {
+          const long q = at(m, a, s);
+          if (!act[q]) { dg[q] = 1.0; continue; }
+          double d = bd[0][q] + bd[1][q] + bd[2][q] + cs[q] + (s > 0 ? cs[q - 1] : 0.0);
+          d += ca[q] + (a > 0 ? ca[at(m, a - 1, s)] : (per_a ? ca[at(m, na - 1, s)] : 0.0));
+          d += cm[q] + (m > 0 ? cm[at(m - 1, a, s)] : (per_m ? cm[at(nmq - 1, a, s)] : 0.0));
+          dg[q] = d;
+        }
   }
 };
 
@@ -67,46 +95,43 @@ HostLev coarsen(const HostLev &F, bool coupled) {
   C.per_a = F.per_a && C.na > 1;
   C.per_m = F.per_m && C.nm > 1;
   C.resize();
+  const double fa = ra ? 0.5 : 1.0, fm = rm ? 0.5 : 1.0;
   for (int M = 0; M < C.nmq; M++)
     for (int I = 0; I < C.na; I++)
       for (int s = 0; s < C.ns; s++) {
         const long qc = C.at(M, I, s);
-        double dg = 0.0, w = 0.0, csum = 0.0, ca = 0.0, cm = 0.0;
-        bool act = false;
         const int m0 = rm ? 2 * M : M, m1 = rm ? std::min(2 * M + 1, F.nmq - 1) : M;
         const int a0 = ra ? 2 * I : I, a1 = ra ? std::min(2 * I + 1, F.na - 1) : I;
+        bool act = false;
+        double w = 0.0, csum = 0.0, bds = 0.0, bda = 0.0, bdm = 0.0, ca = 0.0, cm = 0.0;
         for (int m = m0; m <= m1; m++)
           for (int a = a0; a <= a1; a++) {
             const long q = F.at(m, a, s);
-            if (F.act[q]) {
-              act = true;
-              dg += F.dg[q];
-              w += F.w[q];
-            }
             csum += F.cs[q];
+            if (!F.act[q]) continue;
+            act = true;
+            w += F.w[q];
+            bds += F.bd[0][q];
+            bda += F.bd[1][q];
+            bdm += F.bd[2][q];
           }
-        // links inside the coarse cell count twice on the diagonal (P^T M P)
-        for (int m = m0; m <= m1; m++) {
-          if (ra && a1 > a0) dg -= 2.0 * F.ca[F.at(m, a0, s)];
-          if (ra && C.na == 1 && F.per_a && F.na >= 2) dg -= 2.0 * F.ca[F.at(m, F.na - 1, s)];
-        }
-        for (int a = a0; a <= a1; a++) {
-          if (rm && m1 > m0) dg -= 2.0 * F.cm[F.at(m0, a, s)];
-          if (rm && C.nm == 1 && F.per_m && F.nm >= 2) dg -= 2.0 * F.cm[F.at(F.nm - 1, a, s)];
-        }
-        // links to the next coarse cell along a / m
+        // the link leaving the merged cell towards +a / +m (internal when the
+        // coarse axis collapses to one periodic cell)
         if (C.na > 1 || !ra)
           for (int m = m0; m <= m1; m++) ca += F.ca[F.at(m, a1, s)];
         if (coupled && (C.nm > 1 || !rm))
           for (int a = a0; a <= a1; a++) cm += F.cm[F.at(m1, a, s)];
         if (!act) continue;  // inactive: identity row
         C.act[qc] = 1;
-        C.dg[qc] = dg;
         C.w[qc] = w;
         C.cs[qc] = csum;
-        C.ca[qc] = ca;
-        C.cm[qc] = cm;
+        C.ca[qc] = fa * ca;
+        C.cm[qc] = fm * cm;
+        C.bd[0][qc] = bds;
+        C.bd[1][qc] = fa * bda;
+        C.bd[2][qc] = fm * bdm;
       }
+  C.set_diagonal();
   return C;
 }
 
@@ -194,6 +219,7 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
             ssum += cmv[e] + cpv[e];
           }
           double diag0 = -ssum;
+          double bdv[3] = {0.0, 0.0, 0.0};  // (1 - sgn) * c of the boundary faces
           for (int e = 0; e < nd; e++) {
             const int a = ax[e];
             if (cs.periods[a] != 0) continue;
@@ -203,6 +229,7 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
               const double sg = bc == 'N' ? 1.0 : (bc == 'D' ? -1.0 : 0.0);
               if (bc == 'D') has_dirichlet = true;
               diag0 += sg * cmv[e];
+              bdv[e] += (1.0 - sg) * cmv[e];
               cmv[e] = 0.0;
             }
             if (B.is_bound[a][1] && idx[e] == nn) {
@@ -210,6 +237,7 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
               const double sg = bc == 'N' ? 1.0 : (bc == 'D' ? -1.0 : 0.0);
               if (bc == 'D') has_dirichlet = true;
               diag0 += sg * cpv[e];
+              bdv[e] += (1.0 - sg) * cpv[e];
               cpv[e] = 0.0;
             }
           }
@@ -220,6 +248,7 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
           if (L0.act[q]) { P.why = "overlapping blocks"; return; }
           L0.act[q] = 1;
           L0.dg[q] = -diag0;
+          for (int e = 0; e < nd; e++) L0.bd[e][q] = bdv[e];
           L0.w[q] = coupled ? 0.0 : weight;
           L0.cs[q] = cpv[0];
           L0.ca[q] = cpv[1];

@@ -76,8 +76,10 @@ def test_r2r_lines(tmp_path, kind, n):
 
 
 # ---------------------------------------------------------------- routines
-def test_rk_mom(tmp_path):
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_rk_mom(tmp_path, variant):
     ctx, oc = channel(tmp_path)
+    ctx.set_rk_variant(variant)
     for irk in (1, 2, 3):
         seed_fields(ctx, oc, seed=irk)
         ctx.rk_mom(irk, 1.3e-3)

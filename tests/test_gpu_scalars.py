@@ -69,8 +69,10 @@ def test_scalar_boundp(tmp_path):
         np.testing.assert_array_equal(ctx.get(0, f), oc.get(0, f))
 
 
-def test_rk_mom_buoyancy(tmp_path):
+@pytest.mark.parametrize("variant", [0, 2])
+def test_rk_mom_buoyancy(tmp_path, variant):
     ctx, oc = scalar_pair(tmp_path)
+    ctx.set_rk_variant(variant)
     for irk in (1, 2, 3):
         seed(ctx, oc, seed=10 + irk, names=("u", "v", "w", "p", "s1", "dudtrko", "dvdtrko", "dwdtrko"))
         ctx.rk_mom(irk, 1.5e-3)
