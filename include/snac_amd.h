@@ -158,10 +158,12 @@ int snac_get_timing(snac_ctx *ctx, double *ms, int *count);
  * transform call site: forward ones, then inverse ones, then the
  * tridiagonal variant -- these only with the tensor-product solver; -1 =
  * dense path -- and last the rk_mom kernel variant: 0 one cell per thread,
- * 1/2/3 k-march over 8/16/32 planes) and the best times in ms.  No
+ * 1..6 k-march, (v-1)%3 selecting 8/16/32 planes and (v-1)/3 the tile
+ * 64x4 or 32x8) and the best times in
+ * ms.  No
  * counterpart in the reference (its FFTW plans use FFTW_ESTIMATE).          */
 int snac_get_autotune(snac_ctx *ctx, int *choices, double *ms, int n);
-/* overrides the autotuned rk_mom variant (0..3, as above); every variant
+/* overrides the autotuned rk_mom variant (0..6, as above); every variant
  * computes the same result.  No counterpart in the reference.              */
 int snac_set_rk_variant(snac_ctx *ctx, int variant);
 

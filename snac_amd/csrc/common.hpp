@@ -49,8 +49,8 @@ struct FaceOp {
 // rk_mom: reads u,v,w,p (ghosted) and dudtrko; writes the updated interior
 // velocities to un,vn,wn and dudtrk to dto.
 // s1 (scalar 1, ghosted) non-null adds the Boussinesq term with gb = gacc*beta;
-// variant 0: one cell per thread, 1..3: k-march over chunks of 8/16/32
-// planes (autotuned)
+// variant 0: one cell per thread; 1..6: k-march over chunks of 8/16/32
+// planes with 64x4 or 32x8 tiles (autotuned)
 void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v,
                    const double *w, const double *p, double *un, double *vn, double *wn, double *dtu,
                    double *dtv, double *dtw, double f1, double f2, double f12, double visc, double bfx,

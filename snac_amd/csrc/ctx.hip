@@ -621,14 +621,15 @@ void rk_mom(snac_ctx *c, int irk, double dt) {
   c->cur = nx;
 }
 
-// rk_mom kernel variant: one cell per thread or k-march over 8/16/32-plane
-// chunks, timed on this case's own arrays at context creation (the arrays
-// are still zero then; the buffers it writes are cleared afterwards)
+// rk_mom kernel variant (one cell per thread, or a k-march with one of 6
+// chunk/tile choices), timed on this case's own arrays at context
+// creation (the arrays are still zero then; the buffers it writes are
+// cleared afterwards)
 void autotune_rk(snac_ctx *c) {
   if (c->device < 0) return;
   double best = 1e300;
   int bv = 0;
-  for (int v = 0; v <= 3; v++) {
+  for (int v = 0; v <= 6; v++) {
     c->rk_variant = v;
     const int cur0 = c->cur;
     double ms = time_stage(c, [&] {
@@ -1180,7 +1181,7 @@ int snac_get_autotune(snac_ctx *c, int *choices, double *ms, int n) {
 int snac_set_rk_variant(snac_ctx *c, int variant) {
   return guard([&] {
     on_device(c);
-    if (variant < 0 || variant > 3) throw std::runtime_error("rk_mom variant must be 0..3");
+    if (variant < 0 || variant > 6) throw std::runtime_error("rk_mom variant must be 0..6");
     c->rk_variant = variant;
     drop_graph(c);
   });

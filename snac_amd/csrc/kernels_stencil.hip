@@ -126,8 +126,11 @@ __global__ __launch_bounds__(TX *TY) void k_rk_mom(Geo g, Metrics m, const doubl
 // from loads that neighbouring lanes and wavefronts issue for their own
 // columns.  Metrics along i, j are hoisted out of the march; those along k
 // are uniform per plane.
-template <bool BUOY, int KB>
-__global__ __launch_bounds__(TX *TY) void k_rk_mom_kz(Geo g, Metrics m, const double *__restrict__ u,
+// BX x BY threads per workgroup (256).  (Capping the registers for more
+// waves per SIMD measured slower: the per-wave load parallelism matters
+// more than the wave count here.)
+template <bool BUOY, int KB, int BX, int BY>
+__global__ __launch_bounds__(BX *BY) void k_rk_mom_kz(Geo g, Metrics m, const double *__restrict__ u,
                                                        const double *__restrict__ v, const double *__restrict__ w,
                                                        const double *__restrict__ p, double *__restrict__ un,
                                                        double *__restrict__ vn, double *__restrict__ wn,
@@ -137,9 +140,9 @@ __global__ __launch_bounds__(TX *TY) void k_rk_mom_kz(Geo g, Metrics m, const do
                                                        const double *__restrict__ s1, double gbx, double gby,
                                                        double gbz) {
   int bx, by, bz;
-  tile_of((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, bx, by, bz);
-  const int i = bx * TX + threadIdx.x + 1;
-  const int j = by * TY + threadIdx.y + 1;
+  tile_of((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, bx, by, bz);
+  const int i = bx * BX + threadIdx.x + 1;
+  const int j = by * BY + threadIdx.y + 1;
   const int k0 = bz * KB + 1;
   const int k1 = min(k0 + KB - 1, g.nz);
   if (i > g.nx || j > g.ny) return;
@@ -201,9 +204,6 @@ __global__ __launch_bounds__(TX *TY) void k_rk_mom_kz(Geo g, Metrics m, const do
     double un_ = uc + f1 * a + f2 * dtu[q];
     double vn_ = vc + f1 * b + f2 * dtv[q];
     double wn_ = wc + f1 * cc + f2 * dtw[q];
-    dtu[q] = a;
-    dtv[q] = b;
-    dtw[q] = cc;
     un_ = un_ + f12 * (-(p[xp] - pc) * ixci + bfx);
     vn_ = vn_ + f12 * (-(p[yp] - pc) * iycj + bfy);
     wn_ = wn_ + f12 * (-(pz - pc) * izck + bfz);
@@ -213,6 +213,9 @@ __global__ __launch_bounds__(TX *TY) void k_rk_mom_kz(Geo g, Metrics m, const do
       vn_ = vn_ - f12 * gby * 0.5 * (s1[yp] + sc);
       wn_ = wn_ - f12 * gbz * 0.5 * (s1[zp] + sc);
     }
+    dtu[q] = a;
+    dtv[q] = b;
+    dtw[q] = cc;
     un[c] = un_;
     vn[c] = vn_;
     wn[c] = wn_;
@@ -458,16 +461,28 @@ void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *
                    double f1, double f2, double f12, double visc, double bfx, double bfy, double bfz,
                    const double *s1, const double gb[3], int variant) {
   if (variant > 0) {
-    const int kb = variant == 1 ? 8 : (variant == 2 ? 16 : 32);
-    dim3 grid((unsigned)(((g.nx + TX - 1) / TX) * ((g.ny + TY - 1) / TY) * ((g.nz + kb - 1) / kb)));
-#define SNAC_KZ(B, KB)                                                                                              \
-  hipLaunchKernelGGL((k_rk_mom_kz<B, KB>), grid, dim3(TX, TY), 0, s, g, m, u, v, w, p, un, vn, wn, dtu, dtv, dtw, f1, \
-                     f2, f12, visc, bfx, bfy, bfz, s1, gb[0], gb[1], gb[2])
-    if (s1) {
-      if (kb == 8) SNAC_KZ(true, 8); else if (kb == 16) SNAC_KZ(true, 16); else SNAC_KZ(true, 32);
-    } else {
-      if (kb == 8) SNAC_KZ(false, 8); else if (kb == 16) SNAC_KZ(false, 16); else SNAC_KZ(false, 32);
-    }
+    // 1..6: k-march; (variant-1) % 3 -> KB 8/16/32, / 3 -> tile 64x4, 32x8
+    const int vi = variant - 1, kb = 8 << (vi % 3), shape = vi / 3;
+    const int bxs = shape == 1 ? 32 : 64, bys = 256 / bxs;
+    dim3 grid((unsigned)(((g.nx + bxs - 1) / bxs) * ((g.ny + bys - 1) / bys) * ((g.nz + kb - 1) / kb)));
+#define SNAC_KZ(B, KB, X, Y)                                                                                        \
+  hipLaunchKernelGGL((k_rk_mom_kz<B, KB, X, Y>), grid, dim3(X, Y), 0, s, g, m, u, v, w, p, un, vn, wn, dtu, dtv, dtw,  \
+                     f1, f2, f12, visc, bfx, bfy, bfz, s1, gb[0], gb[1], gb[2])
+#define SNAC_KZ_KB(B, X, Y)                                                                                         \
+  do {                                                                                                              \
+    if (kb == 8) SNAC_KZ(B, 8, X, Y);                                                                               \
+    else if (kb == 16) SNAC_KZ(B, 16, X, Y);                                                                        \
+    else SNAC_KZ(B, 32, X, Y);                                                                                      \
+  } while (0)
+#define SNAC_KZ_SHAPE(B)                                                                                            \
+  do {                                                                                                              \
+    if (shape == 0) SNAC_KZ_KB(B, 64, 4);                                                                           \
+    else SNAC_KZ_KB(B, 32, 8);                                                                                      \
+  } while (0)
+    if (s1) SNAC_KZ_SHAPE(true);
+    else SNAC_KZ_SHAPE(false);
+#undef SNAC_KZ_SHAPE
+#undef SNAC_KZ_KB
 #undef SNAC_KZ
     SNAC_HIP(hipGetLastError());
     return;

@@ -76,7 +76,7 @@ def test_r2r_lines(tmp_path, kind, n):
 
 
 # ---------------------------------------------------------------- routines
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6])
 def test_rk_mom(tmp_path, variant):
     ctx, oc = channel(tmp_path)
     ctx.set_rk_variant(variant)
