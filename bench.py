@@ -14,6 +14,12 @@ Prints one JSON line: Mcell-steps/s, the per-phase split (Poisson ms/step),
 the roofline of the dominant kernel (fused rk_mom, 104 algorithmic bytes
 per cell) from HIP events on its own stream, and the CPU port's rate on the
 host cores (a bounded sample of the same workload).
+
+--workload bfs runs the three-block backward-facing step instead (inlet
+192x64x128 with a Poiseuille inflow, lower 320x64x128, upper 320x96x128, z
+periodic with 128 modes, y tanh-stretched per block, outflow at x+): the
+union is not a box, so the pressure goes through the batched MG-PCG path
+(per-mode relative residual 1e-13); the CPU sample is skipped there.
 """
 import argparse
 import json
@@ -37,6 +43,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", choices=["channel", "bfs"], default="channel")
     ap.add_argument("--size", type=int, nargs=3, default=[256, 128, 256])
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
@@ -48,6 +55,31 @@ def channel_setup(tmp, size):
     import cases
     d, b = cases.write_case(tmp, [cases.channel_block(size)], 5600.0, cfl=0.5, bforce=(12.0 / 5600.0, 0.0, 0.0))
     return d, b
+
+
+def bfs_setup(tmp):
+    import cases
+    d, b = cases.write_case(tmp, cases.bfs_blocks(), 1000.0, cfl=0.5)
+    return d, b
+
+
+def bfs_ic(ctx, seed=20251202, amp=0.1):
+    """Poiseuille u = 6 y (1 - y) below the step height (inlet and lower
+    blocks), rest at zero, seeded +-amp noise on u, v, w interiors."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for b in range(ctx.nblocks):
+        sh = ctx.shape(b, "u")
+        yc = ctx.metric(b, 8)
+        u = np.zeros(sh); v = np.zeros(sh); w = np.zeros(sh)
+        inner = (slice(1, -1),) * 3
+        if yc[1:-1].max() <= 1.0:
+            u[inner] = (6.0 * yc[1:-1] * (1.0 - yc[1:-1]))[None, :, None]
+        for a in (u, v, w):
+            a[inner] += amp * rng.uniform(-1, 1, a[inner].shape)
+        out.append((u, v, w))
+    return out
 
 
 def cpu_baseline(d, b, size, steps, threads):
@@ -83,14 +115,23 @@ def main():
     import snac_amd
 
     size = args.size
-    cells = size[0] * size[1] * size[2]
+    bfs = args.workload == "bfs"
     with tempfile.TemporaryDirectory() as tmp:
-        d, b = channel_setup(tmp, size)
-        print(f"[bench] channel {size} on device 0", file=sys.stderr, flush=True)
+        if bfs:
+            d, b = bfs_setup(tmp)
+        else:
+            d, b = channel_setup(tmp, size)
+        print(f"[bench] {args.workload} {'' if bfs else size} on device 0", file=sys.stderr, flush=True)
         ctx = snac_amd.Context(d, b, 3, device=0)
-        u, v, w = cases.channel_ic(ctx.shape(0, "u"), ctx.metric(0, 8))
-        for f, a in (("u", u), ("v", v), ("w", w)):
-            ctx.set(0, f, a)
+        cells = sum(int(np.prod(np.array(ctx.shape(q, "u")) - 2)) for q in range(ctx.nblocks))
+        if bfs:
+            for q, fields in enumerate(bfs_ic(ctx)):
+                for f, a in zip(("u", "v", "w"), fields):
+                    ctx.set(q, f, a)
+        else:
+            u, v, w = cases.channel_ic(ctx.shape(0, "u"), ctx.metric(0, 8))
+            for f, a in (("u", u), ("v", v), ("w", w)):
+                ctx.set(0, f, a)
         ctx.init_bounds()
         tune_choices, tune_ms = ctx.autotune()
         dt = min(0.5 * ctx.chkdt(), ctx.dtmax)
@@ -99,12 +140,14 @@ def main():
         ctx.sync()
         print(f"[bench] warmup done, dt={dt:.3e}", file=sys.stderr, flush=True)
         # timed region: the production path, one hipGraph-captured RK3 step
-        # replayed K times on the context's stream
+        # replayed K times on the context's stream (eager launches on the
+        # MG-PCG path, whose CG loop checks convergence on the host)
         t0 = time.perf_counter()
         ctx.steps(dt, args.steps)
         ctx.sync()
         el = time.perf_counter() - t0
         divtot, divmax = ctx.chkdiv()
+        cg_iters, cg_relres = ctx.solver_stats()
         print(f"[bench] timed {args.steps} steps in {el:.3f} s, divmax={divmax:.2e}", file=sys.stderr, flush=True)
         # second timed pass: eager launches with HIP events around every
         # phase (on the stream the kernels run on) for the per-phase split
@@ -127,7 +170,7 @@ def main():
         rk_avg_s = rk_ms / rk_n * 1e-3
         achieved = RK_MOM_BYTES_PER_CELL * cells / rk_avg_s / 1e9
         traffic = None
-        if os.path.exists(TRAFFIC_FILE):
+        if not bfs and os.path.exists(TRAFFIC_FILE):
             tr = json.load(open(TRAFFIC_FILE))
             if tr.get("grid") == list(size):
                 traffic = tr.get("bytes_per_launch")
@@ -137,7 +180,7 @@ def main():
                     "stream_copy_GBs": round(copy_gbs, 1)}
 
         cpu = None
-        if args.cpu_steps > 0:
+        if args.cpu_steps > 0 and not bfs:
             thr = args.cpu_threads or min(16, os.cpu_count() or 1)
             print(f"[bench] CPU baseline: {args.cpu_steps} oracle steps on {thr} threads", file=sys.stderr,
                   flush=True)
@@ -148,11 +191,19 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (laminar Poiseuille + seeded +-10% uniform noise, seed 20251202)",
-        "config": {"workload": "plain FP64 FFT-accelerated periodic channel",
-                   "grid": list(size), "stretching": "y tanh two-end, factor 1.8", "fft_axis": "z",
-                   "Re_b": 5600, "forcing": "bforce(1)=12/5600",
-                   "poisson": "r2r along z and x, tridiagonal along y (exact)", "parallelism": "1 GPU",
-                   "autotune": {"choices": tune_choices, "best_ms": tune_ms}},
+        "config": ({"workload": "plain FP64 FFT-accelerated periodic channel",
+                    "grid": list(size), "stretching": "y tanh two-end, factor 1.8", "fft_axis": "z",
+                    "Re_b": 5600, "forcing": "bforce(1)=12/5600",
+                    "poisson": "r2r along z and x, tridiagonal along y (exact)", "parallelism": "1 GPU",
+                    "autotune": {"choices": tune_choices, "best_ms": tune_ms}} if not bfs else
+                   {"workload": "plain FP64 backward-facing step, 3 blocks on one GPU",
+                    "blocks": [[192, 64, 128], [320, 64, 128], [320, 96, 128]], "cells": cells,
+                    "stretching": "y tanh two-end per block, factor 1.2", "fft_axis": "z (128 modes)",
+                    "Re": 1000, "inflow": "Poiseuille (inflow_type 1)", "outflow": "x+ velocity N, pressure D",
+                    "poisson": "DFT along z, batched MG-preconditioned CG per mode (zebra y-line relaxation, "
+                               "x semi-coarsening, re-discretised coarse operators), relative residual 1e-13; the step graph runs a fixed CG count measured on the first step",
+                    "cg_iterations_last_solve": cg_iters, "cg_max_relres_last_solve": cg_relres,
+                    "parallelism": "1 GPU"}),
         "poisson_ms_per_step": round(poisson, 4),
         "phase_ms_per_step": phases,
         "phase_timing": f"eager launches with per-phase HIP events, {nph} steps, "

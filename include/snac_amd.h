@@ -168,9 +168,13 @@ int snac_get_autotune(snac_ctx *ctx, int *choices, double *ms, int n);
 int snac_set_rk_variant(snac_ctx *ctx, int variant);
 
 /* ---- building blocks, exposed for their own parity tests ----
- * Batched r2r transform of howmany contiguous lines of length n, kinds
- * numbered as FFTW's (fftw.f90:31-54), on the device; host in/out.          */
-int snac_r2r_lines(snac_ctx *ctx, int kind, int n, int howmany, const double *in, double *out);
+ * Batched r2r transform of howmany lines of length n, kinds numbered as
+ * FFTW's (fftw.f90:31-54), on the device; host in/out.  strided = 0: line l
+ * is contiguous at in[l*n]; 1: element q of line l is in[q*howmany + l]
+ * (the layout of a transform along a slow axis).  tile = FFT tile choice
+ * (0..2, as the autotuner's).                                               */
+int snac_r2r_lines(snac_ctx *ctx, int kind, int n, int howmany, int strided, int tile, const double *in,
+                   double *out);
 /* STREAM-like copy of n doubles (read n*8 + write n*8 bytes per repetition)
  * on the context's stream, timed with HIP events; returns GB/s.  The HBM
  * reference of the roofline and of the byte-counter calibration; it has no

@@ -73,7 +73,7 @@ def _load():
     L.snac_steps.argtypes = [vp, C.c_double, C.c_int]
     L.snac_sync.argtypes = [vp]
     L.snac_set_graph.argtypes = [vp, C.c_int]
-    L.snac_r2r_lines.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp]
+    L.snac_r2r_lines.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
     L.snac_stream_copy.argtypes = [vp, C.c_long, C.c_int, C.POINTER(C.c_double)]
     L.snac_get_autotune.argtypes = [vp, ip, dp, C.c_int]
     L.snac_set_rk_variant.argtypes = [vp, C.c_int]
@@ -212,12 +212,16 @@ class Context:
     def sync(self):
         _check(lib.snac_sync(self.h), "sync")
 
-    def r2r_lines(self, kind, x):
-        x = np.ascontiguousarray(np.atleast_2d(np.asarray(x, dtype=np.float64)))
-        out = np.zeros_like(x)
-        _check(lib.snac_r2r_lines(self.h, R2R_KINDS[kind], x.shape[1], x.shape[0], x.reshape(-1), out.reshape(-1)),
-               "r2r_lines")
-        return out
+    def r2r_lines(self, kind, x, strided=False, tile=0):
+        """Transforms every row of x (howmany x n); strided=True hands the
+        lines to the device transposed (element stride = howmany)."""
+        x = np.atleast_2d(np.asarray(x, dtype=np.float64))
+        h, n = x.shape
+        src = np.ascontiguousarray(x.T if strided else x)
+        out = np.zeros_like(src)
+        _check(lib.snac_r2r_lines(self.h, R2R_KINDS[kind], n, h, int(bool(strided)), int(tile), src.reshape(-1),
+                                  out.reshape(-1)), "r2r_lines")
+        return out.T if strided else out
 
     def set_timing(self, enable):
         _check(lib.snac_set_timing(self.h, int(bool(enable))), "set_timing")

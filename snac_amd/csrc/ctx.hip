@@ -78,6 +78,7 @@ struct snac_ctx {
   snac::TensorPlan tp;
   snac::MGPlan mg;
   bool mg_planned = false;
+  bool mg_fixed = false;  // solves run a fixed iteration count (graph capture)
   int solver = SNAC_SOLVER_AUTO;
   double *part_sum = nullptr, *part_max = nullptr, *red = nullptr;
   int nparts = 0;
@@ -526,7 +527,7 @@ void solve_pressure(snac_ctx *c) {
   check_solver(c);
   if (use_mg(c)) {
     { Phase tf(c, SNAC_PH_R2R_FWD); mg_stage_in(c); }
-    { Phase tt(c, SNAC_PH_TRIDIAG); mg_solve(c->mg, c->stream); }
+    { Phase tt(c, SNAC_PH_TRIDIAG); mg_solve(c->mg, c->stream, c->mg_fixed ? c->mg.fixed_iters : 0); }
     { Phase tb(c, SNAC_PH_R2R_BWD); mg_stage_out(c); }
     return;
   }
@@ -714,10 +715,21 @@ void drop_graph(snac_ctx *c) {
 // starting parity changes.
 void run_steps(snac_ctx *c, double dt, int nsteps) {
   check_solver(c);
-  if (!c->use_graph || c->timing || use_mg(c)) {
+  if (!c->use_graph || c->timing) {
     for (int s = 0; s < nsteps; s++)
       for (int irk = 1; irk <= 3; irk++) substage(c, irk, dt);
     return;
+  }
+  const bool mgp = use_mg(c);
+  if (mgp && c->mg.fixed_iters == 0) {
+    // MG-PCG in a graph needs a fixed CG iteration count: one step with
+    // host-checked convergence measures it, the graph then runs that count
+    // plus a margin (converged modes skip their work)
+    for (int irk = 1; irk <= 3; irk++) substage(c, irk, dt);
+    nsteps--;
+    MGPlan &P = c->mg;
+    P.fixed_iters = std::min(P.maxiter, std::max(1, P.max_seen + std::max(2, P.max_seen / 4)));
+    drop_graph(c);
   }
   // a step flips the velocity parity six times (back to the start) and the
   // scalar parity three times: one graph per starting scalar parity
@@ -729,14 +741,17 @@ void run_steps(snac_ctx *c, double dt, int nsteps) {
       hipGraph_t g;
       SNAC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       const int cur0 = c->cur;
+      c->mg_fixed = mgp;
       try {
         for (int irk = 1; irk <= 3; irk++) substage(c, irk, dt);
       } catch (...) {
+        c->mg_fixed = false;
         (void)hipStreamEndCapture(c->stream, &g);
         c->cur = cur0;
         c->scur = key;
         throw;
       }
+      c->mg_fixed = false;
       SNAC_HIP(hipStreamEndCapture(c->stream, &g));
       if (c->cur != cur0) throw std::runtime_error("internal: a step must return to the same buffer parity");
       c->scur = key;  // captured, not run yet
@@ -747,6 +762,18 @@ void run_steps(snac_ctx *c, double dt, int nsteps) {
     }
     SNAC_HIP(hipGraphLaunch(c->gexec[key], c->stream));
     if (c->cs.nscal > 0) c->scur = 1 - key;
+  }
+  if (mgp && nsteps > 0) {
+    // a solve that ended above the tolerance raises the iteration count of
+    // the following steps (the reference's solver also stops at maxiter)
+    MGPlan &P = c->mg;
+    SNAC_HIP(hipMemcpyAsync(P.left_host, P.unconv, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SNAC_HIP(hipStreamSynchronize(c->stream));
+    if (*P.left_host > 0 && P.fixed_iters < P.maxiter) {
+      P.fixed_iters = std::min(P.maxiter, P.fixed_iters + std::max(2, P.fixed_iters / 2));
+      mg_izero(c->stream, 1, P.unconv);
+      drop_graph(c);
+    }
   }
 }
 
@@ -1189,16 +1216,20 @@ int snac_set_rk_variant(snac_ctx *c, int variant) {
 
 int snac_sync(snac_ctx *c) { return guard([&] { SNAC_HIP(hipStreamSynchronize(on_device(c)->stream)); }); }
 
-int snac_r2r_lines(snac_ctx *c, int kind, int n, int howmany, const double *in, double *out) {
+int snac_r2r_lines(snac_ctx *c, int kind, int n, int howmany, int strided, int tile, const double *in,
+                   double *out) {
   return guard([&] {
     on_device(c);
+    if (tile < 0 || tile > 2) throw std::runtime_error("r2r_lines: tile must be 0..2");
     R2RPlan pl = r2r_plan(kind, n);
+    pl.lshift = tile;
     size_t tot = (size_t)n * howmany;
     double *din = nullptr, *dout = nullptr;
     SNAC_HIP(hipMalloc(&din, tot * sizeof(double)));
     SNAC_HIP(hipMalloc(&dout, tot * sizeof(double)));
     SNAC_HIP(hipMemcpyAsync(din, in, tot * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    launch_r2r(c->stream, pl, din, 1, n, 0, dout, 1, n, 0, howmany, 1, 1.0);
+    if (strided) launch_r2r(c->stream, pl, din, howmany, 1, 0, dout, howmany, 1, 0, howmany, 1, 1.0);
+    else launch_r2r(c->stream, pl, din, 1, n, 0, dout, 1, n, 0, howmany, 1, 1.0);
     SNAC_HIP(hipMemcpyAsync(out, dout, tot * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     SNAC_HIP(hipStreamSynchronize(c->stream));
     SNAC_HIP(hipFree(din));

@@ -412,6 +412,7 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
   P.done = dev_zeros<int>(P, P.ng);
   P.singular = dev_upload(P, sing);
   P.left = dev_zeros<int>(P, 1);
+  P.unconv = dev_zeros<int>(P, 1);
   SNAC_HIP(hipHostMalloc((void **)&P.left_host, sizeof(int), hipHostMallocDefault));
   P.ok = true;
 }
@@ -449,7 +450,7 @@ void project(MGPlan &P, hipStream_t st, double *v) {
 
 }  // namespace
 
-void mg_solve(MGPlan &P, hipStream_t st) {
+void mg_solve(MGPlan &P, hipStream_t st, int fixed) {
   const long n = (long)P.ng * P.gsize;
   const double tol2 = P.tol * P.tol;
   mg_zero(st, n, P.x);
@@ -462,23 +463,34 @@ void mg_solve(MGPlan &P, hipStream_t st) {
   int cur = 0;
   mg_dot(st, P.ng, P.gsize, P.r, P.z, nullptr, nullptr, P.part, P.nchunk, P.rz[cur], P.done, nullptr);
   mg_cg_dir(st, P.ng, P.gsize, nullptr, nullptr, P.z, P.p, P.done);
+  const int nit = fixed > 0 ? std::min(fixed, P.maxiter) : P.maxiter;
   int it = 0;
-  for (it = 1; it <= P.maxiter; it++) {
+  for (it = 1; it <= nit; it++) {
     mg_apply(st, P.lev[0], P.lam[0], P.p, P.q, P.done);
     mg_dot(st, P.ng, P.gsize, P.p, P.q, nullptr, nullptr, P.part, P.nchunk, P.pq, P.done, nullptr);
     mg_cg_update(st, P.ng, P.gsize, P.rz[cur], P.pq, P.x, P.p, P.r, P.q, P.done);
     mg_dot(st, P.ng, P.gsize, P.r, P.r, nullptr, nullptr, P.part, P.nchunk, P.rr, P.done, nullptr);
     mg_flags(st, P.ng, P.rr, P.bb, tol2, P.done, P.left);
-    SNAC_HIP(hipMemcpyAsync(P.left_host, P.left, sizeof(int), hipMemcpyDeviceToHost, st));
-    SNAC_HIP(hipStreamSynchronize(st));
-    if (*P.left_host == 0) break;
+    if (fixed == 0) {
+      SNAC_HIP(hipMemcpyAsync(P.left_host, P.left, sizeof(int), hipMemcpyDeviceToHost, st));
+      SNAC_HIP(hipStreamSynchronize(st));
+      if (*P.left_host == 0) break;
+    } else if (it == nit) {
+      break;
+    }
     vcycle(P, st, 0);
     project(P, st, P.z);
     mg_dot(st, P.ng, P.gsize, P.r, P.z, nullptr, nullptr, P.part, P.nchunk, P.rz[1 - cur], P.done, nullptr);
     mg_cg_dir(st, P.ng, P.gsize, P.rz[1 - cur], P.rz[cur], P.z, P.p, P.done);
     cur = 1 - cur;
   }
-  P.last_iters = std::min(it, P.maxiter);
+  if (fixed > 0) {
+    mg_accum_max(st, P.unconv, P.left);  // groups still above the tolerance
+    P.last_iters = nit;
+  } else {
+    P.last_iters = std::min(it, P.maxiter);
+    P.max_seen = std::max(P.max_seen, P.last_iters);
+  }
 }
 
 void mg_stats(MGPlan &P, hipStream_t st, int &iters, double &max_relres) {

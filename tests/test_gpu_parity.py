@@ -65,13 +65,20 @@ def channel(tmp_path, n=(32, 24, 16)):
 @pytest.mark.parametrize("kind", ["R2HC", "HC2R", "REDFT10", "REDFT01", "RODFT10", "RODFT01", "REDFT11", "RODFT11"])
 @pytest.mark.parametrize("n", [2, 4, 6, 16, 64, 256])
 def test_r2r_lines(tmp_path, kind, n):
+    """Every kind and tile choice, with contiguous lines and with lines
+    strided across a slow axis (odd line counts leave a packed pair half
+    empty)."""
     import snac_amd
     d, b = cases.write_case(str(tmp_path), [cases.channel_block((4, 4, 4))], 5600.0)
     ctx = snac_amd.Context(d, b, 3)
-    x = np.random.default_rng(n).uniform(-1, 1, (37, n))
-    y = ctx.r2r_lines(kind, x)
-    ref = np.array([oracle_py.r2r(kind, row, direct=True) for row in x])
-    assert np.abs(y - ref).max() <= 1e-13 * n * np.abs(ref).max()
+    for howmany in (37, 128):
+        x = np.random.default_rng(n + howmany).uniform(-1, 1, (howmany, n))
+        ref = np.array([oracle_py.r2r(kind, row, direct=True) for row in x])
+        for strided in (False, True):
+            for tile in (0, 1, 2):
+                y = ctx.r2r_lines(kind, x, strided=strided, tile=tile)
+                err = np.abs(y - ref).max()
+                assert err <= 1e-13 * n * np.abs(ref).max(), f"lines {howmany} strided {strided} tile {tile}: {err:.3e}"
     ctx.close()
 
 
