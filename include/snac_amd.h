@@ -163,9 +163,12 @@ int snac_get_timing(snac_ctx *ctx, double *ms, int *count);
  * ms.  No
  * counterpart in the reference (its FFTW plans use FFTW_ESTIMATE).          */
 int snac_get_autotune(snac_ctx *ctx, int *choices, double *ms, int n);
-/* overrides the autotuned rk_mom variant (0..6, as above); every variant
- * computes the same result.  No counterpart in the reference.              */
-int snac_set_rk_variant(snac_ctx *ctx, int variant);
+/* overrides an autotuned kernel variant: SNAC_KV_RK_MOM (0..6, as above) or
+ * SNAC_KV_TRIDIAG (0: c' in LDS, lines up to 128; 1: c' in scratch; 2:
+ * chunked, no scratch, lines of 8..256); every variant computes the same
+ * result.  No counterpart in the reference.                                 */
+enum { SNAC_KV_RK_MOM = 0, SNAC_KV_TRIDIAG = 1 };
+int snac_set_kernel_variant(snac_ctx *ctx, int kernel, int variant);
 
 /* ---- building blocks, exposed for their own parity tests ----
  * Batched r2r transform of howmany lines of length n, kinds numbered as

@@ -76,7 +76,7 @@ def _load():
     L.snac_r2r_lines.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
     L.snac_stream_copy.argtypes = [vp, C.c_long, C.c_int, C.POINTER(C.c_double)]
     L.snac_get_autotune.argtypes = [vp, ip, dp, C.c_int]
-    L.snac_set_rk_variant.argtypes = [vp, C.c_int]
+    L.snac_set_kernel_variant.argtypes = [vp, C.c_int, C.c_int]
     L.snac_set_timing.argtypes = [vp, C.c_int]
     L.snac_get_timing.argtypes = [vp, dp, ip]
     return L
@@ -264,8 +264,12 @@ class Context:
         return ch[:k].tolist(), [round(float(x), 4) for x in ms[:k] if x > 0]
 
     def set_rk_variant(self, v):
-        """Overrides the autotuned rk_mom kernel variant (0..3)."""
-        _check(lib.snac_set_rk_variant(self.h, int(v)), "set_rk_variant")
+        """Overrides the autotuned rk_mom kernel variant (0..6)."""
+        _check(lib.snac_set_kernel_variant(self.h, 0, int(v)), "set_kernel_variant")
+
+    def set_tridiag_variant(self, v):
+        """Overrides the autotuned tridiagonal kernel variant (0..2)."""
+        _check(lib.snac_set_kernel_variant(self.h, 1, int(v)), "set_kernel_variant")
 
     def new_dt(self):
         """main.f90:660-661: dt = dt_f if > 0 else min(cfl*dt_cfl, dtmax)."""

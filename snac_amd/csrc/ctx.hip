@@ -593,7 +593,7 @@ void autotune(snac_ctx *c) {
   tune(t.bwd[0], [&] { stage_last(c); });
   int best = 1;
   double bt = 1e300;
-  for (int v = (t.G[t.s] <= 128 ? 0 : 1); v <= 1; v++) {
+  for (int v = (t.G[t.s] <= 128 ? 0 : 1); v <= (tridiag_chunks(t.G[t.s]) > 0 ? 2 : 1); v++) {
     t.tvariant = v;
     double ms = time_stage(c, [&] { stage_tridiag(c); });
     if (ms < bt) { bt = ms; best = v; }
@@ -1205,11 +1205,21 @@ int snac_get_autotune(snac_ctx *c, int *choices, double *ms, int n) {
   });
 }
 
-int snac_set_rk_variant(snac_ctx *c, int variant) {
+int snac_set_kernel_variant(snac_ctx *c, int kernel, int variant) {
   return guard([&] {
     on_device(c);
-    if (variant < 0 || variant > 6) throw std::runtime_error("rk_mom variant must be 0..6");
-    c->rk_variant = variant;
+    if (kernel == SNAC_KV_RK_MOM) {
+      if (variant < 0 || variant > 6) throw std::runtime_error("rk_mom variant must be 0..6");
+      c->rk_variant = variant;
+    } else if (kernel == SNAC_KV_TRIDIAG) {
+      if (!c->tp.ok) throw std::runtime_error("no tensor-product plan on this case");
+      const int m = c->tp.G[c->tp.s];
+      if (variant < 0 || variant > 2 || (variant == 0 && m > 128) || (variant == 2 && tridiag_chunks(m) == 0))
+        throw std::runtime_error("tridiagonal variant not available for lines of " + std::to_string(m));
+      c->tp.tvariant = variant;
+    } else {
+      throw std::runtime_error("unknown kernel id");
+    }
     drop_graph(c);
   });
 }

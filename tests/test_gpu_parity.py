@@ -152,8 +152,9 @@ def test_chkdiv_chkdt(tmp_path):
     assert ctx.chkdt() == pytest.approx(oc.chkdt(), rel=1e-14)
 
 
+@pytest.mark.parametrize("tvariant", [0, 1, 2])
 @pytest.mark.parametrize("geom", ["channel", "cavity", "ldc_fft", "ldc_3d", "triperiodic", "triperiodic_3d"])
-def test_solve_pressure(tmp_path, geom):
+def test_solve_pressure(tmp_path, geom, tvariant):
     import snac_amd
     if geom == "channel":
         ctx, oc = channel(tmp_path)
@@ -168,6 +169,10 @@ def test_solve_pressure(tmp_path, geom):
         ctx = snac_amd.Context(dn, bn, fa)
         oc = oracle_py.OracleCase(nmlparse.load_case(dn, bn, fa))
         oc.set_solver("band")
+    try:
+        ctx.set_tridiag_variant(tvariant)
+    except snac_amd.SnacError:
+        pytest.skip(f"tridiagonal variant {tvariant} does not apply to {geom}")
     # a compatible right-hand side: fillps of a random velocity whose ghosts
     # obey the boundary conditions
     seed_fields(ctx, oc)
