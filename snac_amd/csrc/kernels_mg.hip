@@ -75,21 +75,12 @@ __device__ __forceinline__ Row pcr(const Row &r, const Row &lo, const Row &hi) {
 // rows per lane that form a tridiagonal system in (x_first, x_last) of all
 // chunks; PCR over the lanes solves it in log2(2*nch) steps; the interior
 // rows follow from the two interface values.
+// relaxes line (m, a) with the calling wavefront (all 64 lanes must call)
 template <int K>
-__global__ __launch_bounds__(256) void k_line(MGLev L, const double *__restrict__ lam, double *__restrict__ x,
-                                              const double *__restrict__ b, int col_a, int col_m, int nla, int nlm,
-                                              int zero_nb, int pin_s, const int *__restrict__ singular,
-                                              const int *__restrict__ done) {
+__device__ __forceinline__ void line_solve(const MGLev &L, double lmv, double *__restrict__ x,
+                                           const double *__restrict__ b, int m, int a, int zero_nb, int pin_s,
+                                           bool pin) {
   const int lane = threadIdx.x & 63;
-  const long line = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (line >= (long)nla * nlm) return;
-  const int a = 2 * (int)(line % nla) + col_a;
-  const int lm_ = (int)(line / nla);
-  const int m = L.coupled ? 2 * lm_ + col_m : lm_;
-  const int g = group_of(L, m);
-  if (done && done[g]) return;
-  const double lmv = lam ? lam[m] : 0.0;
-  const bool pin = pin_s >= 0 && singular && singular[g];
   const long e0 = vidx(L, m, a, 0), q0 = cidx(L, m, a, 0);
   const int ns = L.ns;
   const int cnt = lane < L.nch ? L.base + (lane < L.extra ? 1 : 0) : 0;
@@ -178,6 +169,80 @@ __global__ __launch_bounds__(256) void k_line(MGLev L, const double *__restrict_
 #pragma unroll
   for (int j = 1; j < K - 1; j++)
     if (j <= cnt - 2) x[e0 + p0 + j] = D[j] - A[j] * xp - C[j] * xq;
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_line(MGLev L, const double *__restrict__ lam, double *__restrict__ x,
+                                              const double *__restrict__ b, int col_a, int col_m, int nla, int nlm,
+                                              int zero_nb, int pin_s, const int *__restrict__ singular,
+                                              const int *__restrict__ done) {
+  const long line = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (line >= (long)nla * nlm) return;
+  const int a = 2 * (int)(line % nla) + col_a;
+  const int lm_ = (int)(line / nla);
+  const int m = L.coupled ? 2 * lm_ + col_m : lm_;
+  const int g = group_of(L, m);
+  if (done && done[g]) return;
+  const double lmv = lam ? lam[m] : 0.0;
+  const bool pin = pin_s >= 0 && singular && singular[g];
+  line_solve<K>(L, lmv, x, b, m, a, zero_nb, pin_s, pin);
+}
+
+// The coarse end of the V-cycle (levels t.l0 .. t.nl-1, the last one a
+// single line per mode) for one mode per workgroup: zebra sweeps with one
+// wavefront per line, residual restriction and prolongation with the whole
+// workgroup, barriers between the steps.  One launch replaces six per level.
+// FFT path only (independent modes).
+template <int K>
+__global__ __launch_bounds__(1024) void k_tail(MGTail t, const double *__restrict__ lam, int pin_s,
+                                               const int *__restrict__ singular, const int *__restrict__ done) {
+  const int m = blockIdx.x;
+  if (done && done[m]) return;
+  const double lmv = lam[m];
+  const int wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const bool pin = pin_s >= 0 && singular && singular[m];
+  // down: pre-smooth (colour 0 with zero neighbours, then colour 1), restrict
+  for (int l = 0; l < t.nl - 1; l++) {
+    const MGLev &L = t.lev[l];
+    for (int c = 0; c < 2; c++) {
+      for (int a = 2 * wv + c; a < L.na; a += 2 * nw) line_solve<K>(L, lmv, t.x[l], t.b[l], m, a, c == 0, -1, false);
+      __syncthreads();
+    }
+    const MGLev &Cl = t.lev[l + 1];
+    const int ra = L.na != Cl.na;
+    for (int e = threadIdx.x; e < Cl.na * Cl.ns; e += blockDim.x) {
+      const int s = e % Cl.ns, I = e / Cl.ns;
+      double sum = 0.0;
+      for (int da = 0; da <= ra; da++) {
+        const int a = ra ? 2 * I + da : I;
+        if (a >= L.na || !L.act[cidx(L, m, a, s)]) continue;
+        sum += t.b[l][vidx(L, m, a, s)] - mx_at(L, lmv, t.x[l], m, a, s);
+      }
+      t.b[l + 1][vidx(Cl, m, I, s)] = sum;
+    }
+    __syncthreads();
+  }
+  // coarsest: one line, solved exactly
+  {
+    const int l = t.nl - 1;
+    if (wv == 0) line_solve<K>(t.lev[l], lmv, t.x[l], t.b[l], m, 0, 1, pin_s, pin);
+    __syncthreads();
+  }
+  // up: prolong, post-smooth (colour 1, then colour 0)
+  for (int l = t.nl - 2; l >= 0; l--) {
+    const MGLev &L = t.lev[l], &Cl = t.lev[l + 1];
+    const int ra = L.na != Cl.na;
+    for (int e = threadIdx.x; e < L.na * L.ns; e += blockDim.x) {
+      const int s = e % L.ns, a = e / L.ns;
+      if (!L.act[cidx(L, m, a, s)]) continue;
+      t.x[l][vidx(L, m, a, s)] += t.x[l + 1][vidx(Cl, m, ra ? a >> 1 : a, s)];
+    }
+    __syncthreads();
+    for (int c = 1; c >= 0; c--) {
+      for (int a = 2 * wv + c; a < L.na; a += 2 * nw) line_solve<K>(L, lmv, t.x[l], t.b[l], m, a, 0, -1, false);
+      __syncthreads();
+    }
+  }
 }
 
 // coarse b = sum over the active fine members of (b - M x)
@@ -423,6 +488,17 @@ void mg_line(hipStream_t st, const MGLev &L, const double *lam, double *x, const
                        singular, done);
   else
     throw std::runtime_error("multigrid line relaxation: lines longer than 1088 cells are not supported");
+  SNAC_HIP(hipGetLastError());
+}
+
+void mg_tail(hipStream_t st, const MGTail &t, int nm, const double *lam, int pin_s, const int *singular,
+             const int *done) {
+  const MGLev &L = t.lev[0];
+  const int rows = L.base + (L.extra ? 1 : 0);
+  if (rows <= 3) hipLaunchKernelGGL(k_tail<3>, dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
+  else if (rows <= 5) hipLaunchKernelGGL(k_tail<5>, dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
+  else if (rows <= 9) hipLaunchKernelGGL(k_tail<9>, dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
+  else hipLaunchKernelGGL(k_tail<17>, dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
   SNAC_HIP(hipGetLastError());
 }
 

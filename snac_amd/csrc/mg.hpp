@@ -39,7 +39,19 @@ struct MGLev {
   int nch, base, extra; // a line of ns rows over nch lanes: base (+1 for the first `extra`) rows each
 };
 
+// the coarse end of the hierarchy handled by one fused kernel (FFT path)
+constexpr int MG_TAIL_MAX = 12;
+struct MGTail {
+  int nl;                   // levels in the tail; the last one has na == 1
+  MGLev lev[MG_TAIL_MAX];
+  double *x[MG_TAIL_MAX], *b[MG_TAIL_MAX];
+};
+
 // ---- kernels (kernels_mg.hip) ----
+// V-cycle over the tail levels, one workgroup per mode (b of the first
+// tail level is given; x of every tail level is overwritten)
+void mg_tail(hipStream_t st, const MGTail &t, int nm, const double *lam, int pin_s, const int *singular,
+             const int *done);
 // zebra line relaxation of the lines (m, a) with a % 2 == col_a (and, when
 // coupled, m % 2 == col_m); zero_nb: neighbouring lines read as zero;
 // pin_s >= 0: in singular groups row pin_s is replaced by x = 0
@@ -101,6 +113,8 @@ struct MGPlan {
   int maxiter = 50;
   int colors = 2;              // zebra colours per sweep (2, or 4 when coupled)
   int nlev = 0;
+  int tail0 = -1;              // first level handled by the fused tail kernel (-1: none)
+  MGTail tail;
   int last_iters = 0;
   // fixed-iteration mode (graph capture): CG iteration count, the largest
   // count an eagerly checked solve needed, and a device counter of groups

@@ -400,6 +400,22 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
       P.bl.push_back(dev_zeros<double>(P, n));
     }
   }
+  // fused tail: the levels with at most 32 columns (FFT path, modes independent)
+  P.tail0 = -1;
+  if (!coupled) {
+    for (size_t l = 0; l < P.lev.size(); l++)
+      if (P.lev[l].na <= 32) { P.tail0 = (int)l; break; }
+    if (P.tail0 >= 0 && (int)P.lev.size() - P.tail0 > MG_TAIL_MAX) P.tail0 = (int)P.lev.size() - MG_TAIL_MAX;
+    if (P.tail0 >= 0 && (int)P.lev.size() - P.tail0 < 2) P.tail0 = -1;  // nothing to fuse
+    if (P.tail0 >= 0) {
+      P.tail.nl = (int)P.lev.size() - P.tail0;
+      for (int q = 0; q < P.tail.nl; q++) {
+        P.tail.lev[q] = P.lev[P.tail0 + q];
+        P.tail.x[q] = P.xl[P.tail0 + q];
+        P.tail.b[q] = P.bl[P.tail0 + q];
+      }
+    }
+  }
   P.nchunk = (int)std::max<long>(1, std::min<long>((P.gsize + 2047) / 2048, std::max(1, 8192 / P.ng)));
   P.part = dev_zeros<double>(P, (size_t)2 * P.ng * P.nchunk);
   P.bb = dev_zeros<double>(P, 2 * P.ng);
@@ -423,6 +439,10 @@ void vcycle(MGPlan &P, hipStream_t st, size_t l) {
   const MGLev &L = P.lev[l];
   double *x = P.xl[l];
   const double *b = P.bl[l];
+  if ((int)l == P.tail0) {  // the coarse end in one launch, one workgroup per mode
+    mg_tail(st, P.tail, P.ng, P.lam[l], P.pin_s, P.singular, P.done);
+    return;
+  }
   if (l + 1 == P.lev.size()) {  // one line per group: direct solve
     mg_line(st, L, P.lam[l], x, b, 0, 0, 1, P.pin_s, P.singular, P.done);
     return;
