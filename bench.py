@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", choices=["channel", "bfs"], default="channel")
     ap.add_argument("--size", type=int, nargs=3, default=[256, 128, 256])
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="mixed: FP32 MG V-cycle under flexible CG (MG-PCG path only)")
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
     ap.add_argument("--calib", action="store_true", help="also run the plain copy kernel (counter calibration)")
@@ -123,6 +125,7 @@ def main():
             d, b = channel_setup(tmp, size)
         print(f"[bench] {args.workload} {'' if bfs else size} on device 0", file=sys.stderr, flush=True)
         ctx = snac_amd.Context(d, b, 3, device=0)
+        ctx.set_precision(args.precision)
         cells = sum(int(np.prod(np.array(ctx.shape(q, "u")) - 2)) for q in range(ctx.nblocks))
         if bfs:
             for q, fields in enumerate(bfs_ic(ctx)):
@@ -189,7 +192,7 @@ def main():
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mcell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f64+f32",
         "data": "synthetic (laminar Poiseuille + seeded +-10% uniform noise, seed 20251202)",
         "config": ({"workload": "plain FP64 FFT-accelerated periodic channel",
                     "grid": list(size), "stretching": "y tanh two-end, factor 1.8", "fft_axis": "z",
@@ -203,6 +206,9 @@ def main():
                     "poisson": "DFT along z, batched MG-preconditioned CG per mode (zebra y-line relaxation, "
                                "x semi-coarsening, re-discretised coarse operators), relative residual 1e-13; the step graph runs a fixed CG count measured on the first step",
                     "cg_iterations_last_solve": cg_iters, "cg_max_relres_last_solve": cg_relres,
+                    "precision": ("FP64 throughout" if args.precision == "fp64" else
+                                  "FP32 V-cycle (smoothing, transfers, coarse solves) under flexible CG with "
+                                  "FP64 vectors, operator, dot products and null-space removal"),
                     "parallelism": "1 GPU"}),
         "poisson_ms_per_step": round(poisson, 4),
         "phase_ms_per_step": phases,

@@ -77,6 +77,17 @@ int snac_ctx_get_params(const snac_ctx *ctx, double *dparams, int *iparams);
  * mode at ||r|| <= hypre_tol ||b|| or after hypre_maxiter iterations, the
  * &hypre parameters of param.f90:101, solver.f90:389-451)                   */
 int snac_ctx_set_solver(snac_ctx *ctx, int solver);
+/* Arithmetic of the pressure solve (no counterpart in the reference, whose
+ * precision is the compile-time real(rp), types.f90:17-19):
+ * SNAC_PREC_FP64 (the default) runs everything in FP64; SNAC_PREC_MIXED
+ * runs the MG-PCG V-cycle (line smoothing, restriction, prolongation,
+ * coarse solves) in FP32 under a flexible CG whose vectors, operator, dot
+ * products and null-space removal stay FP64, so the solve still stops at
+ * hypre_tol.  The stencils and the direct tensor-product solver stay FP64.
+ * Drops the captured step graph; the next graph step re-measures the CG
+ * count. */
+enum { SNAC_PREC_FP64 = 0, SNAC_PREC_MIXED = 1 };
+int snac_set_precision(snac_ctx *ctx, int mode);
 /* CG iterations of the last MG-PCG pressure solve and its largest relative
  * residual over the modes (HYPRE's StructPCGGetNumIterations /
  * GetFinalRelativeResidualNorm, solver.f90:553-569); synchronises.  Both

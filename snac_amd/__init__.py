@@ -53,6 +53,7 @@ def _load():
     L.snac_ctx_get_metric.argtypes = [vp, C.c_int, C.c_int, dp]
     L.snac_ctx_get_params.argtypes = [vp, dp, ip]
     L.snac_ctx_set_solver.argtypes = [vp, C.c_int]
+    L.snac_set_precision.argtypes = [vp, C.c_int]
     L.snac_solver_stats.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.snac_upload_field.argtypes = [vp, C.c_int, C.c_int, dp]
     L.snac_download_field.argtypes = [vp, C.c_int, C.c_int, dp]
@@ -239,6 +240,15 @@ class Context:
         """Poisson solver: 'auto' (tensor-product when the union is a box with
         uniform transform axes, else MG-PCG), 'tensor' or 'mg'."""
         _check(lib.snac_ctx_set_solver(self.h, self.SOLVERS[name]), "set_solver")
+
+    PRECISIONS = {"fp64": 0, "mixed": 1}
+
+    def set_precision(self, mode):
+        """'fp64' (default) or 'mixed': the MG-PCG V-cycle in FP32 under a
+        flexible CG whose vectors, operator, dot products and null-space
+        removal stay FP64 (same stopping tolerance).  Stencils and the direct
+        tensor-product solver stay FP64."""
+        _check(lib.snac_set_precision(self.h, self.PRECISIONS[mode]), "set_precision")
 
     def solver_stats(self):
         """(CG iterations, largest relative residual over the modes) of the

@@ -80,6 +80,7 @@ struct snac_ctx {
   bool mg_planned = false;
   bool mg_fixed = false;  // solves run a fixed iteration count (graph capture)
   int solver = SNAC_SOLVER_AUTO;
+  int prec = SNAC_PREC_FP64;
   double *part_sum = nullptr, *part_max = nullptr, *red = nullptr;
   int nparts = 0;
   double dlmin = 0;
@@ -367,6 +368,7 @@ void plan_mg(snac_ctx *c) {
   c->mg_planned = true;
   MGPlan &P = c->mg;
   mg_plan(P, c->cs, c->device);
+  P.prec = c->prec;
   if (!P.ok || c->device < 0 || P.fa < 0) return;
   P.fwd = r2r_plan(P.kind_fwd, P.G[P.fa]);
   P.bwd = r2r_plan(P.kind_bwd, P.G[P.fa]);
@@ -1010,6 +1012,19 @@ int snac_solver_stats(snac_ctx *c, int *iters, double *max_relres) {
     if (use_mg(c) && c->mg.ok) mg_stats(c->mg, c->stream, it, rel);
     if (iters) *iters = it;
     if (max_relres) *max_relres = rel;
+  });
+}
+
+int snac_set_precision(snac_ctx *c, int mode) {
+  return guard([&] {
+    checked(c);
+    if (mode != SNAC_PREC_FP64 && mode != SNAC_PREC_MIXED) throw std::runtime_error("unknown precision mode");
+    if (c->device >= 0) SNAC_HIP(hipStreamSynchronize(c->stream));
+    c->prec = mode;
+    c->mg.prec = mode;
+    c->mg.fixed_iters = 0;  // the graph's CG count is measured again
+    c->mg.max_seen = 0;
+    drop_graph(c);
   });
 }
 

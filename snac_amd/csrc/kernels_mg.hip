@@ -9,12 +9,15 @@
 namespace snac {
 namespace {
 
-__device__ __forceinline__ long vidx(const MGLev &L, int m, int a, int s) { return ((long)m * L.na + a) * L.ns + s; }
-__device__ __forceinline__ long cidx(const MGLev &L, int m, int a, int s) { return m * L.qm + (long)a * L.ns + s; }
+template <typename T>
+__device__ __forceinline__ long vidx(const MGLevT<T> &L, int m, int a, int s) { return ((long)m * L.na + a) * L.ns + s; }
+template <typename T>
+__device__ __forceinline__ long cidx(const MGLevT<T> &L, int m, int a, int s) { return m * L.qm + (long)a * L.ns + s; }
 
 // sum of the couplings to the neighbouring lines (a and m directions) times x
-__device__ __forceinline__ double nb_sum(const MGLev &L, const double *__restrict__ x, int m, int a, long e, long q) {
-  double r = 0.0;
+template <typename T>
+__device__ __forceinline__ T nb_sum(const MGLevT<T> &L, const T *__restrict__ x, int m, int a, long e, long q) {
+  T r = 0;
   const long sa = L.ns;
   if (a < L.na - 1) r += L.ca[q] * x[e + sa];
   else if (L.per_a) r += L.ca[q] * x[e - (long)(L.na - 1) * sa];
@@ -30,38 +33,44 @@ __device__ __forceinline__ double nb_sum(const MGLev &L, const double *__restric
   return r;
 }
 
-__device__ __forceinline__ double mx_at(const MGLev &L, double lm, const double *__restrict__ x, int m, int a, int s) {
+template <typename T>
+__device__ __forceinline__ T mx_at(const MGLevT<T> &L, T lm, const T *__restrict__ x, int m, int a, int s) {
   const long e = vidx(L, m, a, s), q = cidx(L, m, a, s);
-  double y = (L.dg[q] - lm * L.w[q]) * x[e];
+  T y = (L.dg[q] - lm * L.w[q]) * x[e];
   if (s > 0) y -= L.cs[q - 1] * x[e - 1];
   if (s < L.ns - 1) y -= L.cs[q] * x[e + 1];
   return y - nb_sum(L, x, m, a, e, q);
 }
 
-__device__ __forceinline__ int group_of(const MGLev &L, int m) { return L.coupled ? 0 : m; }
+template <typename T>
+__device__ __forceinline__ int group_of(const MGLevT<T> &L, int m) { return L.coupled ? 0 : m; }
 
-__device__ __forceinline__ double shfl(double v, int src) { return __shfl(v, src, 64); }
+template <typename T>
+__device__ __forceinline__ T shfl(T v, int src) { return __shfl(v, src, 64); }
 
 // One reduced row  al*u[i-d] + be*u[i] + ga*u[i+d] = de  of the interface
 // system; rows outside the line are the identity.
+template <typename T>
 struct Row {
-  double al, be, ga, de;
+  T al, be, ga, de;
 };
 
-__device__ __forceinline__ Row fetch(const Row &r, int src, bool valid) {
-  Row o;
+template <typename T>
+__device__ __forceinline__ Row<T> fetch(const Row<T> &r, int src, bool valid) {
+  Row<T> o;
   o.al = shfl(r.al, src);
   o.be = shfl(r.be, src);
   o.ga = shfl(r.ga, src);
   o.de = shfl(r.de, src);
-  if (!valid) { o.al = 0.0; o.be = 1.0; o.ga = 0.0; o.de = 0.0; }
+  if (!valid) { o.al = 0; o.be = 1; o.ga = 0; o.de = 0; }
   return o;
 }
 
 // PCR elimination of row r against its neighbours lo (index i-d) and hi (i+d)
-__device__ __forceinline__ Row pcr(const Row &r, const Row &lo, const Row &hi) {
-  const double k1 = r.al / lo.be, k2 = r.ga / hi.be;
-  Row o;
+template <typename T>
+__device__ __forceinline__ Row<T> pcr(const Row<T> &r, const Row<T> &lo, const Row<T> &hi) {
+  const T k1 = r.al / lo.be, k2 = r.ga / hi.be;
+  Row<T> o;
   o.al = -k1 * lo.al;
   o.be = r.be - k1 * lo.ga - k2 * hi.al;
   o.ga = -k2 * hi.ga;
@@ -76,29 +85,28 @@ __device__ __forceinline__ Row pcr(const Row &r, const Row &lo, const Row &hi) {
 // chunks; PCR over the lanes solves it in log2(2*nch) steps; the interior
 // rows follow from the two interface values.
 // relaxes line (m, a) with the calling wavefront (all 64 lanes must call)
-template <int K>
-__device__ __forceinline__ void line_solve(const MGLev &L, double lmv, double *__restrict__ x,
-                                           const double *__restrict__ b, int m, int a, int zero_nb, int pin_s,
-                                           bool pin) {
+template <int K, typename T>
+__device__ __forceinline__ void line_solve(const MGLevT<T> &L, T lmv, T *__restrict__ x, const T *__restrict__ b,
+                                           int m, int a, int zero_nb, int pin_s, bool pin) {
   const int lane = threadIdx.x & 63;
   const long e0 = vidx(L, m, a, 0), q0 = cidx(L, m, a, 0);
   const int ns = L.ns;
   const int cnt = lane < L.nch ? L.base + (lane < L.extra ? 1 : 0) : 0;
   const int p0 = lane * L.base + min(lane, L.extra);
 
-  double lo[K], dg[K], up[K], rh[K];
+  T lo[K], dg[K], up[K], rh[K];
 #pragma unroll
   for (int j = 0; j < K; j++) {
-    lo[j] = 0.0; dg[j] = 1.0; up[j] = 0.0; rh[j] = 0.0;
+    lo[j] = 0; dg[j] = 1; up[j] = 0; rh[j] = 0;
     if (j < cnt) {
       const int s = p0 + j;
       const long e = e0 + s, q = q0 + s;
-      double d = L.dg[q] - lmv * L.w[q];
-      double l = s > 0 ? -L.cs[q - 1] : 0.0;
-      double u = s < ns - 1 ? -L.cs[q] : 0.0;
-      double r = b[e];
+      T d = L.dg[q] - lmv * L.w[q];
+      T l = s > 0 ? -L.cs[q - 1] : T(0);
+      T u = s < ns - 1 ? -L.cs[q] : T(0);
+      T r = b[e];
       if (!zero_nb) r += nb_sum(L, x, m, a, e, q);
-      if (pin && s == pin_s) { d = 1.0; l = 0.0; u = 0.0; r = 0.0; }
+      if (pin && s == pin_s) { d = 1; l = 0; u = 0; r = 0; }
       lo[j] = l; dg[j] = d; up[j] = u; rh[j] = r;
     }
   }
@@ -107,25 +115,25 @@ __device__ __forceinline__ void line_solve(const MGLev &L, double lmv, double *_
     return;
   }
   // downward elimination: row j (1 <= j < cnt) becomes A x_p0 + x_j + C x_{j+1} = D
-  double A[K], C[K], D[K];
-  double aq = 0.0, cq = 0.0, dq = 0.0;
+  T A[K], C[K], D[K];
+  T aq = 0, cq = 0, dq = 0;
 #pragma unroll
-  for (int j = 0; j < K; j++) { A[j] = 0.0; C[j] = 0.0; D[j] = 0.0; }
+  for (int j = 0; j < K; j++) { A[j] = 0; C[j] = 0; D[j] = 0; }
 #pragma unroll
   for (int j = 1; j < K; j++) {
     if (j < cnt) {
       if (j == 1) {
-        const double ir = 1.0 / dg[1];
+        const T ir = T(1) / dg[1];
         A[1] = lo[1] * ir; C[1] = up[1] * ir; D[1] = rh[1] * ir;
       } else {
-        const double ir = 1.0 / (dg[j] - lo[j] * C[j - 1]);
+        const T ir = T(1) / (dg[j] - lo[j] * C[j - 1]);
         A[j] = -lo[j] * A[j - 1] * ir; C[j] = up[j] * ir; D[j] = (rh[j] - lo[j] * D[j - 1]) * ir;
       }
       if (j == cnt - 1) { aq = A[j]; cq = C[j]; dq = D[j]; }
     }
   }
   // upward elimination: x_j = D - A x_p0 - C x_last for 1 <= j <= cnt-2
-  double dn = 0.0, an = 0.0, cn = 0.0;
+  T dn = 0, an = 0, cn = 0;
 #pragma unroll
   for (int j = K - 2; j >= 1; j--) {
     if (j <= cnt - 2) {
@@ -134,22 +142,22 @@ __device__ __forceinline__ void line_solve(const MGLev &L, double lmv, double *_
       D[j] = dn; A[j] = an; C[j] = cn;
     }
   }
-  Row r0, r1;
+  Row<T> r0, r1;
   if (cnt >= 2) {
     r0.al = lo[0];
     if (cnt == 2) { r0.be = dg[0]; r0.ga = up[0]; r0.de = rh[0]; }
     else { r0.be = dg[0] - up[0] * an; r0.ga = -up[0] * cn; r0.de = rh[0] - up[0] * dn; }
-    r1.al = aq; r1.be = 1.0; r1.ga = cq; r1.de = dq;
+    r1.al = aq; r1.be = 1; r1.ga = cq; r1.de = dq;
   } else {
-    r0 = Row{0.0, 1.0, 0.0, 0.0};
+    r0 = Row<T>{0, 1, 0, 0};
     r1 = r0;
   }
   // interface system: u[2c] = x_first(c), u[2c+1] = x_last(c); PCR, d = 1
   {
-    const Row r1m = fetch(r1, lane > 0 ? lane - 1 : 0, lane > 0);
-    const Row r0p = fetch(r0, lane < 63 ? lane + 1 : 63, lane < 63);
-    const Row n0 = pcr(r0, r1m, r1);
-    const Row n1 = pcr(r1, r0, r0p);
+    const Row<T> r1m = fetch(r1, lane > 0 ? lane - 1 : 0, lane > 0);
+    const Row<T> r0p = fetch(r0, lane < 63 ? lane + 1 : 63, lane < 63);
+    const Row<T> n0 = pcr(r0, r1m, r1);
+    const Row<T> n1 = pcr(r1, r0, r0p);
     r0 = n0;
     r1 = n1;
   }
@@ -157,13 +165,13 @@ __device__ __forceinline__ void line_solve(const MGLev &L, double lmv, double *_
   for (int h = 1; 2 * h < nred; h <<= 1) {  // d = 2h: lanes -+h, same slot
     const bool vm = lane >= h, vp = lane + h < 64;
     const int sm = vm ? lane - h : lane, sp = vp ? lane + h : lane;
-    const Row a0 = fetch(r0, sm, vm), b0 = fetch(r0, sp, vp);
-    const Row a1 = fetch(r1, sm, vm), b1 = fetch(r1, sp, vp);
+    const Row<T> a0 = fetch(r0, sm, vm), b0 = fetch(r0, sp, vp);
+    const Row<T> a1 = fetch(r1, sm, vm), b1 = fetch(r1, sp, vp);
     r0 = pcr(r0, a0, b0);
     r1 = pcr(r1, a1, b1);
   }
   if (cnt == 0) return;
-  const double xp = r0.de / r0.be, xq = r1.de / r1.be;
+  const T xp = r0.de / r0.be, xq = r1.de / r1.be;
   x[e0 + p0] = xp;
   x[e0 + p0 + cnt - 1] = xq;
 #pragma unroll
@@ -171,9 +179,9 @@ __device__ __forceinline__ void line_solve(const MGLev &L, double lmv, double *_
     if (j <= cnt - 2) x[e0 + p0 + j] = D[j] - A[j] * xp - C[j] * xq;
 }
 
-template <int K>
-__global__ __launch_bounds__(256) void k_line(MGLev L, const double *__restrict__ lam, double *__restrict__ x,
-                                              const double *__restrict__ b, int col_a, int col_m, int nla, int nlm,
+template <int K, typename T>
+__global__ __launch_bounds__(256) void k_line(MGLevT<T> L, const T *__restrict__ lam, T *__restrict__ x,
+                                              const T *__restrict__ b, int col_a, int col_m, int nla, int nlm,
                                               int zero_nb, int pin_s, const int *__restrict__ singular,
                                               const int *__restrict__ done) {
   const long line = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -183,9 +191,9 @@ __global__ __launch_bounds__(256) void k_line(MGLev L, const double *__restrict_
   const int m = L.coupled ? 2 * lm_ + col_m : lm_;
   const int g = group_of(L, m);
   if (done && done[g]) return;
-  const double lmv = lam ? lam[m] : 0.0;
+  const T lmv = lam ? lam[m] : T(0);
   const bool pin = pin_s >= 0 && singular && singular[g];
-  line_solve<K>(L, lmv, x, b, m, a, zero_nb, pin_s, pin);
+  line_solve<K, T>(L, lmv, x, b, m, a, zero_nb, pin_s, pin);
 }
 
 // The coarse end of the V-cycle (levels t.l0 .. t.nl-1, the last one a
@@ -193,26 +201,26 @@ __global__ __launch_bounds__(256) void k_line(MGLev L, const double *__restrict_
 // wavefront per line, residual restriction and prolongation with the whole
 // workgroup, barriers between the steps.  One launch replaces six per level.
 // FFT path only (independent modes).
-template <int K>
-__global__ __launch_bounds__(1024) void k_tail(MGTail t, const double *__restrict__ lam, int pin_s,
+template <int K, typename T>
+__global__ __launch_bounds__(1024) void k_tail(MGTailT<T> t, const T *__restrict__ lam, int pin_s,
                                                const int *__restrict__ singular, const int *__restrict__ done) {
   const int m = blockIdx.x;
   if (done && done[m]) return;
-  const double lmv = lam[m];
+  const T lmv = lam[m];
   const int wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const bool pin = pin_s >= 0 && singular && singular[m];
   // down: pre-smooth (colour 0 with zero neighbours, then colour 1), restrict
   for (int l = 0; l < t.nl - 1; l++) {
-    const MGLev &L = t.lev[l];
+    const MGLevT<T> &L = t.lev[l];
     for (int c = 0; c < 2; c++) {
-      for (int a = 2 * wv + c; a < L.na; a += 2 * nw) line_solve<K>(L, lmv, t.x[l], t.b[l], m, a, c == 0, -1, false);
+      for (int a = 2 * wv + c; a < L.na; a += 2 * nw) line_solve<K, T>(L, lmv, t.x[l], t.b[l], m, a, c == 0, -1, false);
       __syncthreads();
     }
-    const MGLev &Cl = t.lev[l + 1];
+    const MGLevT<T> &Cl = t.lev[l + 1];
     const int ra = L.na != Cl.na;
     for (int e = threadIdx.x; e < Cl.na * Cl.ns; e += blockDim.x) {
       const int s = e % Cl.ns, I = e / Cl.ns;
-      double sum = 0.0;
+      T sum = 0;
       for (int da = 0; da <= ra; da++) {
         const int a = ra ? 2 * I + da : I;
         if (a >= L.na || !L.act[cidx(L, m, a, s)]) continue;
@@ -225,12 +233,12 @@ __global__ __launch_bounds__(1024) void k_tail(MGTail t, const double *__restric
   // coarsest: one line, solved exactly
   {
     const int l = t.nl - 1;
-    if (wv == 0) line_solve<K>(t.lev[l], lmv, t.x[l], t.b[l], m, 0, 1, pin_s, pin);
+    if (wv == 0) line_solve<K, T>(t.lev[l], lmv, t.x[l], t.b[l], m, 0, 1, pin_s, pin);
     __syncthreads();
   }
   // up: prolong, post-smooth (colour 1, then colour 0)
   for (int l = t.nl - 2; l >= 0; l--) {
-    const MGLev &L = t.lev[l], &Cl = t.lev[l + 1];
+    const MGLevT<T> &L = t.lev[l], &Cl = t.lev[l + 1];
     const int ra = L.na != Cl.na;
     for (int e = threadIdx.x; e < L.na * L.ns; e += blockDim.x) {
       const int s = e % L.ns, a = e / L.ns;
@@ -239,16 +247,17 @@ __global__ __launch_bounds__(1024) void k_tail(MGTail t, const double *__restric
     }
     __syncthreads();
     for (int c = 1; c >= 0; c--) {
-      for (int a = 2 * wv + c; a < L.na; a += 2 * nw) line_solve<K>(L, lmv, t.x[l], t.b[l], m, a, 0, -1, false);
+      for (int a = 2 * wv + c; a < L.na; a += 2 * nw) line_solve<K, T>(L, lmv, t.x[l], t.b[l], m, a, 0, -1, false);
       __syncthreads();
     }
   }
 }
 
 // coarse b = sum over the active fine members of (b - M x)
-__global__ __launch_bounds__(256) void k_restrict(MGLev F, MGLev Cl, const double *__restrict__ lamf,
-                                                  const double *__restrict__ x, const double *__restrict__ b,
-                                                  double *__restrict__ bc, const int *__restrict__ done) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_restrict(MGLevT<T> F, MGLevT<T> Cl, const T *__restrict__ lamf,
+                                                  const T *__restrict__ x, const T *__restrict__ b,
+                                                  T *__restrict__ bc, const int *__restrict__ done) {
   const long idx = (long)blockIdx.x * 256 + threadIdx.x;
   const long n = (long)Cl.nm * Cl.na * Cl.ns;
   if (idx >= n) return;
@@ -257,11 +266,11 @@ __global__ __launch_bounds__(256) void k_restrict(MGLev F, MGLev Cl, const doubl
   const int I = (int)(t % Cl.na), M = (int)(t / Cl.na);
   if (done && done[group_of(Cl, M)]) return;
   const int ra = F.na != Cl.na, rm = F.nm != Cl.nm;
-  double sum = 0.0;
+  T sum = 0;
   for (int dm = 0; dm <= rm; dm++) {
     const int m = rm ? 2 * M + dm : M;
     if (m >= F.nm) break;
-    const double lm = lamf ? lamf[m] : 0.0;
+    const T lm = lamf ? lamf[m] : T(0);
     for (int da = 0; da <= ra; da++) {
       const int a = ra ? 2 * I + da : I;
       if (a >= F.na) break;
@@ -272,8 +281,9 @@ __global__ __launch_bounds__(256) void k_restrict(MGLev F, MGLev Cl, const doubl
   bc[idx] = sum;
 }
 
-__global__ __launch_bounds__(256) void k_prolong(MGLev F, MGLev Cl, const double *__restrict__ xc,
-                                                 double *__restrict__ xf, const int *__restrict__ done) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_prolong(MGLevT<T> F, MGLevT<T> Cl, const T *__restrict__ xc,
+                                                 T *__restrict__ xf, const int *__restrict__ done) {
   const long idx = (long)blockIdx.x * 256 + threadIdx.x;
   const long n = (long)F.nm * F.na * F.ns;
   if (idx >= n) return;
@@ -298,9 +308,15 @@ __global__ __launch_bounds__(256) void k_apply(MGLev L, const double *__restrict
   out[idx] = mx_at(L, lam ? lam[m] : 0.0, x, m, a, s);
 }
 
-__global__ __launch_bounds__(256) void k_zero(long n, double *__restrict__ x) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_zero(long n, T *__restrict__ x) {
   const long idx = (long)blockIdx.x * 256 + threadIdx.x;
-  if (idx < n) x[idx] = 0.0;
+  if (idx < n) x[idx] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_to_float(long n, const double *__restrict__ src, float *__restrict__ dst) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx < n) dst[idx] = (float)src[idx];
 }
 
 __global__ __launch_bounds__(256) void k_izero(long n, int *__restrict__ x) {
@@ -315,8 +331,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // blockIdx.y = group, blockIdx.x = chunk of the group's elements
-__global__ __launch_bounds__(256) void k_dot(long gsize, const double *__restrict__ x, const double *__restrict__ y,
-                                             const double *__restrict__ x2, const double *__restrict__ y2,
+template <typename TY, typename TX2>
+__global__ __launch_bounds__(256) void k_dot(long gsize, const double *__restrict__ x, const TY *__restrict__ y,
+                                             const TX2 *__restrict__ x2, const double *__restrict__ y2,
                                              double *__restrict__ part, int nchunk, const int *__restrict__ done,
                                              const int *__restrict__ only) {
   __shared__ double sh[2][4];
@@ -326,8 +343,8 @@ __global__ __launch_bounds__(256) void k_dot(long gsize, const double *__restric
   if (!skip) {
     const long base = (long)g * gsize;
     for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < gsize; i += (long)nchunk * 256) {
-      if (x) s1 += x[base + i] * y[base + i];
-      if (x2) s2 += y2 ? x2[base + i] * y2[base + i] : x2[base + i];
+      if (x) s1 += x[base + i] * (double)y[base + i];
+      if (x2) s2 += y2 ? (double)x2[base + i] * y2[base + i] : (double)x2[base + i];
     }
   }
   s1 = wave_sum(s1);
@@ -358,7 +375,8 @@ __global__ __launch_bounds__(64) void k_dot_final(const double *__restrict__ par
 __global__ __launch_bounds__(256) void k_cg_update(int ng, long gsize, const double *__restrict__ rz,
                                                    const double *__restrict__ pq, double *__restrict__ x,
                                                    const double *__restrict__ p, double *__restrict__ r,
-                                                   const double *__restrict__ q, const int *__restrict__ done) {
+                                                   const double *__restrict__ q, float *__restrict__ rf,
+                                                   const int *__restrict__ done) {
   const long idx = (long)blockIdx.x * 256 + threadIdx.x;
   if (idx >= (long)ng * gsize) return;
   const int g = (int)(idx / gsize);
@@ -366,12 +384,16 @@ __global__ __launch_bounds__(256) void k_cg_update(int ng, long gsize, const dou
   const double den = pq[g * 2];
   const double al = den != 0.0 ? rz[g * 2] / den : 0.0;
   x[idx] += al * p[idx];
-  r[idx] -= al * q[idx];
+  const double rn = r[idx] - al * q[idx];
+  r[idx] = rn;
+  if (rf) rf[idx] = (float)rn;
 }
 
+template <typename T>
 __global__ __launch_bounds__(256) void k_cg_dir(int ng, long gsize, const double *__restrict__ rzn,
-                                                const double *__restrict__ rzo, const double *__restrict__ z,
-                                                double *__restrict__ p, const int *__restrict__ done) {
+                                                const double *__restrict__ rzo, const double *__restrict__ cross,
+                                                const T *__restrict__ z, double *__restrict__ p,
+                                                const int *__restrict__ done) {
   const long idx = (long)blockIdx.x * 256 + threadIdx.x;
   if (idx >= (long)ng * gsize) return;
   const int g = (int)(idx / gsize);
@@ -379,12 +401,14 @@ __global__ __launch_bounds__(256) void k_cg_dir(int ng, long gsize, const double
   double be = 0.0;
   if (rzo) {
     const double den = rzo[g * 2];
-    be = den != 0.0 ? rzn[g * 2] / den : 0.0;
+    const double num = rzn[g * 2] - (cross ? cross[g * 2 + 1] : 0.0);
+    be = den != 0.0 ? num / den : 0.0;
   }
-  p[idx] = z[idx] + be * p[idx];
+  p[idx] = (double)z[idx] + be * p[idx];
 }
 
-__global__ __launch_bounds__(256) void k_sub_mean(MGLev L, double *__restrict__ x, const double *__restrict__ sums,
+template <typename T>
+__global__ __launch_bounds__(256) void k_sub_mean(MGLev L, T *__restrict__ x, const double *__restrict__ sums,
                                                   const int *__restrict__ singular, const double *__restrict__ nact,
                                                   const int *__restrict__ done) {
   const long idx = (long)blockIdx.x * 256 + threadIdx.x;
@@ -396,7 +420,7 @@ __global__ __launch_bounds__(256) void k_sub_mean(MGLev L, double *__restrict__ 
   const int g = group_of(L, m);
   if (!singular[g] || (done && done[g])) return;
   if (!L.act[cidx(L, m, a, s)]) return;
-  x[idx] -= sums[g * 2 + 1] / nact[g];
+  x[idx] -= (T)(sums[g * 2 + 1] / nact[g]);
 }
 
 __global__ __launch_bounds__(256) void k_flags(int ng, const double *__restrict__ rr, const double *__restrict__ bb,
@@ -466,8 +490,9 @@ unsigned blocks_for(long n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
 
-void mg_line(hipStream_t st, const MGLev &L, const double *lam, double *x, const double *b, int col_a, int col_m,
-             int zero_nb, int pin_s, const int *singular, const int *done) {
+template <typename T>
+void mg_line(hipStream_t st, const MGLevT<T> &L, const T *lam, T *x, const T *b, int col_a, int col_m, int zero_nb,
+             int pin_s, const int *singular, const int *done) {
   const int nla = (L.na - col_a + 1) / 2;
   const int nlm = L.coupled ? (L.nm - col_m + 1) / 2 : L.nm;
   if (nla <= 0 || nlm <= 0) return;
@@ -475,43 +500,46 @@ void mg_line(hipStream_t st, const MGLev &L, const double *lam, double *x, const
   dim3 grid((unsigned)((lines + 3) / 4));
   const int rows = L.base + (L.extra ? 1 : 0);
   if (rows <= 3)
-    hipLaunchKernelGGL(k_line<3>, grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb, pin_s,
+    hipLaunchKernelGGL((k_line<3, T>), grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb, pin_s,
                        singular, done);
   else if (rows <= 5)
-    hipLaunchKernelGGL(k_line<5>, grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb, pin_s,
+    hipLaunchKernelGGL((k_line<5, T>), grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb, pin_s,
                        singular, done);
   else if (rows <= 9)
-    hipLaunchKernelGGL(k_line<9>, grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb, pin_s,
+    hipLaunchKernelGGL((k_line<9, T>), grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb, pin_s,
                        singular, done);
   else if (rows <= 17)
-    hipLaunchKernelGGL(k_line<17>, grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb, pin_s,
+    hipLaunchKernelGGL((k_line<17, T>), grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb, pin_s,
                        singular, done);
   else
     throw std::runtime_error("multigrid line relaxation: lines longer than 1088 cells are not supported");
   SNAC_HIP(hipGetLastError());
 }
 
-void mg_tail(hipStream_t st, const MGTail &t, int nm, const double *lam, int pin_s, const int *singular,
+template <typename T>
+void mg_tail(hipStream_t st, const MGTailT<T> &t, int nm, const T *lam, int pin_s, const int *singular,
              const int *done) {
-  const MGLev &L = t.lev[0];
+  const MGLevT<T> &L = t.lev[0];
   const int rows = L.base + (L.extra ? 1 : 0);
-  if (rows <= 3) hipLaunchKernelGGL(k_tail<3>, dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
-  else if (rows <= 5) hipLaunchKernelGGL(k_tail<5>, dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
-  else if (rows <= 9) hipLaunchKernelGGL(k_tail<9>, dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
-  else hipLaunchKernelGGL(k_tail<17>, dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
+  if (rows <= 3) hipLaunchKernelGGL((k_tail<3, T>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
+  else if (rows <= 5) hipLaunchKernelGGL((k_tail<5, T>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
+  else if (rows <= 9) hipLaunchKernelGGL((k_tail<9, T>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
+  else hipLaunchKernelGGL((k_tail<17, T>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
   SNAC_HIP(hipGetLastError());
 }
 
-void mg_restrict_residual(hipStream_t st, const MGLev &F, const MGLev &C, const double *lam_f, const double *,
-                          const double *x, const double *b, double *bc, const int *done) {
+template <typename T>
+void mg_restrict_residual(hipStream_t st, const MGLevT<T> &F, const MGLevT<T> &C, const T *lam_f, const T *x,
+                          const T *b, T *bc, const int *done) {
   const long n = (long)C.nm * C.na * C.ns;
-  hipLaunchKernelGGL(k_restrict, dim3(blocks_for(n)), dim3(256), 0, st, F, C, lam_f, x, b, bc, done);
+  hipLaunchKernelGGL(k_restrict<T>, dim3(blocks_for(n)), dim3(256), 0, st, F, C, lam_f, x, b, bc, done);
   SNAC_HIP(hipGetLastError());
 }
 
-void mg_prolong(hipStream_t st, const MGLev &F, const MGLev &C, const double *xc, double *xf, const int *done) {
+template <typename T>
+void mg_prolong(hipStream_t st, const MGLevT<T> &F, const MGLevT<T> &C, const T *xc, T *xf, const int *done) {
   const long n = (long)F.nm * F.na * F.ns;
-  hipLaunchKernelGGL(k_prolong, dim3(blocks_for(n)), dim3(256), 0, st, F, C, xc, xf, done);
+  hipLaunchKernelGGL(k_prolong<T>, dim3(blocks_for(n)), dim3(256), 0, st, F, C, xc, xf, done);
   SNAC_HIP(hipGetLastError());
 }
 
@@ -521,8 +549,14 @@ void mg_apply(hipStream_t st, const MGLev &L, const double *lam, const double *x
   SNAC_HIP(hipGetLastError());
 }
 
-void mg_zero(hipStream_t st, long n, double *x) {
-  hipLaunchKernelGGL(k_zero, dim3(blocks_for(n)), dim3(256), 0, st, n, x);
+template <typename T>
+void mg_zero(hipStream_t st, long n, T *x) {
+  hipLaunchKernelGGL(k_zero<T>, dim3(blocks_for(n)), dim3(256), 0, st, n, x);
+  SNAC_HIP(hipGetLastError());
+}
+
+void mg_to_float(hipStream_t st, long n, const double *src, float *dst) {
+  hipLaunchKernelGGL(k_to_float, dim3(blocks_for(n)), dim3(256), 0, st, n, src, dst);
   SNAC_HIP(hipGetLastError());
 }
 
@@ -531,31 +565,35 @@ void mg_izero(hipStream_t st, long n, int *x) {
   SNAC_HIP(hipGetLastError());
 }
 
-void mg_dot(hipStream_t st, int ng, long gsize, const double *x, const double *y, const double *x2, const double *y2,
+template <typename TY, typename TX2>
+void mg_dot(hipStream_t st, int ng, long gsize, const double *x, const TY *y, const TX2 *x2, const double *y2,
             double *part, int nchunk, double *out, const int *done, const int *only) {
-  hipLaunchKernelGGL(k_dot, dim3(nchunk, ng), dim3(256), 0, st, gsize, x, y, x2, y2, part, nchunk, done, only);
+  hipLaunchKernelGGL((k_dot<TY, TX2>), dim3(nchunk, ng), dim3(256), 0, st, gsize, x, y, x2, y2, part, nchunk, done, only);
   SNAC_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_dot_final, dim3(ng), dim3(64), 0, st, part, nchunk, out, done, only);
   SNAC_HIP(hipGetLastError());
 }
 
 void mg_cg_update(hipStream_t st, int ng, long gsize, const double *rz, const double *pq, double *x, const double *p,
-                  double *r, const double *q, const int *done) {
+                  double *r, const double *q, float *rf, const int *done) {
   hipLaunchKernelGGL(k_cg_update, dim3(blocks_for((long)ng * gsize)), dim3(256), 0, st, ng, gsize, rz, pq, x, p, r, q,
-                     done);
+                     rf, done);
   SNAC_HIP(hipGetLastError());
 }
 
-void mg_cg_dir(hipStream_t st, int ng, long gsize, const double *rzn, const double *rzo, const double *z, double *p,
-               const int *done) {
-  hipLaunchKernelGGL(k_cg_dir, dim3(blocks_for((long)ng * gsize)), dim3(256), 0, st, ng, gsize, rzn, rzo, z, p, done);
+template <typename T>
+void mg_cg_dir(hipStream_t st, int ng, long gsize, const double *rzn, const double *rzo, const double *cross,
+               const T *z, double *p, const int *done) {
+  hipLaunchKernelGGL(k_cg_dir<T>, dim3(blocks_for((long)ng * gsize)), dim3(256), 0, st, ng, gsize, rzn, rzo, cross, z,
+                     p, done);
   SNAC_HIP(hipGetLastError());
 }
 
-void mg_sub_mean(hipStream_t st, const MGLev &L, double *x, const double *sums, const int *singular,
-                 const double *nact, const int *done) {
+template <typename T>
+void mg_sub_mean(hipStream_t st, const MGLev &L, T *x, const double *sums, const int *singular, const double *nact,
+                 const int *done) {
   const long n = (long)L.nm * L.na * L.ns;
-  hipLaunchKernelGGL(k_sub_mean, dim3(blocks_for(n)), dim3(256), 0, st, L, x, sums, singular, nact, done);
+  hipLaunchKernelGGL(k_sub_mean<T>, dim3(blocks_for(n)), dim3(256), 0, st, L, x, sums, singular, nact, done);
   SNAC_HIP(hipGetLastError());
 }
 
@@ -584,5 +622,25 @@ void mg_copy3(hipStream_t st, const double *src, long sm, long sa, long ss, doub
   }
   SNAC_HIP(hipGetLastError());
 }
+
+#define SNAC_MG_INST(T)                                                                                               \
+  template void mg_line<T>(hipStream_t, const MGLevT<T> &, const T *, T *, const T *, int, int, int, int, const int *, \
+                           const int *);                                                                              \
+  template void mg_tail<T>(hipStream_t, const MGTailT<T> &, int, const T *, int, const int *, const int *);           \
+  template void mg_restrict_residual<T>(hipStream_t, const MGLevT<T> &, const MGLevT<T> &, const T *, const T *,       \
+                                        const T *, T *, const int *);                                                 \
+  template void mg_prolong<T>(hipStream_t, const MGLevT<T> &, const MGLevT<T> &, const T *, T *, const int *);        \
+  template void mg_zero<T>(hipStream_t, long, T *);                                                                   \
+  template void mg_dot<T, T>(hipStream_t, int, long, const double *, const T *, const T *, const double *, double *,  \
+                             int, double *, const int *, const int *);                                                \
+  template void mg_cg_dir<T>(hipStream_t, int, long, const double *, const double *, const double *, const T *,       \
+                             double *, const int *);                                                                  \
+  template void mg_sub_mean<T>(hipStream_t, const MGLev &, T *, const double *, const int *, const double *,          \
+                               const int *);
+SNAC_MG_INST(double)
+SNAC_MG_INST(float)
+#undef SNAC_MG_INST
+template void mg_dot<double, float>(hipStream_t, int, long, const double *, const double *, const float *,
+                                    const double *, double *, int, double *, const int *, const int *);
 
 }  // namespace snac

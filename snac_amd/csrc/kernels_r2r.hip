@@ -38,8 +38,6 @@ struct Args {
   const double *tab;
 };
 
-__device__ __forceinline__ int brev(int x, int logn) { return (int)(__brev((unsigned)x) >> (32 - logn)); }
-
 // lines per workgroup for an N1 x N2 FFT: one task per thread in each pass
 // and at most LDS_COMPLEX complex values of LDS
 constexpr int fft_lines(int n1, int n2) {

@@ -28,58 +28,88 @@
 
 namespace snac {
 
-// one level, passed by value to the kernels
-struct MGLev {
+// one level, passed by value to the kernels; T is the element type of the
+// level's vectors and coefficients (double, or float for the mixed-precision
+// V-cycle)
+template <typename T>
+struct MGLevT {
   int nm, na, ns;
   int coupled;          // couplings along m (3D path)
   int per_a, per_m;     // periodic links along a / m
   long qm;              // coefficient stride of m: na*ns when coupled, else 0 (mode-independent)
-  const double *dg, *w, *cs, *ca, *cm;
+  const T *dg, *w, *cs, *ca, *cm;
   const unsigned char *act;
   int nch, base, extra; // a line of ns rows over nch lanes: base (+1 for the first `extra`) rows each
 };
+using MGLev = MGLevT<double>;
 
 // the coarse end of the hierarchy handled by one fused kernel (FFT path)
 constexpr int MG_TAIL_MAX = 12;
-struct MGTail {
+template <typename T>
+struct MGTailT {
   int nl;                   // levels in the tail; the last one has na == 1
-  MGLev lev[MG_TAIL_MAX];
-  double *x[MG_TAIL_MAX], *b[MG_TAIL_MAX];
+  MGLevT<T> lev[MG_TAIL_MAX];
+  T *x[MG_TAIL_MAX], *b[MG_TAIL_MAX];
 };
 
-// ---- kernels (kernels_mg.hip) ----
+// the V-cycle hierarchy in one element type: levels, the per-mode
+// eigenvalues (null on the 3D path) and per-level x / b (level 0: z and r of
+// the CG, or their FP32 shadows in mixed precision)
+template <typename T>
+struct MGHier {
+  std::vector<MGLevT<T>> lev;
+  const T *lam = nullptr;
+  std::vector<T *> xl, bl;
+  int tail0 = -1;           // first level handled by the fused tail kernel (-1: none)
+  MGTailT<T> tail;
+};
+
+// ---- kernels (kernels_mg.hip), instantiated for double and float ----
 // V-cycle over the tail levels, one workgroup per mode (b of the first
 // tail level is given; x of every tail level is overwritten)
-void mg_tail(hipStream_t st, const MGTail &t, int nm, const double *lam, int pin_s, const int *singular,
+template <typename T>
+void mg_tail(hipStream_t st, const MGTailT<T> &t, int nm, const T *lam, int pin_s, const int *singular,
              const int *done);
 // zebra line relaxation of the lines (m, a) with a % 2 == col_a (and, when
 // coupled, m % 2 == col_m); zero_nb: neighbouring lines read as zero;
 // pin_s >= 0: in singular groups row pin_s is replaced by x = 0
-void mg_line(hipStream_t st, const MGLev &L, const double *lam, double *x, const double *b, int col_a, int col_m,
-             int zero_nb, int pin_s, const int *singular, const int *done);
+template <typename T>
+void mg_line(hipStream_t st, const MGLevT<T> &L, const T *lam, T *x, const T *b, int col_a, int col_m, int zero_nb,
+             int pin_s, const int *singular, const int *done);
 // coarse b = restriction of the fine residual b - M x (sum over the active
 // fine cells of each coarse cell)
-void mg_restrict_residual(hipStream_t st, const MGLev &F, const MGLev &C, const double *lam_f, const double *lam_c,
-                          const double *x, const double *b, double *bc, const int *done);
+template <typename T>
+void mg_restrict_residual(hipStream_t st, const MGLevT<T> &F, const MGLevT<T> &C, const T *lam_f, const T *x,
+                          const T *b, T *bc, const int *done);
 // fine x += coarse x on the active fine cells
-void mg_prolong(hipStream_t st, const MGLev &F, const MGLev &C, const double *xc, double *xf, const int *done);
+template <typename T>
+void mg_prolong(hipStream_t st, const MGLevT<T> &F, const MGLevT<T> &C, const T *xc, T *xf, const int *done);
+template <typename T>
+void mg_zero(hipStream_t st, long n, T *x);
 void mg_apply(hipStream_t st, const MGLev &L, const double *lam, const double *x, double *out, const int *done);
-void mg_zero(hipStream_t st, long n, double *x);
 void mg_izero(hipStream_t st, long n, int *x);
+// dst = (float) src
+void mg_to_float(hipStream_t st, long n, const double *src, float *dst);
 // per-group dot products out[g*2] = x.y, out[g*2+1] = x2.y2 (or sum of x2
-// when y2 is null); groups are contiguous runs of gsize elements; groups
-// with done[g] or !only[g] (when given) are skipped
-void mg_dot(hipStream_t st, int ng, long gsize, const double *x, const double *y, const double *x2, const double *y2,
+// when y2 is null), accumulated in FP64; groups are contiguous runs of gsize
+// elements; groups with done[g] or !only[g] (when given) are skipped
+template <typename TY, typename TX2 = TY>
+void mg_dot(hipStream_t st, int ng, long gsize, const double *x, const TY *y, const TX2 *x2, const double *y2,
             double *part, int nchunk, double *out, const int *done, const int *only);
-// x += alpha p ; r -= alpha q  with alpha = rz/pq per group
+// x += alpha p ; r -= alpha q  with alpha = rz/pq per group; rf (when
+// given) receives the new r rounded to FP32
 void mg_cg_update(hipStream_t st, int ng, long gsize, const double *rz, const double *pq, double *x, const double *p,
-                  double *r, const double *q, const int *done);
-// p = z + beta p with beta = rzn/rzo (beta = 0 when rzo is null)
-void mg_cg_dir(hipStream_t st, int ng, long gsize, const double *rzn, const double *rzo, const double *z, double *p,
-               const int *done);
+                  double *r, const double *q, float *rf, const int *done);
+// p = z + beta p with beta = (rzn - cross) / rzo, cross = r_new . z_old in
+// slot 1 of `cross` (flexible CG) or 0 when cross is null; beta = 0 when rzo
+// is null
+template <typename T>
+void mg_cg_dir(hipStream_t st, int ng, long gsize, const double *rzn, const double *rzo, const double *cross,
+               const T *z, double *p, const int *done);
 // x -= (sum / nact) on the active cells of the singular groups
-void mg_sub_mean(hipStream_t st, const MGLev &L, double *x, const double *sums, const int *singular,
-                 const double *nact, const int *done);
+template <typename T>
+void mg_sub_mean(hipStream_t st, const MGLev &L, T *x, const double *sums, const int *singular, const double *nact,
+                 const int *done);
 // done[g] |= rr[g] <= tol2 * bb[g]; *left = number of groups not done
 void mg_flags(hipStream_t st, int ng, const double *rr, const double *bb, double tol2, int *done, int *left);
 // *dst = max(*dst, *src) on the device
@@ -99,10 +129,15 @@ struct MGPlan {
   int gmin[3] = {0, 0, 0}, G[3] = {1, 1, 1};
   int ng = 0;                  // CG groups: modes (FFT path) or 1
   long gsize = 0;
-  std::vector<MGLev> lev;
-  std::vector<double *> lam;   // per level (null on the 3D path)
-  std::vector<double *> xl, bl;  // per level (level 0: z and r of the CG)
+  MGHier<double> h64;          // FP64 V-cycle
+  MGHier<float> h32;           // FP32 V-cycle of the mixed-precision mode
+  // 0: FP64 throughout (PCG); 1: mixed -- the V-cycle (smoothing,
+  // restriction, prolongation, coarse solves) in FP32, the CG vectors, the
+  // operator, the dot products and the null-space removal of x, r in FP64,
+  // flexible CG (Polak-Ribiere beta) for the FP32 preconditioner
+  int prec = 0;
   double *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  float *zf = nullptr, *rf = nullptr;
   double *part = nullptr, *bb = nullptr, *rr = nullptr, *pq = nullptr, *rz[2] = {nullptr, nullptr}, *sums = nullptr;
   double *nact = nullptr;
   int *done = nullptr, *singular = nullptr, *left = nullptr, *left_host = nullptr;
@@ -113,8 +148,6 @@ struct MGPlan {
   int maxiter = 50;
   int colors = 2;              // zebra colours per sweep (2, or 4 when coupled)
   int nlev = 0;
-  int tail0 = -1;              // first level handled by the fused tail kernel (-1: none)
-  MGTail tail;
   int last_iters = 0;
   // fixed-iteration mode (graph capture): CG iteration count, the largest
   // count an eagerly checked solve needed, and a device counter of groups

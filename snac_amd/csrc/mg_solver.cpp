@@ -361,13 +361,15 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
   P.nlev = (int)H.size();
   if (device < 0) { P.ok = true; return; }
 
-  // device arrays
-  P.lev.clear();
-  P.lam.clear();
-  P.xl.clear();
-  P.bl.clear();
+  // device arrays: the FP64 hierarchy and its FP32 twin (mixed precision)
+  P.h64 = MGHier<double>();
+  P.h32 = MGHier<float>();
   const int nch = ns >= 2 ? std::min(64, ns / 2) : 1;
-  double *lam_dev = coupled ? nullptr : dev_upload(P, lamh);
+  auto f32 = [](const std::vector<double> &v) { return std::vector<float>(v.begin(), v.end()); };
+  if (!coupled) {
+    P.h64.lam = dev_upload(P, lamh);
+    P.h32.lam = dev_upload(P, f32(lamh));
+  }
   for (size_t l = 0; l < H.size(); l++) {
     const HostLev &h = H[l];
     MGLev L;
@@ -384,8 +386,18 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
     L.nch = nch;
     L.base = ns >= 2 ? ns / nch : 1;
     L.extra = ns >= 2 ? ns % nch : 0;
-    P.lev.push_back(L);
-    P.lam.push_back(lam_dev);
+    P.h64.lev.push_back(L);
+    MGLevT<float> Lf;
+    Lf.nm = L.nm; Lf.na = L.na; Lf.ns = L.ns;
+    Lf.coupled = L.coupled; Lf.per_a = L.per_a; Lf.per_m = L.per_m; Lf.qm = L.qm;
+    Lf.dg = dev_upload(P, f32(h.dg));
+    Lf.w = dev_upload(P, f32(h.w));
+    Lf.cs = dev_upload(P, f32(h.cs));
+    Lf.ca = dev_upload(P, f32(h.ca));
+    Lf.cm = coupled ? dev_upload(P, f32(h.cm)) : nullptr;
+    Lf.act = L.act;
+    Lf.nch = L.nch; Lf.base = L.base; Lf.extra = L.extra;
+    P.h32.lev.push_back(Lf);
     const size_t n = (size_t)h.nm * h.na * h.ns;
     if (l == 0) {
       P.x = dev_zeros<double>(P, n);
@@ -393,29 +405,40 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
       P.z = dev_zeros<double>(P, n);
       P.p = dev_zeros<double>(P, n);
       P.q = dev_zeros<double>(P, n);
-      P.xl.push_back(P.z);
-      P.bl.push_back(P.r);
+      P.zf = dev_zeros<float>(P, n);
+      P.rf = dev_zeros<float>(P, n);
+      P.h64.xl.push_back(P.z);
+      P.h64.bl.push_back(P.r);
+      P.h32.xl.push_back(P.zf);
+      P.h32.bl.push_back(P.rf);
     } else {
-      P.xl.push_back(dev_zeros<double>(P, n));
-      P.bl.push_back(dev_zeros<double>(P, n));
+      P.h64.xl.push_back(dev_zeros<double>(P, n));
+      P.h64.bl.push_back(dev_zeros<double>(P, n));
+      P.h32.xl.push_back(dev_zeros<float>(P, n));
+      P.h32.bl.push_back(dev_zeros<float>(P, n));
     }
   }
   // fused tail: the levels with at most 32 columns (FFT path, modes independent)
-  P.tail0 = -1;
+  int tail0 = -1;
+  const int nl = (int)H.size();
   if (!coupled) {
-    for (size_t l = 0; l < P.lev.size(); l++)
-      if (P.lev[l].na <= 32) { P.tail0 = (int)l; break; }
-    if (P.tail0 >= 0 && (int)P.lev.size() - P.tail0 > MG_TAIL_MAX) P.tail0 = (int)P.lev.size() - MG_TAIL_MAX;
-    if (P.tail0 >= 0 && (int)P.lev.size() - P.tail0 < 2) P.tail0 = -1;  // nothing to fuse
-    if (P.tail0 >= 0) {
-      P.tail.nl = (int)P.lev.size() - P.tail0;
-      for (int q = 0; q < P.tail.nl; q++) {
-        P.tail.lev[q] = P.lev[P.tail0 + q];
-        P.tail.x[q] = P.xl[P.tail0 + q];
-        P.tail.b[q] = P.bl[P.tail0 + q];
-      }
-    }
+    for (int l = 0; l < nl; l++)
+      if (H[l].na <= 32) { tail0 = l; break; }
+    if (tail0 >= 0 && nl - tail0 > MG_TAIL_MAX) tail0 = nl - MG_TAIL_MAX;
+    if (tail0 >= 0 && nl - tail0 < 2) tail0 = -1;  // nothing to fuse
   }
+  auto set_tail = [&](auto &Hh) {
+    Hh.tail0 = tail0;
+    if (tail0 < 0) return;
+    Hh.tail.nl = nl - tail0;
+    for (int q = 0; q < Hh.tail.nl; q++) {
+      Hh.tail.lev[q] = Hh.lev[tail0 + q];
+      Hh.tail.x[q] = Hh.xl[tail0 + q];
+      Hh.tail.b[q] = Hh.bl[tail0 + q];
+    }
+  };
+  set_tail(P.h64);
+  set_tail(P.h32);
   P.nchunk = (int)std::max<long>(1, std::min<long>((P.gsize + 2047) / 2048, std::max(1, 8192 / P.ng)));
   P.part = dev_zeros<double>(P, (size_t)2 * P.ng * P.nchunk);
   P.bb = dev_zeros<double>(P, 2 * P.ng);
@@ -435,16 +458,17 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
 
 namespace {
 
-void vcycle(MGPlan &P, hipStream_t st, size_t l) {
-  const MGLev &L = P.lev[l];
-  double *x = P.xl[l];
-  const double *b = P.bl[l];
-  if ((int)l == P.tail0) {  // the coarse end in one launch, one workgroup per mode
-    mg_tail(st, P.tail, P.ng, P.lam[l], P.pin_s, P.singular, P.done);
+template <typename T>
+void vcycle(MGPlan &P, MGHier<T> &H, hipStream_t st, size_t l) {
+  const MGLevT<T> &L = H.lev[l];
+  T *x = H.xl[l];
+  const T *b = H.bl[l];
+  if ((int)l == H.tail0) {  // the coarse end in one launch, one workgroup per mode
+    mg_tail(st, H.tail, P.ng, H.lam, P.pin_s, P.singular, P.done);
     return;
   }
-  if (l + 1 == P.lev.size()) {  // one line per group: direct solve
-    mg_line(st, L, P.lam[l], x, b, 0, 0, 1, P.pin_s, P.singular, P.done);
+  if (l + 1 == H.lev.size()) {  // one line per group: direct solve
+    mg_line(st, L, H.lam, x, b, 0, 0, 1, P.pin_s, P.singular, P.done);
     return;
   }
   int col[4][2], nc = 0;
@@ -454,18 +478,35 @@ void vcycle(MGPlan &P, hipStream_t st, size_t l) {
   const bool zero_first = nc > 2;
   if (zero_first) mg_zero(st, (long)L.nm * L.na * L.ns, x);
   for (int k = 0; k < nc; k++)
-    mg_line(st, L, P.lam[l], x, b, col[k][0], col[k][1], (!zero_first && k == 0) ? 1 : 0, -1, nullptr, P.done);
-  mg_restrict_residual(st, L, P.lev[l + 1], P.lam[l], P.lam[l + 1], x, b, P.bl[l + 1], P.done);
-  vcycle(P, st, l + 1);
-  mg_prolong(st, L, P.lev[l + 1], P.xl[l + 1], x, P.done);
+    mg_line(st, L, H.lam, x, b, col[k][0], col[k][1], (!zero_first && k == 0) ? 1 : 0, -1, nullptr, P.done);
+  mg_restrict_residual(st, L, H.lev[l + 1], H.lam, x, b, H.bl[l + 1], P.done);
+  vcycle(P, H, st, l + 1);
+  mg_prolong(st, L, H.lev[l + 1], (const T *)H.xl[l + 1], x, P.done);
   for (int k = nc - 1; k >= 0; k--)
-    mg_line(st, L, P.lam[l], x, b, col[k][0], col[k][1], 0, -1, nullptr, P.done);
+    mg_line(st, L, H.lam, x, b, col[k][0], col[k][1], 0, -1, nullptr, P.done);
 }
 
-void project(MGPlan &P, hipStream_t st, double *v) {
+template <typename T>
+void project(MGPlan &P, hipStream_t st, T *v) {
   if (!P.any_singular) return;
-  mg_dot(st, P.ng, P.gsize, nullptr, nullptr, v, nullptr, P.part, P.nchunk, P.sums, P.done, P.singular);
-  mg_sub_mean(st, P.lev[0], v, P.sums, P.singular, P.nact, P.done);
+  mg_dot<T, T>(st, P.ng, P.gsize, nullptr, nullptr, v, nullptr, P.part, P.nchunk, P.sums, P.done, P.singular);
+  mg_sub_mean(st, P.h64.lev[0], v, P.sums, P.singular, P.nact, P.done);
+}
+
+// z = V-cycle(r) with its null-space component removed; rz = r.z; p = z +
+// beta p (first: p = z)
+void precondition(MGPlan &P, hipStream_t st, double *rz_new, const double *rz_old) {
+  if (P.prec == 1) {
+    vcycle(P, P.h32, st, 0);
+    project(P, st, P.zf);
+    mg_dot<float, float>(st, P.ng, P.gsize, P.r, P.zf, nullptr, nullptr, P.part, P.nchunk, rz_new, P.done, nullptr);
+    mg_cg_dir<float>(st, P.ng, P.gsize, rz_new, rz_old, rz_old ? P.rr : nullptr, P.zf, P.p, P.done);
+  } else {
+    vcycle(P, P.h64, st, 0);
+    project(P, st, P.z);
+    mg_dot<double, double>(st, P.ng, P.gsize, P.r, P.z, nullptr, nullptr, P.part, P.nchunk, rz_new, P.done, nullptr);
+    mg_cg_dir<double>(st, P.ng, P.gsize, rz_new, rz_old, nullptr, P.z, P.p, P.done);
+  }
 }
 
 }  // namespace
@@ -473,23 +514,26 @@ void project(MGPlan &P, hipStream_t st, double *v) {
 void mg_solve(MGPlan &P, hipStream_t st, int fixed) {
   const long n = (long)P.ng * P.gsize;
   const double tol2 = P.tol * P.tol;
+  const bool mixed = P.prec == 1;
   mg_zero(st, n, P.x);
   mg_izero(st, P.ng, P.done);
   project(P, st, P.r);
-  mg_dot(st, P.ng, P.gsize, P.r, P.r, nullptr, nullptr, P.part, P.nchunk, P.bb, P.done, nullptr);
+  mg_dot<double, double>(st, P.ng, P.gsize, P.r, P.r, nullptr, nullptr, P.part, P.nchunk, P.bb, P.done, nullptr);
   mg_flags(st, P.ng, P.bb, P.bb, tol2, P.done, P.left);
-  vcycle(P, st, 0);
-  project(P, st, P.z);
+  if (mixed) mg_to_float(st, n, P.r, P.rf);
   int cur = 0;
-  mg_dot(st, P.ng, P.gsize, P.r, P.z, nullptr, nullptr, P.part, P.nchunk, P.rz[cur], P.done, nullptr);
-  mg_cg_dir(st, P.ng, P.gsize, nullptr, nullptr, P.z, P.p, P.done);
+  precondition(P, st, P.rz[cur], nullptr);
   const int nit = fixed > 0 ? std::min(fixed, P.maxiter) : P.maxiter;
   int it = 0;
   for (it = 1; it <= nit; it++) {
-    mg_apply(st, P.lev[0], P.lam[0], P.p, P.q, P.done);
-    mg_dot(st, P.ng, P.gsize, P.p, P.q, nullptr, nullptr, P.part, P.nchunk, P.pq, P.done, nullptr);
-    mg_cg_update(st, P.ng, P.gsize, P.rz[cur], P.pq, P.x, P.p, P.r, P.q, P.done);
-    mg_dot(st, P.ng, P.gsize, P.r, P.r, nullptr, nullptr, P.part, P.nchunk, P.rr, P.done, nullptr);
+    mg_apply(st, P.h64.lev[0], P.h64.lam, P.p, P.q, P.done);
+    mg_dot<double, double>(st, P.ng, P.gsize, P.p, P.q, nullptr, nullptr, P.part, P.nchunk, P.pq, P.done, nullptr);
+    mg_cg_update(st, P.ng, P.gsize, P.rz[cur], P.pq, P.x, P.p, P.r, P.q, mixed ? P.rf : nullptr, P.done);
+    // r.r and, for the flexible form, r_new . z_old in the second slot
+    if (mixed)
+      mg_dot<double, float>(st, P.ng, P.gsize, P.r, P.r, P.zf, P.r, P.part, P.nchunk, P.rr, P.done, nullptr);
+    else
+      mg_dot<double, double>(st, P.ng, P.gsize, P.r, P.r, nullptr, nullptr, P.part, P.nchunk, P.rr, P.done, nullptr);
     mg_flags(st, P.ng, P.rr, P.bb, tol2, P.done, P.left);
     if (fixed == 0) {
       SNAC_HIP(hipMemcpyAsync(P.left_host, P.left, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -498,10 +542,7 @@ void mg_solve(MGPlan &P, hipStream_t st, int fixed) {
     } else if (it == nit) {
       break;
     }
-    vcycle(P, st, 0);
-    project(P, st, P.z);
-    mg_dot(st, P.ng, P.gsize, P.r, P.z, nullptr, nullptr, P.part, P.nchunk, P.rz[1 - cur], P.done, nullptr);
-    mg_cg_dir(st, P.ng, P.gsize, P.rz[1 - cur], P.rz[cur], P.z, P.p, P.done);
+    precondition(P, st, P.rz[1 - cur], P.rz[cur]);
     cur = 1 - cur;
   }
   if (fixed > 0) {

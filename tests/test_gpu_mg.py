@@ -7,6 +7,10 @@ mode to ||r|| <= 1e-13 ||b|| (hypre_tol of the test inputs), so the pressure
 correction agrees to ~1e-11 and the fields after 10 steps within the 1e-10
 gate.  p / pp of singular (all-Neumann or periodic) problems are compared
 modulo their mean.
+
+Both precisions run every case: "mixed" (FP32 V-cycle under a flexible CG
+with FP64 vectors, operator and dot products) stops at the same FP64
+residual tolerance, so it is held to the same bounds.
 """
 import os
 
@@ -21,7 +25,7 @@ from test_gpu_parity import FIELDS3, GOLD, assert_fields, run_both, seed_fields
 pytestmark = pytest.mark.gpu
 
 
-def pair(tmp_path, geom):
+def pair(tmp_path, geom, prec="fp64"):
     import snac_amd
     fa, singular = 3, False
     if geom == "lchannel":
@@ -45,6 +49,7 @@ def pair(tmp_path, geom):
         print(f"case {geom}: fft_axis={fa}", flush=True)
         ctx = snac_amd.Context(dn, bn, fa)
         ctx.set_solver("mg")
+        ctx.set_precision(prec)
         oc = oracle_py.OracleCase(nmlparse.load_case(dn, bn, fa))
         oc.set_solver("band")
         return ctx, oc, True
@@ -54,21 +59,23 @@ def pair(tmp_path, geom):
     print(f"case {geom}: fft_axis={fa} blocks={[B['ng'] for B in blocks]}", flush=True)
     ctx = snac_amd.Context(d, b, fa)
     ctx.set_solver("mg")
+    ctx.set_precision(prec)
     oc = oracle_py.OracleCase(nmlparse.load_case(d, b, fa))
     oc.set_solver("band")
     return ctx, oc, singular
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("geom", ["lchannel", "bfs", "channel", "cavity3d", "ldc3d", "ldc_fft"])
-def test_mg_solve_pressure(tmp_path, geom):
-    ctx, oc, singular = pair(tmp_path, geom)
+def test_mg_solve_pressure(tmp_path, geom, prec):
+    ctx, oc, singular = pair(tmp_path, geom, prec)
     seed_fields(ctx, oc)
     ctx.bounduvw(False); oc.bounduvw(False)
     ctx.fillps(1e-3); oc.fillps(1e-3)
     ctx.solve_pressure()
     oc.solve_pressure()
     iters, relres = ctx.solver_stats()
-    print(f"{geom}: {iters} CG iterations, max relative residual {relres:.2e}", flush=True)
+    print(f"{geom} {prec}: {iters} CG iterations, max relative residual {relres:.2e}", flush=True)
     assert 0 < iters < 100
     assert relres <= 1e-13
     assert_fields(ctx, oc, ("pp",), 1e-10, interior=True, modmean=("pp",) if singular else ())
@@ -97,13 +104,33 @@ def test_mg_matches_tensor_solver(tmp_path):
     assert np.linalg.norm(ga - gm) / np.linalg.norm(ga) < 1e-10
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("geom", ["lchannel", "bfs"])
-def test_mg_10_steps(tmp_path, geom):
-    ctx, oc, singular = pair(tmp_path, geom)
+def test_mg_10_steps(tmp_path, geom, prec):
+    ctx, oc, singular = pair(tmp_path, geom, prec)
     run_both(ctx, oc, 10)
     assert_fields(ctx, oc, FIELDS3, 1e-10, interior=True)
     assert_fields(ctx, oc, ("p",), 1e-10, interior=True, modmean=("p",) if singular else ())
     _, divmax = ctx.chkdiv()
     _, odivmax = oc.chkdiv()
-    print(f"{geom}: divmax GPU {divmax:.2e} oracle {odivmax:.2e}", flush=True)
+    print(f"{geom} {prec}: divmax GPU {divmax:.2e} oracle {odivmax:.2e}", flush=True)
     assert divmax < 1e-9
+
+
+def test_precision_switch_keeps_results(tmp_path):
+    """Switching precision on a context re-measures the graph's CG count;
+    a mixed solve and an FP64 solve of the same rhs agree to the tolerance."""
+    ctx, oc, _ = pair(tmp_path, "bfs", "fp64")
+    seed_fields(ctx, oc)
+    ctx.bounduvw(False)
+    ctx.fillps(1e-3)
+    rhs = [ctx.get(b, "pp") for b in range(ctx.nblocks)]
+    ctx.solve_pressure()
+    ref = [ctx.get(b, "pp")[1:-1, 1:-1, 1:-1] for b in range(ctx.nblocks)]
+    ctx.set_precision("mixed")
+    for b in range(ctx.nblocks):
+        ctx.set(b, "pp", rhs[b])
+    ctx.solve_pressure()
+    for b in range(ctx.nblocks):
+        g = ctx.get(b, "pp")[1:-1, 1:-1, 1:-1]
+        assert np.linalg.norm(g - ref[b]) / np.linalg.norm(ref[b]) < 1e-10
