@@ -170,11 +170,12 @@ int snac_get_timing(snac_ctx *ctx, double *ms, int *count);
  * tridiagonal variant -- these only with the tensor-product solver; -1 =
  * dense path -- and last the rk_mom kernel variant: 0 one cell per thread,
  * 1..6 k-march, (v-1)%3 selecting 8/16/32 planes and (v-1)/3 the tile
- * 64x4 or 32x8) and the best times in
- * ms.  No
+ * 64x4 or 32x8; 7..10 k-march with the planes staged in LDS, 64x4 tiles
+ * over 16/32 planes (7, 8) or 64x8 tiles over 16/32 planes (9, 10)) and
+ * the best times in ms.  No
  * counterpart in the reference (its FFTW plans use FFTW_ESTIMATE).          */
 int snac_get_autotune(snac_ctx *ctx, int *choices, double *ms, int n);
-/* overrides an autotuned kernel variant: SNAC_KV_RK_MOM (0..6, as above) or
+/* overrides an autotuned kernel variant: SNAC_KV_RK_MOM (0..10, as above) or
  * SNAC_KV_TRIDIAG (0: c' in LDS, lines up to 128; 1: c' in scratch; 2:
  * chunked, no scratch, lines of 8..256); every variant computes the same
  * result.  No counterpart in the reference.                                 */

@@ -50,7 +50,8 @@ struct FaceOp {
 // velocities to un,vn,wn and dudtrk to dto.
 // s1 (scalar 1, ghosted) non-null adds the Boussinesq term with gb = gacc*beta;
 // variant 0: one cell per thread; 1..6: k-march over chunks of 8/16/32
-// planes with 64x4 or 32x8 tiles (autotuned)
+// planes with 64x4 or 32x8 tiles; 7..10: k-march with LDS-staged planes,
+// 64x4 / 64x8 tiles over 16 / 32 planes (autotuned)
 void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v,
                    const double *w, const double *p, double *un, double *vn, double *wn, double *dtu,
                    double *dtv, double *dtw, double f1, double f2, double f12, double visc, double bfx,

@@ -624,15 +624,15 @@ void rk_mom(snac_ctx *c, int irk, double dt) {
   c->cur = nx;
 }
 
-// rk_mom kernel variant (one cell per thread, or a k-march with one of 6
-// chunk/tile choices), timed on this case's own arrays at context
+// rk_mom kernel variant (one cell per thread, a k-march with one of 6
+// chunk/tile choices, or an LDS-staged k-march with one of 4), timed on this case's own arrays at context
 // creation (the arrays are still zero then; the buffers it writes are
 // cleared afterwards)
 void autotune_rk(snac_ctx *c) {
   if (c->device < 0) return;
   double best = 1e300;
   int bv = 0;
-  for (int v = 0; v <= 6; v++) {
+  for (int v = 0; v <= 10; v++) {
     c->rk_variant = v;
     const int cur0 = c->cur;
     double ms = time_stage(c, [&] {
@@ -1224,7 +1224,7 @@ int snac_set_kernel_variant(snac_ctx *c, int kernel, int variant) {
   return guard([&] {
     on_device(c);
     if (kernel == SNAC_KV_RK_MOM) {
-      if (variant < 0 || variant > 6) throw std::runtime_error("rk_mom variant must be 0..6");
+      if (variant < 0 || variant > 10) throw std::runtime_error("rk_mom variant must be 0..10");
       c->rk_variant = variant;
     } else if (kernel == SNAC_KV_TRIDIAG) {
       if (!c->tp.ok) throw std::runtime_error("no tensor-product plan on this case");

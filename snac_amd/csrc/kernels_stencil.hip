@@ -226,6 +226,167 @@ __global__ __launch_bounds__(BX *BY) void k_rk_mom_kz(Geo g, Metrics m, const do
   }
 }
 
+// The k-march with the planes staged in LDS: a workgroup of 64 x BY
+// threads owns a 64 x BY column tile and walks KB planes; u, v, w, p of
+// planes k-1, k, k+1 over the tile plus a one-cell halo sit in a ring of
+// three LDS planes, so every in-plane and cross-plane neighbour is an LDS
+// read and each plane is fetched from memory once per tile, with row-wise
+// coalesced loads.  The loads of plane k+2 are issued before plane k is
+// computed and land in the ring slot of plane k-1 after it, so their
+// latency overlaps the arithmetic.
+constexpr int LX = 66;  // row pitch of an LDS plane: 64 + 2 halo cells
+template <bool BUOY, int KB, int BY>
+__global__ __launch_bounds__(64 * BY) void k_rk_mom_lds(Geo g, Metrics m, const double *__restrict__ u,
+                                                        const double *__restrict__ v, const double *__restrict__ w,
+                                                        const double *__restrict__ p, double *__restrict__ un,
+                                                        double *__restrict__ vn, double *__restrict__ wn,
+                                                        double *__restrict__ dtu, double *__restrict__ dtv,
+                                                        double *__restrict__ dtw, double f1, double f2, double f12,
+                                                        double visc, double bfx, double bfy, double bfz,
+                                                        const double *__restrict__ s1, double gbx, double gby,
+                                                        double gbz) {
+  constexpr int NT = 64 * BY, PL = LX * (BY + 2), NL = (PL + NT - 1) / NT;
+  __shared__ double sh[4][3][PL];  // field (u, v, w, p) x ring slot x plane
+  int bx, by, bz;
+  tile_of((g.nx + 63) / 64, (g.ny + BY - 1) / BY, bx, by, bz);
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
+  const int i0 = bx * 64 + 1, j0 = by * BY + 1;
+  const int i = i0 + tx, j = j0 + ty;
+  const int k0 = bz * KB + 1;
+  const int k1 = min(k0 + KB - 1, g.nz);
+  const long sx = g.sx, sy = g.sy, sz = sx * sy;
+  // this thread's share of a plane load: LDS offset and global offset
+  // within the plane (-1: outside the array)
+  int lo[NL];
+  long go[NL];
+#pragma unroll
+  for (int r = 0; r < NL; r++) {
+    const int e = tid + r * NT;
+    const int row = e / LX, col = e - row * LX;
+    const int gi = i0 - 1 + col, gj = j0 - 1 + row;
+    lo[r] = e;
+    go[r] = (e < PL && gi <= g.nx + 1 && gj <= g.ny + 1) ? gidx(gi, gj, 0, sx, sy) : -1;
+  }
+  auto load_plane = [&](int k, double (&reg)[4][NL]) {
+    const long kz = (long)k * sz;
+#pragma unroll
+    for (int r = 0; r < NL; r++) {
+      if (go[r] >= 0) {
+        reg[0][r] = u[kz + go[r]];
+        reg[1][r] = v[kz + go[r]];
+        reg[2][r] = w[kz + go[r]];
+        reg[3][r] = p[kz + go[r]];
+      }
+    }
+  };
+  auto store_plane = [&](int slot, const double (&reg)[4][NL]) {
+#pragma unroll
+    for (int r = 0; r < NL; r++)
+      if (go[r] >= 0)
+#pragma unroll
+        for (int f = 0; f < 4; f++) sh[f][slot][lo[r]] = reg[f][r];
+  };
+  {
+    double reg[4][NL];
+    for (int k = k0 - 1; k <= k0 + 1 && k <= g.nz + 1; k++) {
+      load_plane(k, reg);
+      store_plane(k % 3, reg);
+    }
+  }
+  __syncthreads();
+  const bool live = i <= g.nx && j <= g.ny;
+  double dxfi = 0, dxfip = 0, dyfj = 0, dyfjp = 0, ixci = 0, ixcim = 0, ixfi = 0, ixfip = 0;
+  double iycj = 0, iycjm = 0, iyfj = 0, iyfjp = 0;
+  if (live) {
+    dxfi = m.dxf[i]; dxfip = m.dxf[i + 1]; dyfj = m.dyf[j]; dyfjp = m.dyf[j + 1];
+    ixci = m.idxc[i]; ixcim = m.idxc[i - 1]; ixfi = m.idxf[i]; ixfip = m.idxf[i + 1];
+    iycj = m.idyc[j]; iycjm = m.idyc[j - 1]; iyfj = m.idyf[j]; iyfjp = m.idyf[j + 1];
+  }
+  const int L = (ty + 1) * LX + tx + 1;  // this cell in an LDS plane
+  const int LXP = L + 1, LXM = L - 1, LYP = L + LX, LYM = L - LX;
+  long c = gidx(i, j, k0, sx, sy);
+  long q = (long)(i - 1) + (long)g.nx * ((long)(j - 1) + (long)g.ny * (long)(k0 - 1));
+  const long qs = (long)g.nx * g.ny;
+  for (int k = k0; k <= k1; k++, c += sz, q += qs) {
+    double nxt[4][NL];
+    const bool pre = k + 2 <= k1 + 1;
+    if (pre) load_plane(k + 2, nxt);
+    const int sm_ = (k + 2) % 3, s0 = k % 3, sp_ = (k + 1) % 3;  // slots of k-1, k, k+1
+    if (live) {
+      const double *U0 = sh[0][s0], *V0 = sh[1][s0], *W0 = sh[2][s0], *P0 = sh[3][s0];
+      const double *Um = sh[0][sm_], *Vm = sh[1][sm_], *Wm = sh[2][sm_];
+      const double *Up = sh[0][sp_], *Vp = sh[1][sp_], *Wp = sh[2][sp_], *Pp = sh[3][sp_];
+      const double uc = U0[L], vc = V0[L], wc = W0[L], pc = P0[L];
+      const double um = Um[L], vm = Vm[L], wm = Wm[L];
+      const double up = Up[L], vp = Vp[L], wp = Wp[L], pz = Pp[L];
+      const double dzfk = m.dzf[k], dzfkp = m.dzf[k + 1];
+      const double izck = m.idzc[k], izckm = m.idzc[k - 1], izfk = m.idzf[k], izfkp = m.idzf[k + 1];
+      const double uxp = U0[LXP], uxm = U0[LXM], uyp = U0[LYP], uym = U0[LYM];
+      const double vxp = V0[LXP], vxm = V0[LXM], vyp = V0[LYP], vym = V0[LYM];
+      const double wxp = W0[LXP], wxm = W0[LXM], wyp = W0[LYP], wym = W0[LYM];
+      double a, b, cc;
+      {
+        double dudxp = (uxp - uc) * ixfip, dudxm = (uc - uxm) * ixfi;
+        double dudyp = (uyp - uc) * iycj, dudym = (uc - uym) * iycjm;
+        double dudzp = (up - uc) * izck, dudzm = (uc - um) * izckm;
+        a = visc * (dudxp - dudxm) * ixci + visc * (dudyp - dudym) * iyfj + visc * (dudzp - dudzm) * izfk;
+        double dvdxp = (vxp - vc) * ixci, dvdxm = (vc - vxm) * ixcim;
+        double dvdyp = (vyp - vc) * iyfjp, dvdym = (vc - vym) * iyfj;
+        double dvdzp = (vp - vc) * izck, dvdzm = (vc - vm) * izckm;
+        b = visc * (dvdxp - dvdxm) * ixfi + visc * (dvdyp - dvdym) * iycj + visc * (dvdzp - dvdzm) * izfk;
+        double dwdxp = (wxp - wc) * ixci, dwdxm = (wc - wxm) * ixcim;
+        double dwdyp = (wyp - wc) * iycj, dwdym = (wc - wym) * iycjm;
+        double dwdzp = (wp - wc) * izfkp, dwdzm = (wc - wm) * izfk;
+        cc = visc * (dwdxp - dwdxm) * ixfi + visc * (dwdyp - dwdym) * iyfj + visc * (dwdzp - dwdzm) * izck;
+      }
+      {
+        double uuip = 0.25 * (uc * dyfj * dzfk + uxp * dyfj * dzfk) * uxp;
+        double uuim = 0.25 * (uc * dyfj * dzfk + uxm * dyfj * dzfk) * uxm;
+        double uvjp = 0.25 * (vc * dxfi * dzfk + vxp * dxfip * dzfk) * uyp;
+        double uvjm = 0.25 * (vym * dxfi * dzfk + V0[LYM + 1] * dxfip * dzfk) * uym;
+        double uwkp = 0.25 * (wc * dxfi * dyfj + wxp * dxfip * dyfj) * up;
+        double uwkm = 0.25 * (wm * dxfi * dyfj + Wm[LXP] * dxfip * dyfj) * um;
+        a = a + ((-uuip + uuim) + (-uvjp + uvjm) + (-uwkp + uwkm)) * (ixci * iyfj * izfk);
+        double uvip = 0.25 * (uc * dyfj * dzfk + uyp * dyfjp * dzfk) * vxp;
+        double uvim = 0.25 * (uxm * dyfj * dzfk + U0[LXM + LX] * dyfjp * dzfk) * vxm;
+        double vvjp = 0.25 * (vc * dxfi * dzfk + vyp * dxfi * dzfk) * vyp;
+        double vvjm = 0.25 * (vc * dxfi * dzfk + vym * dxfi * dzfk) * vym;
+        double wvkp = 0.25 * (wc * dyfj * dxfi + wyp * dyfjp * dxfi) * vp;
+        double wvkm = 0.25 * (wm * dyfj * dxfi + Wm[LYP] * dyfjp * dxfi) * vm;
+        b = b + ((-uvip + uvim) + (-vvjp + vvjm) + (-wvkp + wvkm)) * (ixfi * iycj * izfk);
+        double uwip = 0.25 * (uc * dzfk * dyfj + up * dzfkp * dyfj) * wxp;
+        double uwim = 0.25 * (uxm * dzfk * dyfj + Up[LXM] * dzfkp * dyfj) * wxm;
+        double vwjp = 0.25 * (vc * dzfk * dxfi + vp * dzfkp * dxfi) * wyp;
+        double vwjm = 0.25 * (vym * dzfk * dxfi + Vp[LYM] * dzfkp * dxfi) * wym;
+        double wwkp = 0.25 * (wc * dxfi * dyfj + wp * dxfi * dyfj) * wp;
+        double wwkm = 0.25 * (wc * dxfi * dyfj + wm * dxfi * dyfj) * wm;
+        cc = cc + ((-uwip + uwim) + (-vwjp + vwjm) + (-wwkp + wwkm)) * (ixfi * iyfj * izck);
+      }
+      double un_ = uc + f1 * a + f2 * dtu[q];
+      double vn_ = vc + f1 * b + f2 * dtv[q];
+      double wn_ = wc + f1 * cc + f2 * dtw[q];
+      un_ = un_ + f12 * (-(P0[LXP] - pc) * ixci + bfx);
+      vn_ = vn_ + f12 * (-(P0[LYP] - pc) * iycj + bfy);
+      wn_ = wn_ + f12 * (-(pz - pc) * izck + bfz);
+      if (BUOY) {
+        const double sc = s1[c];
+        un_ = un_ - f12 * gbx * 0.5 * (s1[c + 1] + sc);
+        vn_ = vn_ - f12 * gby * 0.5 * (s1[c + sx] + sc);
+        wn_ = wn_ - f12 * gbz * 0.5 * (s1[c + sz] + sc);
+      }
+      dtu[q] = a;
+      dtv[q] = b;
+      dtw[q] = cc;
+      un[c] = un_;
+      vn[c] = vn_;
+      wn[c] = wn_;
+    }
+    __syncthreads();  // plane k-1's slot is free
+    if (pre) store_plane(sm_, nxt);
+    __syncthreads();
+  }
+}
+
 // rk_scal (rk.f90:130-204) with scal_d / scal_a (scal.f90:89-155) fused:
 // reads s (ghosted) and u, v, w; writes the updated interior into sn (the
 // other buffer of the pair) and dsdtrk into dso.
@@ -460,6 +621,28 @@ void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *
                    const double *p, double *un, double *vn, double *wn, double *dtu, double *dtv, double *dtw,
                    double f1, double f2, double f12, double visc, double bfx, double bfy, double bfz,
                    const double *s1, const double gb[3], int variant) {
+  if (variant >= 7) {
+    // 7..10: LDS-staged k-march; 7/8: 64x4 tiles, KB 16/32; 9/10: 64x8 tiles, KB 16/32
+    // (capping the registers at 128 for a fourth wave per SIMD spills and measured slower)
+    const int vi = variant - 7, kb = vi % 2 ? 32 : 16, by = vi >= 2 ? 8 : 4;
+    dim3 grid((unsigned)(((g.nx + 63) / 64) * ((g.ny + by - 1) / by) * ((g.nz + kb - 1) / kb)));
+#define SNAC_KL(B, KB, Y)                                                                                           \
+  hipLaunchKernelGGL((k_rk_mom_lds<B, KB, Y>), grid, dim3(64, Y), 0, s, g, m, u, v, w, p, un, vn, wn, dtu, dtv, dtw,  \
+                     f1, f2, f12, visc, bfx, bfy, bfz, s1, gb[0], gb[1], gb[2])
+#define SNAC_KL_ALL(B)                                                                                              \
+  do {                                                                                                              \
+    if (vi == 0) SNAC_KL(B, 16, 4);                                                                                 \
+    else if (vi == 1) SNAC_KL(B, 32, 4);                                                                            \
+    else if (vi == 2) SNAC_KL(B, 16, 8);                                                                            \
+    else SNAC_KL(B, 32, 8);                                                                                         \
+  } while (0)
+    if (s1) SNAC_KL_ALL(true);
+    else SNAC_KL_ALL(false);
+#undef SNAC_KL_ALL
+#undef SNAC_KL
+    SNAC_HIP(hipGetLastError());
+    return;
+  }
   if (variant > 0) {
     // 1..6: k-march; (variant-1) % 3 -> KB 8/16/32, / 3 -> tile 64x4, 32x8
     const int vi = variant - 1, kb = 8 << (vi % 3), shape = vi / 3;

@@ -83,9 +83,12 @@ def test_r2r_lines(tmp_path, kind, n):
 
 
 # ---------------------------------------------------------------- routines
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6])
-def test_rk_mom(tmp_path, variant):
-    ctx, oc = channel(tmp_path)
+@pytest.mark.parametrize("n", [(32, 24, 16), (70, 13, 37)])
+@pytest.mark.parametrize("variant", range(11))
+def test_rk_mom(tmp_path, variant, n):
+    """Every kernel variant, on a grid inside one tile and on a ragged one
+    that leaves partial tiles along i, j and k."""
+    ctx, oc = channel(tmp_path, n)
     ctx.set_rk_variant(variant)
     for irk in (1, 2, 3):
         seed_fields(ctx, oc, seed=irk)

@@ -69,7 +69,7 @@ def test_scalar_boundp(tmp_path):
         np.testing.assert_array_equal(ctx.get(0, f), oc.get(0, f))
 
 
-@pytest.mark.parametrize("variant", [0, 2])
+@pytest.mark.parametrize("variant", [0, 2, 8])
 def test_rk_mom_buoyancy(tmp_path, variant):
     ctx, oc = scalar_pair(tmp_path)
     ctx.set_rk_variant(variant)
