@@ -45,6 +45,25 @@ constexpr int fft_lines(int n1, int n2) {
                                                                 : (LDS_COMPLEX / (n1 * n2));
 }
 
+// Input / output offsets of the workgroup's L lines, one division each,
+// kept in LDS so the per-element address is one multiply-add (-1: past the
+// last line).  Ends with a barrier.
+template <int L>
+__device__ __forceinline__ void line_offsets(const Args &a, int line0, long (&oin)[L], long (&oout)[L]) {
+  for (int t = threadIdx.x; t < L; t += NT) {
+    const int line = line0 + t;
+    long oi = -1, oo = -1;
+    if (line < a.nlines) {
+      const int b1 = line % a.nb1, b2 = line / a.nb1;
+      oi = b1 * a.is1 + b2 * a.is2;
+      oo = b1 * a.os1 + b2 * a.os2;
+    }
+    oin[t] = oi;
+    oout[t] = oo;
+  }
+  __syncthreads();
+}
+
 __device__ __forceinline__ void elem_of(const Args &a, int e, int &line, int &q) {
   if (a.coal) { line = e % a.L; q = e / a.L; }
   else { q = e % a.n; line = e / a.n; }
@@ -139,6 +158,8 @@ __global__ __launch_bounds__(NT) void k_r2r_fft(Args a) {
   const double2 *tk = reinterpret_cast<const double2 *>(a.tab) + n;  // (cos, sin)(pi k/2n)
   constexpr bool inv = (KIND == K_HC2R || KIND == K_REDFT01 || KIND == K_RODFT01);
   double *R = reinterpret_cast<double *>(S);
+  __shared__ long oin[L], oout[L];
+  line_offsets<L>(a, line0, oin, oout);
   // 1. raw lines into LDS, line l at doubles [2 n l, 2 n l + n)
   {
     double xr[IT];
@@ -146,12 +167,8 @@ __global__ __launch_bounds__(NT) void k_r2r_fft(Args a) {
     for (int it = 0; it < IT; it++) {
       const int e = threadIdx.x + it * NT;
       const int l = a.coal ? e % L : e / n, q = a.coal ? e / L : e % n;
-      const int line = line0 + l;
       xr[it] = 0.0;
-      if (e < L * n && line < a.nlines) {
-        const int b1 = line % a.nb1, b2 = line / a.nb1;
-        xr[it] = a.in[q * a.ies + b1 * a.is1 + b2 * a.is2];
-      }
+      if (e < L * n && oin[l] >= 0) xr[it] = a.in[q * a.ies + oin[l]];
     }
 #pragma unroll
     for (int it = 0; it < IT; it++) {
@@ -213,11 +230,142 @@ __global__ __launch_bounds__(NT) void k_r2r_fft(Args a) {
   for (int it = 0; it < IT; it++) {
     const int e = threadIdx.x + it * NT;
     const int l = a.coal ? e % L : e / n, q = a.coal ? e / L : e % n;
-    const int line = line0 + l;
-    if (e < L * n && line < a.nlines) {
+    if (e < L * n && oout[l] >= 0) {
       const double y = post_val<KIND>(S + n * l, q, n, tk);
-      const int b1 = line % a.nb1, b2 = line / a.nb1;
-      a.out[q * a.oes + b1 * a.os1 + b2 * a.os2] = y * a.scale;
+      a.out[q * a.oes + oout[l]] = y * a.scale;
+    }
+  }
+}
+
+// R2HC / HC2R of n = 2h points through one h-point complex FFT per line
+// (h = N1*N2), half the work and half the LDS of the n-point form:
+//   forward: z[m] = x[2m] + i x[2m+1]; Z = FFT_h(z); for k = 0..h
+//     X[k] = E + W_n^k O,  E = (Z[k] + conj Z[h-k]) / 2,  O = (Z[k] - conj Z[h-k]) / 2i
+//   inverse: Z[k] = A + i B,  A = X[k] + conj X[h-k],  B = (X[k] - conj X[h-k]) w^k
+//     (w = e^{2 pi i/n}), z = IFFT_h(Z), y[2m] + i y[2m+1] = z[m]
+// (indices of Z modulo h).  Every line is transformed on its own, so a line
+// of zeros stays exactly zero.  The raw line and its h complex values share
+// one n-double slot of LDS.
+#ifndef SNAC_RFFT_PAD
+#define SNAC_RFFT_PAD 2
+#endif
+constexpr int RPAD = SNAC_RFFT_PAD;  // doubles of padding after each LDS line (spreads lines over the banks)
+constexpr int rfft_lines(int n1, int n2) {
+  return (NT / (n1 > n2 ? n1 : n2)) < (LDS_COMPLEX / (2 * n1 * n2)) ? (NT / (n1 > n2 ? n1 : n2))
+                                                                    : (LDS_COMPLEX / (2 * n1 * n2));
+}
+
+template <int N1, int N2, bool INV, int LSHIFT>
+__global__ __launch_bounds__(NT) void k_rfft(Args a) {
+  extern __shared__ double2 S[];
+  constexpr int h = N1 * N2, n = 2 * h;
+  constexpr int L = (rfft_lines(N1, N2) >> LSHIFT) > 0 ? (rfft_lines(N1, N2) >> LSHIFT) : 1;
+  constexpr int IT = (L * n + NT - 1) / NT;
+  const int line0 = blockIdx.x * L;
+  const double2 *tw = reinterpret_cast<const double2 *>(a.tab);  // e^{-2 pi i m/n}, m < n
+  constexpr int LS = n + RPAD;                                     // LDS doubles per line
+  double *R = reinterpret_cast<double *>(S);                     // line l at doubles [LS l, LS l + n)
+  __shared__ long oin[L], oout[L];
+  line_offsets<L>(a, line0, oin, oout);
+  // 1. raw lines into LDS
+  {
+    double xr[IT];
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+      const int e = threadIdx.x + it * NT;
+      const int l = a.coal ? e % L : e / n, q = a.coal ? e / L : e % n;
+      xr[it] = 0.0;
+      if (e < L * n && oin[l] >= 0) xr[it] = a.in[q * a.ies + oin[l]];
+    }
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+      const int e = threadIdx.x + it * NT;
+      const int l = a.coal ? e % L : e / n, q = a.coal ? e / L : e % n;
+      if (e < L * n) R[LS * l + q] = xr[it];
+    }
+  }
+  __syncthreads();
+  // 2. first pass: N1-point DFTs over j1 for every (line, j2)
+  {
+    const int t = threadIdx.x;
+    const bool act = t < L * N2;
+    const int j2 = t % N2, l = t / N2;
+    double2 v[N1];
+    if (act) {
+      const double *r = R + LS * l;
+#pragma unroll
+      for (int j1 = 0; j1 < N1; j1++) {
+        const int k = N2 * j1 + j2;
+        if (!INV) {
+          v[j1] = make_double2(r[2 * k], r[2 * k + 1]);
+        } else {
+          // X[k] and conj X[h-k] from the halfcomplex line
+          const double2 xk = (k == 0) ? make_double2(r[0], 0.0) : make_double2(r[k], r[n - k]);
+          const double2 xc = (k == 0) ? make_double2(r[h], 0.0) : make_double2(r[h - k], -r[h + k]);
+          const double2 w = tw[k];  // conj of w^k
+          const double dx = xk.x - xc.x, dy = xk.y - xc.y;
+          const double bx = dx * w.x + dy * w.y, by = dy * w.x - dx * w.y;  // (dx + i dy) e^{+2 pi i k/n}
+          v[j1] = make_double2(xk.x + xc.x - by, xk.y + xc.y + bx);        // A + i B
+        }
+      }
+      fft_reg<N1>(v, tw, 2 * N2, INV);
+#pragma unroll
+      for (int k1 = 1; k1 < N1; k1++) {
+        double2 w = tw[(2 * j2 * k1) & (n - 1)];
+        if (INV) w.y = -w.y;
+        double2 y = v[k1];
+        v[k1] = make_double2(y.x * w.x - y.y * w.y, y.x * w.y + y.y * w.x);
+      }
+    }
+    __syncthreads();
+    if (act) {
+      double2 *T = S + (LS / 2) * l;
+#pragma unroll
+      for (int k1 = 0; k1 < N1; k1++) T[k1 * N2 + (j2 ^ (k1 & (N2 - 1)))] = v[k1];
+    }
+    __syncthreads();
+  }
+  // 3. second pass: N2-point DFTs over j2 for every (line, k1)
+  {
+    const int t = threadIdx.x;
+    const bool act = t < L * N1;
+    const int k1 = t % N1, l = t / N1;
+    double2 v[N2];
+    if (act) {
+      const double2 *T = S + (LS / 2) * l;
+#pragma unroll
+      for (int j2 = 0; j2 < N2; j2++) v[j2] = T[k1 * N2 + (j2 ^ (k1 & (N2 - 1)))];
+      fft_reg<N2>(v, tw, 2 * N1, INV);
+    }
+    __syncthreads();
+    if (act) {
+      double2 *Z = S + (LS / 2) * l;
+#pragma unroll
+      for (int k2 = 0; k2 < N2; k2++) Z[k1 + N1 * k2] = v[k2];
+    }
+    __syncthreads();
+  }
+  // 4. post-mapping and store
+#pragma unroll
+  for (int it = 0; it < IT; it++) {
+    const int e = threadIdx.x + it * NT;
+    const int l = a.coal ? e % L : e / n, q = a.coal ? e / L : e % n;
+    if (e < L * n && oout[l] >= 0) {
+      const double2 *Z = S + (LS / 2) * l;
+      double y;
+      if (!INV) {
+        const int k = q <= h ? q : n - q;
+        const double2 zk = Z[k & (h - 1)], zc = Z[(h - k) & (h - 1)];  // Z[k], Z[h-k] (mod h)
+        const double ex = 0.5 * (zk.x + zc.x), ey = 0.5 * (zk.y - zc.y);
+        const double ox = 0.5 * (zk.y + zc.y), oy = -0.5 * (zk.x - zc.x);
+        const double2 w = tw[k];
+        const double xr = ex + (w.x * ox - w.y * oy), xi = ey + (w.x * oy + w.y * ox);
+        y = q <= h ? xr : xi;
+      } else {
+        const double2 z = Z[q >> 1];
+        y = (q & 1) ? z.y : z.x;
+      }
+      a.out[q * a.oes + oout[l]] = y * a.scale;
     }
   }
 }
@@ -292,6 +440,31 @@ void launch_fft_kind(int kind, int n, int ls, dim3 grid, size_t lds, hipStream_t
   case 512: launch_sized<32, 16>(kind, ls, grid, lds, s, a); break;
   case 1024: launch_sized<32, 32>(kind, ls, grid, lds, s, a); break;
   default: throw std::runtime_error("r2r: no FFT path for length " + std::to_string(n));
+  }
+}
+
+template <int N1, int N2>
+void launch_rfft_sized(bool inv, int ls, dim3 grid, size_t lds, hipStream_t s, const Args &a) {
+#define SNAC_RF(I, LS) hipLaunchKernelGGL((k_rfft<N1, N2, I, LS>), grid, dim3(NT), lds, s, a)
+  if (ls == 0) { if (inv) SNAC_RF(true, 0); else SNAC_RF(false, 0); }
+  else if (ls == 1) { if (inv) SNAC_RF(true, 1); else SNAC_RF(false, 1); }
+  else if (ls == 2) { if (inv) SNAC_RF(true, 2); else SNAC_RF(false, 2); }
+  else throw std::runtime_error("r2r: bad tile choice");
+#undef SNAC_RF
+}
+
+// h-point split of an n = 2h point real transform
+void launch_rfft(bool inv, int n, int ls, dim3 grid, size_t lds, hipStream_t s, const Args &a) {
+  switch (n) {
+  case 8: launch_rfft_sized<2, 2>(inv, ls, grid, lds, s, a); break;
+  case 16: launch_rfft_sized<4, 2>(inv, ls, grid, lds, s, a); break;
+  case 32: launch_rfft_sized<4, 4>(inv, ls, grid, lds, s, a); break;
+  case 64: launch_rfft_sized<8, 4>(inv, ls, grid, lds, s, a); break;
+  case 128: launch_rfft_sized<8, 8>(inv, ls, grid, lds, s, a); break;
+  case 256: launch_rfft_sized<16, 8>(inv, ls, grid, lds, s, a); break;
+  case 512: launch_rfft_sized<16, 16>(inv, ls, grid, lds, s, a); break;
+  case 1024: launch_rfft_sized<32, 16>(inv, ls, grid, lds, s, a); break;
+  default: throw std::runtime_error("r2r: no real-FFT path for length " + std::to_string(n));
   }
 }
 
@@ -374,7 +547,17 @@ void launch_r2r(hipStream_t s, const R2RPlan &pl, const double *in, long ies, lo
   a.nlines = nb1 * nb2;
   a.coal = (ies != 1) ? 1 : 0;
   if (a.nlines <= 0) return;
-  if (pl.fft) {
+  if (pl.fft && (pl.kind == K_R2HC || pl.kind == K_HC2R) && pl.n >= 8) {
+    int n1 = 0, n2 = 0;
+    fft_split(pl.n / 2, n1, n2);
+    const int L = std::max(1, rfft_lines(n1, n2) >> pl.lshift);  // the kernel's compile-time line count
+    a.L = L;
+    a.scale = scale;
+    a.tab = pl.tab;
+    dim3 grid((unsigned)((a.nlines + L - 1) / L));
+    size_t lds = (size_t)L * (pl.n + RPAD) * sizeof(double);
+    launch_rfft(pl.kind == K_HC2R, pl.n, pl.lshift, grid, lds, s, a);
+  } else if (pl.fft) {
     int n1 = 0, n2 = 0;
     fft_split(pl.n, n1, n2);
     const int L = std::max(1, fft_lines(n1, n2) >> pl.lshift);  // the kernel's compile-time line count

@@ -82,6 +82,28 @@ def test_r2r_lines(tmp_path, kind, n):
     ctx.close()
 
 
+@pytest.mark.parametrize("kind", ["R2HC", "HC2R"])
+@pytest.mark.parametrize("n", [8, 32, 128, 512, 1024])
+def test_rfft_lengths(tmp_path, kind, n):
+    """The half-length real-FFT path of R2HC / HC2R at every length it
+    serves; a line of zeros stays exactly zero next to nonzero lines."""
+    import snac_amd
+    d, b = cases.write_case(str(tmp_path), [cases.channel_block((4, 4, 4))], 5600.0)
+    ctx = snac_amd.Context(d, b, 3)
+    rng = np.random.default_rng(n)
+    x = rng.uniform(-1, 1, (41, n))
+    x[5] = 0.0
+    x[40] = 0.0
+    ref = np.array([oracle_py.r2r(kind, row, direct=True) for row in x])
+    for strided in (False, True):
+        for tile in (0, 1, 2):
+            y = ctx.r2r_lines(kind, x, strided=strided, tile=tile)
+            err = np.abs(y - ref).max()
+            assert err <= 1e-13 * n * np.abs(ref).max(), f"strided {strided} tile {tile}: {err:.3e}"
+            assert not y[5].any() and not y[40].any()
+    ctx.close()
+
+
 # ---------------------------------------------------------------- routines
 @pytest.mark.parametrize("n", [(32, 24, 16), (70, 13, 37)])
 @pytest.mark.parametrize("variant", range(11))
