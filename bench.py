@@ -15,6 +15,9 @@ the roofline of the dominant kernel (fused rk_mom, 104 algorithmic bytes
 per cell) from HIP events on its own stream, and the CPU port's rate on the
 host cores (a bounded sample of the same workload).
 
+--precision mixed runs the MG-PCG V-cycle in FP32 (MG-PCG path only; the
+channel's direct solver and the stencils stay FP64).
+
 --workload bfs runs the three-block backward-facing step instead (inlet
 192x64x128 with a Poiseuille inflow, lower 320x64x128, upper 320x96x128, z
 periodic with 128 modes, y tanh-stretched per block, outflow at x+): the
@@ -179,7 +182,7 @@ def main():
                 traffic = tr.get("bytes_per_launch")
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                    "kernel": "k_rk_mom (fused rk_mom, 104 B/cell)", "avg_launch_ms": round(rk_avg_s * 1e3, 4),
+                    "kernel": "rk_mom (fused momentum RHS + RK3 update, autotuned variant; 104 B/cell)", "avg_launch_ms": round(rk_avg_s * 1e3, 4),
                     "stream_copy_GBs": round(copy_gbs, 1)}
 
         cpu = None
