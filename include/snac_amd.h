@@ -98,6 +98,34 @@ int snac_solver_stats(snac_ctx *ctx, int *iters, double *max_relres);
 int snac_upload_field(snac_ctx *ctx, int b, int fid, const double *host);
 int snac_download_field(snac_ctx *ctx, int b, int fid, double *host);
 
+/* ---- files in the reference's formats ----
+ * load_one (load.f90:84-187): io 'w' writes block b's field fid as its
+ * interior values (Fortran order, FP64) followed by [time, istep] (null
+ * pointers write 0); io 'r' checks that the file holds exactly that many
+ * doubles (the reference aborts on a mismatch; here the call fails with the
+ * same message), sets the interior with zero ghosts and returns time and
+ * nint(istep).                                                              */
+int snac_load_one(snac_ctx *ctx, char io, const char *filename, int b, int fid, double *time, int *istep);
+/* The checkpoint of main.f90:543-560 ('r') and 1090-1134 ('w', the
+ * is_overwrite_save naming): <datadir>fld_b_NNN_{u,v,w,p}.bin and
+ * fld_b_NNN_s_MMM.bin per scalar, NNN the 1-based block number.  After 'r',
+ * po = p as main.f90:557; call snac_init_bounds next, as main.f90:562-639. */
+int snac_restart_io(snac_ctx *ctx, char io, const char *datadir, double *time, int *istep);
+/* write_visu_3d (output.f90:289-303): out3d writes the interior of the field
+ * to <datadir><fname_bin> (the whole field: the reference ignores nskip,
+ * output.f90:235-239) and write_log_output appends
+ * '(A,A,A,9i5,E15.7,i10)' fname_bin varname nmin nmax nskip time istep to
+ * <datadir><fname_log>, the input of the XDMF scripts.                     */
+int snac_write_visu_3d(snac_ctx *ctx, const char *datadir, const char *fname_bin, const char *fname_log,
+                       const char *varname, int b, int fid, const int *nmin, const int *nmax, const int *nskip,
+                       double time, int istep);
+/* save_grid (initgrid.f90:128-143) of every block and direction,
+ * <datadir>grid_{x,y,z}_b_NNN.bin / .out, and geometry_b_NNN.out
+ * (main.f90:429-437).  Works on host-only contexts.  Grid files
+ * grid/grid_{x,y,z}_b_NNN.bin in the working directory are read at context
+ * creation in place of the generated grid (load_grid, main.f90:419-428).    */
+int snac_save_grid(const snac_ctx *ctx, const char *datadir);
+
 /* ---- hot-path routines, all blocks at once, queued on the context's
  *      stream; results are visible after snac_sync or a download ----       */
 /* rk_mom rk.f90:7-128 with momx/y/z_d, momx/y/z_a_vv, momx/y/z_p

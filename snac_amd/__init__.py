@@ -57,6 +57,13 @@ def _load():
     L.snac_solver_stats.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.snac_upload_field.argtypes = [vp, C.c_int, C.c_int, dp]
     L.snac_download_field.argtypes = [vp, C.c_int, C.c_int, dp]
+    i3 = np.ctypeslib.ndpointer(dtype=np.int32, shape=(3,), flags="C_CONTIGUOUS")
+    L.snac_load_one.argtypes = [vp, C.c_char, C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                C.POINTER(C.c_int)]
+    L.snac_restart_io.argtypes = [vp, C.c_char, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.snac_write_visu_3d.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int, i3, i3,
+                                     i3, C.c_double, C.c_int]
+    L.snac_save_grid.argtypes = [vp, C.c_char_p]
     L.snac_rk_mom.argtypes = [vp, C.c_int, C.c_double]
     L.snac_rk_scal.argtypes = [vp, C.c_int, C.c_double]
     L.snac_scalar_forcing.argtypes = [vp, dp]
@@ -154,6 +161,37 @@ class Context:
             raise ValueError(f"field {fid} of block {b} must have shape {shp}, got {a.shape}")
         _check(lib.snac_upload_field(self.h, b, FIELDS[fid], np.ascontiguousarray(a.reshape(-1, order="F"))),
                "snac_upload_field")
+
+    # ---- files in the reference's formats ----
+    def load_one(self, io, filename, b, fid, time=0.0, istep=0):
+        """load_one (load.f90:84-187) of block b's field: io 'w' writes the
+        interior then [time, istep]; 'r' checks the size, sets the interior
+        (zero ghosts) and returns (time, istep)."""
+        t, s = C.c_double(time), C.c_int(istep)
+        _check(lib.snac_load_one(self.h, io.encode(), str(filename).encode(), b, FIELDS[fid], C.byref(t),
+                                 C.byref(s)), "load_one")
+        return t.value, s.value
+
+    def restart_io(self, io, datadir, time=0.0, istep=0):
+        """The checkpoint of every block, <datadir>fld_b_NNN_{u,v,w,p}.bin
+        (+ _s_MMM per scalar), main.f90:543-560 / 1090-1134; after 'r' po = p
+        and (time, istep) are returned; call init_bounds next."""
+        t, s = C.c_double(time), C.c_int(istep)
+        _check(lib.snac_restart_io(self.h, io.encode(), str(datadir).encode(), C.byref(t), C.byref(s)),
+               "restart_io")
+        return t.value, s.value
+
+    def write_visu_3d(self, datadir, fname_bin, fname_log, varname, b, fid, nmin, nmax, nskip, time, istep):
+        """write_visu_3d (output.f90:289-303): out3d + write_log_output."""
+        a = [np.ascontiguousarray(x, dtype=np.int32) for x in (nmin, nmax, nskip)]
+        _check(lib.snac_write_visu_3d(self.h, str(datadir).encode(), fname_bin.encode(), fname_log.encode(),
+                                      varname.encode(), b, FIELDS[fid], a[0], a[1], a[2], time, istep),
+               "write_visu_3d")
+
+    def save_grid(self, datadir):
+        """grid_{x,y,z}_b_NNN.bin/.out and geometry_b_NNN.out of every block
+        (initgrid.f90:128-143, main.f90:429-437)."""
+        _check(lib.snac_save_grid(self.h, str(datadir).encode()), "save_grid")
 
     # ---- reference routines ----
     def rk_mom(self, irk, dt):

@@ -8,6 +8,7 @@
 #include <stdexcept>
 
 #include "case.hpp"
+#include "io.hpp"
 
 namespace snac {
 namespace {
@@ -316,8 +317,18 @@ Case build_case(const std::string &dns_path, const std::string &blocks_path, int
   for (auto &B : c.blocks) c.vol_all += (B.lmax[0] - B.lmin[0]) * (B.lmax[1] - B.lmin[1]) * (B.lmax[2] - B.lmin[2]);
   // grids: initgrid then bound_grid (initgrid.f90:84-127), the exchanged
   // ghost spacing being the neighbour's interior spacing
-  for (auto &B : c.blocks)
-    for (int q = 0; q < 3; q++) initgrid(B.n[q], B.gt[q], B.gr[q], B.lmin[q], B.lmax[q], B.dc[q], B.df[q], B.rc[q], B.rf[q]);
+  // a grid file grid/grid_{x,y,z}_b_NNN.bin in the working directory
+  // replaces the generated grid of that block and direction (main.f90:415-428)
+  for (size_t b = 0; b < c.blocks.size(); b++) {
+    Block &B = c.blocks[b];
+    for (int q = 0; q < 3; q++) {
+      initgrid(B.n[q], B.gt[q], B.gr[q], B.lmin[q], B.lmax[q], B.dc[q], B.df[q], B.rc[q], B.rf[q]);
+      const std::string base = std::string("grid/grid_") + "xyz"[q] + "_b_" + block_num((int)b);
+      load_grid(base, B.n[q], B.rf[q], B.rc[q], B.df[q], B.dc[q]);
+      B.df0[q] = B.df[q];
+      B.dc0[q] = B.dc[q];
+    }
+  }
   {
     std::vector<std::array<double, 3>> lov(nb), hiv(nb);
     for (int b = 0; b < nb; b++)

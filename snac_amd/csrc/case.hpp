@@ -38,6 +38,9 @@ struct Block {
   bool is_inflow[3][2];
   // metrics, n+2 entries (index 0 = lo-1): dc = d?c, df = d?f, rc/rf coords
   std::array<std::vector<double>, 3> dc, df, rc, rf;
+  // df, dc as initgrid (or a grid file) left them, before bound_grid
+  // overwrites the interface ghosts: what save_grid writes
+  std::array<std::vector<double>, 3> dc0, df0;
   // inflow planes [dir][comp][side] over the ghosted tangential plane
   std::vector<double> vin[3][3][2];
   // pressure BC rhs planes [dir][side] over the interior tangential plane

@@ -12,6 +12,7 @@
 #include "bound_program.hpp"
 #include "case.hpp"
 #include "common.hpp"
+#include "io.hpp"
 #include "mg.hpp"
 
 namespace snac {
@@ -1070,6 +1071,129 @@ int snac_upload_field(snac_ctx *c, int b, int fid, const double *h) {
 
 int snac_download_field(snac_ctx *c, int b, int fid, double *h) {
   return guard([&] { copy_field(on_device(c), b, fid, h, nullptr); });
+}
+
+}  // extern "C"
+
+// ---- files in the reference's formats (io.hpp) ----
+namespace snac {
+namespace {
+
+bool is_ghosted(int fid) { return (fid >= 0 && fid < 6) || (fid >= SNAC_SCAL0 && (fid - SNAC_SCAL0) % 2 == 0); }
+
+// the block's interior values of a field, Fortran order
+std::vector<double> get_interior(snac_ctx *c, int b, int fid) {
+  const DevBlock &D = c->blk.at(b);
+  const size_t nx = D.g.nx, ny = D.g.ny, nz = D.g.nz;
+  std::vector<double> v(nx * ny * nz);
+  if (!is_ghosted(fid)) {
+    copy_field(c, b, fid, v.data(), nullptr);
+    return v;
+  }
+  std::vector<double> g((nx + 2) * (ny + 2) * (nz + 2));
+  copy_field(c, b, fid, g.data(), nullptr);
+  for (size_t k = 1; k <= nz; k++)
+    for (size_t j = 1; j <= ny; j++)
+      std::memcpy(&v[((k - 1) * ny + (j - 1)) * nx], &g[(k * (ny + 2) + j) * (nx + 2) + 1], nx * sizeof(double));
+  return v;
+}
+
+// sets a field to the given interior values with zero ghosts (the reference
+// zeroes the fields before it loads a checkpoint, main.f90:527-530)
+void set_interior(snac_ctx *c, int b, int fid, const std::vector<double> &v) {
+  const DevBlock &D = c->blk.at(b);
+  const size_t nx = D.g.nx, ny = D.g.ny, nz = D.g.nz;
+  if (!is_ghosted(fid)) {
+    copy_field(c, b, fid, nullptr, v.data());
+    return;
+  }
+  std::vector<double> g((nx + 2) * (ny + 2) * (nz + 2), 0.0);
+  for (size_t k = 1; k <= nz; k++)
+    for (size_t j = 1; j <= ny; j++)
+      std::memcpy(&g[(k * (ny + 2) + j) * (nx + 2) + 1], &v[((k - 1) * ny + (j - 1)) * nx], nx * sizeof(double));
+  copy_field(c, b, fid, nullptr, g.data());
+}
+
+void load_one(snac_ctx *c, char io, const std::string &path, int b, int fid, double *time, int *istep) {
+  const DevBlock &D = c->blk.at(b);
+  const size_t n = (size_t)D.g.nx * D.g.ny * D.g.nz;
+  if (io == 'w') {
+    const std::vector<double> v = get_interior(c, b, fid);
+    write_field_file(path, v.data(), n, time ? *time : 0.0, istep ? *istep : 0);
+  } else if (io == 'r') {
+    std::vector<double> v(n);
+    double t;
+    int is;
+    read_field_file(path, v.data(), n, t, is);
+    set_interior(c, b, fid, v);
+    if (time) *time = t;
+    if (istep) *istep = is;
+  } else {
+    throw std::runtime_error("load_one: io must be 'r' or 'w'");
+  }
+}
+
+}  // namespace
+}  // namespace snac
+
+extern "C" {
+
+int snac_load_one(snac_ctx *c, char io, const char *filename, int b, int fid, double *time, int *istep) {
+  return guard([&] {
+    on_device(c);
+    if (!filename) throw std::runtime_error("load_one: no file name");
+    load_one(c, io, filename, b, fid, time, istep);
+  });
+}
+
+int snac_restart_io(snac_ctx *c, char io, const char *datadir, double *time, int *istep) {
+  return guard([&] {
+    on_device(c);
+    if (io != 'r' && io != 'w') throw std::runtime_error("restart_io: io must be 'r' or 'w'");
+    const std::string dir = datadir ? datadir : "";
+    static const char *names[4] = {"u", "v", "w", "p"};
+    for (size_t b = 0; b < c->blk.size(); b++) {
+      const std::string base = dir + "fld_b_" + block_num((int)b);
+      for (int f = 0; f < 4; f++) load_one(c, io, base + "_" + names[f] + ".bin", (int)b, f, time, istep);
+      for (int is = 0; is < c->cs.nscal; is++)
+        load_one(c, io, base + "_s_" + block_num(is) + ".bin", (int)b, SNAC_SCAL0 + 2 * is, time, istep);
+      if (io == 'r') set_interior(c, (int)b, SNAC_PO, get_interior(c, (int)b, SNAC_P));  // po = p
+    }
+  });
+}
+
+int snac_write_visu_3d(snac_ctx *c, const char *datadir, const char *fname_bin, const char *fname_log,
+                       const char *varname, int b, int fid, const int *nmin, const int *nmax, const int *nskip,
+                       double time, int istep) {
+  return guard([&] {
+    on_device(c);
+    if (!fname_bin || !fname_log || !varname || !nmin || !nmax || !nskip)
+      throw std::runtime_error("write_visu_3d: null argument");
+    const std::string dir = datadir ? datadir : "";
+    const std::vector<double> v = get_interior(c, b, fid);
+    write_raw_field(dir + fname_bin, v.data(), v.size());
+    append_visu_log(dir + fname_log, fname_bin, varname, nmin, nmax, nskip, time, istep);
+  });
+}
+
+int snac_save_grid(const snac_ctx *c, const char *datadir) {
+  return guard([&] {
+    const Case &cs = checked(const_cast<snac_ctx *>(c))->cs;
+    const std::string dir = datadir ? datadir : "";
+    for (size_t b = 0; b < cs.blocks.size(); b++) {
+      const Block &B = cs.blocks[b];
+      int lo[3], hi[3];
+      double lmin[3], lmax[3];
+      for (int q = 0; q < 3; q++) {
+        save_grid(dir + "grid_" + "xyz"[q] + "_b_" + block_num((int)b), B.n[q], B.rf[q], B.rc[q], B.df0[q], B.dc0[q]);
+        lo[q] = B.lo_g[q];
+        hi[q] = B.lo_g[q] + B.n[q] - 1;
+        lmin[q] = B.lmin[q];
+        lmax[q] = B.lmax[q];
+      }
+      write_geometry(dir + "geometry_b_" + block_num((int)b) + ".out", lo, hi, lmin, lmax);
+    }
+  });
 }
 
 int snac_rk_mom(snac_ctx *c, int irk, double dt) {

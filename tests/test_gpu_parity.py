@@ -259,9 +259,12 @@ def test_ldc_multiblock_10_steps():
 
 
 @pytest.mark.parametrize("fft_axis", [1, 0])
-def test_ldc_golden_gpu(fft_axis):
+def test_ldc_golden_gpu(fft_axis, tmp_path):
     """The reference's known-answer test (tests/lid_driven_cavity) through
-    the HIP path: 1500 steps, v-profile rtol 1e-7 / atol 5e-8."""
+    the HIP path: 1500 steps, v-profile rtol 1e-7 / atol 5e-8.  The profile
+    is checked twice: from the device fields, and the way the reference's
+    test.py does it, from the vey_fld_0001500_b_00N.bin files out3d.h90
+    writes, sized by geometry_b_00N.out and assembled block by block."""
     import snac_amd
     from test_oracle_ldc_golden import ldc_profile
     dn, bn = os.path.join(GOLD, "ldc", "dns.nml"), os.path.join(GOLD, "ldc", "blocks.nml")
@@ -277,6 +280,20 @@ def test_ldc_golden_gpu(fft_axis):
     prof = ldc_profile(lambda b: ctx.get(b, "v"))
     ref = np.loadtxt(os.path.join(GOLD, "ldc", "data_ldc_re1000.txt"))
     np.testing.assert_allclose(prof, ref[:, 1], rtol=1e-7, atol=5e-8)
+    # the reference's own procedure (test.py) on the files this library writes
+    data = str(tmp_path) + "/"
+    ctx.save_grid(data)
+    blocks = []
+    for b in range(ctx.nblocks):
+        n = ctx.block_info(b)["n"]
+        ctx.write_visu_3d(data, f"vey_fld_0001500_b_{b + 1:03d}.bin", f"log_visu_3d_b_{b + 1:03d}.out", "Velocity_Y",
+                          b, "v", [1, 1, 1], n, [1, 1, 1], 0.0, 1500)
+        geo = np.loadtxt(f"{data}geometry_b_{b + 1:03d}.out", max_rows=4)
+        ng = (geo[1] - geo[0] + 1).astype(int)
+        fld = np.fromfile(f"{data}vey_fld_0001500_b_{b + 1:03d}.bin", dtype="float64")
+        blocks.append(fld.reshape(ng, order="F")[0, :, :])
+    full = np.block([[blocks[0], blocks[2]], [blocks[1], blocks[3]]])
+    np.testing.assert_allclose(full[full.shape[1] // 2, :], ref[:, 1], rtol=1e-7, atol=5e-8)
 
 
 def test_graph_replays_match_eager(tmp_path):
