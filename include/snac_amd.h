@@ -125,6 +125,16 @@ int snac_write_visu_3d(snac_ctx *ctx, const char *datadir, const char *fname_bin
  * grid/grid_{x,y,z}_b_NNN.bin in the working directory are read at context
  * creation in place of the generated grid (load_grid, main.f90:419-428).    */
 int snac_save_grid(const snac_ctx *ctx, const char *datadir);
+/* initflow / initscal (initflow.f90:10-262) for every block's block_inivel
+ * and every scalar's iniscal, as main.f90:531-542 calls them (is_wallturb
+ * false): 'zer', 'uni', 'cou', 'poi', 'hcp', 'pdc' (laminar), 'tgv', 'kov';
+ * scalars 'zer', 'uni', 'cou', 'dhc'.  'log' and 'hcl' add noise from the
+ * Fortran compiler's random_number and are refused (upload those fields).
+ * Ghosts and p are zero; call snac_init_bounds next.                        */
+int snac_initflow(snac_ctx *ctx);
+/* iparams: [iout0d, iout1d, iout2d, iout3d, isave, nsaves_max,
+ *           is_overwrite_save, restart] of &dns (param.f90:88-112)         */
+int snac_ctx_get_run_params(const snac_ctx *ctx, int *iparams);
 
 /* ---- hot-path routines, all blocks at once, queued on the context's
  *      stream; results are visible after snac_sync or a download ----       */

@@ -193,6 +193,7 @@ Case build_case(const std::string &dns_path, const std::string &blocks_path, int
       B.lmax[q] = num(bl, "block_lmax", 1.0, {q + 1, b + 1});
       B.gt[q] = (int)num(bl, "block_gt", 0, {q + 1, b + 1});
       B.gr[q] = num(bl, "block_gr", 0.0, {q + 1, b + 1});
+      if (q == 0) B.inivel = str(bl, "block_inivel", "zer", {b + 1}).substr(0, 3);
       if (B.n[q] < 1) throw std::runtime_error("invalid block_ng");
       if (B.gr[q] < 0) throw std::runtime_error("grid growth parameter must be positive.");
       B.cbcsca.resize((size_t)c.nscal * 6, 'N');
@@ -225,6 +226,20 @@ Case build_case(const std::string &dns_path, const std::string &blocks_path, int
       for (int cp = 0; cp < 3; cp++)
         if (!valid_pair(B.cbcvel[cp][q][0], B.cbcvel[cp][q][1])) throw std::runtime_error("velocity BCs not valid.");
     }
+  // chk_bc sanity.f90:136-157: the normal velocity component's pair and the
+  // pressure pair of each direction must be one of these combinations
+  for (auto &B : c.blocks)
+    for (int q = 0; q < 3; q++) {
+      const std::string v{B.cbcvel[q][q][0], B.cbcvel[q][q][1]}, p{B.cbcpre[q][0], B.cbcpre[q][1]};
+      static const char *ok[][2] = {{"FF", "FF"}, {"ND", "DN"}, {"DN", "ND"}, {"DD", "NN"}, {"FD", "FN"},
+                                    {"DF", "NF"}, {"FN", "FD"}, {"NF", "DF"}, {"FN", "FN"}, {"NF", "NF"},
+                                    {"NN", "NN"}, {"DN", "NN"}, {"ND", "NN"}, {"NN", "DD"}};
+      bool found = false;
+      for (auto &o : ok) found = found || (v == o[0] && p == o[1]);
+      if (!found) throw std::runtime_error("velocity and pressure BCs not compatible.");
+    }
+  // chk_stop_type sanity.f90:41-51
+  if (!c.stop_type[0] && !c.stop_type[1] && !c.stop_type[2]) throw std::runtime_error("stopping criterion not chosen.");
   // is_periodic param.f90:220-225
   for (int q = 0; q < 3; q++) {
     c.is_periodic[q] = true;

@@ -33,6 +33,7 @@ struct Block {
   char cbcpre[3][2];      // [dir][side]
   double bcpre[3][2];
   int inflow_type[3][2];
+  std::string inivel = "zer";  // block_inivel, the initial velocity field
   int nb[3][2];           // neighbour block per [dir][side], -1 = boundary
   bool is_bound[3][2];
   bool is_inflow[3][2];

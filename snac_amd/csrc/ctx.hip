@@ -1176,6 +1176,137 @@ int snac_write_visu_3d(snac_ctx *c, const char *datadir, const char *fname_bin, 
   });
 }
 
+int snac_initflow(snac_ctx *c) {
+  return guard([&] {
+    on_device(c);
+    const Case &cs = c->cs;
+    const double pi = std::acos(-1.0);
+    for (size_t b = 0; b < cs.blocks.size(); b++) {
+      const Block &B = cs.blocks[b];
+      const int nx = B.n[0], ny = B.n[1], nz = B.n[2];
+      const double l[3] = {B.lmax[0] - B.lmin[0], B.lmax[1] - B.lmin[1], B.lmax[2] - B.lmin[2]};
+      const size_t ntot = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
+      auto at = [&](int i, int j, int k) { return ((size_t)k * (ny + 2) + j) * (nx + 2) + i; };
+      const std::vector<double> &xc = B.rc[0], &xf = B.rf[0], &yc = B.rc[1], &yf = B.rf[1], &zc = B.rc[2],
+                                &zf = B.rf[2];
+      std::vector<double> u(ntot, 0.0), v(ntot, 0.0), w(ntot, 0.0), p(ntot, 0.0);
+      // initflow (initflow.f90:10-137), is_wallturb = .false. as main.f90:534 calls it
+      const std::string iv = B.inivel;
+      std::vector<double> u1d(nz + 2, 0.0);
+      double uref = 1.0, ubulk = uref;
+      const double lref1 = 1.0;
+      bool is_mean = false;
+      auto poiseuille = [&](double len, double norm) {
+        for (int k = 1; k <= nz; k++) {
+          const double z = zc[k] / len;
+          u1d[k] = 6.0 * z * (1.0 - z) * norm;
+        }
+      };
+      if (iv == "cou") {
+        for (int k = 1; k <= nz; k++) {
+          const double z = zc[k] / l[2];
+          u1d[k] = 0.5 * (1.0 - 2.0 * z) * 1.0 + 0.5;
+          u1d[k] = B.bcvel[0][2][0] * u1d[k] + B.bcvel[0][2][1] * (1.0 - u1d[k]);
+        }
+      } else if (iv == "poi") {
+        poiseuille(l[2], ubulk);
+        is_mean = true;
+      } else if (iv == "zer") {
+      } else if (iv == "uni") {
+        for (int k = 1; k <= nz; k++) u1d[k] = uref;
+      } else if (iv == "hcp") {
+        poiseuille(2.0 * l[2], ubulk);
+        is_mean = true;
+      } else if (iv == "pdc") {
+        const double lref = l[2] / 2.0;
+        ubulk = cs.bforce[0] * lref * lref / (3.0 * cs.visc);
+        poiseuille(l[2], ubulk);
+        is_mean = true;
+      } else if (iv == "tgv") {
+        for (int k = 1; k <= nz; k++)
+          for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+              const double zcl = zc[k] / l[2] * 2.0 * pi, ycl = yc[j] / l[1] * 2.0 * pi, yfl = yf[j] / l[1] * 2.0 * pi;
+              const double xcl = xc[i] / l[0] * 2.0 * pi, xfl = xf[i] / l[0] * 2.0 * pi;
+              u[at(i, j, k)] = std::sin(xfl) * std::cos(ycl) * std::cos(zcl);
+              v[at(i, j, k)] = -std::cos(xcl) * std::sin(yfl) * std::cos(zcl);
+            }
+      } else if (iv == "kov") {
+        const double reb = uref * lref1 / cs.visc;
+        const double lambda = reb / 2 - std::sqrt(reb * reb / 4 + 4 * pi * pi);
+        for (int k = 1; k <= nz; k++)
+          for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+              u[at(i, j, k)] = 1.0 - std::exp(lambda * xf[i]) * std::cos(2 * pi * zc[k]);
+              w[at(i, j, k)] = lambda / (2.0 * pi) * std::exp(lambda * xc[i]) * std::sin(2 * pi * zf[k]);
+            }
+      } else if (iv == "log" || iv == "hcl") {
+        throw std::runtime_error("initial velocity field '" + iv +
+                                 "' adds noise from the Fortran compiler's random_number; set the fields with "
+                                 "snac_upload_field instead");
+      } else {
+        throw std::runtime_error("ERROR: invalid name for initial velocity field");
+      }
+      if (iv != "tgv" && iv != "kov")
+        for (int k = 1; k <= nz; k++)
+          for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) u[at(i, j, k)] = u1d[k];
+      if (is_mean) {  // set_mean (initflow.f90:288-317) with dxc, dyf, dzf over the block
+        double meanold = 0.0;
+        for (int k = 1; k <= nz; k++)
+          for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++)
+              meanold = meanold + u[at(i, j, k)] * B.dc[0][i] * B.df[1][j] * B.df[2][k] / (l[0] * l[1] * l[2]);
+        if (meanold != 0.0)
+          for (double &x : u) x = x / meanold * ubulk;
+      }
+      copy_field(c, (int)b, SNAC_U, nullptr, u.data());
+      copy_field(c, (int)b, SNAC_V, nullptr, v.data());
+      copy_field(c, (int)b, SNAC_W, nullptr, w.data());
+      copy_field(c, (int)b, SNAC_P, nullptr, p.data());
+      // initscal (initflow.f90:190-262)
+      for (int is = 0; is < cs.nscal; is++) {
+        auto bcs = [&](int q, int side) { return B.bcsca[(size_t)is * 6 + q * 2 + side]; };
+        std::vector<double> s(ntot, 0.0), s1d(nz + 2, 0.0);
+        double sref = 0.5 * (bcs(2, 0) + bcs(2, 1));
+        if (cs.is_sforced[is]) sref = cs.scalf[is];
+        const std::string in = cs.iniscal[is].substr(0, 3);
+        if (in == "zer" || in == "dhc") {
+        } else if (in == "uni") {
+          for (int k = 1; k <= nz; k++) s1d[k] = sref;
+        } else if (in == "cou") {
+          for (int k = 1; k <= nz; k++) {
+            s1d[k] = 0.5 * (1.0 - 2.0 * (zc[k] / l[2])) * 1.0 + 0.5;
+            s1d[k] = bcs(2, 0) * s1d[k] + bcs(2, 1) * (1.0 - s1d[k]);
+          }
+        } else {
+          throw std::runtime_error("ERROR: invalid name for initial scalar field");
+        }
+        for (int k = 1; k <= nz; k++)
+          for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) s[at(i, j, k)] = s1d[k];
+        if (in == "dhc")
+          for (int k = 1; k <= nz; k++)
+            for (int j = 1; j <= ny; j++)
+              for (int i = 1; i <= nx; i++) {
+                const double xx = xc[i] / l[0];
+                s[at(i, j, k)] = (1.0 - xx) * bcs(0, 0) + xx * bcs(0, 1);
+              }
+        copy_field(c, (int)b, SNAC_SCAL0 + 2 * is, nullptr, s.data());
+      }
+    }
+  });
+}
+
+int snac_ctx_get_run_params(const snac_ctx *c, int *iparams) {
+  return guard([&] {
+    const Case &cs = checked(const_cast<snac_ctx *>(c))->cs;
+    const int v[8] = {cs.iout0d, cs.iout1d, cs.iout2d, cs.iout3d, cs.isave, cs.nsaves_max, cs.is_overwrite_save ? 1 : 0,
+                      cs.restart ? 1 : 0};
+    std::copy(v, v + 8, iparams);
+  });
+}
+
 int snac_save_grid(const snac_ctx *c, const char *datadir) {
   return guard([&] {
     const Case &cs = checked(const_cast<snac_ctx *>(c))->cs;

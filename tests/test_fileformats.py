@@ -191,3 +191,55 @@ def test_write_visu_3d(tmp_path):
     assert lines == [ff.visu_log_line(f"vez_fld_{s:07d}_b_001.bin", "Velocity_Z", [1, 1, 1], n, [1, 1, 1], 0.01 * s, s)
                      for s in (0, 10)]
     ctx.close()
+
+
+# ------------------------------------------------------------------ driver
+DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "snac_amd", "snac")
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_driver_refuses_bad_inputs(tmp_path):
+    """The main program fails with the reference's sanity message (here: no
+    stopping criterion, sanity.f90:41-51) before it touches a device."""
+    import subprocess
+    d, b = cases.write_case(str(tmp_path), [cases.channel_block((8, 8, 8))], 1000.0)
+    txt = open(d).read().replace("stop_type(1:3) = T, F, F", "stop_type(1:3) = F, F, F")
+    open(d, "w").write(txt)
+    r = subprocess.run([DRIVER, "3", "-1"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "stopping criterion not chosen." in r.stderr
+
+
+@pytest.mark.gpu
+def test_driver_ldc_reference_procedure(tmp_path):
+    """The reference's lid-driven-cavity test (tests/lid_driven_cavity/
+    testit.sh + test.py) with this library's main program in place of the
+    reference binary: run it in a directory holding dns.nml and blocks.nml,
+    then read data/vey_fld_0001500_b_00N.bin the way read_single_field_binary
+    does (sizes from geometry_b_00N.out, Fortran order), assemble the four
+    blocks and compare the mid-line profile with data_ldc_re1000.txt (rtol
+    1e-7, atol 5e-8)."""
+    import shutil
+    import subprocess
+    for f in ("dns.nml", "blocks.nml"):
+        shutil.copy(os.path.join(GOLD, "ldc", f), tmp_path / f)
+    print("driver: ldc, FFT_AXIS=1", flush=True)
+    r = subprocess.run([DRIVER, "1"], cwd=tmp_path, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "*** Fim ***" in r.stdout
+    data = tmp_path / "data"
+    blocks = []
+    for b in range(1, 5):
+        geo = np.loadtxt(data / f"geometry_b_{b:03d}.out", max_rows=4)
+        ng = (geo[1] - geo[0] + 1).astype(int)
+        fld = np.fromfile(data / f"vey_fld_0001500_b_{b:03d}.bin", dtype="float64")
+        blocks.append(fld.reshape(ng, order="F")[0, :, :])
+    full = np.block([[blocks[0], blocks[2]], [blocks[1], blocks[3]]])
+    ref = np.loadtxt(os.path.join(GOLD, "ldc", "data_ldc_re1000.txt"))
+    np.testing.assert_allclose(full[full.shape[1] // 2, :], ref[:, 1], rtol=1e-7, atol=5e-8)
+    # time.out every iout0d = 10 steps: istep, dt, time
+    t = np.loadtxt(data / "time.out")
+    assert t.shape == (150, 3) and t[-1, 0] == 1500
+    # the final checkpoint (isave = 2000 > nstep: written when the run ends)
+    u = np.fromfile(data / "fld_b_001_u.bin", dtype="float64")
+    geo = np.loadtxt(data / "geometry_b_001.out", max_rows=4)
+    assert u.size == int(np.prod(geo[1] - geo[0] + 1)) + 2 and u[-1] == 1500.0
