@@ -96,10 +96,12 @@ void launch_face_ops(hipStream_t s, const FaceOp *dev_ops, int nops, int max_pla
 struct R2RPlan {
   int n = 0, kind = 0;
   bool fft = false;        // LDS FFT path (n a power of two >= 4, supported kind)
-  int lshift = 0;          // FFT tile: lines per workgroup = max >> lshift (0..2, autotuned)
+  int lshift = 0;          // FFT tile (autotuned): 0..2 lines per workgroup = max >> lshift;
+                           // 3, 4 (R2HC/HC2R): full tiles, 2 / 4 per workgroup, pipelined
   const double *tab = nullptr;  // device twiddles (fft) or dense matrix
 };
 R2RPlan r2r_plan(int kind, int n);  // cached per (kind, n); throws if unsupported
+int r2r_tile_choices(const R2RPlan &pl);  // number of tile choices of the plan's kernel
 void launch_r2r(hipStream_t s, const R2RPlan &pl, const double *in, long ies, long is1, long is2,
                 double *out, long oes, long os1, long os2, int nb1, int nb2, double scale);
 // host reference of FFTW's r2r definitions, used to build dense matrices

@@ -580,7 +580,8 @@ void autotune(snac_ctx *c) {
     if (!pl.fft) return;
     int best = 0;
     double bt = 1e300;
-    for (int ls = 0; ls <= 2; ls++) {
+    const int nt = r2r_tile_choices(pl);
+    for (int ls = 0; ls < nt; ls++) {
       pl.lshift = ls;
       double ms = time_stage(c, run);
       if (ms < bt) { bt = ms; best = ls; }
@@ -1500,7 +1501,7 @@ int snac_r2r_lines(snac_ctx *c, int kind, int n, int howmany, int strided, int t
                    double *out) {
   return guard([&] {
     on_device(c);
-    if (tile < 0 || tile > 2) throw std::runtime_error("r2r_lines: tile must be 0..2");
+    if (tile < 0 || tile > 4) throw std::runtime_error("r2r_lines: tile must be 0..4");
     R2RPlan pl = r2r_plan(kind, n);
     pl.lshift = tile;
     size_t tot = (size_t)n * howmany;

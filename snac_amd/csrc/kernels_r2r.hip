@@ -255,36 +255,62 @@ constexpr int rfft_lines(int n1, int n2) {
                                                                     : (LDS_COMPLEX / (2 * n1 * n2));
 }
 
-template <int N1, int N2, bool INV, int LSHIFT>
+// TPB tiles of L lines per workgroup: the loads of tile t+1 are issued
+// before tile t is transformed and land in LDS after it, so their latency
+// overlaps the arithmetic (TPB = 1: one tile, no overlap).
+template <int N1, int N2, bool INV, int LSHIFT, int TPB>
 __global__ __launch_bounds__(NT) void k_rfft(Args a) {
   extern __shared__ double2 S[];
   constexpr int h = N1 * N2, n = 2 * h;
   constexpr int L = (rfft_lines(N1, N2) >> LSHIFT) > 0 ? (rfft_lines(N1, N2) >> LSHIFT) : 1;
   constexpr int IT = (L * n + NT - 1) / NT;
-  const int line0 = blockIdx.x * L;
   const double2 *tw = reinterpret_cast<const double2 *>(a.tab);  // e^{-2 pi i m/n}, m < n
   constexpr int LS = n + RPAD;                                     // LDS doubles per line
   double *R = reinterpret_cast<double *>(S);                     // line l at doubles [LS l, LS l + n)
-  __shared__ long oin[L], oout[L];
-  line_offsets<L>(a, line0, oin, oout);
-  // 1. raw lines into LDS
-  {
-    double xr[IT];
+  __shared__ long oin[2][L], oout[2][L];
+  const int ntile = (a.nlines + L - 1) / L, tile0 = blockIdx.x * TPB;
+  auto offsets = [&](int slot, int tile) {
+    for (int t = threadIdx.x; t < L; t += NT) {
+      const int line = tile * L + t;
+      long oi = -1, oo = -1;
+      if (line < a.nlines) {
+        const int b1 = line % a.nb1, b2 = line / a.nb1;
+        oi = b1 * a.is1 + b2 * a.is2;
+        oo = b1 * a.os1 + b2 * a.os2;
+      }
+      oin[slot][t] = oi;
+      oout[slot][t] = oo;
+    }
+  };
+  double xr[IT];
+  auto load = [&](int slot) {
 #pragma unroll
     for (int it = 0; it < IT; it++) {
       const int e = threadIdx.x + it * NT;
       const int l = a.coal ? e % L : e / n, q = a.coal ? e / L : e % n;
       xr[it] = 0.0;
-      if (e < L * n && oin[l] >= 0) xr[it] = a.in[q * a.ies + oin[l]];
+      if (e < L * n && oin[slot][l] >= 0) xr[it] = a.in[q * a.ies + oin[slot][l]];
     }
-#pragma unroll
-    for (int it = 0; it < IT; it++) {
-      const int e = threadIdx.x + it * NT;
-      const int l = a.coal ? e % L : e / n, q = a.coal ? e / L : e % n;
-      if (e < L * n) R[LS * l + q] = xr[it];
-    }
-  }
+  };
+  offsets(0, tile0);
   __syncthreads();
+  load(0);
+  for (int tt = 0; tt < TPB; tt++) {
+  const int tile = tile0 + tt;
+  if (tile >= ntile) break;  // uniform over the workgroup
+  const int sl = tt & 1;
+  const bool more = tt + 1 < TPB && tile + 1 < ntile;
+  __syncthreads();  // the previous tile is done with the LDS lines and its offsets
+  // 1. raw lines into LDS
+#pragma unroll
+  for (int it = 0; it < IT; it++) {
+    const int e = threadIdx.x + it * NT;
+    const int l = a.coal ? e % L : e / n, q = a.coal ? e / L : e % n;
+    if (e < L * n) R[LS * l + q] = xr[it];
+  }
+  if (more) offsets(sl ^ 1, tile + 1);
+  __syncthreads();
+  if (more) load(sl ^ 1);
   // 2. first pass: N1-point DFTs over j1 for every (line, j2)
   {
     const int t = threadIdx.x;
@@ -350,7 +376,7 @@ __global__ __launch_bounds__(NT) void k_rfft(Args a) {
   for (int it = 0; it < IT; it++) {
     const int e = threadIdx.x + it * NT;
     const int l = a.coal ? e % L : e / n, q = a.coal ? e / L : e % n;
-    if (e < L * n && oout[l] >= 0) {
+    if (e < L * n && oout[sl][l] >= 0) {
       const double2 *Z = S + (LS / 2) * l;
       double y;
       if (!INV) {
@@ -365,8 +391,9 @@ __global__ __launch_bounds__(NT) void k_rfft(Args a) {
         const double2 z = Z[q >> 1];
         y = (q & 1) ? z.y : z.x;
       }
-      a.out[q * a.oes + oout[l]] = y * a.scale;
+      a.out[q * a.oes + oout[sl][l]] = y * a.scale;
     }
+  }
   }
 }
 
@@ -443,12 +470,17 @@ void launch_fft_kind(int kind, int n, int ls, dim3 grid, size_t lds, hipStream_t
   }
 }
 
+// tile 0..2: 2^-tile of the lines per workgroup; 3 / 4: full tiles, 2 / 4 per workgroup
+int rfft_tpb(int tile) { return tile == 3 ? 2 : (tile == 4 ? 4 : 1); }
+
 template <int N1, int N2>
-void launch_rfft_sized(bool inv, int ls, dim3 grid, size_t lds, hipStream_t s, const Args &a) {
-#define SNAC_RF(I, LS) hipLaunchKernelGGL((k_rfft<N1, N2, I, LS>), grid, dim3(NT), lds, s, a)
-  if (ls == 0) { if (inv) SNAC_RF(true, 0); else SNAC_RF(false, 0); }
-  else if (ls == 1) { if (inv) SNAC_RF(true, 1); else SNAC_RF(false, 1); }
-  else if (ls == 2) { if (inv) SNAC_RF(true, 2); else SNAC_RF(false, 2); }
+void launch_rfft_sized(bool inv, int tile, dim3 grid, size_t lds, hipStream_t s, const Args &a) {
+#define SNAC_RF(I, LS, T) hipLaunchKernelGGL((k_rfft<N1, N2, I, LS, T>), grid, dim3(NT), lds, s, a)
+  if (tile == 0) { if (inv) SNAC_RF(true, 0, 1); else SNAC_RF(false, 0, 1); }
+  else if (tile == 1) { if (inv) SNAC_RF(true, 1, 1); else SNAC_RF(false, 1, 1); }
+  else if (tile == 2) { if (inv) SNAC_RF(true, 2, 1); else SNAC_RF(false, 2, 1); }
+  else if (tile == 3) { if (inv) SNAC_RF(true, 0, 2); else SNAC_RF(false, 0, 2); }
+  else if (tile == 4) { if (inv) SNAC_RF(true, 0, 4); else SNAC_RF(false, 0, 4); }
   else throw std::runtime_error("r2r: bad tile choice");
 #undef SNAC_RF
 }
@@ -497,6 +529,11 @@ double r2r_coef(int kind, int n, int k, int j) {
   case K_RODFT11: return 2.0 * std::sin(pi * (j + 0.5) * (k + 0.5) / n);
   default: throw std::runtime_error("unsupported r2r kind " + std::to_string(kind));
   }
+}
+
+int r2r_tile_choices(const R2RPlan &pl) {
+  if (!pl.fft) return 1;
+  return (pl.kind == K_R2HC || pl.kind == K_HC2R) && pl.n >= 8 ? 5 : 3;
 }
 
 R2RPlan r2r_plan(int kind, int n) {
@@ -550,23 +587,26 @@ void launch_r2r(hipStream_t s, const R2RPlan &pl, const double *in, long ies, lo
   if (pl.fft && (pl.kind == K_R2HC || pl.kind == K_HC2R) && pl.n >= 8) {
     int n1 = 0, n2 = 0;
     fft_split(pl.n / 2, n1, n2);
-    const int L = std::max(1, rfft_lines(n1, n2) >> pl.lshift);  // the kernel's compile-time line count
+    const int ls = pl.lshift <= 2 ? pl.lshift : 0, tpb = rfft_tpb(pl.lshift);
+    const int L = std::max(1, rfft_lines(n1, n2) >> ls);  // the kernel's compile-time line count
     a.L = L;
     a.scale = scale;
     a.tab = pl.tab;
-    dim3 grid((unsigned)((a.nlines + L - 1) / L));
+    const int ntile = (a.nlines + L - 1) / L;
+    dim3 grid((unsigned)((ntile + tpb - 1) / tpb));
     size_t lds = (size_t)L * (pl.n + RPAD) * sizeof(double);
     launch_rfft(pl.kind == K_HC2R, pl.n, pl.lshift, grid, lds, s, a);
   } else if (pl.fft) {
     int n1 = 0, n2 = 0;
     fft_split(pl.n, n1, n2);
-    const int L = std::max(1, fft_lines(n1, n2) >> pl.lshift);  // the kernel's compile-time line count
+    const int ls = pl.lshift <= 2 ? pl.lshift : 0;
+    const int L = std::max(1, fft_lines(n1, n2) >> ls);  // the kernel's compile-time line count
     a.L = L;
     a.scale = scale;
     a.tab = pl.tab;
     dim3 grid((unsigned)((a.nlines + L - 1) / L));
     size_t lds = (size_t)L * pl.n * sizeof(double2);
-    launch_fft_kind(pl.kind, pl.n, pl.lshift, grid, lds, s, a);
+    launch_fft_kind(pl.kind, pl.n, ls, grid, lds, s, a);
   } else {
     int L = std::max(1, std::min(2 * LDS_COMPLEX / pl.n, a.nlines));
     a.L = L;
