@@ -213,11 +213,15 @@ int snac_get_timing(snac_ctx *ctx, double *ms, int *count);
  * the best times in ms.  No
  * counterpart in the reference (its FFTW plans use FFTW_ESTIMATE).          */
 int snac_get_autotune(snac_ctx *ctx, int *choices, double *ms, int n);
-/* overrides an autotuned kernel variant: SNAC_KV_RK_MOM (0..10, as above) or
+/* overrides an autotuned kernel variant: SNAC_KV_RK_MOM (0..10, as above),
  * SNAC_KV_TRIDIAG (0: c' in LDS, lines up to 128; 1: c' in scratch; 2:
- * chunked, no scratch, lines of 8..256); every variant computes the same
- * result.  No counterpart in the reference.                                 */
-enum { SNAC_KV_RK_MOM = 0, SNAC_KV_TRIDIAG = 1 };
+ * chunked, no scratch, lines of 8..256) or SNAC_KV_GHOST (0: ghost fills as
+ * one launch per phase of the reference's fill order; 1, the default: each
+ * fill composed at set-up into one launch that computes every ghost cell's
+ * final value through the same chain of operations, where the fill allows
+ * it); every variant computes the same result.  No counterpart in the
+ * reference.                                                                */
+enum { SNAC_KV_RK_MOM = 0, SNAC_KV_TRIDIAG = 1, SNAC_KV_GHOST = 2 };
 int snac_set_kernel_variant(snac_ctx *ctx, int kernel, int variant);
 
 /* ---- building blocks, exposed for their own parity tests ----

@@ -319,6 +319,11 @@ class Context:
         """Overrides the autotuned tridiagonal kernel variant (0..2)."""
         _check(lib.snac_set_kernel_variant(self.h, 1, int(v)), "set_kernel_variant")
 
+    def set_ghost_variant(self, v):
+        """Ghost fills: 0 one launch per phase, 1 (default) composed into one
+        launch per fill where the fill allows it."""
+        _check(lib.snac_set_kernel_variant(self.h, 2, int(v)), "set_kernel_variant")
+
     def new_dt(self):
         """main.f90:660-661: dt = dt_f if > 0 else min(cfl*dt_cfl, dtmax)."""
         if self.dt_f > 0:

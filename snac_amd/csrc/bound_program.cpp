@@ -13,6 +13,8 @@
 #include "bound_program.hpp"
 
 #include <algorithm>
+#include <cstring>
+#include <unordered_map>
 
 namespace snac {
 namespace {
@@ -272,6 +274,150 @@ std::vector<std::vector<FaceOp>> build_correc_fill_program(const Case &cs, const
   std::vector<Rec> ps = pressure_ops(cs, bl, 0);
   seq.insert(seq.end(), ps.begin(), ps.end());
   return merge_and_pack(seq);
+}
+
+namespace {
+
+long plane_index(int d, int q, int t1, int t2, long sx, long sy) {
+  if (d == 0) return (long)q + sx * ((long)t1 + sy * (long)t2);
+  if (d == 1) return (long)t1 + sx * ((long)q + sy * (long)t2);
+  return (long)t1 + sx * ((long)t2 + sy * (long)q);
+}
+
+struct CellKey {
+  const void *a;
+  long i;
+  bool operator==(const CellKey &o) const { return a == o.a && i == o.i; }
+};
+struct CellKeyHash {
+  size_t operator()(const CellKey &k) const {
+    return std::hash<const void *>()(k.a) ^ (std::hash<long>()(k.i) * 0x9E3779B97F4A7C15ull);
+  }
+};
+
+// the chain of one written cell, with the op (and plane row) that wrote it last
+struct Chain {
+  const double *base;
+  long bidx;
+  int nlev;
+  CellLev lev[SEG_LEV];
+  double *dst;
+  long didx;
+  int op, t1, t2;
+};
+
+bool same_levels(const Chain &a, const Chain &b) {
+  if (a.nlev != b.nlev || (a.base == nullptr) != (b.base == nullptr) || a.base != b.base) return false;
+  for (int l = 0; l < a.nlev; l++) {
+    const CellLev &x = a.lev[l], &y = b.lev[l];
+    if (std::memcmp(&x.c0, &y.c0, sizeof(double)) || std::memcmp(&x.c1, &y.c1, sizeof(double)) ||
+        std::memcmp(&x.c2, &y.c2, sizeof(double)) || x.arr != y.arr || x.src != y.src)
+      return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+bool compose_program(const std::vector<std::vector<FaceOp>> &phases, std::vector<CellSeg> &segs) {
+  std::unordered_map<CellKey, int, CellKeyHash> at;
+  std::vector<Chain> ch;
+  size_t ncell = 0;
+  for (auto &ph : phases)
+    for (auto &o : ph) ncell += (size_t)o.m1 * o.m2;
+  at.reserve(ncell);
+  ch.reserve(ncell);
+  int opi = 0;
+  for (auto &ph : phases)
+    for (auto &o : ph) {
+      opi++;
+      for (int t2 = 0; t2 < o.m2; t2++)
+        for (int t1 = 0; t1 < o.m1; t1++) {
+          Chain e;
+          const long t = t1 + (long)o.m1 * t2;
+          const CellLev lv{o.c0, o.c1, o.c2, o.arr, o.arr ? t : 0, o.src ? 1 : 0};
+          if (o.src) {
+            const long sidx = plane_index(o.d, o.qsrc, t1, t2, o.ssx, o.ssy);
+            auto it = at.find({o.src, sidx});
+            if (it != at.end()) {
+              e = ch[it->second];
+            } else {
+              e.base = o.src;
+              e.bidx = sidx;
+              e.nlev = 0;
+            }
+            if (e.nlev == SEG_LEV) return false;
+            e.lev[e.nlev++] = lv;
+          } else {
+            e.base = nullptr;
+            e.bidx = 0;
+            e.nlev = 1;
+            e.lev[0] = lv;
+          }
+          e.dst = o.dst;
+          e.didx = plane_index(o.d, o.qdst, t1, t2, o.dsx, o.dsy);
+          e.op = opi;
+          e.t1 = t1;
+          e.t2 = t2;
+          auto it = at.find({o.dst, e.didx});
+          if (it == at.end()) {
+            at.emplace(CellKey{o.dst, e.didx}, (int)ch.size());
+            ch.push_back(e);
+          } else {
+            ch[it->second] = e;
+          }
+        }
+    }
+  // a final value that read a cell the program writes (after the read) would
+  // race with that write in one launch
+  for (const Chain &e : ch)
+    if (e.base && at.count({e.base, e.bidx})) return false;
+  std::vector<int> ord(ch.size());
+  for (size_t q = 0; q < ch.size(); q++) ord[q] = (int)q;
+  std::sort(ord.begin(), ord.end(), [&](int x, int y) {
+    const Chain &a = ch[x], &b = ch[y];
+    if (a.op != b.op) return a.op < b.op;
+    if (a.t2 != b.t2) return a.t2 < b.t2;
+    return a.t1 < b.t1;
+  });
+  std::vector<CellSeg> out;
+  size_t q = 0;
+  while (q < ord.size()) {
+    const Chain &f = ch[ord[q]];
+    size_t r = q + 1;
+    long dstr = 0, bstr = 0;
+    while (r < ord.size()) {
+      const Chain &prev = ch[ord[r - 1]], &e = ch[ord[r]];
+      if (e.op != f.op || e.t2 != f.t2 || e.t1 != prev.t1 + 1 || e.dst != f.dst || !same_levels(f, e)) break;
+      const long ds = e.didx - prev.didx, bs = e.base ? e.bidx - prev.bidx : 0;
+      if (r == q + 1) {
+        dstr = ds;
+        bstr = bs;
+      } else if (ds != dstr || bs != bstr) {
+        break;
+      }
+      bool arr_ok = true;
+      for (int l = 0; l < e.nlev; l++)
+        if (e.lev[l].arr && e.lev[l].a0 != prev.lev[l].a0 + 1) arr_ok = false;
+      if (!arr_ok) break;
+      r++;
+    }
+    CellSeg g;
+    std::memset(&g, 0, sizeof g);
+    g.dst = f.dst;
+    g.d0 = f.didx;
+    g.dstr = dstr;
+    g.base = f.base;
+    g.b0 = f.bidx;
+    g.bstr = bstr;
+    g.len = (int)(r - q);
+    g.nlev = f.nlev;
+    for (int l = 0; l < f.nlev; l++) g.lev[l] = f.lev[l];
+    out.push_back(g);
+    q = r;
+  }
+  segs.swap(out);
+  return true;
 }
 
 }  // namespace snac

@@ -31,5 +31,11 @@ std::vector<std::vector<FaceOp>> build_scalar_program(const Case &cs, const std:
 // a substage (main.f90:1026-1032), merged into shared phases
 std::vector<std::vector<FaceOp>> build_correc_fill_program(const Case &cs, const std::vector<BlockArrays> &bl,
                                                            int par);
+// Composes a phased program into cell segments that one launch can run:
+// every written cell's chain of ops is followed back to a cell the program
+// does not write.  False (segs untouched) when a final value depends on a
+// cell the program writes later (it read the old value there) or a chain is
+// longer than SEG_LEV; the phased program must run then.
+bool compose_program(const std::vector<std::vector<FaceOp>> &phases, std::vector<CellSeg> &segs);
 
 }  // namespace snac

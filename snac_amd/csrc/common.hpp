@@ -45,6 +45,29 @@ struct FaceOp {
   double c0, c1, c2;
 };
 
+// A ghost-fill program composed down to its final values: each ghost cell
+// the program writes is one chain  v = base cell (or 0); for each level
+// v = c0 + c1 * v + c2 * arr[a0 + i]  -- the ops that produced the cell in
+// program order, evaluated in that order (so the values are the phased
+// program's bit for bit).  A segment is a run of such cells along one row of
+// one op's plane with the same chain shape and constant index steps.
+constexpr int SEG_LEV = 4;
+struct CellLev {
+  double c0, c1, c2;
+  const double *arr;  // null: no array term
+  long a0;            // arr index of the segment's first cell (step 1)
+  int src;            // 0: the level does not read the previous value
+};
+struct CellSeg {
+  double *dst;
+  long d0, dstr;
+  const double *base;  // null: the chain starts from a level with src == 0
+  long b0, bstr;
+  int len, nlev;
+  CellLev lev[SEG_LEV];
+};
+void launch_cell_segs(hipStream_t s, const CellSeg *dev_segs, int nseg);
+
 // ---- stencil kernels (kernels_stencil.hip) ----
 // rk_mom: reads u,v,w,p (ghosted) and dudtrko; writes the updated interior
 // velocities to un,vn,wn and dudtrk to dto.

@@ -40,6 +40,8 @@ struct DevBlock {
 struct Program {
   std::vector<FaceOp *> dev;  // one device array per phase
   std::vector<int> nops, maxplane;
+  CellSeg *seg = nullptr;     // the same program composed into one launch (null: not composable)
+  int nseg = 0;
 };
 
 // Tensor-product Poisson plan: r2r along every uniform axis (the FFT axis
@@ -82,6 +84,7 @@ struct snac_ctx {
   bool mg_fixed = false;  // solves run a fixed iteration count (graph capture)
   int solver = SNAC_SOLVER_AUTO;
   int prec = SNAC_PREC_FP64;
+  bool ghost_fused = true;  // run composed ghost-fill programs in one launch
   double *part_sum = nullptr, *part_max = nullptr, *red = nullptr;
   int nparts = 0;
   double dlmin = 0;
@@ -160,10 +163,20 @@ Program upload_program(snac_ctx *c, const std::vector<std::vector<FaceOp>> &phas
     pr.nops.push_back((int)ph.size());
     pr.maxplane.push_back(mp);
   }
+  std::vector<CellSeg> segs;
+  if (phases.size() > 1 && compose_program(phases, segs) && !segs.empty()) {
+    pr.seg = dalloc<CellSeg>(c, segs.size());
+    SNAC_HIP(hipMemcpy(pr.seg, segs.data(), segs.size() * sizeof(CellSeg), hipMemcpyHostToDevice));
+    pr.nseg = (int)segs.size();
+  }
   return pr;
 }
 
 void run_program(snac_ctx *c, const Program &pr) {
+  if (pr.nseg > 0 && c->ghost_fused) {
+    launch_cell_segs(c->stream, pr.seg, pr.nseg);
+    return;
+  }
   for (size_t q = 0; q < pr.dev.size(); q++) launch_face_ops(c->stream, pr.dev[q], pr.nops[q], pr.maxplane[q]);
 }
 
@@ -1488,6 +1501,9 @@ int snac_set_kernel_variant(snac_ctx *c, int kernel, int variant) {
       if (variant < 0 || variant > 2 || (variant == 0 && m > 128) || (variant == 2 && tridiag_chunks(m) == 0))
         throw std::runtime_error("tridiagonal variant not available for lines of " + std::to_string(m));
       c->tp.tvariant = variant;
+    } else if (kernel == SNAC_KV_GHOST) {
+      if (variant < 0 || variant > 1) throw std::runtime_error("ghost-fill variant must be 0 or 1");
+      c->ghost_fused = variant == 1;
     } else {
       throw std::runtime_error("unknown kernel id");
     }

@@ -24,7 +24,29 @@ __global__ __launch_bounds__(256) void k_face_ops(const FaceOp *__restrict__ ops
   op.dst[plane_idx(op.d, op.qdst, t1, t2, op.dsx, op.dsy)] = val;
 }
 
+// one workgroup per segment of a composed ghost program
+__global__ __launch_bounds__(256) void k_cell_segs(const CellSeg *__restrict__ segs) {
+  const CellSeg &g = segs[blockIdx.x];
+  for (int i = threadIdx.x; i < g.len; i += 256) {
+    double v = g.base ? g.base[g.b0 + (long)i * g.bstr] : 0.0;
+    for (int l = 0; l < g.nlev; l++) {
+      const CellLev &L = g.lev[l];
+      double val = L.c0;
+      if (L.src) val += L.c1 * v;
+      if (L.arr) val += L.c2 * L.arr[L.a0 + i];
+      v = val;
+    }
+    g.dst[g.d0 + (long)i * g.dstr] = v;
+  }
+}
+
 }  // namespace
+
+void launch_cell_segs(hipStream_t s, const CellSeg *dev_segs, int nseg) {
+  if (nseg <= 0) return;
+  hipLaunchKernelGGL(k_cell_segs, dim3(nseg), dim3(256), 0, s, dev_segs);
+  SNAC_HIP(hipGetLastError());
+}
 
 void launch_face_ops(hipStream_t s, const FaceOp *dev_ops, int nops, int max_plane) {
   if (nops <= 0) return;
