@@ -296,18 +296,6 @@ __global__ __launch_bounds__(256) void k_prolong(MGLevT<T> F, MGLevT<T> Cl, cons
   xf[idx] += xc[vidx(Cl, M, I, s)];
 }
 
-__global__ __launch_bounds__(256) void k_apply(MGLev L, const double *__restrict__ lam, const double *__restrict__ x,
-                                               double *__restrict__ out, const int *__restrict__ done) {
-  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
-  const long n = (long)L.nm * L.na * L.ns;
-  if (idx >= n) return;
-  const int s = (int)(idx % L.ns);
-  const long t = idx / L.ns;
-  const int a = (int)(t % L.na), m = (int)(t / L.na);
-  if (done && done[group_of(L, m)]) return;
-  out[idx] = mx_at(L, lam ? lam[m] : 0.0, x, m, a, s);
-}
-
 template <typename T>
 __global__ __launch_bounds__(256) void k_zero(long n, T *__restrict__ x) {
   const long idx = (long)blockIdx.x * 256 + threadIdx.x;
@@ -357,6 +345,73 @@ __global__ __launch_bounds__(256) void k_dot(long gsize, const double *__restric
   }
 }
 
+// the CG's operator application and its first dot product in one pass:
+// q = M p and the per-group partial sums of p.q, laid out as k_dot's (same
+// element order per thread, so the sums are k_dot's to the bit)
+__global__ __launch_bounds__(256) void k_apply_dot(MGLev L, const double *__restrict__ lam,
+                                                   const double *__restrict__ p, double *__restrict__ q,
+                                                   double *__restrict__ part, int nchunk, long gsize,
+                                                   const int *__restrict__ done) {
+  __shared__ double sh[4];
+  const int g = blockIdx.y;
+  double s1 = 0.0;
+  if (!(done && done[g])) {
+    const long base = (long)g * gsize;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < gsize; i += (long)nchunk * 256) {
+      const long idx = base + i;
+      const int s = (int)(idx % L.ns);
+      const long t = idx / L.ns;
+      const int a = (int)(t % L.na), m = (int)(t / L.na);
+      const double y = mx_at(L, lam ? lam[m] : 0.0, p, m, a, s);
+      q[idx] = y;
+      s1 += p[idx] * y;
+    }
+  }
+  s1 = wave_sum(s1);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s1;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[((long)g * nchunk + blockIdx.x) * 2] = sh[0] + sh[1] + sh[2] + sh[3];
+    part[((long)g * nchunk + blockIdx.x) * 2 + 1] = 0.0;
+  }
+}
+
+// x += alpha p ; r -= alpha q (alpha = rz / pq per group) with the partial
+// sums of r.r and, when zf is given, zf.r (the flexible-CG cross term), and
+// the FP32 copy of r when rf is given
+__global__ __launch_bounds__(256) void k_cg_update_dot(long gsize, const double *__restrict__ rz,
+                                                       const double *__restrict__ pq, double *__restrict__ x,
+                                                       const double *__restrict__ p, double *__restrict__ r,
+                                                       const double *__restrict__ q, float *__restrict__ rf,
+                                                       const float *__restrict__ zf, double *__restrict__ part,
+                                                       int nchunk, const int *__restrict__ done) {
+  __shared__ double sh[2][4];
+  const int g = blockIdx.y;
+  double s1 = 0.0, s2 = 0.0;
+  if (!done[g]) {
+    const double den = pq[g * 2];
+    const double al = den != 0.0 ? rz[g * 2] / den : 0.0;
+    const long base = (long)g * gsize;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < gsize; i += (long)nchunk * 256) {
+      const long idx = base + i;
+      x[idx] += al * p[idx];
+      const double rn = r[idx] - al * q[idx];
+      r[idx] = rn;
+      s1 += rn * rn;
+      if (rf) rf[idx] = (float)rn;
+      if (zf) s2 += (double)zf[idx] * rn;
+    }
+  }
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  if ((threadIdx.x & 63) == 0) { sh[0][threadIdx.x >> 6] = s1; sh[1][threadIdx.x >> 6] = s2; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[((long)g * nchunk + blockIdx.x) * 2] = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
+    part[((long)g * nchunk + blockIdx.x) * 2 + 1] = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+  }
+}
+
 // one wavefront per group sums its chunks (skipped groups keep their values)
 __global__ __launch_bounds__(64) void k_dot_final(const double *__restrict__ part, int nchunk, double *__restrict__ out,
                                                   const int *__restrict__ done, const int *__restrict__ only) {
@@ -370,23 +425,6 @@ __global__ __launch_bounds__(64) void k_dot_final(const double *__restrict__ par
   a = wave_sum(a);
   b = wave_sum(b);
   if (threadIdx.x == 0) { out[g * 2] = a; out[g * 2 + 1] = b; }
-}
-
-__global__ __launch_bounds__(256) void k_cg_update(int ng, long gsize, const double *__restrict__ rz,
-                                                   const double *__restrict__ pq, double *__restrict__ x,
-                                                   const double *__restrict__ p, double *__restrict__ r,
-                                                   const double *__restrict__ q, float *__restrict__ rf,
-                                                   const int *__restrict__ done) {
-  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= (long)ng * gsize) return;
-  const int g = (int)(idx / gsize);
-  if (done[g]) return;
-  const double den = pq[g * 2];
-  const double al = den != 0.0 ? rz[g * 2] / den : 0.0;
-  x[idx] += al * p[idx];
-  const double rn = r[idx] - al * q[idx];
-  r[idx] = rn;
-  if (rf) rf[idx] = (float)rn;
 }
 
 template <typename T>
@@ -543,15 +581,27 @@ void mg_prolong(hipStream_t st, const MGLevT<T> &F, const MGLevT<T> &C, const T 
   SNAC_HIP(hipGetLastError());
 }
 
-void mg_apply(hipStream_t st, const MGLev &L, const double *lam, const double *x, double *out, const int *done) {
-  const long n = (long)L.nm * L.na * L.ns;
-  hipLaunchKernelGGL(k_apply, dim3(blocks_for(n)), dim3(256), 0, st, L, lam, x, out, done);
-  SNAC_HIP(hipGetLastError());
-}
-
 template <typename T>
 void mg_zero(hipStream_t st, long n, T *x) {
   hipLaunchKernelGGL(k_zero<T>, dim3(blocks_for(n)), dim3(256), 0, st, n, x);
+  SNAC_HIP(hipGetLastError());
+}
+
+void mg_apply_dot(hipStream_t st, const MGLev &L, const double *lam, const double *p, double *q, int ng, long gsize,
+                  double *part, int nchunk, double *out, const int *done) {
+  hipLaunchKernelGGL(k_apply_dot, dim3(nchunk, ng), dim3(256), 0, st, L, lam, p, q, part, nchunk, gsize, done);
+  SNAC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_dot_final, dim3(ng), dim3(64), 0, st, part, nchunk, out, done, nullptr);
+  SNAC_HIP(hipGetLastError());
+}
+
+void mg_cg_update_dot(hipStream_t st, int ng, long gsize, const double *rz, const double *pq, double *x,
+                      const double *p, double *r, const double *q, float *rf, const float *zf, double *part,
+                      int nchunk, double *out, const int *done) {
+  hipLaunchKernelGGL(k_cg_update_dot, dim3(nchunk, ng), dim3(256), 0, st, gsize, rz, pq, x, p, r, q, rf, zf, part,
+                     nchunk, done);
+  SNAC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_dot_final, dim3(ng), dim3(64), 0, st, part, nchunk, out, done, nullptr);
   SNAC_HIP(hipGetLastError());
 }
 
@@ -571,13 +621,6 @@ void mg_dot(hipStream_t st, int ng, long gsize, const double *x, const TY *y, co
   hipLaunchKernelGGL((k_dot<TY, TX2>), dim3(nchunk, ng), dim3(256), 0, st, gsize, x, y, x2, y2, part, nchunk, done, only);
   SNAC_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_dot_final, dim3(ng), dim3(64), 0, st, part, nchunk, out, done, only);
-  SNAC_HIP(hipGetLastError());
-}
-
-void mg_cg_update(hipStream_t st, int ng, long gsize, const double *rz, const double *pq, double *x, const double *p,
-                  double *r, const double *q, float *rf, const int *done) {
-  hipLaunchKernelGGL(k_cg_update, dim3(blocks_for((long)ng * gsize)), dim3(256), 0, st, ng, gsize, rz, pq, x, p, r, q,
-                     rf, done);
   SNAC_HIP(hipGetLastError());
 }
 

@@ -86,7 +86,16 @@ template <typename T>
 void mg_prolong(hipStream_t st, const MGLevT<T> &F, const MGLevT<T> &C, const T *xc, T *xf, const int *done);
 template <typename T>
 void mg_zero(hipStream_t st, long n, T *x);
-void mg_apply(hipStream_t st, const MGLev &L, const double *lam, const double *x, double *out, const int *done);
+// q = M p with out[g*2] = p.q per group, in one pass over the vectors (the
+// per-group sums in mg_dot's order)
+void mg_apply_dot(hipStream_t st, const MGLev &L, const double *lam, const double *p, double *q, int ng, long gsize,
+                  double *part, int nchunk, double *out, const int *done);
+// x += alpha p ; r -= alpha q with alpha = rz/pq per group, rf (when given)
+// = the new r rounded to FP32, and out[g*2] = r.r, out[g*2+1] = zf.r (0
+// when zf is null) of the new r, in one pass
+void mg_cg_update_dot(hipStream_t st, int ng, long gsize, const double *rz, const double *pq, double *x,
+                      const double *p, double *r, const double *q, float *rf, const float *zf, double *part,
+                      int nchunk, double *out, const int *done);
 void mg_izero(hipStream_t st, long n, int *x);
 // dst = (float) src
 void mg_to_float(hipStream_t st, long n, const double *src, float *dst);
@@ -96,10 +105,6 @@ void mg_to_float(hipStream_t st, long n, const double *src, float *dst);
 template <typename TY, typename TX2 = TY>
 void mg_dot(hipStream_t st, int ng, long gsize, const double *x, const TY *y, const TX2 *x2, const double *y2,
             double *part, int nchunk, double *out, const int *done, const int *only);
-// x += alpha p ; r -= alpha q  with alpha = rz/pq per group; rf (when
-// given) receives the new r rounded to FP32
-void mg_cg_update(hipStream_t st, int ng, long gsize, const double *rz, const double *pq, double *x, const double *p,
-                  double *r, const double *q, float *rf, const int *done);
 // p = z + beta p with beta = (rzn - cross) / rzo, cross = r_new . z_old in
 // slot 1 of `cross` (flexible CG) or 0 when cross is null; beta = 0 when rzo
 // is null

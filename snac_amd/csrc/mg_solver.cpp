@@ -526,14 +526,10 @@ void mg_solve(MGPlan &P, hipStream_t st, int fixed) {
   const int nit = fixed > 0 ? std::min(fixed, P.maxiter) : P.maxiter;
   int it = 0;
   for (it = 1; it <= nit; it++) {
-    mg_apply(st, P.h64.lev[0], P.h64.lam, P.p, P.q, P.done);
-    mg_dot<double, double>(st, P.ng, P.gsize, P.p, P.q, nullptr, nullptr, P.part, P.nchunk, P.pq, P.done, nullptr);
-    mg_cg_update(st, P.ng, P.gsize, P.rz[cur], P.pq, P.x, P.p, P.r, P.q, mixed ? P.rf : nullptr, P.done);
-    // r.r and, for the flexible form, r_new . z_old in the second slot
-    if (mixed)
-      mg_dot<double, float>(st, P.ng, P.gsize, P.r, P.r, P.zf, P.r, P.part, P.nchunk, P.rr, P.done, nullptr);
-    else
-      mg_dot<double, double>(st, P.ng, P.gsize, P.r, P.r, nullptr, nullptr, P.part, P.nchunk, P.rr, P.done, nullptr);
+    mg_apply_dot(st, P.h64.lev[0], P.h64.lam, P.p, P.q, P.ng, P.gsize, P.part, P.nchunk, P.pq, P.done);
+    // the update with r.r and, for the flexible form, r_new . z_old in the second slot
+    mg_cg_update_dot(st, P.ng, P.gsize, P.rz[cur], P.pq, P.x, P.p, P.r, P.q, mixed ? P.rf : nullptr,
+                     mixed ? P.zf : nullptr, P.part, P.nchunk, P.rr, P.done);
     mg_flags(st, P.ng, P.rr, P.bb, tol2, P.done, P.left);
     if (fixed == 0) {
       SNAC_HIP(hipMemcpyAsync(P.left_host, P.left, sizeof(int), hipMemcpyDeviceToHost, st));
