@@ -85,13 +85,16 @@ __device__ __forceinline__ Row<T> pcr(const Row<T> &r, const Row<T> &lo, const R
 // chunks; PCR over the lanes solves it in log2(2*nch) steps; the interior
 // rows follow from the two interface values.
 // relaxes line (m, a) with the calling wavefront (all 64 lanes must call)
-template <int K, typename T>
+// LPL lanes per line (32 or 64): a wavefront relaxes 64 / LPL lines at once,
+// each in its own lane segment (the shuffles never cross segments); act =
+// false leaves the segment idle (no loads or stores).
+template <int K, int LPL, typename T>
 __device__ __forceinline__ void line_solve(const MGLevT<T> &L, T lmv, T *__restrict__ x, const T *__restrict__ b,
-                                           int m, int a, int zero_nb, int pin_s, bool pin) {
-  const int lane = threadIdx.x & 63;
+                                           int m, int a, int zero_nb, int pin_s, bool pin, bool act) {
+  const int wl = threadIdx.x & 63, lane = wl & (LPL - 1), seg0 = wl - lane;
   const long e0 = vidx(L, m, a, 0), q0 = cidx(L, m, a, 0);
   const int ns = L.ns;
-  const int cnt = lane < L.nch ? L.base + (lane < L.extra ? 1 : 0) : 0;
+  const int cnt = act && lane < L.nch ? L.base + (lane < L.extra ? 1 : 0) : 0;
   const int p0 = lane * L.base + min(lane, L.extra);
 
   T lo[K], dg[K], up[K], rh[K];
@@ -111,7 +114,7 @@ __device__ __forceinline__ void line_solve(const MGLevT<T> &L, T lmv, T *__restr
     }
   }
   if (ns == 1) {  // a one-row line: no interface system
-    if (lane == 0) x[e0] = rh[0] / dg[0];
+    if (act && lane == 0) x[e0] = rh[0] / dg[0];
     return;
   }
   // downward elimination: row j (1 <= j < cnt) becomes A x_p0 + x_j + C x_{j+1} = D
@@ -154,8 +157,8 @@ __device__ __forceinline__ void line_solve(const MGLevT<T> &L, T lmv, T *__restr
   }
   // interface system: u[2c] = x_first(c), u[2c+1] = x_last(c); PCR, d = 1
   {
-    const Row<T> r1m = fetch(r1, lane > 0 ? lane - 1 : 0, lane > 0);
-    const Row<T> r0p = fetch(r0, lane < 63 ? lane + 1 : 63, lane < 63);
+    const Row<T> r1m = fetch(r1, seg0 + (lane > 0 ? lane - 1 : 0), lane > 0);
+    const Row<T> r0p = fetch(r0, seg0 + (lane < LPL - 1 ? lane + 1 : LPL - 1), lane < LPL - 1);
     const Row<T> n0 = pcr(r0, r1m, r1);
     const Row<T> n1 = pcr(r1, r0, r0p);
     r0 = n0;
@@ -163,8 +166,8 @@ __device__ __forceinline__ void line_solve(const MGLevT<T> &L, T lmv, T *__restr
   }
   const int nred = 2 * L.nch;
   for (int h = 1; 2 * h < nred; h <<= 1) {  // d = 2h: lanes -+h, same slot
-    const bool vm = lane >= h, vp = lane + h < 64;
-    const int sm = vm ? lane - h : lane, sp = vp ? lane + h : lane;
+    const bool vm = lane >= h, vp = lane + h < LPL;
+    const int sm = seg0 + (vm ? lane - h : lane), sp = seg0 + (vp ? lane + h : lane);
     const Row<T> a0 = fetch(r0, sm, vm), b0 = fetch(r0, sp, vp);
     const Row<T> a1 = fetch(r1, sm, vm), b1 = fetch(r1, sp, vp);
     r0 = pcr(r0, a0, b0);
@@ -179,13 +182,13 @@ __device__ __forceinline__ void line_solve(const MGLevT<T> &L, T lmv, T *__restr
     if (j <= cnt - 2) x[e0 + p0 + j] = D[j] - A[j] * xp - C[j] * xq;
 }
 
-template <int K, typename T>
+template <int K, int LPL, typename T>
 __global__ __launch_bounds__(256) void k_line(MGLevT<T> L, const T *__restrict__ lam, T *__restrict__ x,
                                               const T *__restrict__ b, int col_a, int col_m, int nla, int nlm,
                                               int zero_nb, int pin_s, const int *__restrict__ singular,
                                               const int *__restrict__ done) {
-  const long line = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (line >= (long)nla * nlm) return;
+  const long line = ((long)blockIdx.x * 256 + threadIdx.x) / LPL;
+  if (line >= (long)nla * nlm) return;  // a whole lane segment leaves together
   const int a = 2 * (int)(line % nla) + col_a;
   const int lm_ = (int)(line / nla);
   const int m = L.coupled ? 2 * lm_ + col_m : lm_;
@@ -193,7 +196,7 @@ __global__ __launch_bounds__(256) void k_line(MGLevT<T> L, const T *__restrict__
   if (done && done[g]) return;
   const T lmv = lam ? lam[m] : T(0);
   const bool pin = pin_s >= 0 && singular && singular[g];
-  line_solve<K, T>(L, lmv, x, b, m, a, zero_nb, pin_s, pin);
+  line_solve<K, LPL, T>(L, lmv, x, b, m, a, zero_nb, pin_s, pin, true);
 }
 
 // The coarse end of the V-cycle (levels t.l0 .. t.nl-1, the last one a
@@ -201,19 +204,25 @@ __global__ __launch_bounds__(256) void k_line(MGLevT<T> L, const T *__restrict__
 // wavefront per line, residual restriction and prolongation with the whole
 // workgroup, barriers between the steps.  One launch replaces six per level.
 // FFT path only (independent modes).
-template <int K, typename T>
+template <int K, int LPL, typename T>
 __global__ __launch_bounds__(1024) void k_tail(MGTailT<T> t, const T *__restrict__ lam, int pin_s,
                                                const int *__restrict__ singular, const int *__restrict__ done) {
   const int m = blockIdx.x;
   if (done && done[m]) return;
   const T lmv = lam[m];
   const int wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  constexpr int SPW = 64 / LPL;                   // lines per wavefront
+  const int hs = (threadIdx.x & 63) / LPL;        // this lane's segment
   const bool pin = pin_s >= 0 && singular && singular[m];
   // down: pre-smooth (colour 0 with zero neighbours, then colour 1), restrict
   for (int l = 0; l < t.nl - 1; l++) {
     const MGLevT<T> &L = t.lev[l];
     for (int c = 0; c < 2; c++) {
-      for (int a = 2 * wv + c; a < L.na; a += 2 * nw) line_solve<K, T>(L, lmv, t.x[l], t.b[l], m, a, c == 0, -1, false);
+      const int nlc = (L.na - c + 1) / 2;  // lines a = 2 j + c of this colour
+      for (int j0 = SPW * wv; j0 < nlc; j0 += SPW * nw) {
+        const int j = j0 + hs;
+        line_solve<K, LPL, T>(L, lmv, t.x[l], t.b[l], m, 2 * j + c, c == 0, -1, false, j < nlc);
+      }
       __syncthreads();
     }
     const MGLevT<T> &Cl = t.lev[l + 1];
@@ -233,7 +242,7 @@ __global__ __launch_bounds__(1024) void k_tail(MGTailT<T> t, const T *__restrict
   // coarsest: one line, solved exactly
   {
     const int l = t.nl - 1;
-    if (wv == 0) line_solve<K, T>(t.lev[l], lmv, t.x[l], t.b[l], m, 0, 1, pin_s, pin);
+    if (wv == 0) line_solve<K, LPL, T>(t.lev[l], lmv, t.x[l], t.b[l], m, 0, 1, pin_s, pin, hs == 0);
     __syncthreads();
   }
   // up: prolong, post-smooth (colour 1, then colour 0)
@@ -247,7 +256,11 @@ __global__ __launch_bounds__(1024) void k_tail(MGTailT<T> t, const T *__restrict
     }
     __syncthreads();
     for (int c = 1; c >= 0; c--) {
-      for (int a = 2 * wv + c; a < L.na; a += 2 * nw) line_solve<K, T>(L, lmv, t.x[l], t.b[l], m, a, 0, -1, false);
+      const int nlc = (L.na - c + 1) / 2;
+      for (int j0 = SPW * wv; j0 < nlc; j0 += SPW * nw) {
+        const int j = j0 + hs;
+        line_solve<K, LPL, T>(L, lmv, t.x[l], t.b[l], m, 2 * j + c, 0, -1, false, j < nlc);
+      }
       __syncthreads();
     }
   }
@@ -528,42 +541,56 @@ unsigned blocks_for(long n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
 
-template <typename T>
-void mg_line(hipStream_t st, const MGLevT<T> &L, const T *lam, T *x, const T *b, int col_a, int col_m, int zero_nb,
-             int pin_s, const int *singular, const int *done) {
-  const int nla = (L.na - col_a + 1) / 2;
-  const int nlm = L.coupled ? (L.nm - col_m + 1) / 2 : L.nm;
-  if (nla <= 0 || nlm <= 0) return;
+template <int LPL, typename T>
+void mg_line_k(hipStream_t st, const MGLevT<T> &L, const T *lam, T *x, const T *b, int col_a, int col_m, int nla,
+               int nlm, int zero_nb, int pin_s, const int *singular, const int *done) {
   const long lines = (long)nla * nlm;
-  dim3 grid((unsigned)((lines + 3) / 4));
+  dim3 grid((unsigned)((lines * LPL + 255) / 256));
   const int rows = L.base + (L.extra ? 1 : 0);
   if (rows <= 3)
-    hipLaunchKernelGGL((k_line<3, T>), grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb, pin_s,
-                       singular, done);
+    hipLaunchKernelGGL((k_line<3, LPL, T>), grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb,
+                       pin_s, singular, done);
   else if (rows <= 5)
-    hipLaunchKernelGGL((k_line<5, T>), grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb, pin_s,
-                       singular, done);
+    hipLaunchKernelGGL((k_line<5, LPL, T>), grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb,
+                       pin_s, singular, done);
   else if (rows <= 9)
-    hipLaunchKernelGGL((k_line<9, T>), grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb, pin_s,
-                       singular, done);
+    hipLaunchKernelGGL((k_line<9, LPL, T>), grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb,
+                       pin_s, singular, done);
   else if (rows <= 17)
-    hipLaunchKernelGGL((k_line<17, T>), grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb, pin_s,
-                       singular, done);
+    hipLaunchKernelGGL((k_line<17, LPL, T>), grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb,
+                       pin_s, singular, done);
   else
     throw std::runtime_error("multigrid line relaxation: lines longer than 1088 cells are not supported");
   SNAC_HIP(hipGetLastError());
 }
 
 template <typename T>
-void mg_tail(hipStream_t st, const MGTailT<T> &t, int nm, const T *lam, int pin_s, const int *singular,
-             const int *done) {
+void mg_line(hipStream_t st, const MGLevT<T> &L, const T *lam, T *x, const T *b, int col_a, int col_m, int zero_nb,
+             int pin_s, const int *singular, const int *done) {
+  const int nla = (L.na - col_a + 1) / 2;
+  const int nlm = L.coupled ? (L.nm - col_m + 1) / 2 : L.nm;
+  if (nla <= 0 || nlm <= 0) return;
+  if (L.lpl == 32) mg_line_k<32>(st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb, pin_s, singular, done);
+  else mg_line_k<64>(st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb, pin_s, singular, done);
+}
+
+template <int LPL, typename T>
+void mg_tail_k(hipStream_t st, const MGTailT<T> &t, int nm, const T *lam, int pin_s, const int *singular,
+               const int *done) {
   const MGLevT<T> &L = t.lev[0];
   const int rows = L.base + (L.extra ? 1 : 0);
-  if (rows <= 3) hipLaunchKernelGGL((k_tail<3, T>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
-  else if (rows <= 5) hipLaunchKernelGGL((k_tail<5, T>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
-  else if (rows <= 9) hipLaunchKernelGGL((k_tail<9, T>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
-  else hipLaunchKernelGGL((k_tail<17, T>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
+  if (rows <= 3) hipLaunchKernelGGL((k_tail<3, LPL, T>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
+  else if (rows <= 5) hipLaunchKernelGGL((k_tail<5, LPL, T>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
+  else if (rows <= 9) hipLaunchKernelGGL((k_tail<9, LPL, T>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
+  else hipLaunchKernelGGL((k_tail<17, LPL, T>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
   SNAC_HIP(hipGetLastError());
+}
+
+template <typename T>
+void mg_tail(hipStream_t st, const MGTailT<T> &t, int nm, const T *lam, int pin_s, const int *singular,
+             const int *done) {
+  if (t.lev[0].lpl == 32) mg_tail_k<32>(st, t, nm, lam, pin_s, singular, done);
+  else mg_tail_k<64>(st, t, nm, lam, pin_s, singular, done);
 }
 
 template <typename T>

@@ -40,6 +40,7 @@ struct MGLevT {
   const T *dg, *w, *cs, *ca, *cm;
   const unsigned char *act;
   int nch, base, extra; // a line of ns rows over nch lanes: base (+1 for the first `extra`) rows each
+  int lpl;              // lanes per line segment (32 or 64; nch <= lpl)
 };
 using MGLev = MGLevT<double>;
 

@@ -364,7 +364,9 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
   // device arrays: the FP64 hierarchy and its FP32 twin (mixed precision)
   P.h64 = MGHier<double>();
   P.h32 = MGHier<float>();
-  const int nch = ns >= 2 ? std::min(64, ns / 2) : 1;
+  // 32 lanes per line (two lines per wavefront) up to 544 rows, else 64
+  const int lpl = ns <= 32 * 17 ? 32 : 64;
+  const int nch = ns >= 2 ? std::min(lpl, ns / 2) : 1;
   auto f32 = [](const std::vector<double> &v) { return std::vector<float>(v.begin(), v.end()); };
   if (!coupled) {
     P.h64.lam = dev_upload(P, lamh);
@@ -386,6 +388,7 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
     L.nch = nch;
     L.base = ns >= 2 ? ns / nch : 1;
     L.extra = ns >= 2 ? ns % nch : 0;
+    L.lpl = lpl;
     P.h64.lev.push_back(L);
     MGLevT<float> Lf;
     Lf.nm = L.nm; Lf.na = L.na; Lf.ns = L.ns;
@@ -396,7 +399,7 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
     Lf.ca = dev_upload(P, f32(h.ca));
     Lf.cm = coupled ? dev_upload(P, f32(h.cm)) : nullptr;
     Lf.act = L.act;
-    Lf.nch = L.nch; Lf.base = L.base; Lf.extra = L.extra;
+    Lf.nch = L.nch; Lf.base = L.base; Lf.extra = L.extra; Lf.lpl = L.lpl;
     P.h32.lev.push_back(Lf);
     const size_t n = (size_t)h.nm * h.na * h.ns;
     if (l == 0) {
