@@ -21,6 +21,17 @@
  *   eigenvalues                   : numpy eigvalsh of the 1D operator
  *   operator assembly + solvers   : scipy sparse direct solve; residual check
  *   chkdiv / chkdt                : closed-form fields
+ *   implicit diffusion (_IMPDIFF) : Helmholtz solves of u, v, w on discrete
+ *                                   eigenmodes of each staggered operator
+ *                                   (periodic, centred/staggered Dirichlet,
+ *                                   staggered Neumann; FFT and 3D forms),
+ *                                   constants / linear fields under
+ *                                   Dirichlet / Neumann values, the per-step
+ *                                   damping of velocity and scalar waves,
+ *                                   the updt_pressure Laplacian term.  The
+ *                                   inflow-array scaling of init_bc_rhs is
+ *                                   checked only on a uniform inflow
+ *                                   profile.
  * The HYPRE iterates and FFTW rounding of the reference cannot be reproduced
  * bit-wise (third-party, not vendored); the oracle solves the same assembled
  * systems to a tight tolerance instead.
@@ -57,8 +68,32 @@ typedef struct {
   char cbcsca[MAXS][3][2];                /* scalar BCs [iscal][dir][side] */
   double bcsca[MAXS][3][2];
   long sx, sy, ntot;                      /* strides of ghosted arrays */
-  long off;                               /* offset in the solver vector */
 } ob;
+
+/* one elliptic operator (the pressure's, or the Helmholtz operator of a
+ * velocity component or scalar with implicit diffusion)                     */
+typedef struct {
+  int which;       /* 0 pressure, 1..3 u, v, w, 4+i scalar i */
+  int stag;        /* staggered axis of a velocity component, -1 otherwise */
+  int nmodes, qfft, kind_fwd, kind_bwd;
+  double *lambda, normfft;
+  int nd, ax[3];   /* reduced problem dimensionality and its axes */
+  int ext[MAXB][3];          /* unknowns of each block along x, y, z */
+  long off[MAXB], nunk;      /* offsets of the blocks in the solver vector */
+  double *diag0, *wgt, *coef; /* per unknown: base diagonal, lambda weight, nd*2 coefs */
+  long *nbr;                   /* per unknown nd*2 neighbour indices (-1 none) */
+  int has_dirichlet;
+  double shift;                /* added to lambda: alphai of a Helmholtz solve */
+  /* band factorizations, lazily built per mode, kept for three shifts */
+  long *perm, *iperm; long bw;
+  double **bandL[3]; double band_shift[3]; int band_next;
+  int band_ok;
+  /* tensor solver data */
+  int tensor_ok, tt[2], nt, ts; double th[2];
+  int t_kind_fwd[2], t_kind_bwd[2]; double t_norm[2]; double *t_lambda[2];
+  double *work;
+  double *rhs[MAXB][3][2];     /* init_bc_rhs planes over the unknowns (Helmholtz) */
+} oop;
 
 struct oc_case {
   int nb, fft_axis;
@@ -72,21 +107,9 @@ struct oc_case {
   /* solver */
   int method, maxiter;
   double tol;
-  int nmodes, kind_fwd, kind_bwd;
-  double *lambda, normfft;
-  int nd, ax[3];   /* reduced problem dimensionality and its axes */
-  long nunk;
-  double *diag0, *wgt, *coef; /* per unknown: base diagonal, lambda weight, nd*2 coefs */
-  long *nbr;                   /* per unknown nd*2 neighbour indices (-1 none) */
-  int has_dirichlet;
-  /* band factorizations, lazily built per mode */
-  long *perm, *iperm; long bw;
-  double **bandL;
-  int band_ok;
-  /* tensor solver data */
-  int tensor_ok, tt[2], nt, ts; double th[2];
-  int t_kind_fwd[2], t_kind_bwd[2]; double t_norm[2]; double *t_lambda[2];
-  double *work;
+  oop P;                 /* pressure */
+  int impdiff;           /* implicit diffusion (the reference's _IMPDIFF build) */
+  oop H[3 + MAXS];       /* Helmholtz operators of u, v, w and the scalars */
 };
 
 static int nthreads_set = 0;
@@ -336,6 +359,10 @@ static void rk_mom_block(oc_case *c, ob *B, int irk, double dt) {
   long nin = (long)nx * ny * nz;
   double *du = xcalloc(nin, sizeof(double)), *dv = xcalloc(nin, sizeof(double)),
          *dw = xcalloc(nin, sizeof(double));
+  /* _IMPDIFF: the diffusion goes to dudtrkd.. (rk.f90:29-31,50-59) */
+  const int imp = c->impdiff;
+  double *dud = imp ? xcalloc(nin, sizeof(double)) : NULL, *dvd = imp ? xcalloc(nin, sizeof(double)) : NULL,
+         *dwd = imp ? xcalloc(nin, sizeof(double)) : NULL;
 #define U(i, j, k) u[IDX(B, i, j, k)]
 #define V(i, j, k) v[IDX(B, i, j, k)]
 #define W(i, j, k) w[IDX(B, i, j, k)]
@@ -379,6 +406,7 @@ static void rk_mom_block(oc_case *c, ob *B, int irk, double dt) {
           cc = cc + visc * (dwdxp - dwdxm) / dxf[i] + visc * (dwdyp - dwdym) / dyf[j] +
                visc * (dwdzp - dwdzm) / dzc[k];
         }
+        if (imp) { dud[q] = a; dvd[q] = b; dwd[q] = cc; a = 0.0; b = 0.0; cc = 0.0; }
         /* momx_a_vv mom.f90:112-147 */
         {
           double uuip = 0.25 * (U(i, j, k) * dyf[j] * dzf[k] + U(i + 1, j, k) * dyf[j] * dzf[k]) * U(i + 1, j, k);
@@ -420,6 +448,11 @@ static void rk_mom_block(oc_case *c, ob *B, int irk, double dt) {
         u[g] = u[g] + f1 * du[q] + f2 * B->dto[0][q];
         v[g] = v[g] + f1 * dv[q] + f2 * B->dto[1][q];
         w[g] = w[g] + f1 * dw[q] + f2 * B->dto[2][q];
+        if (imp) { /* rk.f90:75-79 */
+          u[g] = u[g] + f12 * dud[q];
+          v[g] = v[g] + f12 * dvd[q];
+          w[g] = w[g] + f12 * dwd[q];
+        }
         B->dto[0][q] = du[q]; B->dto[1][q] = dv[q]; B->dto[2][q] = dw[q];
       }
   /* pressure gradient + body force mom.f90:328-386, update rk.f90:86-111 */
@@ -441,11 +474,22 @@ static void rk_mom_block(oc_case *c, ob *B, int irk, double dt) {
           w[g] = w[g] - f12 * c->gacc[2] * c->beta * 0.5 * (S[IDX(B, i, j, k + 1)] + S[g]);
         }
       }
+  if (imp) { /* rhs of the Helmholtz equation rk.f90:112-127 */
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 1; k <= nz; k++)
+      for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+          long q = IIDX(B, i, j, k), g = IDX(B, i, j, k);
+          u[g] = u[g] - .5 * f12 * dud[q];
+          v[g] = v[g] - .5 * f12 * dvd[q];
+          w[g] = w[g] - .5 * f12 * dwd[q];
+        }
+  }
 #undef U
 #undef V
 #undef W
 #undef P
-  free(du); free(dv); free(dw);
+  free(du); free(dv); free(dw); free(dud); free(dvd); free(dwd);
 }
 
 void oc_rk_mom(oc_case *c, int irk, double dt) {
@@ -523,6 +567,32 @@ void oc_updt_pressure(oc_case *c) {
   }
 }
 
+/* updt_pressure.f90:23-45 of the _IMPDIFF build: p += pp, then p += alpha
+ * times the Laplacian of pp (alpha = -visc*dtrk/2, main.f90:935,1031)       */
+void oc_updt_pressure_impdiff(oc_case *c, double alpha) {
+  oc_updt_pressure(c);
+  for (int ib = 0; ib < c->nb; ib++) {
+    ob *B = &c->b[ib];
+    const double *pp = B->f[4];
+    double *p = B->f[3];
+    const double *dxc = B->dc[0], *dxf = B->df[0], *dyc = B->dc[1], *dyf = B->df[1],
+                 *dzc = B->dc[2], *dzf = B->df[2];
+    for (int k = 1; k <= B->n[2]; k++)
+      for (int j = 1; j <= B->n[1]; j++)
+        for (int i = 1; i <= B->n[0]; i++) {
+#define PP(i, j, k) pp[IDX(B, i, j, k)]
+          p[IDX(B, i, j, k)] = p[IDX(B, i, j, k)] +
+                               alpha * (((PP(i + 1, j, k) - PP(i, j, k)) / dxc[i] -
+                                         (PP(i, j, k) - PP(i - 1, j, k)) / dxc[i - 1]) / dxf[i] +
+                                        ((PP(i, j + 1, k) - PP(i, j, k)) / dyc[j] -
+                                         (PP(i, j, k) - PP(i, j - 1, k)) / dyc[j - 1]) / dyf[j] +
+                                        ((PP(i, j, k + 1) - PP(i, j, k)) / dzc[k] -
+                                         (PP(i, j, k) - PP(i, j, k - 1)) / dzc[k - 1]) / dzf[k]);
+#undef PP
+        }
+  }
+}
+
 /* chkdiv.f90:8-44                                                           */
 void oc_chkdiv(const oc_case *c, double *divtot, double *divmax) {
   double tot = 0.0, mx = 0.0;
@@ -543,7 +613,7 @@ void oc_chkdiv(const oc_case *c, double *divtot, double *divmax) {
   *divmax = mx;
 }
 
-/* chkdt.f90:8-56 (explicit diffusion branch)                                */
+/* chkdt.f90:8-56; the diffusive limit only without implicit diffusion      */
 double oc_chkdt(const oc_case *c) {
   double dti = 0.0, dlmin = 1e300;
   for (int ib = 0; ib < c->nb; ib++) {
@@ -582,6 +652,7 @@ double oc_chkdt(const oc_case *c) {
   }
   if (dti == 0.0) dti = 1.0;
   double dtmax = sqrt(3.0) / dti;
+  if (c->impdiff) return dtmax;
   /* main.f90:660 passes max(visc, alpha_max) */
   double ve = c->nscal > 0 && c->alpha_max > c->visc ? c->alpha_max : c->visc;
   double dd = 1.65 / 12.0 / ve * dlmin * dlmin;
@@ -800,7 +871,8 @@ static double poiseuille(double r, double lmin, double lmax) {
   return 6.0 * (rr * (1.0 - rr));
 }
 
-static void assemble_operator(oc_case *c);
+static int assemble_op(oc_case *c, oop *o, char *err, int errlen);
+static void free_op(oop *o);
 
 oc_case *oc_create(int nblocks, const int *ng, const double *lmin, const double *lmax,
                    const int *gt, const double *gr, const char *cbcvel, const double *bcvel,
@@ -1008,7 +1080,8 @@ oc_case *oc_create(int nblocks, const int *ng, const double *lmin, const double 
       }
     }
   }
-  assemble_operator(c);
+  c->P.which = 0; c->P.stag = -1;
+  if (assemble_op(c, &c->P, errbuf, errlen)) { oc_destroy(c); return NULL; }
   return c;
 }
 
@@ -1026,10 +1099,8 @@ void oc_destroy(oc_case *c) {
         for (int cp = 0; cp < 3; cp++) free(B->vin[d][cp][s]);
       }
   }
-  free(c->lambda); free(c->diag0); free(c->wgt); free(c->coef); free(c->nbr);
-  free(c->perm); free(c->iperm);
-  if (c->bandL) { for (int m = 0; m < c->nmodes; m++) free(c->bandL[m]); free(c->bandL); }
-  free(c->t_lambda[0]); free(c->t_lambda[1]); free(c->work);
+  free_op(&c->P);
+  for (int q = 0; q < 3 + MAXS; q++) free_op(&c->H[q]);
   free(c);
 }
 
@@ -1123,6 +1194,8 @@ void oc_rk_scal(oc_case *c, int irk, double dt, double *fout) {
   const double f1 = RKC[irk - 1][0] * dt, f2 = RKC[irk - 1][1] * dt, f12 = f1 + f2;
   for (int is = 0; is < c->nscal; is++) {
     const double alpha = c->salpha[is];
+    /* _IMPDIFF: the diffusion goes to dsdtrkd (rk.f90:151-158) */
+    double *dsd[MAXB] = {NULL};
     for (int ib = 0; ib < c->nb; ib++) {
       ob *B = &c->b[ib];
       const int nx = B->n[0], ny = B->n[1], nz = B->n[2];
@@ -1131,6 +1204,7 @@ void oc_rk_scal(oc_case *c, int irk, double dt, double *fout) {
       const double *dxc = B->dc[0], *dxf = B->df[0], *dyc = B->dc[1], *dyf = B->df[1],
                    *dzc = B->dc[2], *dzf = B->df[2];
       double *ds = xcalloc((long)nx * ny * nz, sizeof(double));
+      double *dd = c->impdiff ? (dsd[ib] = xcalloc((long)nx * ny * nz, sizeof(double))) : NULL;
 #define S(i, j, k) s[IDX(B, i, j, k)]
 #define U(i, j, k) u[IDX(B, i, j, k)]
 #define V(i, j, k) v[IDX(B, i, j, k)]
@@ -1148,6 +1222,7 @@ void oc_rk_scal(oc_case *c, int irk, double dt, double *fout) {
             double dsdzm = (S(i, j, k) - S(i, j, k - 1)) / dzc[k - 1];
             d = d + alpha * (dsdxp - dsdxm) / dxf[i] + alpha * (dsdyp - dsdym) / dyf[j] +
                 alpha * (dsdzp - dsdzm) / dzf[k];
+            if (dd) { dd[IIDX(B, i, j, k)] = d; d = 0.0; }
             double usim = 0.5 * (S(i - 1, j, k) + S(i, j, k)) * U(i - 1, j, k);
             double usip = 0.5 * (S(i + 1, j, k) + S(i, j, k)) * U(i, j, k);
             double vsjm = 0.5 * (S(i, j - 1, k) + S(i, j, k)) * V(i, j - 1, k);
@@ -1163,6 +1238,7 @@ void oc_rk_scal(oc_case *c, int irk, double dt, double *fout) {
           for (int i = 1; i <= nx; i++) {
             long q = IIDX(B, i, j, k);
             S(i, j, k) = S(i, j, k) + f1 * ds[q] + f2 * dso[q] + f12 * c->ssource[is];
+            if (dd) S(i, j, k) = S(i, j, k) + f12 * dd[q];
             dso[q] = ds[q];
           }
 #undef S
@@ -1183,6 +1259,17 @@ void oc_rk_scal(oc_case *c, int irk, double dt, double *fout) {
       }
       mean = mean / c->vol_all;
       f = c->scalf[is] - mean;
+    }
+    if (c->impdiff) /* rhs of the Helmholtz equation rk.f90:190-202, after the mean */
+      for (int ib = 0; ib < c->nb; ib++) {
+        ob *B = &c->b[ib];
+        for (int k = 1; k <= B->n[2]; k++)
+          for (int j = 1; j <= B->n[1]; j++)
+            for (int i = 1; i <= B->n[0]; i++)
+              B->sv[is][IDX(B, i, j, k)] = B->sv[is][IDX(B, i, j, k)] - .5 * f12 * dsd[ib][IIDX(B, i, j, k)];
+        free(dsd[ib]);
+      }
+    if (c->is_sforced[is]) {
       for (int ib = 0; ib < c->nb; ib++) {
         ob *B = &c->b[ib];
         for (int k = 1; k <= B->n[2]; k++)
@@ -1196,169 +1283,265 @@ void oc_rk_scal(oc_case *c, int irk, double dt, double *fout) {
 }
 
 /* ------------------------------------------------------------------------ */
-/* Poisson operator: init_matrix_2d solver.f90:687-842 + init_n_2d_matrices  */
-/* 981-1020 (lambda on the diagonal), init_matrix_3d solver.f90:158-372      */
+/* Elliptic operators: the pressure Poisson operator and, with implicit     */
+/* diffusion, the Helmholtz operator of each velocity component and scalar. */
+/* init_matrix_2d solver.f90:687-842 + init_n_2d_matrices 981-1020 (lambda  */
+/* on the diagonal), init_matrix_3d solver.f90:158-372, init_bc_rhs 44-157. */
+/* A component is staggered along its own axis (is_centered = .false.):     */
+/* its unknowns sit on the faces lo..hi (lo..hi-1 when the block is bounded */
+/* at hi, main.f90:774-775), couplings use dx1 = the face spacing dxf       */
+/* shifted by qqq = 1, weights use dx2 = dxc, and a Dirichlet value lies on */
+/* the boundary face itself (sgn = 0, factor = -bc).                        */
 /* ------------------------------------------------------------------------ */
-static void assemble_operator(oc_case *c) {
+static char op_cbc(const oc_case *c, const oop *o, const ob *B, int d, int s) {
+  (void)c;
+  if (o->which == 0) return B->cbcpre[d][s];
+  if (o->which <= 3) return B->cbcvel[o->which - 1][d][s];
+  return B->cbcsca[o->which - 4][d][s];
+}
+static double op_bcval(const oop *o, const ob *B, int d, int s) {
+  if (o->which == 0) return B->bcpre[d][s];
+  if (o->which <= 3) return B->bcvel[o->which - 1][d][s];
+  return B->bcsca[o->which - 4][d][s];
+}
+/* spacing to the lower / upper neighbour of unknown i (1-based) along a,   */
+/* and the cell width dx2 along a                                           */
+static inline double sp_lo(const oop *o, const ob *B, int a, int i) { return o->stag == a ? B->df[a][i] : B->dc[a][i - 1]; }
+static inline double sp_hi(const oop *o, const ob *B, int a, int i) { return o->stag == a ? B->df[a][i + 1] : B->dc[a][i]; }
+static inline double wdx2(const oop *o, const ob *B, int a, int i) { return o->stag == a ? B->dc[a][i] : B->df[a][i]; }
+/* the operator's weight of one interior cell: dx2 over the non-FFT axes   */
+static double op_weight(const oc_case *c, const oop *o, const ob *B, const int ijk[3]) {
+  double w = 1.0;
+  for (int a = 0; a < 3; a++)
+    if (a != c->fft_axis - 1) w *= wdx2(o, B, a, ijk[a]);
+  return w;
+}
+
+static int assemble_op(oc_case *c, oop *o, char *err, int errlen) {
   int fa = c->fft_axis;
-  if (fa >= 1 && fa <= 3) {
-    int d = fa - 1;
-    c->nd = 2;
-    int k = 0;
-    for (int e = 0; e < 3; e++) if (e != d) c->ax[k++] = e;
-    ob *B0 = &c->b[0];
-    c->nmodes = B0->n[d];
-    c->lambda = xcalloc(c->nmodes, sizeof(double));
-    char bc2[2] = {B0->cbcpre[d][0], B0->cbcpre[d][1]};
-    double norm[2];
-    oc_find_fft(bc2, 1, &c->kind_fwd, &c->kind_bwd, norm);
-    c->normfft = 1.0 / (norm[0] * (c->nmodes + norm[1]));
-    oc_eigenvalues(c->nmodes, bc2, 1, c->lambda);
-    /* dl = dxc_g(lo_g-1) of the FFT direction (main.f90:717-728)          */
-    double dl = B0->df[d][1];
-    for (int m = 0; m < c->nmodes; m++) c->lambda[m] /= dl * dl;
-  } else {
-    c->nd = 3; c->ax[0] = 0; c->ax[1] = 1; c->ax[2] = 2;
-    c->nmodes = 1;
-    c->lambda = xcalloc(1, sizeof(double));
-    c->normfft = 1.0;
-  }
-  int nd = c->nd;
-  long nunk = 0;
   for (int ib = 0; ib < c->nb; ib++) {
     ob *B = &c->b[ib];
-    B->off = nunk;
+    for (int a = 0; a < 3; a++)
+      o->ext[ib][a] = B->n[a] - ((o->stag == a && B->is_bound[1][a]) ? 1 : 0);
+  }
+  if (fa >= 1 && fa <= 3) {
+    int d = fa - 1;
+    o->nd = 2;
+    int k = 0;
+    for (int e = 0; e < 3; e++) if (e != d) o->ax[k++] = e;
+    ob *B0 = &c->b[0];
+    char bc2[2] = {op_cbc(c, o, B0, d, 0), op_cbc(c, o, B0, d, 1)};
+    int centred = o->stag != d;
+    /* fftini fft.f90:31-65: a staggered non-periodic FFT axis transforms its
+     * first n-1 points.  eigenvalues fft.f90:151-189 does not match those
+     * transforms for the staggered NN, ND and DN pairs (its formulas are the
+     * n-point ones, DN has none): only FF and DD are taken on a staggered
+     * FFT axis.                                                              */
+    o->qfft = (!centred && !(bc2[0] == 'F' && bc2[1] == 'F')) ? 1 : 0;
+    if (!centred && o->qfft && !(bc2[0] == 'D' && bc2[1] == 'D')) {
+      if (err) snprintf(err, (size_t)errlen, "staggered %c%c pair along the FFT axis is not supported", bc2[0], bc2[1]);
+      return -1;
+    }
+    double norm[2];
+    if (oc_find_fft(bc2, centred, &o->kind_fwd, &o->kind_bwd, norm)) {
+      if (err) snprintf(err, (size_t)errlen, "unsupported boundary pair %c%c along the FFT axis", bc2[0], bc2[1]);
+      return -1;
+    }
+    int n = B0->n[d];
+    o->nmodes = n - o->qfft;
+    o->normfft = 1.0 / (norm[0] * (n + norm[1] - o->qfft));
+    o->lambda = xcalloc(n, sizeof(double));
+    oc_eigenvalues(n, bc2, centred, o->lambda);
+    /* dl(0,idir) of main.f90:717-728 / 769-771: dxc_g(lo_g-1) = dxf_g(lo_g)
+     * for the pressure, dxf_g(lo_g) for a staggered axis                    */
+    double dl = B0->df[d][1];
+    for (int m = 0; m < n; m++) o->lambda[m] /= dl * dl;
+  } else {
+    o->nd = 3; o->ax[0] = 0; o->ax[1] = 1; o->ax[2] = 2;
+    o->nmodes = 1; o->qfft = 0;
+    o->lambda = xcalloc(1, sizeof(double));
+    o->normfft = 1.0;
+  }
+  int nd = o->nd;
+  long nunk = 0;
+  for (int ib = 0; ib < c->nb; ib++) {
+    o->off[ib] = nunk;
     long cnt = 1;
-    for (int e = 0; e < nd; e++) cnt *= B->n[c->ax[e]];
+    for (int e = 0; e < nd; e++) cnt *= o->ext[ib][o->ax[e]];
     nunk += cnt;
   }
-  c->nunk = nunk;
-  c->diag0 = xcalloc(nunk, sizeof(double));
-  c->wgt = xcalloc(nunk, sizeof(double));
-  c->coef = xcalloc(nunk * 2 * nd, sizeof(double));
-  c->nbr = xcalloc(nunk * 2 * nd, sizeof(long));
-  c->has_dirichlet = 0;
+  o->nunk = nunk;
+  o->diag0 = xcalloc(nunk, sizeof(double));
+  o->wgt = xcalloc(nunk, sizeof(double));
+  o->coef = xcalloc(nunk * 2 * nd, sizeof(double));
+  o->nbr = xcalloc(nunk * 2 * nd, sizeof(long));
+  o->has_dirichlet = 0;
   for (int ib = 0; ib < c->nb; ib++) {
     ob *B = &c->b[ib];
     int nn[3] = {1, 1, 1};
-    for (int e = 0; e < nd; e++) nn[e] = B->n[c->ax[e]];
+    for (int e = 0; e < nd; e++) nn[e] = o->ext[ib][o->ax[e]];
     for (int i3 = 1; i3 <= nn[2]; i3++)
       for (int i2 = 1; i2 <= nn[1]; i2++)
         for (int i1 = 1; i1 <= nn[0]; i1++) {
           int idx[3] = {i1, i2, i3};
-          long q = B->off + (i1 - 1) + (long)nn[0] * ((i2 - 1) + (long)nn[1] * (i3 - 1));
-          /* weight and coefficients */
+          long q = o->off[ib] + (i1 - 1) + (long)nn[0] * ((i2 - 1) + (long)nn[1] * (i3 - 1));
           double weight = 1.0;
-          for (int e = 0; e < nd; e++) weight *= B->df[c->ax[e]][idx[e]];
-          double cc = 0.0;
+          for (int e = 0; e < nd; e++) weight *= wdx2(o, B, o->ax[e], idx[e]);
           double cm[3], cp[3];
           for (int e = 0; e < nd; e++) {
             double other = 1.0;
-            for (int o = 0; o < nd; o++) if (o != e) other *= B->df[c->ax[o]][idx[o]];
-            cm[e] = 1.0 / (B->dc[c->ax[e]][idx[e] - 1]) * other;
-            cp[e] = 1.0 / (B->dc[c->ax[e]][idx[e]]) * other;
+            for (int oo = 0; oo < nd; oo++) if (oo != e) other *= wdx2(o, B, o->ax[oo], idx[oo]);
+            cm[e] = 1.0 / sp_lo(o, B, o->ax[e], idx[e]) * other;
+            cp[e] = 1.0 / sp_hi(o, B, o->ax[e], idx[e]) * other;
           }
           double ssum = 0.0;
           for (int e = 0; e < nd; e++) ssum += cm[e] + cp[e];
-          /* the reference sums (c1m+c1p+c2m+c2p) / (cxm+cxp+cym+cyp+czm+czp) */
-          cc = -ssum;
+          double cc = -ssum;
           for (int e = 0; e < nd; e++) {
-            int a = c->ax[e];
+            int a = o->ax[e];
             if (c->periods[a] != 0) continue;
-            if (B->is_bound[0][a] && idx[e] == 1) {
-              double sgn = B->cbcpre[a][0] == 'N' ? 1.0 : (B->cbcpre[a][0] == 'D' ? -1.0 : 0.0);
-              if (B->cbcpre[a][0] == 'D') c->has_dirichlet = 1;
-              cc = cc + sgn * cm[e]; cm[e] = 0.0;
-            }
-            if (B->is_bound[1][a] && idx[e] == nn[e]) {
-              double sgn = B->cbcpre[a][1] == 'N' ? 1.0 : (B->cbcpre[a][1] == 'D' ? -1.0 : 0.0);
-              if (B->cbcpre[a][1] == 'D') c->has_dirichlet = 1;
-              cc = cc + sgn * cp[e]; cp[e] = 0.0;
+            for (int s = 0; s < 2; s++) {
+              if (!B->is_bound[s][a] || idx[e] != (s == 0 ? 1 : nn[e])) continue;
+              char ct = op_cbc(c, o, B, a, s);
+              double sgn = ct == 'N' ? 1.0 : (ct == 'D' ? (o->stag == a ? 0.0 : -1.0) : 0.0);
+              if (ct == 'D') o->has_dirichlet = 1;
+              if (s == 0) { cc = cc + sgn * cm[e]; cm[e] = 0.0; }
+              else { cc = cc + sgn * cp[e]; cp[e] = 0.0; }
             }
           }
-          c->diag0[q] = cc;
-          c->wgt[q] = weight;
+          o->diag0[q] = cc;
+          o->wgt[q] = weight;
           for (int e = 0; e < nd; e++) {
-            int a = c->ax[e];
+            int a = o->ax[e];
             for (int s = 0; s < 2; s++) {
               long slot = q * 2 * nd + 2 * e + s;
               double cf = s == 0 ? cm[e] : cp[e];
-              c->coef[slot] = cf;
-              c->nbr[slot] = -1;
+              o->coef[slot] = cf;
+              o->nbr[slot] = -1;
               if (cf == 0.0) continue;
               int j2[3] = {idx[0], idx[1], idx[2]};
-              const ob *T = B;
-              if (s == 0 && idx[e] == 1) { T = &c->b[B->nb[0][a]]; j2[e] = T->n[a]; }
-              else if (s == 1 && idx[e] == nn[e]) { T = &c->b[B->nb[1][a]]; j2[e] = 1; }
+              int tb = ib;
+              if (s == 0 && idx[e] == 1) { tb = B->nb[0][a]; j2[e] = o->ext[tb][a]; }
+              else if (s == 1 && idx[e] == nn[e]) { tb = B->nb[1][a]; j2[e] = 1; }
               else j2[e] += s == 0 ? -1 : 1;
               int tn[3] = {1, 1, 1};
-              for (int o = 0; o < nd; o++) tn[o] = T->n[c->ax[o]];
-              c->nbr[slot] = T->off + (j2[0] - 1) + (long)tn[0] * ((j2[1] - 1) + (long)tn[1] * (j2[2] - 1));
+              for (int oo = 0; oo < nd; oo++) tn[oo] = o->ext[tb][o->ax[oo]];
+              o->nbr[slot] = o->off[tb] + (j2[0] - 1) + (long)tn[0] * ((j2[1] - 1) + (long)tn[1] * (j2[2] - 1));
             }
           }
         }
   }
-  c->work = xcalloc(nunk * 6, sizeof(double));
+  o->work = xcalloc(nunk * 6, sizeof(double));
+  /* init_bc_rhs solver.f90:44-157 for the Helmholtz operators (the pressure
+   * keeps its planes in ob.rhsp): over the unknowns of each bounded face,
+   * factor * weight / (dx1(i-(1-ib)+qqq) * dx2(i)), times bc_array / bc
+   * where an inflow array gives the boundary values (main.f90:600-636)     */
+  if (o->which > 0) {
+    for (int ib = 0; ib < c->nb; ib++) {
+      ob *B = &c->b[ib];
+      for (int d = 0; d < 3; d++) {
+        if (c->periods[d] != 0) continue;
+        int a1, a2; tang(d, &a1, &a2);
+        for (int s = 0; s < 2; s++) {
+          if (!B->is_bound[s][d]) continue;
+          char ct = op_cbc(c, o, B, d, s);
+          double bcv = op_bcval(o, B, d, s), factor = 0.0;
+          /* dl: the block's first / last spacing (dxc_g(lo_g-1), dxc_g(hi_g);
+           * dxf_g(lo_g), dxf_g(hi_g) along the staggered axis)              */
+          double dl = s == 0 ? B->df[d][1] : B->df[d][B->n[d]];
+          if (ct == 'N') { factor = dl * bcv; if (s == 1) factor = -factor; }
+          else if (ct == 'D') factor = o->stag == d ? -bcv : -2.0 * bcv;
+          int q = s == 0 ? 1 : o->ext[ib][d];
+          long m1 = o->ext[ib][a1], m2 = o->ext[ib][a2];
+          const double *arr = NULL;
+          if (o->which <= 3 && B->is_inflow[d][s]) arr = B->vin[d][o->which - 1][s];
+          o->rhs[ib][d][s] = xcalloc(m1 * m2, sizeof(double));
+          for (int t2 = 1; t2 <= m2; t2++)
+            for (int t1 = 1; t1 <= m1; t1++) {
+              int ijk[3]; ijk[d] = q; ijk[a1] = t1; ijk[a2] = t2;
+              double spc = s == 0 ? sp_lo(o, B, d, q) : sp_hi(o, B, d, q);
+              double rhs = factor * op_weight(c, o, B, ijk) / (spc * wdx2(o, B, d, q));
+              if (arr && bcv != 0.0) rhs = rhs * arr[t1 + (long)(B->n[a1] + 2) * t2] / bcv;
+              o->rhs[ib][d][s][(t1 - 1) + m1 * (t2 - 1)] = rhs;
+            }
+        }
+      }
+    }
+  }
+  return 0;
 }
 
-int oc_num_unknowns(const oc_case *c) { return (int)c->nunk; }
+static void free_op(oop *o) {
+  free(o->lambda); free(o->diag0); free(o->wgt); free(o->coef); free(o->nbr);
+  free(o->perm); free(o->iperm);
+  for (int s = 0; s < 3; s++)
+    if (o->bandL[s]) { for (int m = 0; m < o->nmodes; m++) free(o->bandL[s][m]); free(o->bandL[s]); }
+  free(o->t_lambda[0]); free(o->t_lambda[1]); free(o->work);
+  for (int ib = 0; ib < MAXB; ib++)
+    for (int d = 0; d < 3; d++) { free(o->rhs[ib][d][0]); free(o->rhs[ib][d][1]); }
+  memset(o, 0, sizeof(*o));
+}
+
+int oc_num_unknowns(const oc_case *c) { return (int)c->P.nunk; }
 int oc_get_lambda(const oc_case *c, double *lambda, double *normfft) {
-  if (lambda) memcpy(lambda, c->lambda, sizeof(double) * c->nmodes);
-  if (normfft) *normfft = c->normfft;
-  return c->nmodes;
+  if (lambda) memcpy(lambda, c->P.lambda, sizeof(double) * c->P.nmodes);
+  if (normfft) *normfft = c->P.normfft;
+  return c->P.nmodes;
 }
 
-static void apply_op(const oc_case *c, int mode, const double *x, double *y) {
-  int nd = c->nd;
-  double lam = c->lambda[mode];
+static inline double op_lam(const oop *o, int mode) { return o->lambda[mode] + o->shift; }
+
+static void apply_op(const oop *o, int mode, const double *x, double *y) {
+  int nd = o->nd;
+  double lam = op_lam(o, mode);
 #pragma omp parallel for schedule(static)
-  for (long q = 0; q < c->nunk; q++) {
-    double s = (c->diag0[q] + lam * c->wgt[q]) * x[q];
+  for (long q = 0; q < o->nunk; q++) {
+    double s = (o->diag0[q] + lam * o->wgt[q]) * x[q];
     for (int t = 0; t < 2 * nd; t++) {
-      long nb = c->nbr[q * 2 * nd + t];
-      if (nb >= 0) s += c->coef[q * 2 * nd + t] * x[nb];
+      long nb = o->nbr[q * 2 * nd + t];
+      if (nb >= 0) s += o->coef[q * 2 * nd + t] * x[nb];
     }
     y[q] = s;
   }
 }
 
 int oc_apply_operator(const oc_case *c, int mode, const double *x, double *y) {
-  if (mode < 0 || mode >= c->nmodes) return -1;
-  apply_op(c, mode, x, y);
+  if (mode < 0 || mode >= c->P.nmodes) return -1;
+  apply_op(&c->P, mode, x, y);
   return 0;
 }
 
-static int mode_singular(const oc_case *c, int mode) {
-  return !c->has_dirichlet && c->lambda[mode] == 0.0;
+static int mode_singular(const oop *o, int mode) {
+  return !o->has_dirichlet && op_lam(o, mode) == 0.0;
 }
 
 /* Jacobi-preconditioned CG on -A (A is negative semi-definite)             */
-static int solve_cg(oc_case *c, int mode, const double *b_in, double *x) {
-  long n = c->nunk;
-  double *r = c->work, *z = r + n, *p = z + n, *q = p + n, *b = q + n;
-  double lam = c->lambda[mode];
+static int solve_cg(const oc_case *c, oop *o, int mode, const double *b_in, double *x) {
+  long n = o->nunk;
+  double *r = o->work, *z = r + n, *p = z + n, *q = p + n, *b = q + n;
+  double lam = op_lam(o, mode);
   memcpy(b, b_in, sizeof(double) * n);
-  if (mode_singular(c, mode)) {
+  if (mode_singular(o, mode)) {
     double m = 0; for (long i = 0; i < n; i++) m += b[i]; m /= n;
     for (long i = 0; i < n; i++) b[i] -= m;
   }
   double bn = 0; for (long i = 0; i < n; i++) bn += b[i] * b[i];
   bn = sqrt(bn);
   if (bn == 0.0) { memset(x, 0, sizeof(double) * n); return 0; }
-  apply_op(c, mode, x, q);
+  apply_op(o, mode, x, q);
   for (long i = 0; i < n; i++) r[i] = -(b[i] - q[i]); /* residual of -A x = -b */
   double rz = 0;
-  for (long i = 0; i < n; i++) { z[i] = r[i] / (-(c->diag0[i] + lam * c->wgt[i])); p[i] = z[i]; rz += r[i] * z[i]; }
+  for (long i = 0; i < n; i++) { z[i] = r[i] / (-(o->diag0[i] + lam * o->wgt[i])); p[i] = z[i]; rz += r[i] * z[i]; }
   int it;
   for (it = 0; it < c->maxiter; it++) {
     double rn = 0; for (long i = 0; i < n; i++) rn += r[i] * r[i];
     if (sqrt(rn) <= c->tol * bn) break;
-    apply_op(c, mode, p, q);
+    apply_op(o, mode, p, q);
     double pq = 0; for (long i = 0; i < n; i++) { q[i] = -q[i]; pq += p[i] * q[i]; }
     double al = rz / pq;
     for (long i = 0; i < n; i++) { x[i] += al * p[i]; r[i] -= al * q[i]; }
     double rz2 = 0;
-    for (long i = 0; i < n; i++) { z[i] = r[i] / (-(c->diag0[i] + lam * c->wgt[i])); rz2 += r[i] * z[i]; }
+    for (long i = 0; i < n; i++) { z[i] = r[i] / (-(o->diag0[i] + lam * o->wgt[i])); rz2 += r[i] * z[i]; }
     double be = rz2 / rz; rz = rz2;
     for (long i = 0; i < n; i++) p[i] = z[i] + be * p[i];
   }
@@ -1366,83 +1549,93 @@ static int solve_cg(oc_case *c, int mode, const double *b_in, double *x) {
 }
 
 /* banded Cholesky of -A in an ordering of the union by global coordinates  */
-static int band_setup(oc_case *c) {
-  if (c->perm) return c->band_ok;
-  long n = c->nunk; int nd = c->nd;
-  c->perm = xcalloc(n, sizeof(long)); c->iperm = xcalloc(n, sizeof(long));
+static int band_setup(const oc_case *c, oop *o) {
+  if (o->perm) return o->band_ok;
+  long n = o->nunk; int nd = o->nd;
+  o->perm = xcalloc(n, sizeof(long)); o->iperm = xcalloc(n, sizeof(long));
   /* global coords of each unknown; order: highest reduced axis that is not
    * periodic is the major key (wrap couplings stay inside the band)         */
   long *key = xcalloc(n, sizeof(long));
   int order[3] = {0, 1, 2};
-  if (nd == 2) { order[0] = 0; order[1] = 1; if (c->periods[c->ax[1]] != 0 && c->periods[c->ax[0]] == 0) { order[0] = 1; order[1] = 0; } }
+  if (nd == 2) { order[0] = 0; order[1] = 1; if (c->periods[o->ax[1]] != 0 && c->periods[o->ax[0]] == 0) { order[0] = 1; order[1] = 0; } }
   else {
-    /* major axis must be non-periodic if possible */
     int maj = 2;
-    for (int e = 2; e >= 0; e--) if (c->periods[c->ax[e]] == 0) { maj = e; break; }
+    for (int e = 2; e >= 0; e--) if (c->periods[o->ax[e]] == 0) { maj = e; break; }
     int k = 0; for (int e = 0; e < 3; e++) if (e != maj) order[k++] = e;
     order[2] = maj;
   }
   int gmin[3] = {1 << 30, 1 << 30, 1 << 30}, gmax[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
   for (int ib = 0; ib < c->nb; ib++)
     for (int e = 0; e < nd; e++) {
-      int a = c->ax[e];
+      int a = o->ax[e];
       if (c->b[ib].lo_g[a] < gmin[e]) gmin[e] = c->b[ib].lo_g[a];
-      int hi = c->b[ib].lo_g[a] + c->b[ib].n[a] - 1;
+      int hi = c->b[ib].lo_g[a] + o->ext[ib][a] - 1;
       if (hi > gmax[e]) gmax[e] = hi;
     }
   long ext[3] = {1, 1, 1};
   for (int e = 0; e < nd; e++) ext[e] = gmax[e] - gmin[e] + 1;
   for (int ib = 0; ib < c->nb; ib++) {
-    ob *B = &c->b[ib];
+    const ob *B = &c->b[ib];
     int nn[3] = {1, 1, 1};
-    for (int e = 0; e < nd; e++) nn[e] = B->n[c->ax[e]];
+    for (int e = 0; e < nd; e++) nn[e] = o->ext[ib][o->ax[e]];
     for (int i3 = 0; i3 < nn[2]; i3++)
       for (int i2 = 0; i2 < nn[1]; i2++)
         for (int i1 = 0; i1 < nn[0]; i1++) {
-          long q = B->off + i1 + (long)nn[0] * (i2 + (long)nn[1] * i3);
+          long q = o->off[ib] + i1 + (long)nn[0] * (i2 + (long)nn[1] * i3);
           int loc[3] = {i1, i2, i3};
           long g[3] = {0, 0, 0};
-          for (int e = 0; e < nd; e++) g[e] = B->lo_g[c->ax[e]] + loc[e] - gmin[e];
+          for (int e = 0; e < nd; e++) g[e] = B->lo_g[o->ax[e]] + loc[e] - gmin[e];
           long kk = 0;
-          for (int o = nd - 1; o >= 0; o--) kk = kk * ext[order[o]] + g[order[o]];
+          for (int oo = nd - 1; oo >= 0; oo--) kk = kk * ext[order[oo]] + g[order[oo]];
           key[q] = kk;
         }
   }
-  /* sort unknowns by key (counting via index into a dense key space)       */
   long kmax = 1; for (int e = 0; e < nd; e++) kmax *= ext[e];
   long *slot = xcalloc(kmax, sizeof(long));
   for (long k = 0; k < kmax; k++) slot[k] = -1;
   for (long q = 0; q < n; q++) slot[key[q]] = q;
   long pos = 0;
-  for (long k = 0; k < kmax; k++) if (slot[k] >= 0) { c->perm[pos] = slot[k]; c->iperm[slot[k]] = pos; pos++; }
+  for (long k = 0; k < kmax; k++) if (slot[k] >= 0) { o->perm[pos] = slot[k]; o->iperm[slot[k]] = pos; pos++; }
   free(slot); free(key);
   long bw = 0;
   for (long q = 0; q < n; q++)
     for (int t = 0; t < 2 * nd; t++) {
-      long nb = c->nbr[q * 2 * nd + t];
+      long nb = o->nbr[q * 2 * nd + t];
       if (nb < 0) continue;
-      long dd = labs(c->iperm[q] - c->iperm[nb]);
+      long dd = labs(o->iperm[q] - o->iperm[nb]);
       if (dd > bw) bw = dd;
     }
-  c->bw = bw;
-  c->band_ok = (double)n * (double)(bw + 1) * (double)c->nmodes < 4.0e8;
-  if (c->band_ok) c->bandL = xcalloc(c->nmodes, sizeof(double *));
-  return c->band_ok;
+  o->bw = bw;
+  o->band_ok = (double)n * (double)(bw + 1) * (double)o->nmodes < 4.0e8;
+  return o->band_ok;
 }
 
-static int band_factor(oc_case *c, int mode) {
-  long n = c->nunk, bw = c->bw; int nd = c->nd;
+/* factorisations are kept per shift (the three RK substages of a Helmholtz
+ * operator have three shifts; the pressure has one)                        */
+static double **band_slot(oop *o) {
+  for (int s = 0; s < 3; s++)
+    if (o->bandL[s] && o->band_shift[s] == o->shift) return o->bandL[s];
+  int s = o->band_next;
+  o->band_next = (o->band_next + 1) % 3;
+  if (o->bandL[s]) { for (int m = 0; m < o->nmodes; m++) free(o->bandL[s][m]); free(o->bandL[s]); }
+  o->bandL[s] = xcalloc(o->nmodes, sizeof(double *));
+  o->band_shift[s] = o->shift;
+  return o->bandL[s];
+}
+
+static int band_factor(oop *o, double **slot, int mode) {
+  long n = o->nunk, bw = o->bw; int nd = o->nd;
   double *L = xcalloc(n * (bw + 1), sizeof(double)); /* L[i*(bw+1) + (i-j)] */
-  double lam = c->lambda[mode];
-  int sing = mode_singular(c, mode);
+  double lam = op_lam(o, mode);
+  int sing = mode_singular(o, mode);
   for (long q = 0; q < n; q++) {
-    long i = c->iperm[q];
-    L[i * (bw + 1)] += -(c->diag0[q] + lam * c->wgt[q]);
+    long i = o->iperm[q];
+    L[i * (bw + 1)] += -(o->diag0[q] + lam * o->wgt[q]);
     for (int t = 0; t < 2 * nd; t++) {
-      long nb = c->nbr[q * 2 * nd + t];
+      long nb = o->nbr[q * 2 * nd + t];
       if (nb < 0) continue;
-      long j = c->iperm[nb];
-      if (j < i) L[i * (bw + 1) + (i - j)] += -c->coef[q * 2 * nd + t];
+      long j = o->iperm[nb];
+      if (j < i) L[i * (bw + 1) + (i - j)] += -o->coef[q * 2 * nd + t];
     }
   }
   if (sing) { /* pin the first unknown of the ordering */
@@ -1462,17 +1655,18 @@ static int band_factor(oc_case *c, int mode) {
       } else L[i * (bw + 1) + (i - j)] = s / L[j * (bw + 1)];
     }
   }
-  c->bandL[mode] = L;
+  slot[mode] = L;
   return 0;
 }
 
-static int solve_band(oc_case *c, int mode, const double *b, double *x) {
-  if (!c->bandL[mode] && band_factor(c, mode)) return -1;
-  long n = c->nunk, bw = c->bw;
-  const double *L = c->bandL[mode];
-  double *y = c->work;
-  for (long i = 0; i < n; i++) y[i] = -b[c->perm[i]];
-  if (mode_singular(c, mode)) y[0] = 0.0;
+static int solve_band(oop *o, int mode, const double *b, double *x) {
+  double **slot = band_slot(o);
+  if (!slot[mode] && band_factor(o, slot, mode)) return -1;
+  long n = o->nunk, bw = o->bw;
+  const double *L = slot[mode];
+  double *y = o->work;
+  for (long i = 0; i < n; i++) y[i] = -b[o->perm[i]];
+  if (mode_singular(o, mode)) y[0] = 0.0;
   for (long i = 0; i < n; i++) {
     double s = y[i];
     long k0 = i - bw < 0 ? 0 : i - bw;
@@ -1485,22 +1679,22 @@ static int solve_band(oc_case *c, int mode, const double *b, double *x) {
     for (long k = i + 1; k <= k1; k++) s -= L[k * (bw + 1) + (k - i)] * y[k];
     y[i] = s / L[i * (bw + 1)];
   }
-  for (long i = 0; i < n; i++) x[c->perm[i]] = y[i];
+  for (long i = 0; i < n; i++) x[o->perm[i]] = y[i];
   return 0;
 }
 
 /* tensor solver: r2r along one uniform in-plane axis (two for 3D) and a
- * tridiagonal solve along the remaining axis; single-block cases only in the
- * oracle.                                                                   */
-static int tensor_setup(oc_case *c) {
-  if (c->tensor_ok) return c->tensor_ok > 0;
-  c->tensor_ok = -1;
-  if (c->nb != 1) return 0;
+ * tridiagonal solve along the remaining axis; single-block cases and the
+ * pressure operator only in the oracle.                                     */
+static int tensor_setup(oc_case *c, oop *o) {
+  if (o->tensor_ok) return o->tensor_ok > 0;
+  o->tensor_ok = -1;
+  if (c->nb != 1 || o->which != 0) return 0;
   ob *B = &c->b[0];
-  int nd = c->nd, nt = nd - 1;
+  int nd = o->nd, nt = nd - 1;
   int cand[3], nc = 0;
   for (int e = 0; e < nd; e++) {
-    int a = c->ax[e];
+    int a = o->ax[e];
     int n = B->n[a];
     double h = B->df[a][1], dev = 0;
     for (int q = 0; q <= n + 1; q++) { dev = fmax(dev, fabs(B->df[a][q] - h)); dev = fmax(dev, fabs(B->dc[a][q] - h)); }
@@ -1517,59 +1711,59 @@ static int tensor_setup(oc_case *c) {
     if (!isc) { ls = e; break; }
   }
   if (ls < 0) {
-    for (int q = nc - 1; q >= 0; q--) if (c->periods[c->ax[cand[q]]] == 0) { ls = cand[q]; break; }
+    for (int q = nc - 1; q >= 0; q--) if (c->periods[o->ax[cand[q]]] == 0) { ls = cand[q]; break; }
   }
-  if (ls < 0 || c->periods[c->ax[ls]] != 0) return 0;
+  if (ls < 0 || c->periods[o->ax[ls]] != 0) return 0;
   int k = 0;
-  for (int e = 0; e < nd; e++) if (e != ls) c->tt[k++] = e;
+  for (int e = 0; e < nd; e++) if (e != ls) o->tt[k++] = e;
   for (int t = 0; t < nt; t++) {
-    int e = c->tt[t]; int isc = 0;
+    int e = o->tt[t]; int isc = 0;
     for (int q = 0; q < nc; q++) if (cand[q] == e) isc = 1;
     if (!isc) return 0;
   }
-  c->nt = nt; c->ts = ls;
+  o->nt = nt; o->ts = ls;
   for (int t = 0; t < nt; t++) {
-    int a = c->ax[c->tt[t]], n = B->n[a];
+    int a = o->ax[o->tt[t]], n = B->n[a];
     char bc2[2] = {B->cbcpre[a][0], B->cbcpre[a][1]};
     double nrm[2];
-    oc_find_fft(bc2, 1, &c->t_kind_fwd[t], &c->t_kind_bwd[t], nrm);
-    c->t_norm[t] = 1.0 / (nrm[0] * (n + nrm[1]));
-    c->t_lambda[t] = xcalloc(n, sizeof(double));
-    oc_eigenvalues(n, bc2, 1, c->t_lambda[t]);
-    c->th[t] = B->df[a][1];
-    for (int q = 0; q < n; q++) c->t_lambda[t][q] /= c->th[t] * c->th[t];
+    oc_find_fft(bc2, 1, &o->t_kind_fwd[t], &o->t_kind_bwd[t], nrm);
+    o->t_norm[t] = 1.0 / (nrm[0] * (n + nrm[1]));
+    o->t_lambda[t] = xcalloc(n, sizeof(double));
+    oc_eigenvalues(n, bc2, 1, o->t_lambda[t]);
+    o->th[t] = B->df[a][1];
+    for (int q = 0; q < n; q++) o->t_lambda[t][q] /= o->th[t] * o->th[t];
   }
-  c->tensor_ok = 1;
+  o->tensor_ok = 1;
   return 1;
 }
 
-static int solve_tensor(oc_case *c, int mode, const double *b, double *x) {
-  ob *B = &c->b[0];
-  int nd = c->nd;
+static int solve_tensor(const oc_case *c, oop *o, int mode, const double *b, double *x) {
+  const ob *B = &c->b[0];
+  int nd = o->nd;
   int nn[3] = {1, 1, 1};
-  for (int e = 0; e < nd; e++) nn[e] = B->n[c->ax[e]];
+  for (int e = 0; e < nd; e++) nn[e] = B->n[o->ax[e]];
   long st[3] = {1, nn[0], (long)nn[0] * nn[1]};
-  long n = c->nunk;
+  long n = o->nunk;
   double *y = xcalloc(n, sizeof(double));
   memcpy(y, b, sizeof(double) * n);
   int maxn = 0; for (int e = 0; e < 3; e++) if (nn[e] > maxn) maxn = nn[e];
   double *lin = xcalloc(maxn, sizeof(double)), *lout = xcalloc(maxn, sizeof(double));
   /* forward transforms along the uniform axes */
-  for (int t = 0; t < c->nt; t++) {
-    int e = c->tt[t], o1 = -1, o2 = -1;
+  for (int t = 0; t < o->nt; t++) {
+    int e = o->tt[t], o1 = -1, o2 = -1;
     for (int q = 0; q < 3; q++) if (q != e) { if (o1 < 0) o1 = q; else o2 = q; }
     for (int j2 = 0; j2 < nn[o2]; j2++)
       for (int j1 = 0; j1 < nn[o1]; j1++) {
         long base = j1 * st[o1] + j2 * st[o2];
         for (int q = 0; q < nn[e]; q++) lin[q] = y[base + q * st[e]];
-        oc_r2r(c->t_kind_fwd[t], nn[e], lin, lout);
+        oc_r2r(o->t_kind_fwd[t], nn[e], lin, lout);
         for (int q = 0; q < nn[e]; q++) y[base + q * st[e]] = lout[q];
       }
   }
   /* tridiagonal solves along the line axis; the line-axis part of the
    * operator is rebuilt from the metrics exactly as the assembly does, the
    * transformed axes enter through their eigenvalues times the cell weight */
-  int e = c->ts, a = c->ax[e], m = nn[e], o1 = -1, o2 = -1;
+  int e = o->ts, a = o->ax[e], m = nn[e], o1 = -1, o2 = -1;
   for (int q = 0; q < 3; q++) if (q != e) { if (o1 < 0) o1 = q; else o2 = q; }
   double sg0 = B->cbcpre[a][0] == 'N' ? 1.0 : (B->cbcpre[a][0] == 'D' ? -1.0 : 0.0);
   double sg1 = B->cbcpre[a][1] == 'N' ? 1.0 : (B->cbcpre[a][1] == 'D' ? -1.0 : 0.0);
@@ -1578,22 +1772,22 @@ static int solve_tensor(oc_case *c, int mode, const double *b, double *x) {
     for (int j1 = 0; j1 < nn[o1]; j1++) {
       long base = j1 * st[o1] + j2 * st[o2];
       int jj[3] = {0, 0, 0}; jj[o1] = j1; jj[o2] = j2;
-      double lsum = c->lambda[mode];
-      for (int t = 0; t < c->nt; t++) lsum += c->t_lambda[t][jj[c->tt[t]]];
-      int sing = !c->has_dirichlet && lsum == 0.0;
+      double lsum = op_lam(o, mode);
+      for (int t = 0; t < o->nt; t++) lsum += o->t_lambda[t][jj[o->tt[t]]];
+      int sing = !o->has_dirichlet && lsum == 0.0;
       int mm = sing ? m - 1 : m; /* singular line: pin the last unknown */
       for (int q = 0; q < mm; q++) {
         long cell = base + q * st[e];
-        int idx[3]; for (int o = 0; o < 3; o++) idx[o] = (o == e) ? q : jj[o];
+        int idx[3]; for (int oo = 0; oo < 3; oo++) idx[oo] = (oo == e) ? q : jj[oo];
         double other = 1.0;
-        for (int o = 0; o < nd; o++) if (o != e) other *= B->df[c->ax[o]][idx[o] + 1];
+        for (int oo = 0; oo < nd; oo++) if (oo != e) other *= B->df[o->ax[oo]][idx[oo] + 1];
         double rm = 1.0 / B->dc[a][q] * other, rp = 1.0 / B->dc[a][q + 1] * other;
         double lo = rm, up = rp, dg = -(rm + rp);
         if (c->periods[a] == 0) {
           if (B->is_bound[0][a] && q == 0) { dg += sg0 * rm; lo = 0.0; }
           if (B->is_bound[1][a] && q == m - 1) { dg += sg1 * rp; up = 0.0; }
         }
-        dg += lsum * c->wgt[cell];
+        dg += lsum * o->wgt[cell];
         double rhs = y[cell];
         if (q == 0) { cp[0] = up / dg; dp[0] = rhs / dg; }
         else { double den = dg - lo * cp[q - 1]; cp[q] = up / den; dp[q] = (rhs - lo * dp[q - 1]) / den; }
@@ -1606,15 +1800,15 @@ static int solve_tensor(oc_case *c, int mode, const double *b, double *x) {
     }
   free(cp); free(dp);
   /* inverse transforms */
-  for (int t = c->nt - 1; t >= 0; t--) {
-    int et = c->tt[t], p1 = -1, p2 = -1;
+  for (int t = o->nt - 1; t >= 0; t--) {
+    int et = o->tt[t], p1 = -1, p2 = -1;
     for (int q = 0; q < 3; q++) if (q != et) { if (p1 < 0) p1 = q; else p2 = q; }
     for (int j2 = 0; j2 < nn[p2]; j2++)
       for (int j1 = 0; j1 < nn[p1]; j1++) {
         long base = j1 * st[p1] + j2 * st[p2];
         for (int q = 0; q < nn[et]; q++) lin[q] = y[base + q * st[et]];
-        oc_r2r(c->t_kind_bwd[t], nn[et], lin, lout);
-        for (int q = 0; q < nn[et]; q++) y[base + q * st[et]] = lout[q] * c->t_norm[t];
+        oc_r2r(o->t_kind_bwd[t], nn[et], lin, lout);
+        for (int q = 0; q < nn[et]; q++) y[base + q * st[et]] = lout[q] * o->t_norm[t];
       }
   }
   memcpy(x, y, sizeof(double) * n);
@@ -1629,112 +1823,192 @@ int oc_set_solver(oc_case *c, int method, double tol, int maxiter) {
   return 0;
 }
 
-static int pick_method(oc_case *c) {
+static int pick_method(oc_case *c, oop *o) {
   int m = c->method;
   if (m == OC_SOLVE_AUTO) {
-    if (tensor_setup(c)) return OC_SOLVE_TENSOR;
-    if (band_setup(c)) return OC_SOLVE_BAND;
+    if (tensor_setup(c, o)) return OC_SOLVE_TENSOR;
+    if (band_setup(c, o)) return OC_SOLVE_BAND;
     return OC_SOLVE_CG;
   }
-  if (m == OC_SOLVE_TENSOR && !tensor_setup(c)) return -1;
-  if (m == OC_SOLVE_BAND && !band_setup(c)) return -1;
+  if (m == OC_SOLVE_TENSOR && !tensor_setup(c, o)) return -1;
+  if (m == OC_SOLVE_BAND && !band_setup(c, o)) return -1;
   return m;
 }
 
-int oc_solve_operator(oc_case *c, int mode, const double *b, double *x) {
-  if (mode < 0 || mode >= c->nmodes) return -1;
-  int m = pick_method(c);
+static int solve_op(oc_case *c, oop *o, int mode, const double *b, double *x) {
+  if (mode < 0 || mode >= o->nmodes) return -1;
+  int m = pick_method(c, o);
   switch (m) {
-  case OC_SOLVE_TENSOR: return solve_tensor(c, mode, b, x);
-  case OC_SOLVE_BAND: return solve_band(c, mode, b, x);
-  case OC_SOLVE_CG: return solve_cg(c, mode, b, x) >= c->maxiter ? -2 : 0;
+  case OC_SOLVE_TENSOR: return solve_tensor(c, o, mode, b, x);
+  case OC_SOLVE_BAND: return solve_band(o, mode, b, x);
+  case OC_SOLVE_CG: return solve_cg(c, o, mode, b, x) >= c->maxiter ? -2 : 0;
   default: return -1;
   }
 }
 
-/* gather/scatter one mode plane (2D) or the whole interior (3D)             */
-static void gather(const oc_case *c, int fid, int mode, double *vec) {
+int oc_solve_operator(oc_case *c, int mode, const double *b, double *x) { return solve_op(c, &c->P, mode, b, x); }
+
+/* gather/scatter the unknowns of one mode plane (2D) or of the whole
+ * interior (3D) of a ghosted field                                          */
+static void gather(const oc_case *c, const oop *o, const double *const *F, int mode, double *vec) {
   int d = c->fft_axis - 1;
   for (int ib = 0; ib < c->nb; ib++) {
     const ob *B = &c->b[ib];
-    const double *F = B->f[fid];
-    if (c->nd == 3) {
-      long q = B->off;
-      for (int k = 1; k <= B->n[2]; k++)
-        for (int j = 1; j <= B->n[1]; j++)
-          for (int i = 1; i <= B->n[0]; i++) vec[q++] = F[IDX(B, i, j, k)];
+    const int *e = o->ext[ib];
+    long q = o->off[ib];
+    if (o->nd == 3) {
+      for (int k = 1; k <= e[2]; k++)
+        for (int j = 1; j <= e[1]; j++)
+          for (int i = 1; i <= e[0]; i++) vec[q++] = F[ib][IDX(B, i, j, k)];
     } else {
-      int a0 = c->ax[0], a1 = c->ax[1];
-      long q = B->off;
-      for (int i2 = 1; i2 <= B->n[a1]; i2++)
-        for (int i1 = 1; i1 <= B->n[a0]; i1++) {
+      int a0 = o->ax[0], a1 = o->ax[1];
+      for (int i2 = 1; i2 <= e[a1]; i2++)
+        for (int i1 = 1; i1 <= e[a0]; i1++) {
           int ijk[3]; ijk[d] = mode + 1; ijk[a0] = i1; ijk[a1] = i2;
-          vec[q++] = F[IDX(B, ijk[0], ijk[1], ijk[2])];
+          vec[q++] = F[ib][IDX(B, ijk[0], ijk[1], ijk[2])];
         }
     }
   }
 }
-static void scatter(oc_case *c, int fid, int mode, const double *vec) {
+static void scatter(const oc_case *c, const oop *o, double *const *F, int mode, const double *vec) {
   int d = c->fft_axis - 1;
   for (int ib = 0; ib < c->nb; ib++) {
-    ob *B = &c->b[ib];
-    double *F = B->f[fid];
-    if (c->nd == 3) {
-      long q = B->off;
-      for (int k = 1; k <= B->n[2]; k++)
-        for (int j = 1; j <= B->n[1]; j++)
-          for (int i = 1; i <= B->n[0]; i++) F[IDX(B, i, j, k)] = vec[q++];
+    const ob *B = &c->b[ib];
+    const int *e = o->ext[ib];
+    long q = o->off[ib];
+    if (o->nd == 3) {
+      for (int k = 1; k <= e[2]; k++)
+        for (int j = 1; j <= e[1]; j++)
+          for (int i = 1; i <= e[0]; i++) F[ib][IDX(B, i, j, k)] = vec[q++];
     } else {
-      int a0 = c->ax[0], a1 = c->ax[1];
-      long q = B->off;
-      for (int i2 = 1; i2 <= B->n[a1]; i2++)
-        for (int i1 = 1; i1 <= B->n[a0]; i1++) {
+      int a0 = o->ax[0], a1 = o->ax[1];
+      for (int i2 = 1; i2 <= e[a1]; i2++)
+        for (int i1 = 1; i1 <= e[a0]; i1++) {
           int ijk[3]; ijk[d] = mode + 1; ijk[a0] = i1; ijk[a1] = i2;
-          F[IDX(B, ijk[0], ijk[1], ijk[2])] = vec[q++];
+          F[ib][IDX(B, ijk[0], ijk[1], ijk[2])] = vec[q++];
         }
     }
   }
 }
 
-/* in-place r2r along the FFT axis on the interior of pp, every block        */
-static void fft_axis_transform(oc_case *c, int kind, double scale) {
+/* in-place r2r of length n along the FFT axis over the interior lines of a
+ * field, every block; the result scaled                                    */
+static void fft_axis_transform(oc_case *c, double *const *F, int kind, int n, double scale) {
   int d = c->fft_axis - 1;
   for (int ib = 0; ib < c->nb; ib++) {
     ob *B = &c->b[ib];
     int a1, a2; tang(d, &a1, &a2);
-    int n = B->n[d];
-    double *F = B->f[4];
+    double *G = F[ib];
 #pragma omp parallel
     {
       double *lin = xcalloc(n, sizeof(double)), *lout = xcalloc(n, sizeof(double));
 #pragma omp for collapse(2) schedule(static)
       for (int t2 = 1; t2 <= B->n[a2]; t2++)
         for (int t1 = 1; t1 <= B->n[a1]; t1++) {
-          for (int q = 0; q < n; q++) lin[q] = F[plane_idx(B, d, q + 1, t1, t2)];
+          for (int q = 0; q < n; q++) lin[q] = G[plane_idx(B, d, q + 1, t1, t2)];
           oc_r2r(kind, n, lin, lout);
-          for (int q = 0; q < n; q++) F[plane_idx(B, d, q + 1, t1, t2)] = lout[q] * scale;
+          for (int q = 0; q < n; q++) G[plane_idx(B, d, q + 1, t1, t2)] = lout[q] * scale;
         }
       free(lin); free(lout);
     }
   }
 }
 
-/* main.f90:1004-1022: transform, per-mode solve (po = warm start and is
- * overwritten with the solution, solver.f90:909-920), inverse, scale        */
-int oc_solve_pressure(oc_case *c) {
-  long n = c->nunk;
+/* transform, per-mode solve, inverse transform (main.f90:1004-1022,
+ * solve_helmholtz.f90:83-99); sol (may be NULL) is the warm start and
+ * receives the solution (po / phio, solver.f90:909-920)                     */
+static int solve_field(oc_case *c, oop *o, double *const *F, double *const *sol) {
+  long n = o->nunk;
   double *b = xcalloc(n, sizeof(double)), *x = xcalloc(n, sizeof(double));
   int rc = 0;
-  if (c->nd == 2) fft_axis_transform(c, c->kind_fwd, 1.0);
-  for (int m = 0; m < c->nmodes && rc == 0; m++) {
-    gather(c, 4, m, b);
-    gather(c, 5, m, x);
-    rc = oc_solve_operator(c, m, b, x);
-    scatter(c, 4, m, x);
-    scatter(c, 5, m, x);
+  int nf = o->nd == 2 ? c->b[0].n[c->fft_axis - 1] - o->qfft : 0;
+  if (o->nd == 2) fft_axis_transform(c, F, o->kind_fwd, nf, 1.0);
+  for (int m = 0; m < o->nmodes && rc == 0; m++) {
+    gather(c, o, (const double *const *)F, m, b);
+    if (sol) gather(c, o, (const double *const *)sol, m, x);
+    else memset(x, 0, sizeof(double) * n);
+    rc = solve_op(c, o, m, b, x);
+    scatter(c, o, F, m, x);
+    if (sol) scatter(c, o, sol, m, x);
   }
-  if (c->nd == 2) fft_axis_transform(c, c->kind_bwd, c->normfft);
+  if (o->nd == 2) {
+    fft_axis_transform(c, F, o->kind_bwd, nf, 1.0);
+    /* the normalisation covers every interior point (main.f90:1020) */
+    for (int ib = 0; ib < c->nb; ib++) {
+      ob *B = &c->b[ib];
+      for (int k = 1; k <= B->n[2]; k++)
+        for (int j = 1; j <= B->n[1]; j++)
+          for (int i = 1; i <= B->n[0]; i++) F[ib][IDX(B, i, j, k)] *= o->normfft;
+    }
+  }
   free(b); free(x);
+  return rc;
+}
+
+int oc_solve_pressure(oc_case *c) {
+  double *F[MAXB], *S[MAXB];
+  for (int ib = 0; ib < c->nb; ib++) { F[ib] = c->b[ib].f[4]; S[ib] = c->b[ib].f[5]; }
+  return solve_field(c, &c->P, F, S);
+}
+
+/* ------------------------------------------------------------------------ */
+/* implicit diffusion (_IMPDIFF)                                             */
+/* ------------------------------------------------------------------------ */
+int oc_set_impdiff(oc_case *c, int on, char *err, int errlen) {
+  for (int q = 0; q < 3 + MAXS; q++) free_op(&c->H[q]);
+  c->impdiff = 0;
+  if (!on) return 0;
+  for (int q = 0; q < 3 + c->nscal; q++) {
+    oop *o = &c->H[q];
+    o->which = q + 1;
+    o->stag = q < 3 ? q : -1;
+    if (assemble_op(c, o, err, errlen)) {
+      for (int r = 0; r <= q; r++) free_op(&c->H[r]);
+      return -1;
+    }
+  }
+  c->impdiff = 1;
+  return 0;
+}
+
+/* solve_impdiff_field solve_helmholtz.f90:30-100 (_fft) / 101-136 (_3d):
+ * phi = phi * alphai * dx2 (non-FFT axes), updt_rhs with the BC planes over
+ * the unknowns, then the Helmholtz solve with alphai on the weighted
+ * diagonal (add_weighted_constant_to_(n_)diagonal(s)).  which: 1..3 = u, v,
+ * w, 4+i = scalar i; alpha = -visc*dtrk/2 (-alpha_s*dtrk/2 for a scalar).   */
+int oc_solve_impdiff(oc_case *c, int which, double alpha) {
+  if (!c->impdiff || which < 1 || which > 3 + c->nscal) return -1;
+  oop *o = &c->H[which - 1];
+  const double alphai = 1.0 / alpha;
+  double *F[MAXB];
+  for (int ib = 0; ib < c->nb; ib++) {
+    ob *B = &c->b[ib];
+    F[ib] = which <= 3 ? B->f[which - 1] : B->sv[which - 4];
+    double *G = F[ib];
+    for (int k = 1; k <= B->n[2]; k++)
+      for (int j = 1; j <= B->n[1]; j++)
+        for (int i = 1; i <= B->n[0]; i++) {
+          int ijk[3] = {i, j, k};
+          double v = G[IDX(B, i, j, k)] * alphai;
+          for (int a = 0; a < 3; a++)
+            if (a != c->fft_axis - 1) v = v * wdx2(o, B, a, ijk[a]);
+          G[IDX(B, i, j, k)] = v;
+        }
+    /* updt_rhs bound.f90:543-585 with hi = the unknowns' upper bound */
+    for (int d = 0; d < 3; d++)
+      for (int s = 0; s < 2; s++) {
+        if (!B->is_bound[s][d] || !o->rhs[ib][d][s]) continue;
+        int a1, a2; tang(d, &a1, &a2);
+        int q = s == 0 ? 1 : o->ext[ib][d];
+        long m1 = o->ext[ib][a1], m2 = o->ext[ib][a2];
+        for (int t2 = 1; t2 <= m2; t2++)
+          for (int t1 = 1; t1 <= m1; t1++)
+            G[plane_idx(B, d, q, (int)t1, (int)t2)] += o->rhs[ib][d][s][(t1 - 1) + m1 * (t2 - 1)];
+      }
+  }
+  o->shift = alphai;
+  int rc = solve_field(c, o, F, NULL);
+  o->shift = 0.0;
   return rc;
 }
 
@@ -1751,9 +2025,21 @@ int oc_step(oc_case *c, double dt) {
     double dtrk = (RKC[irk - 1][0] + RKC[irk - 1][1]) * dt;
     if (c->nscal > 0) {
       oc_rk_scal(c, irk, dt, NULL);
-      for (int is = 0; is < c->nscal; is++) oc_boundp(c, 9 + 2 * is);
+      for (int is = 0; is < c->nscal; is++) {
+        if (c->impdiff) { /* main.f90:914-936 */
+          int rc = oc_solve_impdiff(c, 4 + is, -c->salpha[is] * dtrk / 2.0);
+          if (rc) return rc;
+        }
+        oc_boundp(c, 9 + 2 * is);
+      }
     }
+    const double alpha = -c->visc * dtrk / 2.0;
     oc_rk_mom(c, irk, dt);
+    if (c->impdiff) /* main.f90:947-997 */
+      for (int q = 1; q <= 3; q++) {
+        int rc = oc_solve_impdiff(c, q, alpha);
+        if (rc) return rc;
+      }
     oc_bounduvw(c, 0);
     oc_inflow(c, 0);
     oc_fillps(c, dtrk);
@@ -1763,7 +2049,8 @@ int oc_step(oc_case *c, double dt) {
     oc_correc(c, dtrk);
     oc_bounduvw(c, 1);
     oc_inflow(c, 1);
-    oc_updt_pressure(c);
+    if (c->impdiff) oc_updt_pressure_impdiff(c, alpha);
+    else oc_updt_pressure(c);
     oc_boundp(c, 3);
   }
   return 0;

@@ -91,6 +91,8 @@ int  oc_solve_pressure(oc_case *c);
 void oc_correc(oc_case *c, double dtrk);
 /* updt_pressure.f90:23-29                                                    */
 void oc_updt_pressure(oc_case *c);
+/* updt_pressure.f90:23-45 of the _IMPDIFF build (alpha = -visc*dtrk/2)       */
+void oc_updt_pressure_impdiff(oc_case *c, double alpha);
 /* chkdiv.f90:8-44                                                            */
 void oc_chkdiv(const oc_case *c, double *divtot, double *divmax);
 /* chkdt.f90:8-56                                                             */
@@ -99,6 +101,16 @@ double oc_chkdt(const oc_case *c);
 int  oc_step(oc_case *c, double dt);
 /* initial boundary fill as main.f90:562-639 (bounduvw, boundp, inflow)       */
 void oc_init_bounds(oc_case *c);
+
+/* ---- implicit diffusion (the reference's _IMPDIFF build) ----
+ * oc_set_impdiff assembles the Helmholtz operators of u, v, w and of every
+ * scalar (main.f90:769-886; call after oc_set_scalars) and switches rk_mom,
+ * rk_scal, updt_pressure, chkdt and oc_step to their _IMPDIFF forms; an
+ * unsupported case returns -1 with a message in err.  oc_solve_impdiff is
+ * solve_impdiff_field (solve_helmholtz.f90:30-136) of one field: which 1..3 =
+ * u, v, w, 4+i = scalar i; alpha = -visc*dtrk/2 (-alpha_s*dtrk/2).          */
+int  oc_set_impdiff(oc_case *c, int on, char *err, int errlen);
+int  oc_solve_impdiff(oc_case *c, int which, double alpha);
 
 /* ---- transforms and eigenvalues (fft.f90:96-189) ---- */
 /* direct O(n^2) definitions of FFTW's r2r kinds, on one contiguous line      */

@@ -74,6 +74,9 @@ def lib():
         L.oc_apply_operator.argtypes = [C.c_void_p, C.c_int, dp, dp]
         L.oc_solve_operator.argtypes = [C.c_void_p, C.c_int, dp, dp]
         L.oc_get_lambda.argtypes = [C.c_void_p, dp, dp]
+        L.oc_set_impdiff.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_int]
+        L.oc_solve_impdiff.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.oc_updt_pressure_impdiff.argtypes = [C.c_void_p, C.c_double]
         _lib = L
     return _lib
 
@@ -112,6 +115,27 @@ class OracleCase:
                              np.ascontiguousarray(case["is_sforced"], dtype=np.int32),
                              np.ascontiguousarray(case["scalf"], dtype=float), _chars(case["cbcscal"]),
                              np.ascontiguousarray(case["bcscal"], dtype=float).reshape(-1))
+
+        if case.get("impdiff"):
+            self.set_impdiff(True)
+
+    def set_impdiff(self, on=True):
+        """Implicit diffusion (the reference's _IMPDIFF build)."""
+        err = C.create_string_buffer(256)
+        if lib().oc_set_impdiff(self.h, int(on), err, 256):
+            raise ValueError("oracle rejected implicit diffusion: " + err.value.decode())
+
+    def solve_impdiff(self, which, alpha):
+        """solve_impdiff_field of u/v/w or scalar s<i> with alpha = -nu*dtrk/2."""
+        q = {"u": 1, "v": 2, "w": 3}.get(which)
+        if q is None:
+            q = 3 + int(which[1:])
+        rc = lib().oc_solve_impdiff(self.h, q, float(alpha))
+        if rc:
+            raise RuntimeError(f"oracle Helmholtz solve failed ({rc})")
+
+    def updt_pressure_impdiff(self, alpha):
+        lib().oc_updt_pressure_impdiff(self.h, float(alpha))
 
     def __del__(self):
         if getattr(self, "h", None):
