@@ -169,8 +169,29 @@ int snac_correc(snac_ctx *ctx, double dtrk);
 int snac_updt_pressure(snac_ctx *ctx);
 /* chkdiv chkdiv.f90:8-44 (synchronises)                                     */
 int snac_chkdiv(snac_ctx *ctx, double *divtot, double *divmax);
-/* chkdt chkdt.f90:8-56 (synchronises)                                       */
+/* chkdt chkdt.f90:8-56 (synchronises); without the diffusive limit while
+ * implicit diffusion is on (chkdt.f90:51-55)                                */
 int snac_chkdt(snac_ctx *ctx, double *dtmax);
+
+/* ---- implicit diffusion: the reference's _IMPDIFF build as a run-time
+ *      switch.  On, snac_rk_mom keeps the diffusion out of dudtrk and half
+ *      of it in the update (rk.f90:29-127), snac_step solves the Helmholtz
+ *      problem (I - visc*dtrk/2 L) u = u* of each velocity component after
+ *      rk_mom (main.f90:947-997) and adds alpha*L(pp) in updt_pressure
+ *      (updt_pressure.f90:30-45), and snac_chkdt drops the diffusive limit.
+ *      Needs the tensor-product solver (a box-shaped case with uniform
+ *      transformed axes) and no scalars; a staggered pair along a
+ *      transformed axis must be periodic or Dirichlet-Dirichlet.  ---- */
+int snac_set_impdiff(snac_ctx *ctx, int enable);
+/* solve_impdiff_field solve_helmholtz.f90:30-136 of fid = SNAC_U, SNAC_V or
+ * SNAC_W with alpha = -visc*dtrk/2 (< 0): phi * alphai * dx2 + BC terms
+ * (init_bc_rhs solver.f90:44-157, updt_rhs bound.f90:543-585), solved in
+ * place over the component's unknowns (the bounded upper face along its own
+ * axis excluded)                                                            */
+int snac_solve_impdiff(snac_ctx *ctx, int fid, double alpha);
+/* updt_pressure updt_pressure.f90:7-47 of the _IMPDIFF build:
+ * p += pp + alpha * Laplacian(pp), alpha = -visc*dtrk/2                     */
+int snac_updt_pressure_impdiff(snac_ctx *ctx, double alpha);
 /* bounduvw + boundp + inflow of the initial fields, main.f90:562-639        */
 int snac_init_bounds(snac_ctx *ctx);
 /* one RK3 time step in the order of main.f90:906-1033                       */
@@ -196,7 +217,8 @@ enum {
   SNAC_PH_R2R_BWD = 5,  /* inverse transforms and normfft scaling           */
   SNAC_PH_CORREC = 6,   /* correc + updt_pressure                           */
   SNAC_PH_RK_SCAL = 7,  /* rk_scal + bulk forcing of the scalars            */
-  SNAC_NPHASE = 8
+  SNAC_PH_HELMHOLTZ = 8, /* implicit diffusion: the three Helmholtz solves  */
+  SNAC_NPHASE = 9
 };
 int snac_set_timing(snac_ctx *ctx, int enable);
 /* waits for the stream, returns accumulated milliseconds and the number of

@@ -28,7 +28,7 @@ def _ghosted(fid):
     f = FIELDS[fid]
     return f < 6 or (f >= SCAL0 and (f - SCAL0) % 2 == 0)
 RKCOEFF = ((32.0 / 60.0, 0.0), (25.0 / 60.0, -17.0 / 60.0), (45.0 / 60.0, -25.0 / 60.0))
-PHASES = ("rk_mom", "bound", "fillps", "r2r_fwd", "tridiag", "r2r_bwd", "correc", "rk_scal")
+PHASES = ("rk_mom", "bound", "fillps", "r2r_fwd", "tridiag", "r2r_bwd", "correc", "rk_scal", "helmholtz")
 R2R_KINDS = {"R2HC": 0, "HC2R": 1, "REDFT00": 3, "REDFT01": 4, "REDFT10": 5, "REDFT11": 6,
              "RODFT00": 7, "RODFT01": 8, "RODFT10": 9, "RODFT11": 10}
 
@@ -74,6 +74,9 @@ def _load():
     L.snac_solve_pressure.argtypes = [vp]
     L.snac_correc.argtypes = [vp, C.c_double]
     L.snac_updt_pressure.argtypes = [vp]
+    L.snac_set_impdiff.argtypes = [vp, C.c_int]
+    L.snac_solve_impdiff.argtypes = [vp, C.c_int, C.c_double]
+    L.snac_updt_pressure_impdiff.argtypes = [vp, C.c_double]
     L.snac_chkdiv.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.snac_chkdt.argtypes = [vp, C.POINTER(C.c_double)]
     L.snac_init_bounds.argtypes = [vp]
@@ -225,6 +228,17 @@ class Context:
 
     def updt_pressure(self):
         _check(lib.snac_updt_pressure(self.h), "updt_pressure")
+
+    def set_impdiff(self, enable=True):
+        """Implicit diffusion (the reference's _IMPDIFF build) on or off."""
+        _check(lib.snac_set_impdiff(self.h, int(bool(enable))), "set_impdiff")
+
+    def solve_impdiff(self, which, alpha):
+        """solve_impdiff_field of 'u', 'v' or 'w' with alpha = -visc*dtrk/2."""
+        _check(lib.snac_solve_impdiff(self.h, FIELDS[which], float(alpha)), "solve_impdiff")
+
+    def updt_pressure_impdiff(self, alpha):
+        _check(lib.snac_updt_pressure_impdiff(self.h, float(alpha)), "updt_pressure_impdiff")
 
     def chkdiv(self):
         a, b = C.c_double(), C.c_double()

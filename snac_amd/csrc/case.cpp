@@ -102,9 +102,13 @@ void initgrid(int n, int gt, double gr, double lmin, double lmax, std::vector<do
   }
 }
 
-bool find_fft(char a, char b, int &kf, int &kb, double norm[2]) {
+bool find_fft(char a, char b, int &kf, int &kb, double norm[2], bool centred) {
   // FFTW kind numbers (fftw.f90:31-54)
   if (a == 'F' && b == 'F') { kf = 0; kb = 1; norm[0] = 1; norm[1] = 0; return true; }
+  if (!centred) {  // staggered: DD only (RODFT00 over n-1 points, norm 2(n+1-1))
+    if (a == 'D' && b == 'D') { kf = 7; kb = 7; norm[0] = 2; norm[1] = 1; return true; }
+    return false;
+  }
   if (a == 'N' && b == 'N') { kf = 5; kb = 4; norm[0] = 2; norm[1] = 0; return true; }
   if (a == 'D' && b == 'D') { kf = 9; kb = 8; norm[0] = 2; norm[1] = 0; return true; }
   if (a == 'N' && b == 'D') { kf = 6; kb = 6; norm[0] = 2; norm[1] = 0; return true; }
@@ -112,9 +116,17 @@ bool find_fft(char a, char b, int &kf, int &kb, double norm[2]) {
   return false;
 }
 
-bool eigenvalues(int n, char a, char b, std::vector<double> &lam) {
+bool eigenvalues(int n, char a, char b, std::vector<double> &lam, bool centred) {
   const double pi = std::acos(-1.0);
   lam.assign(n, 0.0);
+  if (!centred && !(a == 'F' && b == 'F')) {
+    if (!(a == 'D' && b == 'D')) return false;
+    for (int l = 1; l <= n - 1; l++) {
+      const double s = std::sin((double)l * pi / (2.0 * n));
+      lam[l - 1] = -4.0 * s * s;
+    }
+    return true;
+  }
   for (int l = 1; l <= n; l++) {
     double s;
     if (a == 'F' && b == 'F') s = std::sin((double)(l - 1) * pi / (double)n);

@@ -82,8 +82,11 @@ Case build_case(const std::string &dns_nml, const std::string &blocks_nml, int f
 void initgrid(int n, int gt, double gr, double lmin, double lmax, std::vector<double> &drc,
               std::vector<double> &drf, std::vector<double> &rc, std::vector<double> &rf);
 
-// fft.f90:96-149 and 151-189 for centered pressure BC pairs.
-bool find_fft(char bc0, char bc1, int &kind_fwd, int &kind_bwd, double norm[2]);
-bool eigenvalues(int n, char bc0, char bc1, std::vector<double> &lambda);
+// fft.f90:96-149 and 151-189: centred BC pairs (the pressure, scalars and
+// the velocity components across their own axis) and, with centred false,
+// the staggered pairs FF and DD (a velocity component along its own axis;
+// DD transforms the first n-1 points, its n-th eigenvalue is left zero)
+bool find_fft(char bc0, char bc1, int &kind_fwd, int &kind_bwd, double norm[2], bool centred = true);
+bool eigenvalues(int n, char bc0, char bc1, std::vector<double> &lambda, bool centred = true);
 
 }  // namespace snac

@@ -74,11 +74,13 @@ void launch_cell_segs(hipStream_t s, const CellSeg *dev_segs, int nseg);
 // s1 (scalar 1, ghosted) non-null adds the Boussinesq term with gb = gacc*beta;
 // variant 0: one cell per thread; 1..6: k-march over chunks of 8/16/32
 // planes with 64x4 or 32x8 tiles; 7..10: k-march with LDS-staged planes,
-// 64x4 / 64x8 tiles over 16 / 32 planes (autotuned)
+// 64x4 / 64x8 tiles over 16 / 32 planes (autotuned).  impdiff: the diffusion
+// is kept out of dudtrk and half of it stays in the update (the reference's
+// _IMPDIFF build, rk.f90:29-127); the Helmholtz solves add the other half.
 void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v,
                    const double *w, const double *p, double *un, double *vn, double *wn, double *dtu,
                    double *dtv, double *dtw, double f1, double f2, double f12, double visc, double bfx,
-                   double bfy, double bfz, const double *s1, const double gb[3], int variant);
+                   double bfy, double bfz, const double *s1, const double gb[3], int variant, bool impdiff);
 // rk_scal of one scalar: reads sc (ghosted), writes the interior of sn and dso
 void launch_rk_scal(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v, const double *w,
                     const double *sc, double *sn, double *dso, double alpha, double f1, double f2, double f12,
@@ -105,6 +107,25 @@ void launch_chkdt_part(hipStream_t s, const Geo &g, const Metrics &m, const doub
                        const double *w, double *part_max);
 // reduces n partial sums and maxima into out[0] (sum) and out[1] (max)
 void launch_reduce_final(hipStream_t s, const double *part_sum, const double *part_max, int n, double *out);
+
+// ---- implicit diffusion (kernels_impdiff.hip) ----
+// Right-hand side of one field's Helmholtz problem over a block: the
+// unknowns' extents (the block's, less the bounded upper face along a
+// velocity component's own axis), per-axis weight factors (dx2 of the
+// non-FFT axes, 1 along the FFT axis; n entries each, device) and the BC
+// planes of the bounded faces over the unknowns (first tangential axis
+// fastest; null where absent).
+struct ImpRhs {
+  int ext[3];
+  const double *w[3];
+  const double *plane[3][2];
+};
+// out (ghosted, same layout as phi) = phi*alphai*w0*w1*w2 + BC planes over
+// the unknowns, 0 past them
+void launch_imp_rhs(hipStream_t s, const Geo &g, const ImpRhs &r, const double *phi, double *out, double alphai);
+// updt_pressure of the _IMPDIFF build: p += pp + alpha * Laplacian(pp)
+void launch_updt_pressure_imp(hipStream_t s, const Geo &g, const Metrics &m, const double *pp, double *p,
+                              double alpha);
 
 // plain device copy of n doubles (n even), the HBM reference kernel
 void launch_stream_copy(hipStream_t s, const double *a, double *b, long n);
@@ -133,7 +154,8 @@ double r2r_coef(int kind, int n, int k, int j);
 // ---- tridiagonal line solves (kernels_tridiag.hip) ----
 // Lines along axis s of a (G0,G1,G2) array (i fastest).  For the line at
 // transformed indices (t0,t1) of the other two axes:
-//   lam = lamA[idxA] + lamB[idxB] ; diag[g] = d0[g] + lam * wt[g]
+//   lam = lamA[idxA] + lamB[idxB] + shift ; diag[g] = d0[g] + lam * wt[g]
+// (shift: 1/alpha of an implicit-diffusion Helmholtz solve, 0 for the pressure)
 //   lower[g] = am[g], upper[g] = ap[g]
 // idxA/idxB are the indices along the two non-line axes (lamA/lamB sized to
 // them).  Lines with lam == 0 and `pin_when_zero` get their last unknown
@@ -145,6 +167,7 @@ struct TridiagLine {
   const double *am, *ap, *d0, *wt;  // size G[s]
   const double *lamA, *lamB;        // along the other axes (ascending order)
   int pin_when_zero;
+  double shift = 0.0;
 };
 // dscratch (d') and cscratch (c', used only for lines longer than the LDS
 // budget) have the size and layout of x

@@ -44,6 +44,17 @@ struct Program {
   int nseg = 0;
 };
 
+// The operator-dependent half of a tensor-product solve: the transforms of
+// each transformed axis, their eigenvalues and the line coefficients.  The
+// pressure's, and with implicit diffusion one per velocity component.
+struct TensorOps {
+  std::vector<R2RPlan> fwd, bwd;   // per transformed axis (n-1 points on a staggered DD axis)
+  std::vector<double> norm;
+  double *lam[3] = {};             // device eigenvalue arrays per axis (line axis: zeros)
+  double *am = nullptr, *ap = nullptr, *d0 = nullptr, *wt = nullptr;
+  int pin = 0;
+};
+
 // Tensor-product Poisson plan: r2r along every uniform axis (the FFT axis
 // first), tridiagonal solves along the remaining axis.
 struct TensorPlan {
@@ -51,15 +62,20 @@ struct TensorPlan {
   std::string why;
   int G[3] = {1, 1, 1}, gmin[3] = {0, 0, 0};
   std::vector<int> xaxes;          // transformed axes, FFT axis first
-  std::vector<R2RPlan> fwd, bwd;
-  std::vector<double> norm;
   int s = -1;
-  double *lam[3] = {};             // device eigenvalue arrays per axis (line axis: zeros)
-  double *am = nullptr, *ap = nullptr, *d0 = nullptr, *wt = nullptr;
-  int pin = 0;
+  bool all_tr = false;             // every axis periodic: the line axis is transformed too
+  std::vector<double> gdf[3], gdcm[3], gdcp[3];  // global 1D grids: df, dc below / above each cell
+  TensorOps P;                     // the pressure operator
   int tvariant = 1;                // tridiagonal kernel variant (autotuned)
   double *ph = nullptr, *scratch = nullptr, *scratch2 = nullptr;
   std::vector<double> tune_ms;     // best time per tuned stage (diagnostics)
+};
+
+// Implicit diffusion of one velocity component: its tensor operator and,
+// per block, the right-hand-side description (weights, BC planes)
+struct ImpField {
+  TensorOps ops;
+  std::vector<ImpRhs> rhs;
 };
 
 }  // namespace
@@ -79,6 +95,8 @@ struct snac_ctx {
   std::vector<snac::Program> prog_scal_one[2];  // boundp of one scalar
   double *sred = nullptr, *sforce = nullptr;  // scalar bulk mean / forcing
   snac::TensorPlan tp;
+  bool impdiff = false;    // implicit diffusion (the reference's _IMPDIFF build)
+  snac::ImpField imp[3];   // its Helmholtz operators of u, v, w
   snac::MGPlan mg;
   bool mg_planned = false;
   bool mg_fixed = false;  // solves run a fixed iteration count (graph capture)
@@ -222,6 +240,102 @@ void collect_timing(snac_ctx *c) {
   c->pending.clear();
 }
 
+// BC character of field comp (-1 pressure, 0..2 velocity component) on the
+// global face (axis a, side) of the plan's box: 'F' where periodic
+bool box_bc(snac_ctx *c, int comp, int a, int side, char &out, std::string &why) {
+  const TensorPlan &t = c->tp;
+  out = 0;
+  for (auto &B : c->cs.blocks) {
+    const bool on = side == 0 ? B.lo_g[a] == t.gmin[a] : B.lo_g[a] + B.n[a] == t.gmin[a] + t.G[a];
+    if (!on) continue;
+    const char bc = !B.is_bound[a][side] ? 'F' : (comp < 0 ? B.cbcpre[a][side] : B.cbcvel[comp][a][side]);
+    if (out && out != bc) { why = "mixed boundary conditions on one face"; return false; }
+    out = bc;
+  }
+  return true;
+}
+
+// Transforms, eigenvalues and line coefficients of one operator on the
+// plan's axes: the pressure (comp -1, centred) or the Helmholtz operator of
+// velocity component comp (implicit diffusion): staggered along its own axis
+// (is_centered = .false., main.f90:774-776), i.e. unknowns on the faces (the
+// bounded upper face excluded: an identity row of the line solve, or a
+// transform over n-1 points), couplings 1/dxf, weights dxc, Dirichlet values
+// on the boundary face itself (sgn 0).  init_matrix_2d/3d solver.f90:158-372,
+// 687-842, init_fft_reduction 1124-1138, fft.f90:96-189.
+bool build_ops(snac_ctx *c, TensorOps &o, int comp, std::string &why) {
+  TensorPlan &t = c->tp;
+  const Case &cs = c->cs;
+  const int fa = cs.fft_axis - 1, s = t.s;
+  char lo[3], hi[3];
+  for (int a = 0; a < 3; a++)
+    if (!box_bc(c, comp, a, 0, lo[a], why) || !box_bc(c, comp, a, 1, hi[a], why)) return false;
+  std::vector<double> lamh[3];
+  for (int a = 0; a < 3; a++) lamh[a].assign(t.G[a], 0.0);
+  for (int a : t.xaxes) {
+    int kf, kb;
+    double nr[2];
+    const bool centred = a != comp;
+    char c0 = (a == fa && comp < 0) ? cs.blocks[0].cbcpre[a][0] : lo[a];
+    char c1 = (a == fa && comp < 0) ? cs.blocks[0].cbcpre[a][1] : hi[a];
+    if (!find_fft(c0, c1, kf, kb, nr, centred)) {
+      why = std::string("unsupported BC pair ") + c0 + c1 + (centred ? "" : " (staggered)") +
+            " along a transformed axis";
+      return false;
+    }
+    const int q = (!centred && c0 != 'F') ? 1 : 0;  // fftini fft.f90:31-33
+    if (c->device >= 0) {
+      o.fwd.push_back(r2r_plan(kf, t.G[a] - q));
+      o.bwd.push_back(r2r_plan(kb, t.G[a] - q));
+    }
+    o.norm.push_back(1.0 / (nr[0] * (t.G[a] + nr[1] - q)));
+    eigenvalues(t.G[a], c0, c1, lamh[a], centred);
+    // spacing used by the reference: dxc_g(lo_g-1) of the FFT axis
+    // (main.f90:717-728, dxf_g(lo_g) when staggered); the uniform spacing
+    // for the other axes
+    double h = (a == fa) ? cs.blocks[0].df[a][1] : t.gdf[a][0];
+    for (auto &l : lamh[a]) l /= h * h;
+  }
+  for (int a = 0; a < 3; a++) o.lam[a] = dupload(c, lamh[a]);
+  // line coefficients: h_other = product of the uniform spacings of the
+  // transformed in-plane axes (the FFT axis is not part of the 2D operator)
+  double hother = 1.0;
+  for (int a = 0; a < 3; a++) if (a != fa && a != s) hother *= t.gdf[a][0];
+  const bool stag = comp == s;
+  const int m = t.G[s];
+  const int mu = (stag && hi[s] != 'F') ? m - 1 : m;  // unknowns along the line
+  std::vector<double> am(m), ap(m), d0(m), wt(m);
+  auto sgn = [&](char bc) { return bc == 'N' ? 1.0 : (bc == 'D' ? (stag ? 0.0 : -1.0) : 0.0); };
+  const double sg0 = sgn(lo[s]), sg1 = sgn(hi[s]);
+  for (int g = 0; g < m; g++) {
+    wt[g] = hother * (stag ? t.gdcp[s][g] : t.gdf[s][g]);
+    if (t.all_tr) {  // the line axis is transformed as well: a diagonal system
+      am[g] = ap[g] = 0.0;
+      d0[g] = lamh[s][g] * wt[g];
+      continue;
+    }
+    if (g >= mu) {  // the bounded upper face of a staggered component
+      am[g] = ap[g] = 0.0;
+      d0[g] = 1.0;
+      wt[g] = 0.0;
+      continue;
+    }
+    const double rm = 1.0 / (stag ? t.gdf[s][g] : t.gdcm[s][g]) * hother;
+    const double rp = 1.0 / (stag ? t.gdf[s][g + 1] : t.gdcp[s][g]) * hother;
+    double dg = -(rm + rp);
+    am[g] = rm; ap[g] = rp;
+    if (g == 0) { dg += sg0 * rm; am[g] = 0.0; }
+    if (g == mu - 1) { dg += sg1 * rp; ap[g] = 0.0; }
+    d0[g] = dg;
+  }
+  o.am = dupload(c, am); o.ap = dupload(c, ap); o.d0 = dupload(c, d0); o.wt = dupload(c, wt);
+  // singular N-N line of the pressure: pin its last unknown; the
+  // all-periodic zero mode has a zero pivot instead, which the line solver
+  // maps to x = 0
+  o.pin = (comp < 0 && !t.all_tr && lo[s] == 'N' && hi[s] == 'N') ? 1 : 0;
+  return true;
+}
+
 // ---- tensor Poisson planner ----
 void plan_tensor(snac_ctx *c) {
   TensorPlan &t = c->tp;
@@ -321,53 +435,9 @@ void plan_tensor(snac_ctx *c) {
     if (first < 0) { t.why = "no transform axis spans every block"; return; }
     std::swap(t.xaxes[0], t.xaxes[first]);
   }
-  // transforms, eigenvalues
-  std::vector<double> lamh[3];
-  for (int a = 0; a < 3; a++) lamh[a].assign(t.G[a], 0.0);
-  for (int a : t.xaxes) {
-    int kf, kb;
-    double nr[2];
-    char c0 = (a == fa) ? cs.blocks[0].cbcpre[a][0] : bclo[a];
-    char c1 = (a == fa) ? cs.blocks[0].cbcpre[a][1] : bchi[a];
-    if (!find_fft(c0, c1, kf, kb, nr)) { t.why = "unsupported BC pair along a transformed axis"; return; }
-    if (c->device >= 0) {
-      t.fwd.push_back(r2r_plan(kf, t.G[a]));
-      t.bwd.push_back(r2r_plan(kb, t.G[a]));
-    }
-    t.norm.push_back(1.0 / (nr[0] * (t.G[a] + nr[1])));
-    eigenvalues(t.G[a], c0, c1, lamh[a]);
-    // spacing used by the reference: dxc_g(lo_g-1) of the FFT axis
-    // (main.f90:717-728); the uniform spacing for the other axes
-    double h = (a == fa) ? cs.blocks[0].df[a][1] : gdf[a][0];
-    for (auto &l : lamh[a]) l /= h * h;
-  }
-  // the two non-line axes in ascending order feed lamA / lamB
-  for (int a = 0; a < 3; a++) t.lam[a] = dupload(c, lamh[a]);
-  // line coefficients: h_other = product of the uniform spacings of the
-  // transformed in-plane axes (the FFT axis is not part of the 2D operator)
-  double hother = 1.0;
-  for (int a : red) if (a != s) hother *= gdf[a][0];
-  int m = t.G[s];
-  std::vector<double> am(m), ap(m), d0(m), wt(m);
-  double sg0 = bclo[s] == 'N' ? 1.0 : -1.0, sg1 = bchi[s] == 'N' ? 1.0 : -1.0;
-  for (int g = 0; g < m; g++) {
-    wt[g] = hother * gdf[s][g];
-    if (all_tr) {  // the line axis is transformed as well: a diagonal system
-      am[g] = ap[g] = 0.0;
-      d0[g] = lamh[s][g] * wt[g];
-      continue;
-    }
-    double rm = 1.0 / gdcm[s][g] * hother, rp = 1.0 / gdcp[s][g] * hother;
-    double dg = -(rm + rp);
-    am[g] = rm; ap[g] = rp;
-    if (g == 0) { dg += sg0 * rm; am[g] = 0.0; }
-    if (g == m - 1) { dg += sg1 * rp; ap[g] = 0.0; }
-    d0[g] = dg;
-  }
-  t.am = dupload(c, am); t.ap = dupload(c, ap); t.d0 = dupload(c, d0); t.wt = dupload(c, wt);
-  // singular N-N line: pin its last unknown; the all-periodic zero mode has
-  // a zero pivot instead, which the line solver maps to x = 0
-  t.pin = (!all_tr && bclo[s] == 'N' && bchi[s] == 'N') ? 1 : 0;
+  t.all_tr = all_tr;
+  for (int a = 0; a < 3; a++) { t.gdf[a] = gdf[a]; t.gdcm[a] = gdcm[a]; t.gdcp[a] = gdcp[a]; }
+  if (!build_ops(c, t.P, -1, t.why)) return;
   size_t tot = (size_t)t.G[0] * t.G[1] * t.G[2];
   t.ph = dalloc<double>(c, tot);
   t.scratch = dalloc<double>(c, tot);
@@ -418,7 +488,7 @@ void others(int a, int &b1, int &b2) {
 }
 
 // first transform: block pp interiors -> global work array
-void stage_first(snac_ctx *c) {
+void stage_first(snac_ctx *c, const TensorOps &o) {
   TensorPlan &t = c->tp;
   const long pst[3] = {1, (long)t.G[0], (long)t.G[0] * t.G[1]};
   int a = t.xaxes[0], b1, b2;
@@ -429,37 +499,39 @@ void stage_first(snac_ctx *c) {
     const double *in = D.pp + 1 + D.g.sx + D.g.sx * D.g.sy;
     long off = 0;
     for (int q = 0; q < 3; q++) off += (long)(B.lo_g[q] - t.gmin[q]) * pst[q];
-    launch_r2r(c->stream, t.fwd[0], in, gstride(D, a), gstride(D, b1), gstride(D, b2), t.ph + off, pst[a], pst[b1],
+    launch_r2r(c->stream, o.fwd[0], in, gstride(D, a), gstride(D, b1), gstride(D, b2), t.ph + off, pst[a], pst[b1],
                pst[b2], bext(D, b1), bext(D, b2), 1.0);
   }
 }
 
 // in-place transform number q on the work array (forward or inverse)
-void stage_inplace(snac_ctx *c, size_t q, bool fwd) {
+void stage_inplace(snac_ctx *c, const TensorOps &o, size_t q, bool fwd) {
   TensorPlan &t = c->tp;
   const long pst[3] = {1, (long)t.G[0], (long)t.G[0] * t.G[1]};
   int a = t.xaxes[q], b1, b2;
   others(a, b1, b2);
-  launch_r2r(c->stream, fwd ? t.fwd[q] : t.bwd[q], t.ph, pst[a], pst[b1], pst[b2], t.ph, pst[a], pst[b1], pst[b2],
-             t.G[b1], t.G[b2], fwd ? 1.0 : t.norm[q]);
+  launch_r2r(c->stream, fwd ? o.fwd[q] : o.bwd[q], t.ph, pst[a], pst[b1], pst[b2], t.ph, pst[a], pst[b1], pst[b2],
+             t.G[b1], t.G[b2], fwd ? 1.0 : o.norm[q]);
 }
 
-void stage_tridiag(snac_ctx *c) {
+void stage_tridiag(snac_ctx *c, const TensorOps &o, double shift) {
   TensorPlan &t = c->tp;
   TridiagLine tl;
   for (int q = 0; q < 3; q++) tl.G[q] = t.G[q];
   tl.s = t.s;
   tl.variant = t.tvariant;
   int o1 = t.s == 0 ? 1 : 0, o2 = t.s == 2 ? 1 : 2;
-  tl.am = t.am; tl.ap = t.ap; tl.d0 = t.d0; tl.wt = t.wt;
-  tl.lamA = t.lam[o1];
-  tl.lamB = t.lam[o2];
-  tl.pin_when_zero = t.pin;
+  tl.am = o.am; tl.ap = o.ap; tl.d0 = o.d0; tl.wt = o.wt;
+  tl.lamA = o.lam[o1];
+  tl.lamB = o.lam[o2];
+  tl.pin_when_zero = o.pin;
+  tl.shift = shift;
   launch_tridiag(c->stream, tl, t.ph, t.scratch, t.scratch2);
 }
 
-// last inverse transform: global work array -> block pp interiors, scaled
-void stage_last(snac_ctx *c) {
+// last inverse transform: global work array -> block interiors (pp for the
+// pressure, velocity component comp for a Helmholtz solve), scaled
+void stage_last(snac_ctx *c, const TensorOps &o, int comp) {
   TensorPlan &t = c->tp;
   const long pst[3] = {1, (long)t.G[0], (long)t.G[0] * t.G[1]};
   int a = t.xaxes[0], b1, b2;
@@ -467,12 +539,123 @@ void stage_last(snac_ctx *c) {
   for (size_t b = 0; b < c->blk.size(); b++) {
     DevBlock &D = c->blk[b];
     const Block &B = c->cs.blocks[b];
-    double *out = D.pp + 1 + D.g.sx + D.g.sx * D.g.sy;
+    double *out = (comp < 0 ? D.pp : D.vel[c->cur][comp]) + 1 + D.g.sx + D.g.sx * D.g.sy;
     long off = 0;
     for (int q = 0; q < 3; q++) off += (long)(B.lo_g[q] - t.gmin[q]) * pst[q];
-    launch_r2r(c->stream, t.bwd[0], t.ph + off, pst[a], pst[b1], pst[b2], out, gstride(D, a), gstride(D, b1),
-               gstride(D, b2), bext(D, b1), bext(D, b2), t.norm[0]);
+    launch_r2r(c->stream, o.bwd[0], t.ph + off, pst[a], pst[b1], pst[b2], out, gstride(D, a), gstride(D, b1),
+               gstride(D, b2), bext(D, b1), bext(D, b2), o.norm[0]);
   }
+}
+
+// the whole tensor-product solve of one operator, pp -> pp (pressure) or
+// pp -> velocity component comp
+void tensor_solve(snac_ctx *c, const TensorOps &o, int comp, double shift) {
+  TensorPlan &t = c->tp;
+  {
+    Phase tf(c, comp < 0 ? SNAC_PH_R2R_FWD : SNAC_PH_HELMHOLTZ);
+    stage_first(c, o);
+    for (size_t q = 1; q < t.xaxes.size(); q++) stage_inplace(c, o, q, true);
+  }
+  {
+    Phase tt(c, comp < 0 ? SNAC_PH_TRIDIAG : SNAC_PH_HELMHOLTZ);
+    stage_tridiag(c, o, shift);
+  }
+  {
+    Phase tb(c, comp < 0 ? SNAC_PH_R2R_BWD : SNAC_PH_HELMHOLTZ);
+    for (size_t q = t.xaxes.size() - 1; q >= 1; q--) stage_inplace(c, o, q, false);
+    stage_last(c, o, comp);
+  }
+}
+
+// Implicit diffusion on the tensor-product plan (the reference's _IMPDIFF
+// build, main.f90:769-886): per velocity component its tensor operator and,
+// per block, the weights dx2 of the non-FFT axes and the BC planes of
+// init_bc_rhs (solver.f90:44-157) over the component's unknowns: factor *
+// weight / (dx1(i-(1-ib)+qqq) * dx2(i)), with factor dl*bc (Neumann, negated
+// on the upper face), -2 bc (centred Dirichlet) or -bc (staggered
+// Dirichlet), times inflow / bc where an inflow plane gives the boundary
+// values (main.f90:600-636).
+void plan_impdiff(snac_ctx *c) {
+  TensorPlan &t = c->tp;
+  const Case &cs = c->cs;
+  if (!t.ok)
+    throw std::runtime_error("implicit diffusion needs the tensor-product solver on this case (" + t.why + ")");
+  if (cs.nscal > 0) throw std::runtime_error("implicit diffusion of scalars is not supported");
+  const int fa = cs.fft_axis - 1;
+  for (int comp = 0; comp < 3; comp++) {
+    ImpField &f = c->imp[comp];
+    f = ImpField();
+    std::string why;
+    if (!build_ops(c, f.ops, comp, why))
+      throw std::runtime_error(std::string("implicit diffusion of ") + "uvw"[comp] + ": " + why);
+    // the tuned transform tiles of the pressure's plans where the kernels match
+    for (size_t q = 0; q < f.ops.fwd.size() && q < t.P.fwd.size(); q++) {
+      if (f.ops.fwd[q].kind == t.P.fwd[q].kind && f.ops.fwd[q].n == t.P.fwd[q].n) f.ops.fwd[q].lshift = t.P.fwd[q].lshift;
+      if (f.ops.bwd[q].kind == t.P.bwd[q].kind && f.ops.bwd[q].n == t.P.bwd[q].n) f.ops.bwd[q].lshift = t.P.bwd[q].lshift;
+    }
+    for (size_t b = 0; b < cs.blocks.size(); b++) {
+      const Block &B = cs.blocks[b];
+      ImpRhs r{};
+      int ext[3];
+      for (int a = 0; a < 3; a++) ext[a] = r.ext[a] = B.n[a] - ((a == comp && B.is_bound[a][1]) ? 1 : 0);
+      auto dx2 = [&](int a, int i) { return a == comp ? B.dc[a][i] : B.df[a][i]; };
+      auto sp_lo = [&](int a, int i) { return a == comp ? B.df[a][i] : B.dc[a][i - 1]; };
+      auto sp_hi = [&](int a, int i) { return a == comp ? B.df[a][i + 1] : B.dc[a][i]; };
+      for (int a = 0; a < 3; a++) {
+        std::vector<double> w(B.n[a], 1.0);
+        if (a != fa)
+          for (int i = 1; i <= B.n[a]; i++) w[i - 1] = dx2(a, i);
+        r.w[a] = dupload(c, w);
+      }
+      for (int d = 0; d < 3; d++)
+        for (int sd = 0; sd < 2; sd++) {
+          r.plane[d][sd] = nullptr;
+          if (!B.is_bound[d][sd] || cs.periods[d] != 0) continue;
+          const char ct = B.cbcvel[comp][d][sd];
+          const double bcv = B.bcvel[comp][d][sd];
+          const double dl = sd == 0 ? B.df[d][1] : B.df[d][B.n[d]];
+          double factor = 0.0;
+          if (ct == 'N') factor = sd == 0 ? dl * bcv : -(dl * bcv);
+          else if (ct == 'D') factor = d == comp ? -1.0 * bcv : -2.0 * bcv;
+          const int a1 = d == 0 ? 1 : 0, a2 = d == 2 ? 1 : 2;
+          const int q = sd == 0 ? 1 : ext[d];
+          const std::vector<double> &vin = B.vin[d][comp][sd];
+          const bool arr = B.is_inflow[d][sd] && !vin.empty() && bcv != 0.0;
+          std::vector<double> pl((size_t)ext[a1] * ext[a2]);
+          bool any = false;
+          for (int t2 = 1; t2 <= ext[a2]; t2++)
+            for (int t1 = 1; t1 <= ext[a1]; t1++) {
+              int ijk[3];
+              ijk[d] = q; ijk[a1] = t1; ijk[a2] = t2;
+              double weight = 1.0;
+              for (int a = 0; a < 3; a++) if (a != fa) weight *= dx2(a, ijk[a]);
+              const double spc = sd == 0 ? sp_lo(d, q) : sp_hi(d, q);
+              double rhs = factor * weight / (spc * dx2(d, q));
+              if (arr) rhs = rhs * vin[t1 + (size_t)(B.n[a1] + 2) * t2] / bcv;
+              pl[(t1 - 1) + (size_t)ext[a1] * (t2 - 1)] = rhs;
+              any = any || rhs != 0.0;
+            }
+          if (any) r.plane[d][sd] = dupload(c, pl);
+        }
+      f.rhs.push_back(r);
+    }
+  }
+}
+
+// solve_impdiff_field (solve_helmholtz.f90:30-136) of velocity component
+// comp, in place on the current velocity buffer: the right-hand side goes
+// to pp, the solve back into the component's unknowns
+void solve_impdiff(snac_ctx *c, int comp, double alpha) {
+  const double alphai = 1.0 / alpha;
+  ImpField &f = c->imp[comp];
+  {
+    Phase tp(c, SNAC_PH_HELMHOLTZ);
+    for (size_t b = 0; b < c->blk.size(); b++) {
+      DevBlock &D = c->blk[b];
+      launch_imp_rhs(c->stream, D.g, f.rhs[b], D.vel[c->cur][comp], D.pp, alphai);
+    }
+  }
+  tensor_solve(c, f.ops, comp, alphai);
 }
 
 // MG path: block pp interiors -> (transform along the FFT axis) -> plan
@@ -547,21 +730,7 @@ void solve_pressure(snac_ctx *c) {
     { Phase tb(c, SNAC_PH_R2R_BWD); mg_stage_out(c); }
     return;
   }
-  TensorPlan &t = c->tp;
-  {
-    Phase tf(c, SNAC_PH_R2R_FWD);
-    stage_first(c);
-    for (size_t q = 1; q < t.xaxes.size(); q++) stage_inplace(c, q, true);
-  }
-  {
-    Phase tt(c, SNAC_PH_TRIDIAG);
-    stage_tridiag(c);
-  }
-  {
-    Phase tb(c, SNAC_PH_R2R_BWD);
-    for (size_t q = t.xaxes.size() - 1; q >= 1; q--) stage_inplace(c, q, false);
-    stage_last(c);
-  }
+  tensor_solve(c, c->tp.P, -1, 0.0);
 }
 
 // Autotuner: times every candidate tile shape of each transform call site
@@ -602,17 +771,17 @@ void autotune(snac_ctx *c) {
     pl.lshift = best;
     t.tune_ms.push_back(bt);
   };
-  tune(t.fwd[0], [&] { stage_first(c); });
+  tune(t.P.fwd[0], [&] { stage_first(c, t.P); });
   for (size_t q = 1; q < t.xaxes.size(); q++) {
-    tune(t.fwd[q], [&] { stage_inplace(c, q, true); });
-    tune(t.bwd[q], [&] { stage_inplace(c, q, false); });
+    tune(t.P.fwd[q], [&] { stage_inplace(c, t.P, q, true); });
+    tune(t.P.bwd[q], [&] { stage_inplace(c, t.P, q, false); });
   }
-  tune(t.bwd[0], [&] { stage_last(c); });
+  tune(t.P.bwd[0], [&] { stage_last(c, t.P, -1); });
   int best = 1;
   double bt = 1e300;
   for (int v = (t.G[t.s] <= 128 ? 0 : 1); v <= (tridiag_chunks(t.G[t.s]) > 0 ? 2 : 1); v++) {
     t.tvariant = v;
-    double ms = time_stage(c, [&] { stage_tridiag(c); });
+    double ms = time_stage(c, [&] { stage_tridiag(c, t.P, 0.0); });
     if (ms < bt) { bt = ms; best = v; }
   }
   t.tvariant = best;
@@ -635,7 +804,7 @@ void rk_mom(snac_ctx *c, int irk, double dt) {
   for (auto &D : c->blk)
     launch_rk_mom(c->stream, D.g, D.m, D.vel[c->cur][0], D.vel[c->cur][1], D.vel[c->cur][2], D.p, D.vel[nx][0],
                   D.vel[nx][1], D.vel[nx][2], D.dto[0], D.dto[1], D.dto[2], f1, f2, f12, cs.visc, cs.bforce[0],
-                  cs.bforce[1], cs.bforce[2], buoy ? D.sv[c->scur][0] : nullptr, gb, c->rk_variant);
+                  cs.bforce[1], cs.bforce[2], buoy ? D.sv[c->scur][0] : nullptr, gb, c->rk_variant, c->impdiff);
   c->cur = nx;
 }
 
@@ -709,11 +878,19 @@ void substage(snac_ctx *c, int irk, double dt) {
     { Phase t(c, SNAC_PH_BOUND); run_program(c, c->prog_scal[c->scur]); }  // boundp of the scalars
   }
   { Phase t(c, SNAC_PH_RK_MOM); rk_mom(c, irk, dt); }
+  const double alpha = -c->cs.visc * dtrk / 2.0;  // main.f90:935
+  if (c->impdiff)  // main.f90:947-997
+    for (int comp = 0; comp < 3; comp++) solve_impdiff(c, comp, alpha);
   { Phase t(c, SNAC_PH_BOUND); run_program(c, c->prog_uvw[c->cur][0]); }  // bounduvw + inflow
   { Phase t(c, SNAC_PH_FILLPS); fillps(c, dtrk); }
   solve_pressure(c);
   { Phase t(c, SNAC_PH_BOUND); run_program(c, c->prog_pp); }               // boundp(pp)
-  { Phase t(c, SNAC_PH_CORREC); correc(c, dtrk, true); }                   // correc + updt_pressure
+  {
+    Phase t(c, SNAC_PH_CORREC);
+    correc(c, dtrk, !c->impdiff);  // correc + updt_pressure (p += pp)
+    if (c->impdiff)                // updt_pressure with the Laplacian term
+      for (auto &D : c->blk) launch_updt_pressure_imp(c->stream, D.g, D.m, D.pp, D.p, alpha);
+  }
   { Phase t(c, SNAC_PH_BOUND); run_program(c, c->prog_end[c->cur]); }  // bounduvw/inflow(correc) + boundp(p)
   if (c->pending.size() > 4096) collect_timing(c);  // bounds the event pool in long timed runs
 }
@@ -1387,6 +1564,36 @@ int snac_updt_pressure(snac_ctx *c) {
   });
 }
 
+int snac_updt_pressure_impdiff(snac_ctx *c, double alpha) {
+  return guard([&] {
+    for (auto &D : on_device(c)->blk) launch_updt_pressure_imp(c->stream, D.g, D.m, D.pp, D.p, alpha);
+  });
+}
+
+int snac_set_impdiff(snac_ctx *c, int enable) {
+  return guard([&] {
+    checked(c);  // a host-only context plans (and validates) without device arrays
+    if (c->stream) SNAC_HIP(hipStreamSynchronize(c->stream));
+    c->impdiff = false;
+    if (enable) {
+      plan_impdiff(c);
+      c->impdiff = true;
+    }
+    drop_graph(c);
+  });
+}
+
+int snac_solve_impdiff(snac_ctx *c, int fid, double alpha) {
+  return guard([&] {
+    on_device(c);
+    if (!c->impdiff) throw std::runtime_error("implicit diffusion is off (snac_set_impdiff)");
+    if (fid < SNAC_U || fid > SNAC_W) throw std::runtime_error("solve_impdiff: field must be u, v or w");
+    if (!(alpha < 0.0)) throw std::runtime_error("solve_impdiff: alpha must be negative (-visc*dtrk/2)");
+    solve_impdiff(c, fid - SNAC_U, alpha);
+    std::fill(c->tcnt, c->tcnt + SNAC_NPHASE, 0);
+  });
+}
+
 int snac_chkdiv(snac_ctx *c, double *divtot, double *divmax) {
   return guard([&] {
     double r[2];
@@ -1402,9 +1609,10 @@ int snac_chkdt(snac_ctx *c, double *dtmax) {
     reduce(on_device(c), false, r);
     double dti = r[1] == 0.0 ? 1.0 : r[1];
     double v = std::sqrt(3.0) / dti;
-    // main.f90:660 passes max(visc, alpha_max) as the diffusivity
+    // main.f90:660 passes max(visc, alpha_max) as the diffusivity; the
+    // diffusive limit does not apply with implicit diffusion (chkdt.f90:51)
     const double ve = c->cs.nscal > 0 ? std::max(c->cs.visc, c->cs.alpha_max) : c->cs.visc;
-    v = std::min(v, 1.65 / 12.0 / ve * c->dlmin * c->dlmin);
+    if (!c->impdiff) v = std::min(v, 1.65 / 12.0 / ve * c->dlmin * c->dlmin);
     if (dtmax) *dtmax = v;
   });
 }
@@ -1476,8 +1684,8 @@ int snac_get_autotune(snac_ctx *c, int *choices, double *ms, int n) {
     on_device(c);
     const TensorPlan &t = c->tp;
     std::vector<int> ch;
-    for (auto &p : t.fwd) ch.push_back(p.fft ? p.lshift : -1);
-    for (auto &p : t.bwd) ch.push_back(p.fft ? p.lshift : -1);
+    for (auto &p : t.P.fwd) ch.push_back(p.fft ? p.lshift : -1);
+    for (auto &p : t.P.bwd) ch.push_back(p.fft ? p.lshift : -1);
     std::vector<double> tm = t.tune_ms;
     if (t.ok) ch.push_back(t.tvariant);
     ch.push_back(c->rk_variant);

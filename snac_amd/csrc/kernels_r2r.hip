@@ -527,6 +527,8 @@ double r2r_coef(int kind, int n, int k, int j) {
   case K_RODFT01: return j == n - 1 ? ((k % 2) ? -1.0 : 1.0) : 2.0 * std::sin(pi * (j + 1) * (k + 0.5) / n);
   case K_REDFT11: return 2.0 * std::cos(pi * (j + 0.5) * (k + 0.5) / n);
   case K_RODFT11: return 2.0 * std::sin(pi * (j + 0.5) * (k + 0.5) / n);
+  // DST-I: the transform of a staggered Dirichlet-Dirichlet axis (fft.f90:130-133)
+  case K_RODFT00: return 2.0 * std::sin(pi * (j + 1) * (k + 1) / (n + 1));
   default: throw std::runtime_error("unsupported r2r kind " + std::to_string(kind));
   }
 }

@@ -30,7 +30,7 @@ __device__ __forceinline__ void tile_of(int gx, int gy, int &bx, int &by, int &b
 
 // BUOY: Boussinesq buoyancy of scalar 1, -f12*gacc*beta*(face average of s)
 // (rk.f90:104-108), gb = gacc*beta
-template <bool BUOY>
+template <bool BUOY, bool IMP>
 __global__ __launch_bounds__(TX *TY) void k_rk_mom(Geo g, Metrics m, const double *__restrict__ u,
                                                     const double *__restrict__ v, const double *__restrict__ w,
                                                     const double *__restrict__ p, double *__restrict__ un,
@@ -71,6 +71,10 @@ __global__ __launch_bounds__(TX *TY) void k_rk_mom(Geo g, Metrics m, const doubl
     double dwdzp = (w[zp] - wc) * izfkp, dwdzm = (wc - w[zm]) * izfk;
     cc = visc * (dwdxp - dwdxm) * ixfi + visc * (dwdyp - dwdym) * iyfj + visc * (dwdzp - dwdzm) * izck;
   }
+  // implicit diffusion: the diffusion is kept apart, dudtrk is the advection
+  // alone (rk.f90:50-59)
+  double ad = 0.0, bd = 0.0, cd = 0.0;
+  if (IMP) { ad = a; bd = b; cd = cc; a = 0.0; b = 0.0; cc = 0.0; }
   // volume-weighted advection, momx/y/z_a_vv (mom.f90:112-219)
   {
     double uuip = 0.25 * (uc * dyfj * dzfk + u[xp] * dyfj * dzfk) * u[xp];
@@ -100,6 +104,7 @@ __global__ __launch_bounds__(TX *TY) void k_rk_mom(Geo g, Metrics m, const doubl
   double un_ = uc + f1 * a + f2 * dtu[q];
   double vn_ = vc + f1 * b + f2 * dtv[q];
   double wn_ = wc + f1 * cc + f2 * dtw[q];
+  if (IMP) { un_ = un_ + f12 * ad; vn_ = vn_ + f12 * bd; wn_ = wn_ + f12 * cd; }  // rk.f90:75-79
   dtu[q] = a;
   dtv[q] = b;
   dtw[q] = cc;
@@ -112,6 +117,11 @@ __global__ __launch_bounds__(TX *TY) void k_rk_mom(Geo g, Metrics m, const doubl
     un_ = un_ - f12 * gbx * 0.5 * (s1[xp] + sc);
     vn_ = vn_ - f12 * gby * 0.5 * (s1[yp] + sc);
     wn_ = wn_ - f12 * gbz * 0.5 * (s1[zp] + sc);
+  }
+  if (IMP) {  // the Helmholtz right-hand side (rk.f90:112-127)
+    un_ = un_ - 0.5 * f12 * ad;
+    vn_ = vn_ - 0.5 * f12 * bd;
+    wn_ = wn_ - 0.5 * f12 * cd;
   }
   un[c] = un_;
   vn[c] = vn_;
@@ -129,7 +139,7 @@ __global__ __launch_bounds__(TX *TY) void k_rk_mom(Geo g, Metrics m, const doubl
 // BX x BY threads per workgroup (256).  (Capping the registers for more
 // waves per SIMD measured slower: the per-wave load parallelism matters
 // more than the wave count here.)
-template <bool BUOY, int KB, int BX, int BY>
+template <bool BUOY, bool IMP, int KB, int BX, int BY>
 __global__ __launch_bounds__(BX *BY) void k_rk_mom_kz(Geo g, Metrics m, const double *__restrict__ u,
                                                        const double *__restrict__ v, const double *__restrict__ w,
                                                        const double *__restrict__ p, double *__restrict__ un,
@@ -178,6 +188,10 @@ __global__ __launch_bounds__(BX *BY) void k_rk_mom_kz(Geo g, Metrics m, const do
       double dwdzp = (wp - wc) * izfkp, dwdzm = (wc - wm) * izfk;
       cc = visc * (dwdxp - dwdxm) * ixfi + visc * (dwdyp - dwdym) * iyfj + visc * (dwdzp - dwdzm) * izck;
     }
+    // implicit diffusion: the diffusion is kept apart, dudtrk is the advection
+    // alone (rk.f90:50-59)
+    double ad = 0.0, bd = 0.0, cd = 0.0;
+    if (IMP) { ad = a; bd = b; cd = cc; a = 0.0; b = 0.0; cc = 0.0; }
     {
       double uuip = 0.25 * (uc * dyfj * dzfk + uxp * dyfj * dzfk) * uxp;
       double uuim = 0.25 * (uc * dyfj * dzfk + uxm * dyfj * dzfk) * uxm;
@@ -204,6 +218,7 @@ __global__ __launch_bounds__(BX *BY) void k_rk_mom_kz(Geo g, Metrics m, const do
     double un_ = uc + f1 * a + f2 * dtu[q];
     double vn_ = vc + f1 * b + f2 * dtv[q];
     double wn_ = wc + f1 * cc + f2 * dtw[q];
+    if (IMP) { un_ = un_ + f12 * ad; vn_ = vn_ + f12 * bd; wn_ = wn_ + f12 * cd; }  // rk.f90:75-79
     un_ = un_ + f12 * (-(p[xp] - pc) * ixci + bfx);
     vn_ = vn_ + f12 * (-(p[yp] - pc) * iycj + bfy);
     wn_ = wn_ + f12 * (-(pz - pc) * izck + bfz);
@@ -216,6 +231,11 @@ __global__ __launch_bounds__(BX *BY) void k_rk_mom_kz(Geo g, Metrics m, const do
     dtu[q] = a;
     dtv[q] = b;
     dtw[q] = cc;
+    if (IMP) {  // the Helmholtz right-hand side (rk.f90:112-127)
+      un_ = un_ - 0.5 * f12 * ad;
+      vn_ = vn_ - 0.5 * f12 * bd;
+      wn_ = wn_ - 0.5 * f12 * cd;
+    }
     un[c] = un_;
     vn[c] = vn_;
     wn[c] = wn_;
@@ -235,7 +255,7 @@ __global__ __launch_bounds__(BX *BY) void k_rk_mom_kz(Geo g, Metrics m, const do
 // computed and land in the ring slot of plane k-1 after it, so their
 // latency overlaps the arithmetic.
 constexpr int LX = 66;  // row pitch of an LDS plane: 64 + 2 halo cells
-template <bool BUOY, int KB, int BY>
+template <bool BUOY, bool IMP, int KB, int BY>
 __global__ __launch_bounds__(64 * BY) void k_rk_mom_lds(Geo g, Metrics m, const double *__restrict__ u,
                                                         const double *__restrict__ v, const double *__restrict__ w,
                                                         const double *__restrict__ p, double *__restrict__ un,
@@ -339,6 +359,10 @@ __global__ __launch_bounds__(64 * BY) void k_rk_mom_lds(Geo g, Metrics m, const 
         double dwdzp = (wp - wc) * izfkp, dwdzm = (wc - wm) * izfk;
         cc = visc * (dwdxp - dwdxm) * ixfi + visc * (dwdyp - dwdym) * iyfj + visc * (dwdzp - dwdzm) * izck;
       }
+      // implicit diffusion: the diffusion is kept apart, dudtrk is the advection
+      // alone (rk.f90:50-59)
+      double ad = 0.0, bd = 0.0, cd = 0.0;
+      if (IMP) { ad = a; bd = b; cd = cc; a = 0.0; b = 0.0; cc = 0.0; }
       {
         double uuip = 0.25 * (uc * dyfj * dzfk + uxp * dyfj * dzfk) * uxp;
         double uuim = 0.25 * (uc * dyfj * dzfk + uxm * dyfj * dzfk) * uxm;
@@ -365,6 +389,7 @@ __global__ __launch_bounds__(64 * BY) void k_rk_mom_lds(Geo g, Metrics m, const 
       double un_ = uc + f1 * a + f2 * dtu[q];
       double vn_ = vc + f1 * b + f2 * dtv[q];
       double wn_ = wc + f1 * cc + f2 * dtw[q];
+      if (IMP) { un_ = un_ + f12 * ad; vn_ = vn_ + f12 * bd; wn_ = wn_ + f12 * cd; }  // rk.f90:75-79
       un_ = un_ + f12 * (-(P0[LXP] - pc) * ixci + bfx);
       vn_ = vn_ + f12 * (-(P0[LYP] - pc) * iycj + bfy);
       wn_ = wn_ + f12 * (-(pz - pc) * izck + bfz);
@@ -377,6 +402,11 @@ __global__ __launch_bounds__(64 * BY) void k_rk_mom_lds(Geo g, Metrics m, const 
       dtu[q] = a;
       dtv[q] = b;
       dtw[q] = cc;
+      if (IMP) {  // the Helmholtz right-hand side (rk.f90:112-127)
+        un_ = un_ - 0.5 * f12 * ad;
+        vn_ = vn_ - 0.5 * f12 * bd;
+        wn_ = wn_ - 0.5 * f12 * cd;
+      }
       un[c] = un_;
       vn[c] = vn_;
       wn[c] = wn_;
@@ -620,14 +650,14 @@ dim3 ghosted_tiles(const Geo &g) { return dim3(((g.nx + 2 + TX - 1) / TX) * ((g.
 void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v, const double *w,
                    const double *p, double *un, double *vn, double *wn, double *dtu, double *dtv, double *dtw,
                    double f1, double f2, double f12, double visc, double bfx, double bfy, double bfz,
-                   const double *s1, const double gb[3], int variant) {
+                   const double *s1, const double gb[3], int variant, bool impdiff) {
   if (variant >= 7) {
     // 7..10: LDS-staged k-march; 7/8: 64x4 tiles, KB 16/32; 9/10: 64x8 tiles, KB 16/32
     // (capping the registers at 128 for a fourth wave per SIMD spills and measured slower)
     const int vi = variant - 7, kb = vi % 2 ? 32 : 16, by = vi >= 2 ? 8 : 4;
     dim3 grid((unsigned)(((g.nx + 63) / 64) * ((g.ny + by - 1) / by) * ((g.nz + kb - 1) / kb)));
 #define SNAC_KL(B, KB, Y)                                                                                           \
-  hipLaunchKernelGGL((k_rk_mom_lds<B, KB, Y>), grid, dim3(64, Y), 0, s, g, m, u, v, w, p, un, vn, wn, dtu, dtv, dtw,  \
+  hipLaunchKernelGGL((k_rk_mom_lds<B, IM, KB, Y>), grid, dim3(64, Y), 0, s, g, m, u, v, w, p, un, vn, wn, dtu, dtv, dtw,  \
                      f1, f2, f12, visc, bfx, bfy, bfz, s1, gb[0], gb[1], gb[2])
 #define SNAC_KL_ALL(B)                                                                                              \
   do {                                                                                                              \
@@ -636,8 +666,15 @@ void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *
     else if (vi == 2) SNAC_KL(B, 16, 8);                                                                            \
     else SNAC_KL(B, 32, 8);                                                                                         \
   } while (0)
-    if (s1) SNAC_KL_ALL(true);
-    else SNAC_KL_ALL(false);
+    if (impdiff) {
+      constexpr bool IM = true;
+      if (s1) SNAC_KL_ALL(true);
+      else SNAC_KL_ALL(false);
+    } else {
+      constexpr bool IM = false;
+      if (s1) SNAC_KL_ALL(true);
+      else SNAC_KL_ALL(false);
+    }
 #undef SNAC_KL_ALL
 #undef SNAC_KL
     SNAC_HIP(hipGetLastError());
@@ -649,7 +686,7 @@ void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *
     const int bxs = shape == 1 ? 32 : 64, bys = 256 / bxs;
     dim3 grid((unsigned)(((g.nx + bxs - 1) / bxs) * ((g.ny + bys - 1) / bys) * ((g.nz + kb - 1) / kb)));
 #define SNAC_KZ(B, KB, X, Y)                                                                                        \
-  hipLaunchKernelGGL((k_rk_mom_kz<B, KB, X, Y>), grid, dim3(X, Y), 0, s, g, m, u, v, w, p, un, vn, wn, dtu, dtv, dtw,  \
+  hipLaunchKernelGGL((k_rk_mom_kz<B, IM, KB, X, Y>), grid, dim3(X, Y), 0, s, g, m, u, v, w, p, un, vn, wn, dtu, dtv, dtw,  \
                      f1, f2, f12, visc, bfx, bfy, bfz, s1, gb[0], gb[1], gb[2])
 #define SNAC_KZ_KB(B, X, Y)                                                                                         \
   do {                                                                                                              \
@@ -662,20 +699,32 @@ void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *
     if (shape == 0) SNAC_KZ_KB(B, 64, 4);                                                                           \
     else SNAC_KZ_KB(B, 32, 8);                                                                                      \
   } while (0)
-    if (s1) SNAC_KZ_SHAPE(true);
-    else SNAC_KZ_SHAPE(false);
+    if (impdiff) {
+      constexpr bool IM = true;
+      if (s1) SNAC_KZ_SHAPE(true);
+      else SNAC_KZ_SHAPE(false);
+    } else {
+      constexpr bool IM = false;
+      if (s1) SNAC_KZ_SHAPE(true);
+      else SNAC_KZ_SHAPE(false);
+    }
 #undef SNAC_KZ_SHAPE
 #undef SNAC_KZ_KB
 #undef SNAC_KZ
     SNAC_HIP(hipGetLastError());
     return;
   }
-  if (s1)
-    hipLaunchKernelGGL(k_rk_mom<true>, interior_tiles(g), dim3(TX, TY), 0, s, g, m, u, v, w, p, un, vn, wn, dtu, dtv,
-                       dtw, f1, f2, f12, visc, bfx, bfy, bfz, s1, gb[0], gb[1], gb[2]);
-  else
-    hipLaunchKernelGGL(k_rk_mom<false>, interior_tiles(g), dim3(TX, TY), 0, s, g, m, u, v, w, p, un, vn, wn, dtu, dtv,
-                       dtw, f1, f2, f12, visc, bfx, bfy, bfz, nullptr, 0.0, 0.0, 0.0);
+#define SNAC_K0(B, I)                                                                                               \
+  hipLaunchKernelGGL((k_rk_mom<B, I>), interior_tiles(g), dim3(TX, TY), 0, s, g, m, u, v, w, p, un, vn, wn, dtu, dtv,  \
+                     dtw, f1, f2, f12, visc, bfx, bfy, bfz, s1, gb[0], gb[1], gb[2])
+  if (s1) {
+    if (impdiff) SNAC_K0(true, true);
+    else SNAC_K0(true, false);
+  } else {
+    if (impdiff) SNAC_K0(false, true);
+    else SNAC_K0(false, false);
+  }
+#undef SNAC_K0
   SNAC_HIP(hipGetLastError());
 }
 

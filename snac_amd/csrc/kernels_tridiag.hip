@@ -33,7 +33,7 @@ __global__ __launch_bounds__(WG) void k_tridiag(TridiagLine t, double *__restric
   const long line = (long)blockIdx.x * WG + threadIdx.x;
   if (line >= (long)n1 * n2) return;
   const int t1 = (int)(line % n1), t2 = (int)(line / n1);
-  const double lam = t.lamA[t1] + t.lamB[t2];
+  const double lam = t.lamA[t1] + t.lamB[t2] + t.shift;
   const int mm = (t.pin_when_zero && lam == 0.0) ? m - 1 : m;
   const long base = t1 * st[o1] + t2 * st[o2];
   const long ss = st[s];
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(64 * NC) void k_tridiag_chunk(TridiagLine t, double
   const long line = (long)blockIdx.x * 64 + lane;
   const bool live = line < (long)n1 * n2;
   const int t1 = live ? (int)(line % n1) : 0, t2 = live ? (int)(line / n1) : 0;
-  const double lam = t.lamA[t1] + t.lamB[t2];
+  const double lam = t.lamA[t1] + t.lamB[t2] + t.shift;
   const bool pin = t.pin_when_zero && lam == 0.0;
   const long ss = st[s];
   double *xl = x + t1 * st[o1] + t2 * st[o2];

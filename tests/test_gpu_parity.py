@@ -62,7 +62,8 @@ def channel(tmp_path, n=(32, 24, 16)):
 
 
 # ---------------------------------------------------------------- transforms
-@pytest.mark.parametrize("kind", ["R2HC", "HC2R", "REDFT10", "REDFT01", "RODFT10", "RODFT01", "REDFT11", "RODFT11"])
+@pytest.mark.parametrize("kind", ["R2HC", "HC2R", "REDFT10", "REDFT01", "RODFT10", "RODFT01", "REDFT11", "RODFT11",
+                                  "RODFT00"])
 @pytest.mark.parametrize("n", [2, 4, 6, 16, 64, 256])
 def test_r2r_lines(tmp_path, kind, n):
     """Every kind and tile choice, with contiguous lines and with lines

@@ -28,7 +28,7 @@ import oracle_py
 
 
 KINDS = {"REDFT10": ("dct", 2), "REDFT01": ("dct", 3), "RODFT10": ("dst", 2), "RODFT01": ("dst", 3),
-         "REDFT11": ("dct", 4), "RODFT11": ("dst", 4)}
+         "REDFT11": ("dct", 4), "RODFT11": ("dst", 4), "RODFT00": ("dst", 1)}
 
 
 @pytest.mark.parametrize("n", [1, 2, 3, 4, 7, 8, 16, 64])
