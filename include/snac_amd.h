@@ -179,12 +179,15 @@ int snac_chkdt(snac_ctx *ctx, double *dtmax);
  *      problem (I - visc*dtrk/2 L) u = u* of each velocity component after
  *      rk_mom (main.f90:947-997) and adds alpha*L(pp) in updt_pressure
  *      (updt_pressure.f90:30-45), and snac_chkdt drops the diffusive limit.
- *      Needs the tensor-product solver (a box-shaped case with uniform
- *      transformed axes) and no scalars; a staggered pair along a
+ *      Scalars alike: rk_scal keeps half its diffusion (rk.f90:151-202)
+ *      and each scalar's Helmholtz problem is solved after its bulk forcing
+ *      (main.f90:914-933).  Needs the tensor-product solver (a box-shaped
+ *      case with uniform transformed axes); a staggered pair along a
  *      transformed axis must be periodic or Dirichlet-Dirichlet.  ---- */
 int snac_set_impdiff(snac_ctx *ctx, int enable);
-/* solve_impdiff_field solve_helmholtz.f90:30-136 of fid = SNAC_U, SNAC_V or
- * SNAC_W with alpha = -visc*dtrk/2 (< 0): phi * alphai * dx2 + BC terms
+/* solve_impdiff_field solve_helmholtz.f90:30-136 of fid = SNAC_U, SNAC_V,
+ * SNAC_W or a scalar (SNAC_SCAL0 + 2*i) with alpha = -visc*dtrk/2 (a
+ * scalar's -alpha_i*dtrk/2; < 0): phi * alphai * dx2 + BC terms
  * (init_bc_rhs solver.f90:44-157, updt_rhs bound.f90:543-585), solved in
  * place over the component's unknowns (the bounded upper face along its own
  * axis excluded)                                                            */

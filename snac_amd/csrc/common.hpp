@@ -81,15 +81,20 @@ void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *
                    const double *w, const double *p, double *un, double *vn, double *wn, double *dtu,
                    double *dtv, double *dtw, double f1, double f2, double f12, double visc, double bfx,
                    double bfy, double bfz, const double *s1, const double gb[3], int variant, bool impdiff);
-// rk_scal of one scalar: reads sc (ghosted), writes the interior of sn and dso
+// rk_scal of one scalar: reads sc (ghosted), writes the interior of sn and dso.
+// imp (implicit diffusion, rk.f90:151-202): 0 off; 1 the diffusion out of
+// dsdtrk, half of it left in sn; 2 likewise but the half is taken off by
+// launch_bulk_force after the bulk mean, the diffusion written to ddo
+// (interior layout)
 void launch_rk_scal(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v, const double *w,
                     const double *sc, double *sn, double *dso, double alpha, double f1, double f2, double f12,
-                    double src);
+                    double src, int imp = 0, double *ddo = nullptr);
 // partial volume integrals of f (reduce_nparts(g) entries), and the bulk
-// forcing f += scalf - red[0]*ivol (fout, if given, receives that value)
+// forcing f += scalf - red[0]*ivol (fout, if given, receives that value);
+// dd non-null first subtracts h*dd (interior layout)
 void launch_volsum_part(hipStream_t s, const Geo &g, const Metrics &m, const double *f, double *psum);
 void launch_bulk_force(hipStream_t s, const Geo &g, double *f, const double *red, double scalf, double ivol,
-                       double *fout);
+                       double *fout, const double *dd = nullptr, double h = 0.0);
 // fillps with updt_rhs fused; rhs planes may be null (ordered x0,x1,y0,y1,z0,z1)
 void launch_fillps(hipStream_t s, const Geo &g, const Metrics &m, int fft_axis, const double *u,
                    const double *v, const double *w, double *pp, double dtrk, const double *const rhsp[6]);

@@ -33,6 +33,7 @@ struct DevBlock {
   double *vin[3][3][2] = {};
   std::vector<double *> sv[2];  // scalar ping-pong buffers [parity][iscal]
   std::vector<double *> sdo;    // dsdtrko per scalar (interior)
+  double *sdd = nullptr;        // a forced scalar's diffusion (implicit diffusion; interior)
   long ntot = 0, nin = 0;
   int part_off = 0, nparts = 0;
 };
@@ -96,7 +97,7 @@ struct snac_ctx {
   double *sred = nullptr, *sforce = nullptr;  // scalar bulk mean / forcing
   snac::TensorPlan tp;
   bool impdiff = false;    // implicit diffusion (the reference's _IMPDIFF build)
-  snac::ImpField imp[3];   // its Helmholtz operators of u, v, w
+  std::vector<snac::ImpField> imp;  // its Helmholtz operators of u, v, w and the scalars
   snac::MGPlan mg;
   bool mg_planned = false;
   bool mg_fixed = false;  // solves run a fixed iteration count (graph capture)
@@ -240,15 +241,19 @@ void collect_timing(snac_ctx *c) {
   c->pending.clear();
 }
 
-// BC character of field comp (-1 pressure, 0..2 velocity component) on the
-// global face (axis a, side) of the plan's box: 'F' where periodic
+// BC character of field comp (-1 pressure, 0..2 velocity component, 3+i
+// scalar i) on the global face (axis a, side) of the plan's box: 'F' where
+// periodic
 bool box_bc(snac_ctx *c, int comp, int a, int side, char &out, std::string &why) {
   const TensorPlan &t = c->tp;
   out = 0;
   for (auto &B : c->cs.blocks) {
     const bool on = side == 0 ? B.lo_g[a] == t.gmin[a] : B.lo_g[a] + B.n[a] == t.gmin[a] + t.G[a];
     if (!on) continue;
-    const char bc = !B.is_bound[a][side] ? 'F' : (comp < 0 ? B.cbcpre[a][side] : B.cbcvel[comp][a][side]);
+    const char bc = !B.is_bound[a][side] ? 'F'
+                    : comp < 0 ? B.cbcpre[a][side]
+                    : comp < 3 ? B.cbcvel[comp][a][side]
+                               : B.cbcsca[((comp - 3) * 3 + a) * 2 + side];
     if (out && out != bc) { why = "mixed boundary conditions on one face"; return false; }
     out = bc;
   }
@@ -256,8 +261,9 @@ bool box_bc(snac_ctx *c, int comp, int a, int side, char &out, std::string &why)
 }
 
 // Transforms, eigenvalues and line coefficients of one operator on the
-// plan's axes: the pressure (comp -1, centred) or the Helmholtz operator of
-// velocity component comp (implicit diffusion): staggered along its own axis
+// plan's axes: the pressure (comp -1, centred), the Helmholtz operator of a
+// scalar (comp 3+i, centred) or of velocity component comp (implicit
+// diffusion): staggered along its own axis
 // (is_centered = .false., main.f90:774-776), i.e. unknowns on the faces (the
 // bounded upper face excluded: an identity row of the line solve, or a
 // transform over n-1 points), couplings 1/dxf, weights dxc, Dirichlet values
@@ -529,8 +535,14 @@ void stage_tridiag(snac_ctx *c, const TensorOps &o, double shift) {
   launch_tridiag(c->stream, tl, t.ph, t.scratch, t.scratch2);
 }
 
+// the current buffer of field comp of a Helmholtz solve (velocity
+// component 0..2, scalar 3+i)
+double *imp_field(snac_ctx *c, DevBlock &D, int comp) {
+  return comp < 3 ? D.vel[c->cur][comp] : D.sv[c->scur][comp - 3];
+}
+
 // last inverse transform: global work array -> block interiors (pp for the
-// pressure, velocity component comp for a Helmholtz solve), scaled
+// pressure, field comp for a Helmholtz solve), scaled
 void stage_last(snac_ctx *c, const TensorOps &o, int comp) {
   TensorPlan &t = c->tp;
   const long pst[3] = {1, (long)t.G[0], (long)t.G[0] * t.G[1]};
@@ -539,7 +551,7 @@ void stage_last(snac_ctx *c, const TensorOps &o, int comp) {
   for (size_t b = 0; b < c->blk.size(); b++) {
     DevBlock &D = c->blk[b];
     const Block &B = c->cs.blocks[b];
-    double *out = (comp < 0 ? D.pp : D.vel[c->cur][comp]) + 1 + D.g.sx + D.g.sx * D.g.sy;
+    double *out = (comp < 0 ? D.pp : imp_field(c, D, comp)) + 1 + D.g.sx + D.g.sx * D.g.sy;
     long off = 0;
     for (int q = 0; q < 3; q++) off += (long)(B.lo_g[q] - t.gmin[q]) * pst[q];
     launch_r2r(c->stream, o.bwd[0], t.ph + off, pst[a], pst[b1], pst[b2], out, gstride(D, a), gstride(D, b1),
@@ -580,14 +592,15 @@ void plan_impdiff(snac_ctx *c) {
   const Case &cs = c->cs;
   if (!t.ok)
     throw std::runtime_error("implicit diffusion needs the tensor-product solver on this case (" + t.why + ")");
-  if (cs.nscal > 0) throw std::runtime_error("implicit diffusion of scalars is not supported");
   const int fa = cs.fft_axis - 1;
-  for (int comp = 0; comp < 3; comp++) {
+  c->imp.assign(3 + cs.nscal, ImpField());
+  for (int comp = 0; comp < 3 + cs.nscal; comp++) {
     ImpField &f = c->imp[comp];
-    f = ImpField();
     std::string why;
     if (!build_ops(c, f.ops, comp, why))
-      throw std::runtime_error(std::string("implicit diffusion of ") + "uvw"[comp] + ": " + why);
+      throw std::runtime_error("implicit diffusion of " +
+                               (comp < 3 ? std::string(1, "uvw"[comp]) : "scalar " + std::to_string(comp - 2)) +
+                               ": " + why);
     // the tuned transform tiles of the pressure's plans where the kernels match
     for (size_t q = 0; q < f.ops.fwd.size() && q < t.P.fwd.size(); q++) {
       if (f.ops.fwd[q].kind == t.P.fwd[q].kind && f.ops.fwd[q].n == t.P.fwd[q].n) f.ops.fwd[q].lshift = t.P.fwd[q].lshift;
@@ -611,16 +624,19 @@ void plan_impdiff(snac_ctx *c) {
         for (int sd = 0; sd < 2; sd++) {
           r.plane[d][sd] = nullptr;
           if (!B.is_bound[d][sd] || cs.periods[d] != 0) continue;
-          const char ct = B.cbcvel[comp][d][sd];
-          const double bcv = B.bcvel[comp][d][sd];
+          const bool vel = comp < 3;
+          const size_t qs = vel ? 0 : ((size_t)(comp - 3) * 3 + d) * 2 + sd;
+          const char ct = vel ? B.cbcvel[comp][d][sd] : B.cbcsca[qs];
+          const double bcv = vel ? B.bcvel[comp][d][sd] : B.bcsca[qs];
           const double dl = sd == 0 ? B.df[d][1] : B.df[d][B.n[d]];
           double factor = 0.0;
           if (ct == 'N') factor = sd == 0 ? dl * bcv : -(dl * bcv);
           else if (ct == 'D') factor = d == comp ? -1.0 * bcv : -2.0 * bcv;
           const int a1 = d == 0 ? 1 : 0, a2 = d == 2 ? 1 : 2;
           const int q = sd == 0 ? 1 : ext[d];
-          const std::vector<double> &vin = B.vin[d][comp][sd];
-          const bool arr = B.is_inflow[d][sd] && !vin.empty() && bcv != 0.0;
+          static const std::vector<double> none;
+          const std::vector<double> &vin = vel ? B.vin[d][comp][sd] : none;
+          const bool arr = vel && B.is_inflow[d][sd] && !vin.empty() && bcv != 0.0;
           std::vector<double> pl((size_t)ext[a1] * ext[a2]);
           bool any = false;
           for (int t2 = 1; t2 <= ext[a2]; t2++)
@@ -640,11 +656,17 @@ void plan_impdiff(snac_ctx *c) {
       f.rhs.push_back(r);
     }
   }
+  // a forced scalar keeps its diffusion until its bulk mean is taken
+  bool forced = false;
+  for (int is = 0; is < cs.nscal; is++) forced = forced || cs.is_sforced[is];
+  if (forced)
+    for (auto &D : c->blk)
+      if (!D.sdd) D.sdd = dalloc<double>(c, D.nin);
 }
 
 // solve_impdiff_field (solve_helmholtz.f90:30-136) of velocity component
-// comp, in place on the current velocity buffer: the right-hand side goes
-// to pp, the solve back into the component's unknowns
+// comp (0..2) or scalar comp-3, in place on its current buffer: the
+// right-hand side goes to pp, the solve back into the field's unknowns
 void solve_impdiff(snac_ctx *c, int comp, double alpha) {
   const double alphai = 1.0 / alpha;
   ImpField &f = c->imp[comp];
@@ -652,7 +674,7 @@ void solve_impdiff(snac_ctx *c, int comp, double alpha) {
     Phase tp(c, SNAC_PH_HELMHOLTZ);
     for (size_t b = 0; b < c->blk.size(); b++) {
       DevBlock &D = c->blk[b];
-      launch_imp_rhs(c->stream, D.g, f.rhs[b], D.vel[c->cur][comp], D.pp, alphai);
+      launch_imp_rhs(c->stream, D.g, f.rhs[b], imp_field(c, D, comp), D.pp, alphai);
     }
   }
   tensor_solve(c, f.ops, comp, alphai);
@@ -844,15 +866,18 @@ void rk_scal(snac_ctx *c, int irk, double dt) {
   const double f1 = RKC[irk - 1][0] * dt, f2 = RKC[irk - 1][1] * dt, f12 = f1 + f2;
   const int nx = 1 - c->scur;
   for (int is = 0; is < cs.nscal; is++) {
+    // implicit diffusion: an unforced scalar takes its half diffusion off in
+    // the kernel, a forced one after its bulk mean (rk.f90:183-202)
+    const int imp = c->impdiff ? (cs.is_sforced[is] ? 2 : 1) : 0;
     for (auto &D : c->blk)
       launch_rk_scal(c->stream, D.g, D.m, D.vel[c->cur][0], D.vel[c->cur][1], D.vel[c->cur][2], D.sv[c->scur][is],
-                     D.sv[nx][is], D.sdo[is], cs.alpha[is], f1, f2, f12, cs.ssource[is]);
+                     D.sv[nx][is], D.sdo[is], cs.alpha[is], f1, f2, f12, cs.ssource[is], imp, D.sdd);
     if (!cs.is_sforced[is]) continue;
     for (auto &D : c->blk) launch_volsum_part(c->stream, D.g, D.m, D.sv[nx][is], c->part_sum + D.part_off);
     launch_reduce_final(c->stream, c->part_sum, nullptr, c->nparts, c->sred);
     for (size_t b = 0; b < c->blk.size(); b++)
       launch_bulk_force(c->stream, c->blk[b].g, c->blk[b].sv[nx][is], c->sred, cs.scalf[is], 1.0 / cs.vol_all,
-                        b == 0 ? c->sforce + is : nullptr);
+                        b == 0 ? c->sforce + is : nullptr, imp == 2 ? c->blk[b].sdd : nullptr, 0.5 * f12);
   }
   c->scur = nx;
 }
@@ -875,6 +900,8 @@ void substage(snac_ctx *c, int irk, double dt) {
   double dtrk = (RKC[irk - 1][0] + RKC[irk - 1][1]) * dt;
   if (c->cs.nscal > 0) {
     { Phase t(c, SNAC_PH_RK_SCAL); rk_scal(c, irk, dt); }
+    if (c->impdiff)  // main.f90:914-933
+      for (int is = 0; is < c->cs.nscal; is++) solve_impdiff(c, 3 + is, -c->cs.alpha[is] * dtrk / 2.0);
     { Phase t(c, SNAC_PH_BOUND); run_program(c, c->prog_scal[c->scur]); }  // boundp of the scalars
   }
   { Phase t(c, SNAC_PH_RK_MOM); rk_mom(c, irk, dt); }
@@ -1587,9 +1614,13 @@ int snac_solve_impdiff(snac_ctx *c, int fid, double alpha) {
   return guard([&] {
     on_device(c);
     if (!c->impdiff) throw std::runtime_error("implicit diffusion is off (snac_set_impdiff)");
-    if (fid < SNAC_U || fid > SNAC_W) throw std::runtime_error("solve_impdiff: field must be u, v or w");
+    int comp = -1;
+    if (fid >= SNAC_U && fid <= SNAC_W) comp = fid - SNAC_U;
+    else if (fid >= SNAC_SCAL0 && fid < SNAC_SCAL0 + 2 * c->cs.nscal && (fid - SNAC_SCAL0) % 2 == 0)
+      comp = 3 + (fid - SNAC_SCAL0) / 2;
+    if (comp < 0) throw std::runtime_error("solve_impdiff: field must be u, v, w or a scalar");
     if (!(alpha < 0.0)) throw std::runtime_error("solve_impdiff: alpha must be negative (-visc*dtrk/2)");
-    solve_impdiff(c, fid - SNAC_U, alpha);
+    solve_impdiff(c, comp, alpha);
     std::fill(c->tcnt, c->tcnt + SNAC_NPHASE, 0);
   });
 }

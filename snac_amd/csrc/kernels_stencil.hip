@@ -420,11 +420,16 @@ __global__ __launch_bounds__(64 * BY) void k_rk_mom_lds(Geo g, Metrics m, const 
 // rk_scal (rk.f90:130-204) with scal_d / scal_a (scal.f90:89-155) fused:
 // reads s (ghosted) and u, v, w; writes the updated interior into sn (the
 // other buffer of the pair) and dsdtrk into dso.
+// IMP (implicit diffusion, rk.f90:151-202): 0 off; 1 the diffusion kept out
+// of dsdtrk, half of it left in the update; 2 the same with the half taken
+// off later (a forced scalar's bulk mean is taken before it): the diffusion
+// goes to ddo
+template <int IMP>
 __global__ __launch_bounds__(TX *TY) void k_rk_scal(Geo g, Metrics m, const double *__restrict__ u,
                                                      const double *__restrict__ v, const double *__restrict__ w,
                                                      const double *__restrict__ s, double *__restrict__ sn,
                                                      double *__restrict__ dso, double alpha, double f1, double f2,
-                                                     double f12, double src) {
+                                                     double f12, double src, double *__restrict__ ddo) {
   int bx, by, bz;
   tile_of((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, bx, by, bz);
   const int i = bx * TX + threadIdx.x + 1;
@@ -442,9 +447,15 @@ __global__ __launch_bounds__(TX *TY) void k_rk_scal(Geo g, Metrics m, const doub
   const double usim = 0.5 * (sxm + sc) * u[xm], usip = 0.5 * (sxp + sc) * u[c];
   const double vsjm = 0.5 * (sym + sc) * v[ym], vsjp = 0.5 * (syp + sc) * v[c];
   const double wskm = 0.5 * (szm + sc) * w[zm], wskp = 0.5 * (szp + sc) * w[c];
+  double dd = 0.0;
+  if (IMP) { dd = d; d = 0.0; }
   d = d + (-usip + usim) * ixfi + (-vsjp + vsjm) * iyfj + (-wskp + wskm) * izfk;
   const long q = (long)(i - 1) + (long)g.nx * ((long)(j - 1) + (long)g.ny * (long)(k - 1));
-  sn[c] = sc + f1 * d + f2 * dso[q] + f12 * src;
+  double snv = sc + f1 * d + f2 * dso[q] + f12 * src;
+  if (IMP) snv = snv + f12 * dd;
+  if (IMP == 1) snv = snv - 0.5 * f12 * dd;
+  if (IMP == 2) ddo[q] = dd;
+  sn[c] = snv;
   dso[q] = d;
 }
 
@@ -613,14 +624,21 @@ __global__ __launch_bounds__(TX *TY) void k_volsum(Geo g, Metrics m, const doubl
 
 // bulk_forcing_s scal.f90:182-199 with f = scalf - (volume sum)/vol_all read
 // from the device; the first thread also records f
+// s += f; with dd (implicit diffusion of a forced scalar) s - h*dd first
 __global__ __launch_bounds__(TX *TY) void k_bulk_force(Geo g, double *__restrict__ s, const double *__restrict__ red,
-                                                        double scalf, double ivol, double *fout) {
+                                                        double scalf, double ivol, double *fout,
+                                                        const double *__restrict__ dd, double h) {
   const int i = blockIdx.x * TX + threadIdx.x + 1;
   const int j = blockIdx.y * TY + threadIdx.y + 1;
   const int k = blockIdx.z + 1;
   const double f = scalf - red[0] * ivol;
   if (fout && i == 1 && j == 1 && k == 1) fout[0] = f;
-  if (i <= g.nx && j <= g.ny) s[gidx(i, j, k, g.sx, g.sy)] += f;
+  if (i <= g.nx && j <= g.ny) {
+    const long c = gidx(i, j, k, g.sx, g.sy);
+    double v = s[c];
+    if (dd) v = v - h * dd[(long)(i - 1) + (long)g.nx * ((long)(j - 1) + (long)g.ny * (long)(k - 1))];
+    s[c] = v + f;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_reduce_final(const double *psum, const double *pmax, int n, double *out) {
@@ -730,9 +748,16 @@ void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *
 
 void launch_rk_scal(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v, const double *w,
                     const double *sc, double *sn, double *dso, double alpha, double f1, double f2, double f12,
-                    double src) {
-  hipLaunchKernelGGL(k_rk_scal, interior_tiles(g), dim3(TX, TY), 0, s, g, m, u, v, w, sc, sn, dso, alpha, f1, f2, f12,
-                     src);
+                    double src, int imp, double *ddo) {
+  if (imp == 2)
+    hipLaunchKernelGGL(k_rk_scal<2>, interior_tiles(g), dim3(TX, TY), 0, s, g, m, u, v, w, sc, sn, dso, alpha, f1, f2,
+                       f12, src, ddo);
+  else if (imp == 1)
+    hipLaunchKernelGGL(k_rk_scal<1>, interior_tiles(g), dim3(TX, TY), 0, s, g, m, u, v, w, sc, sn, dso, alpha, f1, f2,
+                       f12, src, ddo);
+  else
+    hipLaunchKernelGGL(k_rk_scal<0>, interior_tiles(g), dim3(TX, TY), 0, s, g, m, u, v, w, sc, sn, dso, alpha, f1, f2,
+                       f12, src, ddo);
   SNAC_HIP(hipGetLastError());
 }
 
@@ -742,8 +767,8 @@ void launch_volsum_part(hipStream_t s, const Geo &g, const Metrics &m, const dou
 }
 
 void launch_bulk_force(hipStream_t s, const Geo &g, double *f, const double *red, double scalf, double ivol,
-                       double *fout) {
-  hipLaunchKernelGGL(k_bulk_force, interior_grid(g), dim3(TX, TY), 0, s, g, f, red, scalf, ivol, fout);
+                       double *fout, const double *dd, double h) {
+  hipLaunchKernelGGL(k_bulk_force, interior_grid(g), dim3(TX, TY), 0, s, g, f, red, scalf, ivol, fout, dd, h);
   SNAC_HIP(hipGetLastError());
 }
 

@@ -10,6 +10,8 @@ duct     : inflow at y = 0 (Poiseuille profile across the x walls), outflow
            at y = Ly (velocity N, pressure D), walls in x, z periodic, y
            stretched: the inflow plane scales the boundary terms
 box      : a periodic box along every axis (every axis transformed)
+channel_scalar : the channel with a scalar between walls at 0 and 1,
+           forced to a bulk value (is_sforced) or not
 """
 import numpy as np
 
@@ -37,4 +39,15 @@ def box(n=(16, 8, 12)):
     return [cases.triperiodic_block(n, lengths=(2 * np.pi, 2 * np.pi, 2 * np.pi))], {}
 
 
-GEOMS = {"channel": channel, "cavity": cavity, "duct": duct, "box": box}
+def channel_scalar(n=(32, 24, 16), forced=True):
+    blocks, dns = channel(n)
+    blocks[0]["cbcscal"] = [[["F", "F"], ["D", "D"], ["F", "F"]]]
+    blocks[0]["bcscal"] = [[[1.0, 1.0], [0.0, 1.0], [1.0, 1.0]]]
+    dns.update(nscal=1, gacc=(0.0, -1.0, 0.0),
+               scalar=dict(alphai=[80.0], beta=0.3, iniscal=["zer"], ssource=[0.2], is_sforced=[forced],
+                           scalf=[0.5]))
+    return blocks, dns
+
+
+GEOMS = {"channel": channel, "cavity": cavity, "duct": duct, "box": box, "channel_scalar": channel_scalar,
+         "channel_scalar_free": lambda: channel_scalar(forced=False)}

@@ -112,6 +112,77 @@ def test_steps(tmp_path, geom, fft_axis):
     ctx.close()
 
 
+@pytest.mark.parametrize("geom", ["channel_scalar", "channel_scalar_free"])
+def test_scalar_solve_and_steps(tmp_path, geom):
+    """the scalar's Helmholtz solve (Dirichlet values 0 and 1 on the walls:
+    BC planes) and whole steps with the scalar forced to its bulk value (the
+    mean taken before the half diffusion comes off) or free, with buoyancy"""
+    ctx, oc = make_pair(tmp_path, geom, 3)
+    rng = np.random.default_rng(21)
+    a = rng.uniform(-1, 1, ctx.shape(0, "s1"))
+    ctx.set(0, "s1", a)
+    oc.set(0, "s1", a)
+    ctx.solve_impdiff("s1", -0.01)
+    oc.solve_impdiff("s1", -0.01)
+    in3 = (slice(1, -1),) * 3
+    assert rel(ctx.get(0, "s1")[in3], oc.get(0, "s1")[in3]) <= 1e-12
+    for f in ("u", "v", "w", "s1"):
+        a = np.zeros(ctx.shape(0, f))
+        a[in3] = 0.2 * rng.uniform(-1, 1, tuple(q - 2 for q in a.shape))
+        ctx.set(0, f, a)
+        oc.set(0, f, a)
+    ctx.init_bounds()
+    oc.init_bounds()
+    dt = 2.0e-3
+    ctx.steps(dt, 3)
+    for _ in range(3):
+        oc.step(dt)
+    for f in ("u", "v", "w", "p", "s1"):
+        g, o = ctx.get(0, f)[in3], oc.get(0, f)[in3]
+        if f == "p":
+            g, o = g - g.mean(), o - o.mean()
+        r = rel(g, o)
+        assert r <= 1e-10, f"{f}: rel L2 {r:.3e}"
+    ctx.close()
+
+
+@pytest.mark.parametrize("fft_axis", [2, 0])
+def test_dhc_reference_case_steps(tmp_path, fft_axis):
+    """the reference's differentially heated cavity (tests/
+    differentially_heated_cavity inputs) with implicit diffusion: walls in
+    x (scalar -0.5 / 0.5) and z, periodic y, Boussinesq buoyancy"""
+    import os
+    import snac_amd
+    g = os.path.join(os.path.dirname(__file__), "golden", "dhc")
+    d, b = os.path.join(g, "dns.nml"), os.path.join(g, "blocks.nml")
+    ctx = snac_amd.Context(d, b, fft_axis)
+    ctx.set_impdiff(True)
+    c = nmlparse.load_case(d, b, fft_axis)
+    c["impdiff"] = True
+    oc = oracle_py.OracleCase(c)
+    oc.set_solver("band")
+    rng = np.random.default_rng(4)
+    in3 = (slice(1, -1),) * 3
+    for f in ("u", "v", "w", "s1"):
+        a = np.zeros(ctx.shape(0, f))
+        a[in3] = 0.1 * rng.uniform(-1, 1, tuple(q - 2 for q in a.shape))
+        ctx.set(0, f, a)
+        oc.set(0, f, a)
+    ctx.init_bounds()
+    oc.init_bounds()
+    dt = 1.0e-3
+    ctx.steps(dt, 3)
+    for _ in range(3):
+        oc.step(dt)
+    for f in ("u", "v", "w", "p", "s1"):
+        gg, o = ctx.get(0, f)[in3], oc.get(0, f)[in3]
+        if f == "p":
+            gg, o = gg - gg.mean(), o - o.mean()
+        r = rel(gg, o)
+        assert r <= 1e-10, f"{f}: rel L2 {r:.3e}"
+    ctx.close()
+
+
 def test_shear_wave_closed_form(tmp_path):
     """u = A sin(y) on a periodic box (every axis transformed): advection and
     pressure vanish, each substage applies (1 - a)/(1 + a)"""

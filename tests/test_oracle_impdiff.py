@@ -248,3 +248,19 @@ def test_chkdt_without_diffusive_limit(tmp_path):
     assert oc.chkdt() == pytest.approx(np.sqrt(3.0) / dti, rel=1e-14)
     oc.set_impdiff(False)
     assert oc.chkdt() < np.sqrt(3.0) / dti  # the diffusive limit of visc = 100
+
+
+def test_inflow_plane_scaling(tmp_path):
+    """An inflow face (init_bc_rhs with the inflow plane, main.f90:600-636):
+    v = uref = 1.7 entering at y = 0 with periodic x, z (a uniform profile)
+    and zero-gradient outflow at y = Ly is a constant the Laplacian maps to
+    zero, so the solve returns it; a dropped or inverted bc_array / bc factor
+    would not"""
+    uref = 1.7
+    blk = cases._block((6, 8, 4), (0, 0, 0), (1.0, 2.0, 1.0), vel=[[F1, F1], ["D", "N"], [F1, F1]],
+                       pre=[[F1, F1], ["N", "D"], [F1, F1]], vals={(1, 1, 0): uref}, inflow={(1, 0): 1})
+    for fft_axis in (0, 1):
+        oc = make_case(tmp_path / str(fft_axis), [blk], fft_axis)
+        oc.set(0, "v", np.full(oc.shape(0, "v"), uref))
+        oc.solve_impdiff("v", -0.04)
+        np.testing.assert_allclose(oc.get(0, "v")[1:-1, 1:-2, 1:-1], uref, rtol=1e-13)
