@@ -18,6 +18,12 @@ host cores (a bounded sample of the same workload).
 --precision mixed runs the MG-PCG V-cycle in FP32 (MG-PCG path only; the
 channel's direct solver and the stencils stay FP64).
 
+--impdiff runs the channel with implicit diffusion (the reference's
+IMPDIFF=1 build): each substage adds three Helmholtz solves (one per velocity
+component) on the same tensor-product plan; not the headline workload, and
+no CPU sample (the oracle's direct Helmholtz solves are too slow at this
+size).
+
 --workload bfs runs the three-block backward-facing step instead (inlet
 192x64x128 with a Poiseuille inflow, lower 320x64x128, upper 320x96x128, z
 periodic with 128 modes, y tanh-stretched per block, outflow at x+): the
@@ -50,6 +56,7 @@ def parse():
     ap.add_argument("--size", type=int, nargs=3, default=[256, 128, 256])
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
                     help="mixed: FP32 MG V-cycle under flexible CG (MG-PCG path only)")
+    ap.add_argument("--impdiff", action="store_true", help="implicit diffusion (channel workload)")
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
     ap.add_argument("--calib", action="store_true", help="also run the plain copy kernel (counter calibration)")
@@ -129,6 +136,8 @@ def main():
         print(f"[bench] {args.workload} {'' if bfs else size} on device 0", file=sys.stderr, flush=True)
         ctx = snac_amd.Context(d, b, 3, device=0)
         ctx.set_precision(args.precision)
+        if args.impdiff:
+            ctx.set_impdiff(True)
         cells = sum(int(np.prod(np.array(ctx.shape(q, "u")) - 2)) for q in range(ctx.nblocks))
         if bfs:
             for q, fields in enumerate(bfs_ic(ctx)):
@@ -186,7 +195,7 @@ def main():
                     "stream_copy_GBs": round(copy_gbs, 1)}
 
         cpu = None
-        if args.cpu_steps > 0 and not bfs:
+        if args.cpu_steps > 0 and not bfs and not args.impdiff:
             thr = args.cpu_threads or min(16, os.cpu_count() or 1)
             print(f"[bench] CPU baseline: {args.cpu_steps} oracle steps on {thr} threads", file=sys.stderr,
                   flush=True)
@@ -197,7 +206,8 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f64+f32",
         "data": "synthetic (laminar Poiseuille + seeded +-10% uniform noise, seed 20251202)",
-        "config": ({"workload": "plain FP64 FFT-accelerated periodic channel",
+        "config": ({"workload": "plain FP64 FFT-accelerated periodic channel" +
+                                ("; implicit diffusion (three Helmholtz solves per substage)" if args.impdiff else ""),
                     "grid": list(size), "stretching": "y tanh two-end, factor 1.8", "fft_axis": "z",
                     "Re_b": 5600, "forcing": "bforce(1)=12/5600",
                     "poisson": "r2r along z and x, tridiagonal along y (exact)", "parallelism": "1 GPU",

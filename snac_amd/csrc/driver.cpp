@@ -3,7 +3,8 @@
 // writes the same data/ files (grid and geometry files, visualisation
 // fields and their log, checkpoints, time.out).  The argument is the FFT
 // axis (0..3), the reference's compile-time FFT_AXIS; the device ordinal
-// comes second (default 0).
+// comes second (default 0); a third argument "impdiff" selects implicit
+// diffusion (the reference's IMPDIFF=1 build).
 //
 // Order of main.f90: set-up and sanity checks (219-524, in the context's
 // creation), grid files (412-437), initial condition or checkpoint
@@ -81,10 +82,12 @@ void out0d(const std::string &path, const std::vector<double> &v) {
 int main(int argc, char **argv) {
   const int fft_axis = argc > 1 ? std::atoi(argv[1]) : 0;
   const int device = argc > 2 ? std::atoi(argv[2]) : 0;
+  const bool impdiff = argc > 3 && std::string(argv[3]) == "impdiff";
   const std::string datadir = "data/";
   std::filesystem::create_directories(datadir);
   check(snac_ctx_create_from_nml(&g_ctx, "dns.nml", "blocks.nml", fft_axis, device), "set-up");
   snac_ctx *c = g_ctx;
+  if (impdiff) check(snac_set_impdiff(c, 1), "implicit diffusion");
   double dp[11];
   int ip[8], rp[8];
   check(snac_ctx_get_params(c, dp, ip), "params");

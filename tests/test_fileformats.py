@@ -243,3 +243,36 @@ def test_driver_ldc_reference_procedure(tmp_path):
     u = np.fromfile(data / "fld_b_001_u.bin", dtype="float64")
     geo = np.loadtxt(data / "geometry_b_001.out", max_rows=4)
     assert u.size == int(np.prod(geo[1] - geo[0] + 1)) + 2 and u[-1] == 1500.0
+
+
+@pytest.mark.gpu
+def test_driver_impdiff_cavity(tmp_path):
+    """The main program with implicit diffusion (third argument "impdiff",
+    the reference's IMPDIFF=1 build): 10 fixed-dt steps of a lid-driven
+    cavity from rest; the final checkpoint equals the oracle's 10 _IMPDIFF
+    steps (relative L2 1e-10)."""
+    import subprocess
+    import nmlparse
+    import oracle_py
+    d, b = cases.write_case(str(tmp_path), [cases.cavity_block((16, 12, 20), lid=1.0)], 100.0, nstep=10,
+                            dt_f=2.0e-3)
+    txt = open(d).read().replace("isave = 0", "isave = 100").replace("iout0d = 0", "iout0d = 5")
+    open(d, "w").write(txt)
+    print("driver: cavity 16x12x20, FFT_AXIS=0, impdiff", flush=True)
+    r = subprocess.run([DRIVER, "0", "0", "impdiff"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "*** Fim ***" in r.stdout
+    c = nmlparse.load_case(d, b, 0)
+    c["impdiff"] = True
+    oc = oracle_py.OracleCase(c)
+    oc.set_solver("band")
+    oc.init_bounds()
+    for _ in range(10):
+        oc.step(2.0e-3)
+    for f in ("u", "v", "w"):
+        raw = np.fromfile(tmp_path / "data" / f"fld_b_001_{f}.bin", dtype="float64")
+        assert raw[-1] == 10.0 and raw[-2] == pytest.approx(0.02, rel=1e-14)
+        got = raw[:-2].reshape((16, 12, 20), order="F")
+        ref = oc.get(0, f)[1:-1, 1:-1, 1:-1]
+        assert np.linalg.norm(got - ref) <= 1e-10 * np.linalg.norm(ref), f
+    assert np.loadtxt(tmp_path / "data" / "time.out").shape == (2, 3)
