@@ -265,6 +265,29 @@ int snac_r2r_lines(snac_ctx *ctx, int kind, int n, int howmany, int strided, int
  * counterpart in the reference.                                             */
 int snac_stream_copy(snac_ctx *ctx, long n, int reps, double *gbps);
 
+/* ---- the learned preconditioner's network on its own (no counterpart in
+ *      the reference: its Poisson solver is HYPRE's MG-PCG).  A per-mode
+ *      convolutional encoder-decoder (4 levels, 16-64 channels, 3x3 convs,
+ *      BF16 MFMA operands with FP32 accumulation, FP32 master weights,
+ *      Adam).  Planes are H x W with H, W multiples of 8; inputs x are B
+ *      planes of two channels ([b][y][x][2]: the scaled right-hand side and
+ *      the mode's eigenvalue feature), outputs and targets B planes.  The
+ *      loss is mean_b |y_b - t_b|^2 / |t_b|^2; parameters are flattened per
+ *      layer as weights [tap][cout][cin] then biases [cout]. ---- */
+typedef struct snac_net snac_net;
+int  snac_net_create(snac_net **net, int H, int W, int maxbatch, unsigned seed, int device);
+int  snac_net_destroy(snac_net *net);
+long snac_net_num_params(const snac_net *net);
+int  snac_net_get_params(const snac_net *net, float *params);
+int  snac_net_set_params(snac_net *net, const float *params);
+int  snac_net_forward(snac_net *net, int B, const float *x, float *y);
+/* loss and its gradient w.r.t. every parameter (grad may be NULL) */
+int  snac_net_loss_grad(snac_net *net, int B, const float *x, const float *t, float *loss, float *grad);
+/* epochs of Adam over the samples in order, mini-batches of `batch`;
+ * loss_per_epoch (may be NULL) receives each epoch's mean batch loss      */
+int  snac_net_train(snac_net *net, int nsamples, const float *x, const float *t, int batch, int epochs, float lr,
+                    float *loss_per_epoch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -134,6 +134,17 @@ int  oc_apply_operator(const oc_case *c, int mode, const double *x, double *y);
 int  oc_solve_operator(oc_case *c, int mode, const double *b, double *x);
 int  oc_get_lambda(const oc_case *c, double *lambda, double *normfft);
 
+/* ---- the learned preconditioner's network (net_oracle.c; no reference
+ *      counterpart): forward and loss gradient of the encoder-decoder of
+ *      snac_amd/csrc/net.hpp.  x: B planes of 2 channels ([b][y][x][2]),
+ *      t and y: B planes; H, W multiples of 8; parameters flattened per
+ *      layer (weights [tap][cout][cin], then biases).  round = 1 rounds the
+ *      convolution operands to BF16 as the MFMA kernels do. ---- */
+long on_num_params(void);
+int  on_forward(int B, int H, int W, const float *params, const float *x, float *y, int round);
+int  on_loss_grad(int B, int H, int W, const float *params, const float *x, const float *t, float *loss,
+                  float *grad, int round);
+
 #ifdef __cplusplus
 }
 #endif

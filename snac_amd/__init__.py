@@ -90,6 +90,16 @@ def _load():
     L.snac_set_kernel_variant.argtypes = [vp, C.c_int, C.c_int]
     L.snac_set_timing.argtypes = [vp, C.c_int]
     L.snac_get_timing.argtypes = [vp, dp, ip]
+    fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    L.snac_net_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_uint, C.c_int]
+    L.snac_net_destroy.argtypes = [vp]
+    L.snac_net_num_params.argtypes = [vp]
+    L.snac_net_num_params.restype = C.c_long
+    L.snac_net_get_params.argtypes = [vp, fp]
+    L.snac_net_set_params.argtypes = [vp, fp]
+    L.snac_net_forward.argtypes = [vp, C.c_int, fp, fp]
+    L.snac_net_loss_grad.argtypes = [vp, C.c_int, fp, fp, C.POINTER(C.c_float), fp]
+    L.snac_net_train.argtypes = [vp, C.c_int, fp, fp, C.c_int, C.c_int, C.c_float, fp]
     return L
 
 
@@ -343,3 +353,49 @@ class Context:
         if self.dt_f > 0:
             return self.dt_f
         return min(self.cfl * self.chkdt(), self.dtmax)
+
+
+class Net:
+    """The learned preconditioner's network on its own (snac_net_*)."""
+
+    def __init__(self, H, W, maxbatch=8, seed=1, device=0):
+        h = C.c_void_p()
+        _check(lib.snac_net_create(C.byref(h), int(H), int(W), int(maxbatch), int(seed), int(device)), "net_create")
+        self.h, self.H, self.W = h, H, W
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.snac_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def params(self):
+        p = np.zeros(lib.snac_net_num_params(self.h), np.float32)
+        _check(lib.snac_net_get_params(self.h, p), "net_get_params")
+        return p
+
+    def set_params(self, p):
+        _check(lib.snac_net_set_params(self.h, np.ascontiguousarray(p, dtype=np.float32)), "net_set_params")
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros(x.shape[:3], np.float32)
+        _check(lib.snac_net_forward(self.h, x.shape[0], x, y), "net_forward")
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        g = np.zeros(lib.snac_net_num_params(self.h), np.float32)
+        loss = C.c_float()
+        _check(lib.snac_net_loss_grad(self.h, x.shape[0], x, np.ascontiguousarray(t, dtype=np.float32),
+                                      C.byref(loss), g), "net_loss_grad")
+        return loss.value, g
+
+    def train(self, x, t, batch, epochs, lr):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        hist = np.zeros(epochs, np.float32)
+        _check(lib.snac_net_train(self.h, x.shape[0], x, np.ascontiguousarray(t, dtype=np.float32), int(batch),
+                                  int(epochs), float(lr), hist), "net_train")
+        return hist

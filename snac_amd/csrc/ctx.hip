@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/snac_amd.h"
+#include "net.hpp"
 #include "bound_program.hpp"
 #include "case.hpp"
 #include "common.hpp"
@@ -1770,6 +1771,129 @@ int snac_r2r_lines(snac_ctx *c, int kind, int n, int howmany, int strided, int t
     SNAC_HIP(hipStreamSynchronize(c->stream));
     SNAC_HIP(hipFree(din));
     SNAC_HIP(hipFree(dout));
+  });
+}
+
+// ---- the learned preconditioner's network on its own (training, checks)
+struct snac_net {
+  snac::Net *net = nullptr;
+  hipStream_t stream = nullptr;
+  float *x = nullptr, *t = nullptr, *y = nullptr, *loss = nullptr;
+  size_t cap = 0;  // samples the staging buffers hold
+};
+
+namespace {
+void net_stage(snac_net *n, size_t samples) {
+  if (samples <= n->cap) return;
+  for (float *p : {n->x, n->t, n->y, n->loss}) if (p) (void)hipFree(p);
+  const size_t pl = (size_t)n->net->H() * n->net->W();
+  SNAC_HIP(hipMalloc(&n->x, samples * pl * 2 * sizeof(float)));
+  SNAC_HIP(hipMalloc(&n->t, samples * pl * sizeof(float)));
+  SNAC_HIP(hipMalloc(&n->y, samples * pl * sizeof(float)));
+  SNAC_HIP(hipMalloc(&n->loss, sizeof(float)));
+  n->cap = samples;
+}
+snac_net *net_checked(snac_net *n, int B = 1) {
+  if (!n || !n->net) throw std::runtime_error("null network");
+  if (B < 1 || B > n->net->maxbatch())
+    throw std::runtime_error("network batch " + std::to_string(B) + " outside 1.." + std::to_string(n->net->maxbatch()));
+  return n;
+}
+}  // namespace
+
+int snac_net_create(snac_net **out, int H, int W, int maxbatch, unsigned seed, int device) {
+  return guard([&] {
+    if (!out) throw std::runtime_error("null argument");
+    *out = nullptr;
+    SNAC_HIP(hipSetDevice(device));
+    auto *n = new snac_net();
+    try {
+      n->net = new Net(H, W, maxbatch, seed);
+      SNAC_HIP(hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking));
+    } catch (...) {
+      delete n->net;
+      delete n;
+      throw;
+    }
+    *out = n;
+  });
+}
+
+int snac_net_destroy(snac_net *n) {
+  if (!n) return 0;
+  if (n->stream) (void)hipStreamSynchronize(n->stream);
+  for (float *p : {n->x, n->t, n->y, n->loss}) if (p) (void)hipFree(p);
+  delete n->net;
+  if (n->stream) (void)hipStreamDestroy(n->stream);
+  delete n;
+  return 0;
+}
+
+long snac_net_num_params(const snac_net *n) { return n && n->net ? (long)n->net->num_params() : -1; }
+
+int snac_net_get_params(const snac_net *n, float *params) {
+  return guard([&] { net_checked(const_cast<snac_net *>(n))->net->get_params(params); });
+}
+
+int snac_net_set_params(snac_net *n, const float *params) {
+  return guard([&] { net_checked(n)->net->set_params(params); });
+}
+
+int snac_net_forward(snac_net *n, int B, const float *x, float *y) {
+  return guard([&] {
+    net_checked(n, B);
+    net_stage(n, B);
+    const size_t pl = (size_t)n->net->H() * n->net->W();
+    SNAC_HIP(hipMemcpyAsync(n->x, x, (size_t)B * pl * 2 * sizeof(float), hipMemcpyHostToDevice, n->stream));
+    n->net->set_input2(n->stream, B, n->x);
+    n->net->forward(n->stream, B);
+    n->net->get_output(n->stream, B, n->y);
+    SNAC_HIP(hipMemcpyAsync(y, n->y, (size_t)B * pl * sizeof(float), hipMemcpyDeviceToHost, n->stream));
+    SNAC_HIP(hipStreamSynchronize(n->stream));
+  });
+}
+
+int snac_net_loss_grad(snac_net *n, int B, const float *x, const float *t, float *loss, float *grad) {
+  return guard([&] {
+    net_checked(n, B);
+    net_stage(n, B);
+    const size_t pl = (size_t)n->net->H() * n->net->W();
+    SNAC_HIP(hipMemcpyAsync(n->x, x, (size_t)B * pl * 2 * sizeof(float), hipMemcpyHostToDevice, n->stream));
+    SNAC_HIP(hipMemcpyAsync(n->t, t, (size_t)B * pl * sizeof(float), hipMemcpyHostToDevice, n->stream));
+    n->net->set_input2(n->stream, B, n->x);
+    n->net->forward(n->stream, B);
+    n->net->loss_backward(n->stream, B, n->t, n->loss);
+    SNAC_HIP(hipMemcpyAsync(loss, n->loss, sizeof(float), hipMemcpyDeviceToHost, n->stream));
+    SNAC_HIP(hipStreamSynchronize(n->stream));
+    if (grad) n->net->grads_to_host(grad);
+  });
+}
+
+int snac_net_train(snac_net *n, int nsamples, const float *x, const float *t, int batch, int epochs, float lr,
+                   float *loss_per_epoch) {
+  return guard([&] {
+    net_checked(n);
+    if (nsamples < 1 || batch < 1 || batch > n->net->maxbatch() || epochs < 1)
+      throw std::runtime_error("net_train: invalid sizes");
+    net_stage(n, nsamples);
+    const size_t pl = (size_t)n->net->H() * n->net->W();
+    SNAC_HIP(hipMemcpyAsync(n->x, x, (size_t)nsamples * pl * 2 * sizeof(float), hipMemcpyHostToDevice, n->stream));
+    SNAC_HIP(hipMemcpyAsync(n->t, t, (size_t)nsamples * pl * sizeof(float), hipMemcpyHostToDevice, n->stream));
+    for (int e = 0; e < epochs; e++) {
+      double sum = 0.0;
+      int nb = 0;
+      for (int s0 = 0; s0 + batch <= nsamples; s0 += batch, nb++) {
+        n->net->set_input2(n->stream, batch, n->x + (size_t)s0 * pl * 2);
+        n->net->forward(n->stream, batch);
+        n->net->loss_backward(n->stream, batch, n->t + (size_t)s0 * pl, n->loss);
+        n->net->adam_step(n->stream, lr);
+        float l = 0.f;
+        SNAC_HIP(hipMemcpyAsync(&l, n->loss, sizeof(float), hipMemcpyDeviceToHost, n->stream));
+        SNAC_HIP(hipStreamSynchronize(n->stream));
+        sum += l;
+      }
+      if (loss_per_epoch) loss_per_epoch[e] = (float)(sum / std::max(nb, 1));
+    }
   });
 }
 
