@@ -84,10 +84,30 @@ int snac_ctx_set_solver(snac_ctx *ctx, int solver);
  * coarse solves) in FP32 under a flexible CG whose vectors, operator, dot
  * products and null-space removal stay FP64, so the solve still stops at
  * hypre_tol.  The stencils and the direct tensor-product solver stay FP64.
- * Drops the captured step graph; the next graph step re-measures the CG
- * count. */
-enum { SNAC_PREC_FP64 = 0, SNAC_PREC_MIXED = 1 };
+ * SNAC_PREC_LEARNED preconditions the same flexible CG with the trained
+ * network (snac_learned_train) on the FP32 residual of each mode, falling
+ * back to the FP32 V-cycle for the rest of a solve when some mode's residual
+ * reduction stalls; the FP64 residual still decides convergence (FFT path;
+ * steps run eagerly).  Drops the captured step graph; the next graph step
+ * re-measures the CG count. */
+enum { SNAC_PREC_FP64 = 0, SNAC_PREC_MIXED = 1, SNAC_PREC_LEARNED = 2 };
 int snac_set_precision(snac_ctx *ctx, int mode);
+/* ---- the learned preconditioner in the solver (no counterpart in the
+ *      reference).  collect: record the next solves' (right-hand side,
+ *      solution) pairs of every mode, each scaled by the RMS of its
+ *      right-hand side, up to npairs (MG-PCG path with an FFT axis; steps
+ *      run eagerly while collecting).  train: Adam on the recorded pairs
+ *      (the network is created on first use: 4 levels, 16-64 channels,
+ *      BF16 MFMA, planes of the union's bounding box rounded up to 8);
+ *      loss_per_epoch may be NULL.  options: the stall fallback (on by
+ *      default: an iteration that leaves some mode's residual norm above
+ *      0.5 of the previous one hands the rest of the solve to the V-cycle;
+ *      off: the network preconditions every iteration).  stats:
+ *      recorded pairs and solves that fell back to the V-cycle. ---- */
+int snac_learned_collect(snac_ctx *ctx, int npairs);
+int snac_learned_train(snac_ctx *ctx, int epochs, int batch, float lr, unsigned seed, float *loss_per_epoch);
+int snac_learned_options(snac_ctx *ctx, int fallback, double stall_ratio);
+int snac_learned_stats(snac_ctx *ctx, int *pairs, int *fallbacks);
 /* CG iterations of the last MG-PCG pressure solve and its largest relative
  * residual over the modes (HYPRE's StructPCGGetNumIterations /
  * GetFinalRelativeResidualNorm, solver.f90:553-569); synchronises.  Both

@@ -91,6 +91,10 @@ def _load():
     L.snac_set_timing.argtypes = [vp, C.c_int]
     L.snac_get_timing.argtypes = [vp, dp, ip]
     fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    L.snac_learned_collect.argtypes = [vp, C.c_int]
+    L.snac_learned_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_uint, fp]
+    L.snac_learned_options.argtypes = [vp, C.c_int, C.c_double]
+    L.snac_learned_stats.argtypes = [vp, ip, ip]
     L.snac_net_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_uint, C.c_int]
     L.snac_net_destroy.argtypes = [vp]
     L.snac_net_num_params.argtypes = [vp]
@@ -303,7 +307,7 @@ class Context:
         uniform transform axes, else MG-PCG), 'tensor' or 'mg'."""
         _check(lib.snac_ctx_set_solver(self.h, self.SOLVERS[name]), "set_solver")
 
-    PRECISIONS = {"fp64": 0, "mixed": 1}
+    PRECISIONS = {"fp64": 0, "mixed": 1, "learned": 2}
 
     def set_precision(self, mode):
         """'fp64' (default) or 'mixed': the MG-PCG V-cycle in FP32 under a
@@ -311,6 +315,23 @@ class Context:
         removal stay FP64 (same stopping tolerance).  Stencils and the direct
         tensor-product solver stay FP64."""
         _check(lib.snac_set_precision(self.h, self.PRECISIONS[mode]), "set_precision")
+
+    def learned_collect(self, npairs):
+        """Record the next MG-PCG solves' (rhs, solution) pairs (npairs planes)."""
+        _check(lib.snac_learned_collect(self.h, int(npairs)), "learned_collect")
+
+    def learned_train(self, epochs, batch, lr, seed=1):
+        hist = np.zeros(epochs, np.float32)
+        _check(lib.snac_learned_train(self.h, int(epochs), int(batch), float(lr), int(seed), hist), "learned_train")
+        return hist
+
+    def learned_options(self, fallback=True, stall_ratio=0.5):
+        _check(lib.snac_learned_options(self.h, int(bool(fallback)), float(stall_ratio)), "learned_options")
+
+    def learned_stats(self):
+        a, b = np.zeros(1, np.int32), np.zeros(1, np.int32)
+        _check(lib.snac_learned_stats(self.h, a, b), "learned_stats")
+        return int(a[0]), int(b[0])
 
     def solver_stats(self):
         """(CG iterations, largest relative residual over the modes) of the

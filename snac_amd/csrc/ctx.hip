@@ -937,7 +937,10 @@ void drop_graph(snac_ctx *c) {
 // starting parity changes.
 void run_steps(snac_ctx *c, double dt, int nsteps) {
   check_solver(c);
-  if (!c->use_graph || c->timing) {
+  // the learned preconditioner and the pair recording decide on the host
+  // between CG iterations: eager launches
+  const bool host_cg = use_mg(c) && (c->prec == SNAC_PREC_LEARNED || c->mg.collecting);
+  if (!c->use_graph || c->timing || host_cg) {
     for (int s = 0; s < nsteps; s++)
       for (int irk = 1; irk <= 3; irk++) substage(c, irk, dt);
     return;
@@ -1238,13 +1241,84 @@ int snac_solver_stats(snac_ctx *c, int *iters, double *max_relres) {
 int snac_set_precision(snac_ctx *c, int mode) {
   return guard([&] {
     checked(c);
-    if (mode != SNAC_PREC_FP64 && mode != SNAC_PREC_MIXED) throw std::runtime_error("unknown precision mode");
+    if (mode != SNAC_PREC_FP64 && mode != SNAC_PREC_MIXED && mode != SNAC_PREC_LEARNED)
+      throw std::runtime_error("unknown precision mode");
+    if (mode == SNAC_PREC_LEARNED && !(c->mg_planned && c->mg.net))
+      throw std::runtime_error("the learned preconditioner needs a trained network (snac_learned_train)");
     if (c->device >= 0) SNAC_HIP(hipStreamSynchronize(c->stream));
     c->prec = mode;
     c->mg.prec = mode;
     c->mg.fixed_iters = 0;  // the graph's CG count is measured again
     c->mg.max_seen = 0;
     drop_graph(c);
+  });
+}
+
+int snac_learned_collect(snac_ctx *c, int npairs) {
+  return guard([&] {
+    on_device(c);
+    check_solver(c);
+    MGPlan &P = c->mg;
+    if (!use_mg(c) || !c->mg_planned || !P.ok || P.fa < 0)
+      throw std::runtime_error("the learned preconditioner needs the MG-PCG path with an FFT axis");
+    if (npairs < P.ng) throw std::runtime_error("collect at least one solve's modes (" + std::to_string(P.ng) + ")");
+    SNAC_HIP(hipStreamSynchronize(c->stream));
+    const size_t pl = (size_t)P.nH * P.nW;
+    if (npairs > P.pair_cap) {
+      P.pair_x = dalloc<float>(c, (size_t)npairs * pl * 2);
+      P.pair_t = dalloc<float>(c, (size_t)npairs * pl);
+      P.pair_cap = npairs;
+    }
+    P.pair_n = 0;
+    P.collecting = true;
+    drop_graph(c);
+  });
+}
+
+int snac_learned_train(snac_ctx *c, int epochs, int batch, float lr, unsigned seed, float *loss_per_epoch) {
+  return guard([&] {
+    on_device(c);
+    MGPlan &P = c->mg;
+    if (!c->mg_planned || P.pair_n < 1) throw std::runtime_error("no recorded pairs (snac_learned_collect)");
+    if (epochs < 1 || batch < 1 || batch > P.pair_n) throw std::runtime_error("learned_train: invalid sizes");
+    SNAC_HIP(hipStreamSynchronize(c->stream));
+    P.collecting = false;
+    if (!P.net) P.net = new Net(P.nH, P.nW, std::max(batch, P.ng), seed);
+    if (batch > P.net->maxbatch()) throw std::runtime_error("learned_train: batch above the network's");
+    float *loss = dalloc<float>(c, 1);
+    const size_t pl = (size_t)P.nH * P.nW;
+    for (int e = 0; e < epochs; e++) {
+      double sum = 0.0;
+      int nb = 0;
+      for (int s0 = 0; s0 + batch <= P.pair_n; s0 += batch, nb++) {
+        P.net->set_input2(c->stream, batch, P.pair_x + (size_t)s0 * pl * 2);
+        P.net->forward(c->stream, batch);
+        P.net->loss_backward(c->stream, batch, P.pair_t + (size_t)s0 * pl, loss);
+        P.net->adam_step(c->stream, lr);
+        float l = 0.f;
+        SNAC_HIP(hipMemcpyAsync(&l, loss, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        SNAC_HIP(hipStreamSynchronize(c->stream));
+        sum += l;
+      }
+      if (loss_per_epoch) loss_per_epoch[e] = (float)(sum / std::max(nb, 1));
+    }
+  });
+}
+
+int snac_learned_options(snac_ctx *c, int fallback, double stall_ratio) {
+  return guard([&] {
+    checked(c);
+    if (!(stall_ratio > 0.0 && stall_ratio <= 1.0)) throw std::runtime_error("stall ratio must be in (0, 1]");
+    c->mg.fallback = fallback != 0;
+    c->mg.stall_ratio = stall_ratio;
+  });
+}
+
+int snac_learned_stats(snac_ctx *c, int *pairs, int *fallbacks) {
+  return guard([&] {
+    checked(c);
+    if (pairs) *pairs = c->mg.pair_n;
+    if (fallbacks) *fallbacks = c->mg.fallbacks;
   });
 }
 

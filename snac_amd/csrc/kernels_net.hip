@@ -276,6 +276,61 @@ __global__ void k_channel0(long npix, const float *__restrict__ y16, float *__re
     y[q] = y16[q * 16];
 }
 
+__global__ void k_net_in(int ng, int na, int ns, int nH, int nW, const float *__restrict__ r,
+                         const double *__restrict__ ss, double nact, const double *__restrict__ lam, double ilam,
+                         float *__restrict__ x16) {
+  const long n = (long)ng * nH * nW;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    const int x = (int)(q % nW), y = (int)((q / nW) % nH), g = (int)(q / ((long)nH * nW));
+    const double rms = sqrt(ss[2 * g] / nact);
+    const float v = (y < na && x < ns && rms > 0.0) ? (float)(r[((long)g * na + y) * ns + x] / rms) : 0.f;
+    float4 *o = (float4 *)(x16 + q * 16);
+    o[0] = make_float4(v, (float)(lam[g] * ilam), 0.f, 0.f);
+    o[1] = o[2] = o[3] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+__global__ void k_net_out(int ng, int na, int ns, int nH, int nW, const float *__restrict__ y16,
+                          const double *__restrict__ ss, double nact, const unsigned char *__restrict__ act,
+                          float *__restrict__ z) {
+  const long n = (long)ng * na * ns;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    const int x = (int)(q % ns), y = (int)((q / ns) % na), g = (int)(q / ((long)na * ns));
+    const double rms = sqrt(ss[2 * g] / nact);
+    z[q] = act[(long)y * ns + x] ? (float)(rms * y16[(((long)g * nH + y) * nW + x) * 16]) : 0.f;
+  }
+}
+
+__global__ void k_net_pair_in(int ng, int na, int ns, int nH, int nW, const double *__restrict__ b,
+                              const double *__restrict__ bb, double nact, const double *__restrict__ lam, double ilam,
+                              float *__restrict__ x2) {
+  const long n = (long)ng * nH * nW;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    const int x = (int)(q % nW), y = (int)((q / nW) % nH), g = (int)(q / ((long)nH * nW));
+    const double rms = sqrt(bb[2 * g] / nact);
+    x2[2 * q] = (y < na && x < ns && rms > 0.0) ? (float)(b[((long)g * na + y) * ns + x] / rms) : 0.f;
+    x2[2 * q + 1] = (float)(lam[g] * ilam);
+  }
+}
+
+__global__ void k_net_pair_out(int ng, int na, int ns, int nH, int nW, const double *__restrict__ xs,
+                               const double *__restrict__ bb, double nact, float *__restrict__ t) {
+  const long n = (long)ng * nH * nW;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    const int x = (int)(q % nW), y = (int)((q / nW) % nH), g = (int)(q / ((long)nH * nW));
+    const double rms = sqrt(bb[2 * g] / nact);
+    t[q] = (y < na && x < ns && rms > 0.0) ? (float)(xs[((long)g * na + y) * ns + x] / rms) : 0.f;
+  }
+}
+
+__global__ void k_net_stall(int ng, const double *__restrict__ rr, double *__restrict__ prev,
+                            const int *__restrict__ done, double ratio2, int *__restrict__ stall) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += gridDim.x * blockDim.x) {
+    if (!done[g] && rr[2 * g] > ratio2 * prev[2 * g]) stall[0] = 1;
+    prev[2 * g] = rr[2 * g];
+  }
+}
+
 unsigned grid_for(long n) { return (unsigned)std::min<long>((n + 255) / 256, 8192); }
 
 template <int CIN, int COUT>
@@ -318,6 +373,40 @@ void bias_grad_any(hipStream_t s, int cout, long npix, const float *g, float *db
 }
 
 }  // namespace
+
+void launch_net_in(hipStream_t s, int ng, int na, int ns, int nH, int nW, const float *r, const double *ss,
+                   double nact, const double *lam, double lam_ref, float *x16) {
+  hipLaunchKernelGGL(k_net_in, dim3(grid_for((long)ng * nH * nW)), dim3(256), 0, s, ng, na, ns, nH, nW, r, ss, nact,
+                     lam, 1.0 / lam_ref, x16);
+  SNAC_HIP(hipGetLastError());
+}
+
+void launch_net_out(hipStream_t s, int ng, int na, int ns, int nH, int nW, const float *y16, const double *ss,
+                    double nact, const unsigned char *act, float *z) {
+  hipLaunchKernelGGL(k_net_out, dim3(grid_for((long)ng * na * ns)), dim3(256), 0, s, ng, na, ns, nH, nW, y16, ss,
+                     nact, act, z);
+  SNAC_HIP(hipGetLastError());
+}
+
+void launch_net_pair_in(hipStream_t s, int ng, int na, int ns, int nH, int nW, const double *b, const double *bb,
+                        double nact, const double *lam, double lam_ref, float *x2) {
+  hipLaunchKernelGGL(k_net_pair_in, dim3(grid_for((long)ng * nH * nW)), dim3(256), 0, s, ng, na, ns, nH, nW, b, bb,
+                     nact, lam, 1.0 / lam_ref, x2);
+  SNAC_HIP(hipGetLastError());
+}
+
+void launch_net_pair_out(hipStream_t s, int ng, int na, int ns, int nH, int nW, const double *x, const double *bb,
+                         double nact, float *t) {
+  hipLaunchKernelGGL(k_net_pair_out, dim3(grid_for((long)ng * nH * nW)), dim3(256), 0, s, ng, na, ns, nH, nW, x, bb,
+                     nact, t);
+  SNAC_HIP(hipGetLastError());
+}
+
+void launch_net_stall(hipStream_t s, int ng, const double *rr, double *prev, const int *done, double ratio2,
+                      int *stall) {
+  hipLaunchKernelGGL(k_net_stall, dim3((ng + 255) / 256), dim3(256), 0, s, ng, rr, prev, done, ratio2, stall);
+  SNAC_HIP(hipGetLastError());
+}
 
 float bf16_round(float x) {
   const unsigned short b = (unsigned short)bf16_bits(x);

@@ -28,6 +28,8 @@
 
 namespace snac {
 
+class Net;
+
 // one level, passed by value to the kernels; T is the element type of the
 // level's vectors and coefficients (double, or float for the mixed-precision
 // V-cycle)
@@ -141,7 +143,24 @@ struct MGPlan {
   // restriction, prolongation, coarse solves) in FP32, the CG vectors, the
   // operator, the dot products and the null-space removal of x, r in FP64,
   // flexible CG (Polak-Ribiere beta) for the FP32 preconditioner
+  //  2: learned -- the preconditioner is the network (net.hpp) on the FP32
+  //  residual, in the flexible CG of the mixed mode, falling back to the
+  //  FP32 V-cycle for the rest of a solve when some mode's residual norm
+  //  falls by less than the factor stall_ratio in an iteration (FFT path,
+  //  eagerly launched solves)
   int prec = 0;
+  Net *net = nullptr;
+  int nH = 0, nW = 0;             // network planes (na, ns rounded up to 8)
+  double lam_ref = 1.0, nact_plane = 1.0;
+  bool fallback = true;
+  double stall_ratio = 0.5;
+  int fallbacks = 0;              // solves that fell back to the V-cycle
+  int *stall = nullptr, *stall_host = nullptr;
+  double *rr_prev = nullptr;
+  // training pairs recorded from the solves (network planes, FP32)
+  float *pair_x = nullptr, *pair_t = nullptr;
+  int pair_cap = 0, pair_n = 0;
+  bool collecting = false;
   double *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
   float *zf = nullptr, *rf = nullptr;
   double *part = nullptr, *bb = nullptr, *rr = nullptr, *pq = nullptr, *rz[2] = {nullptr, nullptr}, *sums = nullptr;

@@ -23,6 +23,7 @@
 #include <cstring>
 
 #include "mg.hpp"
+#include "net.hpp"
 
 namespace snac {
 namespace {
@@ -138,10 +139,14 @@ HostLev coarsen(const HostLev &F, bool coupled) {
 }  // namespace
 
 void mg_free(MGPlan &P) {
+  delete P.net;
+  P.net = nullptr;
   for (void *p : P.owned) (void)hipFree(p);
   P.owned.clear();
   if (P.left_host) (void)hipHostFree(P.left_host);
   P.left_host = nullptr;
+  if (P.stall_host) (void)hipHostFree(P.stall_host);
+  P.stall_host = nullptr;
 }
 
 void mg_plan(MGPlan &P, const Case &cs, int device) {
@@ -349,6 +354,11 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
   P.gsize = coupled ? (long)nm * na * ns : (long)na * ns;
   std::vector<int> sing(P.ng, 0);
   std::vector<double> nact(P.ng, nact_plane);  // active cells per group
+  P.nact_plane = std::max(nact_plane, 1.0);
+  P.lam_ref = 1.0;
+  for (double l : lamh) P.lam_ref = std::max(P.lam_ref, std::fabs(l));
+  P.nH = (P.G[P.ax_a] + 7) / 8 * 8;
+  P.nW = (P.G[P.ax_s] + 7) / 8 * 8;
   P.any_singular = false;
   for (int g = 0; g < P.ng; g++) {
     sing[g] = zero_rowsum && (coupled || lamh[g] == 0.0);
@@ -456,6 +466,9 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
   P.left = dev_zeros<int>(P, 1);
   P.unconv = dev_zeros<int>(P, 1);
   SNAC_HIP(hipHostMalloc((void **)&P.left_host, sizeof(int), hipHostMallocDefault));
+  P.stall = dev_zeros<int>(P, 1);
+  P.rr_prev = dev_zeros<double>(P, 2 * P.ng);
+  SNAC_HIP(hipHostMalloc((void **)&P.stall_host, sizeof(int), hipHostMallocDefault));
   P.ok = true;
 }
 
@@ -496,11 +509,20 @@ void project(MGPlan &P, hipStream_t st, T *v) {
   mg_sub_mean(st, P.h64.lev[0], v, P.sums, P.singular, P.nact, P.done);
 }
 
-// z = V-cycle(r) with its null-space component removed; rz = r.z; p = z +
-// beta p (first: p = z)
-void precondition(MGPlan &P, hipStream_t st, double *rz_new, const double *rz_old) {
-  if (P.prec == 1) {
-    vcycle(P, P.h32, st, 0);
+// z = V-cycle(r) (or the network's answer, net) with its null-space
+// component removed; rz = r.z; p = z + beta p (first: p = z).  rr: the
+// per-group r.r of the current residual (the network's input scale)
+void precondition(MGPlan &P, hipStream_t st, double *rz_new, const double *rz_old, bool net, const double *rr) {
+  if (P.prec >= 1) {
+    if (net) {
+      const MGLev &L0 = P.h64.lev[0];
+      launch_net_in(st, P.ng, L0.na, L0.ns, P.nH, P.nW, P.rf, rr, P.nact_plane, P.h64.lam, P.lam_ref,
+                    P.net->input());
+      P.net->forward(st, P.ng);
+      launch_net_out(st, P.ng, L0.na, L0.ns, P.nH, P.nW, P.net->output(), rr, P.nact_plane, L0.act, P.zf);
+    } else {
+      vcycle(P, P.h32, st, 0);
+    }
     project(P, st, P.zf);
     mg_dot<float, float>(st, P.ng, P.gsize, P.r, P.zf, nullptr, nullptr, P.part, P.nchunk, rz_new, P.done, nullptr);
     mg_cg_dir<float>(st, P.ng, P.gsize, rz_new, rz_old, rz_old ? P.rr : nullptr, P.zf, P.p, P.done);
@@ -517,15 +539,27 @@ void precondition(MGPlan &P, hipStream_t st, double *rz_new, const double *rz_ol
 void mg_solve(MGPlan &P, hipStream_t st, int fixed) {
   const long n = (long)P.ng * P.gsize;
   const double tol2 = P.tol * P.tol;
-  const bool mixed = P.prec == 1;
+  const bool mixed = P.prec >= 1;  // the flexible CG of the FP32 preconditioners
+  const bool planar = P.fa >= 0;   // one 2D plane per group (the network's input)
+  bool net = P.prec == 2 && P.net && planar && fixed == 0;
+  const bool record = P.collecting && planar && fixed == 0 && P.pair_n + P.ng <= P.pair_cap;
+  const MGLev &L0 = P.h64.lev[0];
+  const size_t pl = (size_t)P.nH * P.nW;
   mg_zero(st, n, P.x);
   mg_izero(st, P.ng, P.done);
   project(P, st, P.r);
   mg_dot<double, double>(st, P.ng, P.gsize, P.r, P.r, nullptr, nullptr, P.part, P.nchunk, P.bb, P.done, nullptr);
   mg_flags(st, P.ng, P.bb, P.bb, tol2, P.done, P.left);
+  if (record)  // the input half of this solve's training pairs
+    launch_net_pair_in(st, P.ng, L0.na, L0.ns, P.nH, P.nW, P.r, P.bb, P.nact_plane, P.h64.lam, P.lam_ref,
+                       P.pair_x + (size_t)P.pair_n * pl * 2);
   if (mixed) mg_to_float(st, n, P.r, P.rf);
+  if (net) {
+    mg_izero(st, 1, P.stall);
+    SNAC_HIP(hipMemcpyAsync(P.rr_prev, P.bb, 2 * P.ng * sizeof(double), hipMemcpyDeviceToDevice, st));
+  }
   int cur = 0;
-  precondition(P, st, P.rz[cur], nullptr);
+  precondition(P, st, P.rz[cur], nullptr, net, P.bb);
   const int nit = fixed > 0 ? std::min(fixed, P.maxiter) : P.maxiter;
   int it = 0;
   for (it = 1; it <= nit; it++) {
@@ -534,15 +568,25 @@ void mg_solve(MGPlan &P, hipStream_t st, int fixed) {
     mg_cg_update_dot(st, P.ng, P.gsize, P.rz[cur], P.pq, P.x, P.p, P.r, P.q, mixed ? P.rf : nullptr,
                      mixed ? P.zf : nullptr, P.part, P.nchunk, P.rr, P.done);
     mg_flags(st, P.ng, P.rr, P.bb, tol2, P.done, P.left);
+    if (net && P.fallback) launch_net_stall(st, P.ng, P.rr, P.rr_prev, P.done, P.stall_ratio * P.stall_ratio, P.stall);
     if (fixed == 0) {
       SNAC_HIP(hipMemcpyAsync(P.left_host, P.left, sizeof(int), hipMemcpyDeviceToHost, st));
+      if (net && P.fallback) SNAC_HIP(hipMemcpyAsync(P.stall_host, P.stall, sizeof(int), hipMemcpyDeviceToHost, st));
       SNAC_HIP(hipStreamSynchronize(st));
       if (*P.left_host == 0) break;
+      if (net && P.fallback && *P.stall_host) {  // the network stopped helping: V-cycles from here on
+        net = false;
+        P.fallbacks++;
+      }
     } else if (it == nit) {
       break;
     }
-    precondition(P, st, P.rz[1 - cur], P.rz[cur]);
+    precondition(P, st, P.rz[1 - cur], P.rz[cur], net, P.rr);
     cur = 1 - cur;
+  }
+  if (record) {  // the target half: the solution, scaled like the input
+    launch_net_pair_out(st, P.ng, L0.na, L0.ns, P.nH, P.nW, P.x, P.bb, P.nact_plane, P.pair_t + (size_t)P.pair_n * pl);
+    P.pair_n += P.ng;
   }
   if (fixed > 0) {
     mg_accum_max(st, P.unconv, P.left);  // groups still above the tolerance

@@ -86,4 +86,30 @@ class Net {
 // bf16 of an FP32 value, rounded to nearest even (the MFMA operands)
 float bf16_round(float x);
 
+// ---- the MG-PCG planes <-> network planes (kernels_net.hip) ----
+// Group g of a plan vector is the na x ns plane (row a, column s, s
+// contiguous) of mode g; a network plane is nH x nW (>= na, ns, multiples
+// of 8), zero outside.  Each plane is scaled by its RMS, sqrt(ss[2g] /
+// nact), so the network sees unit-RMS inputs (ss: per-group sums of squares,
+// stride 2, as the CG's dot products store them); channel 1 carries the
+// mode's eigenvalue feature lam[g] / lam_ref.
+// x16: the network input (ng planes of 16 channels)
+void launch_net_in(hipStream_t s, int ng, int na, int ns, int nH, int nW, const float *r, const double *ss,
+                   double nact, const double *lam, double lam_ref, float *x16);
+// z = scale_g * y (channel 0 of the network output) on the active cells,
+// 0 elsewhere
+void launch_net_out(hipStream_t s, int ng, int na, int ns, int nH, int nW, const float *y16, const double *ss,
+                    double nact, const unsigned char *act, float *z);
+// training pairs from a solve: the input half from the right-hand side
+// (FP64 plan vector b with sums of squares bb), the target half x / scale
+// from the solution
+void launch_net_pair_in(hipStream_t s, int ng, int na, int ns, int nH, int nW, const double *b, const double *bb,
+                        double nact, const double *lam, double lam_ref, float *x2);
+void launch_net_pair_out(hipStream_t s, int ng, int na, int ns, int nH, int nW, const double *x, const double *bb,
+                         double nact, float *t);
+// stall |= some group not done whose r.r fell by less than ratio2 since
+// the last call; then prev = rr (both stride 2)
+void launch_net_stall(hipStream_t s, int ng, const double *rr, double *prev, const int *done, double ratio2,
+                      int *stall);
+
 }  // namespace snac

@@ -1,0 +1,95 @@
+"""The learned preconditioner in the MG-PCG pressure solve (no counterpart
+in the reference): pairs recorded from the plain solver, training on the
+GPU, the network as the flexible-CG preconditioner judged by the FP64
+residual.
+
+* the learned solve (network, V-cycle fallback on stall) reaches the same
+  fields as the plain FP64 solve (relative L2 1e-10; both stop at
+  hypre_tol = 1e-13);
+* the trained network used alone (no fallback) against the same network
+  untrained, both for at most hypre_maxiter iterations: training lowers
+  the residual it reaches (or the iterations it needs) -- what training
+  gains;
+* the CG iteration counts of plain MG-PCG, mixed precision, the network
+  with fallback and the network alone are printed (the A/B comparison
+  against the method without the network).
+"""
+import numpy as np
+import pytest
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def make(tmp_path, n=(64, 48, 16)):
+    import snac_amd
+    d, b = cases.write_case(str(tmp_path), [cases.channel_block(n)], 5600.0, bforce=(12 / 5600, 0, 0))
+    ctx = snac_amd.Context(d, b, 3)
+    ctx.set_solver("mg")
+    u, v, w = cases.channel_ic(ctx.shape(0, "u"), ctx.metric(0, 8))
+    for f, a in (("u", u), ("v", v), ("w", w)):
+        ctx.set(0, f, a)
+    ctx.init_bounds()
+    return ctx
+
+
+def snapshot(ctx):
+    return {f: ctx.get(0, f) for f in ("u", "v", "w", "p", "pp", "dudtrko", "dvdtrko", "dwdtrko")}
+
+
+def restore(ctx, snap):
+    for f, a in snap.items():
+        ctx.set(0, f, a)
+
+
+def one_step(ctx, snap, dt, prec):
+    restore(ctx, snap)
+    ctx.set_precision(prec)
+    ctx.set_graph(False)
+    ctx.step(dt)
+    it, rel = ctx.solver_stats()
+    return it, rel, snapshot(ctx)
+
+
+def test_learned_preconditioner(tmp_path):
+    print("learned: channel 64x48x16 on the MG-PCG path", flush=True)
+    ctx = make(tmp_path / "a")
+    dt = 0.5 * ctx.chkdt()
+    ctx.learned_collect(16 * 3 * 12)
+    ctx.steps(dt, 12)
+    pairs, _ = ctx.learned_stats()
+    assert pairs == 16 * 3 * 12
+    hist = ctx.learned_train(epochs=30, batch=16, lr=2e-3)
+    print("training loss per epoch", np.round(hist, 4), flush=True)
+    assert np.all(np.isfinite(hist)) and hist[-1] < 0.5 * hist[0]
+    snap = snapshot(ctx)
+    res = {}
+    res["fp64"] = one_step(ctx, snap, dt, "fp64")
+    res["mixed"] = one_step(ctx, snap, dt, "mixed")
+    ctx.learned_options(fallback=True)
+    res["learned"] = one_step(ctx, snap, dt, "learned")
+    ctx.learned_options(fallback=False)
+    res["network only"] = one_step(ctx, snap, dt, "learned")
+    for k, (it, rr, _) in res.items():
+        print(f"{k:>13}: {it} CG iterations, max relative residual {rr:.2e}", flush=True)
+    ref = res["fp64"][2]
+    for k in ("mixed", "learned"):
+        assert res[k][1] <= 1e-13 * 1.0001
+        for f in ("u", "v", "w"):
+            g, o = res[k][2][f][1:-1, 1:-1, 1:-1], ref[f][1:-1, 1:-1, 1:-1]
+            assert np.linalg.norm(g - o) <= 1e-10 * np.linalg.norm(o), (k, f)
+    trained_its, trained_rel = res["network only"][:2]
+    ctx.close()
+    # the same network untrained (one epoch at zero learning rate)
+    ctx2 = make(tmp_path / "b")
+    ctx2.learned_collect(16)
+    ctx2.steps(dt, 1)
+    ctx2.learned_train(epochs=1, batch=16, lr=0.0)
+    ctx2.learned_options(fallback=False)
+    untrained_its, untrained_rel = one_step(ctx2, snap, dt, "learned")[:2]
+    print(f"network only, untrained: {untrained_its} CG iterations, max relative residual {untrained_rel:.2e} "
+          f"(trained: {trained_its}, {trained_rel:.2e})", flush=True)
+    assert (trained_its, trained_rel) < (untrained_its, untrained_rel)
+    assert trained_its < untrained_its or trained_rel < 0.1 * untrained_rel
+    ctx2.close()
