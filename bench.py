@@ -16,7 +16,13 @@ per cell) from HIP events on its own stream, and the CPU port's rate on the
 host cores (a bounded sample of the same workload).
 
 --precision mixed runs the MG-PCG V-cycle in FP32 (MG-PCG path only; the
-channel's direct solver and the stencils stay FP64).
+channel's direct solver and the stencils stay FP64).  --precision learned
+(MG-PCG path) first records the plain solver's (rhs, solution) pairs over
+--learn-steps steps, trains the network on them for --learn-epochs epochs,
+then times steps with the network preconditioning the flexible CG (V-cycle
+fallback when its residual reduction stalls; eager launches); the line adds
+the network's MFMA rate (one forward over every mode, HIP events) against
+the 2.5 PFLOP/s dense BF16 peak.
 
 --impdiff runs the channel with implicit diffusion (the reference's
 IMPDIFF=1 build): each substage adds three Helmholtz solves (one per velocity
@@ -54,7 +60,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", choices=["channel", "bfs"], default="channel")
     ap.add_argument("--size", type=int, nargs=3, default=[256, 128, 256])
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+    ap.add_argument("--learn-steps", type=int, default=20)
+    ap.add_argument("--learn-epochs", type=int, default=4)
+    ap.add_argument("--precision", choices=["fp64", "mixed", "learned"], default="fp64",
                     help="mixed: FP32 MG V-cycle under flexible CG (MG-PCG path only)")
     ap.add_argument("--impdiff", action="store_true", help="implicit diffusion (channel workload)")
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0 = skip)")
@@ -135,7 +143,8 @@ def main():
             d, b = channel_setup(tmp, size)
         print(f"[bench] {args.workload} {'' if bfs else size} on device 0", file=sys.stderr, flush=True)
         ctx = snac_amd.Context(d, b, 3, device=0)
-        ctx.set_precision(args.precision)
+        if args.precision != "learned":
+            ctx.set_precision(args.precision)
         if args.impdiff:
             ctx.set_impdiff(True)
         cells = sum(int(np.prod(np.array(ctx.shape(q, "u")) - 2)) for q in range(ctx.nblocks))
@@ -151,6 +160,26 @@ def main():
         tune_choices, tune_ms = ctx.autotune()
         dt = min(0.5 * ctx.chkdt(), ctx.dtmax)
         copy_gbs = ctx.stream_copy(1 << 27, reps=10 if args.calib else 3)
+        learned = None
+        if args.precision == "learned":
+            ng = int(ctx.block_info(0)["n"][2])  # modes along z
+            ctx.learned_collect(ng * 3 * args.learn_steps)
+            t0 = time.perf_counter()
+            ctx.steps(dt, args.learn_steps)
+            ctx.sync()
+            t1 = time.perf_counter()
+            hist = ctx.learned_train(args.learn_epochs, batch=16, lr=2e-3)
+            t2 = time.perf_counter()
+            ctx.set_precision("learned")
+            fwd_ms, gflop = ctx.learned_time(10)
+            learned = {"pairs": ng * 3 * args.learn_steps, "collect_s": round(t1 - t0, 2),
+                       "train_epochs": args.learn_epochs, "train_s": round(t2 - t1, 2),
+                       "loss_per_epoch": [round(float(x), 4) for x in hist],
+                       "network_forward_ms": round(fwd_ms, 3), "network_gflop": round(gflop, 1),
+                       "roofline_net": {"bound": "mfma", "achieved": round(gflop / fwd_ms, 1), "peak": 2500.0,
+                                        "unit": "TFLOP/s", "frac": round(gflop / fwd_ms / 2500.0, 4)}}
+            print(f"[bench] learned: trained {args.learn_epochs} epochs, forward {fwd_ms:.3f} ms", file=sys.stderr,
+                  flush=True)
         ctx.steps(dt, args.warmup)
         ctx.sync()
         print(f"[bench] warmup done, dt={dt:.3e}", file=sys.stderr, flush=True)
@@ -163,6 +192,8 @@ def main():
         el = time.perf_counter() - t0
         divtot, divmax = ctx.chkdiv()
         cg_iters, cg_relres = ctx.solver_stats()
+        if learned is not None:
+            learned["fallback_solves"] = ctx.learned_stats()[1]
         print(f"[bench] timed {args.steps} steps in {el:.3f} s, divmax={divmax:.2e}", file=sys.stderr, flush=True)
         # second timed pass: eager launches with HIP events around every
         # phase (on the stream the kernels run on) for the per-phase split
@@ -204,7 +235,7 @@ def main():
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "Mcell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f64+f32",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": {"fp64": "f64", "mixed": "f64+f32", "learned": "f64+f32+bf16"}[args.precision],
         "data": "synthetic (laminar Poiseuille + seeded +-10% uniform noise, seed 20251202)",
         "config": ({"workload": "plain FP64 FFT-accelerated periodic channel" +
                                 ("; implicit diffusion (three Helmholtz solves per substage)" if args.impdiff else ""),
@@ -219,9 +250,13 @@ def main():
                     "poisson": "DFT along z, batched MG-preconditioned CG per mode (zebra y-line relaxation, "
                                "x semi-coarsening, re-discretised coarse operators), relative residual 1e-13; the step graph runs a fixed CG count measured on the first step",
                     "cg_iterations_last_solve": cg_iters, "cg_max_relres_last_solve": cg_relres,
-                    "precision": ("FP64 throughout" if args.precision == "fp64" else
-                                  "FP32 V-cycle (smoothing, transfers, coarse solves) under flexible CG with "
-                                  "FP64 vectors, operator, dot products and null-space removal"),
+                    "precision": {"fp64": "FP64 throughout",
+                                  "mixed": "FP32 V-cycle (smoothing, transfers, coarse solves) under flexible CG "
+                                           "with FP64 vectors, operator, dot products and null-space removal",
+                                  "learned": "BF16 network preconditioner (FP32 accumulation) under flexible CG "
+                                             "with FP64 vectors, operator, dot products and null-space removal; "
+                                             "FP32 V-cycle fallback"}[args.precision],
+                    "learned": learned,
                     "parallelism": "1 GPU"}),
         "poisson_ms_per_step": round(poisson, 4),
         "phase_ms_per_step": phases,

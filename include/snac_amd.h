@@ -108,6 +108,9 @@ int snac_learned_collect(snac_ctx *ctx, int npairs);
 int snac_learned_train(snac_ctx *ctx, int epochs, int batch, float lr, unsigned seed, float *loss_per_epoch);
 int snac_learned_options(snac_ctx *ctx, int fallback, double stall_ratio);
 int snac_learned_stats(snac_ctx *ctx, int *pairs, int *fallbacks);
+/* average time of one network forward over every mode (HIP events on the
+ * context's stream, reps launches) and its convolution GFLOP             */
+int snac_learned_time(snac_ctx *ctx, int reps, double *ms, double *gflop);
 /* CG iterations of the last MG-PCG pressure solve and its largest relative
  * residual over the modes (HYPRE's StructPCGGetNumIterations /
  * GetFinalRelativeResidualNorm, solver.f90:553-569); synchronises.  Both

@@ -95,6 +95,7 @@ def _load():
     L.snac_learned_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_uint, fp]
     L.snac_learned_options.argtypes = [vp, C.c_int, C.c_double]
     L.snac_learned_stats.argtypes = [vp, ip, ip]
+    L.snac_learned_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.snac_net_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_uint, C.c_int]
     L.snac_net_destroy.argtypes = [vp]
     L.snac_net_num_params.argtypes = [vp]
@@ -327,6 +328,12 @@ class Context:
 
     def learned_options(self, fallback=True, stall_ratio=0.5):
         _check(lib.snac_learned_options(self.h, int(bool(fallback)), float(stall_ratio)), "learned_options")
+
+    def learned_time(self, reps=10):
+        """(ms of one network forward over every mode, its GFLOP)"""
+        a, b = C.c_double(), C.c_double()
+        _check(lib.snac_learned_time(self.h, int(reps), C.byref(a), C.byref(b)), "learned_time")
+        return a.value, b.value
 
     def learned_stats(self):
         a, b = np.zeros(1, np.int32), np.zeros(1, np.int32)

@@ -1314,6 +1314,33 @@ int snac_learned_options(snac_ctx *c, int fallback, double stall_ratio) {
   });
 }
 
+int snac_learned_time(snac_ctx *c, int reps, double *ms, double *gflop) {
+  return guard([&] {
+    on_device(c);
+    MGPlan &P = c->mg;
+    if (!c->mg_planned || !P.net) throw std::runtime_error("no trained network");
+    if (reps < 1) throw std::runtime_error("reps must be positive");
+    hipEvent_t e0, e1;
+    SNAC_HIP(hipEventCreate(&e0));
+    SNAC_HIP(hipEventCreate(&e1));
+    P.net->forward(c->stream, P.ng);  // warm-up
+    SNAC_HIP(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; r++) P.net->forward(c->stream, P.ng);
+    SNAC_HIP(hipEventRecord(e1, c->stream));
+    SNAC_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    SNAC_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms) *ms = t / reps;
+    // the convolutions' multiply-adds as the MFMA runs them (padded channels)
+    double f = 0.0;
+    for (int l = 0; l < NET_LAYERS; l++)
+      f += 2.0 * 9.0 * NET_CIN[l] * NET_COUT[l] * ((double)(P.nH >> NET_LEVEL[l]) * (P.nW >> NET_LEVEL[l]));
+    if (gflop) *gflop = f * P.ng / 1e9;
+  });
+}
+
 int snac_learned_stats(snac_ctx *c, int *pairs, int *fallbacks) {
   return guard([&] {
     checked(c);
