@@ -301,6 +301,9 @@ typedef struct snac_net snac_net;
 int  snac_net_create(snac_net **net, int H, int W, int maxbatch, unsigned seed, int device);
 int  snac_net_destroy(snac_net *net);
 long snac_net_num_params(const snac_net *net);
+/* convolution kernel of every network: 0 per-tap reads from memory, 1 the
+ * input staged in LDS as BF16 (default)                                   */
+int  snac_net_conv_variant(int variant);
 int  snac_net_get_params(const snac_net *net, float *params);
 int  snac_net_set_params(snac_net *net, const float *params);
 int  snac_net_forward(snac_net *net, int B, const float *x, float *y);

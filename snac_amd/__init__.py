@@ -98,6 +98,7 @@ def _load():
     L.snac_learned_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.snac_net_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_uint, C.c_int]
     L.snac_net_destroy.argtypes = [vp]
+    L.snac_net_conv_variant.argtypes = [C.c_int]
     L.snac_net_num_params.argtypes = [vp]
     L.snac_net_num_params.restype = C.c_long
     L.snac_net_get_params.argtypes = [vp, fp]
@@ -385,6 +386,11 @@ class Context:
 
 class Net:
     """The learned preconditioner's network on its own (snac_net_*)."""
+
+    @staticmethod
+    def conv_variant(v):
+        """0: per-tap reads from memory, 1: input staged in LDS (default)."""
+        _check(lib.snac_net_conv_variant(int(v)), "net_conv_variant")
 
     def __init__(self, H, W, maxbatch=8, seed=1, device=0):
         h = C.c_void_p()

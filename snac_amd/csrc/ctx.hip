@@ -1930,6 +1930,8 @@ int snac_net_destroy(snac_net *n) {
   return 0;
 }
 
+int snac_net_conv_variant(int variant) { return guard([&] { net_set_conv_variant(variant); }); }
+
 long snac_net_num_params(const snac_net *n) { return n && n->net ? (long)n->net->num_params() : -1; }
 
 int snac_net_get_params(const snac_net *n, float *params) {

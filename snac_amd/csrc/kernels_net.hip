@@ -80,6 +80,76 @@ __global__ __launch_bounds__(256) void k_conv(int H, int W, const float *__restr
   }
 }
 
+// The same convolution with the input staged in LDS: a 256-thread
+// workgroup computes 4 output rows x 64 pixels (one row per wavefront, four
+// 16-pixel MFMA tiles each); the 6 x 66 input pixels it reads (one-pixel
+// halo, zero outside the image) are converted to BF16 once into LDS, each
+// pixel padded by 8 BF16 so consecutive pixels start on different banks.
+// Every input pixel is then fetched from memory about 1.5 times instead of
+// once per tap.
+template <int CIN, int COUT, bool RELU>
+__global__ __launch_bounds__(256) void k_conv_lds(int H, int W, const float *__restrict__ in,
+                                                  const short *__restrict__ w, const float *__restrict__ bias,
+                                                  float *__restrict__ out) {
+  constexpr int TC = 66, TR = 6, PS = CIN + 8;  // tile columns, rows, pixel stride (BF16)
+  __shared__ short tile[TR * TC * PS];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 4, b = blockIdx.z;
+  const float *img = in + (size_t)b * H * W * CIN;
+  for (int q = threadIdx.x; q < TR * TC * (CIN / 4); q += 256) {
+    const int cq = q % (CIN / 4), pc = q / (CIN / 4);
+    const int c = pc % TC, r = pc / TC;
+    const int yy = y0 - 1 + r, xx = x0 - 1 + c;
+    bf16x4 v = bf16x4{0, 0, 0, 0};
+    if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+      const float4 f = *(const float4 *)(img + ((size_t)yy * W + xx) * CIN + cq * 4);
+      v = bf16x4{bf16_bits(f.x), bf16_bits(f.y), bf16_bits(f.z), bf16_bits(f.w)};
+    }
+    *(bf16x4 *)(tile + pc * PS + cq * 4) = v;
+  }
+  __syncthreads();
+  const int kq = (lane >> 4) * 4, y = y0 + wv;
+  f32x4 acc[4][COUT / 16];
+#pragma unroll
+  for (int s4 = 0; s4 < 4; s4++)
+#pragma unroll
+    for (int t = 0; t < COUT / 16; t++) acc[s4][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int tap = 0; tap < 9; tap++) {
+    const int r = wv + tap / 3, cs = (lane & 15) + tap % 3;  // tile row / column of this lane's pixel (sub-tile 0)
+#pragma unroll
+    for (int c0 = 0; c0 < CIN; c0 += 16) {
+      bf16x4 bv[COUT / 16];
+#pragma unroll
+      for (int t = 0; t < COUT / 16; t++)
+        bv[t] = *(const bf16x4 *)(w + ((size_t)tap * COUT + t * 16 + (lane & 15)) * CIN + c0 + kq);
+#pragma unroll
+      for (int s4 = 0; s4 < 4; s4++) {
+        const bf16x4 a = *(const bf16x4 *)(tile + (r * TC + cs + s4 * 16) * PS + c0 + kq);
+#pragma unroll
+        for (int t = 0; t < COUT / 16; t++)
+          acc[s4][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv[t], acc[s4][t], 0, 0, 0);
+      }
+    }
+  }
+  if (y >= H) return;
+#pragma unroll
+  for (int t = 0; t < COUT / 16; t++) {
+    const int co = t * 16 + (lane & 15);
+    const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int s4 = 0; s4 < 4; s4++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int x = x0 + s4 * 16 + kq + j;
+        if (x < W) {
+          float v = acc[s4][t][j] + bb;
+          if (RELU) v = fmaxf(v, 0.f);
+          out[(((size_t)b * H + y) * W + x) * COUT + co] = v;
+        }
+      }
+  }
+}
+
 // dw[tap][co][ci] += sum_p g[p][co] * in[p + off(tap)][ci] over pixels p of
 // all images; workgroup (pixel range, tap), wavefront = 16 output channels
 template <int CIN, int COUT>
@@ -333,9 +403,19 @@ __global__ void k_net_stall(int ng, const double *__restrict__ rr, double *__res
 
 unsigned grid_for(long n) { return (unsigned)std::min<long>((n + 255) / 256, 8192); }
 
+// 0: one wavefront per 16 pixels reading global memory per tap; 1: the
+// LDS-staged form (default)
+int g_conv_variant = 1;
+
 template <int CIN, int COUT>
 void conv_fwd(hipStream_t s, int B, int H, int W, const float *in, const short *w, const float *bias, float *out,
               bool relu) {
+  if (g_conv_variant == 1) {
+    dim3 grid((W + 63) / 64, (H + 3) / 4, B);
+    if (relu) hipLaunchKernelGGL((k_conv_lds<CIN, COUT, true>), grid, dim3(256), 0, s, H, W, in, w, bias, out);
+    else hipLaunchKernelGGL((k_conv_lds<CIN, COUT, false>), grid, dim3(256), 0, s, H, W, in, w, bias, out);
+    return;
+  }
   dim3 grid((W + 63) / 64, H, B);
   if (relu) hipLaunchKernelGGL((k_conv<CIN, COUT, true>), grid, dim3(256), 0, s, H, W, in, w, bias, out);
   else hipLaunchKernelGGL((k_conv<CIN, COUT, false>), grid, dim3(256), 0, s, H, W, in, w, bias, out);
@@ -406,6 +486,11 @@ void launch_net_stall(hipStream_t s, int ng, const double *rr, double *prev, con
                       int *stall) {
   hipLaunchKernelGGL(k_net_stall, dim3((ng + 255) / 256), dim3(256), 0, s, ng, rr, prev, done, ratio2, stall);
   SNAC_HIP(hipGetLastError());
+}
+
+void net_set_conv_variant(int v) {
+  if (v < 0 || v > 1) throw std::runtime_error("convolution variant must be 0 or 1");
+  g_conv_variant = v;
 }
 
 float bf16_round(float x) {

@@ -85,6 +85,8 @@ class Net {
 
 // bf16 of an FP32 value, rounded to nearest even (the MFMA operands)
 float bf16_round(float x);
+// convolution kernel: 0 per-tap global reads, 1 LDS-staged input (default)
+void net_set_conv_variant(int v);
 
 // ---- the MG-PCG planes <-> network planes (kernels_net.hip) ----
 // Group g of a plan vector is the na x ns plane (row a, column s, s

@@ -16,10 +16,12 @@ def rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
 
 
+@pytest.mark.parametrize("variant", [1, 0])
 @pytest.mark.parametrize("H,W,B", [(16, 8, 2), (24, 72, 3)])
-def test_forward_and_gradients_match_oracle(H, W, B):
+def test_forward_and_gradients_match_oracle(H, W, B, variant):
     import snac_amd
-    print(f"net check H={H} W={W} B={B}", flush=True)
+    print(f"net check H={H} W={W} B={B} conv variant {variant}", flush=True)
+    snac_amd.Net.conv_variant(variant)
     p, x, t = tn.random_problem(B=B, H=H, W=W, seed=H + W)
     net = snac_amd.Net(H, W, maxbatch=4)
     net.set_params(p)
@@ -32,6 +34,7 @@ def test_forward_and_gradients_match_oracle(H, W, B):
     assert abs(loss - lo) <= 1e-3 * lo
     assert rel(g, go) <= 1e-2, rel(g, go)
     net.close()
+    snac_amd.Net.conv_variant(1)
 
 
 def test_training_lowers_the_loss():
