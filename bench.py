@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", choices=["channel", "bfs"], default="channel")
     ap.add_argument("--size", type=int, nargs=3, default=[256, 128, 256])
+    ap.add_argument("--tol", type=float, default=1e-13,
+                    help="MG-PCG relative residual per mode (&hypre hypre_tol), bfs workload")
     ap.add_argument("--learn-steps", type=int, default=20)
     ap.add_argument("--learn-epochs", type=int, default=4)
     ap.add_argument("--precision", choices=["fp64", "mixed", "learned"], default="fp64",
@@ -77,9 +79,11 @@ def channel_setup(tmp, size):
     return d, b
 
 
-def bfs_setup(tmp):
+def bfs_setup(tmp, tol):
     import cases
     d, b = cases.write_case(tmp, cases.bfs_blocks(), 1000.0, cfl=0.5)
+    txt = open(d).read().replace("hypre_tol = 1.e-13", f"hypre_tol = {tol!r}")
+    open(d, "w").write(txt)
     return d, b
 
 
@@ -138,7 +142,7 @@ def main():
     bfs = args.workload == "bfs"
     with tempfile.TemporaryDirectory() as tmp:
         if bfs:
-            d, b = bfs_setup(tmp)
+            d, b = bfs_setup(tmp, args.tol)
         else:
             d, b = channel_setup(tmp, size)
         print(f"[bench] {args.workload} {'' if bfs else size} on device 0", file=sys.stderr, flush=True)
@@ -248,7 +252,8 @@ def main():
                     "stretching": "y tanh two-end per block, factor 1.2", "fft_axis": "z (128 modes)",
                     "Re": 1000, "inflow": "Poiseuille (inflow_type 1)", "outflow": "x+ velocity N, pressure D",
                     "poisson": "DFT along z, batched MG-preconditioned CG per mode (zebra y-line relaxation, "
-                               "x semi-coarsening, re-discretised coarse operators), relative residual 1e-13; the step graph runs a fixed CG count measured on the first step",
+                               f"x semi-coarsening, re-discretised coarse operators), relative residual {args.tol:g}; "
+                               "the step graph runs a fixed CG count measured on the first step",
                     "cg_iterations_last_solve": cg_iters, "cg_max_relres_last_solve": cg_relres,
                     "precision": {"fp64": "FP64 throughout",
                                   "mixed": "FP32 V-cycle (smoothing, transfers, coarse solves) under flexible CG "
