@@ -832,14 +832,15 @@ void rk_mom(snac_ctx *c, int irk, double dt) {
 }
 
 // rk_mom kernel variant (one cell per thread, a k-march with one of 6
-// chunk/tile choices, or an LDS-staged k-march with one of 4), timed on this case's own arrays at context
+// chunk/tile choices, or an LDS-staged k-march with one of 4, with or
+// without non-temporal hints), timed on this case's own arrays at context
 // creation (the arrays are still zero then; the buffers it writes are
 // cleared afterwards)
 void autotune_rk(snac_ctx *c) {
   if (c->device < 0) return;
   double best = 1e300;
   int bv = 0;
-  for (int v = 0; v <= 10; v++) {
+  for (int v = 0; v <= 14; v++) {
     c->rk_variant = v;
     const int cur0 = c->cur;
     double ms = time_stage(c, [&] {
@@ -1834,7 +1835,7 @@ int snac_set_kernel_variant(snac_ctx *c, int kernel, int variant) {
   return guard([&] {
     on_device(c);
     if (kernel == SNAC_KV_RK_MOM) {
-      if (variant < 0 || variant > 10) throw std::runtime_error("rk_mom variant must be 0..10");
+      if (variant < 0 || variant > 14) throw std::runtime_error("rk_mom variant must be 0..14");
       c->rk_variant = variant;
     } else if (kernel == SNAC_KV_TRIDIAG) {
       if (!c->tp.ok) throw std::runtime_error("no tensor-product plan on this case");

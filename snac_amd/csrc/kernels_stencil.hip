@@ -255,7 +255,20 @@ __global__ __launch_bounds__(BX *BY) void k_rk_mom_kz(Geo g, Metrics m, const do
 // computed and land in the ring slot of plane k-1 after it, so their
 // latency overlaps the arithmetic.
 constexpr int LX = 66;  // row pitch of an LDS plane: 64 + 2 halo cells
-template <bool BUOY, bool IMP, int KB, int BY>
+template <bool NT>
+__device__ __forceinline__ double ld_nt(const double *p) {
+  if (NT) return __builtin_nontemporal_load(p);
+  return *p;
+}
+template <bool NT>
+__device__ __forceinline__ void st_nt(double *p, double v) {
+  if (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+// NTS: the read-once / write-once streams (dudtrko in and out, the new
+// velocities) carry the non-temporal hint, leaving the caches to the
+// stencil's planes
+template <bool BUOY, bool IMP, int KB, int BY, bool NTS>
 __global__ __launch_bounds__(64 * BY) void k_rk_mom_lds(Geo g, Metrics m, const double *__restrict__ u,
                                                         const double *__restrict__ v, const double *__restrict__ w,
                                                         const double *__restrict__ p, double *__restrict__ un,
@@ -386,9 +399,9 @@ __global__ __launch_bounds__(64 * BY) void k_rk_mom_lds(Geo g, Metrics m, const 
         double wwkm = 0.25 * (wc * dxfi * dyfj + wm * dxfi * dyfj) * wm;
         cc = cc + ((-uwip + uwim) + (-vwjp + vwjm) + (-wwkp + wwkm)) * (ixfi * iyfj * izck);
       }
-      double un_ = uc + f1 * a + f2 * dtu[q];
-      double vn_ = vc + f1 * b + f2 * dtv[q];
-      double wn_ = wc + f1 * cc + f2 * dtw[q];
+      double un_ = uc + f1 * a + f2 * ld_nt<NTS>(dtu + q);
+      double vn_ = vc + f1 * b + f2 * ld_nt<NTS>(dtv + q);
+      double wn_ = wc + f1 * cc + f2 * ld_nt<NTS>(dtw + q);
       if (IMP) { un_ = un_ + f12 * ad; vn_ = vn_ + f12 * bd; wn_ = wn_ + f12 * cd; }  // rk.f90:75-79
       un_ = un_ + f12 * (-(P0[LXP] - pc) * ixci + bfx);
       vn_ = vn_ + f12 * (-(P0[LYP] - pc) * iycj + bfy);
@@ -399,17 +412,17 @@ __global__ __launch_bounds__(64 * BY) void k_rk_mom_lds(Geo g, Metrics m, const 
         vn_ = vn_ - f12 * gby * 0.5 * (s1[c + sx] + sc);
         wn_ = wn_ - f12 * gbz * 0.5 * (s1[c + sz] + sc);
       }
-      dtu[q] = a;
-      dtv[q] = b;
-      dtw[q] = cc;
+      st_nt<NTS>(dtu + q, a);
+      st_nt<NTS>(dtv + q, b);
+      st_nt<NTS>(dtw + q, cc);
       if (IMP) {  // the Helmholtz right-hand side (rk.f90:112-127)
         un_ = un_ - 0.5 * f12 * ad;
         vn_ = vn_ - 0.5 * f12 * bd;
         wn_ = wn_ - 0.5 * f12 * cd;
       }
-      un[c] = un_;
-      vn[c] = vn_;
-      wn[c] = wn_;
+      st_nt<NTS>(un + c, un_);
+      st_nt<NTS>(vn + c, vn_);
+      st_nt<NTS>(wn + c, wn_);
     }
     __syncthreads();  // plane k-1's slot is free
     if (pre) store_plane(sm_, nxt);
@@ -670,13 +683,21 @@ void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *
                    double f1, double f2, double f12, double visc, double bfx, double bfy, double bfz,
                    const double *s1, const double gb[3], int variant, bool impdiff) {
   if (variant >= 7) {
-    // 7..10: LDS-staged k-march; 7/8: 64x4 tiles, KB 16/32; 9/10: 64x8 tiles, KB 16/32
+    // 7..10: LDS-staged k-march; 7/8: 64x4 tiles, KB 16/32; 9/10: 64x8 tiles, KB 16/32;
+    // 11..14: the same with non-temporal hints on the streams
     // (capping the registers at 128 for a fourth wave per SIMD spills and measured slower)
-    const int vi = variant - 7, kb = vi % 2 ? 32 : 16, by = vi >= 2 ? 8 : 4;
+    const bool nt = variant >= 11;
+    const int vi = (variant - 7) % 4, kb = vi % 2 ? 32 : 16, by = vi >= 2 ? 8 : 4;
     dim3 grid((unsigned)(((g.nx + 63) / 64) * ((g.ny + by - 1) / by) * ((g.nz + kb - 1) / kb)));
 #define SNAC_KL(B, KB, Y)                                                                                           \
-  hipLaunchKernelGGL((k_rk_mom_lds<B, IM, KB, Y>), grid, dim3(64, Y), 0, s, g, m, u, v, w, p, un, vn, wn, dtu, dtv, dtw,  \
-                     f1, f2, f12, visc, bfx, bfy, bfz, s1, gb[0], gb[1], gb[2])
+  do {                                                                                                              \
+    if (nt)                                                                                                         \
+      hipLaunchKernelGGL((k_rk_mom_lds<B, IM, KB, Y, true>), grid, dim3(64, Y), 0, s, g, m, u, v, w, p, un, vn, wn,   \
+                         dtu, dtv, dtw, f1, f2, f12, visc, bfx, bfy, bfz, s1, gb[0], gb[1], gb[2]);                  \
+    else                                                                                                            \
+      hipLaunchKernelGGL((k_rk_mom_lds<B, IM, KB, Y, false>), grid, dim3(64, Y), 0, s, g, m, u, v, w, p, un, vn, wn,  \
+                         dtu, dtv, dtw, f1, f2, f12, visc, bfx, bfy, bfz, s1, gb[0], gb[1], gb[2]);                  \
+  } while (0)
 #define SNAC_KL_ALL(B)                                                                                              \
   do {                                                                                                              \
     if (vi == 0) SNAC_KL(B, 16, 4);                                                                                 \
