@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", choices=["channel", "bfs"], default="channel")
     ap.add_argument("--size", type=int, nargs=3, default=[256, 128, 256])
+    ap.add_argument("--solver", choices=["auto", "tensor", "mg"], default="auto",
+                    help="pressure solver: the channel's direct tensor-product solver (auto) or MG-PCG")
     ap.add_argument("--tol", type=float, default=1e-13,
                     help="MG-PCG relative residual per mode (&hypre hypre_tol), bfs workload")
     ap.add_argument("--learn-steps", type=int, default=20)
@@ -73,9 +75,11 @@ def parse():
     return ap.parse_args()
 
 
-def channel_setup(tmp, size):
+def channel_setup(tmp, size, tol):
     import cases
     d, b = cases.write_case(tmp, [cases.channel_block(size)], 5600.0, cfl=0.5, bforce=(12.0 / 5600.0, 0.0, 0.0))
+    txt = open(d).read().replace("hypre_tol = 1.e-13", f"hypre_tol = {tol!r}")
+    open(d, "w").write(txt)
     return d, b
 
 
@@ -144,9 +148,11 @@ def main():
         if bfs:
             d, b = bfs_setup(tmp, args.tol)
         else:
-            d, b = channel_setup(tmp, size)
+            d, b = channel_setup(tmp, size, args.tol)
         print(f"[bench] {args.workload} {'' if bfs else size} on device 0", file=sys.stderr, flush=True)
         ctx = snac_amd.Context(d, b, 3, device=0)
+        if args.solver != "auto":
+            ctx.set_solver(args.solver)
         if args.precision != "learned":
             ctx.set_precision(args.precision)
         if args.impdiff:
@@ -245,7 +251,9 @@ def main():
                                 ("; implicit diffusion (three Helmholtz solves per substage)" if args.impdiff else ""),
                     "grid": list(size), "stretching": "y tanh two-end, factor 1.8", "fft_axis": "z",
                     "Re_b": 5600, "forcing": "bforce(1)=12/5600",
-                    "poisson": "r2r along z and x, tridiagonal along y (exact)", "parallelism": "1 GPU",
+                    "poisson": ("r2r along z and x, tridiagonal along y (exact)" if args.solver != "mg" else
+                                f"DFT along z, batched MG-preconditioned CG per mode, relative residual {args.tol:g}"),
+                    "parallelism": "1 GPU",
                     "autotune": {"choices": tune_choices, "best_ms": tune_ms}} if not bfs else
                    {"workload": "plain FP64 backward-facing step, 3 blocks on one GPU",
                     "blocks": [[192, 64, 128], [320, 64, 128], [320, 96, 128]], "cells": cells,
