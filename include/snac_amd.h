@@ -102,11 +102,13 @@ int snac_set_precision(snac_ctx *ctx, int mode);
  *      loss_per_epoch may be NULL.  options: the stall fallback (on by
  *      default: an iteration that leaves some mode's residual norm above
  *      0.5 of the previous one hands the rest of the solve to the V-cycle;
- *      off: the network preconditions every iteration).  stats:
+ *      off: the network preconditions every iteration) and the initial
+ *      guess (off by default: on, each solve starts from the network's
+ *      answer for its right-hand side).  stats:
  *      recorded pairs and solves that fell back to the V-cycle. ---- */
 int snac_learned_collect(snac_ctx *ctx, int npairs);
 int snac_learned_train(snac_ctx *ctx, int epochs, int batch, float lr, unsigned seed, float *loss_per_epoch);
-int snac_learned_options(snac_ctx *ctx, int fallback, double stall_ratio);
+int snac_learned_options(snac_ctx *ctx, int fallback, double stall_ratio, int initial_guess);
 int snac_learned_stats(snac_ctx *ctx, int *pairs, int *fallbacks);
 /* average time of one network forward over every mode (HIP events on the
  * context's stream, reps launches) and its convolution GFLOP             */

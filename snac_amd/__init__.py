@@ -93,7 +93,7 @@ def _load():
     fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
     L.snac_learned_collect.argtypes = [vp, C.c_int]
     L.snac_learned_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_uint, fp]
-    L.snac_learned_options.argtypes = [vp, C.c_int, C.c_double]
+    L.snac_learned_options.argtypes = [vp, C.c_int, C.c_double, C.c_int]
     L.snac_learned_stats.argtypes = [vp, ip, ip]
     L.snac_learned_time.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.snac_net_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_uint, C.c_int]
@@ -327,8 +327,9 @@ class Context:
         _check(lib.snac_learned_train(self.h, int(epochs), int(batch), float(lr), int(seed), hist), "learned_train")
         return hist
 
-    def learned_options(self, fallback=True, stall_ratio=0.5):
-        _check(lib.snac_learned_options(self.h, int(bool(fallback)), float(stall_ratio)), "learned_options")
+    def learned_options(self, fallback=True, stall_ratio=0.5, initial_guess=False):
+        _check(lib.snac_learned_options(self.h, int(bool(fallback)), float(stall_ratio), int(bool(initial_guess))),
+               "learned_options")
 
     def learned_time(self, reps=10):
         """(ms of one network forward over every mode, its GFLOP)"""

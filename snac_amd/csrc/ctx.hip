@@ -1306,12 +1306,13 @@ int snac_learned_train(snac_ctx *c, int epochs, int batch, float lr, unsigned se
   });
 }
 
-int snac_learned_options(snac_ctx *c, int fallback, double stall_ratio) {
+int snac_learned_options(snac_ctx *c, int fallback, double stall_ratio, int initial_guess) {
   return guard([&] {
     checked(c);
     if (!(stall_ratio > 0.0 && stall_ratio <= 1.0)) throw std::runtime_error("stall ratio must be in (0, 1]");
     c->mg.fallback = fallback != 0;
     c->mg.stall_ratio = stall_ratio;
+    c->mg.guess = initial_guess != 0;
   });
 }
 

@@ -153,6 +153,7 @@ struct MGPlan {
   int nH = 0, nW = 0;             // network planes (na, ns rounded up to 8)
   double lam_ref = 1.0, nact_plane = 1.0;
   bool fallback = true;
+  bool guess = false;             // also start each solve from the network's answer for b
   double stall_ratio = 0.5;
   int fallbacks = 0;              // solves that fell back to the V-cycle
   int *stall = nullptr, *stall_host = nullptr;

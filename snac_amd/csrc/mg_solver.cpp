@@ -554,12 +554,27 @@ void mg_solve(MGPlan &P, hipStream_t st, int fixed) {
     launch_net_pair_in(st, P.ng, L0.na, L0.ns, P.nH, P.nW, P.r, P.bb, P.nact_plane, P.h64.lam, P.lam_ref,
                        P.pair_x + (size_t)P.pair_n * pl * 2);
   if (mixed) mg_to_float(st, n, P.r, P.rf);
+  const double *rr0 = P.bb;  // r.r of the residual the first preconditioning sees
+  if (net && P.guess) {
+    // initial guess x0 = the network's answer for b, then r = b - M x0 (the
+    // CG update with p = x0 and the step length rz / pq forced to 1)
+    launch_net_in(st, P.ng, L0.na, L0.ns, P.nH, P.nW, P.rf, P.bb, P.nact_plane, P.h64.lam, P.lam_ref, P.net->input());
+    P.net->forward(st, P.ng);
+    launch_net_out(st, P.ng, L0.na, L0.ns, P.nH, P.nW, P.net->output(), P.bb, P.nact_plane, L0.act, P.zf);
+    project(P, st, P.zf);
+    mg_cg_dir<float>(st, P.ng, P.gsize, P.rz[0], nullptr, nullptr, P.zf, P.p, P.done);
+    mg_apply_dot(st, L0, P.h64.lam, P.p, P.q, P.ng, P.gsize, P.part, P.nchunk, P.pq, P.done);
+    mg_cg_update_dot(st, P.ng, P.gsize, P.pq, P.pq, P.x, P.p, P.r, P.q, P.rf, nullptr, P.part, P.nchunk, P.rr,
+                     P.done);
+    mg_flags(st, P.ng, P.rr, P.bb, tol2, P.done, P.left);
+    rr0 = P.rr;
+  }
   if (net) {
     mg_izero(st, 1, P.stall);
-    SNAC_HIP(hipMemcpyAsync(P.rr_prev, P.bb, 2 * P.ng * sizeof(double), hipMemcpyDeviceToDevice, st));
+    SNAC_HIP(hipMemcpyAsync(P.rr_prev, rr0, 2 * P.ng * sizeof(double), hipMemcpyDeviceToDevice, st));
   }
   int cur = 0;
-  precondition(P, st, P.rz[cur], nullptr, net, P.bb);
+  precondition(P, st, P.rz[cur], nullptr, net, rr0);
   const int nit = fixed > 0 ? std::min(fixed, P.maxiter) : P.maxiter;
   int it = 0;
   for (it = 1; it <= nit; it++) {

@@ -3,9 +3,9 @@ in the reference): pairs recorded from the plain solver, training on the
 GPU, the network as the flexible-CG preconditioner judged by the FP64
 residual.
 
-* the learned solve (network, V-cycle fallback on stall) reaches the same
-  fields as the plain FP64 solve (relative L2 1e-10; both stop at
-  hypre_tol = 1e-13);
+* the learned solve (network, V-cycle fallback on stall; also with the
+  network's answer as the initial guess) reaches the same fields as the
+  plain FP64 solve (relative L2 1e-10; both stop at hypre_tol = 1e-13);
 * the trained network used alone (no fallback) against the same network
   untrained, both for at most hypre_maxiter iterations: training lowers
   the residual it reaches (or the iterations it needs) -- what training
@@ -69,12 +69,14 @@ def test_learned_preconditioner(tmp_path):
     res["mixed"] = one_step(ctx, snap, dt, "mixed")
     ctx.learned_options(fallback=True)
     res["learned"] = one_step(ctx, snap, dt, "learned")
+    ctx.learned_options(fallback=True, initial_guess=True)
+    res["learned+guess"] = one_step(ctx, snap, dt, "learned")
     ctx.learned_options(fallback=False)
     res["network only"] = one_step(ctx, snap, dt, "learned")
     for k, (it, rr, _) in res.items():
         print(f"{k:>13}: {it} CG iterations, max relative residual {rr:.2e}", flush=True)
     ref = res["fp64"][2]
-    for k in ("mixed", "learned"):
+    for k in ("mixed", "learned", "learned+guess"):
         assert res[k][1] <= 1e-13 * 1.0001
         for f in ("u", "v", "w"):
             g, o = res[k][2][f][1:-1, 1:-1, 1:-1], ref[f][1:-1, 1:-1, 1:-1]
