@@ -261,10 +261,11 @@ int snac_get_timing(snac_ctx *ctx, double *ms, int *count);
  * 64x4 or 32x8; 7..10 k-march with the planes staged in LDS, 64x4 tiles
  * over 16/32 planes (7, 8) or 64x8 tiles over 16/32 planes (9, 10);
  * 11..14 as 7..10 with non-temporal hints on the read-once / write-once
- * streams) and the best times in ms.  No
+ * streams; 15, 16 64x8 tiles over 64 planes without / with the hints)
+ * and the best times in ms.  No
  * counterpart in the reference (its FFTW plans use FFTW_ESTIMATE).          */
 int snac_get_autotune(snac_ctx *ctx, int *choices, double *ms, int n);
-/* overrides an autotuned kernel variant: SNAC_KV_RK_MOM (0..14, as above),
+/* overrides an autotuned kernel variant: SNAC_KV_RK_MOM (0..16, as above),
  * SNAC_KV_TRIDIAG (0: c' in LDS, lines up to 128; 1: c' in scratch; 2:
  * chunked, no scratch, lines of 8..256) or SNAC_KV_GHOST (0: ghost fills as
  * one launch per phase of the reference's fill order; 1, the default: each

@@ -366,7 +366,7 @@ class Context:
         return ch[:k].tolist(), [round(float(x), 4) for x in ms[:k] if x > 0]
 
     def set_rk_variant(self, v):
-        """Overrides the autotuned rk_mom kernel variant (0..14)."""
+        """Overrides the autotuned rk_mom kernel variant (0..16)."""
         _check(lib.snac_set_kernel_variant(self.h, 0, int(v)), "set_kernel_variant")
 
     def set_tridiag_variant(self, v):

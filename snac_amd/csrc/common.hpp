@@ -75,7 +75,8 @@ void launch_cell_segs(hipStream_t s, const CellSeg *dev_segs, int nseg);
 // variant 0: one cell per thread; 1..6: k-march over chunks of 8/16/32
 // planes with 64x4 or 32x8 tiles; 7..10: k-march with LDS-staged planes,
 // 64x4 / 64x8 tiles over 16 / 32 planes; 11..14: 7..10 with non-temporal
-// hints on the read-once / write-once streams (autotuned).  impdiff: the diffusion
+// hints on the read-once / write-once streams; 15, 16: 64x8 tiles over 64
+// planes without / with the hints (autotuned).  impdiff: the diffusion
 // is kept out of dudtrk and half of it stays in the update (the reference's
 // _IMPDIFF build, rk.f90:29-127); the Helmholtz solves add the other half.
 void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v,

@@ -840,7 +840,7 @@ void autotune_rk(snac_ctx *c) {
   if (c->device < 0) return;
   double best = 1e300;
   int bv = 0;
-  for (int v = 0; v <= 14; v++) {
+  for (int v = 0; v <= 16; v++) {
     c->rk_variant = v;
     const int cur0 = c->cur;
     double ms = time_stage(c, [&] {
@@ -1836,7 +1836,7 @@ int snac_set_kernel_variant(snac_ctx *c, int kernel, int variant) {
   return guard([&] {
     on_device(c);
     if (kernel == SNAC_KV_RK_MOM) {
-      if (variant < 0 || variant > 14) throw std::runtime_error("rk_mom variant must be 0..14");
+      if (variant < 0 || variant > 16) throw std::runtime_error("rk_mom variant must be 0..16");
       c->rk_variant = variant;
     } else if (kernel == SNAC_KV_TRIDIAG) {
       if (!c->tp.ok) throw std::runtime_error("no tensor-product plan on this case");

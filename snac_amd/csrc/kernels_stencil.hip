@@ -684,10 +684,12 @@ void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *
                    const double *s1, const double gb[3], int variant, bool impdiff) {
   if (variant >= 7) {
     // 7..10: LDS-staged k-march; 7/8: 64x4 tiles, KB 16/32; 9/10: 64x8 tiles, KB 16/32;
-    // 11..14: the same with non-temporal hints on the streams
+    // 11..14: the same with non-temporal hints on the streams;
     // (capping the registers at 128 for a fourth wave per SIMD spills and measured slower)
-    const bool nt = variant >= 11;
-    const int vi = (variant - 7) % 4, kb = vi % 2 ? 32 : 16, by = vi >= 2 ? 8 : 4;
+    // 15, 16: 64x8 tiles over 64 planes, without / with the hints
+    const bool nt = variant >= 11 && variant != 15;
+    const int vi = variant >= 15 ? 4 : (variant - 7) % 4;
+    const int kb = vi == 4 ? 64 : (vi % 2 ? 32 : 16), by = vi >= 2 ? 8 : 4;
     dim3 grid((unsigned)(((g.nx + 63) / 64) * ((g.ny + by - 1) / by) * ((g.nz + kb - 1) / kb)));
 #define SNAC_KL(B, KB, Y)                                                                                           \
   do {                                                                                                              \
@@ -703,7 +705,8 @@ void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *
     if (vi == 0) SNAC_KL(B, 16, 4);                                                                                 \
     else if (vi == 1) SNAC_KL(B, 32, 4);                                                                            \
     else if (vi == 2) SNAC_KL(B, 16, 8);                                                                            \
-    else SNAC_KL(B, 32, 8);                                                                                         \
+    else if (vi == 3) SNAC_KL(B, 32, 8);                                                                            \
+    else SNAC_KL(B, 64, 8);                                                                                         \
   } while (0)
     if (impdiff) {
       constexpr bool IM = true;

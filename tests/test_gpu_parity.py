@@ -107,7 +107,7 @@ def test_rfft_lengths(tmp_path, kind, n):
 
 # ---------------------------------------------------------------- routines
 @pytest.mark.parametrize("n", [(32, 24, 16), (70, 13, 37)])
-@pytest.mark.parametrize("variant", range(15))
+@pytest.mark.parametrize("variant", range(17))
 def test_rk_mom(tmp_path, variant, n):
     """Every kernel variant, on a grid inside one tile and on a ragged one
     that leaves partial tiles along i, j and k."""
