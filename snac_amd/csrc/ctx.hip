@@ -1286,7 +1286,8 @@ int snac_learned_train(snac_ctx *c, int epochs, int batch, float lr, unsigned se
     P.collecting = false;
     if (!P.net) P.net = new Net(P.nH, P.nW, std::max(batch, P.ng), seed);
     if (batch > P.net->maxbatch()) throw std::runtime_error("learned_train: batch above the network's");
-    float *loss = dalloc<float>(c, 1);
+    if (!P.train_loss) P.train_loss = dalloc<float>(c, 1);
+    float *loss = P.train_loss;
     const size_t pl = (size_t)P.nH * P.nW;
     for (int e = 0; e < epochs; e++) {
       double sum = 0.0;

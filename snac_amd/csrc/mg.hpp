@@ -159,7 +159,7 @@ struct MGPlan {
   int *stall = nullptr, *stall_host = nullptr;
   double *rr_prev = nullptr;
   // training pairs recorded from the solves (network planes, FP32)
-  float *pair_x = nullptr, *pair_t = nullptr;
+  float *pair_x = nullptr, *pair_t = nullptr, *train_loss = nullptr;
   int pair_cap = 0, pair_n = 0;
   bool collecting = false;
   double *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
