@@ -70,7 +70,9 @@ void launch_cell_segs(hipStream_t s, const CellSeg *dev_segs, int nseg);
 
 // ---- stencil kernels (kernels_stencil.hip) ----
 // rk_mom: reads u,v,w,p (ghosted) and dudtrko; writes the updated interior
-// velocities to un,vn,wn and dudtrk to dto.
+// velocities to un,vn,wn and dudtrk to dto.  The three tendencies share one
+// array of 3 nx ny nz doubles laid out [k][j][component][i]: dtv = dtu + nx,
+// dtw = dtu + 2 nx, rows 3 nx apart.
 // s1 (scalar 1, ghosted) non-null adds the Boussinesq term with gb = gacc*beta;
 // variant 0: one cell per thread; 1..6: k-march over chunks of 8/16/32
 // planes with 64x4 or 32x8 tiles; 7..10: k-march with LDS-staged planes,

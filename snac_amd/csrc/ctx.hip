@@ -854,7 +854,7 @@ void autotune_rk(snac_ctx *c) {
   for (auto &D : c->blk) {
     for (int par = 0; par < 2; par++)
       for (int q = 0; q < 3; q++) SNAC_HIP(hipMemsetAsync(D.vel[par][q] - 15, 0, (D.ntot + 16) * sizeof(double), c->stream));
-    for (int q = 0; q < 3; q++) SNAC_HIP(hipMemsetAsync(D.dto[q], 0, D.nin * sizeof(double), c->stream));
+    SNAC_HIP(hipMemsetAsync(D.dto[0], 0, 3 * D.nin * sizeof(double), c->stream));
   }
   SNAC_HIP(hipStreamSynchronize(c->stream));
 }
@@ -1074,7 +1074,9 @@ void create(snac_ctx *c) {
     D.p = dalloc_field(c, D.ntot);
     D.pp = dalloc_field(c, D.ntot);
     D.po = dalloc_field(c, D.ntot);
-    for (int q = 0; q < 3; q++) D.dto[q] = dalloc<double>(c, D.nin);
+    // dudtrko, dvdtrko, dwdtrko row by row in one array ([k][j][component][i])
+    D.dto[0] = dalloc<double>(c, 3 * D.nin);
+    for (int q = 1; q < 3; q++) D.dto[q] = D.dto[0] + q * (long)D.g.nx;
     for (int is = 0; is < cs.nscal; is++) {
       for (int par = 0; par < 2; par++) D.sv[par].push_back(dalloc_field(c, D.ntot));
       D.sdo.push_back(dalloc<double>(c, D.nin));
@@ -1376,7 +1378,12 @@ void copy_field(snac_ctx *c, int b, int fid, double *dst_host, const double *src
   double *d = field_ptr(c, b, fid, n);
   DevBlock &D = c->blk.at(b);
   const bool ghosted = (fid >= 0 && fid < 6) || (fid >= SNAC_SCAL0 && (fid - SNAC_SCAL0) % 2 == 0);
-  if (ghosted) {
+  if (fid >= SNAC_DUDTRKO && fid <= SNAC_DWDTRKO) {
+    // one component's rows, 3 nx apart on the device, contiguous on the host
+    size_t w = (size_t)D.g.nx * sizeof(double), pitch = 3 * w, rows = (size_t)D.g.ny * D.g.nz;
+    if (src_host) SNAC_HIP(hipMemcpy2DAsync(d, pitch, src_host, w, w, rows, hipMemcpyHostToDevice, c->stream));
+    else SNAC_HIP(hipMemcpy2DAsync(dst_host, w, d, pitch, w, rows, hipMemcpyDeviceToHost, c->stream));
+  } else if (ghosted) {
     size_t w = (size_t)(D.g.nx + 2) * sizeof(double), pitch = (size_t)D.g.sx * sizeof(double);
     size_t rows = (size_t)(D.g.ny + 2) * (D.g.nz + 2);
     if (src_host) SNAC_HIP(hipMemcpy2DAsync(d, pitch, src_host, w, w, rows, hipMemcpyHostToDevice, c->stream));

@@ -28,6 +28,14 @@ __device__ __forceinline__ void tile_of(int gx, int gy, int &bx, int &by, int &b
   bz = t / gy;
 }
 
+// The old tendencies dudtrko, dvdtrko, dwdtrko share one array, row by row:
+// [k][j][component][i], so the three streams of a tile row sit side by side
+// in HBM (dtv = dtu + nx, dtw = dtu + 2 nx); this is the index of interior
+// cell (i, j, k) (1-based) in component 0.
+__device__ __forceinline__ long dto_index(const Geo &g, int i, int j, int k) {
+  return (long)(i - 1) + 3L * g.nx * ((long)(j - 1) + (long)g.ny * (long)(k - 1));
+}
+
 // BUOY: Boussinesq buoyancy of scalar 1, -f12*gacc*beta*(face average of s)
 // (rk.f90:104-108), gb = gacc*beta
 template <bool BUOY, bool IMP>
@@ -100,7 +108,7 @@ __global__ __launch_bounds__(TX *TY) void k_rk_mom(Geo g, Metrics m, const doubl
     cc = cc + ((-uwip + uwim) + (-vwjp + vwjm) + (-wwkp + wwkm)) * (ixfi * iyfj * izck);
   }
   // RK3 update (rk.f90:68-85), then pressure gradient + body force (86-111)
-  const long q = (long)(i - 1) + (long)g.nx * ((long)(j - 1) + (long)g.ny * (long)(k - 1));
+  const long q = dto_index(g, i, j, k);
   double un_ = uc + f1 * a + f2 * dtu[q];
   double vn_ = vc + f1 * b + f2 * dtv[q];
   double wn_ = wc + f1 * cc + f2 * dtw[q];
@@ -163,8 +171,8 @@ __global__ __launch_bounds__(BX *BY) void k_rk_mom_kz(Geo g, Metrics m, const do
   long c = gidx(i, j, k0, sx, sy);
   double um = u[c - sz], vm = v[c - sz], wm = w[c - sz];
   double uc = u[c], vc = v[c], wc = w[c], pc = p[c];
-  long q = (long)(i - 1) + (long)g.nx * ((long)(j - 1) + (long)g.ny * (long)(k0 - 1));
-  const long qs = (long)g.nx * g.ny;
+  long q = dto_index(g, i, j, k0);
+  const long qs = 3L * g.nx * g.ny;
   for (int k = k0; k <= k1; k++, c += sz, q += qs) {
     const long xp = c + 1, xm = c - 1, yp = c + sx, ym = c - sx, zp = c + sz, zm = c - sz;
     const double up = u[zp], vp = v[zp], wp = w[zp], pz = p[zp];
@@ -338,8 +346,8 @@ __global__ __launch_bounds__(64 * BY) void k_rk_mom_lds(Geo g, Metrics m, const 
   const int L = (ty + 1) * LX + tx + 1;  // this cell in an LDS plane
   const int LXP = L + 1, LXM = L - 1, LYP = L + LX, LYM = L - LX;
   long c = gidx(i, j, k0, sx, sy);
-  long q = (long)(i - 1) + (long)g.nx * ((long)(j - 1) + (long)g.ny * (long)(k0 - 1));
-  const long qs = (long)g.nx * g.ny;
+  long q = dto_index(g, i, j, k0);
+  const long qs = 3L * g.nx * g.ny;
   for (int k = k0; k <= k1; k++, c += sz, q += qs) {
     double nxt[4][NL];
     const bool pre = k + 2 <= k1 + 1;
