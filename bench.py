@@ -229,10 +229,13 @@ def main():
         if not bfs and os.path.exists(TRAFFIC_FILE):
             tr = json.load(open(TRAFFIC_FILE))
             if tr.get("grid") == list(size):
-                traffic = tr.get("bytes_per_launch")
+                # HBM bytes per launch of the variant the autotuner chose in
+                # this run (the last autotune choice is the rk_mom variant)
+                traffic = tr.get("variants", {}).get(str(tune_choices[-1]), {}).get("bytes_per_launch")
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                    "kernel": "rk_mom (fused momentum RHS + RK3 update, autotuned variant; 104 B/cell)", "avg_launch_ms": round(rk_avg_s * 1e3, 4),
+                    "kernel": "rk_mom (fused momentum RHS + RK3 update, autotuned variant; 104 B/cell)",
+                    "rk_variant": int(tune_choices[-1]) if not bfs else None, "avg_launch_ms": round(rk_avg_s * 1e3, 4),
                     "stream_copy_GBs": round(copy_gbs, 1)}
 
         cpu = None

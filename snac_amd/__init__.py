@@ -358,7 +358,8 @@ class Context:
 
     def autotune(self):
         """(choices, best ms) of the autotuner: transform tile per call site
-        (forward, then inverse), then the tridiagonal variant."""
+        (forward, then inverse), then the tridiagonal variant (both on the
+        tensor-product solver path only), then the rk_mom variant."""
         ch = np.zeros(16, np.int32)
         ms = np.zeros(16)
         _check(lib.snac_get_autotune(self.h, ch, ms, 16), "get_autotune")
