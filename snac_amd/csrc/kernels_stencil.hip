@@ -510,7 +510,7 @@ __global__ __launch_bounds__(TX *TY) void k_fillps(Geo g, Metrics m, int fa, con
   if (ry1 && j == g.ny) r += ry1[(i - 1) + (long)g.nx * (k - 1)];
   if (rz0 && k == 1) r += rz0[(i - 1) + (long)g.nx * (j - 1)];
   if (rz1 && k == g.nz) r += rz1[(i - 1) + (long)g.nx * (j - 1)];
-  pp[c] = r;
+  st_nt<true>(pp + c, r);  // written once, read by the next kernel
 }
 
 // correc over the whole ghosted box; cells outside the reference's loop
