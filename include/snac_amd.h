@@ -283,7 +283,9 @@ int snac_set_kernel_variant(snac_ctx *ctx, int kernel, int variant);
  * (the layout of a transform along a slow axis).  tile = FFT tile choice,
  * as the autotuner's: 0..2 (2^-tile of the most lines per workgroup); 3, 4
  * (R2HC/HC2R of n >= 8 only, the others treat them as 0): full tiles, 2 or 4
- * per workgroup with the next tile's loads overlapping the arithmetic.      */
+ * per workgroup with the next tile's loads overlapping the arithmetic; 5
+ * (R2HC/HC2R of n >= 8 only): full tiles, each wavefront transforming its
+ * own quarter of the lines with no workgroup barrier between the passes.   */
 int snac_r2r_lines(snac_ctx *ctx, int kind, int n, int howmany, int strided, int tile, const double *in,
                    double *out);
 /* STREAM-like copy of n doubles (read n*8 + write n*8 bytes per repetition)

@@ -255,6 +255,75 @@ constexpr int rfft_lines(int n1, int n2) {
                                                                     : (LDS_COMPLEX / (2 * n1 * n2));
 }
 
+// The two passes of the h-point complex FFT of one line (h = N1*N2) and
+// its unfolding.  The line's LDS slot holds the n = 2h raw doubles (r) and,
+// after pass 1, its h complex values (T, XOR-swizzled between the passes,
+// then Z in natural order).
+// Pass 1: N1-point DFT over j1 of column j2, the inverse's folding of the
+// halfcomplex input on the way in, the twiddles W_h^{j2 k1} on the way out.
+template <int N1, int N2, bool INV>
+__device__ __forceinline__ void rfft_pass1(const double *r, int j2, const double2 *tw, double2 (&v)[N1]) {
+  constexpr int h = N1 * N2, n = 2 * h;
+#pragma unroll
+  for (int j1 = 0; j1 < N1; j1++) {
+    const int k = N2 * j1 + j2;
+    if (!INV) {
+      v[j1] = make_double2(r[2 * k], r[2 * k + 1]);
+    } else {
+      // X[k] and conj X[h-k] from the halfcomplex line
+      const double2 xk = (k == 0) ? make_double2(r[0], 0.0) : make_double2(r[k], r[n - k]);
+      const double2 xc = (k == 0) ? make_double2(r[h], 0.0) : make_double2(r[h - k], -r[h + k]);
+      const double2 w = tw[k];  // conj of w^k
+      const double dx = xk.x - xc.x, dy = xk.y - xc.y;
+      const double bx = dx * w.x + dy * w.y, by = dy * w.x - dx * w.y;  // (dx + i dy) e^{+2 pi i k/n}
+      v[j1] = make_double2(xk.x + xc.x - by, xk.y + xc.y + bx);        // A + i B
+    }
+  }
+  fft_reg<N1>(v, tw, 2 * N2, INV);
+#pragma unroll
+  for (int k1 = 1; k1 < N1; k1++) {
+    double2 w = tw[(2 * j2 * k1) & (n - 1)];
+    if (INV) w.y = -w.y;
+    const double2 y = v[k1];
+    v[k1] = make_double2(y.x * w.x - y.y * w.y, y.x * w.y + y.y * w.x);
+  }
+}
+template <int N1, int N2>
+__device__ __forceinline__ void rfft_put1(double2 *T, int j2, const double2 (&v)[N1]) {
+#pragma unroll
+  for (int k1 = 0; k1 < N1; k1++) T[k1 * N2 + (j2 ^ (k1 & (N2 - 1)))] = v[k1];
+}
+// Pass 2: N2-point DFT over j2 of row k1
+template <int N1, int N2, bool INV>
+__device__ __forceinline__ void rfft_pass2(const double2 *T, int k1, const double2 *tw, double2 (&v)[N2]) {
+#pragma unroll
+  for (int j2 = 0; j2 < N2; j2++) v[j2] = T[k1 * N2 + (j2 ^ (k1 & (N2 - 1)))];
+  fft_reg<N2>(v, tw, 2 * N1, INV);
+}
+template <int N1, int N2>
+__device__ __forceinline__ void rfft_put2(double2 *Z, int k1, const double2 (&v)[N2]) {
+#pragma unroll
+  for (int k2 = 0; k2 < N2; k2++) Z[k1 + N1 * k2] = v[k2];
+}
+// Output q of the line from its h-point spectrum Z: forward, the halfcomplex
+// X[q] (q <= h: real part, else the imaginary part of X[n-q]); inverse, the
+// real sample y[q]
+template <int N1, int N2, bool INV>
+__device__ __forceinline__ double rfft_out(const double2 *Z, int q, const double2 *tw) {
+  constexpr int h = N1 * N2, n = 2 * h;
+  if (!INV) {
+    const int k = q <= h ? q : n - q;
+    const double2 zk = Z[k & (h - 1)], zc = Z[(h - k) & (h - 1)];  // Z[k], Z[h-k] (mod h)
+    const double ex = 0.5 * (zk.x + zc.x), ey = 0.5 * (zk.y - zc.y);
+    const double ox = 0.5 * (zk.y + zc.y), oy = -0.5 * (zk.x - zc.x);
+    const double2 w = tw[k];
+    const double xr = ex + (w.x * ox - w.y * oy), xi = ey + (w.x * oy + w.y * ox);
+    return q <= h ? xr : xi;
+  }
+  const double2 z = Z[q >> 1];
+  return (q & 1) ? z.y : z.x;
+}
+
 // TPB tiles of L lines per workgroup: the loads of tile t+1 are issued
 // before tile t is transformed and land in LDS after it, so their latency
 // overlaps the arithmetic (TPB = 1: one tile, no overlap).
@@ -317,38 +386,9 @@ __global__ __launch_bounds__(NT) void k_rfft(Args a) {
     const bool act = t < L * N2;
     const int j2 = t % N2, l = t / N2;
     double2 v[N1];
-    if (act) {
-      const double *r = R + LS * l;
-#pragma unroll
-      for (int j1 = 0; j1 < N1; j1++) {
-        const int k = N2 * j1 + j2;
-        if (!INV) {
-          v[j1] = make_double2(r[2 * k], r[2 * k + 1]);
-        } else {
-          // X[k] and conj X[h-k] from the halfcomplex line
-          const double2 xk = (k == 0) ? make_double2(r[0], 0.0) : make_double2(r[k], r[n - k]);
-          const double2 xc = (k == 0) ? make_double2(r[h], 0.0) : make_double2(r[h - k], -r[h + k]);
-          const double2 w = tw[k];  // conj of w^k
-          const double dx = xk.x - xc.x, dy = xk.y - xc.y;
-          const double bx = dx * w.x + dy * w.y, by = dy * w.x - dx * w.y;  // (dx + i dy) e^{+2 pi i k/n}
-          v[j1] = make_double2(xk.x + xc.x - by, xk.y + xc.y + bx);        // A + i B
-        }
-      }
-      fft_reg<N1>(v, tw, 2 * N2, INV);
-#pragma unroll
-      for (int k1 = 1; k1 < N1; k1++) {
-        double2 w = tw[(2 * j2 * k1) & (n - 1)];
-        if (INV) w.y = -w.y;
-        double2 y = v[k1];
-        v[k1] = make_double2(y.x * w.x - y.y * w.y, y.x * w.y + y.y * w.x);
-      }
-    }
+    if (act) rfft_pass1<N1, N2, INV>(R + LS * l, j2, tw, v);
     __syncthreads();
-    if (act) {
-      double2 *T = S + (LS / 2) * l;
-#pragma unroll
-      for (int k1 = 0; k1 < N1; k1++) T[k1 * N2 + (j2 ^ (k1 & (N2 - 1)))] = v[k1];
-    }
+    if (act) rfft_put1<N1, N2>(S + (LS / 2) * l, j2, v);
     __syncthreads();
   }
   // 3. second pass: N2-point DFTs over j2 for every (line, k1)
@@ -357,18 +397,9 @@ __global__ __launch_bounds__(NT) void k_rfft(Args a) {
     const bool act = t < L * N1;
     const int k1 = t % N1, l = t / N1;
     double2 v[N2];
-    if (act) {
-      const double2 *T = S + (LS / 2) * l;
-#pragma unroll
-      for (int j2 = 0; j2 < N2; j2++) v[j2] = T[k1 * N2 + (j2 ^ (k1 & (N2 - 1)))];
-      fft_reg<N2>(v, tw, 2 * N1, INV);
-    }
+    if (act) rfft_pass2<N1, N2, INV>(S + (LS / 2) * l, k1, tw, v);
     __syncthreads();
-    if (act) {
-      double2 *Z = S + (LS / 2) * l;
-#pragma unroll
-      for (int k2 = 0; k2 < N2; k2++) Z[k1 + N1 * k2] = v[k2];
-    }
+    if (act) rfft_put2<N1, N2>(S + (LS / 2) * l, k1, v);
     __syncthreads();
   }
   // 4. post-mapping and store
@@ -376,24 +407,111 @@ __global__ __launch_bounds__(NT) void k_rfft(Args a) {
   for (int it = 0; it < IT; it++) {
     const int e = threadIdx.x + it * NT;
     const int l = a.coal ? e % L : e / n, q = a.coal ? e / L : e % n;
-    if (e < L * n && oout[sl][l] >= 0) {
-      const double2 *Z = S + (LS / 2) * l;
-      double y;
-      if (!INV) {
-        const int k = q <= h ? q : n - q;
-        const double2 zk = Z[k & (h - 1)], zc = Z[(h - k) & (h - 1)];  // Z[k], Z[h-k] (mod h)
-        const double ex = 0.5 * (zk.x + zc.x), ey = 0.5 * (zk.y - zc.y);
-        const double ox = 0.5 * (zk.y + zc.y), oy = -0.5 * (zk.x - zc.x);
-        const double2 w = tw[k];
-        const double xr = ex + (w.x * ox - w.y * oy), xi = ey + (w.x * oy + w.y * ox);
-        y = q <= h ? xr : xi;
-      } else {
-        const double2 z = Z[q >> 1];
-        y = (q & 1) ? z.y : z.x;
-      }
-      a.out[q * a.oes + oout[sl][l]] = y * a.scale;
+    if (e < L * n && oout[sl][l] >= 0)
+      a.out[q * a.oes + oout[sl][l]] = rfft_out<N1, N2, INV>(S + (LS / 2) * l, q, tw) * a.scale;
+  }
+  }
+}
+
+// Orders this wavefront's LDS accesses before and after it: a wavefront's
+// LDS instructions execute in issue order, so lanes of one wavefront see each
+// other's writes without a workgroup barrier once the compiler keeps the
+// order (the fences at wavefront scope and the scheduling barrier).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The same transform with each wavefront owning LW = L/4 of the workgroup's
+// L lines through both passes, so the passes need no workgroup barrier.
+// Contiguous lines (stride 1) are also loaded and stored by their own
+// wavefront (one barrier in all, for the line offsets); strided lines are
+// loaded and stored by the whole workgroup, 8L contiguous bytes per row,
+// with a barrier after the load and one before the store.
+template <int N1, int N2, bool INV>
+__global__ __launch_bounds__(NT) void k_rfft_w(Args a) {
+  extern __shared__ double2 S[];
+  constexpr int h = N1 * N2, n = 2 * h;
+  constexpr int L = rfft_lines(N1, N2), LW = L / 4;
+  static_assert(NT == 256 && L % 4 == 0 && LW * N1 <= 64 && LW * N2 <= 64 && (L * n) % NT == 0,
+                "one pass task per lane of the owning wavefront");
+  constexpr int IT = L * n / NT;  // = LW * n / 64
+  const double2 *tw = reinterpret_cast<const double2 *>(a.tab);
+  constexpr int LS = n + RPAD;
+  double *R = reinterpret_cast<double *>(S);
+  __shared__ long oin[L], oout[L];
+  const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  for (int t = threadIdx.x; t < L; t += NT) {
+    const int line = blockIdx.x * L + t;
+    long oi = -1, oo = -1;
+    if (line < a.nlines) {
+      const int b1 = line % a.nb1, b2 = line / a.nb1;
+      oi = b1 * a.is1 + b2 * a.is2;
+      oo = b1 * a.os1 + b2 * a.os2;
+    }
+    oin[t] = oi;
+    oout[t] = oo;
+  }
+  __syncthreads();
+  const bool coal = a.coal;
+  // element it of this thread: line l, position q
+  auto elem = [&](int it, int &l, int &q) {
+    if (coal) {
+      const int e = threadIdx.x + it * NT;
+      l = e % L;
+      q = e / L;
+    } else {
+      const int e = ln + it * 64;
+      l = wv * LW + e / n;
+      q = e % n;
+    }
+  };
+  // 1. raw lines into LDS
+  {
+    double xr[IT];
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+      int l, q;
+      elem(it, l, q);
+      xr[it] = oin[l] >= 0 ? a.in[q * a.ies + oin[l]] : 0.0;
+    }
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+      int l, q;
+      elem(it, l, q);
+      R[LS * l + q] = xr[it];
     }
   }
+  if (coal) __syncthreads();
+  else wave_sync();
+  // 2. first pass over this wavefront's lines
+  {
+    const bool act = ln < LW * N2;
+    const int j2 = ln % N2, l = wv * LW + ln / N2;
+    double2 v[N1];
+    if (act) rfft_pass1<N1, N2, INV>(R + LS * l, j2, tw, v);
+    wave_sync();
+    if (act) rfft_put1<N1, N2>(S + (LS / 2) * l, j2, v);
+    wave_sync();
+  }
+  // 3. second pass
+  {
+    const bool act = ln < LW * N1;
+    const int k1 = ln % N1, l = wv * LW + ln / N1;
+    double2 v[N2];
+    if (act) rfft_pass2<N1, N2, INV>(S + (LS / 2) * l, k1, tw, v);
+    wave_sync();
+    if (act) rfft_put2<N1, N2>(S + (LS / 2) * l, k1, v);
+    wave_sync();
+  }
+  if (coal) __syncthreads();
+  // 4. post-mapping and store
+#pragma unroll
+  for (int it = 0; it < IT; it++) {
+    int l, q;
+    elem(it, l, q);
+    if (oout[l] >= 0) a.out[q * a.oes + oout[l]] = rfft_out<N1, N2, INV>(S + (LS / 2) * l, q, tw) * a.scale;
   }
 }
 
@@ -470,7 +588,8 @@ void launch_fft_kind(int kind, int n, int ls, dim3 grid, size_t lds, hipStream_t
   }
 }
 
-// tile 0..2: 2^-tile of the lines per workgroup; 3 / 4: full tiles, 2 / 4 per workgroup
+// tile 0..2: 2^-tile of the lines per workgroup; 3 / 4: full tiles, 2 / 4 per workgroup;
+// 5: full tiles, wavefront-local passes (k_rfft_w)
 int rfft_tpb(int tile) { return tile == 3 ? 2 : (tile == 4 ? 4 : 1); }
 
 template <int N1, int N2>
@@ -481,6 +600,10 @@ void launch_rfft_sized(bool inv, int tile, dim3 grid, size_t lds, hipStream_t s,
   else if (tile == 2) { if (inv) SNAC_RF(true, 2, 1); else SNAC_RF(false, 2, 1); }
   else if (tile == 3) { if (inv) SNAC_RF(true, 0, 2); else SNAC_RF(false, 0, 2); }
   else if (tile == 4) { if (inv) SNAC_RF(true, 0, 4); else SNAC_RF(false, 0, 4); }
+  else if (tile == 5) {
+    if (inv) hipLaunchKernelGGL((k_rfft_w<N1, N2, true>), grid, dim3(NT), lds, s, a);
+    else hipLaunchKernelGGL((k_rfft_w<N1, N2, false>), grid, dim3(NT), lds, s, a);
+  }
   else throw std::runtime_error("r2r: bad tile choice");
 #undef SNAC_RF
 }
@@ -535,7 +658,7 @@ double r2r_coef(int kind, int n, int k, int j) {
 
 int r2r_tile_choices(const R2RPlan &pl) {
   if (!pl.fft) return 1;
-  return (pl.kind == K_R2HC || pl.kind == K_HC2R) && pl.n >= 8 ? 5 : 3;
+  return (pl.kind == K_R2HC || pl.kind == K_HC2R) && pl.n >= 8 ? 6 : 3;
 }
 
 R2RPlan r2r_plan(int kind, int n) {
