@@ -258,7 +258,9 @@ constexpr int rfft_lines(int n1, int n2) {
 // The two passes of the h-point complex FFT of one line (h = N1*N2) and
 // its unfolding.  The line's LDS slot holds the n = 2h raw doubles (r) and,
 // after pass 1, its h complex values (T, XOR-swizzled between the passes,
-// then Z in natural order).
+// then Z in natural order).  k_rfft_w uses these; k_rfft keeps the same
+// steps written inline, which the compiler allocates in fewer registers
+// there (164 against 172 VGPRs for the n = 256 inverse: a wave per SIMD).
 // Pass 1: N1-point DFT over j1 of column j2, the inverse's folding of the
 // halfcomplex input on the way in, the twiddles W_h^{j2 k1} on the way out.
 template <int N1, int N2, bool INV>
@@ -386,9 +388,38 @@ __global__ __launch_bounds__(NT) void k_rfft(Args a) {
     const bool act = t < L * N2;
     const int j2 = t % N2, l = t / N2;
     double2 v[N1];
-    if (act) rfft_pass1<N1, N2, INV>(R + LS * l, j2, tw, v);
+    if (act) {
+      const double *r = R + LS * l;
+#pragma unroll
+      for (int j1 = 0; j1 < N1; j1++) {
+        const int k = N2 * j1 + j2;
+        if (!INV) {
+          v[j1] = make_double2(r[2 * k], r[2 * k + 1]);
+        } else {
+          // X[k] and conj X[h-k] from the halfcomplex line
+          const double2 xk = (k == 0) ? make_double2(r[0], 0.0) : make_double2(r[k], r[n - k]);
+          const double2 xc = (k == 0) ? make_double2(r[h], 0.0) : make_double2(r[h - k], -r[h + k]);
+          const double2 w = tw[k];  // conj of w^k
+          const double dx = xk.x - xc.x, dy = xk.y - xc.y;
+          const double bx = dx * w.x + dy * w.y, by = dy * w.x - dx * w.y;  // (dx + i dy) e^{+2 pi i k/n}
+          v[j1] = make_double2(xk.x + xc.x - by, xk.y + xc.y + bx);        // A + i B
+        }
+      }
+      fft_reg<N1>(v, tw, 2 * N2, INV);
+#pragma unroll
+      for (int k1 = 1; k1 < N1; k1++) {
+        double2 w = tw[(2 * j2 * k1) & (n - 1)];
+        if (INV) w.y = -w.y;
+        double2 y = v[k1];
+        v[k1] = make_double2(y.x * w.x - y.y * w.y, y.x * w.y + y.y * w.x);
+      }
+    }
     __syncthreads();
-    if (act) rfft_put1<N1, N2>(S + (LS / 2) * l, j2, v);
+    if (act) {
+      double2 *T = S + (LS / 2) * l;
+#pragma unroll
+      for (int k1 = 0; k1 < N1; k1++) T[k1 * N2 + (j2 ^ (k1 & (N2 - 1)))] = v[k1];
+    }
     __syncthreads();
   }
   // 3. second pass: N2-point DFTs over j2 for every (line, k1)
@@ -397,9 +428,18 @@ __global__ __launch_bounds__(NT) void k_rfft(Args a) {
     const bool act = t < L * N1;
     const int k1 = t % N1, l = t / N1;
     double2 v[N2];
-    if (act) rfft_pass2<N1, N2, INV>(S + (LS / 2) * l, k1, tw, v);
+    if (act) {
+      const double2 *T = S + (LS / 2) * l;
+#pragma unroll
+      for (int j2 = 0; j2 < N2; j2++) v[j2] = T[k1 * N2 + (j2 ^ (k1 & (N2 - 1)))];
+      fft_reg<N2>(v, tw, 2 * N1, INV);
+    }
     __syncthreads();
-    if (act) rfft_put2<N1, N2>(S + (LS / 2) * l, k1, v);
+    if (act) {
+      double2 *Z = S + (LS / 2) * l;
+#pragma unroll
+      for (int k2 = 0; k2 < N2; k2++) Z[k1 + N1 * k2] = v[k2];
+    }
     __syncthreads();
   }
   // 4. post-mapping and store
@@ -407,8 +447,23 @@ __global__ __launch_bounds__(NT) void k_rfft(Args a) {
   for (int it = 0; it < IT; it++) {
     const int e = threadIdx.x + it * NT;
     const int l = a.coal ? e % L : e / n, q = a.coal ? e / L : e % n;
-    if (e < L * n && oout[sl][l] >= 0)
-      a.out[q * a.oes + oout[sl][l]] = rfft_out<N1, N2, INV>(S + (LS / 2) * l, q, tw) * a.scale;
+    if (e < L * n && oout[sl][l] >= 0) {
+      const double2 *Z = S + (LS / 2) * l;
+      double y;
+      if (!INV) {
+        const int k = q <= h ? q : n - q;
+        const double2 zk = Z[k & (h - 1)], zc = Z[(h - k) & (h - 1)];  // Z[k], Z[h-k] (mod h)
+        const double ex = 0.5 * (zk.x + zc.x), ey = 0.5 * (zk.y - zc.y);
+        const double ox = 0.5 * (zk.y + zc.y), oy = -0.5 * (zk.x - zc.x);
+        const double2 w = tw[k];
+        const double xr = ex + (w.x * ox - w.y * oy), xi = ey + (w.x * oy + w.y * ox);
+        y = q <= h ? xr : xi;
+      } else {
+        const double2 z = Z[q >> 1];
+        y = (q & 1) ? z.y : z.x;
+      }
+      a.out[q * a.oes + oout[sl][l]] = y * a.scale;
+    }
   }
   }
 }
@@ -425,12 +480,14 @@ __device__ __forceinline__ void wave_sync() {
 
 // The same transform with each wavefront owning LW = L/4 of the workgroup's
 // L lines through both passes, so the passes need no workgroup barrier.
+// Four waves per SIMD (at most 128 VGPRs, a few spilled) measured faster
+// than three without spills.
 // Contiguous lines (stride 1) are also loaded and stored by their own
 // wavefront (one barrier in all, for the line offsets); strided lines are
 // loaded and stored by the whole workgroup, 8L contiguous bytes per row,
 // with a barrier after the load and one before the store.
 template <int N1, int N2, bool INV>
-__global__ __launch_bounds__(NT) void k_rfft_w(Args a) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_rfft_w(Args a) {
   extern __shared__ double2 S[];
   constexpr int h = N1 * N2, n = 2 * h;
   constexpr int L = rfft_lines(N1, N2), LW = L / 4;
