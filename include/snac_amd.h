@@ -267,7 +267,8 @@ int snac_get_timing(snac_ctx *ctx, double *ms, int *count);
 int snac_get_autotune(snac_ctx *ctx, int *choices, double *ms, int n);
 /* overrides an autotuned kernel variant: SNAC_KV_RK_MOM (0..16, as above),
  * SNAC_KV_TRIDIAG (0: c' in LDS, lines up to 128; 1: c' in scratch; 2:
- * chunked, no scratch, lines of 8..256) or SNAC_KV_GHOST (0: ghost fills as
+ * chunked, no scratch, lines of 8..256; 3: as 2 with twice the chunks,
+ * lines of 16..128) or SNAC_KV_GHOST (0: ghost fills as
  * one launch per phase of the reference's fill order; 1, the default: each
  * fill composed at set-up into one launch that computes every ghost cell's
  * final value through the same chain of operations, where the fill allows

@@ -371,7 +371,7 @@ class Context:
         _check(lib.snac_set_kernel_variant(self.h, 0, int(v)), "set_kernel_variant")
 
     def set_tridiag_variant(self, v):
-        """Overrides the autotuned tridiagonal kernel variant (0..2)."""
+        """Overrides the autotuned tridiagonal kernel variant (0..3)."""
         _check(lib.snac_set_kernel_variant(self.h, 1, int(v)), "set_kernel_variant")
 
     def set_ghost_variant(self, v):

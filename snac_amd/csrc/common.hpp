@@ -172,7 +172,8 @@ double r2r_coef(int kind, int n, int k, int j);
 struct TridiagLine {
   int G[3];
   int s;        // line axis
-  int variant;  // 0: c' in LDS (lines up to 128), 1: c' in cscratch, 2: chunked, no scratch (autotuned)
+  int variant;  // 0: c' in LDS (lines up to 128), 1: c' in cscratch, 2: chunked, no scratch,
+                // 3: chunked with twice the chunks (autotuned)
   const double *am, *ap, *d0, *wt;  // size G[s]
   const double *lamA, *lamB;        // along the other axes (ascending order)
   int pin_when_zero;
@@ -181,7 +182,8 @@ struct TridiagLine {
 // dscratch (d') and cscratch (c', used only for lines longer than the LDS
 // budget) have the size and layout of x
 void launch_tridiag(hipStream_t st, const TridiagLine &t, double *x, double *dscratch, double *cscratch);
-// chunks per line of variant 2 for lines of length m (0: not available)
-int tridiag_chunks(int m);
+// chunks per line of variant 2 (fine: variant 3) for lines of length m
+// (0: not available)
+int tridiag_chunks(int m, bool fine = false);
 
 }  // namespace snac

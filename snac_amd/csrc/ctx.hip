@@ -802,7 +802,8 @@ void autotune(snac_ctx *c) {
   tune(t.P.bwd[0], [&] { stage_last(c, t.P, -1); });
   int best = 1;
   double bt = 1e300;
-  for (int v = (t.G[t.s] <= 128 ? 0 : 1); v <= (tridiag_chunks(t.G[t.s]) > 0 ? 2 : 1); v++) {
+  const int vmax = tridiag_chunks(t.G[t.s], true) > 0 ? 3 : (tridiag_chunks(t.G[t.s]) > 0 ? 2 : 1);
+  for (int v = (t.G[t.s] <= 128 ? 0 : 1); v <= vmax; v++) {
     t.tvariant = v;
     double ms = time_stage(c, [&] { stage_tridiag(c, t.P, 0.0); });
     if (ms < bt) { bt = ms; best = v; }
@@ -1849,7 +1850,8 @@ int snac_set_kernel_variant(snac_ctx *c, int kernel, int variant) {
     } else if (kernel == SNAC_KV_TRIDIAG) {
       if (!c->tp.ok) throw std::runtime_error("no tensor-product plan on this case");
       const int m = c->tp.G[c->tp.s];
-      if (variant < 0 || variant > 2 || (variant == 0 && m > 128) || (variant == 2 && tridiag_chunks(m) == 0))
+      if (variant < 0 || variant > 3 || (variant == 0 && m > 128) || (variant == 2 && tridiag_chunks(m) == 0) ||
+          (variant == 3 && tridiag_chunks(m, true) == 0))
         throw std::runtime_error("tridiagonal variant not available for lines of " + std::to_string(m));
       c->tp.tvariant = variant;
     } else if (kernel == SNAC_KV_GHOST) {

@@ -201,10 +201,14 @@ __global__ __launch_bounds__(64 * NC) void k_tridiag_chunk(TridiagLine t, double
 }  // namespace
 
 // chunk count for the chunked variant (0: not applicable to this length):
-// at most 16 rows per chunk, at least 2
-int tridiag_chunks(int m) {
+// at most 16 rows per chunk, at least 2; fine: twice that many chunks (at
+// most 16, at least 2 rows each), for more lines in flight per CU
+int tridiag_chunks(int m, bool fine) {
   for (int nc : {4, 8, 16})
-    if (m >= 2 * nc && (m + nc - 1) / nc <= 16) return nc;
+    if (m >= 2 * nc && (m + nc - 1) / nc <= 16) {
+      if (!fine) return nc;
+      return 2 * nc <= 16 && m >= 4 * nc ? 2 * nc : 0;
+    }
   return 0;
 }
 
@@ -213,7 +217,7 @@ void launch_tridiag(hipStream_t st, const TridiagLine &t, double *x, double *dsc
   const int o1 = s == 0 ? 1 : 0, o2 = s == 2 ? 1 : 2;
   long nlines = (long)t.G[o1] * t.G[o2];
   dim3 grid((unsigned)((nlines + WG - 1) / WG));
-  const int nc = t.variant == 2 ? tridiag_chunks(t.G[s]) : 0;
+  const int nc = t.variant == 2 ? tridiag_chunks(t.G[s]) : (t.variant == 3 ? tridiag_chunks(t.G[s], true) : 0);
   if (nc > 0) {
     const int base = t.G[s] / nc, extra = t.G[s] % nc;
     const bool small = base + (extra ? 1 : 0) <= 9;

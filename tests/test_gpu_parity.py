@@ -178,7 +178,7 @@ def test_chkdiv_chkdt(tmp_path):
     assert ctx.chkdt() == pytest.approx(oc.chkdt(), rel=1e-14)
 
 
-@pytest.mark.parametrize("tvariant", [0, 1, 2])
+@pytest.mark.parametrize("tvariant", [0, 1, 2, 3])
 @pytest.mark.parametrize("geom", ["channel", "cavity", "ldc_fft", "ldc_3d", "triperiodic", "triperiodic_3d"])
 def test_solve_pressure(tmp_path, geom, tvariant):
     import snac_amd
