@@ -480,12 +480,12 @@ __device__ __forceinline__ void wave_sync() {
 
 // The same transform with each wavefront owning LW = L/4 of the workgroup's
 // L lines through both passes, so the passes need no workgroup barrier.
-// Four waves per SIMD (at most 128 VGPRs, a few spilled) measured faster
-// than three without spills.
 // Contiguous lines (stride 1) are also loaded and stored by their own
 // wavefront (one barrier in all, for the line offsets); strided lines are
 // loaded and stored by the whole workgroup, 8L contiguous bytes per row,
-// with a barrier after the load and one before the store.
+// with a barrier after the load and one before the store.  Compiled for
+// four waves per SIMD (at most 128 VGPRs, a few spilled), which measured
+// faster than three without spills.
 template <int N1, int N2, bool INV>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_rfft_w(Args a) {
   extern __shared__ double2 S[];
