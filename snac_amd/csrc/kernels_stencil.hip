@@ -290,7 +290,9 @@ __global__ __launch_bounds__(64 * BY) void k_rk_mom_lds(Geo g, Metrics m, const 
   __shared__ double sh[4][3][PL];  // field (u, v, w, p) x ring slot x plane
   int bx, by, bz;
   tile_of((g.nx + 63) / 64, (g.ny + BY - 1) / BY, bx, by, bz);
-  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tx;
+  // a wavefront is one tile row: ty (and with it j and the y metrics) is
+  // wave-uniform, held in scalar registers
+  const int tx = threadIdx.x, ty = __builtin_amdgcn_readfirstlane(threadIdx.y), tid = ty * 64 + tx;
   const int i0 = bx * 64 + 1, j0 = by * BY + 1;
   const int i = i0 + tx, j = j0 + ty;
   const int k0 = bz * KB + 1;
@@ -695,6 +697,7 @@ void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *
     // 11..14: the same with non-temporal hints on the streams;
     // (capping the registers at 128 for a fourth wave per SIMD spills and measured slower)
     // 15, 16: 64x8 tiles over 64 planes, without / with the hints
+    // (64x12 tiles, twelve waves per workgroup, measured 6-25 % slower)
     const bool nt = variant >= 11 && variant != 15;
     const int vi = variant >= 15 ? 4 : (variant - 7) % 4;
     const int kb = vi == 4 ? 64 : (vi % 2 ? 32 : 16), by = vi >= 2 ? 8 : 4;
