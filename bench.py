@@ -134,8 +134,8 @@ def cpu_baseline(d, b, size, steps, threads):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is a design, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: the time step runs on one GPU; --gpus {args.gpus} is not supported "
+              "(sharding the blocks over GPUs is not implemented)", file=sys.stderr)
         sys.exit(2)
     import numpy as np
 
