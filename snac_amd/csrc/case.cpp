@@ -396,7 +396,7 @@ Case build_case(const std::string &dns_path, const std::string &blocks_path, int
               B.vin[q][q][s][t1 + (size_t)m1 * t2] = uref * f1 * f2;
             }
         } else if (B.inflow_type[q][s] != 0) {
-          throw std::runtime_error("inflow type " + std::to_string(B.inflow_type[q][s]) + " is not available in this build");
+          throw std::runtime_error("inflow type " + std::to_string(B.inflow_type[q][s]) + " is not implemented");
         }
       }
   // pressure BC rhs planes, init_bc_rhs solver.f90:44-157 (centered);
