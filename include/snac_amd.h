@@ -288,7 +288,7 @@ int snac_set_kernel_variant(snac_ctx *ctx, int kernel, int variant);
  * (R2HC/HC2R of n >= 8 only): full tiles, each wavefront transforming its
  * own quarter of the lines with no workgroup barrier between the passes; 6
  * (R2HC/HC2R of n >= 8 only): full tiles with the n/2-point FFT's two
- * factors in the other order.                                              */
+ * factors in the other order; 7: as 5 with the factors in the other order. */
 int snac_r2r_lines(snac_ctx *ctx, int kind, int n, int howmany, int strided, int tile, const double *in,
                    double *out);
 /* STREAM-like copy of n doubles (read n*8 + write n*8 bytes per repetition)

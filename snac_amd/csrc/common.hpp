@@ -152,7 +152,8 @@ struct R2RPlan {
   int lshift = 0;          // FFT tile (autotuned): 0..2 lines per workgroup = max >> lshift;
                            // 3, 4 (R2HC/HC2R): full tiles, 2 / 4 per workgroup, pipelined
                            // 5 (R2HC/HC2R): full tiles, wavefront-local passes;
-                           // 6 (R2HC/HC2R): full tiles, the FFT's factors swapped
+                           // 6 (R2HC/HC2R): full tiles, the FFT's factors swapped;
+                           // 7 (R2HC/HC2R): wavefront-local passes, factors swapped
   const double *tab = nullptr;  // device twiddles (fft) or dense matrix
 };
 R2RPlan r2r_plan(int kind, int n);  // cached per (kind, n); throws if unsupported

@@ -1870,7 +1870,7 @@ int snac_r2r_lines(snac_ctx *c, int kind, int n, int howmany, int strided, int t
                    double *out) {
   return guard([&] {
     on_device(c);
-    if (tile < 0 || tile > 6) throw std::runtime_error("r2r_lines: tile must be 0..6");
+    if (tile < 0 || tile > 7) throw std::runtime_error("r2r_lines: tile must be 0..7");
     R2RPlan pl = r2r_plan(kind, n);
     pl.lshift = tile;
     size_t tot = (size_t)n * howmany;

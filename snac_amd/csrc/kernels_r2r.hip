@@ -646,7 +646,8 @@ void launch_fft_kind(int kind, int n, int ls, dim3 grid, size_t lds, hipStream_t
 }
 
 // tile 0..2: 2^-tile of the lines per workgroup; 3 / 4: full tiles, 2 / 4 per workgroup;
-// 5: full tiles, wavefront-local passes (k_rfft_w); 6: full tiles, swapped split
+// 5: full tiles, wavefront-local passes (k_rfft_w); 6: full tiles, swapped split;
+// 7: wavefront-local passes, swapped split
 int rfft_tpb(int tile) { return tile == 3 ? 2 : (tile == 4 ? 4 : 1); }
 
 template <int N1, int N2>
@@ -665,36 +666,42 @@ void launch_rfft_sized(bool inv, int tile, dim3 grid, size_t lds, hipStream_t s,
 #undef SNAC_RF
 }
 
-// tile 6: full tiles with the h-point FFT split the other way round (the
-// N1-point pass over all of a tile's threads, the longer pass second)
+// tiles 6 / 7: full tiles (7: wavefront-local passes) with the h-point FFT
+// split the other way round (the shorter pass first, over all of a tile's
+// threads)
 template <int N1, int N2>
-void launch_rfft_swapped(bool inv, dim3 grid, size_t lds, hipStream_t s, const Args &a) {
-  if (inv) hipLaunchKernelGGL((k_rfft<N1, N2, true, 0, 1>), grid, dim3(NT), lds, s, a);
-  else hipLaunchKernelGGL((k_rfft<N1, N2, false, 0, 1>), grid, dim3(NT), lds, s, a);
+void launch_rfft_swapped(bool inv, bool wave, dim3 grid, size_t lds, hipStream_t s, const Args &a) {
+  if (wave) {
+    if (inv) hipLaunchKernelGGL((k_rfft_w<N1, N2, true>), grid, dim3(NT), lds, s, a);
+    else hipLaunchKernelGGL((k_rfft_w<N1, N2, false>), grid, dim3(NT), lds, s, a);
+  } else {
+    if (inv) hipLaunchKernelGGL((k_rfft<N1, N2, true, 0, 1>), grid, dim3(NT), lds, s, a);
+    else hipLaunchKernelGGL((k_rfft<N1, N2, false, 0, 1>), grid, dim3(NT), lds, s, a);
+  }
 }
 
 // h-point split of an n = 2h point real transform
 void launch_rfft(bool inv, int n, int ls, dim3 grid, size_t lds, hipStream_t s, const Args &a) {
-  const bool sw = ls == 6;
+  const bool sw = ls == 6 || ls == 7, wv = ls == 7;
   switch (n) {
-  case 8: launch_rfft_sized<2, 2>(inv, sw ? 0 : ls, grid, lds, s, a); break;
+  case 8: launch_rfft_sized<2, 2>(inv, sw ? (wv ? 5 : 0) : ls, grid, lds, s, a); break;
   case 16:
-    if (sw) launch_rfft_swapped<2, 4>(inv, grid, lds, s, a);
+    if (sw) launch_rfft_swapped<2, 4>(inv, wv, grid, lds, s, a);
     else launch_rfft_sized<4, 2>(inv, ls, grid, lds, s, a);
     break;
-  case 32: launch_rfft_sized<4, 4>(inv, sw ? 0 : ls, grid, lds, s, a); break;
+  case 32: launch_rfft_sized<4, 4>(inv, sw ? (wv ? 5 : 0) : ls, grid, lds, s, a); break;
   case 64:
-    if (sw) launch_rfft_swapped<4, 8>(inv, grid, lds, s, a);
+    if (sw) launch_rfft_swapped<4, 8>(inv, wv, grid, lds, s, a);
     else launch_rfft_sized<8, 4>(inv, ls, grid, lds, s, a);
     break;
-  case 128: launch_rfft_sized<8, 8>(inv, sw ? 0 : ls, grid, lds, s, a); break;
+  case 128: launch_rfft_sized<8, 8>(inv, sw ? (wv ? 5 : 0) : ls, grid, lds, s, a); break;
   case 256:
-    if (sw) launch_rfft_swapped<8, 16>(inv, grid, lds, s, a);
+    if (sw) launch_rfft_swapped<8, 16>(inv, wv, grid, lds, s, a);
     else launch_rfft_sized<16, 8>(inv, ls, grid, lds, s, a);
     break;
-  case 512: launch_rfft_sized<16, 16>(inv, sw ? 0 : ls, grid, lds, s, a); break;
+  case 512: launch_rfft_sized<16, 16>(inv, sw ? (wv ? 5 : 0) : ls, grid, lds, s, a); break;
   case 1024:
-    if (sw) launch_rfft_swapped<16, 32>(inv, grid, lds, s, a);
+    if (sw) launch_rfft_swapped<16, 32>(inv, wv, grid, lds, s, a);
     else launch_rfft_sized<32, 16>(inv, ls, grid, lds, s, a);
     break;
   default: throw std::runtime_error("r2r: no real-FFT path for length " + std::to_string(n));
@@ -736,7 +743,7 @@ double r2r_coef(int kind, int n, int k, int j) {
 
 int r2r_tile_choices(const R2RPlan &pl) {
   if (!pl.fft) return 1;
-  return (pl.kind == K_R2HC || pl.kind == K_HC2R) && pl.n >= 8 ? 7 : 3;
+  return (pl.kind == K_R2HC || pl.kind == K_HC2R) && pl.n >= 8 ? 8 : 3;
 }
 
 R2RPlan r2r_plan(int kind, int n) {

@@ -76,7 +76,7 @@ def test_r2r_lines(tmp_path, kind, n):
         x = np.random.default_rng(n + howmany).uniform(-1, 1, (howmany, n))
         ref = np.array([oracle_py.r2r(kind, row, direct=True) for row in x])
         for strided in (False, True):
-            for tile in range(7):
+            for tile in range(8):
                 y = ctx.r2r_lines(kind, x, strided=strided, tile=tile)
                 err = np.abs(y - ref).max()
                 assert err <= 1e-13 * n * np.abs(ref).max(), f"lines {howmany} strided {strided} tile {tile}: {err:.3e}"
@@ -97,7 +97,7 @@ def test_rfft_lengths(tmp_path, kind, n):
     x[40] = 0.0
     ref = np.array([oracle_py.r2r(kind, row, direct=True) for row in x])
     for strided in (False, True):
-        for tile in range(7):
+        for tile in range(8):
             y = ctx.r2r_lines(kind, x, strided=strided, tile=tile)
             err = np.abs(y - ref).max()
             assert err <= 1e-13 * n * np.abs(ref).max(), f"strided {strided} tile {tile}: {err:.3e}"
