@@ -253,7 +253,10 @@ int snac_set_timing(snac_ctx *ctx, int enable);
  * timed intervals per phase since the last call, then resets them          */
 int snac_get_timing(snac_ctx *ctx, double *ms, int *count);
 
-/* Choices of the autotuner that runs at context creation (tile shape of each
+/* Choices of the autotuner that runs at context creation -- once per problem
+ * shape in a process: a later context of the same block and transform
+ * shapes reuses them, so the two run the same kernels and agree bit for bit
+ * (tile shape of each
  * transform call site: forward ones, then inverse ones, then the
  * tridiagonal variant -- these only with the tensor-product solver; -1 =
  * dense path -- and last the rk_mom kernel variant: 0 one cell per thread,

@@ -4,7 +4,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -1043,6 +1046,65 @@ void reduce(snac_ctx *c, bool div, double out[2]) {
   SNAC_HIP(hipStreamSynchronize(c->stream));
 }
 
+// The autotuner's choices per problem shape, kept for the life of the
+// process: a second context of the same shapes runs the same kernels
+// without tuning again, so the two agree bit for bit (tiles that factor a
+// transform differently round differently).
+struct TuneChoice {
+  std::vector<int> fwd, bwd;
+  int tvariant = 1, rk_variant = 0;
+  std::vector<double> tune_ms;
+  double rk_ms = 0.0;
+};
+std::mutex g_tune_mu;
+std::map<std::string, TuneChoice> g_tune;
+
+std::string tune_key(const snac_ctx *c) {
+  std::ostringstream k;
+  k << "dev " << c->device << " scal " << c->cs.nscal << " blocks";
+  for (const auto &D : c->blk) k << ' ' << D.g.nx << 'x' << D.g.ny << 'x' << D.g.nz;
+  const TensorPlan &t = c->tp;
+  if (t.ok) {
+    k << " tensor " << t.G[0] << 'x' << t.G[1] << 'x' << t.G[2] << " line " << t.s << " axes";
+    for (int a : t.xaxes) k << ' ' << a;
+    for (const auto &p : t.P.fwd) k << " f" << p.kind << ':' << p.n;
+    for (const auto &p : t.P.bwd) k << " b" << p.kind << ':' << p.n;
+  }
+  return k.str();
+}
+
+void tune_or_recall(snac_ctx *c) {
+  if (c->device < 0) return;
+  const std::string key = tune_key(c);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  TensorPlan &t = c->tp;
+  auto it = g_tune.find(key);
+  if (it != g_tune.end()) {
+    const TuneChoice &tc = it->second;
+    if (t.ok) {
+      for (size_t q = 0; q < t.P.fwd.size(); q++) t.P.fwd[q].lshift = tc.fwd[q];
+      for (size_t q = 0; q < t.P.bwd.size(); q++) t.P.bwd[q].lshift = tc.bwd[q];
+      t.tvariant = tc.tvariant;
+      t.tune_ms = tc.tune_ms;
+    }
+    c->rk_variant = tc.rk_variant;
+    c->rk_tune_ms = tc.rk_ms;
+    return;
+  }
+  autotune(c);
+  autotune_rk(c);
+  TuneChoice tc;
+  if (t.ok) {
+    for (const auto &p : t.P.fwd) tc.fwd.push_back(p.lshift);
+    for (const auto &p : t.P.bwd) tc.bwd.push_back(p.lshift);
+    tc.tvariant = t.tvariant;
+    tc.tune_ms = t.tune_ms;
+  }
+  tc.rk_variant = c->rk_variant;
+  tc.rk_ms = c->rk_tune_ms;
+  g_tune.emplace(key, tc);
+}
+
 void create(snac_ctx *c) {
   if (c->device >= 0) {
     SNAC_HIP(hipSetDevice(c->device));
@@ -1114,8 +1176,7 @@ void create(snac_ctx *c) {
     c->sforce = dalloc<double>(c, cs.nscal);
   }
   plan_tensor(c);
-  autotune(c);
-  autotune_rk(c);
+  tune_or_recall(c);
   if (!c->tp.ok) plan_mg(c);
   if (c->device >= 0) SNAC_HIP(hipDeviceSynchronize());
 }
