@@ -297,6 +297,21 @@ def test_ldc_golden_gpu(fft_axis, tmp_path):
     np.testing.assert_allclose(full[full.shape[1] // 2, :], ref[:, 1], rtol=1e-7, atol=5e-8)
 
 
+def test_autotune_choices_kept_per_shape(tmp_path):
+    """A second context of the same shapes reuses the first one's kernel
+    choices (and their timings) instead of tuning again; a context of other
+    shapes tunes for itself."""
+    import snac_amd
+    d, b = cases.write_case(str(tmp_path / "a"), [cases.channel_block((32, 24, 16))], 5600.0)
+    d2, b2 = cases.write_case(str(tmp_path / "b"), [cases.channel_block((32, 24, 32))], 5600.0)
+    first, second, other = snac_amd.Context(d, b, 3), snac_amd.Context(d, b, 3), snac_amd.Context(d2, b2, 3)
+    assert first.autotune() == second.autotune()
+    ch, ms = other.autotune()
+    assert len(ch) == len(first.autotune()[0]) and all(x > 0 for x in ms)
+    for c in (first, second, other):
+        c.close()
+
+
 def test_graph_replays_match_eager(tmp_path):
     """The hipGraph-captured step against eager launches: several replays in
     a row, with host synchronisations and downloads between them, every
