@@ -283,7 +283,9 @@ class Context:
 
     def r2r_lines(self, kind, x, strided=False, tile=0):
         """Transforms every row of x (howmany x n); strided=True hands the
-        lines to the device transposed (element stride = howmany)."""
+        lines to the device transposed (element stride = howmany).  tile:
+        the kernel choice 0..7 the autotuner picks from (5..7 and 3, 4 for
+        R2HC/HC2R only; the other kinds treat them as 0)."""
         x = np.atleast_2d(np.asarray(x, dtype=np.float64))
         h, n = x.shape
         src = np.ascontiguousarray(x.T if strided else x)
