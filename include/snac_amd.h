@@ -119,6 +119,19 @@ int snac_learned_time(snac_ctx *ctx, int reps, double *ms, double *gflop);
  * are 0 when the tensor-product solver ran.                                 */
 int snac_solver_stats(snac_ctx *ctx, int *iters, double *max_relres);
 
+/* Counters of MG-PCG pressure solves since the last call (then reset):
+ * all solves; those whose captured iterations (graph steps) left some mode
+ * above the tolerance, so that the host resumed them; those that stopped
+ * at hypre_maxiter above the tolerance (HYPRE's StructPCGGetNumIterations
+ * reaching maxiter, solver.f90:553-569).  No device work.                  */
+int snac_solver_counters(snac_ctx *ctx, int *solves, int *continued, int *unconverged);
+/* CG iterations each pressure solve runs inside a captured step graph on
+ * the MG-PCG path before the host resumes it with convergence checks (the
+ * solve still stops at hypre_tol either way).  0 (the default): one
+ * host-checked step measures it (largest count + 1, raised when later
+ * solves need more).  No counterpart in the reference.                      */
+int snac_set_graph_cg_iters(snac_ctx *ctx, int n);
+
 /* host <-> device copies of one field of one block (layout above)           */
 int snac_upload_field(snac_ctx *ctx, int b, int fid, const double *host);
 int snac_download_field(snac_ctx *ctx, int b, int fid, double *host);

@@ -55,6 +55,8 @@ def _load():
     L.snac_ctx_set_solver.argtypes = [vp, C.c_int]
     L.snac_set_precision.argtypes = [vp, C.c_int]
     L.snac_solver_stats.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.snac_solver_counters.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.snac_set_graph_cg_iters.argtypes = [vp, C.c_int]
     L.snac_upload_field.argtypes = [vp, C.c_int, C.c_int, dp]
     L.snac_download_field.argtypes = [vp, C.c_int, C.c_int, dp]
     i3 = np.ctypeslib.ndpointer(dtype=np.int32, shape=(3,), flags="C_CONTIGUOUS")
@@ -351,6 +353,17 @@ class Context:
         rel = C.c_double()
         _check(lib.snac_solver_stats(self.h, C.byref(it), C.byref(rel)), "solver_stats")
         return it.value, rel.value
+
+    def solver_counters(self):
+        """(solves, host-resumed solves, solves that stopped above the
+        tolerance) of the MG-PCG path since the last call."""
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _check(lib.snac_solver_counters(self.h, C.byref(a), C.byref(b), C.byref(c)), "solver_counters")
+        return a.value, b.value, c.value
+
+    def set_graph_cg_iters(self, n):
+        """CG iterations of each solve captured in the step graph (0: measured)."""
+        _check(lib.snac_set_graph_cg_iters(self.h, int(n)), "set_graph_cg_iters")
 
     def stream_copy(self, n, reps=10):
         """GB/s of a plain device copy of n doubles (HBM reference)."""

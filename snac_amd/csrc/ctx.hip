@@ -104,7 +104,6 @@ struct snac_ctx {
   std::vector<snac::ImpField> imp;  // its Helmholtz operators of u, v, w and the scalars
   snac::MGPlan mg;
   bool mg_planned = false;
-  bool mg_fixed = false;  // solves run a fixed iteration count (graph capture)
   int solver = SNAC_SOLVER_AUTO;
   int prec = SNAC_PREC_FP64;
   bool ghost_fused = true;  // run composed ghost-fill programs in one launch
@@ -120,11 +119,18 @@ struct snac_ctx {
   double tacc[SNAC_NPHASE] = {};
   int tcnt[SNAC_NPHASE] = {};
   // one captured RK3 step per starting scalar parity, replayed by
-  // snac_step / snac_steps
+  // snac_step / snac_steps.  On the MG-PCG path the step is captured as
+  // segments that each end after the captured iterations of a pressure
+  // solve; between segments the host resumes that solve (gres: its CG state
+  // at the end of the segment) until it meets the tolerance
   bool use_graph = true;
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  std::vector<hipGraphExec_t> gseg[2];
+  std::vector<snac::MGRun> gres[2];
   double graph_dt[2] = {0.0, 0.0};
   int graph_cur[2] = {-1, -1};
+  bool capturing = false;  // a step is being captured (segments go to cap_key)
+  int cap_key = 0;
+  int force_fixed = 0;     // CG iterations captured per solve (0: measured)
 };
 
 namespace snac {
@@ -748,11 +754,30 @@ void mg_stage_out(snac_ctx *c) {
   }
 }
 
+// ends the graph segment being captured at the end of a solve's captured
+// CG iterations, keeps the solve's state for the host's resumption, and
+// starts capturing the next segment
+void split_capture(snac_ctx *c, const MGRun &R) {
+  hipGraph_t g;
+  SNAC_HIP(hipStreamEndCapture(c->stream, &g));
+  hipGraphExec_t e = nullptr;
+  const hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  SNAC_HIP(err);
+  c->gseg[c->cap_key].push_back(e);
+  c->gres[c->cap_key].push_back(R);
+  SNAC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+}
+
 void solve_pressure(snac_ctx *c) {
   check_solver(c);
   if (use_mg(c)) {
     { Phase tf(c, SNAC_PH_R2R_FWD); mg_stage_in(c); }
-    { Phase tt(c, SNAC_PH_TRIDIAG); mg_solve(c->mg, c->stream, c->mg_fixed ? c->mg.fixed_iters : 0); }
+    {
+      Phase tt(c, SNAC_PH_TRIDIAG);
+      if (c->capturing) split_capture(c, mg_solve(c->mg, c->stream, c->mg.fixed_iters));
+      else mg_solve(c->mg, c->stream, 0);
+    }
     { Phase tb(c, SNAC_PH_R2R_BWD); mg_stage_out(c); }
     return;
   }
@@ -928,20 +953,68 @@ void substage(snac_ctx *c, int irk, double dt) {
   if (c->pending.size() > 4096) collect_timing(c);  // bounds the event pool in long timed runs
 }
 
-void drop_graph(snac_ctx *c) {
+// destroys the captured step graphs of one starting parity (all: key < 0);
+// a launched graph may still be running on the context's stream, so the
+// stream is drained first
+void drop_graph(snac_ctx *c, int key = -1) {
+  bool any = false;
+  for (int q = 0; q < 2; q++) any = any || ((key < 0 || key == q) && !c->gseg[q].empty());
+  if (!any) return;
+  if (c->stream) SNAC_HIP(hipStreamSynchronize(c->stream));
   for (int q = 0; q < 2; q++) {
-    if (c->gexec[q]) (void)hipGraphExecDestroy(c->gexec[q]);
-    c->gexec[q] = nullptr;
+    if (key >= 0 && key != q) continue;
+    for (auto e : c->gseg[q]) (void)hipGraphExecDestroy(e);
+    c->gseg[q].clear();
+    c->gres[q].clear();
   }
 }
 
+// captures one RK3 step into the segments of key (not run)
+void capture_step(snac_ctx *c, int key, double dt) {
+  const int cur0 = c->cur;
+  c->capturing = true;
+  c->cap_key = key;
+  SNAC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    for (int irk = 1; irk <= 3; irk++) substage(c, irk, dt);
+    hipGraph_t g;
+    SNAC_HIP(hipStreamEndCapture(c->stream, &g));
+    c->capturing = false;
+    hipGraphExec_t e = nullptr;
+    const hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    SNAC_HIP(err);
+    c->gseg[key].push_back(e);
+  } catch (...) {
+    if (c->capturing) {
+      hipGraph_t g = nullptr;
+      if (hipStreamEndCapture(c->stream, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+      c->capturing = false;
+    }
+    for (auto e : c->gseg[key]) (void)hipGraphExecDestroy(e);
+    c->gseg[key].clear();
+    c->gres[key].clear();
+    c->cur = cur0;
+    c->scur = key;
+    throw;
+  }
+  if (c->cur != cur0) throw std::runtime_error("internal: a step must return to the same buffer parity");
+  c->scur = key;  // captured, not run yet
+  c->graph_dt[key] = dt;
+  c->graph_cur[key] = c->cur;
+}
+
 // nsteps RK3 steps.  Eagerly when timing (events bracket every phase) or
-// when graphs are off; otherwise one step is captured into a hipGraph (the
+// when graphs are off; otherwise one step is captured into hipGraphs (the
 // kernels of a step address the same buffers every step: the velocity
 // parity flips six times per step) and replayed, re-captured when dt or the
-// starting parity changes.
+// starting parity changes.  On the MG-PCG path each pressure solve's first
+// fixed_iters CG iterations are part of the graph; the host then resumes the
+// solve until every mode meets hypre_tol (or hypre_maxiter), so a graph step
+// ends every solve exactly where an eager one does.
 void run_steps(snac_ctx *c, double dt, int nsteps) {
   check_solver(c);
+  if (nsteps <= 0) return;
   // the learned preconditioner and the pair recording decide on the host
   // between CG iterations: eager launches
   const bool host_cg = use_mg(c) && (c->prec == SNAC_PREC_LEARNED || c->mg.collecting);
@@ -951,59 +1024,39 @@ void run_steps(snac_ctx *c, double dt, int nsteps) {
     return;
   }
   const bool mgp = use_mg(c);
-  if (mgp && c->mg.fixed_iters == 0) {
-    // MG-PCG in a graph needs a fixed CG iteration count: one step with
-    // host-checked convergence measures it, the graph then runs that count
-    // plus a margin (converged modes skip their work)
+  MGPlan &P = c->mg;
+  if (mgp && P.fixed_iters == 0) {
+    // one step with host-checked convergence measures the CG count the
+    // captured part of each solve runs
     for (int irk = 1; irk <= 3; irk++) substage(c, irk, dt);
     nsteps--;
-    MGPlan &P = c->mg;
-    P.fixed_iters = std::min(P.maxiter, std::max(1, P.max_seen + std::max(2, P.max_seen / 4)));
+    P.fixed_iters = c->force_fixed > 0 ? std::min(c->force_fixed, P.maxiter)
+                                       : std::min(P.maxiter, std::max(1, P.max_seen + 1));
     drop_graph(c);
   }
   // a step flips the velocity parity six times (back to the start) and the
   // scalar parity three times: one graph per starting scalar parity
   for (int s = 0; s < nsteps; s++) {
     const int key = c->scur;
-    if (!c->gexec[key] || c->graph_dt[key] != dt || c->graph_cur[key] != c->cur) {
-      if (c->gexec[key]) (void)hipGraphExecDestroy(c->gexec[key]);
-      c->gexec[key] = nullptr;
-      hipGraph_t g;
-      SNAC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      const int cur0 = c->cur;
-      c->mg_fixed = mgp;
-      try {
-        for (int irk = 1; irk <= 3; irk++) substage(c, irk, dt);
-      } catch (...) {
-        c->mg_fixed = false;
-        (void)hipStreamEndCapture(c->stream, &g);
-        c->cur = cur0;
-        c->scur = key;
-        throw;
-      }
-      c->mg_fixed = false;
-      SNAC_HIP(hipStreamEndCapture(c->stream, &g));
-      if (c->cur != cur0) throw std::runtime_error("internal: a step must return to the same buffer parity");
-      c->scur = key;  // captured, not run yet
-      SNAC_HIP(hipGraphInstantiate(&c->gexec[key], g, nullptr, nullptr, 0));
-      SNAC_HIP(hipGraphDestroy(g));
-      c->graph_dt[key] = dt;
-      c->graph_cur[key] = c->cur;
+    if (c->gseg[key].empty() || c->graph_dt[key] != dt || c->graph_cur[key] != c->cur) {
+      drop_graph(c, key);
+      capture_step(c, key, dt);
     }
-    SNAC_HIP(hipGraphLaunch(c->gexec[key], c->stream));
+    const size_t nseg = c->gseg[key].size();
+    for (size_t q = 0; q < nseg; q++) {
+      SNAC_HIP(hipGraphLaunch(c->gseg[key][q], c->stream));
+      if (q + 1 < nseg) {
+        MGRun R = c->gres[key][q];
+        mg_continue(P, c->stream, R);
+      }
+    }
     if (c->cs.nscal > 0) c->scur = 1 - key;
   }
-  if (mgp && nsteps > 0) {
-    // a solve that ended above the tolerance raises the iteration count of
-    // the following steps (the reference's solver also stops at maxiter)
-    MGPlan &P = c->mg;
-    SNAC_HIP(hipMemcpyAsync(P.left_host, P.unconv, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    SNAC_HIP(hipStreamSynchronize(c->stream));
-    if (*P.left_host > 0 && P.fixed_iters < P.maxiter) {
-      P.fixed_iters = std::min(P.maxiter, P.fixed_iters + std::max(2, P.fixed_iters / 2));
-      mg_izero(c->stream, 1, P.unconv);
-      drop_graph(c);
-    }
+  // solves that needed more than the captured count raise it for the next
+  // capture (the count only grows, up to maxiter)
+  if (mgp && c->force_fixed == 0 && P.max_seen + 1 > P.fixed_iters && P.fixed_iters < P.maxiter) {
+    P.fixed_iters = std::min(P.maxiter, P.max_seen + 1);
+    drop_graph(c);
   }
 }
 
@@ -1300,6 +1353,27 @@ int snac_solver_stats(snac_ctx *c, int *iters, double *max_relres) {
     if (use_mg(c) && c->mg.ok) mg_stats(c->mg, c->stream, it, rel);
     if (iters) *iters = it;
     if (max_relres) *max_relres = rel;
+  });
+}
+
+int snac_solver_counters(snac_ctx *c, int *solves, int *continued, int *unconverged) {
+  return guard([&] {
+    checked(c);
+    MGPlan &P = c->mg;
+    if (solves) *solves = P.n_solves;
+    if (continued) *continued = P.n_continued;
+    if (unconverged) *unconverged = P.n_unconverged;
+    P.n_solves = P.n_continued = P.n_unconverged = 0;
+  });
+}
+
+int snac_set_graph_cg_iters(snac_ctx *c, int n) {
+  return guard([&] {
+    checked(c);
+    if (n < 0) throw std::runtime_error("graph CG iteration count must be >= 0");
+    drop_graph(c);
+    c->force_fixed = n;
+    c->mg.fixed_iters = 0;  // measured (or set) again on the next graph step
   });
 }
 

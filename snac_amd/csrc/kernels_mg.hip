@@ -489,10 +489,6 @@ __global__ __launch_bounds__(256) void k_flags(int ng, const double *__restrict_
   if (threadIdx.x == 0) left[0] = cnt;
 }
 
-__global__ void k_accum_max(int *dst, const int *src) {
-  if (threadIdx.x == 0 && blockIdx.x == 0 && src[0] > dst[0]) dst[0] = src[0];
-}
-
 // strided 3D copy; TILE 1: lanes along a on the read side (src stride(a) 1),
 // along s on the write side (dst stride(s) 1); TILE 2 the reverse
 constexpr int TT = 32;
@@ -669,11 +665,6 @@ void mg_sub_mean(hipStream_t st, const MGLev &L, T *x, const double *sums, const
 
 void mg_flags(hipStream_t st, int ng, const double *rr, const double *bb, double tol2, int *done, int *left) {
   hipLaunchKernelGGL(k_flags, dim3(1), dim3(256), 0, st, ng, rr, bb, tol2, done, left);
-  SNAC_HIP(hipGetLastError());
-}
-
-void mg_accum_max(hipStream_t st, int *dst, const int *src) {
-  hipLaunchKernelGGL(k_accum_max, dim3(1), dim3(64), 0, st, dst, src);
   SNAC_HIP(hipGetLastError());
 }
 

@@ -120,8 +120,6 @@ void mg_sub_mean(hipStream_t st, const MGLev &L, T *x, const double *sums, const
                  const int *done);
 // done[g] |= rr[g] <= tol2 * bb[g]; *left = number of groups not done
 void mg_flags(hipStream_t st, int ng, const double *rr, const double *bb, double tol2, int *done, int *left);
-// *dst = max(*dst, *src) on the device
-void mg_accum_max(hipStream_t st, int *dst, const int *src);
 // strided 3D copy (em x ea x es elements) with scaling; `tile` selects an
 // LDS transpose for src stride(a) == 1 -> dst stride(s) == 1 (1) or the
 // reverse (2); 0 = direct with lanes along s
@@ -175,11 +173,13 @@ struct MGPlan {
   int colors = 2;              // zebra colours per sweep (2, or 4 when coupled)
   int nlev = 0;
   int last_iters = 0;
-  // fixed-iteration mode (graph capture): CG iteration count, the largest
-  // count an eagerly checked solve needed, and a device counter of groups
-  // left above the tolerance at the end of fixed-iteration solves
+  // fixed-iteration mode (graph capture): CG iteration count of the
+  // captured part of a solve and the largest count an eagerly checked solve
+  // needed; solves counted since the last query: all, those the host had to
+  // continue past the captured count, those that stopped at maxiter above
+  // the tolerance
   int fixed_iters = 0, max_seen = 0;
-  int *unconv = nullptr;
+  int n_solves = 0, n_continued = 0, n_unconverged = 0;
   // FFT path: transform kinds and normalisation along the FFT axis, the
   // bounding-box work array and the transform plans (set up by the caller)
   int kind_fwd = 0, kind_bwd = 0;
@@ -193,14 +193,27 @@ struct MGPlan {
 // when the case cannot use it); device < 0 plans without allocating.
 void mg_plan(MGPlan &P, const Case &cs, int device);
 void mg_free(MGPlan &P);
+// Host-side position of one CG solve: the iterations launched so far and
+// the parity of the rz buffers.  A solve launched with a fixed iteration
+// count (graph capture) leaves this state behind, so the host can resume it
+// with convergence checks (mg_continue).
+struct MGRun {
+  int it = 0, cur = 0;
+  bool net = false, record = false;
+};
 // Solves M x = -rhs where rhs (in the plan's layout) has been copied into
 // P.r scaled by -1; on return P.x holds the solution of A x = rhs.
 // fixed == 0: the host checks the per-group convergence flags after every
 // CG iteration (stops at tol or maxiter).  fixed > 0: exactly `fixed`
 // iterations with no host synchronisation (converged groups skip their
-// work), so the solve can be captured in a graph; groups still above the
-// tolerance at the end raise *P.unconv.
-void mg_solve(MGPlan &P, hipStream_t st, int fixed = 0);
+// work), so the solve can be captured in a graph; the returned state then
+// goes to mg_continue once the launches have run.
+MGRun mg_solve(MGPlan &P, hipStream_t st, int fixed = 0);
+// Resumes a fixed-count solve: synchronises, and while some group is above
+// the tolerance and fewer than maxiter iterations ran, launches further
+// iterations with a check after each.  Every solve thus stops at tol (or
+// maxiter), whether its first iterations ran from a graph or not.
+void mg_continue(MGPlan &P, hipStream_t st, MGRun &R);
 // iterations of the last solve and the largest relative residual
 // ||r_g|| / ||b_g|| over the groups (synchronises the stream)
 void mg_stats(MGPlan &P, hipStream_t st, int &iters, double &max_relres);

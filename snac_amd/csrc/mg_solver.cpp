@@ -464,7 +464,6 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
   P.done = dev_zeros<int>(P, P.ng);
   P.singular = dev_upload(P, sing);
   P.left = dev_zeros<int>(P, 1);
-  P.unconv = dev_zeros<int>(P, 1);
   SNAC_HIP(hipHostMalloc((void **)&P.left_host, sizeof(int), hipHostMallocDefault));
   P.stall = dev_zeros<int>(P, 1);
   P.rr_prev = dev_zeros<double>(P, 2 * P.ng);
@@ -536,13 +535,43 @@ void precondition(MGPlan &P, hipStream_t st, double *rz_new, const double *rz_ol
 
 }  // namespace
 
-void mg_solve(MGPlan &P, hipStream_t st, int fixed) {
+namespace {
+
+// one CG iteration after the first: precondition the current residual, then
+// the operator application and the update with their dot products and the
+// convergence flags
+void mg_iteration(MGPlan &P, hipStream_t st, MGRun &R) {
+  const bool mixed = P.prec >= 1;
+  if (R.it > 0) {
+    precondition(P, st, P.rz[1 - R.cur], P.rz[R.cur], R.net, P.rr);
+    R.cur = 1 - R.cur;
+  }
+  mg_apply_dot(st, P.h64.lev[0], P.h64.lam, P.p, P.q, P.ng, P.gsize, P.part, P.nchunk, P.pq, P.done);
+  // the update with r.r and, for the flexible form, r_new . z_old in the second slot
+  mg_cg_update_dot(st, P.ng, P.gsize, P.rz[R.cur], P.pq, P.x, P.p, P.r, P.q, mixed ? P.rf : nullptr,
+                   mixed ? P.zf : nullptr, P.part, P.nchunk, P.rr, P.done);
+  mg_flags(st, P.ng, P.rr, P.bb, P.tol * P.tol, P.done, P.left);
+  if (R.net && P.fallback) launch_net_stall(st, P.ng, P.rr, P.rr_prev, P.done, P.stall_ratio * P.stall_ratio, P.stall);
+  R.it++;
+}
+
+// groups still above the tolerance (synchronises)
+int groups_left(MGPlan &P, hipStream_t st) {
+  SNAC_HIP(hipMemcpyAsync(P.left_host, P.left, sizeof(int), hipMemcpyDeviceToHost, st));
+  SNAC_HIP(hipStreamSynchronize(st));
+  return *P.left_host;
+}
+
+}  // namespace
+
+MGRun mg_solve(MGPlan &P, hipStream_t st, int fixed) {
   const long n = (long)P.ng * P.gsize;
   const double tol2 = P.tol * P.tol;
   const bool mixed = P.prec >= 1;  // the flexible CG of the FP32 preconditioners
   const bool planar = P.fa >= 0;   // one 2D plane per group (the network's input)
-  bool net = P.prec == 2 && P.net && planar && fixed == 0;
-  const bool record = P.collecting && planar && fixed == 0 && P.pair_n + P.ng <= P.pair_cap;
+  MGRun R;
+  R.net = P.prec == 2 && P.net && planar && fixed == 0;
+  R.record = P.collecting && planar && fixed == 0 && P.pair_n + P.ng <= P.pair_cap;
   const MGLev &L0 = P.h64.lev[0];
   const size_t pl = (size_t)P.nH * P.nW;
   mg_zero(st, n, P.x);
@@ -550,12 +579,12 @@ void mg_solve(MGPlan &P, hipStream_t st, int fixed) {
   project(P, st, P.r);
   mg_dot<double, double>(st, P.ng, P.gsize, P.r, P.r, nullptr, nullptr, P.part, P.nchunk, P.bb, P.done, nullptr);
   mg_flags(st, P.ng, P.bb, P.bb, tol2, P.done, P.left);
-  if (record)  // the input half of this solve's training pairs
+  if (R.record)  // the input half of this solve's training pairs
     launch_net_pair_in(st, P.ng, L0.na, L0.ns, P.nH, P.nW, P.r, P.bb, P.nact_plane, P.h64.lam, P.lam_ref,
                        P.pair_x + (size_t)P.pair_n * pl * 2);
   if (mixed) mg_to_float(st, n, P.r, P.rf);
   const double *rr0 = P.bb;  // r.r of the residual the first preconditioning sees
-  if (net && P.guess) {
+  if (R.net && P.guess) {
     // initial guess x0 = the network's answer for b, then r = b - M x0 (the
     // CG update with p = x0 and the step length rz / pq forced to 1)
     launch_net_in(st, P.ng, L0.na, L0.ns, P.nH, P.nW, P.rf, P.bb, P.nact_plane, P.h64.lam, P.lam_ref, P.net->input());
@@ -569,47 +598,47 @@ void mg_solve(MGPlan &P, hipStream_t st, int fixed) {
     mg_flags(st, P.ng, P.rr, P.bb, tol2, P.done, P.left);
     rr0 = P.rr;
   }
-  if (net) {
+  if (R.net) {
     mg_izero(st, 1, P.stall);
     SNAC_HIP(hipMemcpyAsync(P.rr_prev, rr0, 2 * P.ng * sizeof(double), hipMemcpyDeviceToDevice, st));
   }
-  int cur = 0;
-  precondition(P, st, P.rz[cur], nullptr, net, rr0);
-  const int nit = fixed > 0 ? std::min(fixed, P.maxiter) : P.maxiter;
-  int it = 0;
-  for (it = 1; it <= nit; it++) {
-    mg_apply_dot(st, P.h64.lev[0], P.h64.lam, P.p, P.q, P.ng, P.gsize, P.part, P.nchunk, P.pq, P.done);
-    // the update with r.r and, for the flexible form, r_new . z_old in the second slot
-    mg_cg_update_dot(st, P.ng, P.gsize, P.rz[cur], P.pq, P.x, P.p, P.r, P.q, mixed ? P.rf : nullptr,
-                     mixed ? P.zf : nullptr, P.part, P.nchunk, P.rr, P.done);
-    mg_flags(st, P.ng, P.rr, P.bb, tol2, P.done, P.left);
-    if (net && P.fallback) launch_net_stall(st, P.ng, P.rr, P.rr_prev, P.done, P.stall_ratio * P.stall_ratio, P.stall);
-    if (fixed == 0) {
-      SNAC_HIP(hipMemcpyAsync(P.left_host, P.left, sizeof(int), hipMemcpyDeviceToHost, st));
-      if (net && P.fallback) SNAC_HIP(hipMemcpyAsync(P.stall_host, P.stall, sizeof(int), hipMemcpyDeviceToHost, st));
-      SNAC_HIP(hipStreamSynchronize(st));
-      if (*P.left_host == 0) break;
-      if (net && P.fallback && *P.stall_host) {  // the network stopped helping: V-cycles from here on
-        net = false;
-        P.fallbacks++;
-      }
-    } else if (it == nit) {
-      break;
-    }
-    precondition(P, st, P.rz[1 - cur], P.rz[cur], net, P.rr);
-    cur = 1 - cur;
+  precondition(P, st, P.rz[R.cur], nullptr, R.net, rr0);
+  if (fixed > 0) {  // captured: counted when the host resumes it (mg_continue)
+    const int nit = std::min(fixed, P.maxiter);
+    while (R.it < nit) mg_iteration(P, st, R);
+    return R;
   }
-  if (record) {  // the target half: the solution, scaled like the input
+  P.n_solves++;
+  while (R.it < P.maxiter) {
+    mg_iteration(P, st, R);
+    if (R.net && P.fallback) SNAC_HIP(hipMemcpyAsync(P.stall_host, P.stall, sizeof(int), hipMemcpyDeviceToHost, st));
+    if (groups_left(P, st) == 0) break;
+    if (R.net && P.fallback && *P.stall_host) {  // the network stopped helping: V-cycles from here on
+      R.net = false;
+      P.fallbacks++;
+    }
+  }
+  if (*P.left_host > 0) P.n_unconverged++;
+  if (R.record) {  // the target half: the solution, scaled like the input
     launch_net_pair_out(st, P.ng, L0.na, L0.ns, P.nH, P.nW, P.x, P.bb, P.nact_plane, P.pair_t + (size_t)P.pair_n * pl);
     P.pair_n += P.ng;
   }
-  if (fixed > 0) {
-    mg_accum_max(st, P.unconv, P.left);  // groups still above the tolerance
-    P.last_iters = nit;
-  } else {
-    P.last_iters = std::min(it, P.maxiter);
-    P.max_seen = std::max(P.max_seen, P.last_iters);
+  P.last_iters = R.it;
+  P.max_seen = std::max(P.max_seen, P.last_iters);
+  return R;
+}
+
+void mg_continue(MGPlan &P, hipStream_t st, MGRun &R) {
+  P.n_solves++;
+  int left = groups_left(P, st);
+  if (left > 0 && R.it < P.maxiter) P.n_continued++;
+  while (left > 0 && R.it < P.maxiter) {
+    mg_iteration(P, st, R);
+    left = groups_left(P, st);
   }
+  if (left > 0) P.n_unconverged++;
+  P.last_iters = R.it;
+  P.max_seen = std::max(P.max_seen, P.last_iters);
 }
 
 void mg_stats(MGPlan &P, hipStream_t st, int &iters, double &max_relres) {

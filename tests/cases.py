@@ -102,6 +102,23 @@ def cavity_block(ng, lid=1.0, gr=(0.0, 0.0, 0.0)):
                 bcpre=[[0.0, 0.0], [0.0, 0.0], [0.0, 0.0]])
 
 
+def quadratic_box(ng, lengths=(2.0, 1.0, 1.5), gr=(1.3, 0.8, 1.1)):
+    """A walled box stretched along all three axes (tanh two-end), for the
+    chkdiv closed form: u = x^2, v = y^2, w = z^2 at the faces make every
+    cell's flux differences telescope, so divtot = Lx + Ly + Lz."""
+    B = cavity_block(ng, lid=0.0, gr=gr)
+    B["lmax"] = list(lengths)
+    return B
+
+
+def quadratic_fields(shape, xf, yf, zf):
+    """u = xf^2, v = yf^2, w = zf^2 on the ghosted arrays (ghosts included)."""
+    u = np.zeros(shape) + (xf ** 2)[:, None, None]
+    v = np.zeros(shape) + (yf ** 2)[None, :, None]
+    w = np.zeros(shape) + (zf ** 2)[None, None, :]
+    return u, v, w
+
+
 def _block(ng, lmin, lmax, vel, pre, gr=(0.0, 0.0, 0.0), vals=None, inflow=None):
     """vel / pre: [dir][side] BC characters (shared by the three velocity
     components); vals: {(comp, dir, side): value} for velocity BC values and

@@ -134,3 +134,51 @@ def test_precision_switch_keeps_results(tmp_path):
     for b in range(ctx.nblocks):
         g = ctx.get(b, "pp")[1:-1, 1:-1, 1:-1]
         assert np.linalg.norm(g - ref[b]) / np.linalg.norm(ref[b]) < 1e-10
+
+
+@pytest.mark.parametrize("captured", [0, 2])
+def test_graph_steps_meet_tolerance(tmp_path, captured):
+    """Graph steps on the MG-PCG path end every solve at hypre_tol.  With the
+    captured CG count measured (0) and with it forced far below what the
+    solves need (2), every solve of every step is resumed by the host until
+    it converges, no solve stops above the tolerance, and the fields agree
+    bit for bit with eager steps; the right-hand side is then made harder (a
+    rough velocity field) between replays and the solves still converge."""
+    import snac_amd
+    d, b = cases.write_case(str(tmp_path), cases.bfs_blocks((12, 8), (20, 8, 12), 8), 1000.0)
+    print(f"case bfs graph captured={captured}", flush=True)
+    eager, graph = snac_amd.Context(d, b, 3), snac_amd.Context(d, b, 3)
+    eager.set_graph(False)
+    graph.set_graph_cg_iters(captured)
+    rng = np.random.default_rng(5)
+    for q in range(eager.nblocks):
+        for f in FIELDS3:
+            a = 0.05 * rng.uniform(-1, 1, eager.shape(q, f))
+            eager.set(q, f, a)
+            graph.set(q, f, a)
+    for ctx in (eager, graph):
+        ctx.init_bounds()
+        ctx.solver_counters()
+    dt = 2e-3
+    for rep in range(4):
+        if rep == 2:  # a much rougher field between replays
+            for q in range(eager.nblocks):
+                for f in FIELDS3:
+                    a = eager.get(q, f) + rng.uniform(-1, 1, eager.shape(q, f))
+                    eager.set(q, f, a)
+                    graph.set(q, f, a)
+        eager.step(dt)
+        graph.step(dt)
+        solves, resumed, unconverged = graph.solver_counters()
+        it, relres = graph.solver_stats()
+        print(f"step {rep}: {solves} solves, {resumed} resumed, {unconverged} unconverged, last {it} its "
+              f"relres {relres:.2e}", flush=True)
+        assert solves == 3 and unconverged == 0 and relres <= 1e-13
+        if captured == 2 and rep > 0:
+            assert resumed == 3
+        assert eager.solver_counters()[2] == 0
+        for q in range(eager.nblocks):
+            for f in FIELDS3 + ("p",):
+                np.testing.assert_array_equal(graph.get(q, f), eager.get(q, f), err_msg=f"step {rep} block {q} {f}")
+    eager.close()
+    graph.close()

@@ -241,3 +241,22 @@ def test_projection_makes_divergence_free(tmp_path):
     oc.bounduvw(True)
     tot, mx = oc.chkdiv()
     assert mx < 1e-12
+
+
+def test_divtot_closed_form(tmp_path):
+    """chkdiv (chkdiv.f90:31-43) on u = x^2, v = y^2, w = z^2 at the faces of
+    a box stretched along every axis: the divergence of cell (i,j,k) is
+    (xf_i^2 - xf_{i-1}^2)/dxf_i + ..., so the volume-weighted sum telescopes
+    to divtot = (Lx^2 Ly Lz + Lx Ly^2 Lz + Lx Ly Lz^2) / vol_all = Lx+Ly+Lz,
+    and divmax = (xf_n + xf_{n-1}) + (yf_n + yf_{n-1}) + (zf_n + zf_{n-1}).
+    A wrong weight (dxc for dxf, a missing factor or 1/vol_all) fails."""
+    L = (2.0, 1.0, 1.5)
+    oc = small_case(tmp_path, [cases.quadratic_box((12, 10, 8), L)], 0)
+    xf, yf, zf = oc.metric(0, 7), oc.metric(0, 9), oc.metric(0, 11)
+    for f, a in zip(("u", "v", "w"), cases.quadratic_fields(oc.shape(0, "u"), xf, yf, zf)):
+        oc.set(0, f, a)
+    tot, mx = oc.chkdiv()
+    assert tot == pytest.approx(sum(L), rel=1e-13)
+    n = (12, 10, 8)
+    expect = sum(f[q] + f[q - 1] for f, q in zip((xf, yf, zf), n))
+    assert mx == pytest.approx(expect, rel=1e-13)
