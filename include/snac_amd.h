@@ -77,14 +77,20 @@ int snac_ctx_get_params(const snac_ctx *ctx, double *dparams, int *iparams);
  * mode at ||r|| <= hypre_tol ||b|| or after hypre_maxiter iterations, the
  * &hypre parameters of param.f90:101, solver.f90:389-451)                   */
 int snac_ctx_set_solver(snac_ctx *ctx, int solver);
-/* Arithmetic of the pressure solve (no counterpart in the reference, whose
+/* Arithmetic of the time step (no counterpart in the reference, whose
  * precision is the compile-time real(rp), types.f90:17-19):
- * SNAC_PREC_FP64 (the default) runs everything in FP64; SNAC_PREC_MIXED
- * runs the MG-PCG V-cycle (line smoothing, restriction, prolongation,
- * coarse solves) in FP32 under a flexible CG whose vectors, operator, dot
- * products and null-space removal stay FP64, so the solve still stops at
- * hypre_tol.  The stencils and the direct tensor-product solver stay FP64.
- * SNAC_PREC_LEARNED preconditions the same flexible CG with the trained
+ * SNAC_PREC_FP64 (the default) runs everything in FP64.  SNAC_PREC_MIXED
+ * evaluates the momentum right-hand side of rk_mom (mom.f90:112-327) in
+ * FP32 and keeps the old tendencies dudtrko/dvdtrko/dwdtrko in FP32, while
+ * the RK3 accumulation, the pressure gradient and the body forces of the
+ * update stay FP64 (the velocities are FP64 fields); on the MG-PCG path it
+ * also runs the V-cycle (line smoothing, restriction, prolongation,
+ * coarse solves) in FP32 under a flexible CG whose residuals, operator,
+ * dot products and null-space removal stay FP64, so the solve still stops
+ * at hypre_tol.  fillps, correc, updt_pressure, the ghost fills and the
+ * direct tensor-product solver stay FP64 in every mode.
+ * SNAC_PREC_LEARNED (FP32 momentum right-hand side as MIXED) preconditions
+ * the same flexible CG with the trained
  * network (snac_learned_train) on the FP32 residual of each mode, falling
  * back to the FP32 V-cycle for the rest of a solve when some mode's residual
  * reduction stalls; the FP64 residual still decides convergence (FFT path;
@@ -277,8 +283,9 @@ int snac_get_timing(snac_ctx *ctx, double *ms, int *count);
  * 64x4 or 32x8; 7..10 k-march with the planes staged in LDS, 64x4 tiles
  * over 16/32 planes (7, 8) or 64x8 tiles over 16/32 planes (9, 10);
  * 11..14 as 7..10 with non-temporal hints on the read-once / write-once
- * streams; 15, 16 64x8 tiles over 64 planes without / with the hints)
- * and the best times in ms.  No
+ * streams; 15, 16 64x8 tiles over 64 planes without / with the hints;
+ * then the FP32 rk_mom's variant of the mixed-precision modes, 7..16 as
+ * above) and the best times in ms.  No
  * counterpart in the reference (its FFTW plans use FFTW_ESTIMATE).          */
 int snac_get_autotune(snac_ctx *ctx, int *choices, double *ms, int n);
 /* overrides an autotuned kernel variant: SNAC_KV_RK_MOM (0..16, as above),
@@ -288,9 +295,10 @@ int snac_get_autotune(snac_ctx *ctx, int *choices, double *ms, int n);
  * one launch per phase of the reference's fill order; 1, the default: each
  * fill composed at set-up into one launch that computes every ghost cell's
  * final value through the same chain of operations, where the fill allows
- * it); every variant computes the same result.  No counterpart in the
+ * it); every variant computes the same result; SNAC_KV_RK_MOM32 (7..16):
+ * the FP32 rk_mom of the mixed-precision modes.  No counterpart in the
  * reference.                                                                */
-enum { SNAC_KV_RK_MOM = 0, SNAC_KV_TRIDIAG = 1, SNAC_KV_GHOST = 2 };
+enum { SNAC_KV_RK_MOM = 0, SNAC_KV_TRIDIAG = 1, SNAC_KV_GHOST = 2, SNAC_KV_RK_MOM32 = 3 };
 int snac_set_kernel_variant(snac_ctx *ctx, int kernel, int variant);
 
 /* ---- building blocks, exposed for their own parity tests ----

@@ -316,10 +316,14 @@ class Context:
     PRECISIONS = {"fp64": 0, "mixed": 1, "learned": 2}
 
     def set_precision(self, mode):
-        """'fp64' (default) or 'mixed': the MG-PCG V-cycle in FP32 under a
-        flexible CG whose vectors, operator, dot products and null-space
-        removal stay FP64 (same stopping tolerance).  Stencils and the direct
-        tensor-product solver stay FP64."""
+        """'fp64' (default), 'mixed' or 'learned'.  mixed: the momentum
+        right-hand side of rk_mom in FP32 with FP32 old tendencies (the RK3
+        accumulation, pressure gradient and forcing stay FP64) and, on the
+        MG-PCG path, the V-cycle in FP32 under a flexible CG whose residuals,
+        operator, dot products and null-space removal stay FP64 (same
+        stopping tolerance).  fillps, correc, the ghost fills and the direct
+        tensor-product solver stay FP64.  learned: as mixed, with the trained
+        network preconditioning the CG."""
         _check(lib.snac_set_precision(self.h, self.PRECISIONS[mode]), "set_precision")
 
     def learned_collect(self, npairs):
@@ -374,7 +378,8 @@ class Context:
     def autotune(self):
         """(choices, best ms) of the autotuner: transform tile per call site
         (forward, then inverse), then the tridiagonal variant (both on the
-        tensor-product solver path only), then the rk_mom variant."""
+        tensor-product solver path only), then the rk_mom variant, then the
+        FP32 rk_mom variant of the mixed-precision modes."""
         ch = np.zeros(16, np.int32)
         ms = np.zeros(16)
         _check(lib.snac_get_autotune(self.h, ch, ms, 16), "get_autotune")
@@ -384,6 +389,11 @@ class Context:
     def set_rk_variant(self, v):
         """Overrides the autotuned rk_mom kernel variant (0..16)."""
         _check(lib.snac_set_kernel_variant(self.h, 0, int(v)), "set_kernel_variant")
+
+    def set_rk32_variant(self, v):
+        """Overrides the autotuned FP32 rk_mom kernel variant (7..16) of the
+        mixed-precision modes."""
+        _check(lib.snac_set_kernel_variant(self.h, 3, int(v)), "set_kernel_variant")
 
     def set_tridiag_variant(self, v):
         """Overrides the autotuned tridiagonal kernel variant (0..3)."""

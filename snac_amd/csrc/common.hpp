@@ -85,6 +85,18 @@ void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *
                    const double *w, const double *p, double *un, double *vn, double *wn, double *dtu,
                    double *dtv, double *dtw, double f1, double f2, double f12, double visc, double bfx,
                    double bfy, double bfz, const double *s1, const double gb[3], int variant, bool impdiff);
+// Mixed precision: the same substage with the right-hand side evaluated in
+// FP32 from FP32 copies of u, v, w staged in LDS and the old tendencies
+// stored in FP32 (same [k][j][component][i] layout, in floats); the RK3
+// accumulation, the pressure gradient and the body forces stay FP64.  LDS
+// variants 7..16 only.
+void launch_rk_mom_f32(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v,
+                       const double *w, const double *p, double *un, double *vn, double *wn, float *dtu, float *dtv,
+                       float *dtw, double f1, double f2, double f12, double visc, double bfx, double bfy, double bfz,
+                       const double *s1, const double gb[3], int variant, bool impdiff);
+// element-wise conversions between FP64 and FP32 arrays of n values
+void launch_cvt_d2f(hipStream_t s, const double *a, float *b, long n);
+void launch_cvt_f2d(hipStream_t s, const float *a, double *b, long n);
 // rk_scal of one scalar: reads sc (ghosted), writes the interior of sn and dso.
 // imp (implicit diffusion, rk.f90:151-202): 0 off; 1 the diffusion out of
 // dsdtrk, half of it left in sn; 2 likewise but the half is taken off by

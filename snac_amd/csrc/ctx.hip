@@ -33,6 +33,7 @@ struct DevBlock {
   double *vel[2][3] = {};       // ping-pong velocity buffers
   double *p = nullptr, *pp = nullptr, *po = nullptr;
   double *dto[3] = {};
+  float *dto32 = nullptr;       // the old tendencies in FP32 (mixed-precision stencils), same layout
   double *rhsp[6] = {};
   double *vin[3][3][2] = {};
   std::vector<double *> sv[2];  // scalar ping-pong buffers [parity][iscal]
@@ -93,6 +94,9 @@ struct snac_ctx {
   std::vector<snac::DevBlock> blk;
   int rk_variant = 0;     // rk_mom kernel variant (autotuned)
   double rk_tune_ms = 0.0;
+  int rk32_variant = 16;  // the FP32 rk_mom's (LDS variants 7..16, autotuned)
+  double rk32_tune_ms = 0.0;
+  bool st32 = false;      // momentum right-hand side in FP32 (precision MIXED / LEARNED)
   int cur = 0;   // parity of the current velocity buffers
   int scur = 0;  // parity of the current scalar buffers (flips once per substage)
   snac::Program prog_uvw[2][2], prog_bound_only[2][2], prog_inflow[2][2], prog_p, prog_pp, prog_end[2];
@@ -853,10 +857,17 @@ void rk_mom(snac_ctx *c, int irk, double dt) {
   const double gb[3] = {cs.gacc[0] * cs.beta, cs.gacc[1] * cs.beta, cs.gacc[2] * cs.beta};
   const bool buoy = cs.nscal > 0 && (gb[0] != 0.0 || gb[1] != 0.0 || gb[2] != 0.0);
   int nx = 1 - c->cur;
-  for (auto &D : c->blk)
-    launch_rk_mom(c->stream, D.g, D.m, D.vel[c->cur][0], D.vel[c->cur][1], D.vel[c->cur][2], D.p, D.vel[nx][0],
-                  D.vel[nx][1], D.vel[nx][2], D.dto[0], D.dto[1], D.dto[2], f1, f2, f12, cs.visc, cs.bforce[0],
-                  cs.bforce[1], cs.bforce[2], buoy ? D.sv[c->scur][0] : nullptr, gb, c->rk_variant, c->impdiff);
+  for (auto &D : c->blk) {
+    if (c->st32)
+      launch_rk_mom_f32(c->stream, D.g, D.m, D.vel[c->cur][0], D.vel[c->cur][1], D.vel[c->cur][2], D.p,
+                        D.vel[nx][0], D.vel[nx][1], D.vel[nx][2], D.dto32, D.dto32 + D.g.nx, D.dto32 + 2 * D.g.nx,
+                        f1, f2, f12, cs.visc, cs.bforce[0], cs.bforce[1], cs.bforce[2],
+                        buoy ? D.sv[c->scur][0] : nullptr, gb, c->rk32_variant, c->impdiff);
+    else
+      launch_rk_mom(c->stream, D.g, D.m, D.vel[c->cur][0], D.vel[c->cur][1], D.vel[c->cur][2], D.p, D.vel[nx][0],
+                    D.vel[nx][1], D.vel[nx][2], D.dto[0], D.dto[1], D.dto[2], f1, f2, f12, cs.visc, cs.bforce[0],
+                    cs.bforce[1], cs.bforce[2], buoy ? D.sv[c->scur][0] : nullptr, gb, c->rk_variant, c->impdiff);
+  }
   c->cur = nx;
 }
 
@@ -880,7 +891,24 @@ void autotune_rk(snac_ctx *c) {
   }
   c->rk_variant = bv;
   c->rk_tune_ms = best;
+  // the FP32 form (mixed precision) among the LDS variants
+  best = 1e300;
+  bv = 16;
+  c->st32 = true;
+  for (int v = 7; v <= 16; v++) {
+    c->rk32_variant = v;
+    const int cur0 = c->cur;
+    double ms = time_stage(c, [&] {
+      rk_mom(c, 1, 1e-3);
+      c->cur = cur0;
+    });
+    if (ms < best) { best = ms; bv = v; }
+  }
+  c->st32 = false;
+  c->rk32_variant = bv;
+  c->rk32_tune_ms = best;
   for (auto &D : c->blk) {
+    SNAC_HIP(hipMemsetAsync(D.dto32, 0, 3 * D.nin * sizeof(float), c->stream));
     for (int par = 0; par < 2; par++)
       for (int q = 0; q < 3; q++) SNAC_HIP(hipMemsetAsync(D.vel[par][q] - 15, 0, (D.ntot + 16) * sizeof(double), c->stream));
     SNAC_HIP(hipMemsetAsync(D.dto[0], 0, 3 * D.nin * sizeof(double), c->stream));
@@ -1105,9 +1133,9 @@ void reduce(snac_ctx *c, bool div, double out[2]) {
 // transform differently round differently).
 struct TuneChoice {
   std::vector<int> fwd, bwd;
-  int tvariant = 1, rk_variant = 0;
+  int tvariant = 1, rk_variant = 0, rk32_variant = 16;
   std::vector<double> tune_ms;
-  double rk_ms = 0.0;
+  double rk_ms = 0.0, rk32_ms = 0.0;
 };
 std::mutex g_tune_mu;
 std::map<std::string, TuneChoice> g_tune;
@@ -1142,6 +1170,8 @@ void tune_or_recall(snac_ctx *c) {
     }
     c->rk_variant = tc.rk_variant;
     c->rk_tune_ms = tc.rk_ms;
+    c->rk32_variant = tc.rk32_variant;
+    c->rk32_tune_ms = tc.rk32_ms;
     return;
   }
   autotune(c);
@@ -1155,6 +1185,8 @@ void tune_or_recall(snac_ctx *c) {
   }
   tc.rk_variant = c->rk_variant;
   tc.rk_ms = c->rk_tune_ms;
+  tc.rk32_variant = c->rk32_variant;
+  tc.rk32_ms = c->rk32_tune_ms;
   g_tune.emplace(key, tc);
 }
 
@@ -1193,6 +1225,7 @@ void create(snac_ctx *c) {
     // dudtrko, dvdtrko, dwdtrko row by row in one array ([k][j][component][i])
     D.dto[0] = dalloc<double>(c, 3 * D.nin);
     for (int q = 1; q < 3; q++) D.dto[q] = D.dto[0] + q * (long)D.g.nx;
+    D.dto32 = dalloc<float>(c, 3 * D.nin);
     for (int is = 0; is < cs.nscal; is++) {
       for (int par = 0; par < 2; par++) D.sv[par].push_back(dalloc_field(c, D.ntot));
       D.sdo.push_back(dalloc<double>(c, D.nin));
@@ -1385,6 +1418,17 @@ int snac_set_precision(snac_ctx *c, int mode) {
     if (mode == SNAC_PREC_LEARNED && !(c->mg_planned && c->mg.net))
       throw std::runtime_error("the learned preconditioner needs a trained network (snac_learned_train)");
     if (c->device >= 0) SNAC_HIP(hipStreamSynchronize(c->stream));
+    // the momentum right-hand side runs in FP32 in the mixed and learned
+    // modes; the old tendencies move to the other precision's array
+    const bool st32 = mode != SNAC_PREC_FP64;
+    if (st32 != c->st32 && c->device >= 0) {
+      for (auto &D : c->blk) {
+        if (st32) launch_cvt_d2f(c->stream, D.dto[0], D.dto32, 3 * D.nin);
+        else launch_cvt_f2d(c->stream, D.dto32, D.dto[0], 3 * D.nin);
+      }
+      SNAC_HIP(hipStreamSynchronize(c->stream));
+    }
+    c->st32 = st32;
     c->prec = mode;
     c->mg.prec = mode;
     c->mg.fixed_iters = 0;  // the graph's CG count is measured again
@@ -1515,10 +1559,14 @@ void copy_field(snac_ctx *c, int b, int fid, double *dst_host, const double *src
   DevBlock &D = c->blk.at(b);
   const bool ghosted = (fid >= 0 && fid < 6) || (fid >= SNAC_SCAL0 && (fid - SNAC_SCAL0) % 2 == 0);
   if (fid >= SNAC_DUDTRKO && fid <= SNAC_DWDTRKO) {
-    // one component's rows, 3 nx apart on the device, contiguous on the host
+    // one component's rows, 3 nx apart on the device, contiguous on the host;
+    // with FP32 stencils the tendencies live in dto32 and pass through the
+    // FP64 array
     size_t w = (size_t)D.g.nx * sizeof(double), pitch = 3 * w, rows = (size_t)D.g.ny * D.g.nz;
+    if (c->st32) launch_cvt_f2d(c->stream, D.dto32, D.dto[0], 3 * D.nin);
     if (src_host) SNAC_HIP(hipMemcpy2DAsync(d, pitch, src_host, w, w, rows, hipMemcpyHostToDevice, c->stream));
     else SNAC_HIP(hipMemcpy2DAsync(dst_host, w, d, pitch, w, rows, hipMemcpyDeviceToHost, c->stream));
+    if (c->st32 && src_host) launch_cvt_d2f(c->stream, D.dto[0], D.dto32, 3 * D.nin);
   } else if (ghosted) {
     size_t w = (size_t)(D.g.nx + 2) * sizeof(double), pitch = (size_t)D.g.sx * sizeof(double);
     size_t rows = (size_t)(D.g.ny + 2) * (D.g.nz + 2);
@@ -1969,6 +2017,8 @@ int snac_get_autotune(snac_ctx *c, int *choices, double *ms, int n) {
     if (t.ok) ch.push_back(t.tvariant);
     ch.push_back(c->rk_variant);
     tm.push_back(c->rk_tune_ms);
+    ch.push_back(c->rk32_variant);
+    tm.push_back(c->rk32_tune_ms);
     for (int q = 0; q < n; q++) {
       if (choices) choices[q] = q < (int)ch.size() ? ch[q] : -2;
       if (ms) ms[q] = q < (int)tm.size() ? tm[q] : 0.0;
@@ -1982,6 +2032,9 @@ int snac_set_kernel_variant(snac_ctx *c, int kernel, int variant) {
     if (kernel == SNAC_KV_RK_MOM) {
       if (variant < 0 || variant > 16) throw std::runtime_error("rk_mom variant must be 0..16");
       c->rk_variant = variant;
+    } else if (kernel == SNAC_KV_RK_MOM32) {
+      if (variant < 7 || variant > 16) throw std::runtime_error("FP32 rk_mom variant must be 7..16");
+      c->rk32_variant = variant;
     } else if (kernel == SNAC_KV_TRIDIAG) {
       if (!c->tp.ok) throw std::runtime_error("no tensor-product plan on this case");
       const int m = c->tp.G[c->tp.s];

@@ -263,29 +263,42 @@ __global__ __launch_bounds__(BX *BY) void k_rk_mom_kz(Geo g, Metrics m, const do
 // computed and land in the ring slot of plane k-1 after it, so their
 // latency overlaps the arithmetic.
 constexpr int LX = 66;  // row pitch of an LDS plane: 64 + 2 halo cells
-template <bool NT>
-__device__ __forceinline__ double ld_nt(const double *p) {
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_nt(const T *p) {
   if (NT) return __builtin_nontemporal_load(p);
   return *p;
 }
-template <bool NT>
-__device__ __forceinline__ void st_nt(double *p, double v) {
+template <bool NT, typename T>
+__device__ __forceinline__ void st_nt(T *p, T v) {
   if (NT) __builtin_nontemporal_store(v, p);
   else *p = v;
 }
+// R: arithmetic of the momentum right-hand side (advection and diffusion)
+// and storage type of the old tendencies.  double: the reference's FP64,
+// in its expression order.  float (mixed precision): the right-hand side is
+// evaluated in FP32 on offsets from the cell's own values -- every
+// neighbour value enters as (neighbour - centre), formed in FP64 and
+// rounded once -- with the advective flux differences rearranged so that
+// their O(u^2) parts cancel exactly, e.g. for the x-flux of u
+//   (uc+u_m) u_m - (uc+u_p) u_p = (d_m - d_p) (3 uc + d_m + d_p),
+// d = u_nb - uc; rounding full-size velocities to FP32 would instead put
+// an error of |u|^2 * 2^-24 into terms of size |u| |du|.  The RK3
+// accumulation u + f1*rhs + f2*rhs_old, the pressure gradient and the
+// body forces run in FP64.
 // NTS: the read-once / write-once streams (dudtrko in and out, the new
 // velocities) carry the non-temporal hint, leaving the caches to the
 // stencil's planes
-template <bool BUOY, bool IMP, int KB, int BY, bool NTS>
+template <typename R, bool BUOY, bool IMP, int KB, int BY, bool NTS>
 __global__ __launch_bounds__(64 * BY) void k_rk_mom_lds(Geo g, Metrics m, const double *__restrict__ u,
                                                         const double *__restrict__ v, const double *__restrict__ w,
                                                         const double *__restrict__ p, double *__restrict__ un,
                                                         double *__restrict__ vn, double *__restrict__ wn,
-                                                        double *__restrict__ dtu, double *__restrict__ dtv,
-                                                        double *__restrict__ dtw, double f1, double f2, double f12,
-                                                        double visc, double bfx, double bfy, double bfz,
+                                                        R *__restrict__ dtu, R *__restrict__ dtv,
+                                                        R *__restrict__ dtw, double f1, double f2, double f12,
+                                                        double visc_, double bfx, double bfy, double bfz,
                                                         const double *__restrict__ s1, double gbx, double gby,
                                                         double gbz) {
+  constexpr bool F32 = sizeof(R) == 4;
   constexpr int NT = 64 * BY, PL = LX * (BY + 2), NL = (PL + NT - 1) / NT;
   __shared__ double sh[4][3][PL];  // field (u, v, w, p) x ring slot x plane
   int bx, by, bz;
@@ -338,13 +351,16 @@ __global__ __launch_bounds__(64 * BY) void k_rk_mom_lds(Geo g, Metrics m, const 
   }
   __syncthreads();
   const bool live = i <= g.nx && j <= g.ny;
-  double dxfi = 0, dxfip = 0, dyfj = 0, dyfjp = 0, ixci = 0, ixcim = 0, ixfi = 0, ixfip = 0;
-  double iycj = 0, iycjm = 0, iyfj = 0, iyfjp = 0;
+  R dxfi = 0, dxfip = 0, dyfj = 0, dyfjp = 0, ixci = 0, ixcim = 0, ixfi = 0, ixfip = 0;
+  R iycj = 0, iycjm = 0, iyfj = 0, iyfjp = 0;
+  double ixci_d = 0, iycj_d = 0;  // the pressure gradient's, FP64
   if (live) {
-    dxfi = m.dxf[i]; dxfip = m.dxf[i + 1]; dyfj = m.dyf[j]; dyfjp = m.dyf[j + 1];
-    ixci = m.idxc[i]; ixcim = m.idxc[i - 1]; ixfi = m.idxf[i]; ixfip = m.idxf[i + 1];
-    iycj = m.idyc[j]; iycjm = m.idyc[j - 1]; iyfj = m.idyf[j]; iyfjp = m.idyf[j + 1];
+    dxfi = (R)m.dxf[i]; dxfip = (R)m.dxf[i + 1]; dyfj = (R)m.dyf[j]; dyfjp = (R)m.dyf[j + 1];
+    ixci_d = m.idxc[i]; iycj_d = m.idyc[j];
+    ixci = (R)ixci_d; ixcim = (R)m.idxc[i - 1]; ixfi = (R)m.idxf[i]; ixfip = (R)m.idxf[i + 1];
+    iycj = (R)iycj_d; iycjm = (R)m.idyc[j - 1]; iyfj = (R)m.idyf[j]; iyfjp = (R)m.idyf[j + 1];
   }
+  const R visc = (R)visc_, qr = (R)0.25;
   const int L = (ty + 1) * LX + tx + 1;  // this cell in an LDS plane
   const int LXP = L + 1, LXM = L - 1, LYP = L + LX, LYM = L - LX;
   long c = gidx(i, j, k0, sx, sy);
@@ -359,63 +375,113 @@ __global__ __launch_bounds__(64 * BY) void k_rk_mom_lds(Geo g, Metrics m, const 
       const double *U0 = sh[0][s0], *V0 = sh[1][s0], *W0 = sh[2][s0], *P0 = sh[3][s0];
       const double *Um = sh[0][sm_], *Vm = sh[1][sm_], *Wm = sh[2][sm_];
       const double *Up = sh[0][sp_], *Vp = sh[1][sp_], *Wp = sh[2][sp_], *Pp = sh[3][sp_];
-      const double uc = U0[L], vc = V0[L], wc = W0[L], pc = P0[L];
-      const double um = Um[L], vm = Vm[L], wm = Wm[L];
-      const double up = Up[L], vp = Vp[L], wp = Wp[L], pz = Pp[L];
-      const double dzfk = m.dzf[k], dzfkp = m.dzf[k + 1];
-      const double izck = m.idzc[k], izckm = m.idzc[k - 1], izfk = m.idzf[k], izfkp = m.idzf[k + 1];
-      const double uxp = U0[LXP], uxm = U0[LXM], uyp = U0[LYP], uym = U0[LYM];
-      const double vxp = V0[LXP], vxm = V0[LXM], vyp = V0[LYP], vym = V0[LYM];
-      const double wxp = W0[LXP], wxm = W0[LXM], wyp = W0[LYP], wym = W0[LYM];
-      double a, b, cc;
-      {
-        double dudxp = (uxp - uc) * ixfip, dudxm = (uc - uxm) * ixfi;
-        double dudyp = (uyp - uc) * iycj, dudym = (uc - uym) * iycjm;
-        double dudzp = (up - uc) * izck, dudzm = (uc - um) * izckm;
-        a = visc * (dudxp - dudxm) * ixci + visc * (dudyp - dudym) * iyfj + visc * (dudzp - dudzm) * izfk;
-        double dvdxp = (vxp - vc) * ixci, dvdxm = (vc - vxm) * ixcim;
-        double dvdyp = (vyp - vc) * iyfjp, dvdym = (vc - vym) * iyfj;
-        double dvdzp = (vp - vc) * izck, dvdzm = (vc - vm) * izckm;
-        b = visc * (dvdxp - dvdxm) * ixfi + visc * (dvdyp - dvdym) * iycj + visc * (dvdzp - dvdzm) * izfk;
-        double dwdxp = (wxp - wc) * ixci, dwdxm = (wc - wxm) * ixcim;
-        double dwdyp = (wyp - wc) * iycj, dwdym = (wc - wym) * iycjm;
-        double dwdzp = (wp - wc) * izfkp, dwdzm = (wc - wm) * izfk;
-        cc = visc * (dwdxp - dwdxm) * ixfi + visc * (dwdyp - dwdym) * iyfj + visc * (dwdzp - dwdzm) * izck;
+      const double uc_ = U0[L], vc_ = V0[L], wc_ = W0[L], pc = P0[L], pz = Pp[L];
+      const double izck_d = m.idzc[k];
+      const R dzfk = (R)m.dzf[k], dzfkp = (R)m.dzf[k + 1];
+      const R izck = (R)izck_d, izckm = (R)m.idzc[k - 1], izfk = (R)m.idzf[k], izfkp = (R)m.idzf[k + 1];
+      R a, b, cc;
+      R ad = 0, bd = 0, cd = 0;
+      if constexpr (F32) {
+        // offsets from the centre values, one rounding each
+        auto d = [](double nb, double ctr) { return (float)(nb - ctr); };
+        const float uc = (float)uc_, vc = (float)vc_, wc = (float)wc_;
+        const float uxp = d(U0[LXP], uc_), uxm = d(U0[LXM], uc_), uyp = d(U0[LYP], uc_), uym = d(U0[LYM], uc_);
+        const float uzp = d(Up[L], uc_), uzm = d(Um[L], uc_);
+        const float uxmyp = d(U0[LXM + LX], uc_), uxmzp = d(Up[LXM], uc_);
+        const float vxp = d(V0[LXP], vc_), vxm = d(V0[LXM], vc_), vyp = d(V0[LYP], vc_), vym = d(V0[LYM], vc_);
+        const float vzp = d(Vp[L], vc_), vzm = d(Vm[L], vc_);
+        const float vymxp = d(V0[LYM + 1], vc_), vymzp = d(Vp[LYM], vc_);
+        const float wxp = d(W0[LXP], wc_), wxm = d(W0[LXM], wc_), wyp = d(W0[LYP], wc_), wym = d(W0[LYM], wc_);
+        const float wzp = d(Wp[L], wc_), wzm = d(Wm[L], wc_);
+        const float wzmxp = d(Wm[LXP], wc_), wzmyp = d(Wm[LYP], wc_);
+        // diffusion, momx_d / momy_d / momz_d (mom.f90:221-327)
+        a = visc * (uxp * ixfip + uxm * ixfi) * ixci + visc * (uyp * iycj + uym * iycjm) * iyfj +
+            visc * (uzp * izck + uzm * izckm) * izfk;
+        b = visc * (vxp * ixci + vxm * ixcim) * ixfi + visc * (vyp * iyfjp + vym * iyfj) * iycj +
+            visc * (vzp * izck + vzm * izckm) * izfk;
+        cc = visc * (wxp * ixci + wxm * ixcim) * ixfi + visc * (wyp * iycj + wym * iycjm) * iyfj +
+             visc * (wzp * izfkp + wzm * izfk) * izck;
+        if (IMP) { ad = a; bd = b; cd = cc; a = 0; b = 0; cc = 0; }
+        // volume-weighted advection, momx/y/z_a_vv (mom.f90:112-219): each
+        // pair of face fluxes F_m u_m - F_p u_p as uc (F_m - F_p) + F_m d_m - F_p d_p
+        {
+          const float sxf = dxfi + dxfip, syf = dyfj + dyfjp, szf = dzfk + dzfkp;
+          const float fuu = qr * dyfj * dzfk * (uxm - uxp) * (3.f * uc + uxm + uxp);
+          const float Vp_ = vc * sxf + vxp * dxfip, Vm_ = vc * sxf + vym * dxfi + vymxp * dxfip;
+          const float fuv = qr * dzfk * (uc * (vym * dxfi + (vymxp - vxp) * dxfip) + Vm_ * uym - Vp_ * uyp);
+          const float Wp_ = wc * sxf + wxp * dxfip, Wm_ = wc * sxf + wzm * dxfi + wzmxp * dxfip;
+          const float fuw = qr * dyfj * (uc * (wzm * dxfi + (wzmxp - wxp) * dxfip) + Wm_ * uzm - Wp_ * uzp);
+          a = a + (fuu + fuv + fuw) * (ixci * iyfj * izfk);
+          const float Up2 = uc * syf + uyp * dyfjp, Um2 = uc * syf + uxm * dyfj + uxmyp * dyfjp;
+          const float fvu = qr * dzfk * (vc * (uxm * dyfj + (uxmyp - uyp) * dyfjp) + Um2 * vxm - Up2 * vxp);
+          const float fvv = qr * dxfi * dzfk * (vym - vyp) * (3.f * vc + vym + vyp);
+          const float Wp2 = wc * syf + wyp * dyfjp, Wm2 = wc * syf + wzm * dyfj + wzmyp * dyfjp;
+          const float fvw = qr * dxfi * (vc * (wzm * dyfj + (wzmyp - wyp) * dyfjp) + Wm2 * vzm - Wp2 * vzp);
+          b = b + (fvu + fvv + fvw) * (ixfi * iycj * izfk);
+          const float Up3 = uc * szf + uzp * dzfkp, Um3 = uc * szf + uxm * dzfk + uxmzp * dzfkp;
+          const float fwu = qr * dyfj * (wc * (uxm * dzfk + (uxmzp - uzp) * dzfkp) + Um3 * wxm - Up3 * wxp);
+          const float Vp3 = vc * szf + vzp * dzfkp, Vm3 = vc * szf + vym * dzfk + vymzp * dzfkp;
+          const float fwv = qr * dxfi * (wc * (vym * dzfk + (vymzp - vzp) * dzfkp) + Vm3 * wym - Vp3 * wyp);
+          const float fww = qr * dxfi * dyfj * (wzm - wzp) * (3.f * wc + wzm + wzp);
+          cc = cc + (fwu + fwv + fww) * (ixfi * iyfj * izck);
+        }
+      } else {
+        const double uc = uc_, vc = vc_, wc = wc_;
+        const double um = Um[L], vm = Vm[L], wm = Wm[L];
+        const double up = Up[L], vp = Vp[L], wp = Wp[L];
+        const double uxp = U0[LXP], uxm = U0[LXM], uyp = U0[LYP], uym = U0[LYM];
+        const double vxp = V0[LXP], vxm = V0[LXM], vyp = V0[LYP], vym = V0[LYM];
+        const double wxp = W0[LXP], wxm = W0[LXM], wyp = W0[LYP], wym = W0[LYM];
+        {
+          double dudxp = (uxp - uc) * ixfip, dudxm = (uc - uxm) * ixfi;
+          double dudyp = (uyp - uc) * iycj, dudym = (uc - uym) * iycjm;
+          double dudzp = (up - uc) * izck, dudzm = (uc - um) * izckm;
+          a = visc * (dudxp - dudxm) * ixci + visc * (dudyp - dudym) * iyfj + visc * (dudzp - dudzm) * izfk;
+          double dvdxp = (vxp - vc) * ixci, dvdxm = (vc - vxm) * ixcim;
+          double dvdyp = (vyp - vc) * iyfjp, dvdym = (vc - vym) * iyfj;
+          double dvdzp = (vp - vc) * izck, dvdzm = (vc - vm) * izckm;
+          b = visc * (dvdxp - dvdxm) * ixfi + visc * (dvdyp - dvdym) * iycj + visc * (dvdzp - dvdzm) * izfk;
+          double dwdxp = (wxp - wc) * ixci, dwdxm = (wc - wxm) * ixcim;
+          double dwdyp = (wyp - wc) * iycj, dwdym = (wc - wym) * iycjm;
+          double dwdzp = (wp - wc) * izfkp, dwdzm = (wc - wm) * izfk;
+          cc = visc * (dwdxp - dwdxm) * ixfi + visc * (dwdyp - dwdym) * iyfj + visc * (dwdzp - dwdzm) * izck;
+        }
+        // implicit diffusion: the diffusion is kept apart, dudtrk is the advection
+        // alone (rk.f90:50-59)
+        if (IMP) { ad = a; bd = b; cd = cc; a = 0; b = 0; cc = 0; }
+        {
+          double uuip = qr * (uc * dyfj * dzfk + uxp * dyfj * dzfk) * uxp;
+          double uuim = qr * (uc * dyfj * dzfk + uxm * dyfj * dzfk) * uxm;
+          double uvjp = qr * (vc * dxfi * dzfk + vxp * dxfip * dzfk) * uyp;
+          double uvjm = qr * (vym * dxfi * dzfk + V0[LYM + 1] * dxfip * dzfk) * uym;
+          double uwkp = qr * (wc * dxfi * dyfj + wxp * dxfip * dyfj) * up;
+          double uwkm = qr * (wm * dxfi * dyfj + Wm[LXP] * dxfip * dyfj) * um;
+          a = a + ((-uuip + uuim) + (-uvjp + uvjm) + (-uwkp + uwkm)) * (ixci * iyfj * izfk);
+          double uvip = qr * (uc * dyfj * dzfk + uyp * dyfjp * dzfk) * vxp;
+          double uvim = qr * (uxm * dyfj * dzfk + U0[LXM + LX] * dyfjp * dzfk) * vxm;
+          double vvjp = qr * (vc * dxfi * dzfk + vyp * dxfi * dzfk) * vyp;
+          double vvjm = qr * (vc * dxfi * dzfk + vym * dxfi * dzfk) * vym;
+          double wvkp = qr * (wc * dyfj * dxfi + wyp * dyfjp * dxfi) * vp;
+          double wvkm = qr * (wm * dyfj * dxfi + Wm[LYP] * dyfjp * dxfi) * vm;
+          b = b + ((-uvip + uvim) + (-vvjp + vvjm) + (-wvkp + wvkm)) * (ixfi * iycj * izfk);
+          double uwip = qr * (uc * dzfk * dyfj + up * dzfkp * dyfj) * wxp;
+          double uwim = qr * (uxm * dzfk * dyfj + Up[LXM] * dzfkp * dyfj) * wxm;
+          double vwjp = qr * (vc * dzfk * dxfi + vp * dzfkp * dxfi) * wyp;
+          double vwjm = qr * (vym * dzfk * dxfi + Vp[LYM] * dzfkp * dxfi) * wym;
+          double wwkp = qr * (wc * dxfi * dyfj + wp * dxfi * dyfj) * wp;
+          double wwkm = qr * (wc * dxfi * dyfj + wm * dxfi * dyfj) * wm;
+          cc = cc + ((-uwip + uwim) + (-vwjp + vwjm) + (-wwkp + wwkm)) * (ixfi * iyfj * izck);
+        }
       }
-      // implicit diffusion: the diffusion is kept apart, dudtrk is the advection
-      // alone (rk.f90:50-59)
-      double ad = 0.0, bd = 0.0, cd = 0.0;
-      if (IMP) { ad = a; bd = b; cd = cc; a = 0.0; b = 0.0; cc = 0.0; }
-      {
-        double uuip = 0.25 * (uc * dyfj * dzfk + uxp * dyfj * dzfk) * uxp;
-        double uuim = 0.25 * (uc * dyfj * dzfk + uxm * dyfj * dzfk) * uxm;
-        double uvjp = 0.25 * (vc * dxfi * dzfk + vxp * dxfip * dzfk) * uyp;
-        double uvjm = 0.25 * (vym * dxfi * dzfk + V0[LYM + 1] * dxfip * dzfk) * uym;
-        double uwkp = 0.25 * (wc * dxfi * dyfj + wxp * dxfip * dyfj) * up;
-        double uwkm = 0.25 * (wm * dxfi * dyfj + Wm[LXP] * dxfip * dyfj) * um;
-        a = a + ((-uuip + uuim) + (-uvjp + uvjm) + (-uwkp + uwkm)) * (ixci * iyfj * izfk);
-        double uvip = 0.25 * (uc * dyfj * dzfk + uyp * dyfjp * dzfk) * vxp;
-        double uvim = 0.25 * (uxm * dyfj * dzfk + U0[LXM + LX] * dyfjp * dzfk) * vxm;
-        double vvjp = 0.25 * (vc * dxfi * dzfk + vyp * dxfi * dzfk) * vyp;
-        double vvjm = 0.25 * (vc * dxfi * dzfk + vym * dxfi * dzfk) * vym;
-        double wvkp = 0.25 * (wc * dyfj * dxfi + wyp * dyfjp * dxfi) * vp;
-        double wvkm = 0.25 * (wm * dyfj * dxfi + Wm[LYP] * dyfjp * dxfi) * vm;
-        b = b + ((-uvip + uvim) + (-vvjp + vvjm) + (-wvkp + wvkm)) * (ixfi * iycj * izfk);
-        double uwip = 0.25 * (uc * dzfk * dyfj + up * dzfkp * dyfj) * wxp;
-        double uwim = 0.25 * (uxm * dzfk * dyfj + Up[LXM] * dzfkp * dyfj) * wxm;
-        double vwjp = 0.25 * (vc * dzfk * dxfi + vp * dzfkp * dxfi) * wyp;
-        double vwjm = 0.25 * (vym * dzfk * dxfi + Vp[LYM] * dzfkp * dxfi) * wym;
-        double wwkp = 0.25 * (wc * dxfi * dyfj + wp * dxfi * dyfj) * wp;
-        double wwkm = 0.25 * (wc * dxfi * dyfj + wm * dxfi * dyfj) * wm;
-        cc = cc + ((-uwip + uwim) + (-vwjp + vwjm) + (-wwkp + wwkm)) * (ixfi * iyfj * izck);
+      double un_ = uc_ + f1 * (double)a + f2 * (double)ld_nt<NTS>(dtu + q);
+      double vn_ = vc_ + f1 * (double)b + f2 * (double)ld_nt<NTS>(dtv + q);
+      double wn_ = wc_ + f1 * (double)cc + f2 * (double)ld_nt<NTS>(dtw + q);
+      if (IMP) {  // rk.f90:75-79
+        un_ = un_ + f12 * (double)ad; vn_ = vn_ + f12 * (double)bd; wn_ = wn_ + f12 * (double)cd;
       }
-      double un_ = uc + f1 * a + f2 * ld_nt<NTS>(dtu + q);
-      double vn_ = vc + f1 * b + f2 * ld_nt<NTS>(dtv + q);
-      double wn_ = wc + f1 * cc + f2 * ld_nt<NTS>(dtw + q);
-      if (IMP) { un_ = un_ + f12 * ad; vn_ = vn_ + f12 * bd; wn_ = wn_ + f12 * cd; }  // rk.f90:75-79
-      un_ = un_ + f12 * (-(P0[LXP] - pc) * ixci + bfx);
-      vn_ = vn_ + f12 * (-(P0[LYP] - pc) * iycj + bfy);
-      wn_ = wn_ + f12 * (-(pz - pc) * izck + bfz);
+      un_ = un_ + f12 * (-(P0[LXP] - pc) * ixci_d + bfx);
+      vn_ = vn_ + f12 * (-(P0[LYP] - pc) * iycj_d + bfy);
+      wn_ = wn_ + f12 * (-(pz - pc) * izck_d + bfz);
       if (BUOY) {
         const double sc = s1[c];
         un_ = un_ - f12 * gbx * 0.5 * (s1[c + 1] + sc);
@@ -426,9 +492,9 @@ __global__ __launch_bounds__(64 * BY) void k_rk_mom_lds(Geo g, Metrics m, const 
       st_nt<NTS>(dtv + q, b);
       st_nt<NTS>(dtw + q, cc);
       if (IMP) {  // the Helmholtz right-hand side (rk.f90:112-127)
-        un_ = un_ - 0.5 * f12 * ad;
-        vn_ = vn_ - 0.5 * f12 * bd;
-        wn_ = wn_ - 0.5 * f12 * cd;
+        un_ = un_ - 0.5 * f12 * (double)ad;
+        vn_ = vn_ - 0.5 * f12 * (double)bd;
+        wn_ = wn_ - 0.5 * f12 * (double)cd;
       }
       st_nt<NTS>(un + c, un_);
       st_nt<NTS>(vn + c, vn_);
@@ -688,28 +754,31 @@ dim3 ghosted_tiles(const Geo &g) { return dim3(((g.nx + 2 + TX - 1) / TX) * ((g.
 
 }  // namespace
 
-void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v, const double *w,
-                   const double *p, double *un, double *vn, double *wn, double *dtu, double *dtv, double *dtw,
-                   double f1, double f2, double f12, double visc, double bfx, double bfy, double bfz,
-                   const double *s1, const double gb[3], int variant, bool impdiff) {
-  if (variant >= 7) {
-    // 7..10: LDS-staged k-march; 7/8: 64x4 tiles, KB 16/32; 9/10: 64x8 tiles, KB 16/32;
-    // 11..14: the same with non-temporal hints on the streams;
-    // (capping the registers at 128 for a fourth wave per SIMD spills and measured slower)
-    // 15, 16: 64x8 tiles over 64 planes, without / with the hints
-    // (64x12 tiles, twelve waves per workgroup, measured 6-25 % slower)
-    const bool nt = variant >= 11 && variant != 15;
-    const int vi = variant >= 15 ? 4 : (variant - 7) % 4;
-    const int kb = vi == 4 ? 64 : (vi % 2 ? 32 : 16), by = vi >= 2 ? 8 : 4;
-    dim3 grid((unsigned)(((g.nx + 63) / 64) * ((g.ny + by - 1) / by) * ((g.nz + kb - 1) / kb)));
+namespace {
+// the LDS-staged k-march forms (variants 7..16) with FP64 or FP32
+// right-hand-side arithmetic
+template <typename R>
+void launch_rk_mom_lds(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v,
+                       const double *w, const double *p, double *un, double *vn, double *wn, R *dtu, R *dtv,
+                       R *dtw, double f1, double f2, double f12, double visc, double bfx, double bfy, double bfz,
+                       const double *s1, const double gb[3], int variant, bool impdiff) {
+  // 7..10: LDS-staged k-march; 7/8: 64x4 tiles, KB 16/32; 9/10: 64x8 tiles, KB 16/32;
+  // 11..14: the same with non-temporal hints on the streams;
+  // (capping the registers at 128 for a fourth wave per SIMD spills and measured slower)
+  // 15, 16: 64x8 tiles over 64 planes, without / with the hints
+  // (64x12 tiles, twelve waves per workgroup, measured 6-25 % slower)
+  const bool nt = variant >= 11 && variant != 15;
+  const int vi = variant >= 15 ? 4 : (variant - 7) % 4;
+  const int kb = vi == 4 ? 64 : (vi % 2 ? 32 : 16), by = vi >= 2 ? 8 : 4;
+  dim3 grid((unsigned)(((g.nx + 63) / 64) * ((g.ny + by - 1) / by) * ((g.nz + kb - 1) / kb)));
 #define SNAC_KL(B, KB, Y)                                                                                           \
   do {                                                                                                              \
     if (nt)                                                                                                         \
-      hipLaunchKernelGGL((k_rk_mom_lds<B, IM, KB, Y, true>), grid, dim3(64, Y), 0, s, g, m, u, v, w, p, un, vn, wn,   \
-                         dtu, dtv, dtw, f1, f2, f12, visc, bfx, bfy, bfz, s1, gb[0], gb[1], gb[2]);                  \
+      hipLaunchKernelGGL((k_rk_mom_lds<R, B, IM, KB, Y, true>), grid, dim3(64, Y), 0, s, g, m, u, v, w, p, un, vn,   \
+                         wn, dtu, dtv, dtw, f1, f2, f12, visc, bfx, bfy, bfz, s1, gb[0], gb[1], gb[2]);              \
     else                                                                                                            \
-      hipLaunchKernelGGL((k_rk_mom_lds<B, IM, KB, Y, false>), grid, dim3(64, Y), 0, s, g, m, u, v, w, p, un, vn, wn,  \
-                         dtu, dtv, dtw, f1, f2, f12, visc, bfx, bfy, bfz, s1, gb[0], gb[1], gb[2]);                  \
+      hipLaunchKernelGGL((k_rk_mom_lds<R, B, IM, KB, Y, false>), grid, dim3(64, Y), 0, s, g, m, u, v, w, p, un, vn,  \
+                         wn, dtu, dtv, dtw, f1, f2, f12, visc, bfx, bfy, bfz, s1, gb[0], gb[1], gb[2]);              \
   } while (0)
 #define SNAC_KL_ALL(B)                                                                                              \
   do {                                                                                                              \
@@ -719,18 +788,37 @@ void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *
     else if (vi == 3) SNAC_KL(B, 32, 8);                                                                            \
     else SNAC_KL(B, 64, 8);                                                                                         \
   } while (0)
-    if (impdiff) {
-      constexpr bool IM = true;
-      if (s1) SNAC_KL_ALL(true);
-      else SNAC_KL_ALL(false);
-    } else {
-      constexpr bool IM = false;
-      if (s1) SNAC_KL_ALL(true);
-      else SNAC_KL_ALL(false);
-    }
+  if (impdiff) {
+    constexpr bool IM = true;
+    if (s1) SNAC_KL_ALL(true);
+    else SNAC_KL_ALL(false);
+  } else {
+    constexpr bool IM = false;
+    if (s1) SNAC_KL_ALL(true);
+    else SNAC_KL_ALL(false);
+  }
 #undef SNAC_KL_ALL
 #undef SNAC_KL
-    SNAC_HIP(hipGetLastError());
+  SNAC_HIP(hipGetLastError());
+}
+}  // namespace
+
+void launch_rk_mom_f32(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v,
+                       const double *w, const double *p, double *un, double *vn, double *wn, float *dtu, float *dtv,
+                       float *dtw, double f1, double f2, double f12, double visc, double bfx, double bfy, double bfz,
+                       const double *s1, const double gb[3], int variant, bool impdiff) {
+  if (variant < 7 || variant > 16) throw std::runtime_error("FP32 rk_mom: LDS variants 7..16 only");
+  launch_rk_mom_lds<float>(s, g, m, u, v, w, p, un, vn, wn, dtu, dtv, dtw, f1, f2, f12, visc, bfx, bfy, bfz, s1, gb,
+                           variant, impdiff);
+}
+
+void launch_rk_mom(hipStream_t s, const Geo &g, const Metrics &m, const double *u, const double *v, const double *w,
+                   const double *p, double *un, double *vn, double *wn, double *dtu, double *dtv, double *dtw,
+                   double f1, double f2, double f12, double visc, double bfx, double bfy, double bfz,
+                   const double *s1, const double gb[3], int variant, bool impdiff) {
+  if (variant >= 7) {
+    launch_rk_mom_lds<double>(s, g, m, u, v, w, p, un, vn, wn, dtu, dtv, dtw, f1, f2, f12, visc, bfx, bfy, bfz, s1,
+                              gb, variant, impdiff);
     return;
   }
   if (variant > 0) {
@@ -854,12 +942,28 @@ namespace {
 // plain copy, one 16-byte element per lane over an exact grid (the fastest
 // copy form measured on MI355X): the HBM reference kernel for the roofline
 // and for scaling the FETCH_SIZE / WRITE_SIZE counters
+__global__ __launch_bounds__(256) void k_cvt_d2f(const double *__restrict__ a, float *__restrict__ b, long n) {
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < n; e += (long)gridDim.x * 256) b[e] = (float)a[e];
+}
+__global__ __launch_bounds__(256) void k_cvt_f2d(const float *__restrict__ a, double *__restrict__ b, long n) {
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < n; e += (long)gridDim.x * 256) b[e] = (double)a[e];
+}
 __global__ __launch_bounds__(256) void k_stream_copy(const double2 *__restrict__ a, double2 *__restrict__ b, long n2) {
   const long i = blockIdx.x * 256L + threadIdx.x;
   if (i < n2) b[i] = a[i];
 }
 }  // namespace
 
+void launch_cvt_d2f(hipStream_t s, const double *a, float *b, long n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_cvt_d2f, dim3((unsigned)std::min<long>((n + 255) / 256, 8192)), dim3(256), 0, s, a, b, n);
+  SNAC_HIP(hipGetLastError());
+}
+void launch_cvt_f2d(hipStream_t s, const float *a, double *b, long n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_cvt_f2d, dim3((unsigned)std::min<long>((n + 255) / 256, 8192)), dim3(256), 0, s, a, b, n);
+  SNAC_HIP(hipGetLastError());
+}
 void launch_stream_copy(hipStream_t s, const double *a, double *b, long n) {
   long n2 = n / 2;
   hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, (const double2 *)a,
