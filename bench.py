@@ -1,44 +1,52 @@
 #!/usr/bin/env python3
 """Benchmark of the SNaC time-step hot path on one MI355X.
 
-Workload: the FFT-accelerated periodic channel, plain FP64 -- 256x128x256
-cells (x periodic, y wall-normal tanh-stretched with factor 1.8, z periodic
-and the FFT axis), Re_b = 5600 with the laminar body force bforce(1) = 12/5600,
+Workload (default): the FFT-accelerated periodic channel 512x256x256 (x
+periodic, y wall-normal tanh-stretched with factor 1.8, z periodic and the
+FFT axis), Re_b = 5600 with the laminar body force bforce(1) = 12/5600,
 laminar Poiseuille profile plus seeded uniform +-10% noise.  A step is one
 full RK3 time step (3 substages: fused momentum RHS + update, ghost fills,
-fillps, r2r transforms along z and x, tridiagonal solves along y, inverse
-transforms, correc + pressure update).  Inputs are resident in HBM before
-the timed region; dt is fixed from the CFL check at the start.
+fillps, r2r transform along z, the per-mode 2D Helmholtz solves, inverse
+transform, correc + pressure update).  Inputs are resident in HBM before
+the timed region; dt is fixed from the CFL check at the start and is the
+same for every form.
 
-Prints one JSON line: Mcell-steps/s, the per-phase split (Poisson ms/step),
-the roofline of the dominant kernel (fused rk_mom, 104 algorithmic bytes
-per cell) from HIP events on its own stream, and the CPU port's rate on the
-host cores (a bounded sample of the same workload).
+Forms, each timed on its own context in the same run (--forms):
+  plain   the plain FP64 pipeline: batched MG-preconditioned CG per Fourier
+          mode (zebra y-line relaxation, x semi-coarsening), every mode to
+          the relative residual --tol (&hypre hypre_tol);
+  mixed   the same solve with the V-cycle and the CG search directions in
+          FP32 under a flexible CG whose residuals, operator, dot products
+          and null-space removal stay FP64 (same --tol), and the momentum
+          right-hand side in FP32 (RK3 accumulation and projection FP64);
+  learned as mixed with the trained network preconditioning the CG (pairs
+          recorded by the plain solver over --learn-steps steps, then
+          --learn-epochs epochs of training; eager launches);
+  direct  FP64 with the pressure solved directly (r2r along z and x,
+          tridiagonal lines along y) -- the same discrete systems, not the
+          plain form's method;
+  direct_mixed, direct_impdiff: the direct solver with the FP32 momentum
+          right-hand side / with implicit diffusion.
+The line's value is the mixed (or learned) form's Mcell-steps/s; the plain
+form's figures and the ratio sit beside it, with every form's divergence.
 
---precision mixed runs the MG-PCG V-cycle in FP32 (MG-PCG path only; the
-channel's direct solver and the stencils stay FP64).  --precision learned
-(MG-PCG path) first records the plain solver's (rhs, solution) pairs over
---learn-steps steps, trains the network on them for --learn-epochs epochs,
-then times steps with the network preconditioning the flexible CG (V-cycle
-fallback when its residual reduction stalls; eager launches); the line adds
-the network's MFMA rate (one forward over every mode, HIP events) against
-the 2.5 PFLOP/s dense BF16 peak.
+Roofline objects: `roofline` is the dominant kernel of the headline form
+(the finest MG level's zebra colour sweep on the MG forms: 1.5 x unknowns x
+element size per sweep -- b and x of half the lines, x of the other half;
+rk_mom on the direct forms), `roofline_stencil` the fused rk_mom of the same
+form (104 B/cell FP64, 80 B/cell with the FP32 right-hand side), both from
+HIP events on the context's stream during an eager pass after the timed
+region.  cpu_baseline: the C oracle (FP64, OpenMP, direct Poisson solver)
+on a bounded sample of the same channel.
 
---impdiff runs the channel with implicit diffusion (the reference's
-IMPDIFF=1 build): each substage adds three Helmholtz solves (one per velocity
-component) on the same tensor-product plan; not the headline workload, and
-no CPU sample (the oracle's direct Helmholtz solves are too slow at this
-size).
-
---workload bfs runs the three-block backward-facing step instead (inlet
-192x64x128 with a Poiseuille inflow, lower 320x64x128, upper 320x96x128, z
-periodic with 128 modes, y tanh-stretched per block, outflow at x+): the
-union is not a box, so the pressure goes through the batched MG-PCG path
-(per-mode relative residual 1e-13); the CPU sample is skipped there.
+--workload bfs runs the three-block backward-facing step (inlet 192x64x128
+with a Poiseuille inflow, lower 320x64x128, upper 320x96x128, z periodic
+with 128 modes, outflow at x+): forms plain / mixed / learned only.
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import tempfile
 import time
@@ -49,43 +57,44 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 METRIC = "Mcell-steps/s + Poisson ms/step, learned/mixed vs plain FP64, equal div tol"
 HBM_PEAK_GBS = 8000.0
-RK_MOM_BYTES_PER_CELL = 104  # R u,v,w,p,3x dudtrko + W u,v,w,3x dudtrko, FP64
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "rk_mom_channel_256x128x256_traffic.json")
+RK_BYTES = {False: 104, True: 80}  # per cell: FP64 / FP32 right-hand side (FP32 old tendencies)
+PROFILES = os.path.join(ROOT, "profiles")
+
+FORMS = {
+    # name: (solver, precision, impdiff, dtype)
+    "plain": ("mg", "fp64", False, "f64"),
+    "mixed": ("mg", "mixed", False, "f64+f32"),
+    "learned": ("mg", "learned", False, "f64+f32+bf16"),
+    "direct": ("tensor", "fp64", False, "f64"),
+    "direct_mixed": ("tensor", "mixed", False, "f64+f32"),
+    "direct_impdiff": ("tensor", "fp64", True, "f64"),
+}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", choices=["channel", "bfs"], default="channel")
-    ap.add_argument("--size", type=int, nargs=3, default=[256, 128, 256])
-    ap.add_argument("--solver", choices=["auto", "tensor", "mg"], default="auto",
-                    help="pressure solver: the channel's direct tensor-product solver (auto) or MG-PCG")
-    ap.add_argument("--tol", type=float, default=1e-13,
-                    help="MG-PCG relative residual per mode (&hypre hypre_tol), bfs workload")
+    ap.add_argument("--size", type=int, nargs=3, default=[512, 256, 256])
+    ap.add_argument("--forms", default=None,
+                    help="comma-separated forms (default: plain,mixed,direct; bfs: plain,mixed)")
+    ap.add_argument("--tol", type=float, default=1e-10, help="MG-PCG relative residual per mode (&hypre hypre_tol)")
+    ap.add_argument("--eager-steps", type=int, default=3, help="steps of the eager pass with per-phase HIP events")
     ap.add_argument("--learn-steps", type=int, default=20)
     ap.add_argument("--learn-epochs", type=int, default=4)
-    ap.add_argument("--precision", choices=["fp64", "mixed", "learned"], default="fp64",
-                    help="mixed: FP32 MG V-cycle under flexible CG (MG-PCG path only)")
-    ap.add_argument("--impdiff", action="store_true", help="implicit diffusion (channel workload)")
-    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0 = skip)")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
-    ap.add_argument("--calib", action="store_true", help="also run the plain copy kernel (counter calibration)")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, usable cpus)")
     return ap.parse_args()
 
 
-def channel_setup(tmp, size, tol):
+def case_setup(tmp, workload, size, tol):
     import cases
-    d, b = cases.write_case(tmp, [cases.channel_block(size)], 5600.0, cfl=0.5, bforce=(12.0 / 5600.0, 0.0, 0.0))
-    txt = open(d).read().replace("hypre_tol = 1.e-13", f"hypre_tol = {tol!r}")
-    open(d, "w").write(txt)
-    return d, b
-
-
-def bfs_setup(tmp, tol):
-    import cases
-    d, b = cases.write_case(tmp, cases.bfs_blocks(), 1000.0, cfl=0.5)
+    if workload == "bfs":
+        d, b = cases.write_case(tmp, cases.bfs_blocks(), 1000.0, cfl=0.5)
+    else:
+        d, b = cases.write_case(tmp, [cases.channel_block(size)], 5600.0, cfl=0.5, bforce=(12.0 / 5600.0, 0.0, 0.0))
     txt = open(d).read().replace("hypre_tol = 1.e-13", f"hypre_tol = {tol!r}")
     open(d, "w").write(txt)
     return d, b
@@ -110,6 +119,28 @@ def bfs_ic(ctx, seed=20251202, amp=0.1):
     return out
 
 
+def set_ic(ctx, workload):
+    import cases
+    if workload == "bfs":
+        for q, fields in enumerate(bfs_ic(ctx)):
+            for f, a in zip(("u", "v", "w"), fields):
+                ctx.set(q, f, a)
+    else:
+        u, v, w = cases.channel_ic(ctx.shape(0, "u"), ctx.metric(0, 8))
+        for f, a in (("u", u), ("v", v), ("w", w)):
+            ctx.set(0, f, a)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(d, b, size, steps, threads):
     import cases
     import nmlparse
@@ -126,52 +157,39 @@ def cpu_baseline(d, b, size, steps, threads):
         oc.step(dt)
     el = time.perf_counter() - t0
     cells = size[0] * size[1] * size[2]
-    return {"value": cells * steps / el / 1e6, "unit": "Mcell-steps/s", "cores": int(n), "kind": "port",
+    return {"value": round(cells * steps / el / 1e6, 3), "unit": "Mcell-steps/s", "cores": int(n), "kind": "port",
             "sample": f"{steps} full RK3 steps of the {size[0]}x{size[1]}x{size[2]} channel on the C oracle "
-                      f"(OpenMP, tensor Poisson solver, radix-2 transforms), {el:.1f} s"}
+                      f"(FP64, OpenMP on {n} threads, direct Poisson solver: r2r along z and x, tridiagonal "
+                      f"lines along y), {el:.1f} s",
+            "host_cpus": os.cpu_count(), "cpu_model": cpu_model()}
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py: the time step runs on one GPU; --gpus {args.gpus} is not supported "
-              "(sharding the blocks over GPUs is not implemented)", file=sys.stderr)
-        sys.exit(2)
-    import numpy as np
+def traffic_for(kernel, workload, size, form):
+    """HBM bytes per launch from the committed PMC summary of this kernel,
+    workload and form (FETCH_SIZE and WRITE_SIZE runs), if there is one."""
+    tag = "bfs" if workload == "bfs" else "channel_" + "x".join(map(str, size))
+    f = os.path.join(PROFILES, f"{kernel}_{tag}_traffic.json")
+    if not os.path.exists(f):
+        return None
+    return json.load(open(f)).get(form, {}).get("bytes_per_launch")
 
-    import cases
+
+def run_form(name, d, b, args, cells, dt_ref):
     import snac_amd
-
-    size = args.size
-    bfs = args.workload == "bfs"
-    with tempfile.TemporaryDirectory() as tmp:
-        if bfs:
-            d, b = bfs_setup(tmp, args.tol)
-        else:
-            d, b = channel_setup(tmp, size, args.tol)
-        print(f"[bench] {args.workload} {'' if bfs else size} on device 0", file=sys.stderr, flush=True)
-        ctx = snac_amd.Context(d, b, 3, device=0)
-        if args.solver != "auto":
-            ctx.set_solver(args.solver)
-        if args.precision != "learned":
-            ctx.set_precision(args.precision)
-        if args.impdiff:
+    solver, prec, impdiff, dtype = FORMS[name]
+    ctx = snac_amd.Context(d, b, 3, device=0)
+    try:
+        ctx.set_solver(solver)
+        if prec != "learned":
+            ctx.set_precision(prec)
+        if impdiff:
             ctx.set_impdiff(True)
-        cells = sum(int(np.prod(np.array(ctx.shape(q, "u")) - 2)) for q in range(ctx.nblocks))
-        if bfs:
-            for q, fields in enumerate(bfs_ic(ctx)):
-                for f, a in zip(("u", "v", "w"), fields):
-                    ctx.set(q, f, a)
-        else:
-            u, v, w = cases.channel_ic(ctx.shape(0, "u"), ctx.metric(0, 8))
-            for f, a in (("u", u), ("v", v), ("w", w)):
-                ctx.set(0, f, a)
+        set_ic(ctx, args.workload)
         ctx.init_bounds()
+        dt = dt_ref if dt_ref is not None else min(0.5 * ctx.chkdt(), ctx.dtmax)
         tune_choices, tune_ms = ctx.autotune()
-        dt = min(0.5 * ctx.chkdt(), ctx.dtmax)
-        copy_gbs = ctx.stream_copy(1 << 27, reps=10 if args.calib else 3)
         learned = None
-        if args.precision == "learned":
+        if prec == "learned":
             ng = int(ctx.block_info(0)["n"][2])  # modes along z
             ctx.learned_collect(ng * 3 * args.learn_steps)
             t0 = time.perf_counter()
@@ -188,99 +206,149 @@ def main():
                        "network_forward_ms": round(fwd_ms, 3), "network_gflop": round(gflop, 1),
                        "roofline_net": {"bound": "mfma", "achieved": round(gflop / fwd_ms, 1), "peak": 2500.0,
                                         "unit": "TFLOP/s", "frac": round(gflop / fwd_ms / 2500.0, 4)}}
-            print(f"[bench] learned: trained {args.learn_epochs} epochs, forward {fwd_ms:.3f} ms", file=sys.stderr,
-                  flush=True)
         ctx.steps(dt, args.warmup)
         ctx.sync()
-        print(f"[bench] warmup done, dt={dt:.3e}", file=sys.stderr, flush=True)
-        # timed region: the production path, one hipGraph-captured RK3 step
-        # replayed K times on the context's stream (eager launches on the
-        # MG-PCG path, whose CG loop checks convergence on the host)
+        ctx.solver_counters()
+        print(f"[bench] {name}: warmup done, dt={dt:.3e}", file=sys.stderr, flush=True)
+        # timed region: one hipGraph-captured RK3 step replayed K times on
+        # the context's stream (on the MG-PCG path the host resumes a solve
+        # left above the tolerance after its captured iterations)
         t0 = time.perf_counter()
         ctx.steps(dt, args.steps)
         ctx.sync()
         el = time.perf_counter() - t0
         divtot, divmax = ctx.chkdiv()
         cg_iters, cg_relres = ctx.solver_stats()
-        if learned is not None:
-            learned["fallback_solves"] = ctx.learned_stats()[1]
-        print(f"[bench] timed {args.steps} steps in {el:.3f} s, divmax={divmax:.2e}", file=sys.stderr, flush=True)
-        # second timed pass: eager launches with HIP events around every
-        # phase (on the stream the kernels run on) for the per-phase split
-        # and the dominant kernel's average duration
-        nph = max(1, min(args.steps, 50))
+        solves, continued, unconverged = ctx.solver_counters()
+        print(f"[bench] {name}: timed {args.steps} steps in {el:.3f} s, divmax={divmax:.2e}", file=sys.stderr,
+              flush=True)
+        # eager pass: HIP events around every phase, on the stream the
+        # kernels run on
+        ne = max(1, args.eager_steps)
         ctx.set_timing(True)
         t1 = time.perf_counter()
-        ctx.steps(dt, nph)
+        ctx.steps(dt, ne)
         ctx.sync()
         el_eager = time.perf_counter() - t1
         ph = ctx.get_timing()
         ctx.set_timing(False)
+        out = {"solver": solver, "precision": prec, "dtype": dtype, "impdiff": impdiff,
+               "mcell_steps_per_s": round(cells * args.steps / el / 1e6, 2),
+               "ms_per_step": round(el * 1e3 / args.steps, 4),
+               "poisson_ms_per_step": round(sum(ph[k][0] for k in ("r2r_fwd", "tridiag", "r2r_bwd")) / ne, 4),
+               "phase_ms_per_step": {k: round(v[0] / ne, 4) for k, v in ph.items() if v[1] > 0},
+               "phase_timing": f"eager launches with per-phase HIP events, {ne} steps, "
+                               f"{el_eager * 1e3 / ne:.4f} ms/step",
+               "divtot": divtot, "divmax": divmax, "dt": dt}
+        if solver == "mg":
+            out.update({"hypre_tol": args.tol, "cg_iterations_last_solve": cg_iters,
+                        "cg_max_relres_last_solve": cg_relres, "solves_timed": solves,
+                        "solves_resumed_after_graph": continued, "solves_at_maxiter": unconverged})
+        if learned is not None:
+            learned["fallback_solves"] = ctx.learned_stats()[1]
+            out["learned"] = learned
+        # rk_mom: FP64 or FP32 right-hand side
+        f32 = prec != "fp64"
+        rk_ms, rk_n = ph["rk_mom"]
+        rk_avg = rk_ms / max(rk_n, 1)
+        rkv = tune_choices[-1] if f32 else tune_choices[-2]
+        out["roofline_stencil"] = {
+            "bound": "hbm", "achieved": round(RK_BYTES[f32] * cells / (rk_avg * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(RK_BYTES[f32] * cells / (rk_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "traffic": traffic_for("rk_mom", args.workload, args.size, name),
+            "kernel": f"rk_mom ({'FP32' if f32 else 'FP64'} right-hand side, variant {rkv}; "
+                      f"{RK_BYTES[f32]} B/cell)", "avg_launch_ms": round(rk_avg, 4)}
+        if solver == "mg":
+            l0_ms, l0_n = ph["mg_line0"]
+            if l0_n > 0:
+                esz = 4 if f32 else 8
+                avg = l0_ms / l0_n
+                bytes_ = 1.5 * cells * esz
+                out["roofline_line0"] = {
+                    "bound": "hbm", "achieved": round(bytes_ / (avg * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(bytes_ / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    "traffic": traffic_for("k_line", args.workload, args.size, name),
+                    "kernel": f"k_line, finest level, one zebra colour ({'FP32' if f32 else 'FP64'}; "
+                              f"1.5 x {esz} B per unknown)", "avg_launch_ms": round(avg, 4),
+                    "launches": l0_n}
+        out["autotune"] = {"choices": tune_choices, "best_ms": tune_ms}
+        return out
+    finally:
         ctx.close()
 
-        ms_step = el * 1e3 / args.steps
-        value = cells * args.steps / el / 1e6
-        phases = {k: round(v[0] / nph, 4) for k, v in ph.items()}
-        poisson = sum(ph[k][0] for k in ("r2r_fwd", "tridiag", "r2r_bwd")) / nph
-        rk_ms, rk_n = ph["rk_mom"]
-        rk_avg_s = rk_ms / rk_n * 1e-3
-        achieved = RK_MOM_BYTES_PER_CELL * cells / rk_avg_s / 1e9
-        traffic = None
-        if not bfs and os.path.exists(TRAFFIC_FILE):
-            tr = json.load(open(TRAFFIC_FILE))
-            if tr.get("grid") == list(size):
-                # HBM bytes per launch of the variant the autotuner chose in
-                # this run (the last autotune choice is the rk_mom variant)
-                traffic = tr.get("variants", {}).get(str(tune_choices[-1]), {}).get("bytes_per_launch")
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                    "kernel": "rk_mom (fused momentum RHS + RK3 update, autotuned variant; 104 B/cell)",
-                    "rk_variant": int(tune_choices[-1]) if not bfs else None, "avg_launch_ms": round(rk_avg_s * 1e3, 4),
-                    "stream_copy_GBs": round(copy_gbs, 1)}
 
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: the time step runs on one GPU; --gpus {args.gpus} is not supported "
+              "(sharding the blocks over GPUs is not implemented)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+
+    import snac_amd
+
+    bfs = args.workload == "bfs"
+    forms = (args.forms or ("plain,mixed" if bfs else "plain,mixed,direct")).split(",")
+    for f in forms:
+        if f not in FORMS:
+            sys.exit(f"bench.py: unknown form {f!r} (forms: {', '.join(FORMS)})")
+        if bfs and FORMS[f][0] == "tensor":
+            sys.exit(f"bench.py: form {f!r} needs a box-shaped case; the backward-facing step is not one")
+    size = args.size
+    results = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        d, b = case_setup(tmp, args.workload, size, args.tol)
+        probe = snac_amd.Context(d, b, 3, device=-1)  # host-only: the case's cell count
+        cells = sum(int(np.prod(np.array(probe.shape(q, "u")) - 2)) for q in range(probe.nblocks))
+        probe.close()
+        copy_gbs = None
+        dt_ref = None
+        for name in forms:
+            print(f"[bench] {args.workload} {'' if bfs else size}: form {name}", file=sys.stderr, flush=True)
+            results[name] = run_form(name, d, b, args, cells, dt_ref)
+            dt_ref = results[name]["dt"]
+        if not bfs:
+            # the plain copy kernel: the practical HBM ceiling on this part
+            ctx = snac_amd.Context(d, b, 3, device=0)
+            copy_gbs = round(ctx.stream_copy(1 << 27, reps=5), 1)
+            ctx.close()
         cpu = None
-        if args.cpu_steps > 0 and not bfs and not args.impdiff:
-            thr = args.cpu_threads or min(16, os.cpu_count() or 1)
+        if args.cpu_steps > 0 and not bfs:
+            thr = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
             print(f"[bench] CPU baseline: {args.cpu_steps} oracle steps on {thr} threads", file=sys.stderr,
                   flush=True)
             cpu = cpu_baseline(d, b, size, args.cpu_steps, thr)
 
+    head = next((f for f in ("learned", "mixed") if f in results), forms[0])
+    H = results[head]
+    plain = results.get("plain")
+    roof = H.get("roofline_line0") or H["roofline_stencil"]
     out = {
-        "metric": METRIC, "value": round(value, 2), "unit": "Mcell-steps/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": {"fp64": "f64", "mixed": "f64+f32", "learned": "f64+f32+bf16"}[args.precision],
+        "metric": METRIC, "value": H["mcell_steps_per_s"], "unit": "Mcell-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": H["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": H["dtype"],
         "data": "synthetic (laminar Poiseuille + seeded +-10% uniform noise, seed 20251202)",
-        "config": ({"workload": "plain FP64 FFT-accelerated periodic channel" +
-                                ("; implicit diffusion (three Helmholtz solves per substage)" if args.impdiff else ""),
+        "config": ({"workload": f"FFT-accelerated periodic channel {size[0]}x{size[1]}x{size[2]}, form '{head}'"
+                                + (" against the plain FP64 form" if plain and head != "plain" else ""),
                     "grid": list(size), "stretching": "y tanh two-end, factor 1.8", "fft_axis": "z",
-                    "Re_b": 5600, "forcing": "bforce(1)=12/5600",
-                    "poisson": ("r2r along z and x, tridiagonal along y (exact)" if args.solver != "mg" else
-                                f"DFT along z, batched MG-preconditioned CG per mode, relative residual {args.tol:g}"),
-                    "parallelism": "1 GPU",
-                    "autotune": {"choices": tune_choices, "best_ms": tune_ms}} if not bfs else
-                   {"workload": "plain FP64 backward-facing step, 3 blocks on one GPU",
+                    "Re_b": 5600, "forcing": "bforce(1)=12/5600", "hypre_tol": args.tol,
+                    "parallelism": "1 GPU"} if not bfs else
+                   {"workload": f"backward-facing step, 3 blocks on one GPU, form '{head}'"
+                                + (" against the plain FP64 form" if plain and head != "plain" else ""),
                     "blocks": [[192, 64, 128], [320, 64, 128], [320, 96, 128]], "cells": cells,
                     "stretching": "y tanh two-end per block, factor 1.2", "fft_axis": "z (128 modes)",
                     "Re": 1000, "inflow": "Poiseuille (inflow_type 1)", "outflow": "x+ velocity N, pressure D",
-                    "poisson": "DFT along z, batched MG-preconditioned CG per mode (zebra y-line relaxation, "
-                               f"x semi-coarsening, re-discretised coarse operators), relative residual {args.tol:g}; "
-                               "the step graph runs a fixed CG count measured on the first step",
-                    "cg_iterations_last_solve": cg_iters, "cg_max_relres_last_solve": cg_relres,
-                    "precision": {"fp64": "FP64 throughout",
-                                  "mixed": "FP32 V-cycle (smoothing, transfers, coarse solves) under flexible CG "
-                                           "with FP64 vectors, operator, dot products and null-space removal",
-                                  "learned": "BF16 network preconditioner (FP32 accumulation) under flexible CG "
-                                             "with FP64 vectors, operator, dot products and null-space removal; "
-                                             "FP32 V-cycle fallback"}[args.precision],
-                    "learned": learned,
-                    "parallelism": "1 GPU"}),
-        "poisson_ms_per_step": round(poisson, 4),
-        "phase_ms_per_step": phases,
-        "phase_timing": f"eager launches with per-phase HIP events, {nph} steps, "
-                        f"{el_eager * 1e3 / nph:.4f} ms/step",
-        "divtot": divtot, "divmax": divmax,
-        "roofline": roofline,
+                    "hypre_tol": args.tol, "parallelism": "1 GPU"}),
+        "poisson_ms_per_step": H["poisson_ms_per_step"],
+        "vs_plain_fp64": (round(H["mcell_steps_per_s"] / plain["mcell_steps_per_s"], 3)
+                          if plain and head != "plain" else None),
+        "poisson_vs_plain_fp64": (round(plain["poisson_ms_per_step"] / H["poisson_ms_per_step"], 3)
+                                  if plain and head != "plain" else None),
+        "divtot": H["divtot"], "divmax": H["divmax"],
+        "roofline": dict(roof, stream_copy_GBs=copy_gbs),
+        "roofline_stencil": H["roofline_stencil"],
         "cpu_baseline": cpu,
+        "forms": results,
     }
     print(json.dumps(out))
 

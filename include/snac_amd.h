@@ -265,7 +265,9 @@ enum {
   SNAC_PH_CORREC = 6,   /* correc + updt_pressure                           */
   SNAC_PH_RK_SCAL = 7,  /* rk_scal + bulk forcing of the scalars            */
   SNAC_PH_HELMHOLTZ = 8, /* implicit diffusion: the three Helmholtz solves  */
-  SNAC_NPHASE = 9
+  SNAC_PH_MG_LINE0 = 9, /* MG-PCG: each finest-level zebra colour sweep that
+                           reads its neighbour lines (inside SNAC_PH_TRIDIAG) */
+  SNAC_NPHASE = 10
 };
 int snac_set_timing(snac_ctx *ctx, int enable);
 /* waits for the stream, returns accumulated milliseconds and the number of

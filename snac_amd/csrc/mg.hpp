@@ -20,6 +20,7 @@
 //                  - ca(a-1) x(a-1) - ca(a) x(a+1) [- cm(m-1) x(m-1) - cm(m) x(m+1)]
 // with c?(i) the coupling of the link i -> i+1 stored at cell i.
 #pragma once
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -187,6 +188,9 @@ struct MGPlan {
   R2RPlan fwd, bwd;
   double *ph = nullptr;
   std::vector<void *> owned;
+  // optional timing marks around the finest level's colour sweeps that read
+  // their neighbour lines (begin = true before the launch, false after)
+  std::function<void(bool)> mark_line0;
 };
 
 // Builds the hierarchy for the case (returns with ok == false and a reason

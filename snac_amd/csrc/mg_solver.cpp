@@ -492,13 +492,22 @@ void vcycle(MGPlan &P, MGHier<T> &H, hipStream_t st, size_t l) {
     for (int ca = 0; ca <= (two_a ? 1 : 0); ca++) { col[nc][0] = ca; col[nc][1] = cm; nc++; }
   const bool zero_first = nc > 2;
   if (zero_first) mg_zero(st, (long)L.nm * L.na * L.ns, x);
-  for (int k = 0; k < nc; k++)
-    mg_line(st, L, H.lam, x, b, col[k][0], col[k][1], (!zero_first && k == 0) ? 1 : 0, -1, nullptr, P.done);
+  for (int k = 0; k < nc; k++) {
+    const int znb = (!zero_first && k == 0) ? 1 : 0;
+    const bool mk = l == 0 && znb == 0 && P.mark_line0;
+    if (mk) P.mark_line0(true);
+    mg_line(st, L, H.lam, x, b, col[k][0], col[k][1], znb, -1, nullptr, P.done);
+    if (mk) P.mark_line0(false);
+  }
   mg_restrict_residual(st, L, H.lev[l + 1], H.lam, x, b, H.bl[l + 1], P.done);
   vcycle(P, H, st, l + 1);
   mg_prolong(st, L, H.lev[l + 1], (const T *)H.xl[l + 1], x, P.done);
-  for (int k = nc - 1; k >= 0; k--)
+  for (int k = nc - 1; k >= 0; k--) {
+    const bool mk = l == 0 && P.mark_line0;
+    if (mk) P.mark_line0(true);
     mg_line(st, L, H.lam, x, b, col[k][0], col[k][1], 0, -1, nullptr, P.done);
+    if (mk) P.mark_line0(false);
+  }
 }
 
 template <typename T>

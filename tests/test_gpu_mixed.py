@@ -1,8 +1,9 @@
 """Mixed precision against the FP64 oracle.
 
 The mixed mode evaluates the momentum right-hand side of rk_mom
-(mom.f90:112-327) in FP32 from FP32 copies of u, v, w and keeps the old
-tendencies in FP32, while the RK3 accumulation (rk.f90:68-85), the pressure
+(mom.f90:112-327) in FP32, on neighbour offsets from each cell's FP64 values,
+and keeps the old tendencies in FP32, while the RK3 accumulation
+(rk.f90:68-85), the pressure
 gradient and the body force (rk.f90:86-111) stay FP64; on the MG-PCG path
 the V-cycle also runs in FP32 under a flexible CG with FP64 residuals.  The
 oracle is the FP64 restatement of the same routines, so these gates bound
