@@ -89,8 +89,8 @@ int snac_ctx_set_solver(snac_ctx *ctx, int solver);
  * dot products and null-space removal stay FP64, so the solve still stops
  * at hypre_tol.  fillps, correc, updt_pressure, the ghost fills and the
  * direct tensor-product solver stay FP64 in every mode.
- * SNAC_PREC_LEARNED (FP32 momentum right-hand side as MIXED) preconditions
- * the same flexible CG with the trained
+ * SNAC_PREC_LEARNED keeps the FP64 stencils and preconditions the same
+ * flexible CG with the trained
  * network (snac_learned_train) on the FP32 residual of each mode, falling
  * back to the FP32 V-cycle for the rest of a solve when some mode's residual
  * reduction stalls; the FP64 residual still decides convergence (FFT path;
@@ -184,10 +184,15 @@ int snac_ctx_get_run_params(const snac_ctx *ctx, int *iparams);
  *      stream; results are visible after snac_sync or a download ----       */
 /* rk_mom rk.f90:7-128 with momx/y/z_d, momx/y/z_a_vv, momx/y/z_p
  * (mom.f90:112-386); irk in 1..3 selects rkcoeff (param.f90:11-13).  With
- * scalars (nscal > 0) it adds the Boussinesq buoyancy of scalar 1
- * (rk.f90:104-108, the reference's _BOUSSINESQ_BUOYANCY build; beta = 0 or
- * gacc = 0 switch it off)                                                   */
+ * snac_set_buoyancy on it adds the Boussinesq buoyancy of scalar 1
+ * (rk.f90:104-108)                                                          */
 int snac_rk_mom(snac_ctx *ctx, int irk, double dt);
+/* The reference's BOUSSINESQ_BUOYANCY=1 build (rk.f90:104-108,
+ * param.f90:166-170) as a run-time switch: off by default, as in a default
+ * build; on, rk_mom adds -f12*gacc*beta*(face average of scalar 1).  Needs
+ * nscal > 0 ("_BOUSSINESQ_BUOYANCY requires nscal > 0." otherwise).  Drops
+ * the captured step graph.                                                  */
+int snac_set_buoyancy(snac_ctx *ctx, int enable);
 /* rk_scal rk.f90:130-204 (scal_d / scal_a scal.f90:89-155) followed by
  * bulk_mean_s / bulk_forcing_s scal.f90:157-199 for every scalar, as
  * main.f90:908-913; ghosts are filled by snac_boundp(SNAC_SCAL0 + 2*i)      */

@@ -101,6 +101,7 @@ struct oc_case {
   /* scalars (param.f90:96-100, scal.f90:22-49) */
   int nscal;
   double salpha[MAXS], ssource[MAXS], scalf[MAXS], sf[MAXS], beta, gacc[3], alpha_max;
+  int buoyancy; /* the _BOUSSINESQ_BUOYANCY build */
   int is_sforced[MAXS];
   int periods[3], is_periodic[3];
   ob b[MAXB];
@@ -467,7 +468,7 @@ static void rk_mom_block(oc_case *c, ob *B, int irk, double dt) {
         u[g] = u[g] + f12 * px;
         v[g] = v[g] + f12 * py;
         w[g] = w[g] + f12 * pz;
-        if (c->nscal > 0) { /* Boussinesq buoyancy rk.f90:104-108 */
+        if (c->buoyancy && c->nscal > 0) { /* Boussinesq buoyancy rk.f90:104-108 */
           const double *S = B->sv[0];
           u[g] = u[g] - f12 * c->gacc[0] * c->beta * 0.5 * (S[IDX(B, i + 1, j, k)] + S[g]);
           v[g] = v[g] - f12 * c->gacc[1] * c->beta * 0.5 * (S[IDX(B, i, j + 1, k)] + S[g]);
@@ -1029,6 +1030,10 @@ oc_case *oc_create(int nblocks, const int *ng, const double *lmin, const double 
         for (int cp = 0; cp < 3; cp++) B->vin[d][cp][s] = xcalloc((long)m1 * m2, sizeof(double));
         double uref = B->bcvel[d][d][s];
         int q1 = c->periods[a1] != 0 ? 0 : 1, q2 = c->periods[a2] != 0 ? 0 : 1;
+        /* INFLOW_KOVASZNAY initflow.f90:407-416: reb = uref*lref/visc with
+         * lref = 1 (main.f90:600-623), the profile taken at x0 = -0.5 */
+        double pi = acos(-1.0), reb = uref * 1.0 / c->visc;
+        double lambda = reb / 2 - sqrt(reb * reb / 4 + 4 * (pi * pi)), x0 = -0.5;
         for (int t2 = 0; t2 < m2; t2++)
           for (int t1 = 0; t1 < m1; t1++) {
             double val = 0.0;
@@ -1036,6 +1041,9 @@ oc_case *oc_create(int nblocks, const int *ng, const double *lmin, const double 
               double f1 = q1 ? poiseuille(B->rc[a1][t1], B->lmin[a1], B->lmax[a1]) : 1.0;
               double f2 = q2 ? poiseuille(B->rc[a2][t2], B->lmin[a2], B->lmax[a2]) : 1.0;
               val = uref * f1 * f2;
+            } else if (B->inflow_type[d][s] == 2) {
+              val = 1.0 - exp(lambda * x0) * cos(2 * pi * B->rc[a2][t2]);
+              B->vin[d][a2][s][t1 + (long)m1 * t2] = lambda / (2.0 * pi) * exp(lambda * x0) * sin(2 * pi * B->rf[a2][t2]);
             }
             B->vin[d][d][s][t1 + (long)m1 * t2] = val;
           }
@@ -1153,6 +1161,12 @@ int oc_set_field(oc_case *c, int b, int fid, const double *in) {
 
 /* scalar inputs: &scalar of dns.nml (param.f90:96-100,148-163) and
  * block_cbcscal/block_bcscal of blocks.nml (flat [b][iscal][dir][side])    */
+int oc_set_buoyancy(oc_case *c, int enable) {
+  if (enable && c->nscal <= 0) return -1;
+  c->buoyancy = enable != 0;
+  return 0;
+}
+
 int oc_set_scalars(oc_case *c, int nscal, const double *alphai, double beta, const double *gacc,
                    const double *ssource, const int *is_sforced, const double *scalf,
                    const char *cbcscal, const double *bcscal) {

@@ -64,12 +64,14 @@ int  oc_get_field(const oc_case *c, int b, int fid, double *out);
 int  oc_set_field(oc_case *c, int b, int fid, const double *in);
 
 /* scalar transport inputs (param.f90:96-100,148-163): per-scalar alphai,
- * ssource, is_sforced, scalf; beta and gacc(3) of the Boussinesq term
- * (applied whenever nscal > 0, the reference's _BOUSSINESQ_BUOYANCY build);
+ * ssource, is_sforced, scalf; beta and gacc(3) of the Boussinesq term;
  * cbcscal/bcscal flat [b][iscal][dir][side].                                */
 int oc_set_scalars(oc_case *c, int nscal, const double *alphai, double beta, const double *gacc,
                    const double *ssource, const int *is_sforced, const double *scalf,
                    const char *cbcscal, const double *bcscal);
+/* the _BOUSSINESQ_BUOYANCY build (rk.f90:104-108) as a switch, off by
+ * default; on needs nscal > 0 (param.f90:166-170): returns -1 otherwise   */
+int oc_set_buoyancy(oc_case *c, int enable);
 
 /* ---- hot-path routines (all blocks at once) ---- */
 /* rk_scal rk.f90:130-204 + bulk_forcing_s scal.f90:182-199 for every scalar;

@@ -83,6 +83,8 @@ def _load():
     L.snac_chkdiv.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.snac_chkdt.argtypes = [vp, C.POINTER(C.c_double)]
     L.snac_init_bounds.argtypes = [vp]
+    L.snac_initflow.argtypes = [vp]
+    L.snac_set_buoyancy.argtypes = [vp, C.c_int]
     L.snac_step.argtypes = [vp, C.c_double]
     L.snac_steps.argtypes = [vp, C.c_double, C.c_int]
     L.snac_sync.argtypes = [vp]
@@ -272,6 +274,16 @@ class Context:
     def init_bounds(self):
         _check(lib.snac_init_bounds(self.h), "init_bounds")
 
+    def set_buoyancy(self, enable):
+        """The reference's _BOUSSINESQ_BUOYANCY build as a switch (off by
+        default): rk_mom adds the buoyancy of scalar 1 (rk.f90:104-108)."""
+        _check(lib.snac_set_buoyancy(self.h, int(bool(enable))), "set_buoyancy")
+
+    def initflow(self):
+        """Initial fields of every block's block_inivel / iniscal
+        (initflow.f90:10-262, no noise); ghosts and p zero."""
+        _check(lib.snac_initflow(self.h), "initflow")
+
     def step(self, dt):
         _check(lib.snac_step(self.h, dt), "step")
 
@@ -323,8 +335,8 @@ class Context:
         MG-PCG path, the V-cycle in FP32 under a flexible CG whose residuals,
         operator, dot products and null-space removal stay FP64 (same
         stopping tolerance).  fillps, correc, the ghost fills and the direct
-        tensor-product solver stay FP64.  learned: as mixed, with the trained
-        network preconditioning the CG."""
+        tensor-product solver stay FP64.  learned: FP64 stencils, the trained
+        network preconditioning the flexible CG of the mixed mode."""
         _check(lib.snac_set_precision(self.h, self.PRECISIONS[mode]), "set_precision")
 
     def learned_collect(self, npairs):

@@ -395,6 +395,21 @@ Case build_case(const std::string &dns_path, const std::string &blocks_path, int
               double f2 = q2 ? pois(B.rc[a2][t2], B.lmin[a2], B.lmax[a2]) : 1.0;
               B.vin[q][q][s][t1 + (size_t)m1 * t2] = uref * f1 * f2;
             }
+        } else if (B.inflow_type[q][s] == 2) {
+          // Kovasznay flow at x0 = -0.5 with lref = 1 (initflow.f90:407-416):
+          // normal 1 - e^(lambda x0) cos(2 pi x2c), first tangential 0,
+          // second tangential lambda/(2 pi) e^(lambda x0) sin(2 pi x2f), x2 the
+          // second tangential axis
+          const double pi = std::acos(-1.0);
+          const double reb = uref * 1.0 / c.visc;
+          const double lambda = reb / 2 - std::sqrt(reb * reb / 4 + 4 * (pi * pi));
+          const double x0 = -0.5;
+          for (int t2 = 0; t2 < m2; t2++)
+            for (int t1 = 0; t1 < m1; t1++) {
+              const size_t e = t1 + (size_t)m1 * t2;
+              B.vin[q][q][s][e] = 1.0 - std::exp(lambda * x0) * std::cos(2 * pi * B.rc[a2][t2]);
+              B.vin[q][a2][s][e] = lambda / (2.0 * pi) * std::exp(lambda * x0) * std::sin(2 * pi * B.rf[a2][t2]);
+            }
         } else if (B.inflow_type[q][s] != 0) {
           throw std::runtime_error("inflow type " + std::to_string(B.inflow_type[q][s]) + " is not implemented");
         }

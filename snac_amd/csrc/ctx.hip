@@ -105,6 +105,7 @@ struct snac_ctx {
   double *sred = nullptr, *sforce = nullptr;  // scalar bulk mean / forcing
   snac::TensorPlan tp;
   bool impdiff = false;    // implicit diffusion (the reference's _IMPDIFF build)
+  bool buoyancy = false;   // Boussinesq buoyancy (the reference's _BOUSSINESQ_BUOYANCY build)
   std::vector<snac::ImpField> imp;  // its Helmholtz operators of u, v, w and the scalars
   snac::MGPlan mg;
   bool mg_planned = false;
@@ -848,14 +849,14 @@ void autotune(snac_ctx *c) {
   SNAC_HIP(hipStreamSynchronize(c->stream));
 }
 
-// Boussinesq buoyancy of scalar 1 whenever scalars are transported (the
-// reference's _BOUSSINESQ_BUOYANCY build); skipped when gacc*beta vanishes,
-// where the term is exactly zero
+// Boussinesq buoyancy of scalar 1 when switched on (the reference's
+// _BOUSSINESQ_BUOYANCY build, snac_set_buoyancy); skipped when gacc*beta
+// vanishes, where the term is exactly zero
 void rk_mom(snac_ctx *c, int irk, double dt) {
   const double f1 = RKC[irk - 1][0] * dt, f2 = RKC[irk - 1][1] * dt, f12 = f1 + f2;
   const Case &cs = c->cs;
   const double gb[3] = {cs.gacc[0] * cs.beta, cs.gacc[1] * cs.beta, cs.gacc[2] * cs.beta};
-  const bool buoy = cs.nscal > 0 && (gb[0] != 0.0 || gb[1] != 0.0 || gb[2] != 0.0);
+  const bool buoy = c->buoyancy && cs.nscal > 0 && (gb[0] != 0.0 || gb[1] != 0.0 || gb[2] != 0.0);
   int nx = 1 - c->cur;
   for (auto &D : c->blk) {
     if (c->st32)
@@ -1418,9 +1419,10 @@ int snac_set_precision(snac_ctx *c, int mode) {
     if (mode == SNAC_PREC_LEARNED && !(c->mg_planned && c->mg.net))
       throw std::runtime_error("the learned preconditioner needs a trained network (snac_learned_train)");
     if (c->device >= 0) SNAC_HIP(hipStreamSynchronize(c->stream));
-    // the momentum right-hand side runs in FP32 in the mixed and learned
-    // modes; the old tendencies move to the other precision's array
-    const bool st32 = mode != SNAC_PREC_FP64;
+    // the momentum right-hand side runs in FP32 in the mixed mode (the
+    // learned mode changes only the preconditioner); the old tendencies move
+    // to the other precision's array
+    const bool st32 = mode == SNAC_PREC_MIXED;
     if (st32 != c->st32 && c->device >= 0) {
       for (auto &D : c->blk) {
         if (st32) launch_cvt_d2f(c->stream, D.dto[0], D.dto32, 3 * D.nin);
@@ -1890,6 +1892,16 @@ int snac_updt_pressure(snac_ctx *c) {
 int snac_updt_pressure_impdiff(snac_ctx *c, double alpha) {
   return guard([&] {
     for (auto &D : on_device(c)->blk) launch_updt_pressure_imp(c->stream, D.g, D.m, D.pp, D.p, alpha);
+  });
+}
+
+int snac_set_buoyancy(snac_ctx *c, int enable) {
+  return guard([&] {
+    checked(c);
+    if (enable && c->cs.nscal <= 0) throw std::runtime_error("_BOUSSINESQ_BUOYANCY requires nscal > 0.");
+    if (c->stream) SNAC_HIP(hipStreamSynchronize(c->stream));
+    c->buoyancy = enable != 0;
+    drop_graph(c);
   });
 }
 

@@ -82,12 +82,23 @@ void out0d(const std::string &path, const std::vector<double> &v) {
 int main(int argc, char **argv) {
   const int fft_axis = argc > 1 ? std::atoi(argv[1]) : 0;
   const int device = argc > 2 ? std::atoi(argv[2]) : 0;
-  const bool impdiff = argc > 3 && std::string(argv[3]) == "impdiff";
+  // the reference's build options IMPDIFF=1 and BOUSSINESQ_BUOYANCY=1 as words after the device
+  bool impdiff = false, buoyancy = false;
+  for (int q = 3; q < argc; q++) {
+    const std::string o = argv[q];
+    if (o == "impdiff") impdiff = true;
+    else if (o == "buoyancy") buoyancy = true;
+    else {
+      std::fprintf(stderr, "unknown option %s (impdiff, buoyancy)\n", argv[q]);
+      return 2;
+    }
+  }
   const std::string datadir = "data/";
   std::filesystem::create_directories(datadir);
   check(snac_ctx_create_from_nml(&g_ctx, "dns.nml", "blocks.nml", fft_axis, device), "set-up");
   snac_ctx *c = g_ctx;
   if (impdiff) check(snac_set_impdiff(c, 1), "implicit diffusion");
+  if (buoyancy) check(snac_set_buoyancy(c, 1), "Boussinesq buoyancy");
   double dp[11];
   int ip[8], rp[8];
   check(snac_ctx_get_params(c, dp, ip), "params");

@@ -33,6 +33,29 @@ __device__ __forceinline__ T nb_sum(const MGLevT<T> &L, const T *__restrict__ x,
   return r;
 }
 
+// nb_sum of an FFT-path level (no m couplings) whose neighbour lines still
+// lack the prolongation: each neighbour reads x + xc of its coarse cell
+// (active cells only), the value the separate prolongation pass would have
+// stored there (xc: the next coarser level's x, cna columns)
+template <typename T>
+__device__ __forceinline__ T nb_sum_pro(const MGLevT<T> &L, const T *__restrict__ x, int m, int a, int s, long e,
+                                        long q, const T *__restrict__ xc, int cna) {
+  const long sa = L.ns;
+  const int ra = cna != L.na;
+  auto v = [&](int an, long en, long qn) -> T {
+    T xv = x[en];
+    if (L.act[qn]) xv += xc[((long)m * cna + (ra ? an >> 1 : an)) * sa + s];
+    return xv;
+  };
+  const long w = (long)(L.na - 1) * sa;
+  T r = 0;
+  if (a < L.na - 1) r += L.ca[q] * v(a + 1, e + sa, q + sa);
+  else if (L.per_a) r += L.ca[q] * v(0, e - w, q - w);
+  if (a > 0) r += L.ca[q - sa] * v(a - 1, e - sa, q - sa);
+  else if (L.per_a) r += L.ca[q + w] * v(L.na - 1, e + w, q + w);
+  return r;
+}
+
 template <typename T>
 __device__ __forceinline__ T mx_at(const MGLevT<T> &L, T lm, const T *__restrict__ x, int m, int a, int s) {
   const long e = vidx(L, m, a, s), q = cidx(L, m, a, s);
@@ -88,9 +111,12 @@ __device__ __forceinline__ Row<T> pcr(const Row<T> &r, const Row<T> &lo, const R
 // LPL lanes per line (32 or 64): a wavefront relaxes 64 / LPL lines at once,
 // each in its own lane segment (the shuffles never cross segments); act =
 // false leaves the segment idle (no loads or stores).
+// xc (FFT path only): the neighbour lines get the prolongation of the
+// coarser level's xc (cna columns) added as they are read (nb_sum_pro)
 template <int K, int LPL, typename T>
 __device__ __forceinline__ void line_solve(const MGLevT<T> &L, T lmv, T *__restrict__ x, const T *__restrict__ b,
-                                           int m, int a, int zero_nb, int pin_s, bool pin, bool act) {
+                                           int m, int a, int zero_nb, int pin_s, bool pin, bool act,
+                                           const T *__restrict__ xc = nullptr, int cna = 0) {
   const int wl = threadIdx.x & 63, lane = wl & (LPL - 1), seg0 = wl - lane;
   const long e0 = vidx(L, m, a, 0), q0 = cidx(L, m, a, 0);
   const int ns = L.ns;
@@ -108,7 +134,7 @@ __device__ __forceinline__ void line_solve(const MGLevT<T> &L, T lmv, T *__restr
       T l = s > 0 ? -L.cs[q - 1] : T(0);
       T u = s < ns - 1 ? -L.cs[q] : T(0);
       T r = b[e];
-      if (!zero_nb) r += nb_sum(L, x, m, a, e, q);
+      if (!zero_nb) r += xc ? nb_sum_pro(L, x, m, a, s, e, q, xc, cna) : nb_sum(L, x, m, a, e, q);
       if (pin && s == pin_s) { d = 1; l = 0; u = 0; r = 0; }
       lo[j] = l; dg[j] = d; up[j] = u; rh[j] = r;
     }
@@ -186,7 +212,7 @@ template <int K, int LPL, typename T>
 __global__ __launch_bounds__(256) void k_line(MGLevT<T> L, const T *__restrict__ lam, T *__restrict__ x,
                                               const T *__restrict__ b, int col_a, int col_m, int nla, int nlm,
                                               int zero_nb, int pin_s, const int *__restrict__ singular,
-                                              const int *__restrict__ done) {
+                                              const int *__restrict__ done, const T *__restrict__ xc, int cna) {
   const long line = ((long)blockIdx.x * 256 + threadIdx.x) / LPL;
   if (line >= (long)nla * nlm) return;  // a whole lane segment leaves together
   const int a = 2 * (int)(line % nla) + col_a;
@@ -196,7 +222,7 @@ __global__ __launch_bounds__(256) void k_line(MGLevT<T> L, const T *__restrict__
   if (done && done[g]) return;
   const T lmv = lam ? lam[m] : T(0);
   const bool pin = pin_s >= 0 && singular && singular[g];
-  line_solve<K, LPL, T>(L, lmv, x, b, m, a, zero_nb, pin_s, pin, true);
+  line_solve<K, LPL, T>(L, lmv, x, b, m, a, zero_nb, pin_s, pin, true, xc, cna);
 }
 
 // The coarse end of the V-cycle (levels t.l0 .. t.nl-1, the last one a
@@ -425,6 +451,122 @@ __global__ __launch_bounds__(256) void k_cg_update_dot(long gsize, const double 
   }
 }
 
+// (M v)(m, a, s) of an FP64 level where v is given element-wise by f(e)
+template <typename F>
+__device__ __forceinline__ double mx_f(const MGLev &L, double lm, F &&f, int m, int a, int s, long e, long q) {
+  double y = (L.dg[q] - lm * L.w[q]) * f(e);
+  if (s > 0) y -= L.cs[q - 1] * f(e - 1);
+  if (s < L.ns - 1) y -= L.cs[q] * f(e + 1);
+  double r = 0;
+  const long sa = L.ns;
+  if (a < L.na - 1) r += L.ca[q] * f(e + sa);
+  else if (L.per_a) r += L.ca[q] * f(e - (long)(L.na - 1) * sa);
+  if (a > 0) r += L.ca[q - sa] * f(e - sa);
+  else if (L.per_a) r += L.ca[q + (long)(L.na - 1) * sa] * f(e + (long)(L.na - 1) * sa);
+  if (L.coupled) {
+    const long sm = (long)L.na * L.ns;
+    if (m < L.nm - 1) r += L.cm[q] * f(e + sm);
+    else if (L.per_m) r += L.cm[q] * f(e - (long)(L.nm - 1) * sm);
+    if (m > 0) r += L.cm[q - L.qm] * f(e - sm);
+    else if (L.per_m) r += L.cm[q + (long)(L.nm - 1) * L.qm] * f(e + (long)(L.nm - 1) * sm);
+  }
+  return y - r;
+}
+
+__device__ __forceinline__ double cg_beta(int g, const double *__restrict__ rzn, const double *__restrict__ rzo,
+                                          const double *__restrict__ cross) {
+  if (!rzo) return 0.0;
+  const double den = rzo[g * 2];
+  const double num = rzn[g * 2] - (cross ? cross[g * 2 + 1] : 0.0);
+  return den != 0.0 ? num / den : 0.0;
+}
+
+// The search direction and the operator in one pass: p = z + beta p_prev
+// (rounded to TP, the stored type), and the per-group partial sums of
+// p . (M p), M p computed from the new directions of the neighbours formed
+// on the fly (p_prev and p are different buffers).  Same element order per
+// thread as k_dot, so the sums match k_apply_dot's on the same p.
+template <typename TZ, typename TP>
+__global__ __launch_bounds__(256) void k_dir_apply_dot(MGLev L, const double *__restrict__ lam, long gsize,
+                                                       const double *__restrict__ rzn, const double *__restrict__ rzo,
+                                                       const double *__restrict__ cross, const TZ *__restrict__ z,
+                                                       const TP *__restrict__ pp, TP *__restrict__ p,
+                                                       double *__restrict__ part, int nchunk,
+                                                       const int *__restrict__ done) {
+  __shared__ double sh[4];
+  const int g = blockIdx.y;
+  double s1 = 0.0;
+  if (!done[g]) {
+    const double be = cg_beta(g, rzn, rzo, cross);
+    auto pn = [&](long e) -> double {
+      if (be == 0.0) return (double)(TP)(double)z[e];
+      return (double)(TP)((double)z[e] + be * (double)pp[e]);
+    };
+    const long base = (long)g * gsize;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < gsize; i += (long)nchunk * 256) {
+      const long idx = base + i;
+      const int s = (int)(idx % L.ns);
+      const long t = idx / L.ns;
+      const int a = (int)(t % L.na), m = (int)(t / L.na);
+      const double pc = pn(idx);
+      p[idx] = (TP)pc;
+      const double y = mx_f(L, lam ? lam[m] : 0.0, pn, m, a, s, idx, cidx(L, m, a, s));
+      s1 += pc * y;
+    }
+  }
+  s1 = wave_sum(s1);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s1;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[((long)g * nchunk + blockIdx.x) * 2] = sh[0] + sh[1] + sh[2] + sh[3];
+    part[((long)g * nchunk + blockIdx.x) * 2 + 1] = 0.0;
+  }
+}
+
+// x += alpha p ; r -= alpha (M p) with alpha = rz / pq per group, M p
+// recomputed from p (bit for bit the value k_dir_apply_dot summed), the
+// partial sums of r.r and, when zf is given, zf.r, and rf = (float) r when
+// given
+template <typename TP>
+__global__ __launch_bounds__(256) void k_update_apply_dot(MGLev L, const double *__restrict__ lam, long gsize,
+                                                          const double *__restrict__ rz,
+                                                          const double *__restrict__ pq, double *__restrict__ x,
+                                                          const TP *__restrict__ p, double *__restrict__ r,
+                                                          float *__restrict__ rf, const float *__restrict__ zf,
+                                                          double *__restrict__ part, int nchunk,
+                                                          const int *__restrict__ done) {
+  __shared__ double sh[2][4];
+  const int g = blockIdx.y;
+  double s1 = 0.0, s2 = 0.0;
+  if (!done[g]) {
+    const double den = pq[g * 2];
+    const double al = den != 0.0 ? rz[g * 2] / den : 0.0;
+    auto pv = [&](long e) -> double { return (double)p[e]; };
+    const long base = (long)g * gsize;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < gsize; i += (long)nchunk * 256) {
+      const long idx = base + i;
+      const int s = (int)(idx % L.ns);
+      const long t = idx / L.ns;
+      const int a = (int)(t % L.na), m = (int)(t / L.na);
+      const double q = mx_f(L, lam ? lam[m] : 0.0, pv, m, a, s, idx, cidx(L, m, a, s));
+      x[idx] += al * (double)p[idx];
+      const double rn = r[idx] - al * q;
+      r[idx] = rn;
+      s1 += rn * rn;
+      if (rf) rf[idx] = (float)rn;
+      if (zf) s2 += (double)zf[idx] * rn;
+    }
+  }
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  if ((threadIdx.x & 63) == 0) { sh[0][threadIdx.x >> 6] = s1; sh[1][threadIdx.x >> 6] = s2; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[((long)g * nchunk + blockIdx.x) * 2] = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
+    part[((long)g * nchunk + blockIdx.x) * 2 + 1] = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+  }
+}
+
 // one wavefront per group sums its chunks (skipped groups keep their values)
 __global__ __launch_bounds__(64) void k_dot_final(const double *__restrict__ part, int nchunk, double *__restrict__ out,
                                                   const int *__restrict__ done, const int *__restrict__ only) {
@@ -539,22 +681,22 @@ unsigned blocks_for(long n) { return (unsigned)((n + 255) / 256); }
 
 template <int LPL, typename T>
 void mg_line_k(hipStream_t st, const MGLevT<T> &L, const T *lam, T *x, const T *b, int col_a, int col_m, int nla,
-               int nlm, int zero_nb, int pin_s, const int *singular, const int *done) {
+               int nlm, int zero_nb, int pin_s, const int *singular, const int *done, const T *xc, int cna) {
   const long lines = (long)nla * nlm;
   dim3 grid((unsigned)((lines * LPL + 255) / 256));
   const int rows = L.base + (L.extra ? 1 : 0);
   if (rows <= 3)
     hipLaunchKernelGGL((k_line<3, LPL, T>), grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb,
-                       pin_s, singular, done);
+                       pin_s, singular, done, xc, cna);
   else if (rows <= 5)
     hipLaunchKernelGGL((k_line<5, LPL, T>), grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb,
-                       pin_s, singular, done);
+                       pin_s, singular, done, xc, cna);
   else if (rows <= 9)
     hipLaunchKernelGGL((k_line<9, LPL, T>), grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb,
-                       pin_s, singular, done);
+                       pin_s, singular, done, xc, cna);
   else if (rows <= 17)
     hipLaunchKernelGGL((k_line<17, LPL, T>), grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb,
-                       pin_s, singular, done);
+                       pin_s, singular, done, xc, cna);
   else
     throw std::runtime_error("multigrid line relaxation: lines longer than 1088 cells are not supported");
   SNAC_HIP(hipGetLastError());
@@ -562,12 +704,13 @@ void mg_line_k(hipStream_t st, const MGLevT<T> &L, const T *lam, T *x, const T *
 
 template <typename T>
 void mg_line(hipStream_t st, const MGLevT<T> &L, const T *lam, T *x, const T *b, int col_a, int col_m, int zero_nb,
-             int pin_s, const int *singular, const int *done) {
+             int pin_s, const int *singular, const int *done, const T *xc, int cna) {
   const int nla = (L.na - col_a + 1) / 2;
   const int nlm = L.coupled ? (L.nm - col_m + 1) / 2 : L.nm;
   if (nla <= 0 || nlm <= 0) return;
-  if (L.lpl == 32) mg_line_k<32>(st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb, pin_s, singular, done);
-  else mg_line_k<64>(st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb, pin_s, singular, done);
+  if (xc && (L.coupled || zero_nb)) throw std::runtime_error("internal: fused prolongation needs an FFT-path sweep");
+  if (L.lpl == 32) mg_line_k<32>(st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb, pin_s, singular, done, xc, cna);
+  else mg_line_k<64>(st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb, pin_s, singular, done, xc, cna);
 }
 
 template <int LPL, typename T>
@@ -628,6 +771,40 @@ void mg_cg_update_dot(hipStream_t st, int ng, long gsize, const double *rz, cons
   SNAC_HIP(hipGetLastError());
 }
 
+template <typename TZ, typename TP>
+void mg_dir_apply_dot(hipStream_t st, const MGLev &L, const double *lam, int ng, long gsize, const double *rzn,
+                      const double *rzo, const double *cross, const TZ *z, const TP *p_prev, TP *p, double *part,
+                      int nchunk, double *out, const int *done) {
+  hipLaunchKernelGGL((k_dir_apply_dot<TZ, TP>), dim3(nchunk, ng), dim3(256), 0, st, L, lam, gsize, rzn, rzo, cross, z,
+                     p_prev, p, part, nchunk, done);
+  SNAC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_dot_final, dim3(ng), dim3(64), 0, st, part, nchunk, out, done, nullptr);
+  SNAC_HIP(hipGetLastError());
+}
+
+template <typename TP>
+void mg_update_apply_dot(hipStream_t st, const MGLev &L, const double *lam, int ng, long gsize, const double *rz,
+                         const double *pq, double *x, const TP *p, double *r, float *rf, const float *zf,
+                         double *part, int nchunk, double *out, const int *done) {
+  hipLaunchKernelGGL(k_update_apply_dot<TP>, dim3(nchunk, ng), dim3(256), 0, st, L, lam, gsize, rz, pq, x, p, r, rf,
+                     zf, part, nchunk, done);
+  SNAC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_dot_final, dim3(ng), dim3(64), 0, st, part, nchunk, out, done, nullptr);
+  SNAC_HIP(hipGetLastError());
+}
+template void mg_dir_apply_dot<double, double>(hipStream_t, const MGLev &, const double *, int, long, const double *,
+                                               const double *, const double *, const double *, const double *,
+                                               double *, double *, int, double *, const int *);
+template void mg_dir_apply_dot<float, float>(hipStream_t, const MGLev &, const double *, int, long, const double *,
+                                             const double *, const double *, const float *, const float *, float *,
+                                             double *, int, double *, const int *);
+template void mg_update_apply_dot<double>(hipStream_t, const MGLev &, const double *, int, long, const double *,
+                                          const double *, double *, const double *, double *, float *, const float *,
+                                          double *, int, double *, const int *);
+template void mg_update_apply_dot<float>(hipStream_t, const MGLev &, const double *, int, long, const double *,
+                                         const double *, double *, const float *, double *, float *, const float *,
+                                         double *, int, double *, const int *);
+
 void mg_to_float(hipStream_t st, long n, const double *src, float *dst) {
   hipLaunchKernelGGL(k_to_float, dim3(blocks_for(n)), dim3(256), 0, st, n, src, dst);
   SNAC_HIP(hipGetLastError());
@@ -686,7 +863,7 @@ void mg_copy3(hipStream_t st, const double *src, long sm, long sa, long ss, doub
 
 #define SNAC_MG_INST(T)                                                                                               \
   template void mg_line<T>(hipStream_t, const MGLevT<T> &, const T *, T *, const T *, int, int, int, int, const int *, \
-                           const int *);                                                                              \
+                           const int *, const T *, int);                                                              \
   template void mg_tail<T>(hipStream_t, const MGTailT<T> &, int, const T *, int, const int *, const int *);           \
   template void mg_restrict_residual<T>(hipStream_t, const MGLevT<T> &, const MGLevT<T> &, const T *, const T *,       \
                                         const T *, T *, const int *);                                                 \

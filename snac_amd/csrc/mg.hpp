@@ -76,10 +76,12 @@ void mg_tail(hipStream_t st, const MGTailT<T> &t, int nm, const T *lam, int pin_
              const int *done);
 // zebra line relaxation of the lines (m, a) with a % 2 == col_a (and, when
 // coupled, m % 2 == col_m); zero_nb: neighbouring lines read as zero;
-// pin_s >= 0: in singular groups row pin_s is replaced by x = 0
+// pin_s >= 0: in singular groups row pin_s is replaced by x = 0; xc (FFT
+// path): the neighbour lines are read with the prolongation of xc (the
+// next coarser level's x, cna columns) added, as mg_prolong would store it
 template <typename T>
 void mg_line(hipStream_t st, const MGLevT<T> &L, const T *lam, T *x, const T *b, int col_a, int col_m, int zero_nb,
-             int pin_s, const int *singular, const int *done);
+             int pin_s, const int *singular, const int *done, const T *xc = nullptr, int cna = 0);
 // coarse b = restriction of the fine residual b - M x (sum over the active
 // fine cells of each coarse cell)
 template <typename T>
@@ -100,6 +102,18 @@ void mg_apply_dot(hipStream_t st, const MGLev &L, const double *lam, const doubl
 void mg_cg_update_dot(hipStream_t st, int ng, long gsize, const double *rz, const double *pq, double *x,
                       const double *p, double *r, const double *q, float *rf, const float *zf, double *part,
                       int nchunk, double *out, const int *done);
+// the CG's direction update and operator in one pass: p = z + beta p_prev
+// (beta as mg_cg_dir; p stored as TP, p_prev and p distinct buffers) and
+// out[g*2] = p . (M p) per group
+template <typename TZ, typename TP>
+void mg_dir_apply_dot(hipStream_t st, const MGLev &L, const double *lam, int ng, long gsize, const double *rzn,
+                      const double *rzo, const double *cross, const TZ *z, const TP *p_prev, TP *p, double *part,
+                      int nchunk, double *out, const int *done);
+// mg_cg_update_dot with q = M p recomputed from p instead of read
+template <typename TP>
+void mg_update_apply_dot(hipStream_t st, const MGLev &L, const double *lam, int ng, long gsize, const double *rz,
+                         const double *pq, double *x, const TP *p, double *r, float *rf, const float *zf,
+                         double *part, int nchunk, double *out, const int *done);
 void mg_izero(hipStream_t st, long n, int *x);
 // dst = (float) src
 void mg_to_float(hipStream_t st, long n, const double *src, float *dst);
@@ -163,6 +177,11 @@ struct MGPlan {
   bool collecting = false;
   double *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
   float *zf = nullptr, *rf = nullptr;
+  // search directions of the CG iterations, ping-pong (the direction update
+  // reads the previous one around each cell while it writes the new one):
+  // FP64 (p and q) or FP32 (mixed and learned modes)
+  double *pd[2] = {nullptr, nullptr};
+  float *pf[2] = {nullptr, nullptr};
   double *part = nullptr, *bb = nullptr, *rr = nullptr, *pq = nullptr, *rz[2] = {nullptr, nullptr}, *sums = nullptr;
   double *nact = nullptr;
   int *done = nullptr, *singular = nullptr, *left = nullptr, *left_host = nullptr;
@@ -202,7 +221,7 @@ void mg_free(MGPlan &P);
 // count (graph capture) leaves this state behind, so the host can resume it
 // with convergence checks (mg_continue).
 struct MGRun {
-  int it = 0, cur = 0;
+  int it = 0, cur = 0, pc = 0;
   bool net = false, record = false;
 };
 // Solves M x = -rhs where rhs (in the plan's layout) has been copied into

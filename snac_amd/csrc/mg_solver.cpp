@@ -418,6 +418,10 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
       P.z = dev_zeros<double>(P, n);
       P.p = dev_zeros<double>(P, n);
       P.q = dev_zeros<double>(P, n);
+      P.pd[0] = P.p;
+      P.pd[1] = P.q;
+      P.pf[0] = dev_zeros<float>(P, n);
+      P.pf[1] = dev_zeros<float>(P, n);
       P.zf = dev_zeros<float>(P, n);
       P.rf = dev_zeros<float>(P, n);
       P.h64.xl.push_back(P.z);
@@ -501,11 +505,19 @@ void vcycle(MGPlan &P, MGHier<T> &H, hipStream_t st, size_t l) {
   }
   mg_restrict_residual(st, L, H.lev[l + 1], H.lam, x, b, H.bl[l + 1], P.done);
   vcycle(P, H, st, l + 1);
-  mg_prolong(st, L, H.lev[l + 1], (const T *)H.xl[l + 1], x, P.done);
+  // two colours (FFT path): the first post-sweep relaxes colour 1 reading
+  // the colour-0 lines with the prolongation added on the fly, and colour
+  // 0 is then relaxed from scratch, so the prolongation is never stored.
+  // The coupled colours of the 3D path read not-yet-relaxed lines twice:
+  // prolong first.
+  const bool fuse = nc == 2 && !L.coupled;
+  if (!fuse) mg_prolong(st, L, H.lev[l + 1], (const T *)H.xl[l + 1], x, P.done);
   for (int k = nc - 1; k >= 0; k--) {
     const bool mk = l == 0 && P.mark_line0;
     if (mk) P.mark_line0(true);
-    mg_line(st, L, H.lam, x, b, col[k][0], col[k][1], 0, -1, nullptr, P.done);
+    const bool pro = fuse && k == nc - 1;
+    mg_line(st, L, H.lam, x, b, col[k][0], col[k][1], 0, -1, nullptr, P.done, pro ? (const T *)H.xl[l + 1] : nullptr,
+            pro ? H.lev[l + 1].na : 0);
     if (mk) P.mark_line0(false);
   }
 }
@@ -518,9 +530,9 @@ void project(MGPlan &P, hipStream_t st, T *v) {
 }
 
 // z = V-cycle(r) (or the network's answer, net) with its null-space
-// component removed; rz = r.z; p = z + beta p (first: p = z).  rr: the
-// per-group r.r of the current residual (the network's input scale)
-void precondition(MGPlan &P, hipStream_t st, double *rz_new, const double *rz_old, bool net, const double *rr) {
+// component removed; rz = r.z.  rr: the per-group r.r of the current
+// residual (the network's input scale)
+void precondition(MGPlan &P, hipStream_t st, double *rz_new, bool net, const double *rr) {
   if (P.prec >= 1) {
     if (net) {
       const MGLev &L0 = P.h64.lev[0];
@@ -533,12 +545,10 @@ void precondition(MGPlan &P, hipStream_t st, double *rz_new, const double *rz_ol
     }
     project(P, st, P.zf);
     mg_dot<float, float>(st, P.ng, P.gsize, P.r, P.zf, nullptr, nullptr, P.part, P.nchunk, rz_new, P.done, nullptr);
-    mg_cg_dir<float>(st, P.ng, P.gsize, rz_new, rz_old, rz_old ? P.rr : nullptr, P.zf, P.p, P.done);
   } else {
     vcycle(P, P.h64, st, 0);
     project(P, st, P.z);
     mg_dot<double, double>(st, P.ng, P.gsize, P.r, P.z, nullptr, nullptr, P.part, P.nchunk, rz_new, P.done, nullptr);
-    mg_cg_dir<double>(st, P.ng, P.gsize, rz_new, rz_old, nullptr, P.z, P.p, P.done);
   }
 }
 
@@ -551,14 +561,31 @@ namespace {
 // convergence flags
 void mg_iteration(MGPlan &P, hipStream_t st, MGRun &R) {
   const bool mixed = P.prec >= 1;
+  const double *rzo = nullptr;
   if (R.it > 0) {
-    precondition(P, st, P.rz[1 - R.cur], P.rz[R.cur], R.net, P.rr);
+    precondition(P, st, P.rz[1 - R.cur], R.net, P.rr);
+    rzo = P.rz[R.cur];
     R.cur = 1 - R.cur;
   }
-  mg_apply_dot(st, P.h64.lev[0], P.h64.lam, P.p, P.q, P.ng, P.gsize, P.part, P.nchunk, P.pq, P.done);
-  // the update with r.r and, for the flexible form, r_new . z_old in the second slot
-  mg_cg_update_dot(st, P.ng, P.gsize, P.rz[R.cur], P.pq, P.x, P.p, P.r, P.q, mixed ? P.rf : nullptr,
-                   mixed ? P.zf : nullptr, P.part, P.nchunk, P.rr, P.done);
+  // p = z + beta p (first iteration: p = z) with p.(M p); beta of the
+  // flexible form (FP32 preconditioners) takes r_new . z_old off rz_new.
+  // Then x += alpha p, r -= alpha M p with r.r and, for the flexible form,
+  // r_new . z_old in the second slot.  M p is applied twice rather than
+  // stored: the second pass recomputes it from p.
+  const MGLev &L0 = P.h64.lev[0];
+  if (mixed) {
+    mg_dir_apply_dot<float, float>(st, L0, P.h64.lam, P.ng, P.gsize, P.rz[R.cur], rzo, rzo ? P.rr : nullptr, P.zf,
+                                   P.pf[R.pc], P.pf[1 - R.pc], P.part, P.nchunk, P.pq, P.done);
+    R.pc = 1 - R.pc;
+    mg_update_apply_dot<float>(st, L0, P.h64.lam, P.ng, P.gsize, P.rz[R.cur], P.pq, P.x, P.pf[R.pc], P.r, P.rf, P.zf,
+                               P.part, P.nchunk, P.rr, P.done);
+  } else {
+    mg_dir_apply_dot<double, double>(st, L0, P.h64.lam, P.ng, P.gsize, P.rz[R.cur], rzo, nullptr, P.z, P.pd[R.pc],
+                                     P.pd[1 - R.pc], P.part, P.nchunk, P.pq, P.done);
+    R.pc = 1 - R.pc;
+    mg_update_apply_dot<double>(st, L0, P.h64.lam, P.ng, P.gsize, P.rz[R.cur], P.pq, P.x, P.pd[R.pc], P.r, nullptr,
+                                nullptr, P.part, P.nchunk, P.rr, P.done);
+  }
   mg_flags(st, P.ng, P.rr, P.bb, P.tol * P.tol, P.done, P.left);
   if (R.net && P.fallback) launch_net_stall(st, P.ng, P.rr, P.rr_prev, P.done, P.stall_ratio * P.stall_ratio, P.stall);
   R.it++;
@@ -611,7 +638,7 @@ MGRun mg_solve(MGPlan &P, hipStream_t st, int fixed) {
     mg_izero(st, 1, P.stall);
     SNAC_HIP(hipMemcpyAsync(P.rr_prev, rr0, 2 * P.ng * sizeof(double), hipMemcpyDeviceToDevice, st));
   }
-  precondition(P, st, P.rz[R.cur], nullptr, R.net, rr0);
+  precondition(P, st, P.rz[R.cur], R.net, rr0);
   if (fixed > 0) {  // captured: counted when the host resumes it (mg_continue)
     const int nit = std::min(fixed, P.maxiter);
     while (R.it < nit) mg_iteration(P, st, R);

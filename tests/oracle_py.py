@@ -41,6 +41,7 @@ def lib():
                                 C.c_int, C.c_double, dp, C.c_char_p, C.c_int]
         L.oc_destroy.argtypes = [C.c_void_p]
         L.oc_set_solver.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
+        L.oc_set_buoyancy.argtypes = [C.c_void_p, C.c_int]
         L.oc_set_threads.argtypes = [C.c_int]
         L.oc_nblocks.argtypes = [C.c_void_p]
         L.oc_block_info.argtypes = [C.c_void_p, C.c_int, ip, ip, ip, ip]
@@ -144,6 +145,11 @@ class OracleCase:
 
     def set_solver(self, method="auto", tol=1e-13, maxiter=20000):
         lib().oc_set_solver(self.h, SOLVERS[method], tol, maxiter)
+
+    def set_buoyancy(self, enable):
+        """The _BOUSSINESQ_BUOYANCY build (rk.f90:104-108); off by default."""
+        if lib().oc_set_buoyancy(self.h, int(enable)) != 0:
+            raise RuntimeError("_BOUSSINESQ_BUOYANCY requires nscal > 0.")
 
     def block_info(self, b):
         n = np.zeros(3, np.int32); lo = np.zeros(3, np.int32); nbr = np.zeros(6, np.int32); per = np.zeros(3, np.int32)

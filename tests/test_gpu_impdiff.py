@@ -161,6 +161,8 @@ def test_dhc_reference_case_steps(tmp_path, fft_axis):
     c["impdiff"] = True
     oc = oracle_py.OracleCase(c)
     oc.set_solver("band")
+    ctx.set_buoyancy(True)  # the reference's test builds with BOUSSINESQ_BUOYANCY=1
+    oc.set_buoyancy(True)
     rng = np.random.default_rng(4)
     in3 = (slice(1, -1),) * 3
     for f in ("u", "v", "w", "s1"):

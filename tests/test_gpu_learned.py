@@ -81,9 +81,12 @@ def test_learned_preconditioner(tmp_path):
     ref = res["fp64"][2]
     for k in ("mixed", "learned", "learned+guess"):
         assert res[k][1] <= 1e-13 * 1.0001
+        # the learned mode changes only the preconditioner (FP64 stencils);
+        # the mixed mode also evaluates the momentum right-hand side in FP32
+        tol = 1e-6 if k == "mixed" else 1e-10
         for f in ("u", "v", "w"):
             g, o = res[k][2][f][1:-1, 1:-1, 1:-1], ref[f][1:-1, 1:-1, 1:-1]
-            assert np.linalg.norm(g - o) <= 1e-10 * np.linalg.norm(o), (k, f)
+            assert np.linalg.norm(g - o) <= tol * np.linalg.norm(o), (k, f)
     trained_its, trained_rel = res["network only"][:2]
     ctx.close()
     # the same network untrained (one epoch at zero learning rate)

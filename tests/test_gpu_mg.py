@@ -8,9 +8,11 @@ correction agrees to ~1e-11 and the fields after 10 steps within the 1e-10
 gate.  p / pp of singular (all-Neumann or periodic) problems are compared
 modulo their mean.
 
-Both precisions run every case: "mixed" (FP32 V-cycle under a flexible CG
-with FP64 vectors, operator and dot products) stops at the same FP64
-residual tolerance, so it is held to the same bounds.
+Both precisions run every case.  A "mixed" solve (FP32 V-cycle and search
+directions under a flexible CG with FP64 residuals, operator and dot
+products) stops at the same FP64 residual tolerance, so its single solves
+are held to the same bounds; its 10-step runs also carry the FP32 momentum
+right-hand side and are held to the mixed form's 1e-6.
 """
 import os
 
@@ -107,10 +109,13 @@ def test_mg_matches_tensor_solver(tmp_path):
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("geom", ["lchannel", "bfs"])
 def test_mg_10_steps(tmp_path, geom, prec):
+    """FP64: the plain form's 1e-10; mixed (FP32 momentum right-hand side,
+    FP32 V-cycle and search directions): the mixed form's 1e-6 gate."""
     ctx, oc, singular = pair(tmp_path, geom, prec)
-    run_both(ctx, oc, 10)
-    assert_fields(ctx, oc, FIELDS3, 1e-10, interior=True)
-    assert_fields(ctx, oc, ("p",), 1e-10, interior=True, modmean=("p",) if singular else ())
+    tol = 1e-10 if prec == "fp64" else 1e-6
+    run_both(ctx, oc, 10, dt_rel=1e-12 if prec == "fp64" else 1e-6)
+    assert_fields(ctx, oc, FIELDS3, tol, interior=True)
+    assert_fields(ctx, oc, ("p",), tol, interior=True, modmean=("p",) if singular else ())
     _, divmax = ctx.chkdiv()
     _, odivmax = oc.chkdiv()
     print(f"{geom} {prec}: divmax GPU {divmax:.2e} oracle {odivmax:.2e}", flush=True)

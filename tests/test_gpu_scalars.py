@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 SCAL_FIELDS = ("u", "v", "w", "p", "s1", "dsdtrko1", "s2", "dsdtrko2")
 
 
-def scalar_pair(tmp_path, forced=False, fa=0, gacc=(0.3, -0.2, -1.0), beta=0.7):
+def scalar_pair(tmp_path, forced=False, fa=0, gacc=(0.3, -0.2, -1.0), beta=0.7, buoyancy=True):
     import snac_amd
     blk = cases.cavity_block((12, 10, 8), gr=(0.0, 1.4, 0.0))
     blk["cbcscal"] = [[["D", "N"], ["N", "D"], ["N", "N"]], [["N", "N"], ["D", "D"], ["N", "N"]]]
@@ -29,6 +29,9 @@ def scalar_pair(tmp_path, forced=False, fa=0, gacc=(0.3, -0.2, -1.0), beta=0.7):
     ctx = snac_amd.Context(d, b, fa)
     oc = oracle_py.OracleCase(nmlparse.load_case(d, b, fa))
     oc.set_solver("band")
+    if buoyancy:  # the reference's BOUSSINESQ_BUOYANCY=1 build
+        ctx.set_buoyancy(True)
+        oc.set_buoyancy(True)
     return ctx, oc
 
 
@@ -80,6 +83,21 @@ def test_rk_mom_buoyancy(tmp_path, variant):
         assert_fields(ctx, oc, ("u", "v", "w", "dudtrko", "dvdtrko", "dwdtrko"), 1e-13, interior=True)
 
 
+def test_buoyancy_off_by_default(tmp_path):
+    """Without the switch (a default reference build) gacc and beta leave
+    rk_mom alone, as rk.f90:104-108 compiles the term only under
+    _BOUSSINESQ_BUOYANCY; on, the velocities change."""
+    off, oc = scalar_pair(tmp_path, buoyancy=False)
+    on, _ = scalar_pair(tmp_path / "on")
+    seed(off, oc, seed=5, names=("u", "v", "w", "p", "s1"))
+    seed(on, oc, seed=5, names=("u", "v", "w", "p", "s1"))
+    off.rk_mom(1, 1.5e-3)
+    on.rk_mom(1, 1.5e-3)
+    oc.rk_mom(1, 1.5e-3)
+    assert_fields(off, oc, ("u", "v", "w"), 1e-13, interior=True)
+    assert np.abs(on.get(0, "w") - off.get(0, "w")).max() > 1e-6
+
+
 @pytest.mark.parametrize("forced", [False, True])
 def test_scalars_10_steps(tmp_path, forced):
     ctx, oc = scalar_pair(tmp_path, forced=forced)
@@ -98,6 +116,8 @@ def test_dhc_reference_case():
     ctx = snac_amd.Context(dn, bn, 2)
     oc = oracle_py.OracleCase(nmlparse.load_case(dn, bn, 2))
     oc.set_solver("band")
+    ctx.set_buoyancy(True)  # testit.sh builds with BOUSSINESQ_BUOYANCY=1
+    oc.set_buoyancy(True)
     xc = ctx.metric(0, 6)
     s = np.zeros(ctx.shape(0, "s1"))
     s[:, :, :] = ((1.0 - xc) * -0.5 + xc * 0.5)[:, None, None]  # iniscal 'dhc', initflow.f90:247-256

@@ -75,6 +75,7 @@ def test_boussinesq_term(tmp_path):
     -f12*gacc*beta*c (the uniform velocity it creates has no advection or
     diffusion), so three substages give -dt*gacc*beta*c (no projection)."""
     oc = scalar_box(tmp_path, gacc=(0.0, 0.0, -2.0), beta=0.5)
+    oc.set_buoyancy(True)
     oc.set(0, "s1", np.full(oc.shape(0, "s1"), 3.0))
     dt = 1e-2
     for irk in (1, 2, 3):
@@ -112,6 +113,7 @@ def test_dhc_reference_checks():
     c = nmlparse.load_case(dn, bn, 2)
     oc = oracle_py.OracleCase(c)
     oc.set_solver("band")
+    oc.set_buoyancy(True)  # testit.sh builds with BOUSSINESQ_BUOYANCY=1
     xc = oc.metric(0, 6)
     s = np.zeros(oc.shape(0, "s1"))
     s[:, :, :] = ((1.0 - xc) * -0.5 + xc * 0.5)[:, None, None]
@@ -126,3 +128,22 @@ def test_dhc_reference_checks():
     assert np.abs(w).max() > 0.0  # buoyancy drives a flow
     _, divmax = oc.chkdiv()
     assert divmax < 1e-10
+
+
+def test_buoyancy_switch(tmp_path):
+    """The _BOUSSINESQ_BUOYANCY switch: off by default (the Boussinesq term
+    of test_boussinesq_term is absent), and refused without scalars, as
+    param.f90:166-170 stops such a build."""
+    import snac_amd
+    oc = scalar_box(tmp_path, gacc=(0.0, 0.0, -2.0), beta=0.5)
+    oc.set(0, "s1", np.full(oc.shape(0, "s1"), 3.0))
+    oc.rk_mom(1, 1e-2)
+    assert np.abs(oc.get(0, "w")).max() == 0.0
+    d, b = cases.write_case(str(tmp_path / "ns"), [cases.cavity_block((4, 4, 4))], 100.0)
+    oc0 = oracle_py.OracleCase(nmlparse.load_case(d, b, 0))
+    with pytest.raises(RuntimeError, match="requires nscal > 0"):
+        oc0.set_buoyancy(True)
+    ctx = snac_amd.Context(d, b, 0, device=-1)
+    with pytest.raises(snac_amd.SnacError, match="requires nscal > 0"):
+        ctx.set_buoyancy(True)
+    ctx.close()
