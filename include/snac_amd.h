@@ -218,6 +218,28 @@ int snac_correc(snac_ctx *ctx, double dtrk);
 int snac_updt_pressure(snac_ctx *ctx);
 /* chkdiv chkdiv.f90:8-44 (synchronises)                                     */
 int snac_chkdiv(snac_ctx *ctx, double *divtot, double *divmax);
+/* chkdiv of one block whose fields the caller holds in device memory,
+ * allocating nothing.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Same kernels and summation order
+ * as snac_chkdiv on a one-block context (chkdiv.f90:8-44): the partial sums
+ * and maxima go to a fixed device array of the library (2 x
+ * SNAC_NOALLOC_MAX_PARTS doubles, 4 MiB, one copy per GPU, held until the
+ * process ends) instead of context memory.  Limit: the block may need at
+ * most SNAC_NOALLOC_MAX_PARTS partials, ceil(nx/64) * ceil(ny/4) * nz;
+ * larger blocks are refused with an error before any launch.
+ * u, v, w: ghosted fields, element (i, j, k) (0-based, ghosts at 0 and n+1)
+ * at i + sx*(j + sy*k), sx >= nx+2, sy >= ny+2; dxf, dyf, dzf: the n+2
+ * ghosted face spacings of each axis; out: 2 doubles of device memory that
+ * receive divtot and divmax.  The launches are queued on `stream` (a
+ * hipStream_t of `device`, or NULL for the null stream) and the call
+ * returns without waiting.  Every call on one GPU uses that GPU's one
+ * scratch array: calls that may overlap on the same GPU (other streams,
+ * other host threads) race and must be ordered by the caller.             */
+#define SNAC_NOALLOC_MAX_PARTS 262144
+int snac_chkdiv_noalloc(int device, int nx, int ny, int nz, long sx, long sy, const double *u, const double *v,
+                        const double *w, const double *dxf, const double *dyf, const double *dzf, double vol_all,
+                        double *out, void *stream);
 /* chkdt chkdt.f90:8-56 (synchronises); without the diffusive limit while
  * implicit diffusion is on (chkdt.f90:51-55)                                */
 int snac_chkdt(snac_ctx *ctx, double *dtmax);

@@ -1310,6 +1310,11 @@ snac_ctx *on_device(snac_ctx *c) {
 
 using namespace snac;
 
+namespace snac {
+// the message snac_last_error returns on this host thread
+void set_last_error(const std::string &msg) { g_err = msg; }
+}  // namespace snac
+
 extern "C" {
 
 const char *snac_last_error(void) { return g_err.c_str(); }
