@@ -19,9 +19,11 @@ Forms, each timed on its own context in the same run (--forms):
           FP32 under a flexible CG whose residuals, operator, dot products
           and null-space removal stay FP64 (same --tol), and the momentum
           right-hand side in FP32 (RK3 accumulation and projection FP64);
-  learned as mixed with the trained network preconditioning the CG (pairs
-          recorded by the plain solver over --learn-steps steps, then
-          --learn-epochs epochs of training; eager launches);
+  learned the plain form's FP64 stencils with the trained network
+          preconditioning the flexible CG of the mixed form (FP32 V-cycle
+          fallback when its residual reduction stalls; pairs recorded by the
+          plain solver over --learn-steps steps, then --learn-epochs epochs
+          of training; eager launches);
   direct  FP64 with the pressure solved directly (r2r along z and x,
           tridiagonal lines along y) -- the same discrete systems, not the
           plain form's method;
@@ -82,6 +84,9 @@ def parse():
                     help="comma-separated forms (default: plain,mixed,direct; bfs: plain,mixed)")
     ap.add_argument("--tol", type=float, default=1e-10, help="MG-PCG relative residual per mode (&hypre hypre_tol)")
     ap.add_argument("--eager-steps", type=int, default=3, help="steps of the eager pass with per-phase HIP events")
+    ap.add_argument("--mg-line-variant", type=int, default=None,
+                    help="MG line relaxation kernel (0 rows per lane, 1 LDS-staged, 2 LDS-staged over modes)")
+    ap.add_argument("--mg-fast-rcp", type=int, default=None, help="1: MG eliminations with refined reciprocals")
     ap.add_argument("--learn-steps", type=int, default=20)
     ap.add_argument("--learn-epochs", type=int, default=4)
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline (0 = skip)")
@@ -179,6 +184,10 @@ def run_form(name, d, b, args, cells, dt_ref):
     solver, prec, impdiff, dtype = FORMS[name]
     ctx = snac_amd.Context(d, b, 3, device=0)
     try:
+        if args.mg_line_variant is not None:
+            ctx.set_mg_line_variant(args.mg_line_variant)
+        if args.mg_fast_rcp is not None:
+            ctx.set_mg_fast_rcp(args.mg_fast_rcp)
         ctx.set_solver(solver)
         if prec != "learned":
             ctx.set_precision(prec)

@@ -325,9 +325,19 @@ int snac_get_autotune(snac_ctx *ctx, int *choices, double *ms, int n);
  * fill composed at set-up into one launch that computes every ghost cell's
  * final value through the same chain of operations, where the fill allows
  * it); every variant computes the same result; SNAC_KV_RK_MOM32 (7..16):
- * the FP32 rk_mom of the mixed-precision modes.  No counterpart in the
- * reference.                                                                */
-enum { SNAC_KV_RK_MOM = 0, SNAC_KV_TRIDIAG = 1, SNAC_KV_GHOST = 2, SNAC_KV_RK_MOM32 = 3 };
+ * the FP32 rk_mom of the mixed-precision modes; SNAC_KV_MG_LINE (process-
+ * wide, every context): the MG-PCG line relaxation of the launched levels,
+ * 0 each lane reading its rows from memory, 1 the rows' right-hand sides
+ * and x staged through LDS with coalesced loads and stores, 2 (the
+ * default) as 1 with a workgroup's lines at one position over several
+ * modes sharing the coefficients it stages in LDS (FFT path; elsewhere 1);
+ * every variant computes the same values.  No counterpart in the
+ * reference.                                                               */
+/* SNAC_KV_MG_RCP (process-wide): 1 makes the mode-batched MG sweeps and the
+ * fused tail divide by refined hardware reciprocals (within a few ulp of
+ * the IEEE quotient) instead of IEEE divisions; 0 (the default) divides.   */
+enum { SNAC_KV_RK_MOM = 0, SNAC_KV_TRIDIAG = 1, SNAC_KV_GHOST = 2, SNAC_KV_RK_MOM32 = 3, SNAC_KV_MG_LINE = 4,
+       SNAC_KV_MG_RCP = 5 };
 int snac_set_kernel_variant(snac_ctx *ctx, int kernel, int variant);
 
 /* ---- building blocks, exposed for their own parity tests ----

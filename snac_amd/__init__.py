@@ -408,6 +408,18 @@ class Context:
         mixed-precision modes."""
         _check(lib.snac_set_kernel_variant(self.h, 3, int(v)), "set_kernel_variant")
 
+    def set_mg_line_variant(self, v):
+        """MG-PCG line relaxation kernel, process-wide: 0 rows read per lane,
+        1 right-hand sides staged through LDS, 2 (default) as 1 with the
+        lines of a workgroup over several modes sharing LDS-staged
+        coefficients (FFT path); all compute the same values."""
+        _check(lib.snac_set_kernel_variant(self.h, 4, int(v)), "set_kernel_variant")
+
+    def set_mg_fast_rcp(self, on):
+        """Process-wide: the mode-batched MG sweeps and the fused tail divide
+        by refined hardware reciprocals instead of IEEE divisions."""
+        _check(lib.snac_set_kernel_variant(self.h, 5, int(bool(on))), "set_kernel_variant")
+
     def set_tridiag_variant(self, v):
         """Overrides the autotuned tridiagonal kernel variant (0..3)."""
         _check(lib.snac_set_kernel_variant(self.h, 1, int(v)), "set_kernel_variant")

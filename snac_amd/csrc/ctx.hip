@@ -2064,6 +2064,14 @@ int snac_set_kernel_variant(snac_ctx *c, int kernel, int variant) {
     if (kernel == SNAC_KV_RK_MOM) {
       if (variant < 0 || variant > 16) throw std::runtime_error("rk_mom variant must be 0..16");
       c->rk_variant = variant;
+    } else if (kernel == SNAC_KV_MG_LINE) {
+      if (variant < 0 || variant > 2) throw std::runtime_error("MG line relaxation variant must be 0, 1 or 2");
+      if (c->stream) SNAC_HIP(hipStreamSynchronize(c->stream));
+      mg_set_line_variant(variant);
+    } else if (kernel == SNAC_KV_MG_RCP) {
+      if (variant < 0 || variant > 1) throw std::runtime_error("MG reciprocal variant must be 0 or 1");
+      if (c->stream) SNAC_HIP(hipStreamSynchronize(c->stream));
+      mg_set_fast_rcp(variant == 1);
     } else if (kernel == SNAC_KV_RK_MOM32) {
       if (variant < 7 || variant > 16) throw std::runtime_error("FP32 rk_mom variant must be 7..16");
       c->rk32_variant = variant;

@@ -89,10 +89,33 @@ __device__ __forceinline__ Row<T> fetch(const Row<T> &r, int src, bool valid) {
   return o;
 }
 
+// Reciprocals for the relaxation's eliminations without the IEEE division
+// sequence: the hardware estimate refined by Newton steps (two for FP64,
+// one for FP32), within a few ulp of the correctly rounded quotient.  The
+// relaxation only preconditions a CG that the FP64 residual judges.
+__device__ __forceinline__ double frcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+__device__ __forceinline__ float frcp(float x) {
+  float r = __builtin_amdgcn_rcpf(x);
+  float e = __builtin_fmaf(-x, r, 1.0f);
+  return __builtin_fmaf(r, e, r);
+}
+// a / b, or a times the refined reciprocal of b (FR)
+template <bool FR, typename T>
+__device__ __forceinline__ T dv(T a, T b) {
+  if constexpr (FR) return a * frcp(b);
+  else return a / b;
+}
+
 // PCR elimination of row r against its neighbours lo (index i-d) and hi (i+d)
-template <typename T>
+template <bool FR, typename T>
 __device__ __forceinline__ Row<T> pcr(const Row<T> &r, const Row<T> &lo, const Row<T> &hi) {
-  const T k1 = r.al / lo.be, k2 = r.ga / hi.be;
+  const T k1 = dv<FR>(r.al, lo.be), k2 = dv<FR>(r.ga, hi.be);
   Row<T> o;
   o.al = -k1 * lo.al;
   o.be = r.be - k1 * lo.ga - k2 * hi.al;
@@ -107,40 +130,19 @@ __device__ __forceinline__ Row<T> pcr(const Row<T> &r, const Row<T> &lo, const R
 // rows per lane that form a tridiagonal system in (x_first, x_last) of all
 // chunks; PCR over the lanes solves it in log2(2*nch) steps; the interior
 // rows follow from the two interface values.
-// relaxes line (m, a) with the calling wavefront (all 64 lanes must call)
 // LPL lanes per line (32 or 64): a wavefront relaxes 64 / LPL lines at once,
 // each in its own lane segment (the shuffles never cross segments); act =
-// false leaves the segment idle (no loads or stores).
-// xc (FFT path only): the neighbour lines get the prolongation of the
-// coarser level's xc (cna columns) added as they are read (nb_sum_pro)
-template <int K, int LPL, typename T>
-__device__ __forceinline__ void line_solve(const MGLevT<T> &L, T lmv, T *__restrict__ x, const T *__restrict__ b,
-                                           int m, int a, int zero_nb, int pin_s, bool pin, bool act,
-                                           const T *__restrict__ xc = nullptr, int cna = 0) {
-  const int wl = threadIdx.x & 63, lane = wl & (LPL - 1), seg0 = wl - lane;
-  const long e0 = vidx(L, m, a, 0), q0 = cidx(L, m, a, 0);
-  const int ns = L.ns;
-  const int cnt = act && lane < L.nch ? L.base + (lane < L.extra ? 1 : 0) : 0;
-  const int p0 = lane * L.base + min(lane, L.extra);
+// false leaves the segment idle.
 
-  T lo[K], dg[K], up[K], rh[K];
-#pragma unroll
-  for (int j = 0; j < K; j++) {
-    lo[j] = 0; dg[j] = 1; up[j] = 0; rh[j] = 0;
-    if (j < cnt) {
-      const int s = p0 + j;
-      const long e = e0 + s, q = q0 + s;
-      T d = L.dg[q] - lmv * L.w[q];
-      T l = s > 0 ? -L.cs[q - 1] : T(0);
-      T u = s < ns - 1 ? -L.cs[q] : T(0);
-      T r = b[e];
-      if (!zero_nb) r += xc ? nb_sum_pro(L, x, m, a, s, e, q, xc, cna) : nb_sum(L, x, m, a, e, q);
-      if (pin && s == pin_s) { d = 1; l = 0; u = 0; r = 0; }
-      lo[j] = l; dg[j] = d; up[j] = u; rh[j] = r;
-    }
-  }
+// the elimination of one line whose rows [p0, p0 + cnt) the calling lane
+// holds in lo / dg / up / rh; put(j, v) stores row p0 + j (cnt = 0: an idle
+// lane, which still takes part in the shuffles)
+template <int K, int LPL, typename T, bool FR, typename PUT>
+__device__ __forceinline__ void line_elim(int ns, int nch, const T (&lo)[K], const T (&dg)[K], const T (&up)[K],
+                                          const T (&rh)[K], int cnt, bool act, PUT &&put) {
+  const int wl = threadIdx.x & 63, lane = wl & (LPL - 1), seg0 = wl - lane;
   if (ns == 1) {  // a one-row line: no interface system
-    if (act && lane == 0) x[e0] = rh[0] / dg[0];
+    if (act && lane == 0) put(0, dv<FR>(rh[0], dg[0]));
     return;
   }
   // downward elimination: row j (1 <= j < cnt) becomes A x_p0 + x_j + C x_{j+1} = D
@@ -152,10 +154,10 @@ __device__ __forceinline__ void line_solve(const MGLevT<T> &L, T lmv, T *__restr
   for (int j = 1; j < K; j++) {
     if (j < cnt) {
       if (j == 1) {
-        const T ir = T(1) / dg[1];
+        const T ir = dv<FR>(T(1), dg[1]);
         A[1] = lo[1] * ir; C[1] = up[1] * ir; D[1] = rh[1] * ir;
       } else {
-        const T ir = T(1) / (dg[j] - lo[j] * C[j - 1]);
+        const T ir = dv<FR>(T(1), dg[j] - lo[j] * C[j - 1]);
         A[j] = -lo[j] * A[j - 1] * ir; C[j] = up[j] * ir; D[j] = (rh[j] - lo[j] * D[j - 1]) * ir;
       }
       if (j == cnt - 1) { aq = A[j]; cq = C[j]; dq = D[j]; }
@@ -185,27 +187,67 @@ __device__ __forceinline__ void line_solve(const MGLevT<T> &L, T lmv, T *__restr
   {
     const Row<T> r1m = fetch(r1, seg0 + (lane > 0 ? lane - 1 : 0), lane > 0);
     const Row<T> r0p = fetch(r0, seg0 + (lane < LPL - 1 ? lane + 1 : LPL - 1), lane < LPL - 1);
-    const Row<T> n0 = pcr(r0, r1m, r1);
-    const Row<T> n1 = pcr(r1, r0, r0p);
+    const Row<T> n0 = pcr<FR>(r0, r1m, r1);
+    const Row<T> n1 = pcr<FR>(r1, r0, r0p);
     r0 = n0;
     r1 = n1;
   }
-  const int nred = 2 * L.nch;
+  const int nred = 2 * nch;
   for (int h = 1; 2 * h < nred; h <<= 1) {  // d = 2h: lanes -+h, same slot
     const bool vm = lane >= h, vp = lane + h < LPL;
     const int sm = seg0 + (vm ? lane - h : lane), sp = seg0 + (vp ? lane + h : lane);
     const Row<T> a0 = fetch(r0, sm, vm), b0 = fetch(r0, sp, vp);
     const Row<T> a1 = fetch(r1, sm, vm), b1 = fetch(r1, sp, vp);
-    r0 = pcr(r0, a0, b0);
-    r1 = pcr(r1, a1, b1);
+    r0 = pcr<FR>(r0, a0, b0);
+    r1 = pcr<FR>(r1, a1, b1);
   }
   if (cnt == 0) return;
-  const T xp = r0.de / r0.be, xq = r1.de / r1.be;
-  x[e0 + p0] = xp;
-  x[e0 + p0 + cnt - 1] = xq;
+  const T xp = dv<FR>(r0.de, r0.be), xq = dv<FR>(r1.de, r1.be);
+  put(0, xp);
+  put(cnt - 1, xq);
 #pragma unroll
   for (int j = 1; j < K - 1; j++)
-    if (j <= cnt - 2) x[e0 + p0 + j] = D[j] - A[j] * xp - C[j] * xq;
+    if (j <= cnt - 2) put(j, D[j] - A[j] * xp - C[j] * xq);
+}
+
+// one row of line (m, a): diagonal, couplings below and above and the
+// right-hand side with the neighbour lines' contribution (zero_nb: read as
+// zero; xc: with the prolongation of xc added), or the identity row of the
+// pinned row of a singular group
+template <typename T>
+__device__ __forceinline__ void line_row(const MGLevT<T> &L, T lmv, const T *__restrict__ x, const T *__restrict__ b,
+                                         int m, int a, int s, long e, long q, int zero_nb, int pin_s, bool pin,
+                                         const T *__restrict__ xc, int cna, T &l, T &d, T &u, T &r) {
+  d = L.dg[q] - lmv * L.w[q];
+  l = s > 0 ? -L.cs[q - 1] : T(0);
+  u = s < L.ns - 1 ? -L.cs[q] : T(0);
+  r = b[e];
+  if (!zero_nb) r += xc ? nb_sum_pro(L, x, m, a, s, e, q, xc, cna) : nb_sum(L, x, m, a, e, q);
+  if (pin && s == pin_s) { d = 1; l = 0; u = 0; r = 0; }
+}
+
+// relaxes line (m, a) with the calling lane segment, rows read from and x
+// written to global memory directly (lane c's rows are contiguous)
+// xc (FFT path only): the neighbour lines get the prolongation of the
+// coarser level's xc (cna columns) added as they are read (nb_sum_pro)
+template <int K, int LPL, typename T, bool FR = false>
+__device__ __forceinline__ void line_solve(const MGLevT<T> &L, T lmv, T *__restrict__ x, const T *__restrict__ b,
+                                           int m, int a, int zero_nb, int pin_s, bool pin, bool act,
+                                           const T *__restrict__ xc = nullptr, int cna = 0) {
+  const int wl = threadIdx.x & 63, lane = wl & (LPL - 1);
+  const long e0 = vidx(L, m, a, 0), q0 = cidx(L, m, a, 0);
+  const int cnt = act && lane < L.nch ? L.base + (lane < L.extra ? 1 : 0) : 0;
+  const int p0 = lane * L.base + min(lane, L.extra);
+  T lo[K], dg[K], up[K], rh[K];
+#pragma unroll
+  for (int j = 0; j < K; j++) {
+    lo[j] = 0; dg[j] = 1; up[j] = 0; rh[j] = 0;
+    if (j < cnt) {
+      const int s = p0 + j;
+      line_row(L, lmv, x, b, m, a, s, e0 + s, q0 + s, zero_nb, pin_s, pin, xc, cna, lo[j], dg[j], up[j], rh[j]);
+    }
+  }
+  line_elim<K, LPL, T, FR>(L.ns, L.nch, lo, dg, up, rh, cnt, act, [&](int j, T v) { x[e0 + p0 + j] = v; });
 }
 
 template <int K, int LPL, typename T>
@@ -225,12 +267,210 @@ __global__ __launch_bounds__(256) void k_line(MGLevT<T> L, const T *__restrict__
   line_solve<K, LPL, T>(L, lmv, x, b, m, a, zero_nb, pin_s, pin, true, xc, cna);
 }
 
+// The same relaxation with the right-hand sides staged through LDS: the
+// workgroup's 256 / LPL lines first form b + (neighbour lines' couplings)
+// row by row with consecutive threads on consecutive rows, so the streams
+// of b, of the neighbour lines and of x are read and written whole rows at
+// a time; each lane then takes its own contiguous rows from LDS, eliminates
+// as line_solve does (the same arithmetic, so the same values), and x goes
+// back through LDS to coalesced stores.  The coefficients (mode-independent
+// on the FFT path, so cache-resident) are still read per lane.
+template <int K, int LPL, typename T>
+__global__ __launch_bounds__(256) void k_line_lds(MGLevT<T> L, const T *__restrict__ lam, T *__restrict__ x,
+                                                  const T *__restrict__ b, int col_a, int col_m, int nla, int nlm,
+                                                  int zero_nb, int pin_s, const int *__restrict__ singular,
+                                                  const int *__restrict__ done, const T *__restrict__ xc, int cna) {
+  constexpr int NLW = 256 / LPL;  // lines per workgroup
+  extern __shared__ unsigned char smem_[];
+  T *sh = reinterpret_cast<T *>(smem_);  // [NLW][ns]: right-hand sides, then x
+  const int ns = L.ns;
+  const long nl = (long)nla * nlm;
+  const long line0 = (long)blockIdx.x * NLW;
+  // phase 1: right-hand sides, one line after the other, rows across threads
+  for (int li = 0; li < NLW; li++) {
+    const long line = line0 + li;
+    if (line >= nl) break;
+    const int a = 2 * (int)(line % nla) + col_a;
+    const int lm_ = (int)(line / nla);
+    const int m = L.coupled ? 2 * lm_ + col_m : lm_;
+    if (done && done[group_of(L, m)]) continue;
+    const T lmv = lam ? lam[m] : T(0);
+    const bool pin = pin_s >= 0 && singular && singular[group_of(L, m)];
+    const long e0 = vidx(L, m, a, 0), q0 = cidx(L, m, a, 0);
+    for (int s = threadIdx.x; s < ns; s += 256) {
+      T l, d, u, r;
+      line_row(L, lmv, x, b, m, a, s, e0 + s, q0 + s, zero_nb, pin_s, pin, xc, cna, l, d, u, r);
+      sh[li * ns + s] = r;
+    }
+  }
+  __syncthreads();
+  // phase 2: each lane segment eliminates its line
+  {
+    const int li = threadIdx.x / LPL;
+    const long line = line0 + li;
+    bool act = line < nl;
+    int m = 0, a = 0;
+    if (act) {
+      a = 2 * (int)(line % nla) + col_a;
+      const int lm_ = (int)(line / nla);
+      m = L.coupled ? 2 * lm_ + col_m : lm_;
+      act = !(done && done[group_of(L, m)]);
+    }
+    // a whole segment shares act: idle segments skip the shuffles together
+    if (act) {
+      const T lmv = lam ? lam[m] : T(0);
+      const bool pin = pin_s >= 0 && singular && singular[group_of(L, m)];
+      const int lane = (threadIdx.x & 63) & (LPL - 1);
+      const int cnt = lane < L.nch ? L.base + (lane < L.extra ? 1 : 0) : 0;
+      const int p0 = lane * L.base + min(lane, L.extra);
+      const long q0 = cidx(L, m, a, 0);
+      T *row = sh + li * ns;
+      T lo[K], dg[K], up[K], rh[K];
+#pragma unroll
+      for (int j = 0; j < K; j++) {
+        lo[j] = 0; dg[j] = 1; up[j] = 0; rh[j] = 0;
+        if (j < cnt) {
+          const int s = p0 + j;
+          const long q = q0 + s;
+          dg[j] = L.dg[q] - lmv * L.w[q];
+          lo[j] = s > 0 ? -L.cs[q - 1] : T(0);
+          up[j] = s < ns - 1 ? -L.cs[q] : T(0);
+          rh[j] = row[s];
+          if (pin && s == pin_s) { dg[j] = 1; lo[j] = 0; up[j] = 0; rh[j] = 0; }
+        }
+      }
+      line_elim<K, LPL, T, false>(ns, L.nch, lo, dg, up, rh, cnt, true, [&](int j, T v) { row[p0 + j] = v; });
+    }
+  }
+  __syncthreads();
+  // phase 3: x of the lines, rows across threads
+  for (int li = 0; li < NLW; li++) {
+    const long line = line0 + li;
+    if (line >= nl) break;
+    const int a = 2 * (int)(line % nla) + col_a;
+    const int lm_ = (int)(line / nla);
+    const int m = L.coupled ? 2 * lm_ + col_m : lm_;
+    if (done && done[group_of(L, m)]) continue;
+    const long e0 = vidx(L, m, a, 0);
+    for (int s = threadIdx.x; s < ns; s += 256) x[e0 + s] = sh[li * ns + s];
+  }
+}
+
+// The LDS-staged relaxation with a workgroup's lines taken at one line
+// position a over 256 / LPL consecutive modes (FFT path, where the level's
+// coefficients do not depend on the mode): the line's coefficients are read
+// from memory once per workgroup into LDS and shared by its modes, and each
+// mode's right-hand side and x pass through LDS as in k_line_lds.  The
+// arithmetic is line_solve's, so the values are the same.
+template <int K, int LPL, typename T, bool FR>
+__global__ __launch_bounds__(256) void k_line_modes(MGLevT<T> L, const T *__restrict__ lam, T *__restrict__ x,
+                                                    const T *__restrict__ b, int col_a, int nla, int zero_nb,
+                                                    int pin_s, const int *__restrict__ singular,
+                                                    const int *__restrict__ done, const T *__restrict__ xc,
+                                                    int cna) {
+  constexpr int NLW = 256 / LPL;  // modes per workgroup
+  extern __shared__ unsigned char smem_[];
+  const int ns = L.ns;
+  T *cdg = reinterpret_cast<T *>(smem_);  // [ns] each: dg, w, cs, the couplings to a+1 and a-1
+  T *cw = cdg + ns, *ccs = cw + ns, *cap = ccs + ns, *cam = cap + ns;
+  T *sh = cam + ns;                        // [NLW][ns]: right-hand sides, then x
+  const int mb = (L.nm + NLW - 1) / NLW;   // mode batches per line position
+  const int ai = blockIdx.x / mb, m0 = (blockIdx.x % mb) * NLW;
+  const int a = 2 * ai + col_a;
+  const long sa = ns, wrap = (long)(L.na - 1) * sa;
+  const bool hp = a < L.na - 1 || L.per_a, hm = a > 0 || L.per_a;  // neighbour lines that exist
+  const long dp = a < L.na - 1 ? sa : -wrap, dm = a > 0 ? -sa : wrap;  // their offsets
+  const long q0 = cidx(L, 0, a, 0);
+  for (int s = threadIdx.x; s < ns; s += 256) {
+    const long q = q0 + s;
+    cdg[s] = L.dg[q];
+    cw[s] = L.w[q];
+    ccs[s] = L.cs[q];
+    cap[s] = hp ? L.ca[q] : T(0);
+    cam[s] = a > 0 ? L.ca[q - sa] : (L.per_a ? L.ca[q + wrap] : T(0));
+  }
+  __syncthreads();
+  // phase 1: right-hand sides, mode after mode, rows across threads
+  for (int li = 0; li < NLW; li++) {
+    const int m = m0 + li;
+    if (m >= L.nm) break;
+    if (done && done[m]) continue;
+    const bool pin = pin_s >= 0 && singular && singular[m];
+    const long e0 = vidx(L, m, a, 0);
+    for (int s = threadIdx.x; s < ns; s += 256) {
+      const long e = e0 + s;
+      T r = b[e];
+      if (!zero_nb) {
+        // nb_sum / nb_sum_pro with the coefficients from LDS, same order
+        T nb = 0;
+        if (xc) {
+          const int ra = cna != L.na;
+          if (hp) {
+            T xv = x[e + dp];
+            const int an = a < L.na - 1 ? a + 1 : 0;
+            if (L.act[q0 + s + dp]) xv += xc[((long)m * cna + (ra ? an >> 1 : an)) * sa + s];
+            nb += cap[s] * xv;
+          }
+          if (hm) {
+            T xv = x[e + dm];
+            const int an = a > 0 ? a - 1 : L.na - 1;
+            if (L.act[q0 + s + dm]) xv += xc[((long)m * cna + (ra ? an >> 1 : an)) * sa + s];
+            nb += cam[s] * xv;
+          }
+        } else {
+          if (hp) nb += cap[s] * x[e + dp];
+          if (hm) nb += cam[s] * x[e + dm];
+        }
+        r += nb;
+      }
+      if (pin && s == pin_s) r = 0;
+      sh[li * ns + s] = r;
+    }
+  }
+  __syncthreads();
+  // phase 2: each lane segment eliminates its mode's line
+  {
+    const int li = threadIdx.x / LPL, m = m0 + li;
+    if (m < L.nm && !(done && done[m])) {  // a whole segment shares the test
+      const T lmv = lam ? lam[m] : T(0);
+      const bool pin = pin_s >= 0 && singular && singular[m];
+      const int lane = (threadIdx.x & 63) & (LPL - 1);
+      const int cnt = lane < L.nch ? L.base + (lane < L.extra ? 1 : 0) : 0;
+      const int p0 = lane * L.base + min(lane, L.extra);
+      T *row = sh + li * ns;
+      T lo[K], dg[K], up[K], rh[K];
+#pragma unroll
+      for (int j = 0; j < K; j++) {
+        lo[j] = 0; dg[j] = 1; up[j] = 0; rh[j] = 0;
+        if (j < cnt) {
+          const int s = p0 + j;
+          dg[j] = cdg[s] - lmv * cw[s];
+          lo[j] = s > 0 ? -ccs[s - 1] : T(0);
+          up[j] = s < ns - 1 ? -ccs[s] : T(0);
+          rh[j] = row[s];
+          if (pin && s == pin_s) { dg[j] = 1; lo[j] = 0; up[j] = 0; rh[j] = 0; }
+        }
+      }
+      line_elim<K, LPL, T, FR>(ns, L.nch, lo, dg, up, rh, cnt, true, [&](int j, T v) { row[p0 + j] = v; });
+    }
+  }
+  __syncthreads();
+  // phase 3: x, rows across threads
+  for (int li = 0; li < NLW; li++) {
+    const int m = m0 + li;
+    if (m >= L.nm) break;
+    if (done && done[m]) continue;
+    const long e0 = vidx(L, m, a, 0);
+    for (int s = threadIdx.x; s < ns; s += 256) x[e0 + s] = sh[li * ns + s];
+  }
+}
+
 // The coarse end of the V-cycle (levels t.l0 .. t.nl-1, the last one a
 // single line per mode) for one mode per workgroup: zebra sweeps with one
 // wavefront per line, residual restriction and prolongation with the whole
 // workgroup, barriers between the steps.  One launch replaces six per level.
 // FFT path only (independent modes).
-template <int K, int LPL, typename T>
+template <int K, int LPL, typename T, bool FR>
 __global__ __launch_bounds__(1024) void k_tail(MGTailT<T> t, const T *__restrict__ lam, int pin_s,
                                                const int *__restrict__ singular, const int *__restrict__ done) {
   const int m = blockIdx.x;
@@ -247,7 +487,7 @@ __global__ __launch_bounds__(1024) void k_tail(MGTailT<T> t, const T *__restrict
       const int nlc = (L.na - c + 1) / 2;  // lines a = 2 j + c of this colour
       for (int j0 = SPW * wv; j0 < nlc; j0 += SPW * nw) {
         const int j = j0 + hs;
-        line_solve<K, LPL, T>(L, lmv, t.x[l], t.b[l], m, 2 * j + c, c == 0, -1, false, j < nlc);
+        line_solve<K, LPL, T, FR>(L, lmv, t.x[l], t.b[l], m, 2 * j + c, c == 0, -1, false, j < nlc);
       }
       __syncthreads();
     }
@@ -268,7 +508,7 @@ __global__ __launch_bounds__(1024) void k_tail(MGTailT<T> t, const T *__restrict
   // coarsest: one line, solved exactly
   {
     const int l = t.nl - 1;
-    if (wv == 0) line_solve<K, LPL, T>(t.lev[l], lmv, t.x[l], t.b[l], m, 0, 1, pin_s, pin, hs == 0);
+    if (wv == 0) line_solve<K, LPL, T, FR>(t.lev[l], lmv, t.x[l], t.b[l], m, 0, 1, pin_s, pin, hs == 0);
     __syncthreads();
   }
   // up: prolong, post-smooth (colour 1, then colour 0)
@@ -285,7 +525,7 @@ __global__ __launch_bounds__(1024) void k_tail(MGTailT<T> t, const T *__restrict
       const int nlc = (L.na - c + 1) / 2;
       for (int j0 = SPW * wv; j0 < nlc; j0 += SPW * nw) {
         const int j = j0 + hs;
-        line_solve<K, LPL, T>(L, lmv, t.x[l], t.b[l], m, 2 * j + c, 0, -1, false, j < nlc);
+        line_solve<K, LPL, T, FR>(L, lmv, t.x[l], t.b[l], m, 2 * j + c, 0, -1, false, j < nlc);
       }
       __syncthreads();
     }
@@ -318,6 +558,72 @@ __global__ __launch_bounds__(256) void k_restrict(MGLevT<T> F, MGLevT<T> Cl, con
     }
   }
   bc[idx] = sum;
+}
+
+// k_restrict on the FFT path with a workgroup's coarse cells at one coarse
+// column I over RM consecutive modes: the coefficients of the fine columns
+// 2I, 2I+1 (mode-independent there) are staged in LDS once and shared by the
+// modes; rows across threads.  Same arithmetic as k_restrict (mx_at's
+// order), so the same values.
+template <typename T, int RM>
+__global__ __launch_bounds__(256) void k_restrict_modes(MGLevT<T> F, MGLevT<T> Cl, const T *__restrict__ lamf,
+                                                        const T *__restrict__ x, const T *__restrict__ b,
+                                                        T *__restrict__ bc, const int *__restrict__ done) {
+  extern __shared__ unsigned char smem_[];
+  const int ns = F.ns;
+  // per fine column (2): dg, w, cs, the couplings to a+1 and a-1, and act
+  T *cdg = reinterpret_cast<T *>(smem_);
+  T *cw = cdg + 2 * ns, *ccs = cw + 2 * ns, *cap = ccs + 2 * ns, *cam = cap + 2 * ns;
+  unsigned char *cact = reinterpret_cast<unsigned char *>(cam + 2 * ns);
+  const int ra = F.na != Cl.na;
+  const int mb = (Cl.nm + RM - 1) / RM;
+  const int I = blockIdx.x / mb, m0 = (blockIdx.x % mb) * RM;
+  const int nda = ra ? 2 : 1;
+  const long sa = ns, wrap = (long)(F.na - 1) * sa;
+  for (int da = 0; da < nda; da++) {
+    const int a = ra ? 2 * I + da : I;
+    const bool ok = a < F.na;
+    const long q0 = ok ? cidx(F, 0, a, 0) : 0;
+    for (int s = threadIdx.x; s < ns; s += 256) {
+      const int o = da * ns + s;
+      if (!ok) { cact[o] = 0; continue; }
+      const long q = q0 + s;
+      cdg[o] = F.dg[q];
+      cw[o] = F.w[q];
+      ccs[o] = F.cs[q];
+      cap[o] = (a < F.na - 1 || F.per_a) ? F.ca[q] : T(0);
+      cam[o] = a > 0 ? F.ca[q - sa] : (F.per_a ? F.ca[q + wrap] : T(0));
+      cact[o] = F.act[q];
+    }
+  }
+  __syncthreads();
+  for (int mi = 0; mi < RM; mi++) {
+    const int M = m0 + mi;
+    if (M >= Cl.nm) break;
+    if (done && done[M]) continue;
+    const T lm = lamf ? lamf[M] : T(0);  // the FFT path: modes are not coarsened
+    for (int s = threadIdx.x; s < ns; s += 256) {
+      T sum = 0;
+      for (int da = 0; da < nda; da++) {
+        const int a = ra ? 2 * I + da : I;
+        if (a >= F.na) break;
+        const int o = da * ns + s;
+        if (!cact[o]) continue;
+        const long e = vidx(F, M, a, s);
+        // mx_at: (dg - lm w) x - cs(s-1) x(s-1) - cs(s) x(s+1) - nb_sum
+        T y = (cdg[o] - lm * cw[o]) * x[e];
+        if (s > 0) y -= ccs[o - 1] * x[e - 1];
+        if (s < ns - 1) y -= ccs[o] * x[e + 1];
+        T r = 0;
+        if (a < F.na - 1) r += cap[o] * x[e + sa];
+        else if (F.per_a) r += cap[o] * x[e - wrap];
+        if (a > 0) r += cam[o] * x[e - sa];
+        else if (F.per_a) r += cam[o] * x[e + wrap];
+        sum += b[e] - (y - r);
+      }
+      bc[vidx(Cl, M, I, s)] = sum;
+    }
+  }
 }
 
 template <typename T>
@@ -484,8 +790,7 @@ __device__ __forceinline__ double cg_beta(int g, const double *__restrict__ rzn,
 // The search direction and the operator in one pass: p = z + beta p_prev
 // (rounded to TP, the stored type), and the per-group partial sums of
 // p . (M p), M p computed from the new directions of the neighbours formed
-// on the fly (p_prev and p are different buffers).  Same element order per
-// thread as k_dot, so the sums match k_apply_dot's on the same p.
+// on the fly (p_prev and p are different buffers).
 template <typename TZ, typename TP>
 __global__ __launch_bounds__(256) void k_dir_apply_dot(MGLev L, const double *__restrict__ lam, long gsize,
                                                        const double *__restrict__ rzn, const double *__restrict__ rzo,
@@ -503,7 +808,11 @@ __global__ __launch_bounds__(256) void k_dir_apply_dot(MGLev L, const double *__
       return (double)(TP)((double)z[e] + be * (double)pp[e]);
     };
     const long base = (long)g * gsize;
-    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < gsize; i += (long)nchunk * 256) {
+    // a contiguous chunk of the group per workgroup: the neighbour lines the
+    // operator reads are mostly this workgroup's own (cache hits)
+    const long csz = ((gsize + nchunk - 1) / nchunk + 255) / 256 * 256;
+    const long iend = min(gsize, ((long)blockIdx.x + 1) * csz);
+    for (long i = (long)blockIdx.x * csz + threadIdx.x; i < iend; i += 256) {
       const long idx = base + i;
       const int s = (int)(idx % L.ns);
       const long t = idx / L.ns;
@@ -543,7 +852,11 @@ __global__ __launch_bounds__(256) void k_update_apply_dot(MGLev L, const double 
     const double al = den != 0.0 ? rz[g * 2] / den : 0.0;
     auto pv = [&](long e) -> double { return (double)p[e]; };
     const long base = (long)g * gsize;
-    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < gsize; i += (long)nchunk * 256) {
+    // a contiguous chunk of the group per workgroup: the neighbour lines the
+    // operator reads are mostly this workgroup's own (cache hits)
+    const long csz = ((gsize + nchunk - 1) / nchunk + 255) / 256 * 256;
+    const long iend = min(gsize, ((long)blockIdx.x + 1) * csz);
+    for (long i = (long)blockIdx.x * csz + threadIdx.x; i < iend; i += 256) {
       const long idx = base + i;
       const int s = (int)(idx % L.ns);
       const long t = idx / L.ns;
@@ -679,12 +992,68 @@ unsigned blocks_for(long n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
 
+namespace {
+// 0: rows read per lane; 1: right-hand sides staged through LDS; 2 (the
+// default): 1 with the lines of a workgroup at one position over several
+// modes (FFT path; elsewhere 1)
+int g_line_variant = 2;
+// the eliminations of the mode-batched sweeps and of the fused tail with
+// refined hardware reciprocals instead of IEEE divisions
+bool g_fast_rcp = false;
+}  // namespace
+
 template <int LPL, typename T>
 void mg_line_k(hipStream_t st, const MGLevT<T> &L, const T *lam, T *x, const T *b, int col_a, int col_m, int nla,
                int nlm, int zero_nb, int pin_s, const int *singular, const int *done, const T *xc, int cna) {
   const long lines = (long)nla * nlm;
   dim3 grid((unsigned)((lines * LPL + 255) / 256));
   const int rows = L.base + (L.extra ? 1 : 0);
+  const size_t shb = (size_t)(256 / LPL) * L.ns * sizeof(T);
+  const size_t shm = shb + 5 * (size_t)L.ns * sizeof(T);
+  if (g_line_variant == 2 && !L.coupled && L.qm == 0 && L.ns >= 2 && shm <= 64 * 1024) {
+    constexpr int NLW = 256 / LPL;
+    dim3 gm((unsigned)((long)nla * ((L.nm + NLW - 1) / NLW)));
+#define LAUNCH_MODES(KK)                                                                                            \
+  do {                                                                                                              \
+    if (g_fast_rcp)                                                                                                 \
+      hipLaunchKernelGGL((k_line_modes<KK, LPL, T, true>), gm, dim3(256), shm, st, L, lam, x, b, col_a, nla, zero_nb, \
+                         pin_s, singular, done, xc, cna);                                                           \
+    else                                                                                                            \
+      hipLaunchKernelGGL((k_line_modes<KK, LPL, T, false>), gm, dim3(256), shm, st, L, lam, x, b, col_a, nla,         \
+                         zero_nb, pin_s, singular, done, xc, cna);                                                  \
+  } while (0)
+    if (rows <= 3)
+      LAUNCH_MODES(3);
+    else if (rows <= 5)
+      LAUNCH_MODES(5);
+    else if (rows <= 9)
+      LAUNCH_MODES(9);
+    else if (rows <= 17)
+      LAUNCH_MODES(17);
+    else
+      throw std::runtime_error("multigrid line relaxation: lines longer than 1088 cells are not supported");
+#undef LAUNCH_MODES
+    SNAC_HIP(hipGetLastError());
+    return;
+  }
+  if (g_line_variant >= 1 && L.ns >= 2 && shb <= 64 * 1024) {
+    if (rows <= 3)
+      hipLaunchKernelGGL((k_line_lds<3, LPL, T>), grid, dim3(256), shb, st, L, lam, x, b, col_a, col_m, nla, nlm,
+                         zero_nb, pin_s, singular, done, xc, cna);
+    else if (rows <= 5)
+      hipLaunchKernelGGL((k_line_lds<5, LPL, T>), grid, dim3(256), shb, st, L, lam, x, b, col_a, col_m, nla, nlm,
+                         zero_nb, pin_s, singular, done, xc, cna);
+    else if (rows <= 9)
+      hipLaunchKernelGGL((k_line_lds<9, LPL, T>), grid, dim3(256), shb, st, L, lam, x, b, col_a, col_m, nla, nlm,
+                         zero_nb, pin_s, singular, done, xc, cna);
+    else if (rows <= 17)
+      hipLaunchKernelGGL((k_line_lds<17, LPL, T>), grid, dim3(256), shb, st, L, lam, x, b, col_a, col_m, nla, nlm,
+                         zero_nb, pin_s, singular, done, xc, cna);
+    else
+      throw std::runtime_error("multigrid line relaxation: lines longer than 1088 cells are not supported");
+    SNAC_HIP(hipGetLastError());
+    return;
+  }
   if (rows <= 3)
     hipLaunchKernelGGL((k_line<3, LPL, T>), grid, dim3(256), 0, st, L, lam, x, b, col_a, col_m, nla, nlm, zero_nb,
                        pin_s, singular, done, xc, cna);
@@ -702,6 +1071,11 @@ void mg_line_k(hipStream_t st, const MGLevT<T> &L, const T *lam, T *x, const T *
   SNAC_HIP(hipGetLastError());
 }
 
+void mg_set_line_variant(int v) { g_line_variant = v; }
+void mg_set_fast_rcp(bool on) { g_fast_rcp = on; }
+bool mg_fast_rcp() { return g_fast_rcp; }
+int mg_line_variant() { return g_line_variant; }
+
 template <typename T>
 void mg_line(hipStream_t st, const MGLevT<T> &L, const T *lam, T *x, const T *b, int col_a, int col_m, int zero_nb,
              int pin_s, const int *singular, const int *done, const T *xc, int cna) {
@@ -718,10 +1092,18 @@ void mg_tail_k(hipStream_t st, const MGTailT<T> &t, int nm, const T *lam, int pi
                const int *done) {
   const MGLevT<T> &L = t.lev[0];
   const int rows = L.base + (L.extra ? 1 : 0);
-  if (rows <= 3) hipLaunchKernelGGL((k_tail<3, LPL, T>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
-  else if (rows <= 5) hipLaunchKernelGGL((k_tail<5, LPL, T>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
-  else if (rows <= 9) hipLaunchKernelGGL((k_tail<9, LPL, T>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
-  else hipLaunchKernelGGL((k_tail<17, LPL, T>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);
+#define LAUNCH_TAIL(KK)                                                                                             \
+  do {                                                                                                              \
+    if (g_fast_rcp)                                                                                                 \
+      hipLaunchKernelGGL((k_tail<KK, LPL, T, true>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);    \
+    else                                                                                                            \
+      hipLaunchKernelGGL((k_tail<KK, LPL, T, false>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);   \
+  } while (0)
+  if (rows <= 3) LAUNCH_TAIL(3);
+  else if (rows <= 5) LAUNCH_TAIL(5);
+  else if (rows <= 9) LAUNCH_TAIL(9);
+  else LAUNCH_TAIL(17);
+#undef LAUNCH_TAIL
   SNAC_HIP(hipGetLastError());
 }
 
@@ -736,6 +1118,14 @@ template <typename T>
 void mg_restrict_residual(hipStream_t st, const MGLevT<T> &F, const MGLevT<T> &C, const T *lam_f, const T *x,
                           const T *b, T *bc, const int *done) {
   const long n = (long)C.nm * C.na * C.ns;
+  constexpr int RM = 8;
+  const size_t shm = (size_t)2 * F.ns * (5 * sizeof(T) + 1);
+  if (g_line_variant == 2 && !F.coupled && F.qm == 0 && F.nm == C.nm && shm <= 64 * 1024) {
+    dim3 g((unsigned)((long)C.na * ((C.nm + RM - 1) / RM)));
+    hipLaunchKernelGGL((k_restrict_modes<T, RM>), g, dim3(256), shm, st, F, C, lam_f, x, b, bc, done);
+    SNAC_HIP(hipGetLastError());
+    return;
+  }
   hipLaunchKernelGGL(k_restrict<T>, dim3(blocks_for(n)), dim3(256), 0, st, F, C, lam_f, x, b, bc, done);
   SNAC_HIP(hipGetLastError());
 }
@@ -795,6 +1185,9 @@ void mg_update_apply_dot(hipStream_t st, const MGLev &L, const double *lam, int 
 template void mg_dir_apply_dot<double, double>(hipStream_t, const MGLev &, const double *, int, long, const double *,
                                                const double *, const double *, const double *, const double *,
                                                double *, double *, int, double *, const int *);
+template void mg_dir_apply_dot<float, double>(hipStream_t, const MGLev &, const double *, int, long,
+                                              const double *, const double *, const double *, const float *,
+                                              const double *, double *, double *, int, double *, const int *);
 template void mg_dir_apply_dot<float, float>(hipStream_t, const MGLev &, const double *, int, long, const double *,
                                              const double *, const double *, const float *, const float *, float *,
                                              double *, int, double *, const int *);

@@ -82,6 +82,19 @@ void mg_tail(hipStream_t st, const MGTailT<T> &t, int nm, const T *lam, int pin_
 template <typename T>
 void mg_line(hipStream_t st, const MGLevT<T> &L, const T *lam, T *x, const T *b, int col_a, int col_m, int zero_nb,
              int pin_s, const int *singular, const int *done, const T *xc = nullptr, int cna = 0);
+// the launched levels' line relaxation kernel, process-wide: 0 reads each
+// lane's rows from memory directly, 1 stages the rows' right-hand sides and
+// the new x through LDS with coalesced loads and stores, 2 (the default) is
+// 1 with a workgroup's lines at one position over several modes sharing
+// the coefficients it stages in LDS (FFT path; elsewhere 1); all compute
+// the same values.  The fused tail always reads rows directly.
+void mg_set_line_variant(int v);
+int mg_line_variant();
+// process-wide: the mode-batched sweeps and the fused tail divide by
+// refined hardware reciprocals (the estimate plus Newton steps: two in
+// FP64, one in FP32) instead of IEEE divisions; off by default
+void mg_set_fast_rcp(bool on);
+bool mg_fast_rcp();
 // coarse b = restriction of the fine residual b - M x (sum over the active
 // fine cells of each coarse cell)
 template <typename T>

@@ -572,13 +572,22 @@ void mg_iteration(MGPlan &P, hipStream_t st, MGRun &R) {
   // Then x += alpha p, r -= alpha M p with r.r and, for the flexible form,
   // r_new . z_old in the second slot.  M p is applied twice rather than
   // stored: the second pass recomputes it from p.
+  // The V-cycle's output is well scaled and its search directions are kept
+  // in FP32; the network's need not be (an untrained one's directions grow
+  // past the FP32 range), so the learned mode keeps FP64 directions.
   const MGLev &L0 = P.h64.lev[0];
-  if (mixed) {
+  if (mixed && P.prec == 1) {
     mg_dir_apply_dot<float, float>(st, L0, P.h64.lam, P.ng, P.gsize, P.rz[R.cur], rzo, rzo ? P.rr : nullptr, P.zf,
                                    P.pf[R.pc], P.pf[1 - R.pc], P.part, P.nchunk, P.pq, P.done);
     R.pc = 1 - R.pc;
     mg_update_apply_dot<float>(st, L0, P.h64.lam, P.ng, P.gsize, P.rz[R.cur], P.pq, P.x, P.pf[R.pc], P.r, P.rf, P.zf,
                                P.part, P.nchunk, P.rr, P.done);
+  } else if (mixed) {
+    mg_dir_apply_dot<float, double>(st, L0, P.h64.lam, P.ng, P.gsize, P.rz[R.cur], rzo, rzo ? P.rr : nullptr, P.zf,
+                                    P.pd[R.pc], P.pd[1 - R.pc], P.part, P.nchunk, P.pq, P.done);
+    R.pc = 1 - R.pc;
+    mg_update_apply_dot<double>(st, L0, P.h64.lam, P.ng, P.gsize, P.rz[R.cur], P.pq, P.x, P.pd[R.pc], P.r, P.rf,
+                                P.zf, P.part, P.nchunk, P.rr, P.done);
   } else {
     mg_dir_apply_dot<double, double>(st, L0, P.h64.lam, P.ng, P.gsize, P.rz[R.cur], rzo, nullptr, P.z, P.pd[R.pc],
                                      P.pd[1 - R.pc], P.part, P.nchunk, P.pq, P.done);
@@ -684,8 +693,13 @@ void mg_stats(MGPlan &P, hipStream_t st, int &iters, double &max_relres) {
   SNAC_HIP(hipStreamSynchronize(st));
   iters = P.last_iters;
   max_relres = 0.0;
-  for (int g = 0; g < P.ng; g++)
+  for (int g = 0; g < P.ng; g++) {
+    if (std::isnan(rr[2 * g]) || std::isnan(bb[2 * g])) {  // a broken solve never reads as converged
+      max_relres = std::nan("");
+      break;
+    }
     if (bb[2 * g] > 0.0) max_relres = std::max(max_relres, std::sqrt(rr[2 * g] / bb[2 * g]));
+  }
 }
 
 }  // namespace snac

@@ -114,7 +114,18 @@ def test_mg_10_steps(tmp_path, geom, prec):
     ctx, oc, singular = pair(tmp_path, geom, prec)
     tol = 1e-10 if prec == "fp64" else 1e-6
     run_both(ctx, oc, 10, dt_rel=1e-12 if prec == "fp64" else 1e-6)
-    assert_fields(ctx, oc, FIELDS3, tol, interior=True)
+    if prec == "fp64":
+        assert_fields(ctx, oc, FIELDS3, tol, interior=True)
+    else:
+        # the FP32 rounding is absolute, about 1e-7 of |u|: each component's
+        # error against the size of the whole velocity field (w of these
+        # cases is a small cross-flow)
+        inner = (slice(1, -1),) * 3
+        for b in range(ctx.nblocks):
+            scale = np.sqrt(sum(np.linalg.norm(oc.get(b, f)[inner]) ** 2 for f in FIELDS3))
+            for f in FIELDS3:
+                err = np.linalg.norm(ctx.get(b, f)[inner] - oc.get(b, f)[inner])
+                assert err <= tol * scale, (b, f, err / scale)
     assert_fields(ctx, oc, ("p",), tol, interior=True, modmean=("p",) if singular else ())
     _, divmax = ctx.chkdiv()
     _, odivmax = oc.chkdiv()
@@ -187,3 +198,28 @@ def test_graph_steps_meet_tolerance(tmp_path, captured):
                 np.testing.assert_array_equal(graph.get(q, f), eager.get(q, f), err_msg=f"step {rep} block {q} {f}")
     eager.close()
     graph.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("geom", ["bfs", "cavity3d"])
+def test_line_variants_agree(tmp_path, geom, prec):
+    """The line relaxation kernels -- each lane reading its rows from memory,
+    the rows staged through LDS, and the lines of a workgroup over several
+    modes sharing staged coefficients -- compute the same values: the
+    pressure solves agree bit for bit (FFT path, and the coupled 3D path,
+    where the third falls back to the second)."""
+    import snac_amd
+    out = []
+    for v in (0, 1, 2):
+        ctx, oc, _ = pair(tmp_path / f"v{v}", geom, prec)
+        ctx.set_mg_line_variant(v)
+        seed_fields(ctx, oc)
+        ctx.bounduvw(False)
+        ctx.fillps(1e-3)
+        ctx.solve_pressure()
+        out.append([ctx.get(b, "pp") for b in range(ctx.nblocks)])
+        ctx.set_mg_line_variant(2)
+        ctx.close()
+    for v in (1, 2):
+        for a, b in zip(out[0], out[v]):
+            np.testing.assert_array_equal(a, b)
