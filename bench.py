@@ -85,7 +85,8 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-10, help="MG-PCG relative residual per mode (&hypre hypre_tol)")
     ap.add_argument("--eager-steps", type=int, default=3, help="steps of the eager pass with per-phase HIP events")
     ap.add_argument("--mg-line-variant", type=int, default=None,
-                    help="MG line relaxation kernel (0 rows per lane, 1 LDS-staged, 2 LDS-staged over modes)")
+                    help="MG line relaxation kernel (0 rows per lane, 1 LDS-staged, 2 LDS-staged over modes, "
+                         "3 / 4 as 2 with 16 / 8 lanes per line, -1 each level's fastest: the default)")
     ap.add_argument("--mg-fast-rcp", type=int, default=None, help="1: MG eliminations with refined reciprocals")
     ap.add_argument("--learn-steps", type=int, default=20)
     ap.add_argument("--learn-epochs", type=int, default=4)

@@ -328,11 +328,15 @@ int snac_get_autotune(snac_ctx *ctx, int *choices, double *ms, int n);
  * the FP32 rk_mom of the mixed-precision modes; SNAC_KV_MG_LINE (process-
  * wide, every context): the MG-PCG line relaxation of the launched levels,
  * 0 each lane reading its rows from memory, 1 the rows' right-hand sides
- * and x staged through LDS with coalesced loads and stores, 2 (the
- * default) as 1 with a workgroup's lines at one position over several
- * modes sharing the coefficients it stages in LDS (FFT path; elsewhere 1);
- * every variant computes the same values.  No counterpart in the
- * reference.                                                               */
+ * and x staged through LDS with coalesced loads and stores, 2 as 1 with a
+ * workgroup's lines at one position over several modes sharing the
+ * coefficients it stages in LDS (FFT path; elsewhere 1), 3 and 4 as 2 with
+ * 16 and 8 lanes per line instead of 32 (longer row chunks per lane, a
+ * shorter interface system; lines of 80..272 and 72..264 rows, elsewhere
+ * the next lower variant), a negative value (the default) each level's
+ * fastest, timed when the MG-PCG plan is set up and kept per level shape
+ * for the process; 0..2 compute the same values, 3 and 4 the same systems
+ * in another elimination order.  No counterpart in the reference.          */
 /* SNAC_KV_MG_RCP (process-wide): 1 makes the mode-batched MG sweeps and the
  * fused tail divide by refined hardware reciprocals (within a few ulp of
  * the IEEE quotient) instead of IEEE divisions; 0 (the default) divides.   */

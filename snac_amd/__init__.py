@@ -410,9 +410,12 @@ class Context:
 
     def set_mg_line_variant(self, v):
         """MG-PCG line relaxation kernel, process-wide: 0 rows read per lane,
-        1 right-hand sides staged through LDS, 2 (default) as 1 with the
-        lines of a workgroup over several modes sharing LDS-staged
-        coefficients (FFT path); all compute the same values."""
+        1 right-hand sides staged through LDS, 2 as 1 with the lines of a
+        workgroup over several modes sharing LDS-staged coefficients (FFT
+        path), 3 and 4 as 2 with 16 and 8 lanes per line (longer row chunks;
+        the same systems in another elimination order); a negative value
+        (the default) restores each level's fastest, timed at set-up.  0..2
+        compute the same values."""
         _check(lib.snac_set_kernel_variant(self.h, 4, int(v)), "set_kernel_variant")
 
     def set_mg_fast_rcp(self, on):

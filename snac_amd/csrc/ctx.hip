@@ -474,6 +474,7 @@ void plan_mg(snac_ctx *c) {
   MGPlan &P = c->mg;
   mg_plan(P, c->cs, c->device);
   P.prec = c->prec;
+  if (P.ok && c->device >= 0) mg_tune_lines(P, c->stream);  // each level's fastest line kernel
   if (!P.ok || c->device < 0 || P.fa < 0) return;
   P.fwd = r2r_plan(P.kind_fwd, P.G[P.fa]);
   P.bwd = r2r_plan(P.kind_bwd, P.G[P.fa]);
@@ -2065,9 +2066,9 @@ int snac_set_kernel_variant(snac_ctx *c, int kernel, int variant) {
       if (variant < 0 || variant > 16) throw std::runtime_error("rk_mom variant must be 0..16");
       c->rk_variant = variant;
     } else if (kernel == SNAC_KV_MG_LINE) {
-      if (variant < 0 || variant > 2) throw std::runtime_error("MG line relaxation variant must be 0, 1 or 2");
+      if (variant > 4) throw std::runtime_error("MG line relaxation variant must be 0..4 or negative");
       if (c->stream) SNAC_HIP(hipStreamSynchronize(c->stream));
-      mg_set_line_variant(variant);
+      mg_set_line_variant(variant < 0 ? -1 : variant);
     } else if (kernel == SNAC_KV_MG_RCP) {
       if (variant < 0 || variant > 1) throw std::runtime_error("MG reciprocal variant must be 0 or 1");
       if (c->stream) SNAC_HIP(hipStreamSynchronize(c->stream));

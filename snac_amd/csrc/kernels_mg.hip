@@ -5,6 +5,8 @@
 // operator application, per-group dot products with wavefront reductions and
 // the CG vector updates.
 #include "mg.hpp"
+#include <algorithm>
+#include <type_traits>
 
 namespace snac {
 namespace {
@@ -68,8 +70,33 @@ __device__ __forceinline__ T mx_at(const MGLevT<T> &L, T lm, const T *__restrict
 template <typename T>
 __device__ __forceinline__ int group_of(const MGLevT<T> &L, int m) { return L.coupled ? 0 : m; }
 
+// (m, a, s) of the element at flat index idx of level L
+template <typename T>
+__device__ __forceinline__ void pos_of(const MGLevT<T> &L, long idx, int &m, int &a, int &s) {
+  s = (int)(idx % L.ns);
+  const long t = idx / L.ns;
+  a = (int)(t % L.na);
+  m = (int)(t / L.na);
+}
+// advances (m, a, s) by da * ns + ds elements (0 <= ds < ns): a grid-stride
+// loop steps its position without dividing the flat index every element
+template <typename T>
+__device__ __forceinline__ void pos_step(const MGLevT<T> &L, int da, int ds, int &m, int &a, int &s) {
+  s += ds;
+  a += da;
+  if (s >= L.ns) { s -= L.ns; a++; }
+  while (a >= L.na) { a -= L.na; m++; }
+}
+
 template <typename T>
 __device__ __forceinline__ T shfl(T v, int src) { return __shfl(v, src, 64); }
+
+// LDS position of row s of a staged line: one pad word after every 32 rows,
+// so that the lanes of a segment, which read rows base apart (base a
+// multiple of 8 on power-of-two lines), fall on different banks
+__host__ __device__ __forceinline__ int lpad(int s) { return s + (s >> 5); }
+// LDS words per staged line of ns rows
+__host__ __device__ __forceinline__ int lpad_len(int ns) { return lpad(ns - 1) + 1; }
 
 // One reduced row  al*u[i-d] + be*u[i] + ga*u[i+d] = de  of the interface
 // system; rows outside the line are the identity.
@@ -134,15 +161,19 @@ __device__ __forceinline__ Row<T> pcr(const Row<T> &r, const Row<T> &lo, const R
 // each in its own lane segment (the shuffles never cross segments); act =
 // false leaves the segment idle.
 
-// the elimination of one line whose rows [p0, p0 + cnt) the calling lane
-// holds in lo / dg / up / rh; put(j, v) stores row p0 + j (cnt = 0: an idle
-// lane, which still takes part in the shuffles)
-template <int K, int LPL, typename T, bool FR, typename PUT>
-__device__ __forceinline__ void line_elim(int ns, int nch, const T (&lo)[K], const T (&dg)[K], const T (&up)[K],
-                                          const T (&rh)[K], int cnt, bool act, PUT &&put) {
+// the elimination of one line whose rows [p0, p0 + cnt) belong to the
+// calling lane: row(j, l, d, u, r) gives row p0 + j (j < cnt; each row is
+// asked for once, and before any put), put(j, v) stores its solution
+// (cnt = 0: an idle lane, which still takes part in the shuffles)
+template <int K, int LPL, typename T, bool FR, typename ROW, typename PUT>
+__device__ __forceinline__ void line_elim(int ns, int nch, int cnt, bool act, ROW &&row, PUT &&put) {
   const int wl = threadIdx.x & 63, lane = wl & (LPL - 1), seg0 = wl - lane;
   if (ns == 1) {  // a one-row line: no interface system
-    if (act && lane == 0) put(0, dv<FR>(rh[0], dg[0]));
+    if (act && lane == 0) {
+      T l, d, u, r;
+      row(0, l, d, u, r);
+      put(0, dv<FR>(r, d));
+    }
     return;
   }
   // downward elimination: row j (1 <= j < cnt) becomes A x_p0 + x_j + C x_{j+1} = D
@@ -153,12 +184,14 @@ __device__ __forceinline__ void line_elim(int ns, int nch, const T (&lo)[K], con
 #pragma unroll
   for (int j = 1; j < K; j++) {
     if (j < cnt) {
+      T lo, dg, up, rh;
+      row(j, lo, dg, up, rh);
       if (j == 1) {
-        const T ir = dv<FR>(T(1), dg[1]);
-        A[1] = lo[1] * ir; C[1] = up[1] * ir; D[1] = rh[1] * ir;
+        const T ir = dv<FR>(T(1), dg);
+        A[1] = lo * ir; C[1] = up * ir; D[1] = rh * ir;
       } else {
-        const T ir = dv<FR>(T(1), dg[j] - lo[j] * C[j - 1]);
-        A[j] = -lo[j] * A[j - 1] * ir; C[j] = up[j] * ir; D[j] = (rh[j] - lo[j] * D[j - 1]) * ir;
+        const T ir = dv<FR>(T(1), dg - lo * C[j - 1]);
+        A[j] = -lo * A[j - 1] * ir; C[j] = up * ir; D[j] = (rh - lo * D[j - 1]) * ir;
       }
       if (j == cnt - 1) { aq = A[j]; cq = C[j]; dq = D[j]; }
     }
@@ -175,9 +208,11 @@ __device__ __forceinline__ void line_elim(int ns, int nch, const T (&lo)[K], con
   }
   Row<T> r0, r1;
   if (cnt >= 2) {
-    r0.al = lo[0];
-    if (cnt == 2) { r0.be = dg[0]; r0.ga = up[0]; r0.de = rh[0]; }
-    else { r0.be = dg[0] - up[0] * an; r0.ga = -up[0] * cn; r0.de = rh[0] - up[0] * dn; }
+    T lo, dg, up, rh;
+    row(0, lo, dg, up, rh);
+    r0.al = lo;
+    if (cnt == 2) { r0.be = dg; r0.ga = up; r0.de = rh; }
+    else { r0.be = dg - up * an; r0.ga = -up * cn; r0.de = rh - up * dn; }
     r1.al = aq; r1.be = 1; r1.ga = cq; r1.de = dq;
   } else {
     r0 = Row<T>{0, 1, 0, 0};
@@ -247,7 +282,9 @@ __device__ __forceinline__ void line_solve(const MGLevT<T> &L, T lmv, T *__restr
       line_row(L, lmv, x, b, m, a, s, e0 + s, q0 + s, zero_nb, pin_s, pin, xc, cna, lo[j], dg[j], up[j], rh[j]);
     }
   }
-  line_elim<K, LPL, T, FR>(L.ns, L.nch, lo, dg, up, rh, cnt, act, [&](int j, T v) { x[e0 + p0 + j] = v; });
+  line_elim<K, LPL, T, FR>(
+      L.ns, L.nch, cnt, act, [&](int j, T &l, T &d, T &u, T &r) { l = lo[j]; d = dg[j]; u = up[j]; r = rh[j]; },
+      [&](int j, T v) { x[e0 + p0 + j] = v; });
 }
 
 template <int K, int LPL, typename T>
@@ -282,8 +319,8 @@ __global__ __launch_bounds__(256) void k_line_lds(MGLevT<T> L, const T *__restri
                                                   const int *__restrict__ done, const T *__restrict__ xc, int cna) {
   constexpr int NLW = 256 / LPL;  // lines per workgroup
   extern __shared__ unsigned char smem_[];
-  T *sh = reinterpret_cast<T *>(smem_);  // [NLW][ns]: right-hand sides, then x
-  const int ns = L.ns;
+  T *sh = reinterpret_cast<T *>(smem_);  // [NLW][nsp]: right-hand sides, then x (rows at lpad)
+  const int ns = L.ns, nsp = lpad_len(ns);
   const long nl = (long)nla * nlm;
   const long line0 = (long)blockIdx.x * NLW;
   // phase 1: right-hand sides, one line after the other, rows across threads
@@ -300,7 +337,7 @@ __global__ __launch_bounds__(256) void k_line_lds(MGLevT<T> L, const T *__restri
     for (int s = threadIdx.x; s < ns; s += 256) {
       T l, d, u, r;
       line_row(L, lmv, x, b, m, a, s, e0 + s, q0 + s, zero_nb, pin_s, pin, xc, cna, l, d, u, r);
-      sh[li * ns + s] = r;
+      sh[li * nsp + lpad(s)] = r;
     }
   }
   __syncthreads();
@@ -324,7 +361,7 @@ __global__ __launch_bounds__(256) void k_line_lds(MGLevT<T> L, const T *__restri
       const int cnt = lane < L.nch ? L.base + (lane < L.extra ? 1 : 0) : 0;
       const int p0 = lane * L.base + min(lane, L.extra);
       const long q0 = cidx(L, m, a, 0);
-      T *row = sh + li * ns;
+      T *row = sh + li * nsp;
       T lo[K], dg[K], up[K], rh[K];
 #pragma unroll
       for (int j = 0; j < K; j++) {
@@ -335,11 +372,13 @@ __global__ __launch_bounds__(256) void k_line_lds(MGLevT<T> L, const T *__restri
           dg[j] = L.dg[q] - lmv * L.w[q];
           lo[j] = s > 0 ? -L.cs[q - 1] : T(0);
           up[j] = s < ns - 1 ? -L.cs[q] : T(0);
-          rh[j] = row[s];
+          rh[j] = row[lpad(s)];
           if (pin && s == pin_s) { dg[j] = 1; lo[j] = 0; up[j] = 0; rh[j] = 0; }
         }
       }
-      line_elim<K, LPL, T, false>(ns, L.nch, lo, dg, up, rh, cnt, true, [&](int j, T v) { row[p0 + j] = v; });
+      line_elim<K, LPL, T, false>(
+          ns, L.nch, cnt, true, [&](int j, T &l, T &d, T &u, T &r) { l = lo[j]; d = dg[j]; u = up[j]; r = rh[j]; },
+          [&](int j, T v) { row[lpad(p0 + j)] = v; });
     }
   }
   __syncthreads();
@@ -352,7 +391,7 @@ __global__ __launch_bounds__(256) void k_line_lds(MGLevT<T> L, const T *__restri
     const int m = L.coupled ? 2 * lm_ + col_m : lm_;
     if (done && done[group_of(L, m)]) continue;
     const long e0 = vidx(L, m, a, 0);
-    for (int s = threadIdx.x; s < ns; s += 256) x[e0 + s] = sh[li * ns + s];
+    for (int s = threadIdx.x; s < ns; s += 256) x[e0 + s] = sh[li * nsp + lpad(s)];
   }
 }
 
@@ -360,20 +399,28 @@ __global__ __launch_bounds__(256) void k_line_lds(MGLevT<T> L, const T *__restri
 // position a over 256 / LPL consecutive modes (FFT path, where the level's
 // coefficients do not depend on the mode): the line's coefficients are read
 // from memory once per workgroup into LDS and shared by its modes, and each
-// mode's right-hand side and x pass through LDS as in k_line_lds.  The
-// arithmetic is line_solve's, so the values are the same.
+// mode's right-hand side and x pass through LDS as in k_line_lds.  Each
+// line goes to LPL lanes, which split it into min(LPL, ns / 2) chunks as the
+// level's own chunking does for its lane count, and each lane reads its
+// rows from LDS as the elimination asks for them (only the elimination's
+// three coefficients per row stay in registers).  With the level's lane
+// count the arithmetic is line_solve's, so the values are the same; with
+// fewer lanes per line (longer chunks, a shorter interface system) the
+// same line is solved with another elimination order.
 template <int K, int LPL, typename T, bool FR>
 __global__ __launch_bounds__(256) void k_line_modes(MGLevT<T> L, const T *__restrict__ lam, T *__restrict__ x,
                                                     const T *__restrict__ b, int col_a, int nla, int zero_nb,
                                                     int pin_s, const int *__restrict__ singular,
                                                     const int *__restrict__ done, const T *__restrict__ xc,
                                                     int cna) {
-  constexpr int NLW = 256 / LPL;  // modes per workgroup
+  constexpr int NLW = 256 / LPL;            // modes per workgroup
+  constexpr int MB = NLW < 8 ? NLW : 8;     // modes whose loads are issued together
   extern __shared__ unsigned char smem_[];
-  const int ns = L.ns;
-  T *cdg = reinterpret_cast<T *>(smem_);  // [ns] each: dg, w, cs, the couplings to a+1 and a-1
-  T *cw = cdg + ns, *ccs = cw + ns, *cap = ccs + ns, *cam = cap + ns;
-  T *sh = cam + ns;                        // [NLW][ns]: right-hand sides, then x
+  const int ns = L.ns, nsp = lpad_len(ns);
+  // [nsp] each, rows at lpad: dg, w, cs, the couplings to a+1 and a-1
+  T *cdg = reinterpret_cast<T *>(smem_);
+  T *cw = cdg + nsp, *ccs = cw + nsp, *cap = ccs + nsp, *cam = cap + nsp;
+  T *sh = cam + nsp;  // [NLW][nsp]: right-hand sides, then x
   const int mb = (L.nm + NLW - 1) / NLW;   // mode batches per line position
   const int ai = blockIdx.x / mb, m0 = (blockIdx.x % mb) * NLW;
   const int a = 2 * ai + col_a;
@@ -383,48 +430,58 @@ __global__ __launch_bounds__(256) void k_line_modes(MGLevT<T> L, const T *__rest
   const long q0 = cidx(L, 0, a, 0);
   for (int s = threadIdx.x; s < ns; s += 256) {
     const long q = q0 + s;
-    cdg[s] = L.dg[q];
-    cw[s] = L.w[q];
-    ccs[s] = L.cs[q];
-    cap[s] = hp ? L.ca[q] : T(0);
-    cam[s] = a > 0 ? L.ca[q - sa] : (L.per_a ? L.ca[q + wrap] : T(0));
+    const int o = lpad(s);
+    cdg[o] = L.dg[q];
+    cw[o] = L.w[q];
+    ccs[o] = L.cs[q];
+    cap[o] = hp ? L.ca[q] : T(0);
+    cam[o] = a > 0 ? L.ca[q - sa] : (L.per_a ? L.ca[q + wrap] : T(0));
   }
   __syncthreads();
-  // phase 1: right-hand sides, mode after mode, rows across threads
-  for (int li = 0; li < NLW; li++) {
-    const int m = m0 + li;
-    if (m >= L.nm) break;
-    if (done && done[m]) continue;
-    const bool pin = pin_s >= 0 && singular && singular[m];
-    const long e0 = vidx(L, m, a, 0);
+  // phase 1: right-hand sides of every mode of the workgroup, rows across
+  // threads; MB modes' loads are issued together (no control flow between
+  // them) so that their latencies overlap
+  const int ra = cna != L.na;
+  const int anp = a < L.na - 1 ? a + 1 : 0, anm = a > 0 ? a - 1 : L.na - 1;  // neighbour columns
+  const int cnp = ra ? anp >> 1 : anp, cnm = ra ? anm >> 1 : anm;           // their coarse columns
+  for (int g0 = 0; g0 < NLW; g0 += MB) {
+    bool live[MB];
+    int pinrow[MB];
+#pragma unroll
+    for (int k = 0; k < MB; k++) {
+      const int m = m0 + g0 + k;
+      live[k] = m < L.nm && !(done && done[m]);
+      pinrow[k] = (live[k] && pin_s >= 0 && singular && singular[m]) ? pin_s : -1;
+    }
     for (int s = threadIdx.x; s < ns; s += 256) {
-      const long e = e0 + s;
-      T r = b[e];
-      if (!zero_nb) {
-        // nb_sum / nb_sum_pro with the coefficients from LDS, same order
-        T nb = 0;
-        if (xc) {
-          const int ra = cna != L.na;
-          if (hp) {
-            T xv = x[e + dp];
-            const int an = a < L.na - 1 ? a + 1 : 0;
-            if (L.act[q0 + s + dp]) xv += xc[((long)m * cna + (ra ? an >> 1 : an)) * sa + s];
-            nb += cap[s] * xv;
-          }
-          if (hm) {
-            T xv = x[e + dm];
-            const int an = a > 0 ? a - 1 : L.na - 1;
-            if (L.act[q0 + s + dm]) xv += xc[((long)m * cna + (ra ? an >> 1 : an)) * sa + s];
-            nb += cam[s] * xv;
-          }
-        } else {
-          if (hp) nb += cap[s] * x[e + dp];
-          if (hm) nb += cam[s] * x[e + dm];
-        }
-        r += nb;
+      const bool actp = xc && hp && L.act[q0 + s + dp], actm = xc && hm && L.act[q0 + s + dm];
+      T bv[MB], xp[MB], xm[MB], cp[MB], cm[MB];
+#pragma unroll
+      for (int k = 0; k < MB; k++) {
+        const int m = m0 + g0 + k;
+        const long e = vidx(L, m, a, s);
+        const long ec = (long)m * cna * sa + s;
+        bv[k] = live[k] ? b[e] : T(0);
+        xp[k] = live[k] && !zero_nb && hp ? x[e + dp] : T(0);
+        xm[k] = live[k] && !zero_nb && hm ? x[e + dm] : T(0);
+        cp[k] = live[k] && !zero_nb && actp ? xc[ec + (long)cnp * sa] : T(0);
+        cm[k] = live[k] && !zero_nb && actm ? xc[ec + (long)cnm * sa] : T(0);
       }
-      if (pin && s == pin_s) r = 0;
-      sh[li * ns + s] = r;
+      const int o = lpad(s);
+#pragma unroll
+      for (int k = 0; k < MB; k++) {
+        if (!live[k]) continue;
+        T r = bv[k];
+        if (!zero_nb) {
+          // nb_sum / nb_sum_pro with the coefficients from LDS, same order
+          T nb = 0;
+          if (hp) nb += cap[o] * (actp ? xp[k] + cp[k] : xp[k]);
+          if (hm) nb += cam[o] * (actm ? xm[k] + cm[k] : xm[k]);
+          r += nb;
+        }
+        if (s == pinrow[k]) r = 0;
+        sh[(g0 + k) * nsp + o] = r;
+      }
     }
   }
   __syncthreads();
@@ -435,23 +492,22 @@ __global__ __launch_bounds__(256) void k_line_modes(MGLevT<T> L, const T *__rest
       const T lmv = lam ? lam[m] : T(0);
       const bool pin = pin_s >= 0 && singular && singular[m];
       const int lane = (threadIdx.x & 63) & (LPL - 1);
-      const int cnt = lane < L.nch ? L.base + (lane < L.extra ? 1 : 0) : 0;
-      const int p0 = lane * L.base + min(lane, L.extra);
-      T *row = sh + li * ns;
-      T lo[K], dg[K], up[K], rh[K];
-#pragma unroll
-      for (int j = 0; j < K; j++) {
-        lo[j] = 0; dg[j] = 1; up[j] = 0; rh[j] = 0;
-        if (j < cnt) {
-          const int s = p0 + j;
-          dg[j] = cdg[s] - lmv * cw[s];
-          lo[j] = s > 0 ? -ccs[s - 1] : T(0);
-          up[j] = s < ns - 1 ? -ccs[s] : T(0);
-          rh[j] = row[s];
-          if (pin && s == pin_s) { dg[j] = 1; lo[j] = 0; up[j] = 0; rh[j] = 0; }
-        }
-      }
-      line_elim<K, LPL, T, FR>(ns, L.nch, lo, dg, up, rh, cnt, true, [&](int j, T v) { row[p0 + j] = v; });
+      const int nch = ns >= 2 ? min(LPL, ns / 2) : 1;
+      const int base = ns >= 2 ? ns / nch : 1, extra = ns >= 2 ? ns % nch : 0;
+      const int cnt = lane < nch ? base + (lane < extra ? 1 : 0) : 0;
+      const int p0 = lane * base + min(lane, extra);
+      T *row = sh + li * nsp;
+      line_elim<K, LPL, T, FR>(
+          ns, nch, cnt, true,
+          [&](int j, T &l, T &d, T &u, T &r) {
+            const int s = p0 + j, o = lpad(s);
+            d = cdg[o] - lmv * cw[o];
+            l = s > 0 ? -ccs[lpad(s - 1)] : T(0);
+            u = s < ns - 1 ? -ccs[o] : T(0);
+            r = row[o];
+            if (pin && s == pin_s) { d = 1; l = 0; u = 0; r = 0; }
+          },
+          [&](int j, T v) { row[lpad(p0 + j)] = v; });
     }
   }
   __syncthreads();
@@ -461,15 +517,16 @@ __global__ __launch_bounds__(256) void k_line_modes(MGLevT<T> L, const T *__rest
     if (m >= L.nm) break;
     if (done && done[m]) continue;
     const long e0 = vidx(L, m, a, 0);
-    for (int s = threadIdx.x; s < ns; s += 256) x[e0 + s] = sh[li * ns + s];
+    for (int s = threadIdx.x; s < ns; s += 256) x[e0 + s] = sh[li * nsp + lpad(s)];
   }
 }
 
 // The coarse end of the V-cycle (levels t.l0 .. t.nl-1, the last one a
 // single line per mode) for one mode per workgroup: zebra sweeps with one
-// wavefront per line, residual restriction and prolongation with the whole
-// workgroup, barriers between the steps.  One launch replaces six per level.
-// FFT path only (independent modes).
+// lane segment per line, residual restriction with the whole workgroup,
+// the prolongation folded into the first post-smoothing sweep, barriers
+// between the steps.  One launch replaces five per level.  FFT path only
+// (independent modes).
 template <int K, int LPL, typename T, bool FR>
 __global__ __launch_bounds__(1024) void k_tail(MGTailT<T> t, const T *__restrict__ lam, int pin_s,
                                                const int *__restrict__ singular, const int *__restrict__ done) {
@@ -511,21 +568,18 @@ __global__ __launch_bounds__(1024) void k_tail(MGTailT<T> t, const T *__restrict
     if (wv == 0) line_solve<K, LPL, T, FR>(t.lev[l], lmv, t.x[l], t.b[l], m, 0, 1, pin_s, pin, hs == 0);
     __syncthreads();
   }
-  // up: prolong, post-smooth (colour 1, then colour 0)
+  // up: post-smooth (colour 1, then colour 0); the colour-1 sweep reads its
+  // colour-0 neighbours with the coarse correction added (nb_sum_pro), and
+  // colour 0 is then relaxed from scratch, so the prolongation is never
+  // stored (the values of a separate prolongation pass, bit for bit)
   for (int l = t.nl - 2; l >= 0; l--) {
     const MGLevT<T> &L = t.lev[l], &Cl = t.lev[l + 1];
-    const int ra = L.na != Cl.na;
-    for (int e = threadIdx.x; e < L.na * L.ns; e += blockDim.x) {
-      const int s = e % L.ns, a = e / L.ns;
-      if (!L.act[cidx(L, m, a, s)]) continue;
-      t.x[l][vidx(L, m, a, s)] += t.x[l + 1][vidx(Cl, m, ra ? a >> 1 : a, s)];
-    }
-    __syncthreads();
     for (int c = 1; c >= 0; c--) {
       const int nlc = (L.na - c + 1) / 2;
       for (int j0 = SPW * wv; j0 < nlc; j0 += SPW * nw) {
         const int j = j0 + hs;
-        line_solve<K, LPL, T, FR>(L, lmv, t.x[l], t.b[l], m, 2 * j + c, 0, -1, false, j < nlc);
+        line_solve<K, LPL, T, FR>(L, lmv, t.x[l], t.b[l], m, 2 * j + c, 0, -1, false, j < nlc,
+                                  c == 1 ? t.x[l + 1] : nullptr, Cl.na);
       }
       __syncthreads();
     }
@@ -702,14 +756,16 @@ __global__ __launch_bounds__(256) void k_apply_dot(MGLev L, const double *__rest
   double s1 = 0.0;
   if (!(done && done[g])) {
     const long base = (long)g * gsize;
-    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < gsize; i += (long)nchunk * 256) {
+    const long i0 = (long)blockIdx.x * 256 + threadIdx.x, step = (long)nchunk * 256;
+    int m, a, s;
+    pos_of(L, base + i0, m, a, s);
+    const int da = (int)(step / L.ns), ds = (int)(step % L.ns);
+    for (long i = i0; i < gsize; i += step) {
       const long idx = base + i;
-      const int s = (int)(idx % L.ns);
-      const long t = idx / L.ns;
-      const int a = (int)(t % L.na), m = (int)(t / L.na);
       const double y = mx_at(L, lam ? lam[m] : 0.0, p, m, a, s);
       q[idx] = y;
       s1 += p[idx] * y;
+      pos_step(L, da, ds, m, a, s);
     }
   }
   s1 = wave_sum(s1);
@@ -735,7 +791,11 @@ __global__ __launch_bounds__(256) void k_cg_update_dot(long gsize, const double 
   double s1 = 0.0, s2 = 0.0;
   if (!done[g]) {
     const double den = pq[g * 2];
-    const double al = den != 0.0 ? rz[g * 2] / den : 0.0;
+    // p.Mp > 0 for an SPD operator and p != 0; anything else (p = 0, or a
+    // preconditioner output that is not usable) skips the step instead of
+    // spreading an overflow or a NaN
+    double al = den > 0.0 ? rz[g * 2] / den : 0.0;
+    if (!(__builtin_fabs(al) < 1e300)) al = 0.0;
     const long base = (long)g * gsize;
     for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < gsize; i += (long)nchunk * 256) {
       const long idx = base + i;
@@ -812,15 +872,17 @@ __global__ __launch_bounds__(256) void k_dir_apply_dot(MGLev L, const double *__
     // operator reads are mostly this workgroup's own (cache hits)
     const long csz = ((gsize + nchunk - 1) / nchunk + 255) / 256 * 256;
     const long iend = min(gsize, ((long)blockIdx.x + 1) * csz);
-    for (long i = (long)blockIdx.x * csz + threadIdx.x; i < iend; i += 256) {
+    const long i0 = (long)blockIdx.x * csz + threadIdx.x;
+    int m, a, s;
+    pos_of(L, base + i0, m, a, s);
+    const int da = 256 / L.ns, ds = 256 % L.ns;
+    for (long i = i0; i < iend; i += 256) {
       const long idx = base + i;
-      const int s = (int)(idx % L.ns);
-      const long t = idx / L.ns;
-      const int a = (int)(t % L.na), m = (int)(t / L.na);
       const double pc = pn(idx);
       p[idx] = (TP)pc;
       const double y = mx_f(L, lam ? lam[m] : 0.0, pn, m, a, s, idx, cidx(L, m, a, s));
       s1 += pc * y;
+      pos_step(L, da, ds, m, a, s);
     }
   }
   s1 = wave_sum(s1);
@@ -849,19 +911,25 @@ __global__ __launch_bounds__(256) void k_update_apply_dot(MGLev L, const double 
   double s1 = 0.0, s2 = 0.0;
   if (!done[g]) {
     const double den = pq[g * 2];
-    const double al = den != 0.0 ? rz[g * 2] / den : 0.0;
+    // p.Mp > 0 for an SPD operator and p != 0; anything else (p = 0, or a
+    // preconditioner output that is not usable) skips the step instead of
+    // spreading an overflow or a NaN
+    double al = den > 0.0 ? rz[g * 2] / den : 0.0;
+    if (!(__builtin_fabs(al) < 1e300)) al = 0.0;
     auto pv = [&](long e) -> double { return (double)p[e]; };
     const long base = (long)g * gsize;
     // a contiguous chunk of the group per workgroup: the neighbour lines the
     // operator reads are mostly this workgroup's own (cache hits)
     const long csz = ((gsize + nchunk - 1) / nchunk + 255) / 256 * 256;
     const long iend = min(gsize, ((long)blockIdx.x + 1) * csz);
-    for (long i = (long)blockIdx.x * csz + threadIdx.x; i < iend; i += 256) {
+    const long i0 = (long)blockIdx.x * csz + threadIdx.x;
+    int m, a, s;
+    pos_of(L, base + i0, m, a, s);
+    const int da = 256 / L.ns, ds = 256 % L.ns;
+    for (long i = i0; i < iend; i += 256) {
       const long idx = base + i;
-      const int s = (int)(idx % L.ns);
-      const long t = idx / L.ns;
-      const int a = (int)(t % L.na), m = (int)(t / L.na);
       const double q = mx_f(L, lam ? lam[m] : 0.0, pv, m, a, s, idx, cidx(L, m, a, s));
+      pos_step(L, da, ds, m, a, s);
       x[idx] += al * (double)p[idx];
       const double rn = r[idx] - al * q;
       r[idx] = rn;
@@ -996,11 +1064,59 @@ namespace {
 // 0: rows read per lane; 1: right-hand sides staged through LDS; 2 (the
 // default): 1 with the lines of a workgroup at one position over several
 // modes (FFT path; elsewhere 1)
-int g_line_variant = 2;
+int g_line_variant = -1;  // forced variant, or -1: each level's own
 // the eliminations of the mode-batched sweeps and of the fused tail with
 // refined hardware reciprocals instead of IEEE divisions
 bool g_fast_rcp = false;
 }  // namespace
+
+// the mode-batched sweep (k_line_modes) with LP lanes per line: false when
+// the level does not allow it (coupled modes, mode-dependent coefficients,
+// more LDS than a workgroup gets, or a chunk length with no instance: LP of
+// 32 or 64 takes rows of up to 17 per lane, 16 lanes 5..17, 8 lanes 9..33)
+template <int LP, typename T>
+int line_modes_rows(const MGLevT<T> &L) {
+  if (L.coupled || L.qm != 0 || L.ns < 2) return 0;
+  const size_t shm = (size_t)(256 / LP + 5) * lpad_len(L.ns) * sizeof(T);
+  if (shm > 64 * 1024) return 0;
+  const int nch = std::min(LP, L.ns / 2);
+  const int rows = L.ns / nch + (L.ns % nch ? 1 : 0);
+  const int lo = LP >= 32 ? 1 : (LP == 16 ? 5 : 9), hi = LP >= 32 ? 17 : (LP == 16 ? 17 : 33);
+  return rows >= lo && rows <= hi ? rows : 0;
+}
+
+template <int LP, typename T>
+bool launch_line_modes(hipStream_t st, const MGLevT<T> &L, const T *lam, T *x, const T *b, int col_a, int nla,
+                       int zero_nb, int pin_s, const int *singular, const int *done, const T *xc, int cna) {
+  const int rows = line_modes_rows<LP>(L);
+  if (!rows) return false;
+  constexpr int NLW = 256 / LP;
+  const size_t shm = (size_t)(NLW + 5) * lpad_len(L.ns) * sizeof(T);
+  dim3 gm((unsigned)((long)nla * ((L.nm + NLW - 1) / NLW)));
+  auto go = [&](auto kk) {
+    constexpr int KK = decltype(kk)::value;
+    if (g_fast_rcp)
+      hipLaunchKernelGGL((k_line_modes<KK, LP, T, true>), gm, dim3(256), shm, st, L, lam, x, b, col_a, nla, zero_nb,
+                         pin_s, singular, done, xc, cna);
+    else
+      hipLaunchKernelGGL((k_line_modes<KK, LP, T, false>), gm, dim3(256), shm, st, L, lam, x, b, col_a, nla, zero_nb,
+                         pin_s, singular, done, xc, cna);
+  };
+  if constexpr (LP >= 32) {
+    if (rows <= 3) go(std::integral_constant<int, 3>{});
+    else if (rows <= 5) go(std::integral_constant<int, 5>{});
+    else if (rows <= 9) go(std::integral_constant<int, 9>{});
+    else go(std::integral_constant<int, 17>{});
+  } else if constexpr (LP == 16) {
+    if (rows <= 9) go(std::integral_constant<int, 9>{});
+    else go(std::integral_constant<int, 17>{});
+  } else {
+    if (rows <= 17) go(std::integral_constant<int, 17>{});
+    else go(std::integral_constant<int, 33>{});
+  }
+  SNAC_HIP(hipGetLastError());
+  return true;
+}
 
 template <int LPL, typename T>
 void mg_line_k(hipStream_t st, const MGLevT<T> &L, const T *lam, T *x, const T *b, int col_a, int col_m, int nla,
@@ -1008,35 +1124,16 @@ void mg_line_k(hipStream_t st, const MGLevT<T> &L, const T *lam, T *x, const T *
   const long lines = (long)nla * nlm;
   dim3 grid((unsigned)((lines * LPL + 255) / 256));
   const int rows = L.base + (L.extra ? 1 : 0);
-  const size_t shb = (size_t)(256 / LPL) * L.ns * sizeof(T);
-  const size_t shm = shb + 5 * (size_t)L.ns * sizeof(T);
-  if (g_line_variant == 2 && !L.coupled && L.qm == 0 && L.ns >= 2 && shm <= 64 * 1024) {
-    constexpr int NLW = 256 / LPL;
-    dim3 gm((unsigned)((long)nla * ((L.nm + NLW - 1) / NLW)));
-#define LAUNCH_MODES(KK)                                                                                            \
-  do {                                                                                                              \
-    if (g_fast_rcp)                                                                                                 \
-      hipLaunchKernelGGL((k_line_modes<KK, LPL, T, true>), gm, dim3(256), shm, st, L, lam, x, b, col_a, nla, zero_nb, \
-                         pin_s, singular, done, xc, cna);                                                           \
-    else                                                                                                            \
-      hipLaunchKernelGGL((k_line_modes<KK, LPL, T, false>), gm, dim3(256), shm, st, L, lam, x, b, col_a, nla,         \
-                         zero_nb, pin_s, singular, done, xc, cna);                                                  \
-  } while (0)
-    if (rows <= 3)
-      LAUNCH_MODES(3);
-    else if (rows <= 5)
-      LAUNCH_MODES(5);
-    else if (rows <= 9)
-      LAUNCH_MODES(9);
-    else if (rows <= 17)
-      LAUNCH_MODES(17);
-    else
-      throw std::runtime_error("multigrid line relaxation: lines longer than 1088 cells are not supported");
-#undef LAUNCH_MODES
-    SNAC_HIP(hipGetLastError());
+  const size_t nsp = (size_t)lpad_len(L.ns);
+  const size_t shb = (size_t)(256 / LPL) * nsp * sizeof(T);
+  const int lv = g_line_variant >= 0 ? g_line_variant : (L.lvar >= 0 ? L.lvar : 2);
+  // a variant the level does not allow falls back to the next lower one
+  if (lv >= 4 && launch_line_modes<8>(st, L, lam, x, b, col_a, nla, zero_nb, pin_s, singular, done, xc, cna)) return;
+  if (lv >= 3 && launch_line_modes<16>(st, L, lam, x, b, col_a, nla, zero_nb, pin_s, singular, done, xc, cna))
     return;
-  }
-  if (g_line_variant >= 1 && L.ns >= 2 && shb <= 64 * 1024) {
+  if (lv >= 2 && launch_line_modes<LPL>(st, L, lam, x, b, col_a, nla, zero_nb, pin_s, singular, done, xc, cna))
+    return;
+  if (lv >= 1 && L.ns >= 2 && shb <= 64 * 1024) {
     if (rows <= 3)
       hipLaunchKernelGGL((k_line_lds<3, LPL, T>), grid, dim3(256), shb, st, L, lam, x, b, col_a, col_m, nla, nlm,
                          zero_nb, pin_s, singular, done, xc, cna);
@@ -1072,6 +1169,20 @@ void mg_line_k(hipStream_t st, const MGLevT<T> &L, const T *lam, T *x, const T *
 }
 
 void mg_set_line_variant(int v) { g_line_variant = v; }
+
+template <typename T>
+bool mg_line_variant_runs(const MGLevT<T> &L, int v) {
+  switch (v) {
+    case 0: return true;
+    case 1: return L.ns >= 2 && (size_t)(256 / L.lpl) * lpad_len(L.ns) * sizeof(T) <= 64 * 1024;
+    case 2: return L.lpl == 64 ? line_modes_rows<64>(L) > 0 : line_modes_rows<32>(L) > 0;
+    case 3: return line_modes_rows<16>(L) > 0;
+    case 4: return line_modes_rows<8>(L) > 0;
+    default: return false;
+  }
+}
+template bool mg_line_variant_runs<double>(const MGLevT<double> &, int);
+template bool mg_line_variant_runs<float>(const MGLevT<float> &, int);
 void mg_set_fast_rcp(bool on) { g_fast_rcp = on; }
 bool mg_fast_rcp() { return g_fast_rcp; }
 int mg_line_variant() { return g_line_variant; }
@@ -1120,7 +1231,8 @@ void mg_restrict_residual(hipStream_t st, const MGLevT<T> &F, const MGLevT<T> &C
   const long n = (long)C.nm * C.na * C.ns;
   constexpr int RM = 8;
   const size_t shm = (size_t)2 * F.ns * (5 * sizeof(T) + 1);
-  if (g_line_variant == 2 && !F.coupled && F.qm == 0 && F.nm == C.nm && shm <= 64 * 1024) {
+  const int lv = g_line_variant >= 0 ? g_line_variant : (F.lvar >= 0 ? F.lvar : 2);
+  if (lv >= 2 && !F.coupled && F.qm == 0 && F.nm == C.nm && shm <= 64 * 1024) {
     dim3 g((unsigned)((long)C.na * ((C.nm + RM - 1) / RM)));
     hipLaunchKernelGGL((k_restrict_modes<T, RM>), g, dim3(256), shm, st, F, C, lam_f, x, b, bc, done);
     SNAC_HIP(hipGetLastError());

@@ -396,7 +396,7 @@ __global__ void k_net_pair_out(int ng, int na, int ns, int nH, int nW, const dou
 __global__ void k_net_stall(int ng, const double *__restrict__ rr, double *__restrict__ prev,
                             const int *__restrict__ done, double ratio2, int *__restrict__ stall) {
   for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < ng; g += gridDim.x * blockDim.x) {
-    if (!done[g] && rr[2 * g] > ratio2 * prev[2 * g]) stall[0] = 1;
+    if (!done[g] && !(rr[2 * g] <= ratio2 * prev[2 * g])) stall[0] = 1;  // a NaN residual stalls too
     prev[2 * g] = rr[2 * g];
   }
 }

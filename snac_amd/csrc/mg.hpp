@@ -44,6 +44,7 @@ struct MGLevT {
   const unsigned char *act;
   int nch, base, extra; // a line of ns rows over nch lanes: base (+1 for the first `extra`) rows each
   int lpl;              // lanes per line segment (32 or 64; nch <= lpl)
+  int lvar;             // line relaxation kernel of this level (0..4, chosen by timing at set-up)
 };
 using MGLev = MGLevT<double>;
 
@@ -82,14 +83,27 @@ void mg_tail(hipStream_t st, const MGTailT<T> &t, int nm, const T *lam, int pin_
 template <typename T>
 void mg_line(hipStream_t st, const MGLevT<T> &L, const T *lam, T *x, const T *b, int col_a, int col_m, int zero_nb,
              int pin_s, const int *singular, const int *done, const T *xc = nullptr, int cna = 0);
-// the launched levels' line relaxation kernel, process-wide: 0 reads each
-// lane's rows from memory directly, 1 stages the rows' right-hand sides and
-// the new x through LDS with coalesced loads and stores, 2 (the default) is
-// 1 with a workgroup's lines at one position over several modes sharing
-// the coefficients it stages in LDS (FFT path; elsewhere 1); all compute
-// the same values.  The fused tail always reads rows directly.
+// the launched levels' line relaxation kernel: 0 reads each lane's rows
+// from memory directly, 1 stages the rows' right-hand sides and the new x
+// through LDS with coalesced loads and stores, 2 is 1 with a workgroup's
+// lines at one position over several modes sharing the coefficients it
+// stages in LDS (FFT path; elsewhere 1); 0..2 compute the same values.
+// 3 and 4 are 2 with 16 and 8 lanes per line instead of the level's 32 or
+// 64 (longer row chunks per lane, a shorter interface system: the same
+// systems solved in another elimination order, so values differ in the
+// last bits); where a level does not allow a variant, the next lower one
+// runs.  The level's own choice (lvar, timed at set-up) applies unless a
+// variant is forced process-wide (mg_set_line_variant 0..4; -1 returns to
+// the levels' choices).  The fused tail always reads rows directly.
 void mg_set_line_variant(int v);
 int mg_line_variant();
+// whether variant v runs as itself on level L (rather than falling back)
+template <typename T>
+bool mg_line_variant_runs(const MGLevT<T> &L, int v);
+// times the line relaxation kernels on every launched level of a plan's
+// hierarchies (FP64 and FP32) and keeps the fastest in each level's lvar
+struct MGPlan;
+void mg_tune_lines(MGPlan &P, hipStream_t st);
 // process-wide: the mode-batched sweeps and the fused tail divide by
 // refined hardware reciprocals (the estimate plus Newton steps: two in
 // FP64, one in FP32) instead of IEEE divisions; off by default

@@ -21,6 +21,10 @@
 
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
 
 #include "mg.hpp"
 #include "net.hpp"
@@ -399,6 +403,7 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
     L.base = ns >= 2 ? ns / nch : 1;
     L.extra = ns >= 2 ? ns % nch : 0;
     L.lpl = lpl;
+    L.lvar = -1;
     P.h64.lev.push_back(L);
     MGLevT<float> Lf;
     Lf.nm = L.nm; Lf.na = L.na; Lf.ns = L.ns;
@@ -409,7 +414,7 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
     Lf.ca = dev_upload(P, f32(h.ca));
     Lf.cm = coupled ? dev_upload(P, f32(h.cm)) : nullptr;
     Lf.act = L.act;
-    Lf.nch = L.nch; Lf.base = L.base; Lf.extra = L.extra; Lf.lpl = L.lpl;
+    Lf.nch = L.nch; Lf.base = L.base; Lf.extra = L.extra; Lf.lpl = L.lpl; Lf.lvar = -1;
     P.h32.lev.push_back(Lf);
     const size_t n = (size_t)h.nm * h.na * h.ns;
     if (l == 0) {
@@ -700,6 +705,85 @@ void mg_stats(MGPlan &P, hipStream_t st, int &iters, double &max_relres) {
     }
     if (bb[2 * g] > 0.0) max_relres = std::max(max_relres, std::sqrt(rr[2 * g] / bb[2 * g]));
   }
+}
+
+}  // namespace snac
+
+namespace snac {
+namespace {
+
+// Each level's line kernel per level shape, kept for the life of the
+// process: a later plan of the same shapes reuses the choices without
+// timing, so that same-shape contexts run the same kernels and agree bit
+// for bit (variants 3 and 4 round differently from 0..2).
+std::mutex g_line_mu;
+std::map<std::string, int> g_line_choice;
+
+template <typename T>
+std::string line_key(const MGLevT<T> &L, int dev) {
+  std::ostringstream k;
+  k << "dev " << dev << " T" << sizeof(T) << " nm " << L.nm << " na " << L.na << " ns " << L.ns << " qm " << L.qm
+    << " coupled " << L.coupled << " per " << L.per_a << " lpl " << L.lpl;
+  return k.str();
+}
+
+template <typename T>
+void tune_hier(MGPlan &P, MGHier<T> &H, hipStream_t st) {
+  const int nl = (int)H.lev.size();
+  const int last = H.tail0 >= 0 ? H.tail0 : nl - 1;  // launched levels that relax lines and restrict
+  int dev = 0;
+  SNAC_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_line_mu);
+  hipEvent_t e0, e1;
+  SNAC_HIP(hipEventCreate(&e0));
+  SNAC_HIP(hipEventCreate(&e1));
+  for (int l = 0; l < last; l++) {
+    MGLevT<T> &L = H.lev[l];
+    if (L.na < 2) continue;
+    const std::string key = line_key(L, dev);
+    auto it = g_line_choice.find(key);
+    if (it != g_line_choice.end()) {
+      L.lvar = it->second;
+      continue;
+    }
+    float best = 1e30f;
+    int bv = 2;
+    for (int v = 0; v <= 4; v++) {
+      if (!mg_line_variant_runs(L, v)) continue;
+      L.lvar = v;
+      auto run = [&] {
+        mg_line(st, L, H.lam, H.xl[l], H.bl[l], 0, 0, 0, -1, nullptr, P.done);
+        mg_restrict_residual(st, L, H.lev[l + 1], H.lam, H.xl[l], H.bl[l], H.bl[l + 1], P.done);
+      };
+      run();  // warm-up
+      SNAC_HIP(hipEventRecord(e0, st));
+      for (int r = 0; r < 3; r++) run();
+      SNAC_HIP(hipEventRecord(e1, st));
+      SNAC_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      SNAC_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) { best = ms; bv = v; }
+    }
+    L.lvar = bv;
+    g_line_choice.emplace(key, bv);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // the trials relaxed zero right-hand sides: the vectors are zero again
+  for (int l = 0; l < nl; l++) {
+    const long n = (long)H.lev[l].nm * H.lev[l].na * H.lev[l].ns;
+    mg_zero(st, n, H.xl[l]);
+    mg_zero(st, n, H.bl[l]);
+  }
+  SNAC_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace
+
+void mg_tune_lines(MGPlan &P, hipStream_t st) {
+  if (!P.ok || !P.x) return;
+  tune_hier(P, P.h64, st);
+  tune_hier(P, P.h32, st);
 }
 
 }  // namespace snac

@@ -122,5 +122,8 @@ def test_learned_bfs_against_oracle(tmp_path):
     ctx2.step(dt)
     untrained = ctx2.solver_stats()
     print(f"network only, untrained: {untrained} (trained: {trained})", flush=True)
-    assert (trained[0], trained[1]) < (untrained[0], untrained[1])
+    # a preconditioner that breaks the CG down leaves a NaN residual: the worst outcome
+    un = (untrained[0], np.inf if np.isnan(untrained[1]) else untrained[1])
+    assert not np.isnan(trained[1])
+    assert (trained[0], trained[1]) < un
     ctx2.close()
