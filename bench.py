@@ -282,6 +282,9 @@ def run_form(name, d, b, args, cells, dt_ref):
                               f"1.5 x {esz} B per unknown)", "avg_launch_ms": round(avg, 4),
                     "launches": l0_n}
         out["autotune"] = {"choices": tune_choices, "best_ms": tune_ms}
+        if solver == "mg":
+            l64, l32 = ctx.mg_line_choices()
+            out["mg_line_variants"] = {"fp64": l64, "fp32": l32}
         return out
     finally:
         ctx.close()

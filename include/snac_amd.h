@@ -317,6 +317,12 @@ int snac_get_timing(snac_ctx *ctx, double *ms, int *count);
  * above) and the best times in ms.  No
  * counterpart in the reference (its FFTW plans use FFTW_ESTIMATE).          */
 int snac_get_autotune(snac_ctx *ctx, int *choices, double *ms, int n);
+/* the MG-PCG line relaxation kernel each level runs (SNAC_KV_MG_LINE
+ * numbering, chosen by timing when the MG-PCG plan is set up), for levels
+ * 0..n-1 of the FP64 and of the FP32 V-cycle; -1 for levels in the fused
+ * coarse-end kernel, past the last level, or before the MG-PCG path is
+ * planned (either pointer may be null).  No counterpart in the reference. */
+int snac_mg_line_choices(snac_ctx *ctx, int *fp64, int *fp32, int n);
 /* overrides an autotuned kernel variant: SNAC_KV_RK_MOM (0..16, as above),
  * SNAC_KV_TRIDIAG (0: c' in LDS, lines up to 128; 1: c' in scratch; 2:
  * chunked, no scratch, lines of 8..256; 3: as 2 with twice the chunks,
@@ -340,8 +346,12 @@ int snac_get_autotune(snac_ctx *ctx, int *choices, double *ms, int n);
 /* SNAC_KV_MG_RCP (process-wide): 1 makes the mode-batched MG sweeps and the
  * fused tail divide by refined hardware reciprocals (within a few ulp of
  * the IEEE quotient) instead of IEEE divisions; 0 (the default) divides.   */
+/* SNAC_KV_MG_TAIL (process-wide): 1 (the default) runs the fused coarse
+ * end of the MG-PCG V-cycle with one mode's vectors of every coarse level
+ * resident in LDS where they fit (160 KB per workgroup), 0 with them in
+ * global memory; both compute the same values.                             */
 enum { SNAC_KV_RK_MOM = 0, SNAC_KV_TRIDIAG = 1, SNAC_KV_GHOST = 2, SNAC_KV_RK_MOM32 = 3, SNAC_KV_MG_LINE = 4,
-       SNAC_KV_MG_RCP = 5 };
+       SNAC_KV_MG_RCP = 5, SNAC_KV_MG_TAIL = 6 };
 int snac_set_kernel_variant(snac_ctx *ctx, int kernel, int variant);
 
 /* ---- building blocks, exposed for their own parity tests ----

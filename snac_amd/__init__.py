@@ -92,6 +92,7 @@ def _load():
     L.snac_r2r_lines.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp]
     L.snac_stream_copy.argtypes = [vp, C.c_long, C.c_int, C.POINTER(C.c_double)]
     L.snac_get_autotune.argtypes = [vp, ip, dp, C.c_int]
+    L.snac_mg_line_choices.argtypes = [vp, ip, ip, C.c_int]
     L.snac_set_kernel_variant.argtypes = [vp, C.c_int, C.c_int]
     L.snac_set_timing.argtypes = [vp, C.c_int]
     L.snac_get_timing.argtypes = [vp, dp, ip]
@@ -399,6 +400,16 @@ class Context:
         k = int(np.argmax(ch == -2)) if (ch == -2).any() else 16
         return ch[:k].tolist(), [round(float(x), 4) for x in ms[:k] if x > 0]
 
+    def mg_line_choices(self):
+        """Per level of the FP64 and of the FP32 V-cycle, the line relaxation
+        kernel it runs (set_mg_line_variant's numbering; -1: a level of the
+        fused coarse end, or the MG-PCG path not planned yet)."""
+        a = np.zeros(32, np.int32)
+        b = np.zeros(32, np.int32)
+        _check(lib.snac_mg_line_choices(self.h, a, b, 32), "mg_line_choices")
+        trim = lambda v: [int(x) for x in v[: max([i + 1 for i, x in enumerate(v) if x >= 0] or [0])]]
+        return trim(a), trim(b)
+
     def set_rk_variant(self, v):
         """Overrides the autotuned rk_mom kernel variant (0..16)."""
         _check(lib.snac_set_kernel_variant(self.h, 0, int(v)), "set_kernel_variant")
@@ -417,6 +428,12 @@ class Context:
         (the default) restores each level's fastest, timed at set-up.  0..2
         compute the same values."""
         _check(lib.snac_set_kernel_variant(self.h, 4, int(v)), "set_kernel_variant")
+
+    def set_mg_tail_lds(self, on):
+        """Process-wide: the fused coarse end of the V-cycle with one mode's
+        vectors of every coarse level in LDS where they fit (the default),
+        or in global memory; both compute the same values."""
+        _check(lib.snac_set_kernel_variant(self.h, 6, int(bool(on))), "set_kernel_variant")
 
     def set_mg_fast_rcp(self, on):
         """Process-wide: the mode-batched MG sweeps and the fused tail divide

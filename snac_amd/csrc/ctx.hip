@@ -2059,6 +2059,16 @@ int snac_get_autotune(snac_ctx *c, int *choices, double *ms, int n) {
   });
 }
 
+int snac_mg_line_choices(snac_ctx *c, int *v64, int *v32, int n) {
+  return guard([&] {
+    const MGPlan &P = c->mg;
+    for (int l = 0; l < n; l++) {
+      if (v64) v64[l] = c->mg_planned && l < (int)P.h64.lev.size() ? P.h64.lev[l].lvar : -1;
+      if (v32) v32[l] = c->mg_planned && l < (int)P.h32.lev.size() ? P.h32.lev[l].lvar : -1;
+    }
+  });
+}
+
 int snac_set_kernel_variant(snac_ctx *c, int kernel, int variant) {
   return guard([&] {
     on_device(c);
@@ -2069,6 +2079,10 @@ int snac_set_kernel_variant(snac_ctx *c, int kernel, int variant) {
       if (variant > 4) throw std::runtime_error("MG line relaxation variant must be 0..4 or negative");
       if (c->stream) SNAC_HIP(hipStreamSynchronize(c->stream));
       mg_set_line_variant(variant < 0 ? -1 : variant);
+    } else if (kernel == SNAC_KV_MG_TAIL) {
+      if (variant < 0 || variant > 1) throw std::runtime_error("MG tail variant must be 0 or 1");
+      if (c->stream) SNAC_HIP(hipStreamSynchronize(c->stream));
+      mg_set_tail_lds(variant == 1);
     } else if (kernel == SNAC_KV_MG_RCP) {
       if (variant < 0 || variant > 1) throw std::runtime_error("MG reciprocal variant must be 0 or 1");
       if (c->stream) SNAC_HIP(hipStreamSynchronize(c->stream));

@@ -414,20 +414,36 @@ __global__ __launch_bounds__(256) void k_line_modes(MGLevT<T> L, const T *__rest
                                                     const int *__restrict__ done, const T *__restrict__ xc,
                                                     int cna) {
   constexpr int NLW = 256 / LPL;            // modes per workgroup
-  constexpr int MB = NLW < 8 ? NLW : 8;     // modes whose loads are issued together
+  constexpr int MB = NLW < 4 ? NLW : 4;     // modes whose loads are issued together
   extern __shared__ unsigned char smem_[];
   const int ns = L.ns, nsp = lpad_len(ns);
   // [nsp] each, rows at lpad: dg, w, cs, the couplings to a+1 and a-1
   T *cdg = reinterpret_cast<T *>(smem_);
   T *cw = cdg + nsp, *ccs = cw + nsp, *cap = ccs + nsp, *cam = cap + nsp;
   T *sh = cam + nsp;  // [NLW][nsp]: right-hand sides, then x
+  // per mode slot of the workgroup: its mode, and -1 when skipped (past the
+  // last mode, or converged), else the row to pin (ns: none); read once
+  // from memory by one thread each
+  __shared__ int sflag[NLW], smode[NLW];
   const int mb = (L.nm + NLW - 1) / NLW;   // mode batches per line position
-  const int ai = blockIdx.x / mb, m0 = (blockIdx.x % mb) * NLW;
+  const int ai = blockIdx.x / mb, m0 = (blockIdx.x % mb) * NLW;  // the first mode slot
+  // slots index the modes still iterating (the whole workgroup leaves when
+  // its first slot is past them)
+  const int nlive = L.mlist ? *L.mcount : L.nm;
+  if (m0 >= nlive) return;
   const int a = 2 * ai + col_a;
   const long sa = ns, wrap = (long)(L.na - 1) * sa;
   const bool hp = a < L.na - 1 || L.per_a, hm = a > 0 || L.per_a;  // neighbour lines that exist
   const long dp = a < L.na - 1 ? sa : -wrap, dm = a > 0 ? -sa : wrap;  // their offsets
   const long q0 = cidx(L, 0, a, 0);
+  if (threadIdx.x < NLW) {
+    const int slot = m0 + threadIdx.x;
+    const int m = slot < nlive ? (L.mlist ? L.mlist[slot] : slot) : -1;
+    int f = -1;
+    if (m >= 0 && !(done && done[m])) f = (pin_s >= 0 && singular && singular[m]) ? pin_s : ns;
+    sflag[threadIdx.x] = f;
+    smode[threadIdx.x] = m < 0 ? 0 : m;
+  }
   for (int s = threadIdx.x; s < ns; s += 256) {
     const long q = q0 + s;
     const int o = lpad(s);
@@ -444,28 +460,32 @@ __global__ __launch_bounds__(256) void k_line_modes(MGLevT<T> L, const T *__rest
   const int ra = cna != L.na;
   const int anp = a < L.na - 1 ? a + 1 : 0, anm = a > 0 ? a - 1 : L.na - 1;  // neighbour columns
   const int cnp = ra ? anp >> 1 : anp, cnm = ra ? anm >> 1 : anm;           // their coarse columns
+  // per mode, base pointers of line a (and of the neighbour lines and their
+  // coarse columns) indexed by the row: 32-bit offsets, no per-element
+  // 64-bit address arithmetic
+  const long mst = (long)L.na * ns, cmst = (long)cna * sa;  // elements per mode, fine and coarse
+  const T *b0 = b + vidx(L, 0, a, 0), *x0 = x + vidx(L, 0, a, 0);
   for (int g0 = 0; g0 < NLW; g0 += MB) {
     bool live[MB];
     int pinrow[MB];
 #pragma unroll
     for (int k = 0; k < MB; k++) {
-      const int m = m0 + g0 + k;
-      live[k] = m < L.nm && !(done && done[m]);
-      pinrow[k] = (live[k] && pin_s >= 0 && singular && singular[m]) ? pin_s : -1;
+      pinrow[k] = sflag[g0 + k];
+      live[k] = pinrow[k] >= 0;
     }
     for (int s = threadIdx.x; s < ns; s += 256) {
+      const unsigned us = (unsigned)s;
       const bool actp = xc && hp && L.act[q0 + s + dp], actm = xc && hm && L.act[q0 + s + dm];
       T bv[MB], xp[MB], xm[MB], cp[MB], cm[MB];
 #pragma unroll
       for (int k = 0; k < MB; k++) {
-        const int m = m0 + g0 + k;
-        const long e = vidx(L, m, a, s);
-        const long ec = (long)m * cna * sa + s;
-        bv[k] = live[k] ? b[e] : T(0);
-        xp[k] = live[k] && !zero_nb && hp ? x[e + dp] : T(0);
-        xm[k] = live[k] && !zero_nb && hm ? x[e + dm] : T(0);
-        cp[k] = live[k] && !zero_nb && actp ? xc[ec + (long)cnp * sa] : T(0);
-        cm[k] = live[k] && !zero_nb && actm ? xc[ec + (long)cnm * sa] : T(0);
+        const long m = smode[g0 + k];
+        const long mo = m * mst, co = m * cmst;
+        bv[k] = live[k] ? (b0 + mo)[us] : T(0);
+        xp[k] = live[k] && !zero_nb && hp ? (x0 + mo + dp)[us] : T(0);
+        xm[k] = live[k] && !zero_nb && hm ? (x0 + mo + dm)[us] : T(0);
+        cp[k] = live[k] && !zero_nb && actp ? (xc + co + (long)cnp * sa)[us] : T(0);
+        cm[k] = live[k] && !zero_nb && actm ? (xc + co + (long)cnm * sa)[us] : T(0);
       }
       const int o = lpad(s);
 #pragma unroll
@@ -487,10 +507,11 @@ __global__ __launch_bounds__(256) void k_line_modes(MGLevT<T> L, const T *__rest
   __syncthreads();
   // phase 2: each lane segment eliminates its mode's line
   {
-    const int li = threadIdx.x / LPL, m = m0 + li;
-    if (m < L.nm && !(done && done[m])) {  // a whole segment shares the test
+    const int li = threadIdx.x / LPL, m = smode[li];
+    const int fl = sflag[li];
+    if (fl >= 0) {  // a whole segment shares the test
       const T lmv = lam ? lam[m] : T(0);
-      const bool pin = pin_s >= 0 && singular && singular[m];
+      const bool pin = fl < ns;
       const int lane = (threadIdx.x & 63) & (LPL - 1);
       const int nch = ns >= 2 ? min(LPL, ns / 2) : 1;
       const int base = ns >= 2 ? ns / nch : 1, extra = ns >= 2 ? ns % nch : 0;
@@ -512,12 +533,11 @@ __global__ __launch_bounds__(256) void k_line_modes(MGLevT<T> L, const T *__rest
   }
   __syncthreads();
   // phase 3: x, rows across threads
+  T *xo = x + vidx(L, 0, a, 0);
   for (int li = 0; li < NLW; li++) {
-    const int m = m0 + li;
-    if (m >= L.nm) break;
-    if (done && done[m]) continue;
-    const long e0 = vidx(L, m, a, 0);
-    for (int s = threadIdx.x; s < ns; s += 256) x[e0 + s] = sh[li * nsp + lpad(s)];
+    if (sflag[li] < 0) continue;
+    T *xl = xo + (long)smode[li] * mst;
+    for (int s = threadIdx.x; s < ns; s += 256) xl[(unsigned)s] = sh[li * nsp + lpad(s)];
   }
 }
 
@@ -527,12 +547,27 @@ __global__ __launch_bounds__(256) void k_line_modes(MGLevT<T> L, const T *__rest
 // the prolongation folded into the first post-smoothing sweep, barriers
 // between the steps.  One launch replaces five per level.  FFT path only
 // (independent modes).
-template <int K, int LPL, typename T, bool FR>
+// LX: the mode's x and b of every tail level live in LDS (b of the first
+// level loaded at the start, x of the first level stored at the end), the
+// level arrays then addressed as mode 0 (the coefficients do not depend on
+// the mode); the arithmetic is the same, so the values are.
+template <int K, int LPL, typename T, bool FR, bool LX>
 __global__ __launch_bounds__(1024) void k_tail(MGTailT<T> t, const T *__restrict__ lam, int pin_s,
                                                const int *__restrict__ singular, const int *__restrict__ done) {
+  extern __shared__ unsigned char smem_[];
   const int m = blockIdx.x;
   if (done && done[m]) return;
   const T lmv = lam[m];
+  T *lds = reinterpret_cast<T *>(smem_);
+  auto X = [&](int l) -> T * { return LX ? lds + t.ox[l] : t.x[l]; };
+  auto B = [&](int l) -> T * { return LX ? lds + t.ob[l] : t.b[l]; };
+  const int mi = LX ? 0 : m;  // the mode index the level arrays are addressed with
+  if (LX) {
+    const int n0 = t.lev[0].na * t.lev[0].ns;
+    const T *bg = t.b[0] + vidx(t.lev[0], m, 0, 0);
+    for (int e = threadIdx.x; e < n0; e += blockDim.x) lds[t.ob[0] + e] = bg[e];
+    __syncthreads();
+  }
   const int wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
   constexpr int SPW = 64 / LPL;                   // lines per wavefront
   const int hs = (threadIdx.x & 63) / LPL;        // this lane's segment
@@ -544,7 +579,7 @@ __global__ __launch_bounds__(1024) void k_tail(MGTailT<T> t, const T *__restrict
       const int nlc = (L.na - c + 1) / 2;  // lines a = 2 j + c of this colour
       for (int j0 = SPW * wv; j0 < nlc; j0 += SPW * nw) {
         const int j = j0 + hs;
-        line_solve<K, LPL, T, FR>(L, lmv, t.x[l], t.b[l], m, 2 * j + c, c == 0, -1, false, j < nlc);
+        line_solve<K, LPL, T, FR>(L, lmv, X(l), B(l), mi, 2 * j + c, c == 0, -1, false, j < nlc);
       }
       __syncthreads();
     }
@@ -555,17 +590,17 @@ __global__ __launch_bounds__(1024) void k_tail(MGTailT<T> t, const T *__restrict
       T sum = 0;
       for (int da = 0; da <= ra; da++) {
         const int a = ra ? 2 * I + da : I;
-        if (a >= L.na || !L.act[cidx(L, m, a, s)]) continue;
-        sum += t.b[l][vidx(L, m, a, s)] - mx_at(L, lmv, t.x[l], m, a, s);
+        if (a >= L.na || !L.act[cidx(L, mi, a, s)]) continue;
+        sum += B(l)[vidx(L, mi, a, s)] - mx_at(L, lmv, X(l), mi, a, s);
       }
-      t.b[l + 1][vidx(Cl, m, I, s)] = sum;
+      B(l + 1)[vidx(Cl, mi, I, s)] = sum;
     }
     __syncthreads();
   }
   // coarsest: one line, solved exactly
   {
     const int l = t.nl - 1;
-    if (wv == 0) line_solve<K, LPL, T, FR>(t.lev[l], lmv, t.x[l], t.b[l], m, 0, 1, pin_s, pin, hs == 0);
+    if (wv == 0) line_solve<K, LPL, T, FR>(t.lev[l], lmv, X(l), B(l), mi, 0, 1, pin_s, pin, hs == 0);
     __syncthreads();
   }
   // up: post-smooth (colour 1, then colour 0); the colour-1 sweep reads its
@@ -578,11 +613,16 @@ __global__ __launch_bounds__(1024) void k_tail(MGTailT<T> t, const T *__restrict
       const int nlc = (L.na - c + 1) / 2;
       for (int j0 = SPW * wv; j0 < nlc; j0 += SPW * nw) {
         const int j = j0 + hs;
-        line_solve<K, LPL, T, FR>(L, lmv, t.x[l], t.b[l], m, 2 * j + c, 0, -1, false, j < nlc,
-                                  c == 1 ? t.x[l + 1] : nullptr, Cl.na);
+        line_solve<K, LPL, T, FR>(L, lmv, X(l), B(l), mi, 2 * j + c, 0, -1, false, j < nlc,
+                                  c == 1 ? X(l + 1) : nullptr, Cl.na);
       }
       __syncthreads();
     }
+  }
+  if (LX) {  // the first level's x back to memory (the level above prolongs it)
+    const int n0 = t.lev[0].na * t.lev[0].ns;
+    T *xg = t.x[0] + vidx(t.lev[0], m, 0, 0);
+    for (int e = threadIdx.x; e < n0; e += blockDim.x) xg[e] = lds[t.ox[0] + e];
   }
 }
 
@@ -634,6 +674,22 @@ __global__ __launch_bounds__(256) void k_restrict_modes(MGLevT<T> F, MGLevT<T> C
   const int I = blockIdx.x / mb, m0 = (blockIdx.x % mb) * RM;
   const int nda = ra ? 2 : 1;
   const long sa = ns, wrap = (long)(F.na - 1) * sa;
+  // per mode of the workgroup, read once by one thread each: whether it is
+  // relaxed (not past the last mode, not converged) and its eigenvalue
+  // mode slots index the modes still iterating (the workgroup leaves when
+  // its first slot is past them)
+  __shared__ int slive[RM], smode[RM];
+  __shared__ T slm[RM];
+  const int nlive = F.mlist ? *F.mcount : Cl.nm;
+  if (m0 >= nlive) return;
+  if (threadIdx.x < RM) {
+    const int slot = m0 + threadIdx.x;
+    const int M = slot < nlive ? (F.mlist ? F.mlist[slot] : slot) : -1;
+    const bool lv = M >= 0 && !(done && done[M]);
+    slive[threadIdx.x] = lv;
+    smode[threadIdx.x] = lv ? M : 0;
+    slm[threadIdx.x] = lv && lamf ? lamf[M] : T(0);  // the FFT path: modes are not coarsened
+  }
   for (int da = 0; da < nda; da++) {
     const int a = ra ? 2 * I + da : I;
     const bool ok = a < F.na;
@@ -651,31 +707,54 @@ __global__ __launch_bounds__(256) void k_restrict_modes(MGLevT<T> F, MGLevT<T> C
     }
   }
   __syncthreads();
-  for (int mi = 0; mi < RM; mi++) {
-    const int M = m0 + mi;
-    if (M >= Cl.nm) break;
-    if (done && done[M]) continue;
-    const T lm = lamf ? lamf[M] : T(0);  // the FFT path: modes are not coarsened
+  // MG modes at a time, their loads issued together (no control flow between
+  // them) so that their latencies overlap
+  constexpr int MG = RM < 4 ? RM : 4;
+  for (int g0 = 0; g0 < RM; g0 += MG) {
+    bool live[MG];
+    T lm[MG];
+#pragma unroll
+    for (int k = 0; k < MG; k++) {
+      live[k] = slive[g0 + k];
+      lm[k] = slm[g0 + k];
+    }
     for (int s = threadIdx.x; s < ns; s += 256) {
-      T sum = 0;
+      T sum[MG];
+#pragma unroll
+      for (int k = 0; k < MG; k++) sum[k] = 0;
       for (int da = 0; da < nda; da++) {
         const int a = ra ? 2 * I + da : I;
         if (a >= F.na) break;
         const int o = da * ns + s;
         if (!cact[o]) continue;
-        const long e = vidx(F, M, a, s);
-        // mx_at: (dg - lm w) x - cs(s-1) x(s-1) - cs(s) x(s+1) - nb_sum
-        T y = (cdg[o] - lm * cw[o]) * x[e];
-        if (s > 0) y -= ccs[o - 1] * x[e - 1];
-        if (s < ns - 1) y -= ccs[o] * x[e + 1];
-        T r = 0;
-        if (a < F.na - 1) r += cap[o] * x[e + sa];
-        else if (F.per_a) r += cap[o] * x[e - wrap];
-        if (a > 0) r += cam[o] * x[e - sa];
-        else if (F.per_a) r += cam[o] * x[e + wrap];
-        sum += b[e] - (y - r);
+        const bool hp = a < F.na - 1 || F.per_a, hm = a > 0 || F.per_a;
+        const long dp = a < F.na - 1 ? sa : -wrap, dm = a > 0 ? -sa : wrap;
+        T xc[MG], xs0[MG], xs1[MG], xp[MG], xm[MG], bv[MG];
+#pragma unroll
+        for (int k = 0; k < MG; k++) {
+          const long e = vidx(F, smode[g0 + k], a, s);
+          xc[k] = live[k] ? x[e] : T(0);
+          xs0[k] = live[k] && s > 0 ? x[e - 1] : T(0);
+          xs1[k] = live[k] && s < ns - 1 ? x[e + 1] : T(0);
+          xp[k] = live[k] && hp ? x[e + dp] : T(0);
+          xm[k] = live[k] && hm ? x[e + dm] : T(0);
+          bv[k] = live[k] ? b[e] : T(0);
+        }
+#pragma unroll
+        for (int k = 0; k < MG; k++) {
+          // mx_at: (dg - lm w) x - cs(s-1) x(s-1) - cs(s) x(s+1) - nb_sum
+          T y = (cdg[o] - lm[k] * cw[o]) * xc[k];
+          if (s > 0) y -= ccs[o - 1] * xs0[k];
+          if (s < ns - 1) y -= ccs[o] * xs1[k];
+          T r = 0;
+          if (hp) r += cap[o] * xp[k];
+          if (hm) r += cam[o] * xm[k];
+          sum[k] += bv[k] - (y - r);
+        }
       }
-      bc[vidx(Cl, M, I, s)] = sum;
+#pragma unroll
+      for (int k = 0; k < MG; k++)
+        if (live[k]) bc[vidx(Cl, smode[g0 + k], I, s)] = sum[k];
     }
   }
 }
@@ -693,6 +772,18 @@ __global__ __launch_bounds__(256) void k_prolong(MGLevT<T> F, MGLevT<T> Cl, cons
   if (!F.act[cidx(F, m, a, s)]) return;
   const int I = F.na != Cl.na ? a >> 1 : a, M = F.nm != Cl.nm ? m >> 1 : m;
   xf[idx] += xc[vidx(Cl, M, I, s)];
+}
+
+// a deterministic pattern in [-1, 1) (an integer hash of the index)
+template <typename T>
+__global__ __launch_bounds__(256) void k_pattern(long n, T *__restrict__ x) {
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= n) return;
+  unsigned h = (unsigned)idx * 2654435761u;
+  h ^= h >> 15;
+  h *= 2246822519u;
+  h ^= h >> 13;
+  x[idx] = (T)((double)(h >> 8) * (2.0 / 16777216.0) - 1.0);
 }
 
 template <typename T>
@@ -839,6 +930,9 @@ __device__ __forceinline__ double mx_f(const MGLev &L, double lm, F &&f, int m, 
   return y - r;
 }
 
+// elements per round of a thread in the fused CG kernels
+constexpr int CG_U = 4;
+
 __device__ __forceinline__ double cg_beta(int g, const double *__restrict__ rzn, const double *__restrict__ rzo,
                                           const double *__restrict__ cross) {
   if (!rzo) return 0.0;
@@ -876,13 +970,24 @@ __global__ __launch_bounds__(256) void k_dir_apply_dot(MGLev L, const double *__
     int m, a, s;
     pos_of(L, base + i0, m, a, s);
     const int da = 256 / L.ns, ds = 256 % L.ns;
-    for (long i = i0; i < iend; i += 256) {
-      const long idx = base + i;
-      const double pc = pn(idx);
-      p[idx] = (TP)pc;
-      const double y = mx_f(L, lam ? lam[m] : 0.0, pn, m, a, s, idx, cidx(L, m, a, s));
-      s1 += pc * y;
-      pos_step(L, da, ds, m, a, s);
+    // CG_U elements per round, their loads issued before any store so that
+    // their latencies overlap; the sum keeps the element order
+    for (long i = i0; i < iend; i += 256 * CG_U) {
+      double pc[CG_U], y[CG_U];
+#pragma unroll
+      for (int u = 0; u < CG_U; u++) {
+        const long idx = base + i + 256 * u;
+        const bool ok = i + 256 * u < iend;
+        pc[u] = ok ? pn(idx) : 0.0;
+        y[u] = ok ? mx_f(L, lam ? lam[m] : 0.0, pn, m, a, s, idx, cidx(L, m, a, s)) : 0.0;
+        pos_step(L, da, ds, m, a, s);
+      }
+#pragma unroll
+      for (int u = 0; u < CG_U; u++) {
+        if (i + 256 * u >= iend) break;
+        p[base + i + 256 * u] = (TP)pc[u];
+        s1 += pc[u] * y[u];
+      }
     }
   }
   s1 = wave_sum(s1);
@@ -926,16 +1031,32 @@ __global__ __launch_bounds__(256) void k_update_apply_dot(MGLev L, const double 
     int m, a, s;
     pos_of(L, base + i0, m, a, s);
     const int da = 256 / L.ns, ds = 256 % L.ns;
-    for (long i = i0; i < iend; i += 256) {
-      const long idx = base + i;
-      const double q = mx_f(L, lam ? lam[m] : 0.0, pv, m, a, s, idx, cidx(L, m, a, s));
-      pos_step(L, da, ds, m, a, s);
-      x[idx] += al * (double)p[idx];
-      const double rn = r[idx] - al * q;
-      r[idx] = rn;
-      s1 += rn * rn;
-      if (rf) rf[idx] = (float)rn;
-      if (zf) s2 += (double)zf[idx] * rn;
+    // CG_U elements per round, their loads issued before any store so that
+    // their latencies overlap; the sums keep the element order
+    for (long i = i0; i < iend; i += 256 * CG_U) {
+      double q[CG_U], pc[CG_U], xv[CG_U], rv[CG_U], zv[CG_U];
+#pragma unroll
+      for (int u = 0; u < CG_U; u++) {
+        const long idx = base + i + 256 * u;
+        const bool ok = i + 256 * u < iend;
+        q[u] = ok ? mx_f(L, lam ? lam[m] : 0.0, pv, m, a, s, idx, cidx(L, m, a, s)) : 0.0;
+        pc[u] = ok ? (double)p[idx] : 0.0;
+        xv[u] = ok ? x[idx] : 0.0;
+        rv[u] = ok ? r[idx] : 0.0;
+        zv[u] = ok && zf ? (double)zf[idx] : 0.0;
+        pos_step(L, da, ds, m, a, s);
+      }
+#pragma unroll
+      for (int u = 0; u < CG_U; u++) {
+        if (i + 256 * u >= iend) break;
+        const long idx = base + i + 256 * u;
+        x[idx] = xv[u] + al * pc[u];
+        const double rn = rv[u] - al * q[u];
+        r[idx] = rn;
+        s1 += rn * rn;
+        if (rf) rf[idx] = (float)rn;
+        if (zf) s2 += zv[u] * rn;
+      }
     }
   }
   s1 = wave_sum(s1);
@@ -997,19 +1118,36 @@ __global__ __launch_bounds__(256) void k_sub_mean(MGLev L, T *__restrict__ x, co
   x[idx] -= (T)(sums[g * 2 + 1] / nact[g]);
 }
 
+// one workgroup: updates the done flags, counts the groups left and (list)
+// writes them in increasing order, 256 groups per round (ballot + popcount
+// positions within a wavefront, wavefront totals through LDS)
 __global__ __launch_bounds__(256) void k_flags(int ng, const double *__restrict__ rr, const double *__restrict__ bb,
-                                               double tol2, int *__restrict__ done, int *__restrict__ left) {
-  __shared__ int cnt;
-  if (threadIdx.x == 0) cnt = 0;
-  __syncthreads();
-  int mine = 0;
-  for (int g = threadIdx.x; g < ng; g += 256) {
-    if (!done[g] && rr[g * 2] <= tol2 * bb[g * 2]) done[g] = 1;
-    mine += done[g] ? 0 : 1;
+                                               double tol2, int *__restrict__ done, int *__restrict__ left,
+                                               int *__restrict__ list) {
+  __shared__ int wsum[4];
+  __shared__ int base;
+  if (threadIdx.x == 0) base = 0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int g0 = 0; g0 < ng; g0 += 256) {
+    const int g = g0 + threadIdx.x;
+    int act = 0;
+    if (g < ng) {
+      if (!done[g] && rr[g * 2] <= tol2 * bb[g * 2]) done[g] = 1;
+      act = !done[g];
+    }
+    const unsigned long long bal = __ballot(act);
+    const int pre = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[wv] = __popcll(bal);
+    __syncthreads();
+    int off = base;
+    for (int k = 0; k < wv; k++) off += wsum[k];
+    const int tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    if (act && list) list[off + pre] = g;
+    __syncthreads();  // base and wsum read by every thread before they change
+    if (threadIdx.x == 0) base += tot;
   }
-  if (mine) atomicAdd(&cnt, mine);
   __syncthreads();
-  if (threadIdx.x == 0) left[0] = cnt;
+  if (threadIdx.x == 0) left[0] = base;
 }
 
 // strided 3D copy; TILE 1: lanes along a on the read side (src stride(a) 1),
@@ -1068,6 +1206,8 @@ int g_line_variant = -1;  // forced variant, or -1: each level's own
 // the eliminations of the mode-batched sweeps and of the fused tail with
 // refined hardware reciprocals instead of IEEE divisions
 bool g_fast_rcp = false;
+// the fused tail with its vectors in LDS where they fit
+bool g_tail_lds = true;
 }  // namespace
 
 // the mode-batched sweep (k_line_modes) with LP lanes per line: false when
@@ -1184,6 +1324,8 @@ bool mg_line_variant_runs(const MGLevT<T> &L, int v) {
 template bool mg_line_variant_runs<double>(const MGLevT<double> &, int);
 template bool mg_line_variant_runs<float>(const MGLevT<float> &, int);
 void mg_set_fast_rcp(bool on) { g_fast_rcp = on; }
+void mg_set_tail_lds(bool on) { g_tail_lds = on; }
+bool mg_tail_lds() { return g_tail_lds; }
 bool mg_fast_rcp() { return g_fast_rcp; }
 int mg_line_variant() { return g_line_variant; }
 
@@ -1203,12 +1345,22 @@ void mg_tail_k(hipStream_t st, const MGTailT<T> &t, int nm, const T *lam, int pi
                const int *done) {
   const MGLevT<T> &L = t.lev[0];
   const int rows = L.base + (L.extra ? 1 : 0);
+  const size_t shm = (size_t)t.lds_elems * sizeof(T);
+  const bool lx = g_tail_lds && shm <= 160 * 1024;
 #define LAUNCH_TAIL(KK)                                                                                             \
   do {                                                                                                              \
-    if (g_fast_rcp)                                                                                                 \
-      hipLaunchKernelGGL((k_tail<KK, LPL, T, true>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);    \
+    if (lx && g_fast_rcp)                                                                                           \
+      hipLaunchKernelGGL((k_tail<KK, LPL, T, true, true>), dim3(nm), dim3(1024), shm, st, t, lam, pin_s, singular,   \
+                         done);                                                                                     \
+    else if (lx)                                                                                                    \
+      hipLaunchKernelGGL((k_tail<KK, LPL, T, false, true>), dim3(nm), dim3(1024), shm, st, t, lam, pin_s, singular,  \
+                         done);                                                                                     \
+    else if (g_fast_rcp)                                                                                            \
+      hipLaunchKernelGGL((k_tail<KK, LPL, T, true, false>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular,    \
+                         done);                                                                                     \
     else                                                                                                            \
-      hipLaunchKernelGGL((k_tail<KK, LPL, T, false>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular, done);   \
+      hipLaunchKernelGGL((k_tail<KK, LPL, T, false, false>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular,   \
+                         done);                                                                                     \
   } while (0)
   if (rows <= 3) LAUNCH_TAIL(3);
   else if (rows <= 5) LAUNCH_TAIL(5);
@@ -1252,6 +1404,12 @@ void mg_prolong(hipStream_t st, const MGLevT<T> &F, const MGLevT<T> &C, const T 
 template <typename T>
 void mg_zero(hipStream_t st, long n, T *x) {
   hipLaunchKernelGGL(k_zero<T>, dim3(blocks_for(n)), dim3(256), 0, st, n, x);
+  SNAC_HIP(hipGetLastError());
+}
+
+template <typename T>
+void mg_pattern(hipStream_t st, long n, T *x) {
+  hipLaunchKernelGGL(k_pattern<T>, dim3(blocks_for(n)), dim3(256), 0, st, n, x);
   SNAC_HIP(hipGetLastError());
 }
 
@@ -1345,8 +1503,9 @@ void mg_sub_mean(hipStream_t st, const MGLev &L, T *x, const double *sums, const
   SNAC_HIP(hipGetLastError());
 }
 
-void mg_flags(hipStream_t st, int ng, const double *rr, const double *bb, double tol2, int *done, int *left) {
-  hipLaunchKernelGGL(k_flags, dim3(1), dim3(256), 0, st, ng, rr, bb, tol2, done, left);
+void mg_flags(hipStream_t st, int ng, const double *rr, const double *bb, double tol2, int *done, int *left,
+              int *list) {
+  hipLaunchKernelGGL(k_flags, dim3(1), dim3(256), 0, st, ng, rr, bb, tol2, done, left, list);
   SNAC_HIP(hipGetLastError());
 }
 
@@ -1374,6 +1533,7 @@ void mg_copy3(hipStream_t st, const double *src, long sm, long sa, long ss, doub
                                         const T *, T *, const int *);                                                 \
   template void mg_prolong<T>(hipStream_t, const MGLevT<T> &, const MGLevT<T> &, const T *, T *, const int *);        \
   template void mg_zero<T>(hipStream_t, long, T *);                                                                   \
+  template void mg_pattern<T>(hipStream_t, long, T *);                                                                   \
   template void mg_dot<T, T>(hipStream_t, int, long, const double *, const T *, const T *, const double *, double *,  \
                              int, double *, const int *, const int *);                                                \
   template void mg_cg_dir<T>(hipStream_t, int, long, const double *, const double *, const double *, const T *,       \

@@ -45,6 +45,11 @@ struct MGLevT {
   int nch, base, extra; // a line of ns rows over nch lanes: base (+1 for the first `extra`) rows each
   int lpl;              // lanes per line segment (32 or 64; nch <= lpl)
   int lvar;             // line relaxation kernel of this level (0..4, chosen by timing at set-up)
+  // FFT path: the modes still iterating, in increasing order (mlist[0 ..
+  // *mcount - 1]), kept by mg_flags; the mode-batched kernels take their
+  // modes from it so that converged modes cost no workgroup slots (null on
+  // the coupled 3D path: every mode belongs to the one group)
+  const int *mlist, *mcount;
 };
 using MGLev = MGLevT<double>;
 
@@ -55,6 +60,10 @@ struct MGTailT {
   int nl;                   // levels in the tail; the last one has na == 1
   MGLevT<T> lev[MG_TAIL_MAX];
   T *x[MG_TAIL_MAX], *b[MG_TAIL_MAX];
+  // the LDS-resident form: one mode's x and b of level q at element offsets
+  // ox[q] / ob[q] of the workgroup's LDS, lds_elems elements in all
+  int ox[MG_TAIL_MAX], ob[MG_TAIL_MAX];
+  int lds_elems;
 };
 
 // the V-cycle hierarchy in one element type: levels, the per-mode
@@ -104,6 +113,11 @@ bool mg_line_variant_runs(const MGLevT<T> &L, int v);
 // hierarchies (FP64 and FP32) and keeps the fastest in each level's lvar
 struct MGPlan;
 void mg_tune_lines(MGPlan &P, hipStream_t st);
+// process-wide: 1 (the default) runs the fused tail with one mode's vectors
+// of every tail level resident in LDS where they fit (160 KB per
+// workgroup), 0 with them in global memory; both compute the same values
+void mg_set_tail_lds(bool on);
+bool mg_tail_lds();
 // process-wide: the mode-batched sweeps and the fused tail divide by
 // refined hardware reciprocals (the estimate plus Newton steps: two in
 // FP64, one in FP32) instead of IEEE divisions; off by default
@@ -119,6 +133,9 @@ template <typename T>
 void mg_prolong(hipStream_t st, const MGLevT<T> &F, const MGLevT<T> &C, const T *xc, T *xf, const int *done);
 template <typename T>
 void mg_zero(hipStream_t st, long n, T *x);
+// x = a fixed pseudo-random pattern in [-1, 1) (timing trials)
+template <typename T>
+void mg_pattern(hipStream_t st, long n, T *x);
 // q = M p with out[g*2] = p.q per group, in one pass over the vectors (the
 // per-group sums in mg_dot's order)
 void mg_apply_dot(hipStream_t st, const MGLev &L, const double *lam, const double *p, double *q, int ng, long gsize,
@@ -161,7 +178,10 @@ template <typename T>
 void mg_sub_mean(hipStream_t st, const MGLev &L, T *x, const double *sums, const int *singular, const double *nact,
                  const int *done);
 // done[g] |= rr[g] <= tol2 * bb[g]; *left = number of groups not done
-void mg_flags(hipStream_t st, int ng, const double *rr, const double *bb, double tol2, int *done, int *left);
+// done[g] |= r.r <= tol2 b.b; left = the groups not done; list (when given):
+// those groups in increasing order
+void mg_flags(hipStream_t st, int ng, const double *rr, const double *bb, double tol2, int *done, int *left,
+              int *list);
 // strided 3D copy (em x ea x es elements) with scaling; `tile` selects an
 // LDS transpose for src stride(a) == 1 -> dst stride(s) == 1 (1) or the
 // reverse (2); 0 = direct with lanes along s
@@ -212,6 +232,7 @@ struct MGPlan {
   double *part = nullptr, *bb = nullptr, *rr = nullptr, *pq = nullptr, *rz[2] = {nullptr, nullptr}, *sums = nullptr;
   double *nact = nullptr;
   int *done = nullptr, *singular = nullptr, *left = nullptr, *left_host = nullptr;
+  int *mlist = nullptr;  // FFT path: the groups not done, in order (left of them)
   int nchunk = 1;
   int pin_s = -1;              // coarsest-level pinned row of singular groups
   bool any_singular = false;

@@ -19,6 +19,7 @@
 // stay singular with the constant null vector.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -404,6 +405,8 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
     L.extra = ns >= 2 ? ns % nch : 0;
     L.lpl = lpl;
     L.lvar = -1;
+    L.mlist = nullptr;
+    L.mcount = nullptr;
     P.h64.lev.push_back(L);
     MGLevT<float> Lf;
     Lf.nm = L.nm; Lf.na = L.na; Lf.ns = L.ns;
@@ -415,6 +418,7 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
     Lf.cm = coupled ? dev_upload(P, f32(h.cm)) : nullptr;
     Lf.act = L.act;
     Lf.nch = L.nch; Lf.base = L.base; Lf.extra = L.extra; Lf.lpl = L.lpl; Lf.lvar = -1;
+    Lf.mlist = nullptr; Lf.mcount = nullptr;
     P.h32.lev.push_back(Lf);
     const size_t n = (size_t)h.nm * h.na * h.ns;
     if (l == 0) {
@@ -453,11 +457,17 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
     Hh.tail0 = tail0;
     if (tail0 < 0) return;
     Hh.tail.nl = nl - tail0;
+    int off = 0;
     for (int q = 0; q < Hh.tail.nl; q++) {
       Hh.tail.lev[q] = Hh.lev[tail0 + q];
       Hh.tail.x[q] = Hh.xl[tail0 + q];
       Hh.tail.b[q] = Hh.bl[tail0 + q];
+      const int n = (Hh.lev[tail0 + q].na * Hh.lev[tail0 + q].ns + 3) / 4 * 4;  // 16-byte aligned blocks
+      Hh.tail.ox[q] = off;
+      Hh.tail.ob[q] = off + n;
+      off += 2 * n;
     }
+    Hh.tail.lds_elems = off;
   };
   set_tail(P.h64);
   set_tail(P.h32);
@@ -472,7 +482,17 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
   P.nact = dev_upload(P, nact);
   P.done = dev_zeros<int>(P, P.ng);
   P.singular = dev_upload(P, sing);
-  P.left = dev_zeros<int>(P, 1);
+  // every group iterating until the first mg_flags of a solve says otherwise
+  P.left = dev_upload(P, std::vector<int>{P.ng});
+  if (!coupled) {
+    std::vector<int> ident(P.ng);
+    for (int g = 0; g < P.ng; g++) ident[g] = g;
+    P.mlist = dev_upload(P, ident);
+    for (auto &L : P.h64.lev) { L.mlist = P.mlist; L.mcount = P.left; }
+    for (auto &L : P.h32.lev) { L.mlist = P.mlist; L.mcount = P.left; }
+    set_tail(P.h64);
+    set_tail(P.h32);
+  }
   SNAC_HIP(hipHostMalloc((void **)&P.left_host, sizeof(int), hipHostMallocDefault));
   P.stall = dev_zeros<int>(P, 1);
   P.rr_prev = dev_zeros<double>(P, 2 * P.ng);
@@ -600,7 +620,7 @@ void mg_iteration(MGPlan &P, hipStream_t st, MGRun &R) {
     mg_update_apply_dot<double>(st, L0, P.h64.lam, P.ng, P.gsize, P.rz[R.cur], P.pq, P.x, P.pd[R.pc], P.r, nullptr,
                                 nullptr, P.part, P.nchunk, P.rr, P.done);
   }
-  mg_flags(st, P.ng, P.rr, P.bb, P.tol * P.tol, P.done, P.left);
+  mg_flags(st, P.ng, P.rr, P.bb, P.tol * P.tol, P.done, P.left, P.mlist);
   if (R.net && P.fallback) launch_net_stall(st, P.ng, P.rr, P.rr_prev, P.done, P.stall_ratio * P.stall_ratio, P.stall);
   R.it++;
 }
@@ -628,7 +648,7 @@ MGRun mg_solve(MGPlan &P, hipStream_t st, int fixed) {
   mg_izero(st, P.ng, P.done);
   project(P, st, P.r);
   mg_dot<double, double>(st, P.ng, P.gsize, P.r, P.r, nullptr, nullptr, P.part, P.nchunk, P.bb, P.done, nullptr);
-  mg_flags(st, P.ng, P.bb, P.bb, tol2, P.done, P.left);
+  mg_flags(st, P.ng, P.bb, P.bb, tol2, P.done, P.left, P.mlist);
   if (R.record)  // the input half of this solve's training pairs
     launch_net_pair_in(st, P.ng, L0.na, L0.ns, P.nH, P.nW, P.r, P.bb, P.nact_plane, P.h64.lam, P.lam_ref,
                        P.pair_x + (size_t)P.pair_n * pl * 2);
@@ -645,7 +665,7 @@ MGRun mg_solve(MGPlan &P, hipStream_t st, int fixed) {
     mg_apply_dot(st, L0, P.h64.lam, P.p, P.q, P.ng, P.gsize, P.part, P.nchunk, P.pq, P.done);
     mg_cg_update_dot(st, P.ng, P.gsize, P.pq, P.pq, P.x, P.p, P.r, P.q, P.rf, nullptr, P.part, P.nchunk, P.rr,
                      P.done);
-    mg_flags(st, P.ng, P.rr, P.bb, tol2, P.done, P.left);
+    mg_flags(st, P.ng, P.rr, P.bb, tol2, P.done, P.left, P.mlist);
     rr0 = P.rr;
   }
   if (R.net) {
@@ -712,21 +732,29 @@ void mg_stats(MGPlan &P, hipStream_t st, int &iters, double &max_relres) {
 namespace snac {
 namespace {
 
-// Each level's line kernel per level shape, kept for the life of the
-// process: a later plan of the same shapes reuses the choices without
-// timing, so that same-shape contexts run the same kernels and agree bit
-// for bit (variants 3 and 4 round differently from 0..2).
+// The line kernels of a hierarchy's levels, per hierarchy shape, kept for
+// the life of the process: a later plan of the same shapes reuses the
+// choices without timing, so that same-shape contexts run the same kernels
+// and agree bit for bit (variants 3 and 4 round differently from 0..2).
 std::mutex g_line_mu;
-std::map<std::string, int> g_line_choice;
+std::map<std::string, std::vector<int>> g_line_choice;
 
 template <typename T>
-std::string line_key(const MGLevT<T> &L, int dev) {
+std::string hier_key(const MGHier<T> &H, int dev) {
   std::ostringstream k;
-  k << "dev " << dev << " T" << sizeof(T) << " nm " << L.nm << " na " << L.na << " ns " << L.ns << " qm " << L.qm
-    << " coupled " << L.coupled << " per " << L.per_a << " lpl " << L.lpl;
+  k << "dev " << dev << " T" << sizeof(T) << " tail " << H.tail0;
+  for (const auto &L : H.lev)
+    k << " | nm " << L.nm << " na " << L.na << " ns " << L.ns << " qm " << L.qm << " c " << L.coupled << " p "
+      << L.per_a << " lpl " << L.lpl;
   return k.str();
 }
 
+// Chooses each launched level's line kernel by timing whole V-cycles (the
+// sweeps and restrictions of every level in their real order and cache
+// state; a level's kernel timed alone, back to back on cache-resident
+// vectors, ranks the variants differently): first the fastest variant for
+// all levels at once, then each level's own variant in turn with the others
+// fixed, a change kept only when it saves more than 2 %.
 template <typename T>
 void tune_hier(MGPlan &P, MGHier<T> &H, hipStream_t st) {
   const int nl = (int)H.lev.size();
@@ -734,42 +762,79 @@ void tune_hier(MGPlan &P, MGHier<T> &H, hipStream_t st) {
   int dev = 0;
   SNAC_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lk(g_line_mu);
-  hipEvent_t e0, e1;
-  SNAC_HIP(hipEventCreate(&e0));
-  SNAC_HIP(hipEventCreate(&e1));
-  for (int l = 0; l < last; l++) {
-    MGLevT<T> &L = H.lev[l];
-    if (L.na < 2) continue;
-    const std::string key = line_key(L, dev);
-    auto it = g_line_choice.find(key);
-    if (it != g_line_choice.end()) {
-      L.lvar = it->second;
-      continue;
-    }
-    float best = 1e30f;
-    int bv = 2;
-    for (int v = 0; v <= 4; v++) {
-      if (!mg_line_variant_runs(L, v)) continue;
-      L.lvar = v;
-      auto run = [&] {
-        mg_line(st, L, H.lam, H.xl[l], H.bl[l], 0, 0, 0, -1, nullptr, P.done);
-        mg_restrict_residual(st, L, H.lev[l + 1], H.lam, H.xl[l], H.bl[l], H.bl[l + 1], P.done);
-      };
-      run();  // warm-up
+  const std::string key = hier_key(H, dev);
+  auto it = g_line_choice.find(key);
+  if (it != g_line_choice.end()) {
+    for (int l = 0; l < nl; l++) H.lev[l].lvar = it->second[l];
+    return;
+  }
+  std::vector<int> lv(nl, -1);
+  std::vector<int> tuned;  // the launched levels with a choice to make
+  for (int l = 0; l < last; l++)
+    if (H.lev[l].na >= 2) tuned.push_back(l);
+  if (!tuned.empty()) {
+    hipEvent_t e0, e1;
+    SNAC_HIP(hipEventCreate(&e0));
+    SNAC_HIP(hipEventCreate(&e1));
+    auto apply = [&](const std::vector<int> &v) {
+      for (int l = 0; l < nl; l++) H.lev[l].lvar = v[l];
+    };
+    // a right-hand side with values (zeros would leave every line trivial)
+    mg_pattern(st, (long)H.lev[0].nm * H.lev[0].na * H.lev[0].ns, H.bl[0]);
+    auto time_cycles = [&](const std::vector<int> &v) {
+      apply(v);
+      vcycle(P, H, st, 0);  // warm-up
       SNAC_HIP(hipEventRecord(e0, st));
-      for (int r = 0; r < 3; r++) run();
+      for (int r = 0; r < 3; r++) vcycle(P, H, st, 0);
       SNAC_HIP(hipEventRecord(e1, st));
       SNAC_HIP(hipEventSynchronize(e1));
       float ms = 0.f;
       SNAC_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best) { best = ms; bv = v; }
+      return ms;
+    };
+    // the variant a level runs when v is asked for (the next lower one it allows)
+    auto runs_as = [&](int l, int v) {
+      while (v > 0 && !mg_line_variant_runs(H.lev[l], v)) v--;
+      return v;
+    };
+    // untimed V-cycles first: the clocks of an idle GPU ramp up under load,
+    // and latency-bound and throughput-bound kernels gain differently
+    apply(lv);
+    for (int r = 0; r < 30; r++) vcycle(P, H, st, 0);
+    // each candidate timed in two rounds, the better of the two kept
+    float best = 1e30f;
+    std::vector<int> bestv = lv;
+    std::vector<std::vector<int>> cands;
+    for (int v = 0; v <= 4; v++) {
+      std::vector<int> cand = lv;
+      bool any = false;
+      for (int l : tuned) {
+        cand[l] = runs_as(l, v);
+        any = any || cand[l] == v;
+      }
+      if (any) cands.push_back(cand);  // else every level would fall back to a variant already timed
     }
-    L.lvar = bv;
-    g_line_choice.emplace(key, bv);
+    std::vector<float> ms(cands.size(), 1e30f);
+    for (int round = 0; round < 2; round++)
+      for (size_t c = 0; c < cands.size(); c++) ms[c] = std::min(ms[c], time_cycles(cands[c]));
+    for (size_t c = 0; c < cands.size(); c++)
+      if (ms[c] < best) { best = ms[c]; bestv = cands[c]; }
+    for (int l : tuned) {
+      for (int v = 0; v <= 4; v++) {
+        if (v == bestv[l] || runs_as(l, v) != v) continue;
+        std::vector<int> cand = bestv;
+        cand[l] = v;
+        const float t = std::min(time_cycles(cand), time_cycles(cand));
+        if (t < 0.98f * best) { best = t; bestv = cand; }
+      }
+    }
+    lv = bestv;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  // the trials relaxed zero right-hand sides: the vectors are zero again
+  for (int l = 0; l < nl; l++) H.lev[l].lvar = lv[l];
+  g_line_choice.emplace(key, lv);
+  // the trials ran V-cycles on a patterned right-hand side: zero the vectors again
   for (int l = 0; l < nl; l++) {
     const long n = (long)H.lev[l].nm * H.lev[l].na * H.lev[l].ns;
     mg_zero(st, n, H.xl[l]);

@@ -112,10 +112,11 @@ def test_mg_matches_tensor_solver(tmp_path):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-@pytest.mark.parametrize("geom", ["lchannel", "bfs"])
+@pytest.mark.parametrize("geom", ["lchannel", "bfs", "cavity3d"])
 def test_mg_10_steps(tmp_path, geom, prec):
     """FP64: the plain form's 1e-10; mixed (FP32 momentum right-hand side,
-    FP32 V-cycle and search directions): the mixed form's 1e-6 gate."""
+    FP32 V-cycle and search directions): the mixed form's 1e-6 gate.  The
+    cavity has no FFT axis: its solves run the coupled 3D MG-PCG path."""
     ctx, oc, singular = pair(tmp_path, geom, prec)
     tol = 1e-10 if prec == "fp64" else 1e-6
     run_both(ctx, oc, 10, dt_rel=1e-12 if prec == "fp64" else 1e-6)
@@ -261,3 +262,27 @@ def test_line_variants_long_lines(tmp_path, geom, prec):
         np.testing.assert_array_equal(out[0], out[v])
     for v in (3, 4):
         print(f"variant {v}: max |pp - pp(variant 0)| = {np.abs(out[v] - out[0]).max():.2e}", flush=True)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("geom", ["bfs", "channel_y256"])
+def test_tail_lds_agrees(tmp_path, geom, prec):
+    """The fused coarse end with its vectors resident in LDS computes what
+    the global-memory form computes, bit for bit (where the vectors do not
+    fit, both runs take the global form)."""
+    out = []
+    for lds in (False, True):
+        ctx, oc, singular = pair(tmp_path / f"t{int(lds)}", geom, prec)
+        ctx.set_mg_tail_lds(lds)
+        seed_fields(ctx, oc)
+        ctx.bounduvw(False)
+        ctx.fillps(1e-3)
+        ctx.solve_pressure()
+        iters, relres = ctx.solver_stats()
+        print(f"{geom} {prec} tail in LDS {lds}: {iters} CG iterations, max relative residual {relres:.2e}",
+              flush=True)
+        out.append([ctx.get(b, "pp") for b in range(ctx.nblocks)])
+        ctx.set_mg_tail_lds(True)
+        ctx.close()
+    for a, b in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a, b)
