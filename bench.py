@@ -88,6 +88,8 @@ def parse():
                     help="MG line relaxation kernel (0 rows per lane, 1 LDS-staged, 2 LDS-staged over modes, "
                          "3 / 4 as 2 with 16 / 8 lanes per line, -1 each level's fastest: the default)")
     ap.add_argument("--mg-fast-rcp", type=int, default=None, help="1: MG eliminations with refined reciprocals")
+    ap.add_argument("--mg-tail-lds", type=int, default=None,
+                    help="0: the fused coarse end of the V-cycle with its vectors in global memory instead of LDS")
     ap.add_argument("--learn-steps", type=int, default=20)
     ap.add_argument("--learn-epochs", type=int, default=4)
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps for the CPU baseline (0 = skip)")
@@ -189,6 +191,8 @@ def run_form(name, d, b, args, cells, dt_ref):
             ctx.set_mg_line_variant(args.mg_line_variant)
         if args.mg_fast_rcp is not None:
             ctx.set_mg_fast_rcp(args.mg_fast_rcp)
+        if args.mg_tail_lds is not None:
+            ctx.set_mg_tail_lds(args.mg_tail_lds)
         ctx.set_solver(solver)
         if prec != "learned":
             ctx.set_precision(prec)
@@ -268,6 +272,8 @@ def run_form(name, d, b, args, cells, dt_ref):
             "traffic": traffic_for("rk_mom", args.workload, args.size, name),
             "kernel": f"rk_mom ({'FP32' if f32 else 'FP64'} right-hand side, variant {rkv}; "
                       f"{RK_BYTES[f32]} B/cell)", "avg_launch_ms": round(rk_avg, 4)}
+        if solver == "mg" and ph["mg_tail"][1] > 0:
+            out["mg_tail_avg_ms"] = round(ph["mg_tail"][0] / ph["mg_tail"][1], 4)
         if solver == "mg":
             l0_ms, l0_n = ph["mg_line0"]
             if l0_n > 0:

@@ -294,7 +294,9 @@ enum {
   SNAC_PH_HELMHOLTZ = 8, /* implicit diffusion: the three Helmholtz solves  */
   SNAC_PH_MG_LINE0 = 9, /* MG-PCG: each finest-level zebra colour sweep that
                            reads its neighbour lines (inside SNAC_PH_TRIDIAG) */
-  SNAC_NPHASE = 10
+  SNAC_PH_MG_TAIL = 10, /* MG-PCG: each launch of the fused coarse end of the
+                           V-cycle (inside SNAC_PH_TRIDIAG)                  */
+  SNAC_NPHASE = 11
 };
 int snac_set_timing(snac_ctx *ctx, int enable);
 /* waits for the stream, returns accumulated milliseconds and the number of

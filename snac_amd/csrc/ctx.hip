@@ -2008,20 +2008,26 @@ int snac_set_timing(snac_ctx *c, int enable) {
     c->timing = enable != 0;
     std::fill(c->tacc, c->tacc + SNAC_NPHASE, 0.0);
     std::fill(c->tcnt, c->tcnt + SNAC_NPHASE, 0);
-    // the finest MG level's colour sweeps get their own events
-    if (c->timing) {
+    // the finest MG level's colour sweeps and the fused coarse end get
+    // their own events
+    auto marker = [c](int phase) {
       auto open = std::make_shared<hipEvent_t>();
-      c->mg.mark_line0 = [c, open](bool begin) {
+      return [c, open, phase](bool begin) {
         if (begin) {
           *open = ev_get(c);
           SNAC_HIP(hipEventRecord(*open, c->stream));
         } else {
           hipEvent_t b = ev_get(c);
-          if (hipEventRecord(b, c->stream) == hipSuccess) c->pending.push_back({SNAC_PH_MG_LINE0, *open, b});
+          if (hipEventRecord(b, c->stream) == hipSuccess) c->pending.push_back({phase, *open, b});
         }
       };
+    };
+    if (c->timing) {
+      c->mg.mark_line0 = marker(SNAC_PH_MG_LINE0);
+      c->mg.mark_tail = marker(SNAC_PH_MG_TAIL);
     } else {
       c->mg.mark_line0 = nullptr;
+      c->mg.mark_tail = nullptr;
     }
   });
 }

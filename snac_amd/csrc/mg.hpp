@@ -258,6 +258,8 @@ struct MGPlan {
   // optional timing marks around the finest level's colour sweeps that read
   // their neighbour lines (begin = true before the launch, false after)
   std::function<void(bool)> mark_line0;
+  // the same around each launch of the fused coarse end
+  std::function<void(bool)> mark_tail;
 };
 
 // Builds the hierarchy for the case (returns with ok == false and a reason

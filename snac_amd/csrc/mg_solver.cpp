@@ -508,7 +508,9 @@ void vcycle(MGPlan &P, MGHier<T> &H, hipStream_t st, size_t l) {
   T *x = H.xl[l];
   const T *b = H.bl[l];
   if ((int)l == H.tail0) {  // the coarse end in one launch, one workgroup per mode
+    if (P.mark_tail) P.mark_tail(true);
     mg_tail(st, H.tail, P.ng, H.lam, P.pin_s, P.singular, P.done);
+    if (P.mark_tail) P.mark_tail(false);
     return;
   }
   if (l + 1 == H.lev.size()) {  // one line per group: direct solve
