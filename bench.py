@@ -60,6 +60,10 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 METRIC = "Mcell-steps/s + Poisson ms/step, learned/mixed vs plain FP64, equal div tol"
 HBM_PEAK_GBS = 8000.0
 RK_BYTES = {False: 104, True: 80}  # per cell: FP64 / FP32 right-hand side (FP32 old tendencies)
+# the CG update pass per unknown: p read, x and r read and written (FP64),
+# plus (FP32 preconditioners) z_old read and the FP32 copy of r written;
+# p is FP32 in the mixed form
+UPDATE_BYTES = {"fp64": 8 + 16 + 16, "mixed": 4 + 16 + 16 + 4 + 4, "learned": 8 + 16 + 16 + 4 + 4}
 PROFILES = os.path.join(ROOT, "profiles")
 
 FORMS = {
@@ -76,7 +80,8 @@ FORMS = {
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=20,
+                    help="timed graph-replayed steps (0: none, for eager-only profiling runs)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", choices=["channel", "bfs"], default="channel")
     ap.add_argument("--size", type=int, nargs=3, default=[512, 256, 256])
@@ -172,6 +177,13 @@ def cpu_baseline(d, b, size, steps, threads):
             "host_cpus": os.cpu_count(), "cpu_model": cpu_model()}
 
 
+def box_unknowns(workload, cells):
+    """Unknowns of the MG-PCG vectors: the bounding box of the union of
+    blocks (the channel is its own box; the step's blocks span 512 x 160 x
+    128 with the region below the inlet inactive)."""
+    return (192 + 320) * (64 + 96) * 128 if workload == "bfs" else cells
+
+
 def traffic_for(kernel, workload, size, form):
     """HBM bytes per launch from the committed PMC summary of this kernel,
     workload and form (FETCH_SIZE and WRITE_SIZE runs), if there is one."""
@@ -247,8 +259,9 @@ def run_form(name, d, b, args, cells, dt_ref):
         ph = ctx.get_timing()
         ctx.set_timing(False)
         out = {"solver": solver, "precision": prec, "dtype": dtype, "impdiff": impdiff,
-               "mcell_steps_per_s": round(cells * args.steps / el / 1e6, 2),
-               "ms_per_step": round(el * 1e3 / args.steps, 4),
+               # --steps 0 (eager-only profiling runs): no timed graph steps
+               "mcell_steps_per_s": round(cells * args.steps / el / 1e6, 2) if args.steps > 0 else None,
+               "ms_per_step": round(el * 1e3 / args.steps, 4) if args.steps > 0 else None,
                "poisson_ms_per_step": round(sum(ph[k][0] for k in ("r2r_fwd", "tridiag", "r2r_bwd")) / ne, 4),
                "phase_ms_per_step": {k: round(v[0] / ne, 4) for k, v in ph.items() if v[1] > 0},
                "phase_timing": f"eager launches with per-phase HIP events, {ne} steps, "
@@ -275,18 +288,34 @@ def run_form(name, d, b, args, cells, dt_ref):
         if solver == "mg" and ph["mg_tail"][1] > 0:
             out["mg_tail_avg_ms"] = round(ph["mg_tail"][0] / ph["mg_tail"][1], 4)
         if solver == "mg":
+            # the MG vectors span the bounding box of the union of blocks
+            unk = box_unknowns(args.workload, cells)
+            up_ms, up_n = ph["mg_update"]
+            if up_n > 0:
+                avg = up_ms / up_n
+                bytes_ = UPDATE_BYTES[prec] * unk
+                out["roofline_update"] = {
+                    "bound": "hbm", "achieved": round(bytes_ / (avg * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(bytes_ / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    "traffic": traffic_for("k_update_apply_dot", args.workload, args.size, name),
+                    "kernel": f"k_update_apply_dot, the CG update pass ({UPDATE_BYTES[prec]} B per unknown)",
+                    "avg_launch_ms": round(avg, 4), "launches": up_n, "ms_per_step": round(up_ms / ne, 4)}
             l0_ms, l0_n = ph["mg_line0"]
             if l0_n > 0:
                 esz = 4 if f32 else 8
                 avg = l0_ms / l0_n
-                bytes_ = 1.5 * cells * esz
+                bytes_ = 1.5 * unk * esz
                 out["roofline_line0"] = {
                     "bound": "hbm", "achieved": round(bytes_ / (avg * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(bytes_ / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                     "traffic": traffic_for("k_line", args.workload, args.size, name),
                     "kernel": f"k_line, finest level, one zebra colour ({'FP32' if f32 else 'FP64'}; "
                               f"1.5 x {esz} B per unknown)", "avg_launch_ms": round(avg, 4),
-                    "launches": l0_n}
+                    "launches": l0_n, "ms_per_step": round(l0_ms / ne, 4)}
+            # the dominant MG kernel: the one with more time per step
+            cand = [k for k in ("roofline_update", "roofline_line0") if k in out]
+            if cand:
+                out["roofline_dominant"] = max(cand, key=lambda k: out[k]["ms_per_step"])
         out["autotune"] = {"choices": tune_choices, "best_ms": tune_ms}
         if solver == "mg":
             l64, l32 = ctx.mg_line_choices()
@@ -341,7 +370,7 @@ def main():
     head = next((f for f in ("learned", "mixed") if f in results), forms[0])
     H = results[head]
     plain = results.get("plain")
-    roof = H.get("roofline_line0") or H["roofline_stencil"]
+    roof = H[H["roofline_dominant"]] if "roofline_dominant" in H else H["roofline_stencil"]
     out = {
         "metric": METRIC, "value": H["mcell_steps_per_s"], "unit": "Mcell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": H["ms_per_step"],
@@ -360,12 +389,14 @@ def main():
                     "hypre_tol": args.tol, "parallelism": "1 GPU"}),
         "poisson_ms_per_step": H["poisson_ms_per_step"],
         "vs_plain_fp64": (round(H["mcell_steps_per_s"] / plain["mcell_steps_per_s"], 3)
-                          if plain and head != "plain" else None),
+                          if plain and head != "plain" and args.steps > 0 else None),
         "poisson_vs_plain_fp64": (round(plain["poisson_ms_per_step"] / H["poisson_ms_per_step"], 3)
                                   if plain and head != "plain" else None),
         "divtot": H["divtot"], "divmax": H["divmax"],
         "roofline": dict(roof, stream_copy_GBs=copy_gbs),
         "roofline_stencil": H["roofline_stencil"],
+        "roofline_update": H.get("roofline_update"),
+        "roofline_line0": H.get("roofline_line0"),
         "cpu_baseline": cpu,
         "forms": results,
     }

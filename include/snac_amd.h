@@ -296,7 +296,9 @@ enum {
                            reads its neighbour lines (inside SNAC_PH_TRIDIAG) */
   SNAC_PH_MG_TAIL = 10, /* MG-PCG: each launch of the fused coarse end of the
                            V-cycle (inside SNAC_PH_TRIDIAG)                  */
-  SNAC_NPHASE = 11
+  SNAC_PH_MG_UPDATE = 11, /* MG-PCG: each CG update pass (x += alpha p,
+                           r -= alpha Mp, r.r; inside SNAC_PH_TRIDIAG)       */
+  SNAC_NPHASE = 12
 };
 int snac_set_timing(snac_ctx *ctx, int enable);
 /* waits for the stream, returns accumulated milliseconds and the number of

@@ -29,7 +29,7 @@ def _ghosted(fid):
     return f < 6 or (f >= SCAL0 and (f - SCAL0) % 2 == 0)
 RKCOEFF = ((32.0 / 60.0, 0.0), (25.0 / 60.0, -17.0 / 60.0), (45.0 / 60.0, -25.0 / 60.0))
 PHASES = ("rk_mom", "bound", "fillps", "r2r_fwd", "tridiag", "r2r_bwd", "correc", "rk_scal", "helmholtz",
-          "mg_line0", "mg_tail")
+          "mg_line0", "mg_tail", "mg_update")
 R2R_KINDS = {"R2HC": 0, "HC2R": 1, "REDFT00": 3, "REDFT01": 4, "REDFT10": 5, "REDFT11": 6,
              "RODFT00": 7, "RODFT01": 8, "RODFT10": 9, "RODFT11": 10}
 

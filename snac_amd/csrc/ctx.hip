@@ -2025,9 +2025,11 @@ int snac_set_timing(snac_ctx *c, int enable) {
     if (c->timing) {
       c->mg.mark_line0 = marker(SNAC_PH_MG_LINE0);
       c->mg.mark_tail = marker(SNAC_PH_MG_TAIL);
+      c->mg.mark_update = marker(SNAC_PH_MG_UPDATE);
     } else {
       c->mg.mark_line0 = nullptr;
       c->mg.mark_tail = nullptr;
+      c->mg.mark_update = nullptr;
     }
   });
 }

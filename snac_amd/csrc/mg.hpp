@@ -258,8 +258,9 @@ struct MGPlan {
   // optional timing marks around the finest level's colour sweeps that read
   // their neighbour lines (begin = true before the launch, false after)
   std::function<void(bool)> mark_line0;
-  // the same around each launch of the fused coarse end
-  std::function<void(bool)> mark_tail;
+  // the same around each launch of the fused coarse end, and of the CG
+  // update pass
+  std::function<void(bool)> mark_tail, mark_update;
 };
 
 // Builds the hierarchy for the case (returns with ok == false and a reason

@@ -607,20 +607,26 @@ void mg_iteration(MGPlan &P, hipStream_t st, MGRun &R) {
     mg_dir_apply_dot<float, float>(st, L0, P.h64.lam, P.ng, P.gsize, P.rz[R.cur], rzo, rzo ? P.rr : nullptr, P.zf,
                                    P.pf[R.pc], P.pf[1 - R.pc], P.part, P.nchunk, P.pq, P.done);
     R.pc = 1 - R.pc;
+    if (P.mark_update) P.mark_update(true);
     mg_update_apply_dot<float>(st, L0, P.h64.lam, P.ng, P.gsize, P.rz[R.cur], P.pq, P.x, P.pf[R.pc], P.r, P.rf, P.zf,
                                P.part, P.nchunk, P.rr, P.done);
+    if (P.mark_update) P.mark_update(false);
   } else if (mixed) {
     mg_dir_apply_dot<float, double>(st, L0, P.h64.lam, P.ng, P.gsize, P.rz[R.cur], rzo, rzo ? P.rr : nullptr, P.zf,
                                     P.pd[R.pc], P.pd[1 - R.pc], P.part, P.nchunk, P.pq, P.done);
     R.pc = 1 - R.pc;
+    if (P.mark_update) P.mark_update(true);
     mg_update_apply_dot<double>(st, L0, P.h64.lam, P.ng, P.gsize, P.rz[R.cur], P.pq, P.x, P.pd[R.pc], P.r, P.rf,
                                 P.zf, P.part, P.nchunk, P.rr, P.done);
+    if (P.mark_update) P.mark_update(false);
   } else {
     mg_dir_apply_dot<double, double>(st, L0, P.h64.lam, P.ng, P.gsize, P.rz[R.cur], rzo, nullptr, P.z, P.pd[R.pc],
                                      P.pd[1 - R.pc], P.part, P.nchunk, P.pq, P.done);
     R.pc = 1 - R.pc;
+    if (P.mark_update) P.mark_update(true);
     mg_update_apply_dot<double>(st, L0, P.h64.lam, P.ng, P.gsize, P.rz[R.cur], P.pq, P.x, P.pd[R.pc], P.r, nullptr,
                                 nullptr, P.part, P.nchunk, P.rr, P.done);
+    if (P.mark_update) P.mark_update(false);
   }
   mg_flags(st, P.ng, P.rr, P.bb, P.tol * P.tol, P.done, P.left, P.mlist);
   if (R.net && P.fallback) launch_net_stall(st, P.ng, P.rr, P.rr_prev, P.done, P.stall_ratio * P.stall_ratio, P.stall);
