@@ -165,8 +165,15 @@ __device__ __forceinline__ Row<T> pcr(const Row<T> &r, const Row<T> &lo, const R
 // calling lane: row(j, l, d, u, r) gives row p0 + j (j < cnt; each row is
 // asked for once, and before any put), put(j, v) stores its solution
 // (cnt = 0: an idle lane, which still takes part in the shuffles)
-template <int K, int LPL, typename T, bool FR, typename ROW, typename PUT>
+// FULL: every lane of the segment holds exactly K rows (ns = LPL * K), so the
+// row guards and the interface system's size are compile-time constants
+template <int K, int LPL, typename T, bool FR, bool FULL = false, typename ROW, typename PUT>
 __device__ __forceinline__ void line_elim(int ns, int nch, int cnt, bool act, ROW &&row, PUT &&put) {
+  if constexpr (FULL) {
+    ns = LPL * K;
+    nch = LPL;
+    cnt = K;
+  }
   const int wl = threadIdx.x & 63, lane = wl & (LPL - 1), seg0 = wl - lane;
   if (ns == 1) {  // a one-row line: no interface system
     if (act && lane == 0) {
@@ -407,7 +414,7 @@ __global__ __launch_bounds__(256) void k_line_lds(MGLevT<T> L, const T *__restri
 // count the arithmetic is line_solve's, so the values are the same; with
 // fewer lanes per line (longer chunks, a shorter interface system) the
 // same line is solved with another elimination order.
-template <int K, int LPL, typename T, bool FR>
+template <int K, int LPL, typename T, bool FR, bool FULL = false>
 __global__ __launch_bounds__(256) void k_line_modes(MGLevT<T> L, const T *__restrict__ lam, T *__restrict__ x,
                                                     const T *__restrict__ b, int col_a, int nla, int zero_nb,
                                                     int pin_s, const int *__restrict__ singular,
@@ -513,12 +520,12 @@ __global__ __launch_bounds__(256) void k_line_modes(MGLevT<T> L, const T *__rest
       const T lmv = lam ? lam[m] : T(0);
       const bool pin = fl < ns;
       const int lane = (threadIdx.x & 63) & (LPL - 1);
-      const int nch = ns >= 2 ? min(LPL, ns / 2) : 1;
-      const int base = ns >= 2 ? ns / nch : 1, extra = ns >= 2 ? ns % nch : 0;
+      const int nch = FULL ? LPL : (ns >= 2 ? min(LPL, ns / 2) : 1);
+      const int base = FULL ? K : (ns >= 2 ? ns / nch : 1), extra = FULL ? 0 : (ns >= 2 ? ns % nch : 0);
       const int cnt = lane < nch ? base + (lane < extra ? 1 : 0) : 0;
       const int p0 = lane * base + min(lane, extra);
       T *row = sh + li * nsp;
-      line_elim<K, LPL, T, FR>(
+      line_elim<K, LPL, T, FR, FULL>(
           ns, nch, cnt, true,
           [&](int j, T &l, T &d, T &u, T &r) {
             const int s = p0 + j, o = lpad(s);
@@ -1242,6 +1249,32 @@ bool launch_line_modes(hipStream_t st, const MGLevT<T> &L, const T *lam, T *x, c
       hipLaunchKernelGGL((k_line_modes<KK, LP, T, false>), gm, dim3(256), shm, st, L, lam, x, b, col_a, nla, zero_nb,
                          pin_s, singular, done, xc, cna);
   };
+  // lines of exactly LP * K rows: the instance with compile-time chunks
+  auto full = [&](auto kk) {
+    constexpr int KK = decltype(kk)::value;
+    if (L.ns != LP * KK) return false;
+    if (g_fast_rcp)
+      hipLaunchKernelGGL((k_line_modes<KK, LP, T, true, true>), gm, dim3(256), shm, st, L, lam, x, b, col_a, nla,
+                         zero_nb, pin_s, singular, done, xc, cna);
+    else
+      hipLaunchKernelGGL((k_line_modes<KK, LP, T, false, true>), gm, dim3(256), shm, st, L, lam, x, b, col_a, nla,
+                         zero_nb, pin_s, singular, done, xc, cna);
+    SNAC_HIP(hipGetLastError());
+    return true;
+  };
+  if constexpr (LP == 32) {
+    if (full(std::integral_constant<int, 4>{}) || full(std::integral_constant<int, 5>{}) ||
+        full(std::integral_constant<int, 8>{}))
+      return true;
+  } else if constexpr (LP == 16) {
+    if (full(std::integral_constant<int, 8>{}) || full(std::integral_constant<int, 10>{}) ||
+        full(std::integral_constant<int, 16>{}))
+      return true;
+  } else if constexpr (LP == 8) {
+    if (full(std::integral_constant<int, 16>{}) || full(std::integral_constant<int, 20>{}) ||
+        full(std::integral_constant<int, 32>{}))
+      return true;
+  }
   if constexpr (LP >= 32) {
     if (rows <= 3) go(std::integral_constant<int, 3>{});
     else if (rows <= 5) go(std::integral_constant<int, 5>{});
