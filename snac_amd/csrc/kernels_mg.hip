@@ -272,14 +272,18 @@ __device__ __forceinline__ void line_row(const MGLevT<T> &L, T lmv, const T *__r
 // written to global memory directly (lane c's rows are contiguous)
 // xc (FFT path only): the neighbour lines get the prolongation of the
 // coarser level's xc (cna columns) added as they are read (nb_sum_pro)
-template <int K, int LPL, typename T, bool FR = false>
+// FULL: the line has exactly LPL * K rows (every lane K of them); an idle
+// segment (act false) then leaves at once, its lanes' values unread by the
+// other segments' shuffles
+template <int K, int LPL, typename T, bool FR = false, bool FULL = false>
 __device__ __forceinline__ void line_solve(const MGLevT<T> &L, T lmv, T *__restrict__ x, const T *__restrict__ b,
                                            int m, int a, int zero_nb, int pin_s, bool pin, bool act,
                                            const T *__restrict__ xc = nullptr, int cna = 0) {
+  if (FULL && !act) return;
   const int wl = threadIdx.x & 63, lane = wl & (LPL - 1);
   const long e0 = vidx(L, m, a, 0), q0 = cidx(L, m, a, 0);
-  const int cnt = act && lane < L.nch ? L.base + (lane < L.extra ? 1 : 0) : 0;
-  const int p0 = lane * L.base + min(lane, L.extra);
+  const int cnt = FULL ? K : (act && lane < L.nch ? L.base + (lane < L.extra ? 1 : 0) : 0);
+  const int p0 = FULL ? lane * K : lane * L.base + min(lane, L.extra);
   T lo[K], dg[K], up[K], rh[K];
 #pragma unroll
   for (int j = 0; j < K; j++) {
@@ -289,7 +293,7 @@ __device__ __forceinline__ void line_solve(const MGLevT<T> &L, T lmv, T *__restr
       line_row(L, lmv, x, b, m, a, s, e0 + s, q0 + s, zero_nb, pin_s, pin, xc, cna, lo[j], dg[j], up[j], rh[j]);
     }
   }
-  line_elim<K, LPL, T, FR>(
+  line_elim<K, LPL, T, FR, FULL>(
       L.ns, L.nch, cnt, act, [&](int j, T &l, T &d, T &u, T &r) { l = lo[j]; d = dg[j]; u = up[j]; r = rh[j]; },
       [&](int j, T v) { x[e0 + p0 + j] = v; });
 }
@@ -558,7 +562,7 @@ __global__ __launch_bounds__(256) void k_line_modes(MGLevT<T> L, const T *__rest
 // level loaded at the start, x of the first level stored at the end), the
 // level arrays then addressed as mode 0 (the coefficients do not depend on
 // the mode); the arithmetic is the same, so the values are.
-template <int K, int LPL, typename T, bool FR, bool LX>
+template <int K, int LPL, typename T, bool FR, bool LX, bool FULL = false>
 __global__ __launch_bounds__(1024) void k_tail(MGTailT<T> t, const T *__restrict__ lam, int pin_s,
                                                const int *__restrict__ singular, const int *__restrict__ done) {
   extern __shared__ unsigned char smem_[];
@@ -586,7 +590,7 @@ __global__ __launch_bounds__(1024) void k_tail(MGTailT<T> t, const T *__restrict
       const int nlc = (L.na - c + 1) / 2;  // lines a = 2 j + c of this colour
       for (int j0 = SPW * wv; j0 < nlc; j0 += SPW * nw) {
         const int j = j0 + hs;
-        line_solve<K, LPL, T, FR>(L, lmv, X(l), B(l), mi, 2 * j + c, c == 0, -1, false, j < nlc);
+        line_solve<K, LPL, T, FR, FULL>(L, lmv, X(l), B(l), mi, 2 * j + c, c == 0, -1, false, j < nlc);
       }
       __syncthreads();
     }
@@ -607,7 +611,7 @@ __global__ __launch_bounds__(1024) void k_tail(MGTailT<T> t, const T *__restrict
   // coarsest: one line, solved exactly
   {
     const int l = t.nl - 1;
-    if (wv == 0) line_solve<K, LPL, T, FR>(t.lev[l], lmv, X(l), B(l), mi, 0, 1, pin_s, pin, hs == 0);
+    if (wv == 0) line_solve<K, LPL, T, FR, FULL>(t.lev[l], lmv, X(l), B(l), mi, 0, 1, pin_s, pin, hs == 0);
     __syncthreads();
   }
   // up: post-smooth (colour 1, then colour 0); the colour-1 sweep reads its
@@ -620,7 +624,7 @@ __global__ __launch_bounds__(1024) void k_tail(MGTailT<T> t, const T *__restrict
       const int nlc = (L.na - c + 1) / 2;
       for (int j0 = SPW * wv; j0 < nlc; j0 += SPW * nw) {
         const int j = j0 + hs;
-        line_solve<K, LPL, T, FR>(L, lmv, X(l), B(l), mi, 2 * j + c, 0, -1, false, j < nlc,
+        line_solve<K, LPL, T, FR, FULL>(L, lmv, X(l), B(l), mi, 2 * j + c, 0, -1, false, j < nlc,
                                   c == 1 ? X(l + 1) : nullptr, Cl.na);
       }
       __syncthreads();
@@ -1395,7 +1399,26 @@ void mg_tail_k(hipStream_t st, const MGTailT<T> &t, int nm, const T *lam, int pi
       hipLaunchKernelGGL((k_tail<KK, LPL, T, false, false>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular,   \
                          done);                                                                                     \
   } while (0)
-  if (rows <= 3) LAUNCH_TAIL(3);
+  // lines of exactly LPL * K rows (IEEE divisions): the instance with
+  // compile-time chunks
+  bool done_full = false;
+  if constexpr (LPL == 32) {
+    auto full = [&](auto kk) {
+      constexpr int KK = decltype(kk)::value;
+      if (g_fast_rcp || L.ns != LPL * KK) return false;
+      if (lx)
+        hipLaunchKernelGGL((k_tail<KK, LPL, T, false, true, true>), dim3(nm), dim3(1024), shm, st, t, lam, pin_s,
+                           singular, done);
+      else
+        hipLaunchKernelGGL((k_tail<KK, LPL, T, false, false, true>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s,
+                           singular, done);
+      return true;
+    };
+    done_full = full(std::integral_constant<int, 4>{}) || full(std::integral_constant<int, 5>{}) ||
+                full(std::integral_constant<int, 8>{});
+  }
+  if (done_full) {
+  } else if (rows <= 3) LAUNCH_TAIL(3);
   else if (rows <= 5) LAUNCH_TAIL(5);
   else if (rows <= 9) LAUNCH_TAIL(9);
   else LAUNCH_TAIL(17);
