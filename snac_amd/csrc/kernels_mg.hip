@@ -1052,8 +1052,10 @@ __global__ __launch_bounds__(256) void k_update_apply_dot(MGLev L, const double 
         const bool ok = i + 256 * u < iend;
         q[u] = ok ? mx_f(L, lam ? lam[m] : 0.0, pv, m, a, s, idx, cidx(L, m, a, s)) : 0.0;
         pc[u] = ok ? (double)p[idx] : 0.0;
-        xv[u] = ok ? x[idx] : 0.0;
-        rv[u] = ok ? r[idx] : 0.0;
+        // x and r are read and written once here and not again until after
+        // the next V-cycle: streamed past the caches
+        xv[u] = ok ? __builtin_nontemporal_load(&x[idx]) : 0.0;
+        rv[u] = ok ? __builtin_nontemporal_load(&r[idx]) : 0.0;
         zv[u] = ok && zf ? (double)zf[idx] : 0.0;
         pos_step(L, da, ds, m, a, s);
       }
@@ -1061,9 +1063,9 @@ __global__ __launch_bounds__(256) void k_update_apply_dot(MGLev L, const double 
       for (int u = 0; u < CG_U; u++) {
         if (i + 256 * u >= iend) break;
         const long idx = base + i + 256 * u;
-        x[idx] = xv[u] + al * pc[u];
+        __builtin_nontemporal_store(xv[u] + al * pc[u], &x[idx]);
         const double rn = rv[u] - al * q[u];
-        r[idx] = rn;
+        __builtin_nontemporal_store(rn, &r[idx]);
         s1 += rn * rn;
         if (rf) rf[idx] = (float)rn;
         if (zf) s2 += zv[u] * rn;
@@ -1113,18 +1115,18 @@ __global__ __launch_bounds__(256) void k_cg_dir(int ng, long gsize, const double
   p[idx] = (double)z[idx] + be * p[idx];
 }
 
+// over the singular groups only (glist, one per blockIdx.y): the grid covers
+// their gsize elements each, not the whole vector
 template <typename T>
 __global__ __launch_bounds__(256) void k_sub_mean(MGLev L, T *__restrict__ x, const double *__restrict__ sums,
-                                                  const int *__restrict__ singular, const double *__restrict__ nact,
-                                                  const int *__restrict__ done) {
-  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
-  const long n = (long)L.nm * L.na * L.ns;
-  if (idx >= n) return;
-  const int s = (int)(idx % L.ns);
-  const long t = idx / L.ns;
-  const int a = (int)(t % L.na), m = (int)(t / L.na);
-  const int g = group_of(L, m);
-  if (!singular[g] || (done && done[g])) return;
+                                                  const int *__restrict__ glist, long gsize,
+                                                  const double *__restrict__ nact, const int *__restrict__ done) {
+  const int g = glist[blockIdx.y];
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= gsize || (done && done[g])) return;
+  const long idx = (long)g * gsize + i;
+  int m, a, s;
+  pos_of(L, idx, m, a, s);
   if (!L.act[cidx(L, m, a, s)]) return;
   x[idx] -= (T)(sums[g * 2 + 1] / nact[g]);
 }
@@ -1552,10 +1554,11 @@ void mg_cg_dir(hipStream_t st, int ng, long gsize, const double *rzn, const doub
 }
 
 template <typename T>
-void mg_sub_mean(hipStream_t st, const MGLev &L, T *x, const double *sums, const int *singular, const double *nact,
-                 const int *done) {
-  const long n = (long)L.nm * L.na * L.ns;
-  hipLaunchKernelGGL(k_sub_mean<T>, dim3(blocks_for(n)), dim3(256), 0, st, L, x, sums, singular, nact, done);
+void mg_sub_mean(hipStream_t st, const MGLev &L, T *x, const double *sums, const int *glist, int nlist, long gsize,
+                 const double *nact, const int *done) {
+  if (nlist <= 0) return;
+  hipLaunchKernelGGL(k_sub_mean<T>, dim3((unsigned)((gsize + 255) / 256), nlist), dim3(256), 0, st, L, x, sums, glist,
+                     gsize, nact, done);
   SNAC_HIP(hipGetLastError());
 }
 
@@ -1594,8 +1597,8 @@ void mg_copy3(hipStream_t st, const double *src, long sm, long sa, long ss, doub
                              int, double *, const int *, const int *);                                                \
   template void mg_cg_dir<T>(hipStream_t, int, long, const double *, const double *, const double *, const T *,       \
                              double *, const int *);                                                                  \
-  template void mg_sub_mean<T>(hipStream_t, const MGLev &, T *, const double *, const int *, const double *,          \
-                               const int *);
+  template void mg_sub_mean<T>(hipStream_t, const MGLev &, T *, const double *, const int *, int, long,             \
+                               const double *, const int *);
 SNAC_MG_INST(double)
 SNAC_MG_INST(float)
 #undef SNAC_MG_INST

@@ -175,7 +175,8 @@ void mg_cg_dir(hipStream_t st, int ng, long gsize, const double *rzn, const doub
                const T *z, double *p, const int *done);
 // x -= (sum / nact) on the active cells of the singular groups
 template <typename T>
-void mg_sub_mean(hipStream_t st, const MGLev &L, T *x, const double *sums, const int *singular, const double *nact,
+void mg_sub_mean(hipStream_t st, const MGLev &L, T *x, const double *sums, const int *glist, int nlist, long gsize,
+                 const double *nact,
                  const int *done);
 // done[g] |= rr[g] <= tol2 * bb[g]; *left = number of groups not done
 // done[g] |= r.r <= tol2 b.b; left = the groups not done; list (when given):
@@ -233,6 +234,8 @@ struct MGPlan {
   double *nact = nullptr;
   int *done = nullptr, *singular = nullptr, *left = nullptr, *left_host = nullptr;
   int *mlist = nullptr;  // FFT path: the groups not done, in order (left of them)
+  int *sing_list = nullptr;  // the singular groups (n_sing of them)
+  int n_sing = 0;
   int nchunk = 1;
   int pin_s = -1;              // coarsest-level pinned row of singular groups
   bool any_singular = false;

@@ -482,6 +482,13 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
   P.nact = dev_upload(P, nact);
   P.done = dev_zeros<int>(P, P.ng);
   P.singular = dev_upload(P, sing);
+  {
+    std::vector<int> sl;
+    for (int g = 0; g < P.ng; g++)
+      if (sing[g]) sl.push_back(g);
+    P.n_sing = (int)sl.size();
+    P.sing_list = dev_upload(P, sl);
+  }
   // every group iterating until the first mg_flags of a solve says otherwise
   P.left = dev_upload(P, std::vector<int>{P.ng});
   if (!coupled) {
@@ -553,7 +560,7 @@ template <typename T>
 void project(MGPlan &P, hipStream_t st, T *v) {
   if (!P.any_singular) return;
   mg_dot<T, T>(st, P.ng, P.gsize, nullptr, nullptr, v, nullptr, P.part, P.nchunk, P.sums, P.done, P.singular);
-  mg_sub_mean(st, P.h64.lev[0], v, P.sums, P.singular, P.nact, P.done);
+  mg_sub_mean(st, P.h64.lev[0], v, P.sums, P.sing_list, P.n_sing, P.gsize, P.nact, P.done);
 }
 
 // z = V-cycle(r) (or the network's answer, net) with its null-space
