@@ -944,6 +944,37 @@ __device__ __forceinline__ double mx_f(const MGLev &L, double lm, F &&f, int m, 
 // elements per round of a thread in the fused CG kernels
 constexpr int CG_U = 4;
 
+// the operator's coefficients at one row s of a level whose coefficients do
+// not depend on the position a (auni): read once by a thread whose elements
+// all lie on row s
+struct MxRow {
+  double dg, w, csm, csp, ca;
+};
+__device__ __forceinline__ MxRow mx_row(const MGLev &L, int s) {
+  MxRow c;
+  c.dg = L.dg[s];
+  c.w = L.w[s];
+  c.csm = s > 0 ? L.cs[s - 1] : 0.0;
+  c.csp = s < L.ns - 1 ? L.cs[s] : 0.0;
+  c.ca = L.ca[s];
+  return c;
+}
+// mx_f on such a level from the row's coefficients: the same values in the
+// same order (ca at a and at a - 1 are the same number there)
+template <typename F>
+__device__ __forceinline__ double mx_r(const MGLev &L, double lm, F &&f, int a, int s, long e, const MxRow &c) {
+  double y = (c.dg - lm * c.w) * f(e);
+  if (s > 0) y -= c.csm * f(e - 1);
+  if (s < L.ns - 1) y -= c.csp * f(e + 1);
+  double r = 0;
+  const long sa = L.ns;
+  if (a < L.na - 1) r += c.ca * f(e + sa);
+  else if (L.per_a) r += c.ca * f(e - (long)(L.na - 1) * sa);
+  if (a > 0) r += c.ca * f(e - sa);
+  else if (L.per_a) r += c.ca * f(e + (long)(L.na - 1) * sa);
+  return y - r;
+}
+
 __device__ __forceinline__ double cg_beta(int g, const double *__restrict__ rzn, const double *__restrict__ rzo,
                                           const double *__restrict__ cross) {
   if (!rzo) return 0.0;
@@ -981,6 +1012,10 @@ __global__ __launch_bounds__(256) void k_dir_apply_dot(MGLev L, const double *__
     int m, a, s;
     pos_of(L, base + i0, m, a, s);
     const int da = 256 / L.ns, ds = 256 % L.ns;
+    // a thread's elements keep their row s when 256 is a multiple of ns: on
+    // a level with position-independent coefficients they are read once
+    const bool rowc = L.auni && ds == 0;
+    const MxRow c = rowc ? mx_row(L, s) : MxRow{};
     // CG_U elements per round, their loads issued before any store so that
     // their latencies overlap; the sum keeps the element order
     for (long i = i0; i < iend; i += 256 * CG_U) {
@@ -989,8 +1024,9 @@ __global__ __launch_bounds__(256) void k_dir_apply_dot(MGLev L, const double *__
       for (int u = 0; u < CG_U; u++) {
         const long idx = base + i + 256 * u;
         const bool ok = i + 256 * u < iend;
+        const double lm = lam ? lam[m] : 0.0;
         pc[u] = ok ? pn(idx) : 0.0;
-        y[u] = ok ? mx_f(L, lam ? lam[m] : 0.0, pn, m, a, s, idx, cidx(L, m, a, s)) : 0.0;
+        y[u] = !ok ? 0.0 : rowc ? mx_r(L, lm, pn, a, s, idx, c) : mx_f(L, lm, pn, m, a, s, idx, cidx(L, m, a, s));
         pos_step(L, da, ds, m, a, s);
       }
 #pragma unroll
@@ -1042,6 +1078,10 @@ __global__ __launch_bounds__(256) void k_update_apply_dot(MGLev L, const double 
     int m, a, s;
     pos_of(L, base + i0, m, a, s);
     const int da = 256 / L.ns, ds = 256 % L.ns;
+    // position-independent coefficients on a fixed row: read once (as in
+    // k_dir_apply_dot, so the same M p)
+    const bool rowc = L.auni && ds == 0;
+    const MxRow c = rowc ? mx_row(L, s) : MxRow{};
     // CG_U elements per round, their loads issued before any store so that
     // their latencies overlap; the sums keep the element order
     for (long i = i0; i < iend; i += 256 * CG_U) {
@@ -1050,7 +1090,8 @@ __global__ __launch_bounds__(256) void k_update_apply_dot(MGLev L, const double 
       for (int u = 0; u < CG_U; u++) {
         const long idx = base + i + 256 * u;
         const bool ok = i + 256 * u < iend;
-        q[u] = ok ? mx_f(L, lam ? lam[m] : 0.0, pv, m, a, s, idx, cidx(L, m, a, s)) : 0.0;
+        const double lm = lam ? lam[m] : 0.0;
+        q[u] = !ok ? 0.0 : rowc ? mx_r(L, lm, pv, a, s, idx, c) : mx_f(L, lm, pv, m, a, s, idx, cidx(L, m, a, s));
         pc[u] = ok ? (double)p[idx] : 0.0;
         // x and r are read and written once here and not again until after
         // the next V-cycle: streamed past the caches

@@ -45,6 +45,9 @@ struct MGLevT {
   int nch, base, extra; // a line of ns rows over nch lanes: base (+1 for the first `extra`) rows each
   int lpl;              // lanes per line segment (32 or 64; nch <= lpl)
   int lvar;             // line relaxation kernel of this level (0..4, chosen by timing at set-up)
+  // FFT path: dg, w, cs, ca and act are the same at every position a
+  // (uniform, periodic in a), so a pass can read them once per row s
+  int auni;
   // FFT path: the modes still iterating, in increasing order (mlist[0 ..
   // *mcount - 1]), kept by mg_flags; the mode-batched kernels take their
   // modes from it so that converged modes cost no workgroup slots (null on

@@ -405,6 +405,17 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
     L.extra = ns >= 2 ? ns % nch : 0;
     L.lpl = lpl;
     L.lvar = -1;
+    L.auni = 0;
+    if (!coupled) {
+      bool au = true;
+      const size_t nsz = (size_t)h.ns;
+      for (size_t q = nsz; au && q < (size_t)h.na * nsz; q++) {
+        const size_t q0 = q % nsz;
+        au = h.dg[q] == h.dg[q0] && h.w[q] == h.w[q0] && h.cs[q] == h.cs[q0] && h.ca[q] == h.ca[q0] &&
+             h.act[q] == h.act[q0];
+      }
+      L.auni = au ? 1 : 0;
+    }
     L.mlist = nullptr;
     L.mcount = nullptr;
     P.h64.lev.push_back(L);
@@ -417,7 +428,7 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
     Lf.ca = dev_upload(P, f32(h.ca));
     Lf.cm = coupled ? dev_upload(P, f32(h.cm)) : nullptr;
     Lf.act = L.act;
-    Lf.nch = L.nch; Lf.base = L.base; Lf.extra = L.extra; Lf.lpl = L.lpl; Lf.lvar = -1;
+    Lf.nch = L.nch; Lf.base = L.base; Lf.extra = L.extra; Lf.lpl = L.lpl; Lf.lvar = -1; Lf.auni = L.auni;
     Lf.mlist = nullptr; Lf.mcount = nullptr;
     P.h32.lev.push_back(Lf);
     const size_t n = (size_t)h.nm * h.na * h.ns;
