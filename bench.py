@@ -28,7 +28,10 @@ Forms, each timed on its own context in the same run (--forms):
           tridiagonal lines along y) -- the same discrete systems, not the
           plain form's method;
   direct_mixed, direct_impdiff: the direct solver with the FP32 momentum
-          right-hand side / with implicit diffusion.
+          right-hand side / with implicit diffusion;
+  impdiff, mixed_impdiff: the plain / mixed form with implicit diffusion,
+          each field's Helmholtz problem solved by its own MG-PCG plan
+          (eager launches: the host checks those CG loops).
 The line's value is the mixed (or learned) form's Mcell-steps/s; the plain
 form's figures and the ratio sit beside it, with every form's divergence.
 
@@ -43,7 +46,8 @@ on a bounded sample of the same channel.
 
 --workload bfs runs the three-block backward-facing step (inlet 192x64x128
 with a Poiseuille inflow, lower 320x64x128, upper 320x96x128, z periodic
-with 128 modes, outflow at x+): forms plain / mixed / learned only.
+with 128 modes, outflow at x+): the MG forms only (plain, mixed, learned,
+impdiff, mixed_impdiff).
 """
 import argparse
 import json
@@ -74,6 +78,8 @@ FORMS = {
     "direct": ("tensor", "fp64", False, "f64"),
     "direct_mixed": ("tensor", "mixed", False, "f64+f32"),
     "direct_impdiff": ("tensor", "fp64", True, "f64"),
+    "impdiff": ("mg", "fp64", True, "f64"),
+    "mixed_impdiff": ("mg", "mixed", True, "f64+f32"),
 }
 
 
@@ -271,6 +277,8 @@ def run_form(name, d, b, args, cells, dt_ref):
             out.update({"hypre_tol": args.tol, "cg_iterations_last_solve": cg_iters,
                         "cg_max_relres_last_solve": cg_relres, "solves_timed": solves,
                         "solves_resumed_after_graph": continued, "solves_at_maxiter": unconverged})
+        if solver == "mg" and impdiff:
+            out["helmholtz_cg_iterations_last_solve"] = {n: ctx.impdiff_stats(n)[0] for n in "uvw"}
         if learned is not None:
             learned["fallback_solves"] = ctx.learned_stats()[1]
             out["learned"] = learned

@@ -124,6 +124,10 @@ module mod_snac_amd
     integer(c_int) function snac_solve_impdiff(ctx, fid, alpha) bind(C) ! solve_helmholtz.f90:30-136
       import; type(c_ptr), value :: ctx; integer(c_int), value :: fid; real(c_double), value :: alpha
     end function
+    integer(c_int) function snac_impdiff_stats(ctx, fid, iters, relres) bind(C) ! HYPRE counts of a Helmholtz solve
+      import; type(c_ptr), value :: ctx; integer(c_int), value :: fid; integer(c_int) :: iters
+      real(c_double) :: relres
+    end function
     integer(c_int) function snac_updt_pressure_impdiff(ctx, alpha) bind(C) ! updt_pressure.f90 (_IMPDIFF)
       import; type(c_ptr), value :: ctx; real(c_double), value :: alpha
     end function

@@ -263,6 +263,12 @@ int snac_set_impdiff(snac_ctx *ctx, int enable);
  * place over the component's unknowns (the bounded upper face along its own
  * axis excluded)                                                            */
 int snac_solve_impdiff(snac_ctx *ctx, int fid, double alpha);
+/* CG iterations and largest relative residual over the modes of the last
+ * MG-PCG Helmholtz solve of field fid (as snac_solver_stats for the
+ * pressure: HYPRE's StructPCGGetNumIterations / GetFinalRelativeResidualNorm
+ * after the Helmholtz solve of solve_helmholtz.f90:83-99); synchronises.
+ * Both are 0 when that field was solved on the tensor-product plan.         */
+int snac_impdiff_stats(snac_ctx *ctx, int fid, int *iters, double *max_relres);
 /* updt_pressure updt_pressure.f90:7-47 of the _IMPDIFF build:
  * p += pp + alpha * Laplacian(pp), alpha = -visc*dtrk/2                     */
 int snac_updt_pressure_impdiff(snac_ctx *ctx, double alpha);

@@ -1440,6 +1440,14 @@ static int assemble_op(oc_case *c, oop *o, char *err, int errlen) {
               else j2[e] += s == 0 ? -1 : 1;
               int tn[3] = {1, 1, 1};
               for (int oo = 0; oo < nd; oo++) tn[oo] = o->ext[tb][o->ax[oo]];
+              /* the struct grid of solver.f90:221-224 is the union of the
+               * blocks' extents: a link into a neighbour's excluded bounded
+               * staggered face (e.g. v at the top of a lower block, whose
+               * x-neighbour is the wall face of the block beside it) leaves
+               * the grid and meets a zero value, so it drops out            */
+              int outside = 0;
+              for (int oo = 0; oo < nd; oo++) if (j2[oo] > tn[oo]) outside = 1;
+              if (outside) { o->coef[slot] = 0.0; continue; }
               o->nbr[slot] = o->off[tb] + (j2[0] - 1) + (long)tn[0] * ((j2[1] - 1) + (long)tn[1] * (j2[2] - 1));
             }
           }

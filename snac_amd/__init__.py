@@ -79,6 +79,7 @@ def _load():
     L.snac_updt_pressure.argtypes = [vp]
     L.snac_set_impdiff.argtypes = [vp, C.c_int]
     L.snac_solve_impdiff.argtypes = [vp, C.c_int, C.c_double]
+    L.snac_impdiff_stats.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.snac_updt_pressure_impdiff.argtypes = [vp, C.c_double]
     L.snac_chkdiv.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.snac_chkdt.argtypes = [vp, C.POINTER(C.c_double)]
@@ -258,6 +259,15 @@ class Context:
     def solve_impdiff(self, which, alpha):
         """solve_impdiff_field of 'u', 'v' or 'w' with alpha = -visc*dtrk/2."""
         _check(lib.snac_solve_impdiff(self.h, FIELDS[which], float(alpha)), "solve_impdiff")
+
+    def impdiff_stats(self, which):
+        """(CG iterations, largest relative residual over the modes) of the
+        last MG-PCG Helmholtz solve of 'u', 'v', 'w' or a scalar; (0, 0.0)
+        when the field was solved on the tensor-product plan."""
+        it = C.c_int()
+        rel = C.c_double()
+        _check(lib.snac_impdiff_stats(self.h, FIELDS[which], C.byref(it), C.byref(rel)), "impdiff_stats")
+        return it.value, rel.value
 
     def updt_pressure_impdiff(self, alpha):
         _check(lib.snac_updt_pressure_impdiff(self.h, float(alpha)), "updt_pressure_impdiff")

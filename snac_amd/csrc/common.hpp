@@ -144,6 +144,8 @@ struct ImpRhs {
 // out (ghosted, same layout as phi) = phi*alphai*w0*w1*w2 + BC planes over
 // the unknowns, 0 past them
 void launch_imp_rhs(hipStream_t s, const Geo &g, const ImpRhs &r, const double *phi, double *out, double alphai);
+// dst = src over the unknowns (indices 1..ext of each axis) of a block
+void launch_copy_unknowns(hipStream_t s, const Geo &g, const int ext[3], const double *src, double *dst);
 // updt_pressure of the _IMPDIFF build: p += pp + alpha * Laplacian(pp)
 void launch_updt_pressure_imp(hipStream_t s, const Geo &g, const Metrics &m, const double *pp, double *p,
                               double alpha);

@@ -80,8 +80,10 @@ struct TensorPlan {
 // Implicit diffusion of one velocity component: its tensor operator and,
 // per block, the right-hand-side description (weights, BC planes)
 struct ImpField {
-  TensorOps ops;
+  TensorOps ops;              // the tensor-product path
   std::vector<ImpRhs> rhs;
+  std::shared_ptr<MGPlan> mg;  // the MG-PCG path: the field's own plan, built on first need
+  bool mg_ok = false;          // the field's operator assembles on the MG-PCG path
 };
 
 }  // namespace
@@ -468,16 +470,34 @@ void plan_tensor(snac_ctx *c) {
 
 // batched MG-PCG plan with the FFT-axis transforms and the bounding-box
 // work array of the FFT path; planned once, on first need
+void finish_mg_plan(snac_ctx *c, MGPlan &P);
+
 void plan_mg(snac_ctx *c) {
   if (c->mg_planned) return;
   c->mg_planned = true;
   MGPlan &P = c->mg;
   mg_plan(P, c->cs, c->device);
   P.prec = c->prec;
-  if (P.ok && c->device >= 0) mg_tune_lines(P, c->stream);  // each level's fastest line kernel
+  finish_mg_plan(c, P);
+}
+
+// after mg_plan: each level's fastest line kernel, the FFT-axis transforms
+// and the bounding-box work array (a Helmholtz plan shares the pressure
+// plan's: the solves run one after the other on the context's stream)
+void finish_mg_plan(snac_ctx *c, MGPlan &P) {
+  if (P.ok && c->device >= 0) mg_tune_lines(P, c->stream);
   if (!P.ok || c->device < 0 || P.fa < 0) return;
-  P.fwd = r2r_plan(P.kind_fwd, P.G[P.fa]);
-  P.bwd = r2r_plan(P.kind_bwd, P.G[P.fa]);
+  // ng modes along the FFT axis (its n-1 first planes for a component
+  // staggered along a bounded FFT axis)
+  P.fwd = r2r_plan(P.kind_fwd, P.ng);
+  P.bwd = r2r_plan(P.kind_bwd, P.ng);
+  if (P.comp >= 0) {
+    plan_mg(c);
+    if (c->mg.ok && c->mg.ph) {
+      P.ph = c->mg.ph;
+      return;
+    }
+  }
   P.ph = dalloc<double>(c, (size_t)P.G[0] * P.G[1] * P.G[2]);
 }
 
@@ -606,17 +626,23 @@ void tensor_solve(snac_ctx *c, const TensorOps &o, int comp, double shift) {
 void plan_impdiff(snac_ctx *c) {
   TensorPlan &t = c->tp;
   const Case &cs = c->cs;
-  if (!t.ok)
-    throw std::runtime_error("implicit diffusion needs the tensor-product solver on this case (" + t.why + ")");
   const int fa = cs.fft_axis - 1;
   c->imp.assign(3 + cs.nscal, ImpField());
   for (int comp = 0; comp < 3 + cs.nscal; comp++) {
     ImpField &f = c->imp[comp];
+    const std::string name = comp < 3 ? std::string(1, "uvw"[comp]) : "scalar " + std::to_string(comp - 2);
     std::string why;
-    if (!build_ops(c, f.ops, comp, why))
-      throw std::runtime_error("implicit diffusion of " +
-                               (comp < 3 ? std::string(1, "uvw"[comp]) : "scalar " + std::to_string(comp - 2)) +
-                               ": " + why);
+    if (t.ok) {
+      if (!build_ops(c, f.ops, comp, why)) throw std::runtime_error("implicit diffusion of " + name + ": " + why);
+    }
+    {
+      // the MG-PCG path: whether the field's operator assembles (host-side
+      // check; required when there is no tensor-product plan)
+      MGPlan probe;
+      mg_plan(probe, cs, -1, comp);
+      f.mg_ok = probe.ok;
+      if (!probe.ok && !t.ok) throw std::runtime_error("implicit diffusion of " + name + ": " + probe.why);
+    }
     // the tuned transform tiles of the pressure's plans where the kernels match
     for (size_t q = 0; q < f.ops.fwd.size() && q < t.P.fwd.size(); q++) {
       if (f.ops.fwd[q].kind == t.P.fwd[q].kind && f.ops.fwd[q].n == t.P.fwd[q].n) f.ops.fwd[q].lshift = t.P.fwd[q].lshift;
@@ -680,9 +706,16 @@ void plan_impdiff(snac_ctx *c) {
       if (!D.sdd) D.sdd = dalloc<double>(c, D.nin);
 }
 
+void mg_stage_in(snac_ctx *c, MGPlan &P);
+void mg_stage_out(snac_ctx *c, MGPlan &P);
+void finish_mg_plan(snac_ctx *c, MGPlan &P);
+
 // solve_impdiff_field (solve_helmholtz.f90:30-136) of velocity component
 // comp (0..2) or scalar comp-3, in place on its current buffer: the
-// right-hand side goes to pp, the solve back into the field's unknowns
+// right-hand side goes to pp, the solve back into the field's unknowns.
+// With the pressure on the MG-PCG path the field's Helmholtz problem is
+// solved there too, by its own plan: the per-mode eigenvalues shifted by
+// 1/alpha (the weighted diagonal term of add_weighted_constant_to_diagonal)
 void solve_impdiff(snac_ctx *c, int comp, double alpha) {
   const double alphai = 1.0 / alpha;
   ImpField &f = c->imp[comp];
@@ -693,13 +726,36 @@ void solve_impdiff(snac_ctx *c, int comp, double alpha) {
       launch_imp_rhs(c->stream, D.g, f.rhs[b], imp_field(c, D, comp), D.pp, alphai);
     }
   }
-  tensor_solve(c, f.ops, comp, alphai);
+  if (!use_mg(c) || (!f.mg_ok && c->tp.ok)) {  // the tensor-product path where the field allows it
+    tensor_solve(c, f.ops, comp, alphai);
+    return;
+  }
+  if (!f.mg) {
+    auto P = std::shared_ptr<MGPlan>(new MGPlan(), [](MGPlan *q) {
+      mg_free(*q);
+      delete q;
+    });
+    mg_plan(*P, c->cs, c->device, comp);
+    if (!P->ok) throw std::runtime_error("implicit diffusion on the MG-PCG path: " + P->why);
+    finish_mg_plan(c, *P);
+    f.mg = P;
+  }
+  MGPlan &P = *f.mg;
+  P.prec = c->prec == SNAC_PREC_FP64 ? 0 : 1;  // the V-cycle (the network is the pressure's)
+  Phase tp(c, SNAC_PH_HELMHOLTZ);
+  mg_stage_in(c, P);
+  mg_set_shift(P, c->stream, alphai);
+  mg_solve(P, c->stream, 0);
+  mg_stage_out(c, P);
+  for (size_t b = 0; b < c->blk.size(); b++) {
+    DevBlock &D = c->blk[b];
+    launch_copy_unknowns(c->stream, D.g, f.rhs[b].ext, D.pp, imp_field(c, D, comp));
+  }
 }
 
 // MG path: block pp interiors -> (transform along the FFT axis) -> plan
 // layout scaled by -1 (the solver works with -A) -> MG-PCG -> back
-void mg_stage_in(snac_ctx *c) {
-  MGPlan &P = c->mg;
+void mg_stage_in(snac_ctx *c, MGPlan &P) {
   const long pst[3] = {1, (long)P.G[0], (long)P.G[0] * P.G[1]};
   const long na = P.G[P.ax_a], ns = P.G[P.ax_s];
   if (P.fa >= 0) {
@@ -714,7 +770,7 @@ void mg_stage_in(snac_ctx *c) {
       launch_r2r(c->stream, P.fwd, in, gstride(D, a), gstride(D, b1), gstride(D, b2), P.ph + off, pst[a], pst[b1],
                  pst[b2], bext(D, b1), bext(D, b2), 1.0);
     }
-    mg_copy3(c->stream, P.ph, pst[P.ax_m], pst[P.ax_a], pst[P.ax_s], P.r, na * ns, ns, 1, P.G[P.ax_m], (int)na,
+    mg_copy3(c->stream, P.ph, pst[P.ax_m], pst[P.ax_a], pst[P.ax_s], P.r, na * ns, ns, 1, P.ng, (int)na,
              (int)ns, -1.0, pst[P.ax_a] == 1 ? 1 : 0);
   } else {
     for (size_t b = 0; b < c->blk.size(); b++) {
@@ -729,12 +785,11 @@ void mg_stage_in(snac_ctx *c) {
   }
 }
 
-void mg_stage_out(snac_ctx *c) {
-  MGPlan &P = c->mg;
+void mg_stage_out(snac_ctx *c, MGPlan &P) {
   const long pst[3] = {1, (long)P.G[0], (long)P.G[0] * P.G[1]};
   const long na = P.G[P.ax_a], ns = P.G[P.ax_s];
   if (P.fa >= 0) {
-    mg_copy3(c->stream, P.x, na * ns, ns, 1, P.ph, pst[P.ax_m], pst[P.ax_a], pst[P.ax_s], P.G[P.ax_m], (int)na,
+    mg_copy3(c->stream, P.x, na * ns, ns, 1, P.ph, pst[P.ax_m], pst[P.ax_a], pst[P.ax_s], P.ng, (int)na,
              (int)ns, 1.0, pst[P.ax_a] == 1 ? 2 : 0);
     int a = P.fa, b1, b2;
     others(a, b1, b2);
@@ -778,13 +833,13 @@ void split_capture(snac_ctx *c, const MGRun &R) {
 void solve_pressure(snac_ctx *c) {
   check_solver(c);
   if (use_mg(c)) {
-    { Phase tf(c, SNAC_PH_R2R_FWD); mg_stage_in(c); }
+    { Phase tf(c, SNAC_PH_R2R_FWD); mg_stage_in(c, c->mg); }
     {
       Phase tt(c, SNAC_PH_TRIDIAG);
       if (c->capturing) split_capture(c, mg_solve(c->mg, c->stream, c->mg.fixed_iters));
       else mg_solve(c->mg, c->stream, 0);
     }
-    { Phase tb(c, SNAC_PH_R2R_BWD); mg_stage_out(c); }
+    { Phase tb(c, SNAC_PH_R2R_BWD); mg_stage_out(c, c->mg); }
     return;
   }
   tensor_solve(c, c->tp.P, -1, 0.0);
@@ -1047,7 +1102,10 @@ void run_steps(snac_ctx *c, double dt, int nsteps) {
   if (nsteps <= 0) return;
   // the learned preconditioner and the pair recording decide on the host
   // between CG iterations: eager launches
-  const bool host_cg = use_mg(c) && (c->prec == SNAC_PREC_LEARNED || c->mg.collecting);
+  // (so do implicit-diffusion solves on the MG-PCG path: their CG loops are
+  // checked on the host)
+  const bool host_cg =
+      use_mg(c) && (c->prec == SNAC_PREC_LEARNED || c->mg.collecting || c->impdiff);
   if (!c->use_graph || c->timing || host_cg) {
     for (int s = 0; s < nsteps; s++)
       for (int irk = 1; irk <= 3; irk++) substage(c, irk, dt);
@@ -1924,15 +1982,33 @@ int snac_set_impdiff(snac_ctx *c, int enable) {
   });
 }
 
+// the implicit-diffusion field of a field id: 0..2 = u, v, w, 3+i = scalar i
+int impdiff_comp(snac_ctx *c, int fid, const char *what) {
+  if (!c->impdiff) throw std::runtime_error("implicit diffusion is off (snac_set_impdiff)");
+  int comp = -1;
+  if (fid >= SNAC_U && fid <= SNAC_W) comp = fid - SNAC_U;
+  else if (fid >= SNAC_SCAL0 && fid < SNAC_SCAL0 + 2 * c->cs.nscal && (fid - SNAC_SCAL0) % 2 == 0)
+    comp = 3 + (fid - SNAC_SCAL0) / 2;
+  if (comp < 0) throw std::runtime_error(std::string(what) + ": field must be u, v, w or a scalar");
+  return comp;
+}
+
+int snac_impdiff_stats(snac_ctx *c, int fid, int *iters, double *max_relres) {
+  return guard([&] {
+    on_device(c);
+    const ImpField &f = c->imp[impdiff_comp(c, fid, "impdiff_stats")];
+    int it = 0;
+    double rel = 0.0;
+    if (f.mg && f.mg->ok) mg_stats(*f.mg, c->stream, it, rel);
+    if (iters) *iters = it;
+    if (max_relres) *max_relres = rel;
+  });
+}
+
 int snac_solve_impdiff(snac_ctx *c, int fid, double alpha) {
   return guard([&] {
     on_device(c);
-    if (!c->impdiff) throw std::runtime_error("implicit diffusion is off (snac_set_impdiff)");
-    int comp = -1;
-    if (fid >= SNAC_U && fid <= SNAC_W) comp = fid - SNAC_U;
-    else if (fid >= SNAC_SCAL0 && fid < SNAC_SCAL0 + 2 * c->cs.nscal && (fid - SNAC_SCAL0) % 2 == 0)
-      comp = 3 + (fid - SNAC_SCAL0) / 2;
-    if (comp < 0) throw std::runtime_error("solve_impdiff: field must be u, v, w or a scalar");
+    const int comp = impdiff_comp(c, fid, "solve_impdiff");
     if (!(alpha < 0.0)) throw std::runtime_error("solve_impdiff: alpha must be negative (-visc*dtrk/2)");
     solve_impdiff(c, comp, alpha);
     std::fill(c->tcnt, c->tcnt + SNAC_NPHASE, 0);

@@ -56,12 +56,28 @@ __global__ __launch_bounds__(TX *TY) void k_updt_pressure_imp(Geo g, Metrics m, 
   p[c] = pv;
 }
 
+// dst = src over the unknowns of a field (1..ext per axis)
+__global__ __launch_bounds__(TX *TY) void k_copy_unknowns(Geo g, int e0, int e1, int e2, const double *__restrict__ src,
+                                                          double *__restrict__ dst) {
+  const int i = blockIdx.x * TX + threadIdx.x + 1;
+  const int j = blockIdx.y * TY + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  if (i > e0 || j > e1 || k > e2) return;
+  const long c = i + g.sx * (j + g.sy * (long)k);
+  dst[c] = src[c];
+}
+
 dim3 interior_grid(const Geo &g) { return dim3((g.nx + TX - 1) / TX, (g.ny + TY - 1) / TY, g.nz); }
 
 }  // namespace
 
 void launch_imp_rhs(hipStream_t s, const Geo &g, const ImpRhs &r, const double *phi, double *out, double alphai) {
   hipLaunchKernelGGL(k_imp_rhs, interior_grid(g), dim3(TX, TY), 0, s, g, r, phi, out, alphai);
+  SNAC_HIP(hipGetLastError());
+}
+
+void launch_copy_unknowns(hipStream_t s, const Geo &g, const int ext[3], const double *src, double *dst) {
+  hipLaunchKernelGGL(k_copy_unknowns, interior_grid(g), dim3(TX, TY), 0, s, g, ext[0], ext[1], ext[2], src, dst);
   SNAC_HIP(hipGetLastError());
 }
 

@@ -785,6 +785,16 @@ __global__ __launch_bounds__(256) void k_prolong(MGLevT<T> F, MGLevT<T> Cl, cons
   xf[idx] += xc[vidx(Cl, M, I, s)];
 }
 
+// lam = base + shift in FP64 and FP32 (the Helmholtz plans' per-solve shift)
+__global__ __launch_bounds__(256) void k_shift_lam(int n, const double *__restrict__ base, double shift,
+                                                   double *__restrict__ l64, float *__restrict__ l32) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double v = base[i] + shift;
+  l64[i] = v;
+  l32[i] = (float)v;
+}
+
 // a deterministic pattern in [-1, 1) (an integer hash of the index)
 template <typename T>
 __global__ __launch_bounds__(256) void k_pattern(long n, T *__restrict__ x) {
@@ -1566,6 +1576,13 @@ template void mg_update_apply_dot<double>(hipStream_t, const MGLev &, const doub
 template void mg_update_apply_dot<float>(hipStream_t, const MGLev &, const double *, int, long, const double *,
                                          const double *, double *, const float *, double *, float *, const float *,
                                          double *, int, double *, const int *);
+
+void mg_set_shift(MGPlan &P, hipStream_t st, double shift) {
+  if (P.comp < 0 || !P.lam_base) return;
+  hipLaunchKernelGGL(k_shift_lam, dim3((P.ng + 255) / 256), dim3(256), 0, st, P.ng, P.lam_base, shift, P.lam64,
+                     P.lam32);
+  SNAC_HIP(hipGetLastError());
+}
 
 void mg_to_float(hipStream_t st, long n, const double *src, float *dst) {
   hipLaunchKernelGGL(k_to_float, dim3(blocks_for(n)), dim3(256), 0, st, n, src, dst);

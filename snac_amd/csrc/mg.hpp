@@ -216,6 +216,13 @@ struct MGPlan {
   Net *net = nullptr;
   int nH = 0, nW = 0;             // network planes (na, ns rounded up to 8)
   double lam_ref = 1.0, nact_plane = 1.0;
+  // the field the plan solves for: -1 the pressure, 0..2 the Helmholtz
+  // problem of a velocity component, 3 + i of scalar i (implicit diffusion);
+  // a Helmholtz plan's solves read lam = lam_base + shift (mg_set_shift)
+  int comp = -1;
+  const double *lam_base = nullptr;
+  double *lam64 = nullptr;
+  float *lam32 = nullptr;
   bool fallback = true;
   bool guess = false;             // also start each solve from the network's answer for b
   double stall_ratio = 0.5;
@@ -270,8 +277,15 @@ struct MGPlan {
 };
 
 // Builds the hierarchy for the case (returns with ok == false and a reason
-// when the case cannot use it); device < 0 plans without allocating.
-void mg_plan(MGPlan &P, const Case &cs, int device);
+// when the case cannot use it); device < 0 plans without allocating.  comp
+// -1: the pressure's Poisson operator; 0..2 / 3 + i: the Helmholtz operator
+// of velocity component comp / scalar i (implicit diffusion; its unknowns on
+// the component's staggered positions, its own boundary conditions, a
+// periodic FFT axis)
+void mg_plan(MGPlan &P, const Case &cs, int device, int comp = -1);
+// Helmholtz plans: the per-mode eigenvalues the next solves use become
+// lam_base + shift (shift = 1 / alpha of the substage), in both precisions
+void mg_set_shift(MGPlan &P, hipStream_t st, double shift);
 void mg_free(MGPlan &P);
 // Host-side position of one CG solve: the iterations launched so far and
 // the parity of the rz buffers.  A solve launched with a fixed iteration

@@ -154,9 +154,25 @@ void mg_free(MGPlan &P) {
   P.stall_host = nullptr;
 }
 
-void mg_plan(MGPlan &P, const Case &cs, int device) {
+void mg_plan(MGPlan &P, const Case &cs, int device, int comp) {
   P.ok = false;
+  P.comp = comp;
   const int fa = cs.fft_axis - 1;
+  // the field: the pressure (comp -1), or the Helmholtz operator of a
+  // velocity component (staggered along its own axis) or a scalar
+  const int stag = comp >= 0 && comp < 3 ? comp : -1;
+  auto cbc = [&](const Block &B, int a, int sd) -> char {
+    if (comp < 0) return B.cbcpre[a][sd];
+    if (comp < 3) return B.cbcvel[comp][a][sd];
+    return B.cbcsca[((size_t)(comp - 3) * 3 + a) * 2 + sd];
+  };
+  // unknowns of a block along an axis: a staggered axis bounded at its upper
+  // end keeps the boundary face out (main.f90:774-775)
+  auto extent = [&](const Block &B, int a) { return B.n[a] - ((a == stag && B.is_bound[a][1]) ? 1 : 0); };
+  // spacing to the lower / upper neighbour of unknown i and its width dx2
+  auto sp_lo = [&](const Block &B, int a, int i) { return a == stag ? B.df[a][i] : B.dc[a][i - 1]; };
+  auto sp_hi = [&](const Block &B, int a, int i) { return a == stag ? B.df[a][i + 1] : B.dc[a][i]; };
+  auto wdx2 = [&](const Block &B, int a, int i) { return a == stag ? B.dc[a][i] : B.df[a][i]; };
   P.fa = fa;
   const bool coupled = fa < 0;
   int gmax[3];
@@ -172,6 +188,12 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
   if (fa >= 0)
     for (auto &B : cs.blocks)
       if (B.n[fa] != P.G[fa]) { P.why = "a block does not span the whole FFT axis"; return; }
+  if (comp >= 0 && fa < 0) { P.why = "implicit diffusion on the MG-PCG path needs an FFT axis"; return; }
+  // a component staggered along a bounded FFT axis keeps its upper boundary
+  // face out: the transform covers the first n-1 planes (fftini
+  // fft.f90:31-33), n-1 modes
+  const bool fa_centred = fa >= 0 && fa != stag;
+  const int qfft = (fa >= 0 && !fa_centred && cbc(cs.blocks[0], fa, 0) != 'F') ? 1 : 0;
   // axes: s = line axis (non-periodic; stretched preferred, then longest)
   std::vector<int> cand;
   for (int a = 0; a < 3; a++)
@@ -196,7 +218,7 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
     P.ax_a = rest[0];
     P.ax_m = fa;
   }
-  const int ns = P.G[P.ax_s], na = P.G[P.ax_a], nm = P.G[P.ax_m];
+  const int ns = P.G[P.ax_s], na = P.G[P.ax_a], nm = P.G[P.ax_m] - (coupled ? 0 : qfft);
   if (ns > 64 * 17) { P.why = "line axis longer than 1088 cells"; return; }
 
   // fine-level operator
@@ -212,20 +234,21 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
   else { ax[0] = P.ax_s; ax[1] = P.ax_a; ax[2] = -1; }
   bool has_dirichlet = false;
   for (auto &B : cs.blocks) {
-    const int n0 = B.n[ax[0]], n1 = B.n[ax[1]], n2 = coupled ? B.n[ax[2]] : 1;
+    const int n0 = extent(B, ax[0]), n1 = extent(B, ax[1]), n2 = coupled ? extent(B, ax[2]) : 1;
+    const int nn[3] = {n0, n1, n2};
     for (int i2 = 1; i2 <= n2; i2++)
       for (int i1 = 1; i1 <= n1; i1++)
         for (int i0 = 1; i0 <= n0; i0++) {
           const int idx[3] = {i0, i1, i2};
           double weight = 1.0;
-          for (int e = 0; e < nd; e++) weight *= B.df[ax[e]][idx[e]];
+          for (int e = 0; e < nd; e++) weight *= wdx2(B, ax[e], idx[e]);
           double cmv[3], cpv[3], ssum = 0.0;
           for (int e = 0; e < nd; e++) {
             double other = 1.0;
             for (int o = 0; o < nd; o++)
-              if (o != e) other *= B.df[ax[o]][idx[o]];
-            cmv[e] = other / B.dc[ax[e]][idx[e] - 1];
-            cpv[e] = other / B.dc[ax[e]][idx[e]];
+              if (o != e) other *= wdx2(B, ax[o], idx[o]);
+            cmv[e] = other / sp_lo(B, ax[e], idx[e]);
+            cpv[e] = other / sp_hi(B, ax[e], idx[e]);
             ssum += cmv[e] + cpv[e];
           }
           double diag0 = -ssum;
@@ -233,18 +256,20 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
           for (int e = 0; e < nd; e++) {
             const int a = ax[e];
             if (cs.periods[a] != 0) continue;
-            const int nn = B.n[a];
+            // a Dirichlet value lies on the boundary face itself for a
+            // staggered axis (sgn 0), half a cell out otherwise (sgn -1)
+            const double sgd = a == stag ? 0.0 : -1.0;
             if (B.is_bound[a][0] && idx[e] == 1) {
-              const char bc = B.cbcpre[a][0];
-              const double sg = bc == 'N' ? 1.0 : (bc == 'D' ? -1.0 : 0.0);
+              const char bc = cbc(B, a, 0);
+              const double sg = bc == 'N' ? 1.0 : (bc == 'D' ? sgd : 0.0);
               if (bc == 'D') has_dirichlet = true;
               diag0 += sg * cmv[e];
               bdv[e] += (1.0 - sg) * cmv[e];
               cmv[e] = 0.0;
             }
-            if (B.is_bound[a][1] && idx[e] == nn) {
-              const char bc = B.cbcpre[a][1];
-              const double sg = bc == 'N' ? 1.0 : (bc == 'D' ? -1.0 : 0.0);
+            if (B.is_bound[a][1] && idx[e] == nn[e]) {
+              const char bc = cbc(B, a, 1);
+              const double sg = bc == 'N' ? 1.0 : (bc == 'D' ? sgd : 0.0);
               if (bc == 'D') has_dirichlet = true;
               diag0 += sg * cpv[e];
               bdv[e] += (1.0 - sg) * cpv[e];
@@ -264,6 +289,35 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
           L0.ca[q] = cpv[1];
           if (coupled) L0.cm[q] = cpv[2];
           for (int e = 0; e < nd; e++) cmm[q * 3 + e] = cmv[e];
+        }
+  }
+  // a Helmholtz operator's unknowns exclude the bounded upper faces of the
+  // staggered axis: a link from a neighbouring block into such a face (v at
+  // the top of a lower block, whose x-neighbour is the wall face of the block
+  // beside it) leaves the union of the blocks' extents, the struct grid of
+  // solver.f90:221-224, and meets a zero value there -- it drops out
+  if (comp >= 0) {
+    double *cpw[3] = {L0.cs.data(), L0.ca.data(), L0.cm.data()};
+    const int ext[3] = {ns, na, L0.nmq};
+    const bool per[3] = {false, L0.per_a, L0.per_m};
+    for (int m = 0; m < L0.nmq; m++)
+      for (int a = 0; a < na; a++)
+        for (int s2 = 0; s2 < ns; s2++) {
+          const long q = L0.at(m, a, s2);
+          if (!L0.act[q]) continue;
+          const int id[3] = {s2, a, m};
+          for (int e = 0; e < nd; e++)
+            for (int sd = -1; sd <= 1; sd += 2) {
+              int j[3] = {id[0], id[1], id[2]};
+              j[e] += sd;
+              if (j[e] < 0 || j[e] == ext[e]) {
+                if (!per[e]) continue;
+                j[e] = j[e] < 0 ? ext[e] - 1 : 0;
+              }
+              if (L0.act[L0.at(j[2], j[1], j[0])]) continue;
+              if (sd > 0) cpw[e][q] = 0.0;
+              else cmm[q * 3 + e] = 0.0;
+            }
         }
   }
   // every link must be symmetric and join two active cells that are
@@ -347,8 +401,12 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
     const Block &B0 = cs.blocks[0];
     int kf, kb;
     double nr[2];
-    if (!find_fft(B0.cbcpre[fa][0], B0.cbcpre[fa][1], kf, kb, nr)) { P.why = "unsupported BC pair along the FFT axis"; return; }
-    eigenvalues(nm, B0.cbcpre[fa][0], B0.cbcpre[fa][1], lamh);
+    if (!find_fft(cbc(B0, fa, 0), cbc(B0, fa, 1), kf, kb, nr, fa_centred)) {
+      P.why = std::string("unsupported BC pair along the FFT axis") + (fa_centred ? "" : " (staggered)");
+      return;
+    }
+    eigenvalues(nm + qfft, cbc(B0, fa, 0), cbc(B0, fa, 1), lamh, fa_centred);
+    lamh.resize(nm);
     const double dl = B0.df[fa][1];
     for (auto &l : lamh) l /= dl * dl;
     P.norm = 1.0 / (nr[0] * (nm + nr[1]));
@@ -366,7 +424,8 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
   P.nW = (P.G[P.ax_s] + 7) / 8 * 8;
   P.any_singular = false;
   for (int g = 0; g < P.ng; g++) {
-    sing[g] = zero_rowsum && (coupled || lamh[g] == 0.0);
+    // the Helmholtz operators are shifted by -1/alpha > 0: never singular
+    sing[g] = comp < 0 && zero_rowsum && (coupled || lamh[g] == 0.0);
     P.any_singular = P.any_singular || sing[g];
   }
   if (P.any_singular && P.pin_s < 0) { P.why = "internal: empty coarsest line"; return; }
@@ -386,6 +445,13 @@ void mg_plan(MGPlan &P, const Case &cs, int device) {
   if (!coupled) {
     P.h64.lam = dev_upload(P, lamh);
     P.h32.lam = dev_upload(P, f32(lamh));
+    if (comp >= 0) {  // the eigenvalues without the shift, and the shifted ones the solves read
+      P.lam_base = P.h64.lam;
+      P.lam64 = dev_upload(P, lamh);
+      P.lam32 = dev_upload(P, f32(lamh));
+      P.h64.lam = P.lam64;
+      P.h32.lam = P.lam32;
+    }
   }
   for (size_t l = 0; l < H.size(); l++) {
     const HostLev &h = H[l];

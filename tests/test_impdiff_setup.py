@@ -1,6 +1,6 @@
 """Set-up checks of implicit diffusion (the reference's _IMPDIFF build) on a
-host-only context: the cases the tensor-product Helmholtz solves take, and
-the clear errors for the ones they do not."""
+host-only context: the cases the tensor-product and the MG-PCG Helmholtz
+solves take, and the clear errors for the ones they do not."""
 import pytest
 
 import cases
@@ -23,10 +23,21 @@ def test_accepted(tmp_path, geom, fft_axis):
     ctx.close()
 
 
-def test_multigrid_case_rejected(tmp_path):
+@pytest.mark.parametrize("geom", ["lchannel", "bfs"])
+def test_multigrid_cases_accepted(tmp_path, geom):
+    """unions of blocks that are not boxes: every field's Helmholtz operator
+    assembles on the MG-PCG path (on the step, v's link from the lower
+    block's top face to the inlet's wall face drops out)"""
+    blocks = cases.lchannel_blocks(4, 8) if geom == "lchannel" else cases.bfs_blocks((6, 4), (8, 4, 6), 4)
+    ctx = host_ctx(tmp_path, blocks, 3)
+    ctx.set_impdiff(True)
+    ctx.close()
+
+
+def test_multigrid_case_without_fft_axis_rejected(tmp_path):
     import snac_amd
-    ctx = host_ctx(tmp_path, cases.lchannel_blocks(4, 8), 3)
-    with pytest.raises(snac_amd.SnacError, match="tensor-product solver"):
+    ctx = host_ctx(tmp_path, cases.lchannel_blocks(4, 8), 0)
+    with pytest.raises(snac_amd.SnacError, match="needs an FFT axis"):
         ctx.set_impdiff(True)
     ctx.close()
 

@@ -264,3 +264,72 @@ def test_inflow_plane_scaling(tmp_path):
         oc.set(0, "v", np.full(oc.shape(0, "v"), uref))
         oc.solve_impdiff("v", -0.04)
         np.testing.assert_allclose(oc.get(0, "v")[1:-1, 1:-2, 1:-1], uref, rtol=1e-13)
+
+
+@pytest.mark.parametrize("geom", ["bfs", "lchannel"])
+@pytest.mark.parametrize("comp", [0, 1, 2])
+def test_union_helmholtz_self_adjoint(tmp_path, comp, geom):
+    """On the backward-facing step and the L-shaped channel (unions of blocks
+    that are not boxes) the Helmholtz solve S of every component is self-adjoint in the weighted
+    inner product: S = (A + W/alpha)^-1 W/alpha with A the symmetric
+    assembled operator, so <phi2, W S phi1> = <phi1, W S phi2> for the
+    linear part of S (S(0), from the inflow plane, subtracted).  The CG
+    solver applies the assembled links as they are (the banded Cholesky
+    reads one triangle only, so it is self-adjoint by construction) and must
+    agree with the banded solve.  v meets the one link that leaves the
+    struct grid: the lower block's top face (an interface) next to the
+    inlet's top wall face, which is not an unknown; indexing it like an
+    unknown of the inlet would link two cells that are not neighbours, one
+    way only.  On the L-shaped channel u meets the same situation (the upper
+    left block's x+ face is an interface, the lower block's a wall) and the
+    FFT axis is bounded: sine transforms, w over n-1 faces."""
+    blocks = cases.bfs_blocks((6, 4), (8, 4, 6), 4) if geom == "bfs" else cases.lchannel_blocks(4, 6)
+    oc = make_case(tmp_path, blocks, 3, visci=100.0)
+    name = "uvw"[comp]
+    nb = oc.nblocks
+
+    def sl(q):
+        s = [slice(1, -1)] * 3
+        if oc.block_info(q)["nb"][comp][1] < 0:
+            s[comp] = slice(1, -2)
+        return tuple(s)
+
+    def weight(q):
+        """dx2 of the non-FFT axes (z, the FFT axis, is uniform)"""
+        mx, my = (oc.metric(q, 2 * a if a == comp else 2 * a + 1)[1:-1] for a in range(2))
+        return (mx[:, None] * my[None, :])[..., None]
+
+    def solve(fields):
+        for q in range(nb):
+            oc.set(q, name, fields[q])
+        oc.solve_impdiff(name, -0.05)
+        return [oc.get(q, name) for q in range(nb)]
+
+    def dot(a, b):
+        tot = 0.0
+        for q in range(nb):
+            u, v = a[q][sl(q)], b[q][sl(q)]
+            tot += np.sum(weight(q)[:u.shape[0], :u.shape[1]] * u * v)
+        return tot
+
+    rng = np.random.default_rng(5 + comp)
+    phi = []
+    for _ in range(2):
+        f = []
+        for q in range(nb):
+            a = np.zeros(oc.shape(q, name))
+            a[sl(q)] = rng.uniform(-1, 1, a[sl(q)].shape)
+            f.append(a)
+        phi.append(f)
+    zero = [np.zeros(oc.shape(q, name)) for q in range(nb)]
+    res = {}
+    for method in ("cg", "band"):
+        oc.set_solver(method, tol=1e-15, maxiter=5000)
+        s0 = solve(zero)
+        res[method] = [[s - z for s, z in zip(solve(p), s0)] for p in phi]
+    x = res["cg"]
+    d12, d21 = dot(phi[1], x[0]), dot(phi[0], x[1])
+    assert abs(d12 - d21) <= 1e-10 * max(abs(d12), abs(d21)), (d12, d21)
+    for i in range(2):
+        for q in range(nb):
+            np.testing.assert_allclose(x[i][q][sl(q)], res["band"][i][q][sl(q)], rtol=0, atol=1e-10)
