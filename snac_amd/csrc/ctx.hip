@@ -132,7 +132,7 @@ struct snac_ctx {
   // at the end of the segment) until it meets the tolerance
   bool use_graph = true;
   std::vector<hipGraphExec_t> gseg[2];
-  std::vector<snac::MGRun> gres[2];
+  std::vector<std::pair<snac::MGPlan *, snac::MGRun>> gres[2];
   double graph_dt[2] = {0.0, 0.0};
   int graph_cur[2] = {-1, -1};
   bool capturing = false;  // a step is being captured (segments go to cap_key)
@@ -709,6 +709,7 @@ void plan_impdiff(snac_ctx *c) {
 void mg_stage_in(snac_ctx *c, MGPlan &P);
 void mg_stage_out(snac_ctx *c, MGPlan &P);
 void finish_mg_plan(snac_ctx *c, MGPlan &P);
+void split_capture(snac_ctx *c, MGPlan &P, const MGRun &R);
 
 // solve_impdiff_field (solve_helmholtz.f90:30-136) of velocity component
 // comp (0..2) or scalar comp-3, in place on its current buffer: the
@@ -731,6 +732,7 @@ void solve_impdiff(snac_ctx *c, int comp, double alpha) {
     return;
   }
   if (!f.mg) {
+    if (c->capturing) throw std::runtime_error("internal: a Helmholtz MG plan created during graph capture");
     auto P = std::shared_ptr<MGPlan>(new MGPlan(), [](MGPlan *q) {
       mg_free(*q);
       delete q;
@@ -745,7 +747,8 @@ void solve_impdiff(snac_ctx *c, int comp, double alpha) {
   Phase tp(c, SNAC_PH_HELMHOLTZ);
   mg_stage_in(c, P);
   mg_set_shift(P, c->stream, alphai);
-  mg_solve(P, c->stream, 0);
+  if (c->capturing) split_capture(c, P, mg_solve(P, c->stream, P.fixed_iters));
+  else mg_solve(P, c->stream, 0);
   mg_stage_out(c, P);
   for (size_t b = 0; b < c->blk.size(); b++) {
     DevBlock &D = c->blk[b];
@@ -818,7 +821,7 @@ void mg_stage_out(snac_ctx *c, MGPlan &P) {
 // ends the graph segment being captured at the end of a solve's captured
 // CG iterations, keeps the solve's state for the host's resumption, and
 // starts capturing the next segment
-void split_capture(snac_ctx *c, const MGRun &R) {
+void split_capture(snac_ctx *c, MGPlan &P, const MGRun &R) {
   hipGraph_t g;
   SNAC_HIP(hipStreamEndCapture(c->stream, &g));
   hipGraphExec_t e = nullptr;
@@ -826,7 +829,7 @@ void split_capture(snac_ctx *c, const MGRun &R) {
   (void)hipGraphDestroy(g);
   SNAC_HIP(err);
   c->gseg[c->cap_key].push_back(e);
-  c->gres[c->cap_key].push_back(R);
+  c->gres[c->cap_key].push_back({&P, R});
   SNAC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
 }
 
@@ -836,7 +839,7 @@ void solve_pressure(snac_ctx *c) {
     { Phase tf(c, SNAC_PH_R2R_FWD); mg_stage_in(c, c->mg); }
     {
       Phase tt(c, SNAC_PH_TRIDIAG);
-      if (c->capturing) split_capture(c, mg_solve(c->mg, c->stream, c->mg.fixed_iters));
+      if (c->capturing) split_capture(c, c->mg, mg_solve(c->mg, c->stream, c->mg.fixed_iters));
       else mg_solve(c->mg, c->stream, 0);
     }
     { Phase tb(c, SNAC_PH_R2R_BWD); mg_stage_out(c, c->mg); }
@@ -1093,19 +1096,18 @@ void capture_step(snac_ctx *c, int key, double dt) {
 // when graphs are off; otherwise one step is captured into hipGraphs (the
 // kernels of a step address the same buffers every step: the velocity
 // parity flips six times per step) and replayed, re-captured when dt or the
-// starting parity changes.  On the MG-PCG path each pressure solve's first
-// fixed_iters CG iterations are part of the graph; the host then resumes the
-// solve until every mode meets hypre_tol (or hypre_maxiter), so a graph step
-// ends every solve exactly where an eager one does.
+// starting parity changes.  On the MG-PCG path each solve's first
+// fixed_iters CG iterations are part of the graph (the pressure's, and with
+// implicit diffusion each field's Helmholtz solve, by its own plan's count);
+// the host then resumes the solve until every mode meets hypre_tol (or
+// hypre_maxiter), so a graph step ends every solve exactly where an eager
+// one does.
 void run_steps(snac_ctx *c, double dt, int nsteps) {
   check_solver(c);
   if (nsteps <= 0) return;
   // the learned preconditioner and the pair recording decide on the host
   // between CG iterations: eager launches
-  // (so do implicit-diffusion solves on the MG-PCG path: their CG loops are
-  // checked on the host)
-  const bool host_cg =
-      use_mg(c) && (c->prec == SNAC_PREC_LEARNED || c->mg.collecting || c->impdiff);
+  const bool host_cg = use_mg(c) && (c->prec == SNAC_PREC_LEARNED || c->mg.collecting);
   if (!c->use_graph || c->timing || host_cg) {
     for (int s = 0; s < nsteps; s++)
       for (int irk = 1; irk <= 3; irk++) substage(c, irk, dt);
@@ -1113,13 +1115,29 @@ void run_steps(snac_ctx *c, double dt, int nsteps) {
   }
   const bool mgp = use_mg(c);
   MGPlan &P = c->mg;
-  if (mgp && P.fixed_iters == 0) {
+  // the Helmholtz solves of implicit diffusion on the MG-PCG path (their
+  // plans exist after the first step that solves them)
+  bool imp_new = false;
+  std::vector<MGPlan *> hp;
+  if (mgp && c->impdiff)
+    for (auto &f : c->imp) {
+      if (!f.mg_ok && c->tp.ok) continue;  // solved on the tensor-product plan
+      if (!f.mg || f.mg->fixed_iters == 0) imp_new = true;
+      if (f.mg) hp.push_back(f.mg.get());
+    }
+  if (mgp && (P.fixed_iters == 0 || imp_new)) {
     // one step with host-checked convergence measures the CG count the
     // captured part of each solve runs
     for (int irk = 1; irk <= 3; irk++) substage(c, irk, dt);
     nsteps--;
     P.fixed_iters = c->force_fixed > 0 ? std::min(c->force_fixed, P.maxiter)
                                        : std::min(P.maxiter, std::max(1, P.max_seen + 1));
+    hp.clear();
+    for (auto &f : c->imp)
+      if (f.mg) {
+        f.mg->fixed_iters = std::min(f.mg->maxiter, std::max(1, f.mg->max_seen + 1));
+        hp.push_back(f.mg.get());
+      }
     drop_graph(c);
   }
   // a step flips the velocity parity six times (back to the start) and the
@@ -1134,18 +1152,25 @@ void run_steps(snac_ctx *c, double dt, int nsteps) {
     for (size_t q = 0; q < nseg; q++) {
       SNAC_HIP(hipGraphLaunch(c->gseg[key][q], c->stream));
       if (q + 1 < nseg) {
-        MGRun R = c->gres[key][q];
-        mg_continue(P, c->stream, R);
+        MGRun R = c->gres[key][q].second;
+        mg_continue(*c->gres[key][q].first, c->stream, R);
       }
     }
     if (c->cs.nscal > 0) c->scur = 1 - key;
   }
   // solves that needed more than the captured count raise it for the next
   // capture (the count only grows, up to maxiter)
+  bool grow = false;
   if (mgp && c->force_fixed == 0 && P.max_seen + 1 > P.fixed_iters && P.fixed_iters < P.maxiter) {
     P.fixed_iters = std::min(P.maxiter, P.max_seen + 1);
-    drop_graph(c);
+    grow = true;
   }
+  for (MGPlan *H : hp)
+    if (H->max_seen + 1 > H->fixed_iters && H->fixed_iters < H->maxiter) {
+      H->fixed_iters = std::min(H->maxiter, H->max_seen + 1);
+      grow = true;
+    }
+  if (grow) drop_graph(c);
 }
 
 double *field_ptr(snac_ctx *c, int b, int fid, long &count) {
@@ -1497,8 +1522,13 @@ int snac_set_precision(snac_ctx *c, int mode) {
     c->st32 = st32;
     c->prec = mode;
     c->mg.prec = mode;
-    c->mg.fixed_iters = 0;  // the graph's CG count is measured again
+    c->mg.fixed_iters = 0;  // the graph's CG counts are measured again
     c->mg.max_seen = 0;
+    for (auto &f : c->imp)
+      if (f.mg) {
+        f.mg->fixed_iters = 0;
+        f.mg->max_seen = 0;
+      }
     drop_graph(c);
   });
 }

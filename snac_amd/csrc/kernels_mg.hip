@@ -1187,7 +1187,7 @@ __global__ __launch_bounds__(256) void k_sub_mean(MGLev L, T *__restrict__ x, co
 // positions within a wavefront, wavefront totals through LDS)
 __global__ __launch_bounds__(256) void k_flags(int ng, const double *__restrict__ rr, const double *__restrict__ bb,
                                                double tol2, int *__restrict__ done, int *__restrict__ left,
-                                               int *__restrict__ list) {
+                                               int *__restrict__ list, int it) {
   __shared__ int wsum[4];
   __shared__ int base;
   if (threadIdx.x == 0) base = 0;
@@ -1211,7 +1211,10 @@ __global__ __launch_bounds__(256) void k_flags(int ng, const double *__restrict_
     if (threadIdx.x == 0) base += tot;
   }
   __syncthreads();
-  if (threadIdx.x == 0) left[0] = base;
+  if (threadIdx.x == 0) {
+    left[0] = base;
+    if (base == 0 && left[1] == 0) left[1] = it + 1;
+  }
 }
 
 // strided 3D copy; TILE 1: lanes along a on the read side (src stride(a) 1),
@@ -1579,6 +1582,8 @@ template void mg_update_apply_dot<float>(hipStream_t, const MGLev &, const doubl
 
 void mg_set_shift(MGPlan &P, hipStream_t st, double shift) {
   if (P.comp < 0 || !P.lam_base) return;
+  // the zebra sweeps alone contract the error by about (cross / (|shift| w))^2
+  P.smooth_only = P.cross_max <= 0.1 * std::fabs(shift);
   hipLaunchKernelGGL(k_shift_lam, dim3((P.ng + 255) / 256), dim3(256), 0, st, P.ng, P.lam_base, shift, P.lam64,
                      P.lam32);
   SNAC_HIP(hipGetLastError());
@@ -1621,8 +1626,8 @@ void mg_sub_mean(hipStream_t st, const MGLev &L, T *x, const double *sums, const
 }
 
 void mg_flags(hipStream_t st, int ng, const double *rr, const double *bb, double tol2, int *done, int *left,
-              int *list) {
-  hipLaunchKernelGGL(k_flags, dim3(1), dim3(256), 0, st, ng, rr, bb, tol2, done, left, list);
+              int *list, int it) {
+  hipLaunchKernelGGL(k_flags, dim3(1), dim3(256), 0, st, ng, rr, bb, tol2, done, left, list, it);
   SNAC_HIP(hipGetLastError());
 }
 

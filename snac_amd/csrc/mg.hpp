@@ -181,11 +181,14 @@ template <typename T>
 void mg_sub_mean(hipStream_t st, const MGLev &L, T *x, const double *sums, const int *glist, int nlist, long gsize,
                  const double *nact,
                  const int *done);
-// done[g] |= rr[g] <= tol2 * bb[g]; *left = number of groups not done
-// done[g] |= r.r <= tol2 b.b; left = the groups not done; list (when given):
-// those groups in increasing order
+// done[g] |= r.r <= tol2 b.b; left[0] = the groups not done; list (when
+// given): those groups in increasing order.  it: the CG iterations done so
+// far; the first call that finds every group done sets left[1] = it + 1
+// (left[1] is cleared at the start of a solve), so a solve whose iterations
+// ran ahead without a host check (captured in a graph) still tells how many
+// it needed
 void mg_flags(hipStream_t st, int ng, const double *rr, const double *bb, double tol2, int *done, int *left,
-              int *list);
+              int *list, int it);
 // strided 3D copy (em x ea x es elements) with scaling; `tile` selects an
 // LDS transpose for src stride(a) == 1 -> dst stride(s) == 1 (1) or the
 // reverse (2); 0 = direct with lanes along s
@@ -223,6 +226,10 @@ struct MGPlan {
   const double *lam_base = nullptr;
   double *lam64 = nullptr;
   float *lam32 = nullptr;
+  // largest sum of a finest-level unknown's couplings across lines (along
+  // a) over its weight, and whether the V-cycle stops at the finest level
+  double cross_max = 0.0;
+  bool smooth_only = false;
   bool fallback = true;
   bool guess = false;             // also start each solve from the network's answer for b
   double stall_ratio = 0.5;
@@ -280,11 +287,15 @@ struct MGPlan {
 // when the case cannot use it); device < 0 plans without allocating.  comp
 // -1: the pressure's Poisson operator; 0..2 / 3 + i: the Helmholtz operator
 // of velocity component comp / scalar i (implicit diffusion; its unknowns on
-// the component's staggered positions, its own boundary conditions, a
-// periodic FFT axis)
+// the component's staggered positions, its own boundary conditions and
+// transform along the FFT axis, which a component staggered along a bounded
+// FFT axis takes over its n-1 interior faces)
 void mg_plan(MGPlan &P, const Case &cs, int device, int comp = -1);
 // Helmholtz plans: the per-mode eigenvalues the next solves use become
-// lam_base + shift (shift = 1 / alpha of the substage), in both precisions
+// lam_base + shift (shift = 1 / alpha of the substage), in both precisions.
+// Where the shift outweighs the couplings across lines by far (cross_max <=
+// 0.1 |shift|) the next solves precondition with the finest level's zebra
+// line sweeps alone (smooth_only): the coarse levels would correct nothing
 void mg_set_shift(MGPlan &P, hipStream_t st, double shift);
 void mg_free(MGPlan &P);
 // Host-side position of one CG solve: the iterations launched so far and

@@ -380,6 +380,17 @@ void mg_plan(MGPlan &P, const Case &cs, int device, int comp) {
       if (L0.act[q]) { L0.dg[q] -= 2.0 * L0.cm[q]; L0.cm[q] = 0.0; }
     L0.per_m = false;
   }
+  // a Helmholtz operator's couplings across lines against its weight (the
+  // diagonal gains shift * w): how far the shift dominates them
+  P.cross_max = 0.0;
+  if (comp >= 0 && !coupled)
+    for (int a = 0; a < na; a++)
+      for (int s2 = 0; s2 < ns; s2++) {
+        const long q = L0.at(0, a, s2);
+        if (!L0.act[q] || !(L0.w[q] > 0.0)) continue;
+        const double cm = a > 0 ? L0.ca[L0.at(0, a - 1, s2)] : (L0.per_a ? L0.ca[L0.at(0, na - 1, s2)] : 0.0);
+        P.cross_max = std::max(P.cross_max, (L0.ca[q] + cm) / L0.w[q]);
+      }
   // zero row sums at lambda = 0: all-Neumann/periodic, singular
   bool zero_rowsum = !has_dirichlet;
   double nact_plane = 0.0;
@@ -567,7 +578,7 @@ void mg_plan(MGPlan &P, const Case &cs, int device, int comp) {
     P.sing_list = dev_upload(P, sl);
   }
   // every group iterating until the first mg_flags of a solve says otherwise
-  P.left = dev_upload(P, std::vector<int>{P.ng});
+  P.left = dev_upload(P, std::vector<int>{P.ng, 0});
   if (!coupled) {
     std::vector<int> ident(P.ng);
     for (int g = 0; g < P.ng; g++) ident[g] = g;
@@ -577,7 +588,7 @@ void mg_plan(MGPlan &P, const Case &cs, int device, int comp) {
     set_tail(P.h64);
     set_tail(P.h32);
   }
-  SNAC_HIP(hipHostMalloc((void **)&P.left_host, sizeof(int), hipHostMallocDefault));
+  SNAC_HIP(hipHostMalloc((void **)&P.left_host, 2 * sizeof(int), hipHostMallocDefault));
   P.stall = dev_zeros<int>(P, 1);
   P.rr_prev = dev_zeros<double>(P, 2 * P.ng);
   SNAC_HIP(hipHostMalloc((void **)&P.stall_host, sizeof(int), hipHostMallocDefault));
@@ -614,15 +625,20 @@ void vcycle(MGPlan &P, MGHier<T> &H, hipStream_t st, size_t l) {
     mg_line(st, L, H.lam, x, b, col[k][0], col[k][1], znb, -1, nullptr, P.done);
     if (mk) P.mark_line0(false);
   }
-  mg_restrict_residual(st, L, H.lev[l + 1], H.lam, x, b, H.bl[l + 1], P.done);
-  vcycle(P, H, st, l + 1);
+  // smooth_only: the finest level's symmetric zebra sweeps are the whole
+  // preconditioner (no coarse correction)
+  const bool coarse = !(l == 0 && P.smooth_only);
+  if (coarse) {
+    mg_restrict_residual(st, L, H.lev[l + 1], H.lam, x, b, H.bl[l + 1], P.done);
+    vcycle(P, H, st, l + 1);
+  }
   // two colours (FFT path): the first post-sweep relaxes colour 1 reading
   // the colour-0 lines with the prolongation added on the fly, and colour
   // 0 is then relaxed from scratch, so the prolongation is never stored.
   // The coupled colours of the 3D path read not-yet-relaxed lines twice:
   // prolong first.
-  const bool fuse = nc == 2 && !L.coupled;
-  if (!fuse) mg_prolong(st, L, H.lev[l + 1], (const T *)H.xl[l + 1], x, P.done);
+  const bool fuse = coarse && nc == 2 && !L.coupled;
+  if (coarse && !fuse) mg_prolong(st, L, H.lev[l + 1], (const T *)H.xl[l + 1], x, P.done);
   for (int k = nc - 1; k >= 0; k--) {
     const bool mk = l == 0 && P.mark_line0;
     if (mk) P.mark_line0(true);
@@ -712,14 +728,15 @@ void mg_iteration(MGPlan &P, hipStream_t st, MGRun &R) {
                                 nullptr, P.part, P.nchunk, P.rr, P.done);
     if (P.mark_update) P.mark_update(false);
   }
-  mg_flags(st, P.ng, P.rr, P.bb, P.tol * P.tol, P.done, P.left, P.mlist);
+  mg_flags(st, P.ng, P.rr, P.bb, P.tol * P.tol, P.done, P.left, P.mlist, R.it + 1);
   if (R.net && P.fallback) launch_net_stall(st, P.ng, P.rr, P.rr_prev, P.done, P.stall_ratio * P.stall_ratio, P.stall);
   R.it++;
 }
 
-// groups still above the tolerance (synchronises)
+// groups still above the tolerance (synchronises); left_host[1] is the
+// iteration count at which none was left (0: not reached yet)
 int groups_left(MGPlan &P, hipStream_t st) {
-  SNAC_HIP(hipMemcpyAsync(P.left_host, P.left, sizeof(int), hipMemcpyDeviceToHost, st));
+  SNAC_HIP(hipMemcpyAsync(P.left_host, P.left, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
   SNAC_HIP(hipStreamSynchronize(st));
   return *P.left_host;
 }
@@ -738,9 +755,10 @@ MGRun mg_solve(MGPlan &P, hipStream_t st, int fixed) {
   const size_t pl = (size_t)P.nH * P.nW;
   mg_zero(st, n, P.x);
   mg_izero(st, P.ng, P.done);
+  mg_izero(st, 1, P.left + 1);
   project(P, st, P.r);
   mg_dot<double, double>(st, P.ng, P.gsize, P.r, P.r, nullptr, nullptr, P.part, P.nchunk, P.bb, P.done, nullptr);
-  mg_flags(st, P.ng, P.bb, P.bb, tol2, P.done, P.left, P.mlist);
+  mg_flags(st, P.ng, P.bb, P.bb, tol2, P.done, P.left, P.mlist, 0);
   if (R.record)  // the input half of this solve's training pairs
     launch_net_pair_in(st, P.ng, L0.na, L0.ns, P.nH, P.nW, P.r, P.bb, P.nact_plane, P.h64.lam, P.lam_ref,
                        P.pair_x + (size_t)P.pair_n * pl * 2);
@@ -757,7 +775,7 @@ MGRun mg_solve(MGPlan &P, hipStream_t st, int fixed) {
     mg_apply_dot(st, L0, P.h64.lam, P.p, P.q, P.ng, P.gsize, P.part, P.nchunk, P.pq, P.done);
     mg_cg_update_dot(st, P.ng, P.gsize, P.pq, P.pq, P.x, P.p, P.r, P.q, P.rf, nullptr, P.part, P.nchunk, P.rr,
                      P.done);
-    mg_flags(st, P.ng, P.rr, P.bb, tol2, P.done, P.left, P.mlist);
+    mg_flags(st, P.ng, P.rr, P.bb, tol2, P.done, P.left, P.mlist, 0);
     rr0 = P.rr;
   }
   if (R.net) {
@@ -799,7 +817,9 @@ void mg_continue(MGPlan &P, hipStream_t st, MGRun &R) {
     left = groups_left(P, st);
   }
   if (left > 0) P.n_unconverged++;
-  P.last_iters = R.it;
+  // the iterations the solve needed: the captured ones may have run past the
+  // point where every group met the tolerance (those groups stay frozen)
+  P.last_iters = P.left_host[1] > 0 ? P.left_host[1] - 1 : R.it;
   P.max_seen = std::max(P.max_seen, P.last_iters);
 }
 

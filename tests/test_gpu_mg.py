@@ -199,6 +199,9 @@ def test_graph_steps_meet_tolerance(tmp_path, captured):
         if captured == 2 and rep > 0:
             assert resumed == 3
         assert eager.solver_counters()[2] == 0
+        # the count a graph solve reports is the one it needed (the iteration
+        # at which every mode met the tolerance), not the captured count
+        assert it == eager.solver_stats()[0], (it, eager.solver_stats())
         for q in range(eager.nblocks):
             for f in FIELDS3 + ("p",):
                 np.testing.assert_array_equal(graph.get(q, f), eager.get(q, f), err_msg=f"step {rep} block {q} {f}")
