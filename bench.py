@@ -86,9 +86,10 @@ FORMS = {
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20,
+    # the metric's protocol: 100 timed steps after 10 warm-up steps
+    ap.add_argument("--steps", type=int, default=100,
                     help="timed graph-replayed steps (0: none, for eager-only profiling runs)")
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", choices=["channel", "bfs"], default="channel")
     ap.add_argument("--size", type=int, nargs=3, default=[512, 256, 256])
     ap.add_argument("--forms", default=None,
