@@ -100,6 +100,8 @@ def parse():
                     help="MG line relaxation kernel (0 rows per lane, 1 LDS-staged, 2 LDS-staged over modes, "
                          "3 / 4 as 2 with 16 / 8 lanes per line, -1 each level's fastest: the default)")
     ap.add_argument("--mg-fast-rcp", type=int, default=None, help="1: MG eliminations with refined reciprocals")
+    ap.add_argument("--mg-warm", type=int, default=None,
+                    help="1: MG-PCG pressure solves start from the previous solution (po) instead of zero")
     ap.add_argument("--mg-tail-lds", type=int, default=None,
                     help="0: the fused coarse end of the V-cycle with its vectors in global memory instead of LDS")
     ap.add_argument("--learn-steps", type=int, default=20)
@@ -212,6 +214,8 @@ def run_form(name, d, b, args, cells, dt_ref):
             ctx.set_mg_fast_rcp(args.mg_fast_rcp)
         if args.mg_tail_lds is not None:
             ctx.set_mg_tail_lds(args.mg_tail_lds)
+        if args.mg_warm is not None:
+            ctx.set_mg_warm_start(args.mg_warm)
         ctx.set_solver(solver)
         if prec != "learned":
             ctx.set_precision(prec)

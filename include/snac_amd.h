@@ -360,8 +360,13 @@ int snac_mg_line_choices(snac_ctx *ctx, int *fp64, int *fp32, int n);
  * end of the MG-PCG V-cycle with one mode's vectors of every coarse level
  * resident in LDS where they fit (160 KB per workgroup), 0 with them in
  * global memory; both compute the same values.                             */
+/* SNAC_KV_MG_WARM (this context): 1 starts each MG-PCG pressure solve from
+ * the previous solve's solution, as the reference's HYPRE solves start from
+ * po (solver.f90:866-876, 909-920); 0 (the default) starts from zero: on
+ * the channel the warm start saves no CG iteration and costs one more pass
+ * over the vectors.  Both stop at the same relative residual hypre_tol.    */
 enum { SNAC_KV_RK_MOM = 0, SNAC_KV_TRIDIAG = 1, SNAC_KV_GHOST = 2, SNAC_KV_RK_MOM32 = 3, SNAC_KV_MG_LINE = 4,
-       SNAC_KV_MG_RCP = 5, SNAC_KV_MG_TAIL = 6 };
+       SNAC_KV_MG_RCP = 5, SNAC_KV_MG_TAIL = 6, SNAC_KV_MG_WARM = 7 };
 int snac_set_kernel_variant(snac_ctx *ctx, int kernel, int variant);
 
 /* ---- building blocks, exposed for their own parity tests ----

@@ -445,6 +445,12 @@ class Context:
         or in global memory; both compute the same values."""
         _check(lib.snac_set_kernel_variant(self.h, 6, int(bool(on))), "set_kernel_variant")
 
+    def set_mg_warm_start(self, on):
+        """This context: start each MG-PCG pressure solve from the previous
+        solve's solution (as the reference's solves start from po) or from
+        zero (the default); both stop at the same relative residual."""
+        _check(lib.snac_set_kernel_variant(self.h, 7, int(bool(on))), "set_kernel_variant")
+
     def set_mg_fast_rcp(self, on):
         """Process-wide: the mode-batched MG sweeps and the fused tail divide
         by refined hardware reciprocals instead of IEEE divisions."""

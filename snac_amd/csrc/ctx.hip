@@ -111,6 +111,7 @@ struct snac_ctx {
   std::vector<snac::ImpField> imp;  // its Helmholtz operators of u, v, w and the scalars
   snac::MGPlan mg;
   bool mg_planned = false;
+  bool mg_warm = false;  // pressure solves start from the previous solution (SNAC_KV_MG_WARM)
   int solver = SNAC_SOLVER_AUTO;
   int prec = SNAC_PREC_FP64;
   bool ghost_fused = true;  // run composed ghost-fill programs in one launch
@@ -478,6 +479,7 @@ void plan_mg(snac_ctx *c) {
   MGPlan &P = c->mg;
   mg_plan(P, c->cs, c->device);
   P.prec = c->prec;
+  P.warm = c->mg_warm;
   finish_mg_plan(c, P);
 }
 
@@ -2197,6 +2199,13 @@ int snac_set_kernel_variant(snac_ctx *c, int kernel, int variant) {
       if (variant < 0 || variant > 1) throw std::runtime_error("MG tail variant must be 0 or 1");
       if (c->stream) SNAC_HIP(hipStreamSynchronize(c->stream));
       mg_set_tail_lds(variant == 1);
+    } else if (kernel == SNAC_KV_MG_WARM) {
+      if (variant < 0 || variant > 1) throw std::runtime_error("MG warm-start variant must be 0 or 1");
+      drop_graph(c);
+      c->mg_warm = variant == 1;
+      c->mg.warm = c->mg_warm;
+      c->mg.fixed_iters = 0;  // the graph's CG count is measured again
+      c->mg.max_seen = 0;
     } else if (kernel == SNAC_KV_MG_RCP) {
       if (variant < 0 || variant > 1) throw std::runtime_error("MG reciprocal variant must be 0 or 1");
       if (c->stream) SNAC_HIP(hipStreamSynchronize(c->stream));

@@ -929,6 +929,40 @@ __global__ __launch_bounds__(256) void k_cg_update_dot(long gsize, const double 
   }
 }
 
+// warm start of a solve from x0 (the previous solve's solution, in x): with
+// q = M x0, r = b - q (r holds b) with its r.r and rf = FP32 r, on the
+// groups whose b is not zero; a group with b = 0 has the solution 0 (x and
+// rf cleared, r is already 0)
+__global__ __launch_bounds__(256) void k_warm_residual(long gsize, const double *__restrict__ bb,
+                                                       double *__restrict__ x, double *__restrict__ r,
+                                                       const double *__restrict__ q, float *__restrict__ rf,
+                                                       double *__restrict__ part, int nchunk) {
+  __shared__ double sh[4];
+  const int g = blockIdx.y;
+  const bool zero = !(bb[g * 2] > 0.0);
+  const long base = (long)g * gsize;
+  double s1 = 0.0;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < gsize; i += (long)nchunk * 256) {
+    const long idx = base + i;
+    if (zero) {
+      x[idx] = 0.0;
+      if (rf) rf[idx] = 0.0f;
+      continue;
+    }
+    const double rn = r[idx] - q[idx];
+    r[idx] = rn;
+    s1 += rn * rn;
+    if (rf) rf[idx] = (float)rn;
+  }
+  s1 = wave_sum(s1);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s1;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[((long)g * nchunk + blockIdx.x) * 2] = sh[0] + sh[1] + sh[2] + sh[3];
+    part[((long)g * nchunk + blockIdx.x) * 2 + 1] = 0.0;
+  }
+}
+
 // (M v)(m, a, s) of an FP64 level where v is given element-wise by f(e)
 template <typename F>
 __device__ __forceinline__ double mx_f(const MGLev &L, double lm, F &&f, int m, int a, int s, long e, long q) {
@@ -1530,6 +1564,14 @@ void mg_apply_dot(hipStream_t st, const MGLev &L, const double *lam, const doubl
   hipLaunchKernelGGL(k_apply_dot, dim3(nchunk, ng), dim3(256), 0, st, L, lam, p, q, part, nchunk, gsize, done);
   SNAC_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_dot_final, dim3(ng), dim3(64), 0, st, part, nchunk, out, done, nullptr);
+  SNAC_HIP(hipGetLastError());
+}
+
+void mg_warm_residual(hipStream_t st, int ng, long gsize, const double *bb, double *x, double *r, const double *q,
+                      float *rf, double *part, int nchunk, double *out) {
+  hipLaunchKernelGGL(k_warm_residual, dim3(nchunk, ng), dim3(256), 0, st, gsize, bb, x, r, q, rf, part, nchunk);
+  SNAC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_dot_final, dim3(ng), dim3(64), 0, st, part, nchunk, out, nullptr, nullptr);
   SNAC_HIP(hipGetLastError());
 }
 

@@ -143,6 +143,11 @@ void mg_pattern(hipStream_t st, long n, T *x);
 // per-group sums in mg_dot's order)
 void mg_apply_dot(hipStream_t st, const MGLev &L, const double *lam, const double *p, double *q, int ng, long gsize,
                   double *part, int nchunk, double *out, const int *done);
+// warm start from x0 in x with q = M x0: r = b - q (r holds b), rf (when
+// given) = FP32 r and out[g*2] = r.r on the groups with b.b = bb[g*2] > 0;
+// the groups with b = 0 get x = 0 (their solution), rf = 0 and r.r = 0
+void mg_warm_residual(hipStream_t st, int ng, long gsize, const double *bb, double *x, double *r, const double *q,
+                      float *rf, double *part, int nchunk, double *out);
 // x += alpha p ; r -= alpha q with alpha = rz/pq per group, rf (when given)
 // = the new r rounded to FP32, and out[g*2] = r.r, out[g*2+1] = zf.r (0
 // when zf is null) of the new r, in one pass
@@ -230,6 +235,9 @@ struct MGPlan {
   // a) over its weight, and whether the V-cycle stops at the finest level
   double cross_max = 0.0;
   bool smooth_only = false;
+  // start each solve from the previous solve's solution (po, the initial
+  // guess of solve_n_helmholtz_2d solver.f90:866-876) instead of zero
+  bool warm = false;
   bool fallback = true;
   bool guess = false;             // also start each solve from the network's answer for b
   double stall_ratio = 0.5;

@@ -753,7 +753,10 @@ MGRun mg_solve(MGPlan &P, hipStream_t st, int fixed) {
   R.record = P.collecting && planar && fixed == 0 && P.pair_n + P.ng <= P.pair_cap;
   const MGLev &L0 = P.h64.lev[0];
   const size_t pl = (size_t)P.nH * P.nW;
-  mg_zero(st, n, P.x);
+  // the warm start keeps x (the previous solve's solution) as x0; the
+  // network's modes start from zero or from its own guess
+  const bool warm = P.warm && !R.net && !R.record;
+  if (!warm) mg_zero(st, n, P.x);
   mg_izero(st, P.ng, P.done);
   mg_izero(st, 1, P.left + 1);
   project(P, st, P.r);
@@ -762,7 +765,13 @@ MGRun mg_solve(MGPlan &P, hipStream_t st, int fixed) {
   if (R.record)  // the input half of this solve's training pairs
     launch_net_pair_in(st, P.ng, L0.na, L0.ns, P.nH, P.nW, P.r, P.bb, P.nact_plane, P.h64.lam, P.lam_ref,
                        P.pair_x + (size_t)P.pair_n * pl * 2);
-  if (mixed) mg_to_float(st, n, P.r, P.rf);
+  if (warm) {  // r = b - M x0; groups that x0 already solves stop here
+    mg_apply_dot(st, L0, P.h64.lam, P.x, P.q, P.ng, P.gsize, P.part, P.nchunk, P.pq, P.done);
+    mg_warm_residual(st, P.ng, P.gsize, P.bb, P.x, P.r, P.q, mixed ? P.rf : nullptr, P.part, P.nchunk, P.rr);
+    mg_flags(st, P.ng, P.rr, P.bb, tol2, P.done, P.left, P.mlist, 0);
+  } else if (mixed) {
+    mg_to_float(st, n, P.r, P.rf);
+  }
   const double *rr0 = P.bb;  // r.r of the residual the first preconditioning sees
   if (R.net && P.guess) {
     // initial guess x0 = the network's answer for b, then r = b - M x0 (the

@@ -139,6 +139,47 @@ def test_mg_10_steps(tmp_path, geom, prec):
     assert divmax < 1e-9
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_warm_start_steps(tmp_path, prec):
+    """The warm start (each pressure solve from the previous solution, as the
+    reference's solves start from po, solver.f90:866-876; the oracle does
+    the same): 10 steps on the step, every solve at hypre_tol, against the
+    oracle at the plain form's 1e-10 (FP64; the mixed form's FP32 momentum
+    right-hand side is checked by the 10-step tests above), and graph steps
+    equal to eager steps bit for bit."""
+    ctx, oc, _ = pair(tmp_path, "bfs", prec)
+    ctx.set_mg_warm_start(True)
+    run_both(ctx, oc, 10, dt_rel=1e-12 if prec == "fp64" else 1e-6)
+    if prec == "fp64":
+        assert_fields(ctx, oc, FIELDS3, 1e-10, interior=True)
+        assert_fields(ctx, oc, ("p",), 1e-10, interior=True)
+    it, relres = ctx.solver_stats()
+    print(f"warm start {prec}: last solve {it} CG iterations, relres {relres:.2e}", flush=True)
+    assert 0 < it and relres <= 1e-13 * 1.0001
+    ctx.close()
+    import snac_amd
+    d, b = cases.write_case(str(tmp_path / "g"), cases.bfs_blocks((12, 8), (20, 8, 12), 8), 1000.0)
+    eager, graph = snac_amd.Context(d, b, 3), snac_amd.Context(d, b, 3)
+    eager.set_graph(False)
+    rng = np.random.default_rng(9)
+    for c in (eager, graph):
+        c.set_precision(prec)
+        c.set_mg_warm_start(True)
+    for q in range(eager.nblocks):
+        for f in FIELDS3:
+            a = 0.05 * rng.uniform(-1, 1, eager.shape(q, f))
+            eager.set(q, f, a)
+            graph.set(q, f, a)
+    for c in (eager, graph):
+        c.init_bounds()
+        c.steps(2e-3, 4)
+    for q in range(eager.nblocks):
+        for f in FIELDS3 + ("p",):
+            np.testing.assert_array_equal(graph.get(q, f), eager.get(q, f), err_msg=f"block {q} {f}")
+    eager.close()
+    graph.close()
+
+
 def test_precision_switch_keeps_results(tmp_path):
     """Switching precision on a context re-measures the graph's CG count;
     a mixed solve and an FP64 solve of the same rhs agree to the tolerance."""
