@@ -13,7 +13,9 @@ the CPU oracle's direct banded solves of the assembled systems.
   of blocks that are not boxes, so no tensor-product plan exists) and on the
   forced channel, gate 1e-10 on every field (p modulo its mean where the
   pressure is singular); after the first step the Helmholtz solves run as
-  graph segments with the host resuming any solve left above hypre_tol.
+  graph segments with the host resuming any solve left above hypre_tol;
+* the reference's differentially heated cavity with the scalar and
+  Boussinesq buoyancy, along the periodic and along the walled FFT axis.
 """
 import numpy as np
 import pytest
@@ -147,4 +149,61 @@ def test_steps_mg(tmp_path, geom):
         iters, relres = ctx.impdiff_stats(name)
         print(f"{geom} {name}: last Helmholtz solve {iters} CG iterations, relres {relres:.2e}", flush=True)
         assert iters > 0 and relres <= 1e-13 * 1.0001, (name, iters, relres)
+    ctx.close()
+
+
+@pytest.mark.parametrize("fft_axis", [2, 3])
+def test_dhc_steps_mg(tmp_path, fft_axis):
+    """the reference's differentially heated cavity inputs (walls in x with
+    the scalar at -0.5 / 0.5, walls in z, periodic y, Boussinesq buoyancy)
+    with implicit diffusion of u, v, w and the scalar on the MG-PCG path:
+    along the periodic y (fft_axis 2), and along the walled z (fft_axis 3:
+    sine transforms for u and v, w over its n-1 interior faces, a cosine
+    transform for the scalar and the pressure).  3 steps against the
+    oracle, gate 1e-10 on every field (p modulo its mean)."""
+    import os
+    import snac_amd
+    g = os.path.join(os.path.dirname(__file__), "golden", "dhc")
+    b = os.path.join(g, "blocks.nml")
+    # the reference's inputs with a tight &hypre tolerance (the file's 1e-4 /
+    # 50 iterations would stop the iterative solves far from the oracle's
+    # direct ones)
+    with open(os.path.join(g, "dns.nml")) as f:
+        txt = f.read()
+    assert "hypre_tol = 1.e-4, hypre_maxiter = 50" in txt
+    d = str(tmp_path / "dns.nml")
+    with open(d, "w") as f:
+        f.write(txt.replace("hypre_tol = 1.e-4, hypre_maxiter = 50", "hypre_tol = 1.e-13, hypre_maxiter = 1000"))
+    print(f"dhc impdiff on the MG-PCG path: fft_axis={fft_axis}", flush=True)
+    ctx = snac_amd.Context(d, b, fft_axis)
+    ctx.set_solver("mg")
+    ctx.set_impdiff(True)
+    c = nmlparse.load_case(d, b, fft_axis)
+    c["impdiff"] = True
+    oc = oracle_py.OracleCase(c)
+    oc.set_solver("band")
+    ctx.set_buoyancy(True)
+    oc.set_buoyancy(True)
+    rng = np.random.default_rng(4)
+    in3 = (slice(1, -1),) * 3
+    for f in ("u", "v", "w", "s1"):
+        a = np.zeros(ctx.shape(0, f))
+        a[in3] = 0.1 * rng.uniform(-1, 1, tuple(q - 2 for q in a.shape))
+        ctx.set(0, f, a)
+        oc.set(0, f, a)
+    ctx.init_bounds()
+    oc.init_bounds()
+    dt = 1.0e-3
+    ctx.steps(dt, 3)
+    for _ in range(3):
+        oc.step(dt)
+    for f in ("u", "v", "w", "p", "s1"):
+        gg, o = ctx.get(0, f)[in3], oc.get(0, f)[in3]
+        if f == "p":
+            gg, o = gg - gg.mean(), o - o.mean()
+        r = rel(gg, o)
+        assert r <= 1e-10, f"{f}: rel L2 {r:.3e}"
+    for name in ("u", "v", "w", "s1"):
+        it, relres = ctx.impdiff_stats(name)
+        assert it > 0 and relres <= 1e-13 * 1.0001, (name, it, relres)
     ctx.close()
