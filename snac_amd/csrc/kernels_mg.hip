@@ -911,8 +911,8 @@ __global__ __launch_bounds__(256) void k_cg_update_dot(long gsize, const double 
     const long base = (long)g * gsize;
     for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < gsize; i += (long)nchunk * 256) {
       const long idx = base + i;
-      x[idx] += al * p[idx];
-      const double rn = r[idx] - al * q[idx];
+      if (al != 0.0) x[idx] += al * p[idx];  // a skipped step leaves x and r as they are
+      const double rn = al != 0.0 ? r[idx] - al * q[idx] : r[idx];
       r[idx] = rn;
       s1 += rn * rn;
       if (rf) rf[idx] = (float)rn;
@@ -1148,8 +1148,10 @@ __global__ __launch_bounds__(256) void k_update_apply_dot(MGLev L, const double 
       for (int u = 0; u < CG_U; u++) {
         if (i + 256 * u >= iend) break;
         const long idx = base + i + 256 * u;
-        __builtin_nontemporal_store(xv[u] + al * pc[u], &x[idx]);
-        const double rn = rv[u] - al * q[u];
+        // a skipped step (al = 0) leaves x and r as they are, also where a
+        // broken direction holds an overflow or a NaN (0 * inf is a NaN)
+        __builtin_nontemporal_store(al != 0.0 ? xv[u] + al * pc[u] : xv[u], &x[idx]);
+        const double rn = al != 0.0 ? rv[u] - al * q[u] : rv[u];
         __builtin_nontemporal_store(rn, &r[idx]);
         s1 += rn * rn;
         if (rf) rf[idx] = (float)rn;

@@ -98,7 +98,10 @@ def test_learned_preconditioner(tmp_path):
     untrained_its, untrained_rel = one_step(ctx2, snap, dt, "learned")[:2]
     print(f"network only, untrained: {untrained_its} CG iterations, max relative residual {untrained_rel:.2e} "
           f"(trained: {trained_its}, {trained_rel:.2e})", flush=True)
-    # a preconditioner that breaks the CG down leaves a NaN residual: the worst outcome
+    # a step whose direction broke down (an overflow or a NaN in the
+    # preconditioner's answer) is skipped and leaves x and r as they are,
+    # so a broken solve stalls at a finite residual; a NaN would be the
+    # worst outcome
     untrained_rel = np.inf if np.isnan(untrained_rel) else untrained_rel
     assert not np.isnan(trained_rel)
     assert (trained_its, trained_rel) < (untrained_its, untrained_rel)
