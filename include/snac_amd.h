@@ -365,8 +365,13 @@ int snac_mg_line_choices(snac_ctx *ctx, int *fp64, int *fp32, int n);
  * po (solver.f90:866-876, 909-920); 0 (the default) starts from zero: on
  * the channel the warm start saves no CG iteration and costs one more pass
  * over the vectors.  Both stop at the same relative residual hypre_tol.    */
+/* SNAC_KV_MG_CG (process-wide): 1 (the default) forms a workgroup's new
+ * CG search directions once into LDS for the operator of the direction pass
+ * where the level allows it (coefficients independent of the position
+ * across lines, line length dividing 256); 0 forms each neighbour's
+ * direction again from z and the previous direction.  Same values.         */
 enum { SNAC_KV_RK_MOM = 0, SNAC_KV_TRIDIAG = 1, SNAC_KV_GHOST = 2, SNAC_KV_RK_MOM32 = 3, SNAC_KV_MG_LINE = 4,
-       SNAC_KV_MG_RCP = 5, SNAC_KV_MG_TAIL = 6, SNAC_KV_MG_WARM = 7 };
+       SNAC_KV_MG_RCP = 5, SNAC_KV_MG_TAIL = 6, SNAC_KV_MG_WARM = 7, SNAC_KV_MG_CG = 8 };
 int snac_set_kernel_variant(snac_ctx *ctx, int kernel, int variant);
 
 /* ---- building blocks, exposed for their own parity tests ----

@@ -445,6 +445,12 @@ class Context:
         or in global memory; both compute the same values."""
         _check(lib.snac_set_kernel_variant(self.h, 6, int(bool(on))), "set_kernel_variant")
 
+    def set_mg_cg_lds(self, on):
+        """Process-wide: the CG direction pass with a workgroup's new
+        directions staged in LDS where the level allows it (the default), or
+        formed again for every neighbour; both compute the same values."""
+        _check(lib.snac_set_kernel_variant(self.h, 8, int(bool(on))), "set_kernel_variant")
+
     def set_mg_warm_start(self, on):
         """This context: start each MG-PCG pressure solve from the previous
         solve's solution (as the reference's solves start from po) or from

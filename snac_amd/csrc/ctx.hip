@@ -2199,6 +2199,11 @@ int snac_set_kernel_variant(snac_ctx *c, int kernel, int variant) {
       if (variant < 0 || variant > 1) throw std::runtime_error("MG tail variant must be 0 or 1");
       if (c->stream) SNAC_HIP(hipStreamSynchronize(c->stream));
       mg_set_tail_lds(variant == 1);
+    } else if (kernel == SNAC_KV_MG_CG) {
+      if (variant < 0 || variant > 1) throw std::runtime_error("MG CG direction-pass variant must be 0 or 1");
+      if (c->stream) SNAC_HIP(hipStreamSynchronize(c->stream));
+      drop_graph(c);
+      mg_set_cg_lds(variant == 1);
     } else if (kernel == SNAC_KV_MG_WARM) {
       if (variant < 0 || variant > 1) throw std::runtime_error("MG warm-start variant must be 0 or 1");
       drop_graph(c);

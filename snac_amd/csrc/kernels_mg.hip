@@ -1090,6 +1090,76 @@ __global__ __launch_bounds__(256) void k_dir_apply_dot(MGLev L, const double *__
   }
 }
 
+// k_dir_apply_dot on a 2D level with position-independent coefficients
+// whose rows divide 256 (every thread keeps its row s): the workgroup's
+// chunk is whole lines a0..a1-1, and its new directions, with those of the
+// neighbour lines a0-1 and a1 (wrapped on a periodic a), are formed once
+// into LDS; M p then reads them there instead of forming each element's
+// four neighbours again from z and p_prev.  Same values, same order of the
+// sums, as k_dir_apply_dot.
+template <typename TZ, typename TP>
+__global__ __launch_bounds__(256) void k_dir_apply_dot_lds(MGLev L, const double *__restrict__ lam, long gsize,
+                                                           const double *__restrict__ rzn,
+                                                           const double *__restrict__ rzo,
+                                                           const double *__restrict__ cross,
+                                                           const TZ *__restrict__ z, const TP *__restrict__ pp,
+                                                           TP *__restrict__ p, double *__restrict__ part, int nchunk,
+                                                           const int *__restrict__ done) {
+  extern __shared__ unsigned char dyn_smem[];
+  TP *sp = reinterpret_cast<TP *>(dyn_smem);  // (lines + 2) x ns: below, own lines, above
+  __shared__ double sh[4];
+  const int g = blockIdx.y;
+  double s1 = 0.0;
+  if (!done[g]) {
+    const double be = cg_beta(g, rzn, rzo, cross);
+    auto pn = [&](long e) -> TP {
+      if (be == 0.0) return (TP)(double)z[e];
+      return (TP)((double)z[e] + be * (double)pp[e]);
+    };
+    const int ns = L.ns, na = L.na;
+    const long base = (long)g * gsize;
+    const long csz = ((gsize + nchunk - 1) / nchunk + 255) / 256 * 256;
+    const long c0 = (long)blockIdx.x * csz, c1 = min(gsize, c0 + csz);
+    if (c0 < c1) {
+      const int a0 = (int)(c0 / ns), nl = (int)((c1 - c0) / ns);
+      // the neighbour lines below and above the chunk (-1: none)
+      const int lo = a0 > 0 ? a0 - 1 : (L.per_a ? na - 1 : -1);
+      const int hi = a0 + nl < na ? a0 + nl : (L.per_a ? 0 : -1);
+      const int tot = (nl + 2) * ns;
+      for (int k = threadIdx.x; k < tot; k += 256) {
+        const int la = k / ns, s = k - la * ns;
+        const int a = la == 0 ? lo : (la == nl + 1 ? hi : a0 + la - 1);
+        if (a >= 0) sp[k] = pn(base + (long)a * ns + s);
+      }
+      __syncthreads();
+      const int s = threadIdx.x % ns;
+      const MxRow c = mx_row(L, s);
+      const double lm = lam ? lam[g] : 0.0;
+      for (int k = threadIdx.x; k < nl * ns; k += 256) {
+        const int la = k / ns + 1, a = a0 + la - 1;
+        const TP *row = sp + la * ns;
+        const double pc = (double)row[s];
+        double y = (c.dg - lm * c.w) * pc;
+        if (s > 0) y -= c.csm * (double)row[s - 1];
+        if (s < ns - 1) y -= c.csp * (double)row[s + 1];
+        double r = 0;
+        if (a < na - 1 || L.per_a) r += c.ca * (double)row[s + ns];
+        if (a > 0 || L.per_a) r += c.ca * (double)row[s - ns];
+        y = y - r;
+        p[base + c0 + k] = row[s];
+        s1 += pc * y;
+      }
+    }
+  }
+  s1 = wave_sum(s1);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s1;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[((long)g * nchunk + blockIdx.x) * 2] = sh[0] + sh[1] + sh[2] + sh[3];
+    part[((long)g * nchunk + blockIdx.x) * 2 + 1] = 0.0;
+  }
+}
+
 // x += alpha p ; r -= alpha (M p) with alpha = rz / pq per group, M p
 // recomputed from p (bit for bit the value k_dir_apply_dot summed), the
 // partial sums of r.r and, when zf is given, zf.r, and rf = (float) r when
@@ -1156,6 +1226,96 @@ __global__ __launch_bounds__(256) void k_update_apply_dot(MGLev L, const double 
         s1 += rn * rn;
         if (rf) rf[idx] = (float)rn;
         if (zf) s2 += zv[u] * rn;
+      }
+    }
+  }
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  if ((threadIdx.x & 63) == 0) { sh[0][threadIdx.x >> 6] = s1; sh[1][threadIdx.x >> 6] = s2; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[((long)g * nchunk + blockIdx.x) * 2] = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
+    part[((long)g * nchunk + blockIdx.x) * 2 + 1] = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+  }
+}
+
+// k_update_apply_dot on the levels k_dir_apply_dot_lds serves: the chunk's
+// directions with the two neighbour lines are staged in LDS once, and M p
+// reads them there; same values, same order of the sums
+template <typename TP>
+__global__ __launch_bounds__(256) void k_update_apply_dot_lds(MGLev L, const double *__restrict__ lam, long gsize,
+                                                              const double *__restrict__ rz,
+                                                              const double *__restrict__ pq, double *__restrict__ x,
+                                                              const TP *__restrict__ p, double *__restrict__ r,
+                                                              float *__restrict__ rf, const float *__restrict__ zf,
+                                                              double *__restrict__ part, int nchunk,
+                                                              const int *__restrict__ done) {
+  extern __shared__ unsigned char dyn_smem[];
+  TP *sp = reinterpret_cast<TP *>(dyn_smem);  // (lines + 2) x ns: below, own lines, above
+  __shared__ double sh[2][4];
+  const int g = blockIdx.y;
+  double s1 = 0.0, s2 = 0.0;
+  if (!done[g]) {
+    const double den = pq[g * 2];
+    // as k_update_apply_dot: an unusable step length skips the step
+    double al = den > 0.0 ? rz[g * 2] / den : 0.0;
+    if (!(__builtin_fabs(al) < 1e300)) al = 0.0;
+    const int ns = L.ns, na = L.na;
+    const long base = (long)g * gsize;
+    const long csz = ((gsize + nchunk - 1) / nchunk + 255) / 256 * 256;
+    const long c0 = (long)blockIdx.x * csz, c1 = min(gsize, c0 + csz);
+    if (c0 < c1) {
+      const int a0 = (int)(c0 / ns), nl = (int)((c1 - c0) / ns);
+      const int lo = a0 > 0 ? a0 - 1 : (L.per_a ? na - 1 : -1);
+      const int hi = a0 + nl < na ? a0 + nl : (L.per_a ? 0 : -1);
+      const int tot = (nl + 2) * ns;
+      for (int k = threadIdx.x; k < tot; k += 256) {
+        const int la = k / ns, s = k - la * ns;
+        const int a = la == 0 ? lo : (la == nl + 1 ? hi : a0 + la - 1);
+        if (a >= 0) sp[k] = p[base + (long)a * ns + s];
+      }
+      __syncthreads();
+      const int s = threadIdx.x % ns;
+      const MxRow c = mx_row(L, s);
+      const double lm = lam ? lam[g] : 0.0;
+      const int nel = nl * ns;
+      for (int k0 = threadIdx.x; k0 < nel; k0 += 256 * CG_U) {
+        double q[CG_U], pc[CG_U], xv[CG_U], rv[CG_U], zv[CG_U];
+#pragma unroll
+        for (int u = 0; u < CG_U; u++) {
+          const int k = k0 + 256 * u;
+          const bool ok = k < nel;
+          const long idx = base + c0 + k;
+          const int la = ok ? k / ns + 1 : 1, a = a0 + la - 1;
+          const TP *row = sp + la * ns;
+          double y = 0.0;
+          if (ok) {
+            y = (c.dg - lm * c.w) * (double)row[s];
+            if (s > 0) y -= c.csm * (double)row[s - 1];
+            if (s < ns - 1) y -= c.csp * (double)row[s + 1];
+            double rr = 0;
+            if (a < na - 1 || L.per_a) rr += c.ca * (double)row[s + ns];
+            if (a > 0 || L.per_a) rr += c.ca * (double)row[s - ns];
+            y = y - rr;
+          }
+          q[u] = y;
+          pc[u] = ok ? (double)row[s] : 0.0;
+          xv[u] = ok ? __builtin_nontemporal_load(&x[idx]) : 0.0;
+          rv[u] = ok ? __builtin_nontemporal_load(&r[idx]) : 0.0;
+          zv[u] = ok && zf ? (double)zf[idx] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < CG_U; u++) {
+          const int k = k0 + 256 * u;
+          if (k >= nel) break;
+          const long idx = base + c0 + k;
+          __builtin_nontemporal_store(al != 0.0 ? xv[u] + al * pc[u] : xv[u], &x[idx]);
+          const double rn = al != 0.0 ? rv[u] - al * q[u] : rv[u];
+          __builtin_nontemporal_store(rn, &r[idx]);
+          s1 += rn * rn;
+          if (rf) rf[idx] = (float)rn;
+          if (zf) s2 += zv[u] * rn;
+        }
       }
     }
   }
@@ -1311,6 +1471,7 @@ int g_line_variant = -1;  // forced variant, or -1: each level's own
 bool g_fast_rcp = false;
 // the fused tail with its vectors in LDS where they fit
 bool g_tail_lds = true;
+bool g_cg_lds = true;  // the CG direction pass with the new directions staged in LDS
 }  // namespace
 
 // the mode-batched sweep (k_line_modes) with LP lanes per line: false when
@@ -1454,6 +1615,7 @@ template bool mg_line_variant_runs<double>(const MGLevT<double> &, int);
 template bool mg_line_variant_runs<float>(const MGLevT<float> &, int);
 void mg_set_fast_rcp(bool on) { g_fast_rcp = on; }
 void mg_set_tail_lds(bool on) { g_tail_lds = on; }
+void mg_set_cg_lds(bool on) { g_cg_lds = on; }
 bool mg_tail_lds() { return g_tail_lds; }
 bool mg_fast_rcp() { return g_fast_rcp; }
 int mg_line_variant() { return g_line_variant; }
@@ -1591,8 +1753,16 @@ template <typename TZ, typename TP>
 void mg_dir_apply_dot(hipStream_t st, const MGLev &L, const double *lam, int ng, long gsize, const double *rzn,
                       const double *rzo, const double *cross, const TZ *z, const TP *p_prev, TP *p, double *part,
                       int nchunk, double *out, const int *done) {
-  hipLaunchKernelGGL((k_dir_apply_dot<TZ, TP>), dim3(nchunk, ng), dim3(256), 0, st, L, lam, gsize, rzn, rzo, cross, z,
-                     p_prev, p, part, nchunk, done);
+  // the LDS form where a level's coefficients do not depend on a, its rows
+  // divide 256 and a chunk's lines with their two neighbours fit in 64 KB
+  const long csz = ((gsize + nchunk - 1) / nchunk + 255) / 256 * 256;
+  const size_t lds = L.ns > 0 ? (size_t)(csz / L.ns + 2) * L.ns * sizeof(TP) : 0;
+  if (g_cg_lds && !L.coupled && L.auni && L.ns > 0 && 256 % L.ns == 0 && lds <= 64 * 1024)
+    hipLaunchKernelGGL((k_dir_apply_dot_lds<TZ, TP>), dim3(nchunk, ng), dim3(256), lds, st, L, lam, gsize, rzn, rzo,
+                       cross, z, p_prev, p, part, nchunk, done);
+  else
+    hipLaunchKernelGGL((k_dir_apply_dot<TZ, TP>), dim3(nchunk, ng), dim3(256), 0, st, L, lam, gsize, rzn, rzo, cross,
+                       z, p_prev, p, part, nchunk, done);
   SNAC_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_dot_final, dim3(ng), dim3(64), 0, st, part, nchunk, out, done, nullptr);
   SNAC_HIP(hipGetLastError());
@@ -1602,8 +1772,14 @@ template <typename TP>
 void mg_update_apply_dot(hipStream_t st, const MGLev &L, const double *lam, int ng, long gsize, const double *rz,
                          const double *pq, double *x, const TP *p, double *r, float *rf, const float *zf,
                          double *part, int nchunk, double *out, const int *done) {
-  hipLaunchKernelGGL(k_update_apply_dot<TP>, dim3(nchunk, ng), dim3(256), 0, st, L, lam, gsize, rz, pq, x, p, r, rf,
-                     zf, part, nchunk, done);
+  const long csz = ((gsize + nchunk - 1) / nchunk + 255) / 256 * 256;
+  const size_t lds = L.ns > 0 ? (size_t)(csz / L.ns + 2) * L.ns * sizeof(TP) : 0;
+  if (g_cg_lds && !L.coupled && L.auni && L.ns > 0 && 256 % L.ns == 0 && lds <= 64 * 1024)
+    hipLaunchKernelGGL(k_update_apply_dot_lds<TP>, dim3(nchunk, ng), dim3(256), lds, st, L, lam, gsize, rz, pq, x, p,
+                       r, rf, zf, part, nchunk, done);
+  else
+    hipLaunchKernelGGL(k_update_apply_dot<TP>, dim3(nchunk, ng), dim3(256), 0, st, L, lam, gsize, rz, pq, x, p, r, rf,
+                       zf, part, nchunk, done);
   SNAC_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_dot_final, dim3(ng), dim3(64), 0, st, part, nchunk, out, done, nullptr);
   SNAC_HIP(hipGetLastError());

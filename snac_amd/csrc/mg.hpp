@@ -120,6 +120,10 @@ void mg_tune_lines(MGPlan &P, hipStream_t st);
 // of every tail level resident in LDS where they fit (160 KB per
 // workgroup), 0 with them in global memory; both compute the same values
 void mg_set_tail_lds(bool on);
+// process-wide: the CG direction pass with a workgroup's new directions
+// staged in LDS where the level allows it (the default), or formed for every
+// neighbour from z and p_prev; both compute the same values
+void mg_set_cg_lds(bool on);
 bool mg_tail_lds();
 // process-wide: the mode-batched sweeps and the fused tail divide by
 // refined hardware reciprocals (the estimate plus Newton steps: two in

@@ -36,10 +36,10 @@ def pair(tmp_path, geom, prec="fp64"):
         blocks, visci = cases.bfs_blocks((12, 8), (20, 8, 12), 8), 1000.0
     elif geom == "channel":
         blocks, visci, singular = [cases.channel_block((32, 24, 16))], 5600.0, True
-    elif geom in ("channel_y160", "channel_y256"):
+    elif geom in ("channel_y160", "channel_y256", "channel_y128"):
         # long wall-normal lines, and more than 32 columns, so that the finest
         # levels are relaxed by the launched line kernels, not the fused tail
-        shape = (72, 160, 8) if geom == "channel_y160" else (64, 256, 4)
+        shape = {"channel_y160": (72, 160, 8), "channel_y256": (64, 256, 4), "channel_y128": (96, 128, 6)}[geom]
         blocks, visci, singular = [cases.channel_block(shape)], 5600.0, True
     elif geom == "cavity3d":
         blocks, visci, fa, singular = [cases.cavity_block((12, 10, 16), gr=(0.0, 1.5, 0.0))], 1000.0, 0, True
@@ -306,6 +306,37 @@ def test_line_variants_long_lines(tmp_path, geom, prec):
         np.testing.assert_array_equal(out[0], out[v])
     for v in (3, 4):
         print(f"variant {v}: max |pp - pp(variant 0)| = {np.abs(out[v] - out[0]).max():.2e}", flush=True)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("geom", ["channel_y128", "channel_y256"])
+def test_cg_lds_agrees(tmp_path, geom, prec):
+    """The CG direction pass with a workgroup's new directions staged in LDS
+    (lines of 128 and 256 rows, the lengths that select it; x periodic, so
+    the first and last lines take their wrapped neighbours) computes what
+    the form that forms every neighbour's direction again computes, bit for
+    bit: 3 steps each, against the oracle and against each other."""
+    out = []
+    try:
+        for on in (False, True):
+            ctx, oc, singular = pair(tmp_path / f"c{int(on)}", geom, prec)
+            ctx.set_mg_cg_lds(on)
+            seed_fields(ctx, oc, names=("u", "v", "w"))
+            run_both(ctx, oc, 3, dt_rel=1e-12 if prec == "fp64" else 1e-6)
+            if prec == "fp64":
+                assert_fields(ctx, oc, FIELDS3, 1e-10, interior=True)
+            it, relres = ctx.solver_stats()
+            print(f"{geom} {prec} cg lds {on}: {it} CG iterations, relres {relres:.2e}", flush=True)
+            assert relres <= 1e-13 * 1.0001
+            out.append({f: ctx.get(0, f) for f in FIELDS3 + ("p",)})
+            ctx.close()
+    finally:
+        import snac_amd
+        c = snac_amd.Context(*cases.write_case(str(tmp_path / "r"), [cases.channel_block((4, 4, 4))], 5600.0), 3)
+        c.set_mg_cg_lds(True)  # process-wide: back to the default
+        c.close()
+    for f in out[0]:
+        np.testing.assert_array_equal(out[1][f], out[0][f], err_msg=f)
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
