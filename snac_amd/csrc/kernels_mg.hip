@@ -1031,19 +1031,24 @@ __device__ __forceinline__ double cg_beta(int g, const double *__restrict__ rzn,
 // (rounded to TP, the stored type), and the per-group partial sums of
 // p . (M p), M p computed from the new directions of the neighbours formed
 // on the fly (p_prev and p are different buffers).
-template <typename TZ, typename TP>
+// STAGE (2D levels): the workgroup's chunk of new directions with one line
+// before and after it is formed once into LDS (dynamic, (chunk + 2 ns) x
+// TP) and the operator reads it there; only the wrap of a periodic a at the
+// group's ends forms its neighbour again.  Same values either way.
+template <typename TZ, typename TP, bool STAGE = false>
 __global__ __launch_bounds__(256) void k_dir_apply_dot(MGLev L, const double *__restrict__ lam, long gsize,
                                                        const double *__restrict__ rzn, const double *__restrict__ rzo,
                                                        const double *__restrict__ cross, const TZ *__restrict__ z,
                                                        const TP *__restrict__ pp, TP *__restrict__ p,
                                                        double *__restrict__ part, int nchunk,
                                                        const int *__restrict__ done) {
+  extern __shared__ unsigned char dyn_smem[];
   __shared__ double sh[4];
   const int g = blockIdx.y;
   double s1 = 0.0;
   if (!done[g]) {
     const double be = cg_beta(g, rzn, rzo, cross);
-    auto pn = [&](long e) -> double {
+    auto pnew = [&](long e) -> double {
       if (be == 0.0) return (double)(TP)(double)z[e];
       return (double)(TP)((double)z[e] + be * (double)pp[e]);
     };
@@ -1053,6 +1058,20 @@ __global__ __launch_bounds__(256) void k_dir_apply_dot(MGLev L, const double *__
     const long csz = ((gsize + nchunk - 1) / nchunk + 255) / 256 * 256;
     const long iend = min(gsize, ((long)blockIdx.x + 1) * csz);
     const long i0 = (long)blockIdx.x * csz + threadIdx.x;
+    TP *sp = reinterpret_cast<TP *>(dyn_smem);
+    const long k0s = STAGE ? max(0L, (long)blockIdx.x * csz - L.ns) : 0;
+    const long k1s = STAGE ? min(gsize, iend + L.ns) : 0;
+    if (STAGE) {
+      for (long k = k0s + threadIdx.x; k < k1s; k += 256) sp[k - k0s] = (TP)pnew(base + k);
+      __syncthreads();
+    }
+    auto pn = [&](long e) -> double {
+      if (STAGE) {
+        const long k = e - base;
+        if (k >= k0s && k < k1s) return (double)sp[k - k0s];
+      }
+      return pnew(e);
+    };
     int m, a, s;
     pos_of(L, base + i0, m, a, s);
     const int da = 256 / L.ns, ds = 256 % L.ns;
@@ -1164,7 +1183,9 @@ __global__ __launch_bounds__(256) void k_dir_apply_dot_lds(MGLev L, const double
 // recomputed from p (bit for bit the value k_dir_apply_dot summed), the
 // partial sums of r.r and, when zf is given, zf.r, and rf = (float) r when
 // given
-template <typename TP>
+// STAGE: as k_dir_apply_dot's, the chunk's directions with one line before
+// and after it in LDS
+template <typename TP, bool STAGE = false>
 __global__ __launch_bounds__(256) void k_update_apply_dot(MGLev L, const double *__restrict__ lam, long gsize,
                                                           const double *__restrict__ rz,
                                                           const double *__restrict__ pq, double *__restrict__ x,
@@ -1172,6 +1193,7 @@ __global__ __launch_bounds__(256) void k_update_apply_dot(MGLev L, const double 
                                                           float *__restrict__ rf, const float *__restrict__ zf,
                                                           double *__restrict__ part, int nchunk,
                                                           const int *__restrict__ done) {
+  extern __shared__ unsigned char dyn_smem[];
   __shared__ double sh[2][4];
   const int g = blockIdx.y;
   double s1 = 0.0, s2 = 0.0;
@@ -1182,13 +1204,26 @@ __global__ __launch_bounds__(256) void k_update_apply_dot(MGLev L, const double 
     // spreading an overflow or a NaN
     double al = den > 0.0 ? rz[g * 2] / den : 0.0;
     if (!(__builtin_fabs(al) < 1e300)) al = 0.0;
-    auto pv = [&](long e) -> double { return (double)p[e]; };
     const long base = (long)g * gsize;
     // a contiguous chunk of the group per workgroup: the neighbour lines the
     // operator reads are mostly this workgroup's own (cache hits)
     const long csz = ((gsize + nchunk - 1) / nchunk + 255) / 256 * 256;
     const long iend = min(gsize, ((long)blockIdx.x + 1) * csz);
     const long i0 = (long)blockIdx.x * csz + threadIdx.x;
+    TP *sp = reinterpret_cast<TP *>(dyn_smem);
+    const long k0s = STAGE ? max(0L, (long)blockIdx.x * csz - L.ns) : 0;
+    const long k1s = STAGE ? min(gsize, iend + L.ns) : 0;
+    if (STAGE) {
+      for (long k = k0s + threadIdx.x; k < k1s; k += 256) sp[k - k0s] = p[base + k];
+      __syncthreads();
+    }
+    auto pv = [&](long e) -> double {
+      if (STAGE) {
+        const long k = e - base;
+        if (k >= k0s && k < k1s) return (double)sp[k - k0s];
+      }
+      return (double)p[e];
+    };
     int m, a, s;
     pos_of(L, base + i0, m, a, s);
     const int da = 256 / L.ns, ds = 256 % L.ns;
@@ -1206,7 +1241,7 @@ __global__ __launch_bounds__(256) void k_update_apply_dot(MGLev L, const double 
         const bool ok = i + 256 * u < iend;
         const double lm = lam ? lam[m] : 0.0;
         q[u] = !ok ? 0.0 : rowc ? mx_r(L, lm, pv, a, s, idx, c) : mx_f(L, lm, pv, m, a, s, idx, cidx(L, m, a, s));
-        pc[u] = ok ? (double)p[idx] : 0.0;
+        pc[u] = ok ? pv(idx) : 0.0;
         // x and r are read and written once here and not again until after
         // the next V-cycle: streamed past the caches
         xv[u] = ok ? __builtin_nontemporal_load(&x[idx]) : 0.0;
@@ -1757,8 +1792,12 @@ void mg_dir_apply_dot(hipStream_t st, const MGLev &L, const double *lam, int ng,
   // divide 256 and a chunk's lines with their two neighbours fit in 64 KB
   const long csz = ((gsize + nchunk - 1) / nchunk + 255) / 256 * 256;
   const size_t lds = L.ns > 0 ? (size_t)(csz / L.ns + 2) * L.ns * sizeof(TP) : 0;
+  const size_t ldsg = (size_t)(csz + 2 * L.ns) * sizeof(TP);  // any 2D level: the chunk and a line on each side
   if (g_cg_lds && !L.coupled && L.auni && L.ns > 0 && 256 % L.ns == 0 && lds <= 64 * 1024)
     hipLaunchKernelGGL((k_dir_apply_dot_lds<TZ, TP>), dim3(nchunk, ng), dim3(256), lds, st, L, lam, gsize, rzn, rzo,
+                       cross, z, p_prev, p, part, nchunk, done);
+  else if (g_cg_lds && !L.coupled && ldsg <= 64 * 1024)
+    hipLaunchKernelGGL((k_dir_apply_dot<TZ, TP, true>), dim3(nchunk, ng), dim3(256), ldsg, st, L, lam, gsize, rzn, rzo,
                        cross, z, p_prev, p, part, nchunk, done);
   else
     hipLaunchKernelGGL((k_dir_apply_dot<TZ, TP>), dim3(nchunk, ng), dim3(256), 0, st, L, lam, gsize, rzn, rzo, cross,
@@ -1774,9 +1813,13 @@ void mg_update_apply_dot(hipStream_t st, const MGLev &L, const double *lam, int 
                          double *part, int nchunk, double *out, const int *done) {
   const long csz = ((gsize + nchunk - 1) / nchunk + 255) / 256 * 256;
   const size_t lds = L.ns > 0 ? (size_t)(csz / L.ns + 2) * L.ns * sizeof(TP) : 0;
+  const size_t ldsg = (size_t)(csz + 2 * L.ns) * sizeof(TP);
   if (g_cg_lds && !L.coupled && L.auni && L.ns > 0 && 256 % L.ns == 0 && lds <= 64 * 1024)
     hipLaunchKernelGGL(k_update_apply_dot_lds<TP>, dim3(nchunk, ng), dim3(256), lds, st, L, lam, gsize, rz, pq, x, p,
                        r, rf, zf, part, nchunk, done);
+  else if (g_cg_lds && !L.coupled && ldsg <= 64 * 1024)
+    hipLaunchKernelGGL((k_update_apply_dot<TP, true>), dim3(nchunk, ng), dim3(256), ldsg, st, L, lam, gsize, rz, pq, x,
+                       p, r, rf, zf, part, nchunk, done);
   else
     hipLaunchKernelGGL(k_update_apply_dot<TP>, dim3(nchunk, ng), dim3(256), 0, st, L, lam, gsize, rz, pq, x, p, r, rf,
                        zf, part, nchunk, done);

@@ -309,13 +309,15 @@ def test_line_variants_long_lines(tmp_path, geom, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-@pytest.mark.parametrize("geom", ["channel_y128", "channel_y256"])
+@pytest.mark.parametrize("geom", ["channel_y128", "channel_y256", "bfs", "lchannel"])
 def test_cg_lds_agrees(tmp_path, geom, prec):
-    """The CG direction pass with a workgroup's new directions staged in LDS
-    (lines of 128 and 256 rows, the lengths that select it; x periodic, so
-    the first and last lines take their wrapped neighbours) computes what
-    the form that forms every neighbour's direction again computes, bit for
-    bit: 3 steps each, against the oracle and against each other."""
+    """The CG passes with a workgroup's search directions staged in LDS --
+    whole lines with row coefficients (lines of 128 and 256 rows; x
+    periodic, so the first and last lines take their wrapped neighbours),
+    and chunks that start inside a line with per-cell coefficients (the
+    step's 160-row lines, the L-channel's union) -- compute what the forms
+    reading every neighbour through the caches compute, bit for bit: 3
+    steps each, against the oracle and against each other."""
     out = []
     try:
         for on in (False, True):
@@ -328,7 +330,7 @@ def test_cg_lds_agrees(tmp_path, geom, prec):
             it, relres = ctx.solver_stats()
             print(f"{geom} {prec} cg lds {on}: {it} CG iterations, relres {relres:.2e}", flush=True)
             assert relres <= 1e-13 * 1.0001
-            out.append({f: ctx.get(0, f) for f in FIELDS3 + ("p",)})
+            out.append({(q, f): ctx.get(q, f) for q in range(ctx.nblocks) for f in FIELDS3 + ("p",)})
             ctx.close()
     finally:
         import snac_amd
