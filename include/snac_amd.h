@@ -353,9 +353,11 @@ int snac_mg_line_choices(snac_ctx *ctx, int *fp64, int *fp32, int n);
  * fastest, timed when the MG-PCG plan is set up and kept per level shape
  * for the process; 0..2 compute the same values, 3 and 4 the same systems
  * in another elimination order.  No counterpart in the reference.          */
-/* SNAC_KV_MG_RCP (process-wide): 1 makes the mode-batched MG sweeps and the
- * fused tail divide by refined hardware reciprocals (within a few ulp of
- * the IEEE quotient) instead of IEEE divisions; 0 (the default) divides.   */
+/* SNAC_KV_MG_RCP (process-wide): 1 (the default) makes the mode-batched MG
+ * line sweeps divide by refined hardware reciprocals (within a few ulp of
+ * the IEEE quotient); 0 divides by IEEE quotients, as the other line kernels
+ * and the fused tail always do.  The V-cycle only preconditions a CG that
+ * the FP64 residual stops, so both end at the same tolerance.             */
 /* SNAC_KV_MG_TAIL (process-wide): 1 (the default) runs the fused coarse
  * end of the MG-PCG V-cycle with one mode's vectors of every coarse level
  * resident in LDS where they fit (160 KB per workgroup), 0 with them in

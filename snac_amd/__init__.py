@@ -458,8 +458,8 @@ class Context:
         _check(lib.snac_set_kernel_variant(self.h, 7, int(bool(on))), "set_kernel_variant")
 
     def set_mg_fast_rcp(self, on):
-        """Process-wide: the mode-batched MG sweeps and the fused tail divide
-        by refined hardware reciprocals instead of IEEE divisions."""
+        """Process-wide: the mode-batched MG line sweeps divide by refined
+        hardware reciprocals (the default) or by IEEE quotients."""
         _check(lib.snac_set_kernel_variant(self.h, 5, int(bool(on))), "set_kernel_variant")
 
     def set_tridiag_variant(self, v):

@@ -1503,7 +1503,7 @@ namespace {
 int g_line_variant = -1;  // forced variant, or -1: each level's own
 // the eliminations of the mode-batched sweeps and of the fused tail with
 // refined hardware reciprocals instead of IEEE divisions
-bool g_fast_rcp = false;
+bool g_fast_rcp = true;  // refined reciprocals in the mode-batched line kernels
 // the fused tail with its vectors in LDS where they fit
 bool g_tail_lds = true;
 bool g_cg_lds = true;  // the CG direction pass with the new directions staged in LDS
@@ -1673,16 +1673,12 @@ void mg_tail_k(hipStream_t st, const MGTailT<T> &t, int nm, const T *lam, int pi
   const int rows = L.base + (L.extra ? 1 : 0);
   const size_t shm = (size_t)t.lds_elems * sizeof(T);
   const bool lx = g_tail_lds && shm <= 160 * 1024;
+  // the fused coarse end divides by IEEE quotients whatever the line
+  // kernels do (its compile-time-chunk instances are the fastest form)
 #define LAUNCH_TAIL(KK)                                                                                             \
   do {                                                                                                              \
-    if (lx && g_fast_rcp)                                                                                           \
-      hipLaunchKernelGGL((k_tail<KK, LPL, T, true, true>), dim3(nm), dim3(1024), shm, st, t, lam, pin_s, singular,   \
-                         done);                                                                                     \
-    else if (lx)                                                                                                    \
+    if (lx)                                                                                                         \
       hipLaunchKernelGGL((k_tail<KK, LPL, T, false, true>), dim3(nm), dim3(1024), shm, st, t, lam, pin_s, singular,  \
-                         done);                                                                                     \
-    else if (g_fast_rcp)                                                                                            \
-      hipLaunchKernelGGL((k_tail<KK, LPL, T, true, false>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular,    \
                          done);                                                                                     \
     else                                                                                                            \
       hipLaunchKernelGGL((k_tail<KK, LPL, T, false, false>), dim3(nm), dim3(1024), 0, st, t, lam, pin_s, singular,   \
@@ -1694,7 +1690,7 @@ void mg_tail_k(hipStream_t st, const MGTailT<T> &t, int nm, const T *lam, int pi
   if constexpr (LPL == 32) {
     auto full = [&](auto kk) {
       constexpr int KK = decltype(kk)::value;
-      if (g_fast_rcp || L.ns != LPL * KK) return false;
+      if (L.ns != LPL * KK) return false;
       if (lx)
         hipLaunchKernelGGL((k_tail<KK, LPL, T, false, true, true>), dim3(nm), dim3(1024), shm, st, t, lam, pin_s,
                            singular, done);

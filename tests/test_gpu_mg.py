@@ -257,19 +257,23 @@ def test_line_variants_agree(tmp_path, geom, prec):
     the rows staged through LDS, and the lines of a workgroup over several
     modes sharing staged coefficients -- compute the same values: the
     pressure solves agree bit for bit (FFT path, and the coupled 3D path,
-    where the third falls back to the second)."""
-    import snac_amd
+    where the third falls back to the second) when the mode-batched kernels
+    divide by IEEE quotients as the others do."""
     out = []
-    for v in (0, 1, 2):
-        ctx, oc, _ = pair(tmp_path / f"v{v}", geom, prec)
-        ctx.set_mg_line_variant(v)
-        seed_fields(ctx, oc)
-        ctx.bounduvw(False)
-        ctx.fillps(1e-3)
-        ctx.solve_pressure()
-        out.append([ctx.get(b, "pp") for b in range(ctx.nblocks)])
-        ctx.set_mg_line_variant(-1)
-        ctx.close()
+    try:
+        for v in (0, 1, 2):
+            ctx, oc, _ = pair(tmp_path / f"v{v}", geom, prec)
+            ctx.set_mg_fast_rcp(False)  # process-wide
+            ctx.set_mg_line_variant(v)
+            seed_fields(ctx, oc)
+            ctx.bounduvw(False)
+            ctx.fillps(1e-3)
+            ctx.solve_pressure()
+            out.append([ctx.get(b, "pp") for b in range(ctx.nblocks)])
+            ctx.set_mg_line_variant(-1)
+            ctx.close()
+    finally:
+        restore_fast_rcp(tmp_path)
     for v in (1, 2):
         for a, b in zip(out[0], out[v]):
             np.testing.assert_array_equal(a, b)
@@ -283,10 +287,32 @@ def test_line_variants_long_lines(tmp_path, geom, prec):
     16- and 8-lanes-per-line mode-batched kernels (3, 4: the same systems
     in another elimination order) solve to the same tolerance; every
     variant's pressure correction is checked against the oracle's direct
-    banded solve."""
+    banded solve (the bit-for-bit comparison with IEEE quotients in every
+    kernel)."""
     out = {}
+    try:
+        run_long_line_variants(tmp_path, geom, prec, out)
+    finally:
+        restore_fast_rcp(tmp_path)
+    for v in (1, 2):
+        np.testing.assert_array_equal(out[0], out[v])
+    for v in (3, 4):
+        print(f"variant {v}: max |pp - pp(variant 0)| = {np.abs(out[v] - out[0]).max():.2e}", flush=True)
+
+
+def restore_fast_rcp(tmp_path):
+    """the process-wide refined reciprocals of the mode-batched line kernels
+    back to their default (on)"""
+    import snac_amd
+    c = snac_amd.Context(*cases.write_case(str(tmp_path / "rcp"), [cases.channel_block((4, 4, 4))], 5600.0), 3)
+    c.set_mg_fast_rcp(True)
+    c.close()
+
+
+def run_long_line_variants(tmp_path, geom, prec, out):
     for v in (0, 1, 2, 3, 4):
         ctx, oc, singular = pair(tmp_path / f"v{v}", geom, prec)
+        ctx.set_mg_fast_rcp(False)  # process-wide
         ctx.set_mg_line_variant(v)
         seed_fields(ctx, oc)
         ctx.bounduvw(False); oc.bounduvw(False)
@@ -302,10 +328,6 @@ def test_line_variants_long_lines(tmp_path, geom, prec):
         out[v] = ctx.get(0, "pp")
         ctx.set_mg_line_variant(-1)
         ctx.close()
-    for v in (1, 2):
-        np.testing.assert_array_equal(out[0], out[v])
-    for v in (3, 4):
-        print(f"variant {v}: max |pp - pp(variant 0)| = {np.abs(out[v] - out[0]).max():.2e}", flush=True)
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
