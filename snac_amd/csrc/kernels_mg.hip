@@ -1192,7 +1192,7 @@ __global__ __launch_bounds__(256) void k_update_apply_dot(MGLev L, const double 
                                                           const TP *__restrict__ p, double *__restrict__ r,
                                                           float *__restrict__ rf, const float *__restrict__ zf,
                                                           double *__restrict__ part, int nchunk,
-                                                          const int *__restrict__ done) {
+                                                          const int *__restrict__ done, int first) {
   extern __shared__ unsigned char dyn_smem[];
   __shared__ double sh[2][4];
   const int g = blockIdx.y;
@@ -1244,7 +1244,7 @@ __global__ __launch_bounds__(256) void k_update_apply_dot(MGLev L, const double 
         pc[u] = ok ? pv(idx) : 0.0;
         // x and r are read and written once here and not again until after
         // the next V-cycle: streamed past the caches
-        xv[u] = ok ? __builtin_nontemporal_load(&x[idx]) : 0.0;
+        xv[u] = ok && !first ? __builtin_nontemporal_load(&x[idx]) : 0.0;
         rv[u] = ok ? __builtin_nontemporal_load(&r[idx]) : 0.0;
         zv[u] = ok && zf ? (double)zf[idx] : 0.0;
         pos_step(L, da, ds, m, a, s);
@@ -1263,6 +1263,10 @@ __global__ __launch_bounds__(256) void k_update_apply_dot(MGLev L, const double 
         if (zf) s2 += zv[u] * rn;
       }
     }
+  } else if (first) {  // the solve's first step: a converged group's solution is 0
+    const long csz = ((gsize + nchunk - 1) / nchunk + 255) / 256 * 256;
+    const long c0 = (long)blockIdx.x * csz, c1 = min(gsize, c0 + csz);
+    for (long i = c0 + threadIdx.x; i < c1; i += 256) x[(long)g * gsize + i] = 0.0;
   }
   s1 = wave_sum(s1);
   s2 = wave_sum(s2);
@@ -1284,7 +1288,7 @@ __global__ __launch_bounds__(256) void k_update_apply_dot_lds(MGLev L, const dou
                                                               const TP *__restrict__ p, double *__restrict__ r,
                                                               float *__restrict__ rf, const float *__restrict__ zf,
                                                               double *__restrict__ part, int nchunk,
-                                                              const int *__restrict__ done) {
+                                                              const int *__restrict__ done, int first) {
   extern __shared__ unsigned char dyn_smem[];
   TP *sp = reinterpret_cast<TP *>(dyn_smem);  // (lines + 2) x ns: below, own lines, above
   __shared__ double sh[2][4];
@@ -1335,7 +1339,7 @@ __global__ __launch_bounds__(256) void k_update_apply_dot_lds(MGLev L, const dou
           }
           q[u] = y;
           pc[u] = ok ? (double)row[s] : 0.0;
-          xv[u] = ok ? __builtin_nontemporal_load(&x[idx]) : 0.0;
+          xv[u] = ok && !first ? __builtin_nontemporal_load(&x[idx]) : 0.0;
           rv[u] = ok ? __builtin_nontemporal_load(&r[idx]) : 0.0;
           zv[u] = ok && zf ? (double)zf[idx] : 0.0;
         }
@@ -1353,6 +1357,10 @@ __global__ __launch_bounds__(256) void k_update_apply_dot_lds(MGLev L, const dou
         }
       }
     }
+  } else if (first) {  // the solve's first step: a converged group's solution is 0
+    const long csz = ((gsize + nchunk - 1) / nchunk + 255) / 256 * 256;
+    const long c0 = (long)blockIdx.x * csz, c1 = min(gsize, c0 + csz);
+    for (long i = c0 + threadIdx.x; i < c1; i += 256) x[(long)g * gsize + i] = 0.0;
   }
   s1 = wave_sum(s1);
   s2 = wave_sum(s2);
@@ -1806,19 +1814,20 @@ void mg_dir_apply_dot(hipStream_t st, const MGLev &L, const double *lam, int ng,
 template <typename TP>
 void mg_update_apply_dot(hipStream_t st, const MGLev &L, const double *lam, int ng, long gsize, const double *rz,
                          const double *pq, double *x, const TP *p, double *r, float *rf, const float *zf,
-                         double *part, int nchunk, double *out, const int *done) {
+                         double *part, int nchunk, double *out, const int *done, bool first) {
   const long csz = ((gsize + nchunk - 1) / nchunk + 255) / 256 * 256;
   const size_t lds = L.ns > 0 ? (size_t)(csz / L.ns + 2) * L.ns * sizeof(TP) : 0;
   const size_t ldsg = (size_t)(csz + 2 * L.ns) * sizeof(TP);
+  const int fi = first ? 1 : 0;
   if (g_cg_lds && !L.coupled && L.auni && L.ns > 0 && 256 % L.ns == 0 && lds <= 64 * 1024)
     hipLaunchKernelGGL(k_update_apply_dot_lds<TP>, dim3(nchunk, ng), dim3(256), lds, st, L, lam, gsize, rz, pq, x, p,
-                       r, rf, zf, part, nchunk, done);
+                       r, rf, zf, part, nchunk, done, fi);
   else if (g_cg_lds && !L.coupled && ldsg <= 64 * 1024)
     hipLaunchKernelGGL((k_update_apply_dot<TP, true>), dim3(nchunk, ng), dim3(256), ldsg, st, L, lam, gsize, rz, pq, x,
-                       p, r, rf, zf, part, nchunk, done);
+                       p, r, rf, zf, part, nchunk, done, fi);
   else
     hipLaunchKernelGGL(k_update_apply_dot<TP>, dim3(nchunk, ng), dim3(256), 0, st, L, lam, gsize, rz, pq, x, p, r, rf,
-                       zf, part, nchunk, done);
+                       zf, part, nchunk, done, fi);
   SNAC_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_dot_final, dim3(ng), dim3(64), 0, st, part, nchunk, out, done, nullptr);
   SNAC_HIP(hipGetLastError());
@@ -1834,10 +1843,10 @@ template void mg_dir_apply_dot<float, float>(hipStream_t, const MGLev &, const d
                                              double *, int, double *, const int *);
 template void mg_update_apply_dot<double>(hipStream_t, const MGLev &, const double *, int, long, const double *,
                                           const double *, double *, const double *, double *, float *, const float *,
-                                          double *, int, double *, const int *);
+                                          double *, int, double *, const int *, bool);
 template void mg_update_apply_dot<float>(hipStream_t, const MGLev &, const double *, int, long, const double *,
                                          const double *, double *, const float *, double *, float *, const float *,
-                                         double *, int, double *, const int *);
+                                         double *, int, double *, const int *, bool);
 
 void mg_set_shift(MGPlan &P, hipStream_t st, double shift) {
   if (P.comp < 0 || !P.lam_base) return;

@@ -165,11 +165,13 @@ template <typename TZ, typename TP>
 void mg_dir_apply_dot(hipStream_t st, const MGLev &L, const double *lam, int ng, long gsize, const double *rzn,
                       const double *rzo, const double *cross, const TZ *z, const TP *p_prev, TP *p, double *part,
                       int nchunk, double *out, const int *done);
-// mg_cg_update_dot with q = M p recomputed from p instead of read
+// mg_cg_update_dot with q = M p recomputed from p instead of read.  first:
+// the solve's first step from x = 0, x not read (x = alpha p) and set to 0
+// on the groups that are already done
 template <typename TP>
 void mg_update_apply_dot(hipStream_t st, const MGLev &L, const double *lam, int ng, long gsize, const double *rz,
                          const double *pq, double *x, const TP *p, double *r, float *rf, const float *zf,
-                         double *part, int nchunk, double *out, const int *done);
+                         double *part, int nchunk, double *out, const int *done, bool first = false);
 void mg_izero(hipStream_t st, long n, int *x);
 // dst = (float) src
 void mg_to_float(hipStream_t st, long n, const double *src, float *dst);
@@ -317,6 +319,7 @@ void mg_free(MGPlan &P);
 struct MGRun {
   int it = 0, cur = 0, pc = 0;
   bool net = false, record = false;
+  bool x_unset = false;  // x is zero and left to the first update pass to write
 };
 // Solves M x = -rhs where rhs (in the plan's layout) has been copied into
 // P.r scaled by -1; on return P.x holds the solution of A x = rhs.

@@ -709,7 +709,7 @@ void mg_iteration(MGPlan &P, hipStream_t st, MGRun &R) {
     R.pc = 1 - R.pc;
     if (P.mark_update) P.mark_update(true);
     mg_update_apply_dot<float>(st, L0, P.h64.lam, P.ng, P.gsize, P.rz[R.cur], P.pq, P.x, P.pf[R.pc], P.r, P.rf, P.zf,
-                               P.part, P.nchunk, P.rr, P.done);
+                               P.part, P.nchunk, P.rr, P.done, R.x_unset);
     if (P.mark_update) P.mark_update(false);
   } else if (mixed) {
     mg_dir_apply_dot<float, double>(st, L0, P.h64.lam, P.ng, P.gsize, P.rz[R.cur], rzo, rzo ? P.rr : nullptr, P.zf,
@@ -717,7 +717,7 @@ void mg_iteration(MGPlan &P, hipStream_t st, MGRun &R) {
     R.pc = 1 - R.pc;
     if (P.mark_update) P.mark_update(true);
     mg_update_apply_dot<double>(st, L0, P.h64.lam, P.ng, P.gsize, P.rz[R.cur], P.pq, P.x, P.pd[R.pc], P.r, P.rf,
-                                P.zf, P.part, P.nchunk, P.rr, P.done);
+                                P.zf, P.part, P.nchunk, P.rr, P.done, R.x_unset);
     if (P.mark_update) P.mark_update(false);
   } else {
     mg_dir_apply_dot<double, double>(st, L0, P.h64.lam, P.ng, P.gsize, P.rz[R.cur], rzo, nullptr, P.z, P.pd[R.pc],
@@ -725,9 +725,10 @@ void mg_iteration(MGPlan &P, hipStream_t st, MGRun &R) {
     R.pc = 1 - R.pc;
     if (P.mark_update) P.mark_update(true);
     mg_update_apply_dot<double>(st, L0, P.h64.lam, P.ng, P.gsize, P.rz[R.cur], P.pq, P.x, P.pd[R.pc], P.r, nullptr,
-                                nullptr, P.part, P.nchunk, P.rr, P.done);
+                                nullptr, P.part, P.nchunk, P.rr, P.done, R.x_unset);
     if (P.mark_update) P.mark_update(false);
   }
+  R.x_unset = false;  // written by this iteration's update pass
   mg_flags(st, P.ng, P.rr, P.bb, P.tol * P.tol, P.done, P.left, P.mlist, R.it + 1);
   if (R.net && P.fallback) launch_net_stall(st, P.ng, P.rr, P.rr_prev, P.done, P.stall_ratio * P.stall_ratio, P.stall);
   R.it++;
@@ -756,7 +757,11 @@ MGRun mg_solve(MGPlan &P, hipStream_t st, int fixed) {
   // the warm start keeps x (the previous solve's solution) as x0; the
   // network's modes start from zero or from its own guess
   const bool warm = P.warm && !R.net && !R.record;
-  if (!warm) mg_zero(st, n, P.x);
+  // starting from zero: the first update pass writes x = alpha p without
+  // reading x, and zeroes the groups already done (the network's initial
+  // guess adds to a zeroed x instead)
+  R.x_unset = !warm && !(R.net && P.guess) && P.maxiter > 0;
+  if (!warm && !R.x_unset) mg_zero(st, n, P.x);
   mg_izero(st, P.ng, P.done);
   mg_izero(st, 1, P.left + 1);
   project(P, st, P.r);
