@@ -1857,6 +1857,36 @@ void mg_set_shift(MGPlan &P, hipStream_t st, double shift) {
   SNAC_HIP(hipGetLastError());
 }
 
+// r.r per group (k_dot's element order and sums, so the same values) with
+// rf = (float) r written in the same pass
+__global__ __launch_bounds__(256) void k_dot_self_cvt(long gsize, const double *__restrict__ x, float *__restrict__ xf,
+                                                      double *__restrict__ part, int nchunk) {
+  __shared__ double sh[2][4];
+  const int g = blockIdx.y;
+  double s1 = 0.0;
+  const long base = (long)g * gsize;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < gsize; i += (long)nchunk * 256) {
+    const double v = x[base + i];
+    s1 += v * v;
+    xf[base + i] = (float)v;
+  }
+  s1 = wave_sum(s1);
+  if ((threadIdx.x & 63) == 0) sh[0][threadIdx.x >> 6] = s1;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[((long)g * nchunk + blockIdx.x) * 2] = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
+    part[((long)g * nchunk + blockIdx.x) * 2 + 1] = 0.0;
+  }
+}
+
+void mg_dot_self_cvt(hipStream_t st, int ng, long gsize, const double *x, float *xf, double *part, int nchunk,
+                     double *out) {
+  hipLaunchKernelGGL(k_dot_self_cvt, dim3(nchunk, ng), dim3(256), 0, st, gsize, x, xf, part, nchunk);
+  SNAC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_dot_final, dim3(ng), dim3(64), 0, st, part, nchunk, out, nullptr, nullptr);
+  SNAC_HIP(hipGetLastError());
+}
+
 void mg_to_float(hipStream_t st, long n, const double *src, float *dst) {
   hipLaunchKernelGGL(k_to_float, dim3(blocks_for(n)), dim3(256), 0, st, n, src, dst);
   SNAC_HIP(hipGetLastError());

@@ -175,6 +175,10 @@ void mg_update_apply_dot(hipStream_t st, const MGLev &L, const double *lam, int 
 void mg_izero(hipStream_t st, long n, int *x);
 // dst = (float) src
 void mg_to_float(hipStream_t st, long n, const double *src, float *dst);
+// out[g*2] = x.x per group, as mg_dot computes it, and xf = (float) x in the
+// same pass
+void mg_dot_self_cvt(hipStream_t st, int ng, long gsize, const double *x, float *xf, double *part, int nchunk,
+                     double *out);
 // per-group dot products out[g*2] = x.y, out[g*2+1] = x2.y2 (or sum of x2
 // when y2 is null), accumulated in FP64; groups are contiguous runs of gsize
 // elements; groups with done[g] or !only[g] (when given) are skipped

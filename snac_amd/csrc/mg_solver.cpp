@@ -765,7 +765,10 @@ MGRun mg_solve(MGPlan &P, hipStream_t st, int fixed) {
   mg_izero(st, P.ng, P.done);
   mg_izero(st, 1, P.left + 1);
   project(P, st, P.r);
-  mg_dot<double, double>(st, P.ng, P.gsize, P.r, P.r, nullptr, nullptr, P.part, P.nchunk, P.bb, P.done, nullptr);
+  // b.b, and the FP32 copy of r in the same pass where it is needed as is
+  const bool cvt = mixed && !warm;
+  if (cvt) mg_dot_self_cvt(st, P.ng, P.gsize, P.r, P.rf, P.part, P.nchunk, P.bb);
+  else mg_dot<double, double>(st, P.ng, P.gsize, P.r, P.r, nullptr, nullptr, P.part, P.nchunk, P.bb, P.done, nullptr);
   mg_flags(st, P.ng, P.bb, P.bb, tol2, P.done, P.left, P.mlist, 0);
   if (R.record)  // the input half of this solve's training pairs
     launch_net_pair_in(st, P.ng, L0.na, L0.ns, P.nH, P.nW, P.r, P.bb, P.nact_plane, P.h64.lam, P.lam_ref,
@@ -774,8 +777,6 @@ MGRun mg_solve(MGPlan &P, hipStream_t st, int fixed) {
     mg_apply_dot(st, L0, P.h64.lam, P.x, P.q, P.ng, P.gsize, P.part, P.nchunk, P.pq, P.done);
     mg_warm_residual(st, P.ng, P.gsize, P.bb, P.x, P.r, P.q, mixed ? P.rf : nullptr, P.part, P.nchunk, P.rr);
     mg_flags(st, P.ng, P.rr, P.bb, tol2, P.done, P.left, P.mlist, 0);
-  } else if (mixed) {
-    mg_to_float(st, n, P.r, P.rf);
   }
   const double *rr0 = P.bb;  // r.r of the residual the first preconditioning sees
   if (R.net && P.guess) {
