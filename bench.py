@@ -35,13 +35,14 @@ Forms, each timed on its own context in the same run (--forms):
 The line's value is the mixed (or learned) form's Mcell-steps/s; the plain
 form's figures and the ratio sit beside it, with every form's divergence.
 
-Roofline objects: `roofline` is the dominant kernel of the headline form
-(the finest MG level's zebra colour sweep on the MG forms: 1.5 x unknowns x
-element size per sweep -- b and x of half the lines, x of the other half;
-rk_mom on the direct forms), `roofline_stencil` the fused rk_mom of the same
-form (104 B/cell FP64, 80 B/cell with the FP32 right-hand side), both from
-HIP events on the context's stream during an eager pass after the timed
-region.  cpu_baseline: the C oracle (FP64, OpenMP, direct Poisson solver)
+Roofline objects: `roofline` is the dominant kernel of the headline form:
+on the MG forms the one with more time per step of the CG update pass
+(`roofline_update`: 40 B per unknown FP64, 44 B mixed) and the finest MG
+level's zebra colour sweep (`roofline_line0`: 1.5 x unknowns x element size
+per sweep -- b and x of half the lines, x of the other half), rk_mom on the
+direct forms; `roofline_stencil` is the fused rk_mom of the same form (104
+B/cell FP64, 80 B/cell with the FP32 right-hand side); all from HIP events
+on the context's stream during an eager pass after the timed region.  cpu_baseline: the C oracle (FP64, OpenMP, direct Poisson solver)
 on a bounded sample of the same channel.
 
 --workload bfs runs the three-block backward-facing step (inlet 192x64x128
