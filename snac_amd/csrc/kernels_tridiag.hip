@@ -13,6 +13,17 @@
 namespace snac {
 namespace {
 
+// 1/x from the hardware estimate refined by two Newton steps (within a few
+// ulp of the IEEE quotient): the eliminations' reciprocals without the IEEE
+// division sequence on their dependent chains
+__device__ __forceinline__ double rcp_ref(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
 constexpr int WG = 64;
 constexpr int LDS_M = 128;  // line length served from LDS (64 lines x 128 x 8 B)
 
@@ -138,7 +149,7 @@ __global__ __launch_bounds__(64 * NC) void k_tridiag_chunk(TridiagLine t, double
       double l, d, u;
       coef(p0 + j, l, d, u);
       const double den = (j == 1) ? d : d - l * C[j - 1];
-      const double ir = den != 0.0 ? 1.0 / den : 0.0;
+      const double ir = den != 0.0 ? rcp_ref(den) : 0.0;
       if (j == 1) { A[1] = l * ir; C[1] = u * ir; D[1] = D[1] * ir; }
       else { A[j] = -l * A[j - 1] * ir; C[j] = u * ir; D[j] = (D[j] - l * D[j - 1]) * ir; }
       if (j == cnt - 1) { aq = A[j]; cq = C[j]; dq = D[j]; }
@@ -175,7 +186,7 @@ __global__ __launch_bounds__(64 * NC) void k_tridiag_chunk(TridiagLine t, double
       const double a = red[(0 * NR + k) * 64 + lane], b = red[(1 * NR + k) * 64 + lane];
       const double g = red[(2 * NR + k) * 64 + lane], d = red[(3 * NR + k) * 64 + lane];
       const double den = k == 0 ? b : b - a * cprev;
-      const double ir = den != 0.0 ? 1.0 / den : 0.0;
+      const double ir = den != 0.0 ? rcp_ref(den) : 0.0;
       cprev = g * ir;
       dprev = (d - (k == 0 ? 0.0 : a * dprev)) * ir;
       red[(2 * NR + k) * 64 + lane] = cprev;
